@@ -1,0 +1,95 @@
+/*
+ * qmbed_hip — C ABI of the MI355X-native spin-sector H·v / eigsh / Krylov
+ * exponential engine.
+ *
+ * The first five entry points are drop-ins for the reference's C ABI
+ * (QMBED bindings/capi/src/lib.rs), with the same JSON request and response
+ * schemas for the spin-basis eigsh / evolve / exponential-action commands and
+ * the same ownership rule (responses are heap strings released exactly once
+ * with qmbed_string_free). The qmbed_hip_* functions are a typed fast path
+ * for device-resident vectors; they have no counterpart in the reference.
+ */
+#ifndef QMBED_HIP_H
+#define QMBED_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference-compatible JSON protocol ------------------------------ */
+
+/* Replaces qmbed_run_json (bindings/capi/src/lib.rs:6089-6118): one solve
+ * request {"basis","terms","format","solver"} -> {"status":"ok","result":{...}}
+ * or {"status":"error","error":"..."}. */
+char* qmbed_run_json(const char* request);
+
+/* Replaces qmbed_command_json (bindings/capi/src/lib.rs:6120-6151): tagged
+ * "operation" commands describe_basis, create_model, eigsh_model,
+ * evolve_model, create_expm_action, apply_expm_action, release_model,
+ * release_expm_action. Other operations answer an error. */
+char* qmbed_command_json(const char* request);
+
+/* Replaces qmbed_evolve_model_with_drive_json
+ * (bindings/capi/src/lib.rs:6153-6192); exported so existing loaders resolve
+ * it. Driven evolution is outside this engine: always answers an error. */
+typedef int (*qmbed_drive_callback)(void* context, double time, double* values, size_t count);
+char* qmbed_evolve_model_with_drive_json(const char* request, qmbed_drive_callback callback, void* context);
+
+/* Replace qmbed_register_user_basis_{32,64}_json
+ * (bindings/capi/src/lib.rs:6194-6288); exported for loader compatibility,
+ * always answer an error (user bases are outside this engine). */
+typedef int (*qmbed_user_basis_callback_32)(void* context, const uint32_t* input, uint32_t* output, size_t count);
+typedef int (*qmbed_user_basis_callback_64)(void* context, const uint64_t* input, uint64_t* output, size_t count);
+char* qmbed_register_user_basis_32_json(const char* request, qmbed_user_basis_callback_32 callback, void* context);
+char* qmbed_register_user_basis_64_json(const char* request, qmbed_user_basis_callback_64 callback, void* context);
+
+/* Replaces qmbed_string_free (bindings/capi/src/lib.rs:6290-6297). NULL is
+ * allowed. */
+void qmbed_string_free(char* response);
+
+/* ---- typed fast path (no counterpart in the reference) ---------------- */
+
+typedef struct qmbed_hip_model qmbed_hip_model;
+
+enum { QMBED_F64 = 0, QMBED_F32 = 1, QMBED_C128 = 2, QMBED_C64 = 3 };
+
+/* model_json: {"basis":{...},"terms":[...],"checks":{...}?} as in
+ * create_model. Builds the sector on the GPU and compiles the operator.
+ * On failure returns NULL and stores an error string (free with
+ * qmbed_string_free) in *error when error is non-NULL. */
+qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error);
+void qmbed_hip_model_free(qmbed_hip_model* model);
+long qmbed_hip_model_dimension(const qmbed_hip_model* model);
+int qmbed_hip_model_is_real(const qmbed_hip_model* model);
+/* host copies of the sorted representatives and their orbit sizes */
+int qmbed_hip_model_states(const qmbed_hip_model* model, uint32_t* states, uint8_t* orbits);
+/* the model's HIP stream (hipStream_t) */
+void* qmbed_hip_model_stream(const qmbed_hip_model* model);
+/* y = H x on device pointers (dtype QMBED_*); stream NULL = model stream */
+int qmbed_hip_apply_device(const qmbed_hip_model* model, int dtype, const void* x, void* y, void* stream);
+/* y = H x on host arrays (copies through device memory) */
+int qmbed_hip_apply_host(const qmbed_hip_model* model, int dtype, const void* x, void* y);
+/* average milliseconds of `iters` back-to-back H·v launches (HIP events on
+ * the model stream) */
+double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void* x, void* y, int iters);
+/* eigsh with a SolverRequest JSON; response as qmbed_run_json's result plus
+ * fp32_iterations / fp64_iterations / reorthogonalization counters. Accepts
+ * the extra solver keys "matvec_precision" ("fp64"|"fp32") and
+ * "early_stop" (bool). */
+char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
+
+void* qmbed_hip_malloc(size_t bytes);
+void qmbed_hip_free(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int qmbed_hip_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int qmbed_hip_synchronize(void);
+int qmbed_hip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QMBED_HIP_H */

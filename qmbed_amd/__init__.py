@@ -1,0 +1,260 @@
+"""qmbed_amd — MI355X-native spin-sector H·v, eigsh and Krylov exponential engine.
+
+Host-side mirror of the reference's Python FFI (QMBED
+bindings/python/src/qmbed/_ffi.py:154-215): ``run`` sends one solve request
+through ``qmbed_run_json`` and ``command`` one tagged operation through
+``qmbed_command_json``; both return the decoded ``result`` or raise
+:class:`QmbedError` with the library's error text.  :class:`Model` wraps the
+typed device fast path (``qmbed_hip_*``).
+
+The native library ``libqmbed_hip.so`` is built in-tree by
+``qmbed_amd/build.py`` (``__graft_entry__.build()``).  There is no CPU
+fallback: importing the bindings without the library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.environ.get("QMBED_HIP_LIBRARY", os.path.join(_HERE, "libqmbed_hip.so"))
+
+F64, F32, C128, C64 = 0, 1, 2, 3
+_DTYPES = {np.dtype(np.float64): F64, np.dtype(np.float32): F32,
+           np.dtype(np.complex128): C128, np.dtype(np.complex64): C64}
+
+_lib = None
+
+
+class QmbedError(RuntimeError):
+    """Error text returned by the native library (reference QmbedError display)."""
+
+
+def library():
+    """Load ``libqmbed_hip.so``; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY_PATH):
+        raise RuntimeError(
+            f"qmbed_amd native library not found at {LIBRARY_PATH}; build it with "
+            "`python qmbed_amd/build.py` (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIBRARY_PATH)
+    c_char_p, c_void_p, c_int, c_long = ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_long
+    for name in ("qmbed_run_json", "qmbed_command_json"):
+        f = getattr(L, name)
+        f.restype = c_void_p
+        f.argtypes = [c_char_p]
+    L.qmbed_string_free.argtypes = [c_void_p]
+    L.qmbed_string_free.restype = None
+    L.qmbed_hip_model_create.restype = c_void_p
+    L.qmbed_hip_model_create.argtypes = [c_char_p, ctypes.POINTER(c_void_p)]
+    L.qmbed_hip_model_free.argtypes = [c_void_p]
+    L.qmbed_hip_model_dimension.restype = c_long
+    L.qmbed_hip_model_dimension.argtypes = [c_void_p]
+    L.qmbed_hip_model_is_real.argtypes = [c_void_p]
+    L.qmbed_hip_model_states.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.qmbed_hip_model_stream.restype = c_void_p
+    L.qmbed_hip_model_stream.argtypes = [c_void_p]
+    L.qmbed_hip_apply_device.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
+    L.qmbed_hip_apply_host.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    L.qmbed_hip_time_apply.restype = ctypes.c_double
+    L.qmbed_hip_time_apply.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int]
+    L.qmbed_hip_eigsh_json.restype = c_void_p
+    L.qmbed_hip_eigsh_json.argtypes = [c_void_p, c_char_p]
+    L.qmbed_hip_malloc.restype = c_void_p
+    L.qmbed_hip_malloc.argtypes = [ctypes.c_size_t]
+    L.qmbed_hip_free.argtypes = [c_void_p]
+    L.qmbed_hip_memcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+    L.qmbed_hip_device_count.restype = c_int
+    _lib = L
+    return L
+
+
+def _take(ptr) -> str:
+    L = library()
+    try:
+        return ctypes.string_at(ptr).decode("utf-8")
+    finally:
+        L.qmbed_string_free(ptr)
+
+
+def _call(fn_name: str, request) -> dict:
+    payload = request if isinstance(request, (bytes, str)) else json.dumps(request)
+    if isinstance(payload, str):
+        payload = payload.encode("utf-8")
+    response = json.loads(_take(getattr(library(), fn_name)(payload)))
+    if response.get("status") != "ok":
+        raise QmbedError(response.get("error", "unknown error"))
+    return response["result"]
+
+
+def run(request) -> dict:
+    """One solve request (``qmbed_run_json``)."""
+    return _call("qmbed_run_json", request)
+
+
+def command(request) -> dict:
+    """One tagged command (``qmbed_command_json``)."""
+    return _call("qmbed_command_json", request)
+
+
+def raw_call(fn_name: str, request) -> dict:
+    """Decoded response envelope without raising on errors (for tests)."""
+    payload = request if isinstance(request, (bytes, str)) or request is None else json.dumps(request)
+    if isinstance(payload, str):
+        payload = payload.encode("utf-8")
+    return json.loads(_take(getattr(library(), fn_name)(payload)))
+
+
+# --------------------------------------------------------------- requests --
+
+def spin_basis(sites, up=None, momentum=None, parity=None, pauli=False, up_sectors=None, **extra):
+    b = {"kind": "spin", "sites": sites, "pauli": pauli}
+    if up is not None:
+        b["up"] = up
+    if up_sectors is not None:
+        b["up_sectors"] = list(up_sectors)
+    if momentum is not None:
+        b["momentum"] = momentum
+    if parity is not None:
+        b["parity"] = parity
+    b.update(extra)
+    return b
+
+
+_NAMES = {"I": "identity", "n": "number", "z": "z", "+": "raising", "-": "lowering", "x": "x", "y": "y"}
+
+
+def term(symbols, couplings):
+    """symbols like "zz" / "+-"; couplings: [(coefficient, [sites...]), ...]."""
+    return {"product": {"local": [_NAMES[s] for s in symbols]},
+            "couplings": [{"coefficient": [complex(c).real, complex(c).imag], "sites": list(map(int, s))}
+                          for c, s in couplings]}
+
+
+def terms_from_spec(spec):
+    """[(symbols, [(coeff, sites)...]), ...] -> JSON term list."""
+    return [term(sym, cpl) for sym, cpl in spec]
+
+
+def chain_terms(L, jz=1.0, jxy=0.5, nn=1):
+    bonds = [(i, (i + nn) % L) for i in range(L)]
+    return [("zz", [(jz, b) for b in bonds]), ("+-", [(jxy, b) for b in bonds]), ("-+", [(jxy, b) for b in bonds])]
+
+
+def xxz_terms(L, delta):
+    return chain_terms(L, jz=delta, jxy=0.5)
+
+
+def j1j2_terms(L, j1=1.0, j2=0.5):
+    return chain_terms(L, jz=j1, jxy=0.5 * j1, nn=1) + chain_terms(L, jz=j2, jxy=0.5 * j2, nn=2)
+
+
+def ising_terms(L, hx, hz, j=1.0):
+    bonds = [(i, (i + 1) % L) for i in range(L)]
+    out = [("zz", [(-j, b) for b in bonds])]
+    if hx:
+        out.append(("x", [(-hx, [i]) for i in range(L)]))
+    if hz:
+        out.append(("z", [(-hz, [i]) for i in range(L)]))
+    return out
+
+
+class Model:
+    """Device-resident sector + matrix-free operator (typed fast path)."""
+
+    def __init__(self, basis: dict, terms, checks=None):
+        L = library()
+        req = {"basis": basis, "terms": terms if terms and isinstance(terms[0], dict) else terms_from_spec(terms)}
+        if checks is not None:
+            req["checks"] = checks
+        err = ctypes.c_void_p()
+        h = L.qmbed_hip_model_create(json.dumps(req).encode(), ctypes.byref(err))
+        if not h:
+            msg = _take(err.value) if err.value else "model creation failed"
+            raise QmbedError(msg)
+        self._h = h
+        self.dim = int(L.qmbed_hip_model_dimension(h))
+        self.is_real = bool(L.qmbed_hip_model_is_real(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qmbed_hip_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def states(self):
+        s = np.zeros(self.dim, dtype=np.uint32)
+        o = np.zeros(self.dim, dtype=np.uint8)
+        if library().qmbed_hip_model_states(self._h, s.ctypes.data, o.ctypes.data) != 0:
+            raise QmbedError("state copy failed")
+        return s, o
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x)
+        if x.dtype not in _DTYPES:
+            x = x.astype(np.complex128 if np.iscomplexobj(x) else np.float64)
+        y = np.empty_like(x)
+        rc = library().qmbed_hip_apply_host(self._h, _DTYPES[x.dtype], x.ctypes.data, y.ctypes.data)
+        if rc != 0:
+            raise QmbedError(f"apply failed (code {rc})")
+        return y
+
+    def eigsh(self, k, target="smallest_algebraic", **solver):
+        req = {"eigenpairs": k, "target": {"kind": target}, **solver}
+        return _call_model(self._h, req)
+
+
+def _call_model(h, req):
+    L = library()
+    response = json.loads(_take(L.qmbed_hip_eigsh_json(h, json.dumps(req).encode())))
+    if response.get("status") != "ok":
+        raise QmbedError(response.get("error", "unknown error"))
+    return response["result"]
+
+
+class DeviceBuffer:
+    """Raw device allocation (for timing device-resident work)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = library().qmbed_hip_malloc(self.nbytes)
+        if not self.ptr:
+            raise MemoryError(f"hipMalloc of {nbytes} bytes failed")
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if library().qmbed_hip_memcpy(self.ptr, arr.ctypes.data, arr.nbytes, 0) != 0:
+            raise QmbedError("upload failed")
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes
+        if library().qmbed_hip_memcpy(arr.ctypes.data, self.ptr, arr.nbytes, 1) != 0:
+            raise QmbedError("download failed")
+        return arr
+
+    def free(self):
+        if self.ptr:
+            library().qmbed_hip_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,752 @@
+// C ABI: the JSON protocol of qmbed-capi (QMBED bindings/capi/src/lib.rs:
+// qmbed_run_json :6089-6118, qmbed_command_json :6120-6151,
+// qmbed_string_free :6290-6297) for the spin-sector eigsh / evolve /
+// exponential-action path, plus a typed device-pointer fast path.
+//
+// Every entry point catches all exceptions and answers
+// {"status":"error","error":"..."}; responses are malloc'd C strings owned by
+// the caller until qmbed_string_free.
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "../../include/qmbed_hip.h"
+#include "internal.hpp"
+#include "json.hpp"
+
+namespace qmbed {
+
+namespace {
+
+using json::Value;
+
+char* dup_c(const std::string& s) {
+  char* p = static_cast<char*>(std::malloc(s.size() + 1));
+  std::memcpy(p, s.c_str(), s.size() + 1);
+  return p;
+}
+
+std::string error_json(const std::string& msg) {
+  std::string o = "{\"status\":\"error\",\"error\":";
+  json::write_string(o, msg);
+  o += "}";
+  return o;
+}
+
+// ---------------------------------------------------------- request parse --
+[[noreturn]] void bad(const std::string& what) { throw invalid_options(what); }
+
+const Value& need(const Value& obj, const char* key, const std::string& ctx) {
+  const Value* v = obj.get(key);
+  if (!v) bad(ctx + ": missing field `" + key + "`");
+  return *v;
+}
+
+long as_long(const Value& v, const std::string& what) {
+  if (v.kind != Value::Number || std::floor(v.num) != v.num) bad(what + ": expected an integer");
+  return long(v.num);
+}
+
+uint64_t as_u64(const Value& v, const std::string& what) {
+  if (v.kind != Value::Number) bad(what + ": expected an integer");
+  if (v.raw.find_first_of(".eE-") != std::string::npos) bad(what + ": expected a non-negative integer");
+  return std::strtoull(v.raw.c_str(), nullptr, 10);
+}
+
+double as_double(const Value& v, const std::string& what) {
+  if (v.kind != Value::Number) bad(what + ": expected a number");
+  return v.num;
+}
+
+bool as_bool(const Value* v, bool dflt) {
+  if (!v || v->is_null()) return dflt;
+  if (v->kind != Value::Bool) bad("expected a boolean");
+  return v->b;
+}
+
+cplx as_complex(const Value& v, const std::string& what) {
+  if (v.kind != Value::Arr || v.arr->size() != 2) bad(what + ": expected [re, im]");
+  return cplx(as_double((*v.arr)[0], what), as_double((*v.arr)[1], what));
+}
+
+std::vector<cplx> as_cvec(const Value& v, const std::string& what) {
+  if (v.kind != Value::Arr) bad(what + ": expected an array of [re, im]");
+  std::vector<cplx> out;
+  out.reserve(v.arr->size());
+  for (auto& e : *v.arr) out.push_back(as_complex(e, what));
+  return out;
+}
+
+// BasisRequest::Spin (capi :620-640, :719-721; builder :2849-2920)
+SpinBasisSpec parse_basis(const Value& b) {
+  if (b.kind != Value::Obj) bad("invalid binding request: basis must be an object");
+  const Value* kind = b.get("kind");
+  if (!kind || kind->kind != Value::String) bad("invalid binding request: missing field `kind`");
+  if (kind->raw != "spin")
+    throw unsupported_backend("the MI355X engine implements spin bases; basis kind `" + kind->raw + "` is not built");
+  SpinBasisSpec s;
+  s.sites = int(as_long(need(b, "sites", "basis"), "sites"));
+  if (s.sites < 0) bad("sites must be non-negative");
+  if (auto* v = b.get("spin_twice"); v && !v->is_null()) s.spin_twice = int(as_long(*v, "spin_twice"));
+  const Value* up = b.get("up");
+  const Value* ups = b.get("up_sectors");
+  if (up && !up->is_null() && ups && !ups->is_null()) bad("spin basis accepts either up or up_sectors, not both");
+  if (up && !up->is_null()) s.up = int(as_long(*up, "up"));
+  if (ups && !ups->is_null()) {
+    for (auto& e : ups->array()) s.up_sectors.push_back(int(as_long(e, "up_sectors")));
+    // canonical_particle_sectors: sorted unique, within capacity
+    std::sort(s.up_sectors.begin(), s.up_sectors.end());
+    s.up_sectors.erase(std::unique(s.up_sectors.begin(), s.up_sectors.end()), s.up_sectors.end());
+    if (s.up_sectors.empty()) throw invalid_sector("spin particle sectors must not be empty");
+  }
+  if (auto* v = b.get("momentum"); v && !v->is_null()) s.momentum = int(as_long(*v, "momentum"));
+  if (auto* v = b.get("parity"); v && !v->is_null()) s.parity = int(as_long(*v, "parity"));
+  const Value* norm = b.get("normalization");
+  if (norm && !norm->is_null()) {
+    if (norm->kind != Value::String) bad("normalization must be a string");
+    if (norm->raw == "angular_momentum") s.normalization = Normalization::AngularMomentum;
+    else if (norm->raw == "pauli") s.normalization = Normalization::Pauli;
+    else if (norm->raw == "pauli_cartesian") s.normalization = Normalization::PauliCartesian;
+    else bad("invalid binding request: unknown variant `" + norm->raw + "`");
+  } else {
+    s.normalization = as_bool(b.get("pauli"), false) ? Normalization::PauliCartesian : Normalization::AngularMomentum;
+  }
+  const Value* sym = b.get("symmetries");
+  if (sym && !sym->is_null() && !sym->array().empty()) {
+    if (s.momentum || s.parity) bad("built-in and general spin symmetries cannot be mixed");
+    throw unsupported_backend("general `symmetries` sectors are not built on the MI355X path");
+  }
+  if (auto* v = b.get("matrix_symmetry"); v && !v->is_null())
+    bad("matrix-valued symmetry sectors materialize as projected operator models");
+  if (as_bool(b.get("reverse"), false))
+    throw unsupported_backend("reverse representative ordering is not built on the MI355X path");
+  return s;
+}
+
+// TermRequest (capi :724-742, local names :6053-6071)
+std::vector<OperatorSpec> parse_terms(const Value& t) {
+  if (t.kind != Value::Arr) bad("invalid binding request: terms must be an array");
+  std::vector<OperatorSpec> out;
+  for (auto& term : *t.arr) {
+    const Value& prod = need(term, "product", "term");
+    const Value& local = need(prod, "local", "product");
+    OperatorSpec spec;
+    for (auto& name : local.array()) {
+      if (name.kind != Value::String) bad("local operator names must be strings");
+      const std::string& n = name.raw;
+      if (n == "identity") spec.symbols += 'I';
+      else if (n == "number") spec.symbols += 'n';
+      else if (n == "z") spec.symbols += 'z';
+      else if (n == "raising") spec.symbols += '+';
+      else if (n == "lowering") spec.symbols += '-';
+      else if (n == "x") spec.symbols += 'x';
+      else if (n == "y") spec.symbols += 'y';
+      else throw invalid_operator(n);
+    }
+    for (auto& cpl : need(term, "couplings", "term").array()) {
+      Coupling c;
+      c.coefficient = as_complex(need(cpl, "coefficient", "coupling"), "coefficient");
+      for (auto& s : need(cpl, "sites", "coupling").array()) c.sites.push_back(int(as_long(s, "sites")));
+      spec.couplings.push_back(c);
+    }
+    out.push_back(spec);
+  }
+  validate_terms(out, 0);
+  return out;
+}
+
+AssemblyChecks parse_checks(const Value* v) {
+  AssemblyChecks c;
+  if (!v || v->is_null()) return c;
+  c.hermiticity = as_bool(v->get("hermiticity"), true);
+  c.particle_conservation = as_bool(v->get("particle_conservation"), true);
+  c.symmetry_compatibility = as_bool(v->get("symmetry_compatibility"), true);
+  return c;
+}
+
+MatrixFormat parse_format(const Value* v) {
+  if (!v || v->is_null()) return MatrixFormat::Csc;
+  if (v->kind != Value::String) bad("format must be a string");
+  const std::string& f = v->raw;
+  if (f == "dense") return MatrixFormat::Dense;
+  if (f == "csc") return MatrixFormat::Csc;
+  if (f == "csr") return MatrixFormat::Csr;
+  if (f == "dia") return MatrixFormat::Dia;
+  if (f == "matrix_free") return MatrixFormat::MatrixFree;
+  bad("invalid binding request: unknown variant `" + f + "`");
+}
+
+struct SolverReq {
+  EigshOptions opts;
+  bool eigenvectors = false;
+  std::vector<cplx> initial;
+  bool has_initial = false;
+  EigshControls ctl;
+};
+
+// SolverRequest (capi :2064-2150): defaults tol 1e-10, iterations 1000, seed 0
+SolverReq parse_solver(const Value& s) {
+  SolverReq r;
+  r.opts.eigenpairs = int(as_long(need(s, "eigenpairs", "solver"), "eigenpairs"));
+  const Value& tg = need(s, "target", "solver");
+  const Value* kind = tg.get("kind");
+  if (!kind || kind->kind != Value::String) bad("invalid binding request: missing field `kind`");
+  const std::string& k = kind->raw;
+  if (k == "smallest_algebraic") r.opts.target = SpectrumTarget::SmallestAlgebraic;
+  else if (k == "largest_algebraic") r.opts.target = SpectrumTarget::LargestAlgebraic;
+  else if (k == "smallest_magnitude") r.opts.target = SpectrumTarget::SmallestMagnitude;
+  else if (k == "largest_magnitude") r.opts.target = SpectrumTarget::LargestMagnitude;
+  else if (k == "both_ends") r.opts.target = SpectrumTarget::BothEnds;
+  else if (k == "shift") {
+    r.opts.target = SpectrumTarget::Shift;
+    r.opts.shift = as_double(need(tg, "value", "target"), "value");
+  } else bad("invalid binding request: unknown variant `" + k + "`");
+  if (auto* v = s.get("krylov_dimension"); v && !v->is_null()) r.opts.krylov_dimension = as_long(*v, "krylov_dimension");
+  if (auto* v = s.get("tolerance"); v && !v->is_null()) r.opts.tolerance = as_double(*v, "tolerance");
+  if (auto* v = s.get("max_iterations"); v && !v->is_null()) r.opts.max_iterations = as_long(*v, "max_iterations");
+  if (auto* v = s.get("seed"); v && !v->is_null()) r.opts.seed = as_u64(*v, "seed");
+  r.eigenvectors = as_bool(s.get("eigenvectors"), false);
+  const Value* iv = s.get("initial_vector");
+  if (!iv || iv->is_null()) iv = s.get("v0");
+  if (iv && !iv->is_null()) {
+    r.initial = as_cvec(*iv, "initial_vector");
+    r.has_initial = true;
+  }
+  // engine controls (extensions; absent in reference requests)
+  if (auto* v = s.get("matvec_precision"); v && v->kind == Value::String) {
+    if (v->raw == "fp32") r.ctl.precision = MatvecPrecision::FP32;
+    else if (v->raw == "fp64") r.ctl.precision = MatvecPrecision::FP64;
+    else bad("matvec_precision must be \"fp64\" or \"fp32\"");
+  }
+  if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
+  return r;
+}
+
+// ------------------------------------------------------------------ model --
+struct Model {
+  SpinBasisSpec spec;
+  std::vector<OperatorSpec> terms;
+  AssemblyChecks checks;
+  std::shared_ptr<SpinBasis1D> basis;
+  std::shared_ptr<Operator> op;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  ~Model() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  const Operator& materialized() {
+    if (!op) op = OperatorBuilder::on(basis).terms(terms).checks(checks).build(MatrixFormat::MatrixFree);
+    return *op;
+  }
+};
+
+std::shared_ptr<Model> make_model(const Value& basis, const Value& terms, const Value* checks) {
+  auto m = std::make_shared<Model>();
+  m->spec = parse_basis(basis);
+  m->terms = parse_terms(terms);
+  m->checks = parse_checks(checks);
+  QMBED_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  m->basis = SpinBasis1D::build(m->spec, m->stream);
+  return m;
+}
+
+std::mutex g_reg_mu;
+std::map<uint64_t, std::shared_ptr<Model>> g_models;
+std::atomic<uint64_t> g_next_model{1};
+
+struct ExpmEntry {
+  std::shared_ptr<Model> model;
+  std::shared_ptr<ExpmActionPlan> plan;
+};
+std::map<uint64_t, std::shared_ptr<ExpmEntry>> g_expm;
+std::atomic<uint64_t> g_next_expm{1};
+
+uint64_t parse_model_handle(const std::string& h) {
+  char* end = nullptr;
+  errno = 0;
+  unsigned long long v = h.empty() ? 0 : std::strtoull(h.c_str(), &end, 10);
+  if (h.empty() || !end || *end || errno || h[0] == '-' || h[0] == '+') {
+    std::string q;
+    json::write_string(q, h);
+    bad("model handle " + q + " is not a positive integer");
+  }
+  if (v == 0) bad("model handle must be positive");
+  return v;
+}
+
+std::shared_ptr<Model> registered_model(const std::string& h) {
+  uint64_t id = parse_model_handle(h);
+  std::lock_guard<std::mutex> g(g_reg_mu);
+  auto it = g_models.find(id);
+  if (it == g_models.end()) bad("model handle " + std::to_string(id) + " is not registered");
+  return it->second;
+}
+
+uint64_t parse_expm_handle(const std::string& h) {
+  std::string q;
+  json::write_string(q, h);
+  if (h.rfind("expm:", 0) != 0) bad("exponential-action handle " + q + " does not use the expm: prefix");
+  std::string rest = h.substr(5);
+  char* end = nullptr;
+  unsigned long long v = rest.empty() ? 0 : std::strtoull(rest.c_str(), &end, 10);
+  if (rest.empty() || !end || *end || rest[0] == '-' || rest[0] == '+') bad("exponential-action handle " + q + " is invalid");
+  if (v == 0) bad("exponential-action handle must be positive");
+  return v;
+}
+
+// ---------------------------------------------------------------- writers --
+void write_vec(std::string& o, const std::vector<cplx>& v) {
+  o += '[';
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (i) o += ',';
+    o += '[';
+    json::write_number(o, v[i].real());
+    o += ',';
+    json::write_number(o, v[i].imag());
+    o += ']';
+  }
+  o += ']';
+}
+
+void write_dvec(std::string& o, const std::vector<double>& v) {
+  o += '[';
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (i) o += ',';
+    json::write_number(o, v[i]);
+  }
+  o += ']';
+}
+
+std::string eigensystem_body(size_t dim, const Eigensystem& es, bool vectors, bool tagged) {
+  std::string o = "{";
+  if (tagged) o += "\"kind\":\"eigensystem\",";
+  o += "\"dimension\":" + std::to_string(dim) + ",\"eigenvalues\":";
+  write_dvec(o, es.eigenvalues);
+  o += ",\"residuals\":";
+  write_dvec(o, es.residuals);
+  o += ",\"iterations\":" + std::to_string(es.iterations);
+  o += std::string(",\"converged\":") + (es.converged ? "true" : "false");
+  if (vectors) {
+    o += ",\"eigenvectors\":[";
+    for (size_t q = 0; q < es.eigenvectors.size(); ++q) {
+      if (q) o += ',';
+      write_vec(o, es.eigenvectors[q]);
+    }
+    o += "]";
+  }
+  o += "}";
+  return o;
+}
+
+Eigensystem solve_model(Model& m, const SolverReq& r) {
+  std::lock_guard<std::mutex> g(m.mu);
+  const Operator& op = m.materialized();
+  if (r.has_initial && r.initial.size() != op.dim())
+    throw dimension_mismatch("eigsh initial vector does not match the operator");
+  return eigsh(op, r.opts, r.has_initial ? r.initial.data() : nullptr, r.ctl, r.eigenvectors, m.stream);
+}
+
+std::string run_solve(const std::string& req) {
+  Value v;
+  try {
+    v = json::parse(req);
+  } catch (json::ParseError& e) {
+    throw invalid_options(std::string("invalid binding request: ") + e.what());
+  }
+  if (v.kind != Value::Obj) bad("invalid binding request: expected an object");
+  auto m = make_model(need(v, "basis", "invalid binding request"), need(v, "terms", "invalid binding request"),
+                      nullptr);
+  (void)parse_format(v.get("format"));
+  SolverReq r = parse_solver(need(v, "solver", "invalid binding request"));
+  Eigensystem es = solve_model(*m, r);
+  return "{\"status\":\"ok\",\"result\":" + eigensystem_body(m->basis->len(), es, r.eigenvectors, false) + "}";
+}
+
+std::string ok(const std::string& body) { return "{\"status\":\"ok\",\"result\":" + body + "}"; }
+
+std::string run_command(const std::string& req) {
+  Value v;
+  try {
+    v = json::parse(req);
+  } catch (json::ParseError& e) {
+    throw invalid_options(std::string("invalid binding command: ") + e.what());
+  }
+  const Value* opv = v.get("operation");
+  if (!opv || opv->kind != Value::String) bad("invalid binding command: missing field `operation`");
+  const std::string& op = opv->raw;
+
+  if (op == "describe_basis") {
+    SpinBasisSpec spec = parse_basis(need(v, "basis", "describe_basis"));
+    auto b = SpinBasis1D::build(spec, nullptr);
+    auto states = b->states_host();
+    std::string o = "{\"kind\":\"basis\",\"dimension\":" + std::to_string(b->len()) + ",\"states\":[";
+    for (size_t i = 0; i < states.size(); ++i) {
+      if (i) o += ',';
+      o += '"' + std::to_string(states[i]) + '"';
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "create_model") {
+    if (auto* c = v.get("components"); c && !c->array().empty())
+      throw unsupported_backend("parameterized model components are not built on the MI355X path");
+    if (auto* p = v.get("site_permutation"); p && !p->is_null())
+      throw unsupported_backend("site permutations are not built on the MI355X path");
+    auto m = make_model(need(v, "basis", "create_model"), need(v, "terms", "create_model"), v.get("checks"));
+    uint64_t id = g_next_model.fetch_add(1);
+    size_t dim = m->basis->len();
+    {
+      std::lock_guard<std::mutex> g(g_reg_mu);
+      g_models[id] = m;
+    }
+    return ok("{\"kind\":\"model\",\"handle\":\"" + std::to_string(id) + "\",\"dimension\":" + std::to_string(dim) +
+              "}");
+  }
+  if (op == "eigsh_model") {
+    const Value& h = need(v, "handle", "eigsh_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    (void)parse_format(v.get("format"));
+    if (auto* p = v.get("parameters"); p && p->kind == Value::Obj && !p->obj->empty())
+      throw unsupported_backend("parameterized models are not built on the MI355X path");
+    SolverReq r = parse_solver(need(v, "solver", "eigsh_model"));
+    Eigensystem es = solve_model(*m, r);
+    return ok(eigensystem_body(m->basis->len(), es, r.eigenvectors, true));
+  }
+  if (op == "evolve_model") {
+    const Value& h = need(v, "handle", "evolve_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    const Value& ev = need(v, "evolution", "evolve_model");
+    EvolutionOptions eo;
+    for (auto& t : need(ev, "times", "evolution").array()) eo.times.push_back(as_double(t, "times"));
+    eo.krylov_dimension = 64;  // capi default (:596-606)
+    if (auto* x = ev.get("krylov_dimension"); x && !x->is_null()) eo.krylov_dimension = as_long(*x, "krylov_dimension");
+    if (auto* x = ev.get("tolerance"); x && !x->is_null()) eo.tolerance = as_double(*x, "tolerance");
+    if (auto* x = ev.get("max_substeps"); x && !x->is_null()) eo.max_substeps = as_long(*x, "max_substeps");
+    if (as_bool(ev.get("imaginary_time"), false))
+      bad("static imaginary-time evolution is not exposed by this command yet");
+    std::vector<std::vector<cplx>> cols;
+    for (auto& c : need(v, "vectors", "evolve_model").array()) cols.push_back(as_cvec(c, "vectors"));
+    std::lock_guard<std::mutex> g(m->mu);
+    const Operator& H = m->materialized();
+    auto traj = evolve_batch(H, cols, eo, nullptr, m->stream);
+    std::string o = "{\"kind\":\"trajectory\",\"dimension\":" + std::to_string(H.dim()) + ",\"times\":";
+    write_dvec(o, eo.times);
+    o += ",\"states\":[";
+    for (size_t t = 0; t < traj.size(); ++t) {
+      if (t) o += ',';
+      o += '[';
+      for (size_t c = 0; c < traj[t].size(); ++c) {
+        if (c) o += ',';
+        write_vec(o, traj[t][c]);
+      }
+      o += ']';
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "create_expm_action") {
+    const Value& h = need(v, "handle", "create_expm_action");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    cplx coef = as_complex(need(v, "coefficient", "create_expm_action"), "coefficient");
+    long max_degree = 55;
+    double tol = 0.5 * 2.220446049250313e-16;
+    long max_sub = 10000;
+    if (auto* x = v.get("max_degree"); x && !x->is_null()) max_degree = as_long(*x, "max_degree");
+    if (auto* x = v.get("tolerance"); x && !x->is_null()) tol = as_double(*x, "tolerance");
+    if (auto* x = v.get("max_substeps"); x && !x->is_null()) max_sub = as_long(*x, "max_substeps");
+    if (auto* p = v.get("parameters"); p && p->kind == Value::Obj && !p->obj->empty())
+      throw unsupported_backend("parameterized models are not built on the MI355X path");
+    std::shared_ptr<ExpmEntry> e = std::make_shared<ExpmEntry>();
+    e->model = m;
+    {
+      std::lock_guard<std::mutex> g(m->mu);
+      m->materialized();
+      e->plan = std::make_shared<ExpmActionPlan>(m->op, coef, int(max_degree), tol, max_sub, m->stream);
+    }
+    uint64_t id = g_next_expm.fetch_add(1);
+    {
+      std::lock_guard<std::mutex> g(g_reg_mu);
+      g_expm[id] = e;
+    }
+    return ok("{\"kind\":\"expm_plan\",\"handle\":\"expm:" + std::to_string(id) +
+              "\",\"dimension\":" + std::to_string(m->basis->len()) + "}");
+  }
+  if (op == "apply_expm_action") {
+    const Value& h = need(v, "expm_handle", "apply_expm_action");
+    if (h.kind != Value::String) bad("expm_handle must be a string");
+    uint64_t id = parse_expm_handle(h.raw);
+    std::shared_ptr<ExpmEntry> e;
+    {
+      std::lock_guard<std::mutex> g(g_reg_mu);
+      auto it = g_expm.find(id);
+      if (it == g_expm.end()) bad("exponential-action handle " + h.raw + " is not registered");
+      e = it->second;
+    }
+    if (auto* t = v.get("threads"); t && !t->is_null()) {
+      long th = as_long(*t, "threads");
+      if (th <= 0) bad("runtime thread count must be positive");
+    }
+    std::vector<std::vector<cplx>> cols;
+    for (auto& c : need(v, "vectors", "apply_expm_action").array()) cols.push_back(as_cvec(c, "vectors"));
+    for (auto& c : cols)
+      if (c.size() != e->plan->dimension()) throw dimension_mismatch("exponential batch column has the wrong length");
+    std::vector<std::vector<cplx>> out;
+    {
+      std::lock_guard<std::mutex> g(e->model->mu);
+      out = e->plan->apply(cols, e->model->stream);
+    }
+    size_t dim = out.empty() ? e->plan->dimension() : out[0].size();
+    std::string o = "{\"kind\":\"vectors\",\"dimension\":" + std::to_string(dim) + ",\"vectors\":[";
+    for (size_t c = 0; c < out.size(); ++c) {
+      if (c) o += ',';
+      write_vec(o, out[c]);
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "release_model") {
+    const Value& h = need(v, "handle", "release_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    uint64_t id = parse_model_handle(h.raw);
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    if (!g_models.erase(id)) bad("model handle " + std::to_string(id) + " is not registered");
+    return ok("{\"kind\":\"released\",\"handle\":\"" + std::to_string(id) + "\"}");
+  }
+  if (op == "release_expm_action") {
+    const Value& h = need(v, "expm_handle", "release_expm_action");
+    if (h.kind != Value::String) bad("expm_handle must be a string");
+    uint64_t id = parse_expm_handle(h.raw);
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    if (!g_expm.erase(id)) bad("exponential-action handle " + h.raw + " is not registered");
+    return ok("{\"kind\":\"released\",\"handle\":\"expm:" + std::to_string(id) + "\"}");
+  }
+  throw unsupported_backend("operation `" + op + "` is outside the MI355X spin eigsh/evolve/expm path");
+}
+
+template <typename F>
+char* guarded(const char* request, F&& f) {
+  std::string resp;
+  try {
+    if (!request) {
+      resp = "{\"status\":\"error\",\"error\":\"request pointer is null\"}";
+    } else {
+      resp = f(std::string(request));
+    }
+  } catch (Error& e) {
+    resp = error_json(e.message);
+  } catch (std::exception& e) {
+    resp = error_json(std::string("internal state error: ") + e.what());
+  } catch (...) {
+    resp = error_json("internal state error: native binding failure");
+  }
+  return dup_c(resp);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ typed --
+struct TypedModel {
+  std::shared_ptr<Model> m;
+};
+
+}  // namespace qmbed
+
+using namespace qmbed;
+
+extern "C" {
+
+char* qmbed_run_json(const char* request) { return guarded(request, run_solve); }
+
+char* qmbed_command_json(const char* request) { return guarded(request, run_command); }
+
+char* qmbed_evolve_model_with_drive_json(const char* request, qmbed_drive_callback callback, void* context) {
+  (void)callback;
+  (void)context;
+  return guarded(request, [](const std::string&) -> std::string {
+    throw unsupported_backend("driven (time-dependent) evolution is outside the MI355X static-Hamiltonian path");
+  });
+}
+
+char* qmbed_register_user_basis_32_json(const char* request, qmbed_user_basis_callback_32 callback, void* context) {
+  (void)callback;
+  (void)context;
+  return guarded(request, [](const std::string&) -> std::string {
+    throw unsupported_backend("user-defined bases are outside the MI355X spin-sector path");
+  });
+}
+
+char* qmbed_register_user_basis_64_json(const char* request, qmbed_user_basis_callback_64 callback, void* context) {
+  (void)callback;
+  (void)context;
+  return guarded(request, [](const std::string&) -> std::string {
+    throw unsupported_backend("user-defined bases are outside the MI355X spin-sector path");
+  });
+}
+
+void qmbed_string_free(char* s) { std::free(s); }
+
+// ----------------------------------------------------------- typed path ----
+qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error) {
+  try {
+    if (!model_json) throw invalid_options("request pointer is null");
+    json::Value v;
+    try {
+      v = json::parse(model_json);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    auto tm = new TypedModel();
+    try {
+      tm->m = make_model(need(v, "basis", "model"), need(v, "terms", "model"), v.get("checks"));
+      tm->m->materialized();
+    } catch (...) {
+      delete tm;
+      throw;
+    }
+    if (error) *error = nullptr;
+    return reinterpret_cast<qmbed_hip_model*>(tm);
+  } catch (Error& e) {
+    if (error) *error = dup_c(e.message);
+  } catch (std::exception& e) {
+    if (error) *error = dup_c(std::string("internal state error: ") + e.what());
+  }
+  return nullptr;
+}
+
+void qmbed_hip_model_free(qmbed_hip_model* m) { delete reinterpret_cast<TypedModel*>(m); }
+
+long qmbed_hip_model_dimension(const qmbed_hip_model* m) {
+  return long(reinterpret_cast<const TypedModel*>(m)->m->basis->len());
+}
+
+int qmbed_hip_model_is_real(const qmbed_hip_model* m) {
+  return reinterpret_cast<const TypedModel*>(m)->m->op->is_real() ? 1 : 0;
+}
+
+int qmbed_hip_model_states(const qmbed_hip_model* m, uint32_t* states, uint8_t* orbits) {
+  try {
+    auto& b = *reinterpret_cast<const TypedModel*>(m)->m->basis;
+    auto s = b.states_host();
+    std::memcpy(states, s.data(), s.size() * 4);
+    if (orbits) {
+      auto o = b.orbits_host();
+      std::memcpy(orbits, o.data(), o.size());
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+void* qmbed_hip_model_stream(const qmbed_hip_model* m) { return reinterpret_cast<const TypedModel*>(m)->m->stream; }
+
+int qmbed_hip_apply_device(const qmbed_hip_model* m, int dtype, const void* x, void* y, void* stream) {
+  try {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    const Operator& op = *tm->m->op;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : tm->m->stream;
+    switch (dtype) {
+      case QMBED_F64: op.apply(static_cast<const double*>(x), static_cast<double*>(y), s); break;
+      case QMBED_F32: op.apply(static_cast<const float*>(x), static_cast<float*>(y), s); break;
+      case QMBED_C128: op.apply(static_cast<const cplx*>(x), static_cast<cplx*>(y), s); break;
+      case QMBED_C64:
+        op.apply(static_cast<const std::complex<float>*>(x), static_cast<std::complex<float>*>(y), s);
+        break;
+      default: return -2;
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+static size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case QMBED_F64: return 8;
+    case QMBED_F32: return 4;
+    case QMBED_C128: return 16;
+    case QMBED_C64: return 8;
+    default: return 0;
+  }
+}
+
+int qmbed_hip_apply_host(const qmbed_hip_model* m, int dtype, const void* x, void* y) {
+  try {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    size_t n = tm->m->basis->len();
+    size_t bytes = dtype_size(dtype) * n;
+    if (!bytes) return dtype_size(dtype) ? 0 : -2;
+    DeviceArray<uint8_t> dx(bytes), dy(bytes);
+    QMBED_HIP(hipMemcpy(dx.data(), x, bytes, hipMemcpyHostToDevice));
+    int rc = qmbed_hip_apply_device(m, dtype, dx.data(), dy.data(), nullptr);
+    if (rc) return rc;
+    QMBED_HIP(hipStreamSynchronize(tm->m->stream));
+    QMBED_HIP(hipMemcpy(y, dy.data(), bytes, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+double qmbed_hip_time_apply(const qmbed_hip_model* m, int dtype, const void* x, void* y, int iters) {
+  try {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    hipStream_t s = tm->m->stream;
+    hipEvent_t a, b;
+    QMBED_HIP(hipEventCreate(&a));
+    QMBED_HIP(hipEventCreate(&b));
+    QMBED_HIP(hipEventRecord(a, s));
+    for (int i = 0; i < iters; ++i)
+      if (qmbed_hip_apply_device(m, dtype, x, y, s) != 0) return -1.0;
+    QMBED_HIP(hipEventRecord(b, s));
+    QMBED_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    QMBED_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms) / double(std::max(iters, 1));
+  } catch (...) {
+    return -1.0;
+  }
+}
+
+char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
+  return guarded(solver_json, [&](const std::string& req) {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    json::Value v;
+    try {
+      v = json::parse(req);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    SolverReq r = parse_solver(v);
+    Eigensystem es = solve_model(*tm->m, r);
+    std::string body = eigensystem_body(tm->m->basis->len(), es, r.eigenvectors, false);
+    body.pop_back();
+    body += ",\"fp32_iterations\":" + std::to_string(es.fp32_iterations) +
+            ",\"fp64_iterations\":" + std::to_string(es.fp64_iterations) +
+            ",\"reorthogonalization_passes\":" + std::to_string(es.reorthogonalization_passes) +
+            ",\"conditional_second_passes\":" + std::to_string(es.conditional_second_passes) + "}";
+    return ok(body);
+  });
+}
+
+void* qmbed_hip_malloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  return p;
+}
+void qmbed_hip_free(void* p) { (void)hipFree(p); }
+int qmbed_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : -1;
+}
+int qmbed_hip_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+int qmbed_hip_device_count(void) { return device_count(); }
+
+}  // extern "C"

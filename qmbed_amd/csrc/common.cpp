@@ -1,0 +1,77 @@
+// Error constructors (display strings follow QMBED src/error.rs:39-73) and
+// small host utilities shared by the library.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <sstream>
+
+#include "qmbed.hpp"
+
+namespace qmbed {
+
+Error invalid_operator(const std::string& s) {
+  return Error(ErrorKind::InvalidOperator, "invalid or unregistered operator: " + s);
+}
+Error invalid_coupling(const std::string& s) { return Error(ErrorKind::InvalidCoupling, "invalid coupling: " + s); }
+Error invalid_site(long site, long sites) {
+  std::ostringstream os;
+  os << "site " << site << " is outside a lattice with " << sites << " sites";
+  return Error(ErrorKind::InvalidSite, os.str());
+}
+Error invalid_sector(const std::string& s) {
+  return Error(ErrorKind::InvalidSector, "invalid Hilbert-space sector: " + s);
+}
+Error incompatible_symmetry(const std::string& s) {
+  return Error(ErrorKind::IncompatibleSymmetry, "incompatible symmetry request: " + s);
+}
+Error non_hermitian() { return Error(ErrorKind::NonHermitian, "a Hermitian operator is required"); }
+Error invalid_options(const std::string& s) { return Error(ErrorKind::InvalidOptions, "invalid options: " + s); }
+Error dimension_mismatch(const std::string& s) {
+  return Error(ErrorKind::DimensionMismatch, "dimension mismatch: " + s);
+}
+Error unsupported_backend(const std::string& s) {
+  return Error(ErrorKind::UnsupportedBackend, "unsupported backend: " + s);
+}
+Error internal_state(const std::string& s) { return Error(ErrorKind::InternalState, "internal state error: " + s); }
+
+// Rust's `{:e}` formatting of f64: shortest round-trip mantissa, exponent
+// without padding or '+', e.g. 1.5e-7, 1e0, inf.
+static std::string rust_exp(double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+  if (v == 0.0) return std::signbit(v) ? "-0e0" : "0e0";
+  char buf[64];
+  for (int prec = 0; prec < 18; ++prec) {
+    std::snprintf(buf, sizeof buf, "%.*e", prec, v);
+    if (std::strtod(buf, nullptr) == v) break;
+  }
+  std::string s(buf);
+  auto epos = s.find('e');
+  std::string mant = s.substr(0, epos);
+  int ex = std::atoi(s.c_str() + epos + 1);
+  return mant + "e" + std::to_string(ex);
+}
+
+Error non_convergence(long iterations, double residual) {
+  std::ostringstream os;
+  os << "solver did not converge after " << iterations << " iterations; residual=" << rust_exp(residual);
+  Error e(ErrorKind::NonConvergence, os.str());
+  e.iterations = iterations;
+  e.residual = residual;
+  return e;
+}
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    std::string msg = std::string("HIP call failed: ") + what + ": " + hipGetErrorString(e);
+    throw Error(ErrorKind::InternalState, "internal state error: " + msg);
+  }
+}
+
+int device_count() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+}  // namespace qmbed

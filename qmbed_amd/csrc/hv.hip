@@ -1,0 +1,632 @@
+// Matrix-free, symmetry-reduced H·v for spin-1/2 chains on gfx950.
+//
+// One lane per output row (gather form, no atomics, deterministic order):
+//   r   = representative of row `row`                  (coalesced u32 read)
+//   y_r = Σ_diag-groups a(pattern(r)) · x_r
+//       + Σ_{flip groups g active on r} a_g(pattern(r)) · sqrt(N_r/N_t) · χ(t) · x[idx(rep(t))]
+// where t = r XOR flip_g. rep(t) and the group element mapping rep(t) to t are
+// found by walking t's translation (and reflection) orbit in registers; the
+// characters χ are tabulated in LDS; idx(rep) comes from a binary search of
+// the sorted representative table whose top levels (one splitter per
+// 2^split_shift states) are staged in LDS.
+//
+// This evaluates the same matrix as the reference's column-streaming
+// assembly H[row(rep_t), col(r)] += c·a·sqrt(N_r/N_t)·conj(phase_t)
+// (QMBED src/operator/mod.rs:2119-2205, src/basis/mod.rs:1561-1580), read
+// row-wise: the row entries are the conjugated column entries of the adjoint
+// product, which is what the flip-group tables hold.
+#include <algorithm>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace qmbed {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxGroups = 128;
+constexpr int kMaxAmps = 1024;
+
+struct HvArgs {
+  const uint32_t* states;
+  const uint8_t* orbits;
+  uint32_t n;
+  int L;
+  uint32_t mask;
+  int parity;
+  int n_div;
+  int divisors[16];
+  // search
+  const uint32_t* spl;
+  int nspl;   // power of two
+  int sh;
+  // groups (diag first, then off-diagonal)
+  const uint32_t* meta;  // 4 words per group: flip, packed sites, nzmask, amp offset
+  const double* amp;     // real: 1 double per entry; complex: 2 doubles per entry
+  int n_diag, n_off, n_amp;
+  const double* chars;   // 2*L doubles (re, im) of χ(a) = exp(-2πi k a / L)
+  // batched columns
+  int nvec;
+  size_t ld;
+};
+
+__device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t mask) {
+  return uint32_t(y >> (L - j)) & mask;
+}
+__device__ __forceinline__ uint32_t reflect_l(uint32_t s, int L) { return __brev(s) >> (32 - L); }
+
+__device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
+  const int ns = packed & 7;
+  int pat = 0;
+  for (int i = 0; i < ns; ++i) {
+    int site = (packed >> (3 + 5 * i)) & 31;
+    pat |= int((r >> site) & 1u) << i;
+  }
+  return pat;
+}
+
+__device__ __forceinline__ int find_state(uint32_t key, const uint32_t* spl, int nspl, int sh,
+                                          const uint32_t* __restrict__ states, uint32_t n) {
+  int pos = 0;
+  for (int step = nspl >> 1; step > 0; step >>= 1)
+    if (spl[pos + step] <= key) pos += step;
+  uint32_t base = uint32_t(pos) << sh;
+  for (uint32_t step = (1u << sh) >> 1; step > 0; step >>= 1) {
+    uint32_t c = base + step;
+    if (c < n && __ldg(states + c) <= key) base = c;
+  }
+  return (base < n && __ldg(states + base) == key) ? int(base) : -1;
+}
+
+// Orbit walk of a parent state t.
+// SYM: 1 translation, 2 reflection only, 3 translation + reflection.
+// On return: rep, orbit size, and the phase as (ci: character index, sign)
+// such that phase = sign * χ(ci).
+template <int SYM, bool SHIFT>
+__device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci, int& sign) {
+  const int L = a.L;
+  sign = 1;
+  ci = 0;
+  if (SYM == 2) {
+    uint32_t ps = reflect_l(t, L);
+    if (ps < t) {
+      rep = ps;
+      orbit = 2;
+      sign = a.parity;
+    } else {
+      rep = t;
+      orbit = (ps == t) ? 1 : 2;
+    }
+    return;
+  }
+  const uint64_t y = uint64_t(t) | (uint64_t(t) << L);
+  uint32_t mn = t;
+  int jmin = 0;
+  for (int j = 1; j < L; ++j) {
+    uint32_t r = rotl_l(y, L, j, a.mask);
+    if (SHIFT) {
+      if (r < mn) {
+        mn = r;
+        jmin = j;
+      }
+    } else {
+      mn = min(mn, r);
+    }
+  }
+  int d = L;
+  for (int i = a.n_div - 1; i >= 0; --i) {
+    int dv = a.divisors[i];
+    if (rotl_l(y, L, dv, a.mask) == t) d = dv;
+  }
+  if (SYM == 1) {
+    rep = mn;
+    orbit = d;
+    ci = (L - jmin) % L;
+    return;
+  }
+  const uint32_t ps = reflect_l(t, L);
+  const uint64_t yp = uint64_t(ps) | (uint64_t(ps) << L);
+  uint32_t mnp = ps;
+  int jminp = 0;
+  for (int j = 1; j < L; ++j) {
+    uint32_t r = rotl_l(yp, L, j, a.mask);
+    if (SHIFT) {
+      if (r < mnp) {
+        mnp = r;
+        jminp = j;
+      }
+    } else {
+      mnp = min(mnp, r);
+    }
+  }
+  if (mnp < mn) {
+    rep = mnp;
+    orbit = 2 * d;
+    ci = (L - jminp) % L;
+    sign = a.parity;
+  } else {
+    rep = mn;
+    orbit = (mnp == mn) ? d : 2 * d;
+    ci = (L - jmin) % L;
+  }
+}
+
+template <typename S>
+struct Cx {
+  S re, im;
+};
+
+// CMODE 0: real vectors, real coefficients
+// CMODE 1: complex vectors, real coefficients (real characters)
+// CMODE 2: complex vectors, complex coefficients/characters
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
+__global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
+  extern __shared__ double smem_d[];
+  double* s_amp = smem_d;                                  // n_amp or 2*n_amp
+  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
+  double* s_sq = s_chr + 2 * 32;                           // 65
+  double* s_rsq = s_sq + 65;                               // 65
+  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_rsq + 65);
+  uint32_t* s_spl = s_meta + 4 * (a.n_diag + a.n_off);
+
+  const int G = a.n_diag + a.n_off;
+  const int namp = (CMODE == 2 ? 2 : 1) * a.n_amp;
+  for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
+  for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
+  for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
+  if (SYM == 1 || SYM == 3)
+    for (int i = threadIdx.x; i < 2 * a.L; i += kBlock) s_chr[i] = a.chars[i];
+  for (int i = threadIdx.x; i < 65; i += kBlock) {
+    s_sq[i] = sqrt(double(i));
+    s_rsq[i] = i ? 1.0 / sqrt(double(i)) : 0.0;
+  }
+  __syncthreads();
+
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= a.n) return;
+  const uint32_t r = a.states[row];
+  S sq_r = 1;
+  if (SYM) sq_r = S(s_sq[a.orbits[row]]);
+
+  // diagonal
+  Cx<S> dsum{0, 0};
+  for (int g = 0; g < a.n_diag; ++g) {
+    const int pat = pattern_of(r, s_meta[4 * g + 1]);
+    const int off = int(s_meta[4 * g + 3]) + pat;
+    if (CMODE == 2) {
+      dsum.re += S(s_amp[2 * off]);
+      dsum.im += S(s_amp[2 * off + 1]);
+    } else {
+      dsum.re += S(s_amp[off]);
+    }
+  }
+  Cx<S> acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (CMODE == 0) {
+      acc[v].re = dsum.re * x[v * a.ld + row];
+      acc[v].im = 0;
+    } else {
+      const S xr = x[2 * (v * a.ld + row)], xi = x[2 * (v * a.ld + row) + 1];
+      acc[v].re = dsum.re * xr - dsum.im * xi;
+      acc[v].im = dsum.re * xi + dsum.im * xr;
+    }
+  }
+
+  // off-diagonal
+  for (int w0 = 0; w0 < a.n_off; w0 += 32) {
+    uint32_t act = 0;
+    const int gend = min(32, a.n_off - w0);
+    for (int gi = 0; gi < gend; ++gi) {
+      const int g = a.n_diag + w0 + gi;
+      const int pat = pattern_of(r, s_meta[4 * g + 1]);
+      act |= ((s_meta[4 * g + 2] >> pat) & 1u) << gi;
+    }
+    while (act) {
+      const int gi = __ffs(act) - 1;
+      act &= act - 1;
+      const int g = a.n_diag + w0 + gi;
+      const uint32_t t = r ^ s_meta[4 * g];
+      const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+      uint32_t rep = t;
+      int orbit = 1, ci = 0, sign = 1;
+      if (SYM) walk<SYM, SHIFT>(t, a, rep, orbit, ci, sign);
+      const int idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.n);
+      if (idx < 0) continue;
+      Cx<S> c;
+      if (CMODE == 2) {
+        c.re = S(s_amp[2 * off]);
+        c.im = S(s_amp[2 * off + 1]);
+      } else {
+        c.re = S(s_amp[off]);
+        c.im = 0;
+      }
+      S scale = S(sign);
+      if (SYM) scale *= sq_r * S(s_rsq[orbit]);
+      c.re *= scale;
+      c.im *= scale;
+      if ((SYM == 1 || SYM == 3) && SHIFT) {
+        const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+        if (CMODE == 2) {
+          const S re = c.re * pr - c.im * pi;
+          c.im = c.re * pi + c.im * pr;
+          c.re = re;
+        } else {
+          c.re *= pr;  // real characters (k = L/2): pi == 0
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (CMODE == 0) {
+          acc[v].re += c.re * x[v * a.ld + idx];
+        } else {
+          const S xr = x[2 * (v * a.ld + idx)], xi = x[2 * (v * a.ld + idx) + 1];
+          acc[v].re += c.re * xr - c.im * xi;
+          acc[v].im += c.re * xi + c.im * xr;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (CMODE == 0) {
+      y[v * a.ld + row] = acc[v].re;
+    } else {
+      y[2 * (v * a.ld + row)] = acc[v].re;
+      y[2 * (v * a.ld + row) + 1] = acc[v].im;
+    }
+  }
+}
+
+// Row-wise exact entries for small dense materialization and the one-norm:
+// each lane gathers its row's (column, value) pairs, merged per column.
+constexpr int kMaxRowEntries = 128;
+
+__global__ void __launch_bounds__(kBlock) row_entries_kernel(HvArgs a, int sym, int shift_needed, int complex_coef,
+                                                             cplx* dense, double* rowsum, double* diag_re,
+                                                             double* diag_im, int* overflow, cplx shift) {
+  extern __shared__ double smem_d[];
+  double* s_amp = smem_d;
+  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_amp + 2 * a.n_amp);
+  uint32_t* s_spl = s_meta + 4 * (a.n_diag + a.n_off);
+  const int G = a.n_diag + a.n_off;
+  for (int i = threadIdx.x; i < (complex_coef ? 2 : 1) * a.n_amp; i += kBlock) s_amp[i] = a.amp[i];
+  for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
+  for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= a.n) return;
+  const uint32_t r = a.states[row];
+  const double nr = sym ? double(a.orbits[row]) : 1.0;
+  int cols[kMaxRowEntries];
+  double vre[kMaxRowEntries], vim[kMaxRowEntries];
+  int cnt = 0;
+  double dre = 0, dim_ = 0;
+  for (int g = 0; g < a.n_diag; ++g) {
+    const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+    dre += complex_coef ? s_amp[2 * off] : s_amp[off];
+    dim_ += complex_coef ? s_amp[2 * off + 1] : 0.0;
+  }
+  for (int g = a.n_diag; g < G; ++g) {
+    const int pat = pattern_of(r, s_meta[4 * g + 1]);
+    if (!((s_meta[4 * g + 2] >> pat) & 1u)) continue;
+    const uint32_t t = r ^ s_meta[4 * g];
+    const int off = int(s_meta[4 * g + 3]) + pat;
+    uint32_t rep = t;
+    int orbit = 1, ci = 0, sign = 1;
+    if (sym == 1) {
+      if (shift_needed) walk<1, true>(t, a, rep, orbit, ci, sign);
+      else walk<1, false>(t, a, rep, orbit, ci, sign);
+    } else if (sym == 2) {
+      walk<2, false>(t, a, rep, orbit, ci, sign);
+    } else if (sym == 3) {
+      if (shift_needed) walk<3, true>(t, a, rep, orbit, ci, sign);
+      else walk<3, false>(t, a, rep, orbit, ci, sign);
+    }
+    const int idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.n);
+    if (idx < 0) continue;
+    double cre = complex_coef ? s_amp[2 * off] : s_amp[off];
+    double cim = complex_coef ? s_amp[2 * off + 1] : 0.0;
+    double scale = double(sign) * (sym ? sqrt(nr / double(orbit)) : 1.0);
+    cre *= scale;
+    cim *= scale;
+    if (sym == 1 || sym == 3) {
+      const double pr = a.chars[2 * ci], pi = a.chars[2 * ci + 1];
+      const double re = cre * pr - cim * pi;
+      cim = cre * pi + cim * pr;
+      cre = re;
+    }
+    if (uint32_t(idx) == row) {
+      dre += cre;
+      dim_ += cim;
+      continue;
+    }
+    int k = 0;
+    while (k < cnt && cols[k] != idx) ++k;
+    if (k == cnt) {
+      if (cnt == kMaxRowEntries) {
+        *overflow = 1;
+        return;
+      }
+      cols[cnt] = idx;
+      vre[cnt] = 0;
+      vim[cnt] = 0;
+      ++cnt;
+    }
+    vre[k] += cre;
+    vim[k] += cim;
+  }
+  if (dense) {
+    cplx* drow = dense + size_t(row) * a.n;
+    drow[row] = cplx(dre, dim_);
+    for (int k = 0; k < cnt; ++k) drow[cols[k]] = cplx(vre[k], vim[k]);
+    return;
+  }
+  if (diag_re) {
+    diag_re[row] = dre;
+    diag_im[row] = dim_;
+  }
+  if (rowsum) {
+    // entries with |v| <= eps are dropped by the reference's assembly
+    double s = 0;
+    for (int k = 0; k < cnt; ++k) {
+      double m = sqrt(vre[k] * vre[k] + vim[k] * vim[k]);
+      if (m > 2.220446049250313e-16) s += m;
+    }
+    double ddre = dre - shift.real(), ddim = dim_ - shift.imag();
+    s += sqrt(ddre * ddre + ddim * ddim);
+    rowsum[row] = s;
+  }
+}
+
+}  // namespace
+
+// Device copies of the compiled flip groups and symmetry tables.
+struct Operator::DeviceTables {
+  DeviceArray<uint32_t> meta;
+  DeviceArray<double> amp;
+  DeviceArray<double> chars;
+  int n_diag = 0, n_off = 0, n_amp = 0;
+  bool complex_coef = false;  // amplitudes or characters complex
+  HvArgs args(const SpinBasis1D& b) const {
+    HvArgs a{};
+    const SectorGeometry& g = b.geometry();
+    a.states = b.states();
+    a.orbits = b.orbits();
+    a.n = uint32_t(b.len());
+    a.L = g.L;
+    a.mask = g.mask;
+    a.parity = g.parity;
+    a.n_div = g.n_div;
+    for (int i = 0; i < 16; ++i) a.divisors[i] = g.divisors[i];
+    a.spl = b.splitters();
+    a.nspl = b.split_count();
+    a.sh = b.split_shift();
+    a.meta = meta.data();
+    a.amp = amp.data();
+    a.n_diag = n_diag;
+    a.n_off = n_off;
+    a.n_amp = n_amp;
+    a.chars = chars.data();
+    a.nvec = 1;
+    a.ld = b.len();
+    return a;
+  }
+};
+
+std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
+  if (!basis_) throw invalid_options("operator builder has no basis");
+  auto op = std::shared_ptr<Operator>(new Operator());
+  op->basis_ = basis_;
+  op->format_ = format;
+  bool real_amp = true;
+  compile_groups(basis_->spec(), terms_, checks_, basis_->len() > 0, op->diag_, op->off_, real_amp);
+  const SectorGeometry& geo = basis_->geometry();
+  op->real_ = real_amp && basis_->real_characters();
+  auto t = std::make_shared<Operator::DeviceTables>();
+  t->n_diag = int(op->diag_.size());
+  t->n_off = int(op->off_.size());
+  if (t->n_diag + t->n_off > kMaxGroups)
+    throw unsupported_backend("the operator has more distinct local actions than the H·v kernel tables hold");
+  t->complex_coef = !op->real_;
+  std::vector<uint32_t> meta;
+  std::vector<double> amp;
+  int n_amp = 0;
+  for (auto* v : {&op->diag_, &op->off_}) {
+    for (auto& g : *v) {
+      uint32_t packed = uint32_t(g.nsites);
+      for (int i = 0; i < g.nsites; ++i) packed |= uint32_t(g.sites[i]) << (3 + 5 * i);
+      uint32_t nz = 0;
+      for (size_t p = 0; p < g.amp.size(); ++p)
+        if (g.amp[p] != cplx(0, 0)) nz |= 1u << p;
+      meta.push_back(g.flip);
+      meta.push_back(packed);
+      meta.push_back(nz);
+      meta.push_back(uint32_t(n_amp));
+      for (auto& c : g.amp) {
+        amp.push_back(c.real());
+        if (t->complex_coef) amp.push_back(c.imag());
+      }
+      n_amp += int(g.amp.size());
+    }
+  }
+  if (n_amp > kMaxAmps) throw unsupported_backend("the operator's amplitude tables exceed the H·v kernel's LDS budget");
+  t->n_amp = n_amp;
+  std::vector<double> chars(64, 0.0);
+  if (geo.kmom >= 0) {
+    const double tau = 6.283185307179586;
+    for (int s = 0; s < geo.L; ++s) {
+      double angle = -tau * double(geo.kmom * s) / double(geo.L);
+      if (2 * geo.kmom == geo.L) {
+        chars[2 * s] = (s & 1) ? -1.0 : 1.0;  // exact ±1 at k = L/2
+        chars[2 * s + 1] = 0.0;
+      } else if (geo.kmom == 0) {
+        chars[2 * s] = 1.0;
+      } else {
+        chars[2 * s] = std::cos(angle);
+        chars[2 * s + 1] = std::sin(angle);
+      }
+    }
+  } else {
+    chars[0] = 1.0;
+  }
+  t->meta.resize(std::max<size_t>(meta.size(), 4));
+  t->amp.resize(std::max<size_t>(amp.size(), 2));
+  t->chars.resize(64);
+  if (!meta.empty()) QMBED_HIP(hipMemcpy(t->meta.data(), meta.data(), meta.size() * 4, hipMemcpyHostToDevice));
+  if (!amp.empty()) QMBED_HIP(hipMemcpy(t->amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice));
+  QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), 64 * 8, hipMemcpyHostToDevice));
+  op->tables_ = t;
+  return op;
+}
+
+namespace {
+
+size_t hv_smem(const HvArgs& a, int cmode) {
+  return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 130) +
+         sizeof(uint32_t) * (4 * size_t(a.n_diag + a.n_off) + size_t(a.nspl));
+}
+
+template <typename S, int CMODE, int NV>
+void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
+  if (a.n == 0) return;
+  dim3 grid((a.n + kBlock - 1) / kBlock), block(kBlock);
+  size_t sm = hv_smem(a, CMODE);
+#define QMBED_HV_CASE(SYMV, SH)                                                                        \
+  hipLaunchKernelGGL((hv_kernel<S, CMODE, SYMV, SH, NV>), grid, block, sm, s, a, x, y);
+  switch (sym) {
+    case 0: QMBED_HV_CASE(0, false); break;
+    case 1:
+      if (shift) {
+        QMBED_HV_CASE(1, true);
+      } else {
+        QMBED_HV_CASE(1, false);
+      }
+      break;
+    case 2: QMBED_HV_CASE(2, false); break;
+    default:
+      if (shift) {
+        QMBED_HV_CASE(3, true);
+      } else {
+        QMBED_HV_CASE(3, false);
+      }
+      break;
+  }
+#undef QMBED_HV_CASE
+  QMBED_HIP(hipGetLastError());
+}
+
+int sym_code(const SpinBasis1D& b) {
+  const SectorGeometry& g = b.geometry();
+  if (g.kmom >= 0 && g.parity) return 3;
+  if (g.kmom >= 0) return 1;
+  if (g.parity) return 2;
+  return 0;
+}
+
+}  // namespace
+
+void Operator::apply(const double* x, double* y, hipStream_t s) const {
+  if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+  HvArgs a = tables_->args(*basis_);
+  launch_hv<double, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
+}
+
+void Operator::apply(const float* x, float* y, hipStream_t s) const {
+  if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+  HvArgs a = tables_->args(*basis_);
+  launch_hv<float, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
+}
+
+void Operator::apply(const cplx* x, cplx* y, hipStream_t s) const {
+  HvArgs a = tables_->args(*basis_);
+  const double* xd = reinterpret_cast<const double*>(x);
+  double* yd = reinterpret_cast<double*>(y);
+  if (real_) launch_hv<double, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
+  else launch_hv<double, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
+}
+
+void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s) const {
+  HvArgs a = tables_->args(*basis_);
+  const float* xf = reinterpret_cast<const float*>(x);
+  float* yf = reinterpret_cast<float*>(y);
+  if (real_) launch_hv<float, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
+  else launch_hv<float, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
+}
+
+void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const {
+  HvArgs a = tables_->args(*basis_);
+  const double* xd = reinterpret_cast<const double*>(x);
+  double* yd = reinterpret_cast<double*>(y);
+  const int sym = sym_code(*basis_);
+  const bool sh = basis_->geometry().kmom > 0;
+  int done = 0;
+  while (done < nvec) {
+    int left = nvec - done;
+    const double* xs = xd + size_t(2) * done * dim();
+    double* ys = yd + size_t(2) * done * dim();
+    if (left >= 4) {
+      if (real_) launch_hv<double, 1, 4>(a, sym, sh, xs, ys, s);
+      else launch_hv<double, 2, 4>(a, sym, sh, xs, ys, s);
+      done += 4;
+    } else if (left >= 2) {
+      if (real_) launch_hv<double, 1, 2>(a, sym, sh, xs, ys, s);
+      else launch_hv<double, 2, 2>(a, sym, sh, xs, ys, s);
+      done += 2;
+    } else {
+      if (real_) launch_hv<double, 1, 1>(a, sym, sh, xs, ys, s);
+      else launch_hv<double, 2, 1>(a, sym, sh, xs, ys, s);
+      done += 1;
+    }
+  }
+}
+
+static void run_row_entries(const Operator& op, cplx* dense, double* rowsum, double* dre, double* dim_, cplx shift,
+                            hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  HvArgs a = op.tables_->args(b);
+  if (a.n == 0) return;
+  DeviceArray<int> overflow(1);
+  QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
+  size_t sm = sizeof(double) * 2 * size_t(a.n_amp) + sizeof(uint32_t) * (4 * size_t(a.n_diag + a.n_off) + a.nspl);
+  dim3 grid((a.n + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(row_entries_kernel, grid, dim3(kBlock), sm, s, a, sym_code(b), b.geometry().kmom > 0 ? 1 : 0,
+                     op.tables_->complex_coef ? 1 : 0, dense, rowsum, dre, dim_, overflow.data(), shift);
+  QMBED_HIP(hipGetLastError());
+  int of = 0;
+  QMBED_HIP(hipMemcpyAsync(&of, overflow.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  if (of) throw unsupported_backend("a row has more distinct off-diagonal entries than the row-entry kernel holds");
+}
+
+void Operator::materialize_dense(cplx* dense, hipStream_t s) const {
+  size_t n = dim();
+  QMBED_HIP(hipMemsetAsync(dense, 0, n * n * sizeof(cplx), s));
+  run_row_entries(*this, dense, nullptr, nullptr, nullptr, cplx(0, 0), s);
+}
+
+std::pair<cplx, double> Operator::shifted_trace_and_one_norm(hipStream_t s) const {
+  size_t n = dim();
+  if (n == 0) return {cplx(0, 0), 0.0};
+  DeviceArray<double> dre(n), dimv(n), rowsum(n);
+  run_row_entries(*this, nullptr, nullptr, dre.data(), dimv.data(), cplx(0, 0), s);
+  std::vector<double> hre(n), him(n);
+  QMBED_HIP(hipMemcpy(hre.data(), dre.data(), n * 8, hipMemcpyDeviceToHost));
+  QMBED_HIP(hipMemcpy(him.data(), dimv.data(), n * 8, hipMemcpyDeviceToHost));
+  cplx trace(0, 0);
+  for (size_t i = 0; i < n; ++i) {
+    // diagonal entries with |v| <= eps are dropped by the reference's assembly
+    cplx d(hre[i], him[i]);
+    if (std::abs(d) > 2.220446049250313e-16) trace += d;
+  }
+  cplx shift = trace / double(n);
+  run_row_entries(*this, nullptr, rowsum.data(), nullptr, nullptr, shift, s);
+  std::vector<double> rs(n);
+  QMBED_HIP(hipMemcpy(rs.data(), rowsum.data(), n * 8, hipMemcpyDeviceToHost));
+  double m = 0;
+  for (double v : rs) m = std::max(m, v);
+  return {shift, m};
+}
+
+}  // namespace qmbed

@@ -1,0 +1,407 @@
+// Core host-side types of the MI355X spin-chain engine.
+//
+// Mirrors the reference's SpinBasis1D / OperatorSpec / OperatorBuilder /
+// EigshOptions / ExpmActionPlan surface (QMBED src/basis/mod.rs:1018-1590,
+// src/operator/mod.rs:18-319 and 1941-2216, src/solve/mod.rs:14-526) with
+// device-resident state: every basis, operator and vector lives in HBM and is
+// acted on by the gfx950 kernels in the *.hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace qmbed {
+
+using cplx = std::complex<double>;
+
+// ------------------------------------------------------------------ errors --
+// Display strings follow QmbedError (QMBED src/error.rs:39-73); the C ABI
+// returns them verbatim.
+enum class ErrorKind {
+  InvalidOperator,
+  InvalidCoupling,
+  InvalidSite,
+  StateNotInBasis,
+  InvalidSector,
+  IncompatibleSymmetry,
+  NonHermitian,
+  InvalidOptions,
+  DimensionMismatch,
+  UnsupportedBackend,
+  InternalState,
+  NonConvergence,
+};
+
+struct Error : std::exception {
+  ErrorKind kind;
+  std::string message;
+  long iterations = 0;
+  double residual = 0.0;
+  Error(ErrorKind k, std::string m) : kind(k), message(std::move(m)) {}
+  const char* what() const noexcept override { return message.c_str(); }
+};
+
+Error invalid_operator(const std::string& s);
+Error invalid_coupling(const std::string& s);
+Error invalid_site(long site, long sites);
+Error invalid_sector(const std::string& s);
+Error incompatible_symmetry(const std::string& s);
+Error non_hermitian();
+Error invalid_options(const std::string& s);
+Error dimension_mismatch(const std::string& s);
+Error unsupported_backend(const std::string& s);
+Error internal_state(const std::string& s);
+Error non_convergence(long iterations, double residual);
+
+void hip_check(hipError_t e, const char* what);
+#define QMBED_HIP(call) ::qmbed::hip_check((call), #call)
+
+// ------------------------------------------------------ device buffers -----
+template <typename T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  ~DeviceArray() { release(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : p_(o.p_), n_(o.n_) {
+    o.p_ = nullptr;
+    o.n_ = 0;
+  }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  void resize(size_t n) {
+    if (n == n_) return;
+    release();
+    if (n) QMBED_HIP(hipMalloc(&p_, n * sizeof(T)));
+    n_ = n;
+  }
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  T* data() const { return p_; }
+  size_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ------------------------------------------------------------------ basis --
+enum class Normalization { AngularMomentum = 0, Pauli = 1, PauliCartesian = 2 };
+
+struct SpinBasisSpec {
+  int sites = 0;
+  int spin_twice = 1;
+  std::optional<int> up;
+  std::vector<int> up_sectors;  // union of up sectors (sorted, unique); empty = none
+  std::optional<int> momentum;
+  std::optional<int> parity;
+  Normalization normalization = Normalization::AngularMomentum;
+};
+
+// Symmetry-sector data consumed by the kernels (all integers; characters are
+// tabulated on the host).
+struct SectorGeometry {
+  int L = 0;
+  uint32_t mask = 0;
+  int kmom = -1;     // canonical momentum, -1 when no translation symmetry
+  int parity = 0;    // 0 none, +1/-1
+  int n_div = 0;     // proper divisors of L in ascending order (period search)
+  int divisors[16] = {0};
+};
+
+// SpinBasis1D: QMBED src/basis/mod.rs:1018-1035, 1312-1412.
+// Representatives are the minimum integers of their orbits, sorted ascending
+// (src/basis/mod.rs:785-847); they and the per-row orbit sizes live in HBM.
+class SpinBasis1D {
+ public:
+  static std::shared_ptr<SpinBasis1D> build(const SpinBasisSpec& spec, hipStream_t stream = nullptr);
+
+  size_t len() const { return n_; }
+  const SpinBasisSpec& spec() const { return spec_; }
+  const SectorGeometry& geometry() const { return geo_; }
+  bool symmetric() const { return geo_.kmom >= 0 || geo_.parity != 0; }
+  // character phases are real when the momentum is 0 or L/2
+  bool real_characters() const { return geo_.kmom < 0 || geo_.kmom == 0 || 2 * geo_.kmom == geo_.L; }
+  const uint32_t* states() const { return states_.data(); }
+  const uint8_t* orbits() const { return orbits_.data(); }
+  std::vector<uint32_t> states_host() const;
+  std::vector<uint8_t> orbits_host() const;
+
+  // Top of the sorted-state search tree kept in LDS by the H·v kernels:
+  // splitters[i] = states[i << split_shift], padded to a power of two.
+  const uint32_t* splitters() const { return splitters_.data(); }
+  int split_count() const { return split_count_; }   // padded (power of two)
+  int split_shift() const { return split_shift_; }
+
+ private:
+  SpinBasisSpec spec_;
+  SectorGeometry geo_;
+  size_t n_ = 0;
+  DeviceArray<uint32_t> states_;
+  DeviceArray<uint8_t> orbits_;
+  DeviceArray<uint32_t> splitters_;
+  int split_count_ = 0;
+  int split_shift_ = 0;
+};
+
+// --------------------------------------------------------------- operator --
+struct Coupling {
+  cplx coefficient;
+  std::vector<int> sites;
+};
+
+// OperatorSpec: QMBED src/operator/mod.rs:204-260 (symbols are the compact
+// one-character spelling 'I' 'n' 'z' '+' '-' 'x' 'y').
+struct OperatorSpec {
+  std::string symbols;
+  std::vector<Coupling> couplings;
+};
+
+struct AssemblyChecks {
+  bool hermiticity = true;
+  bool particle_conservation = true;
+  bool symmetry_compatibility = true;
+};
+
+enum class MatrixFormat { Dense, Csc, Csr, Dia, MatrixFree };
+
+// One flip group: every coupling whose spin-1/2 action flips the same set of
+// bits. The gather-form amplitude for row state r is amp[pattern(r)], where
+// pattern packs the bits of r at `sites` (bit i <- site sites[i]).
+struct FlipGroup {
+  uint32_t flip = 0;
+  int nsites = 0;
+  int sites[5] = {0, 0, 0, 0, 0};
+  std::vector<cplx> amp;  // 1 << nsites entries
+};
+
+// Matrix-free Hermitian operator on one spin sector, applied on the fly by
+// the gfx950 H·v kernels (no stored matrix).
+class Operator {
+ public:
+  size_t dim() const { return basis_->len(); }
+  bool is_real() const { return real_; }
+  const SpinBasis1D& basis() const { return *basis_; }
+  std::shared_ptr<const SpinBasis1D> basis_ptr() const { return basis_; }
+  const std::vector<FlipGroup>& diagonal_groups() const { return diag_; }
+  const std::vector<FlipGroup>& offdiagonal_groups() const { return off_; }
+  MatrixFormat format() const { return format_; }
+
+  // y = H x on `stream`. Pointers are device pointers. Complex vectors are
+  // interleaved (re, im) pairs. FP32 variants compute in single precision.
+  void apply(const double* x, double* y, hipStream_t s) const;          // real vectors (requires is_real)
+  void apply(const cplx* x, cplx* y, hipStream_t s) const;              // complex vectors
+  void apply(const float* x, float* y, hipStream_t s) const;            // real FP32
+  void apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s) const;
+  // nvec columns (stride = dim), complex; one lookup shared across columns
+  void apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const;
+
+  // per-row Σ|H_ij| over j != i merged per column, plus |H_ii - shift|;
+  // returns (trace, max row sum) — ‖A-μI‖₁ for Hermitian A
+  std::pair<cplx, double> shifted_trace_and_one_norm(hipStream_t s) const;
+
+  // dense materialization (row-major, complex) on the device
+  void materialize_dense(cplx* dense, hipStream_t s) const;
+
+  friend class OperatorBuilder;
+  struct DeviceTables;  // defined in hv_kernels.hip
+  std::shared_ptr<DeviceTables> tables_;
+
+ private:
+  std::shared_ptr<const SpinBasis1D> basis_;
+  std::vector<FlipGroup> diag_, off_;
+  bool real_ = true;
+  MatrixFormat format_ = MatrixFormat::MatrixFree;
+};
+
+// OperatorBuilder: QMBED src/operator/mod.rs:1941-2216.
+class OperatorBuilder {
+ public:
+  static OperatorBuilder on(std::shared_ptr<const SpinBasis1D> basis) {
+    OperatorBuilder b;
+    b.basis_ = std::move(basis);
+    return b;
+  }
+  OperatorBuilder& terms(const std::vector<OperatorSpec>& t) {
+    terms_.insert(terms_.end(), t.begin(), t.end());
+    return *this;
+  }
+  OperatorBuilder& term(const OperatorSpec& t) {
+    terms_.push_back(t);
+    return *this;
+  }
+  OperatorBuilder& checks(const AssemblyChecks& c) {
+    checks_ = c;
+    return *this;
+  }
+  std::shared_ptr<Operator> build(MatrixFormat format = MatrixFormat::MatrixFree) const;
+
+ private:
+  std::shared_ptr<const SpinBasis1D> basis_;
+  std::vector<OperatorSpec> terms_;
+  AssemblyChecks checks_;
+};
+
+// Validates symbols (QMBED src/operator/mod.rs:226-247 and the spin local
+// action alphabet, src/basis/mod.rs:1160-1228).
+void validate_terms(const std::vector<OperatorSpec>& terms, int sites);
+
+// ------------------------------------------------------------------ solve --
+enum class SpectrumTarget { SmallestAlgebraic, LargestAlgebraic, SmallestMagnitude, LargestMagnitude, BothEnds, Shift };
+
+// EigshOptions: QMBED src/solve/mod.rs:349-449 (same defaults).
+struct EigshOptions {
+  int eigenpairs = 1;
+  SpectrumTarget target = SpectrumTarget::SmallestAlgebraic;
+  double shift = 0.0;
+  std::optional<long> krylov_dimension;
+  double tolerance = 1.0e-10;
+  long max_iterations = 1000;
+  uint64_t seed = 0;
+
+  static EigshOptions smallest_algebraic(int k) {
+    EigshOptions o;
+    o.eigenpairs = k;
+    return o;
+  }
+  EigshOptions& with_tolerance(double t) {
+    tolerance = t;
+    return *this;
+  }
+  EigshOptions& with_krylov_dimension(long d) {
+    krylov_dimension = d;
+    return *this;
+  }
+  EigshOptions& with_max_iterations(long m) {
+    max_iterations = m;
+    return *this;
+  }
+  EigshOptions& with_seed(uint64_t s) {
+    seed = s;
+    return *this;
+  }
+  void validate(size_t dimension) const;
+};
+
+// Precision of the matrix-vector product inside the Krylov loop. The Lanczos
+// coefficients, reorthogonalization, tridiagonal solve and the final
+// Rayleigh-quotient/residual polish always stay FP64.
+enum class MatvecPrecision { FP64 = 0, FP32 = 1 };
+
+struct EigshControls {
+  MatvecPrecision precision = MatvecPrecision::FP64;
+  // check Ritz residual estimates every `check_every` steps inside a window
+  int check_every = 8;
+  // when true, stop inside the window as soon as the requested pairs have
+  // converged (the reference always runs the full window)
+  bool early_stop = true;
+};
+
+struct Eigensystem {
+  std::vector<double> eigenvalues;
+  std::vector<std::vector<cplx>> eigenvectors;  // host copies (optional)
+  std::vector<double> residuals;
+  long iterations = 0;
+  long reorthogonalization_passes = 0;
+  long conditional_second_passes = 0;
+  bool converged = true;
+  long fp32_iterations = 0;  // matvecs spent in the FP32 stage
+  long fp64_iterations = 0;  // matvecs spent in FP64 (incl. polish)
+};
+
+// eigsh: QMBED src/solve/mod.rs:2335-2452 (dispatch), 1116-1456 (Lanczos).
+// `initial` is a host vector of length dim (nullptr: deterministic start).
+Eigensystem eigsh(const Operator& op, const EigshOptions& opts, const cplx* initial = nullptr,
+                  const EigshControls& ctl = EigshControls(), bool want_vectors = false, hipStream_t s = nullptr);
+
+// Device-side variant: initial vector already on the device (real, FP64);
+// returned eigenvectors (if requested) are written to `vectors_out` (device,
+// real FP64, eigenpairs × dim) for real operators.
+Eigensystem eigsh_device(const Operator& op, const EigshOptions& opts, const double* d_initial,
+                         const EigshControls& ctl, double* d_vectors_out, hipStream_t s);
+
+// Krylov time evolution: QMBED src/solve/mod.rs:2454-3454.
+struct EvolutionOptions {
+  std::vector<double> times;
+  long krylov_dimension = 30;
+  double tolerance = 1.0e-10;
+  long max_substeps = 10000;
+  bool hamiltonian = true;
+  void validate() const;
+};
+
+struct EvolutionDiagnostics {
+  long lanczos_projections = 0;
+  long matrix_vector_products = 0;
+  long real_lanczos_projections = 0;
+  long real_matrix_vector_products = 0;
+  long accepted_substeps = 0;
+  long rejected_trial_intervals = 0;
+  double maximum_estimated_error = 0.0;
+};
+
+// Evolve `ncol` host columns (column-major, dim each); output is
+// [time][column][dim] on the host.
+std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, const std::vector<std::vector<cplx>>& columns,
+                                                         const EvolutionOptions& opts, EvolutionDiagnostics* diag,
+                                                         hipStream_t s);
+
+// Truncated-Taylor exponential-action plan: QMBED src/solve/mod.rs:116-248.
+class ExpmActionPlan {
+ public:
+  ExpmActionPlan(std::shared_ptr<const Operator> op, cplx coefficient, int max_degree, double tolerance,
+                 long max_substeps, hipStream_t s);
+  size_t dimension() const { return op_->dim(); }
+  int degree() const { return degree_; }
+  long scaling() const { return scaling_; }
+  cplx shift() const { return shift_; }
+  double one_norm() const { return one_norm_; }
+  cplx coefficient() const { return coefficient_; }
+  // in-place on device columns (nvec × dim, complex)
+  void apply_device(cplx* d_vectors, int nvec, hipStream_t s) const;
+  std::vector<std::vector<cplx>> apply(const std::vector<std::vector<cplx>>& vectors, hipStream_t s) const;
+
+ private:
+  std::shared_ptr<const Operator> op_;
+  cplx coefficient_, shift_;
+  double one_norm_ = 0;
+  int degree_ = 0;
+  long scaling_ = 1;
+  double tolerance_ = 0;
+};
+
+// Host symmetric tridiagonal eigensolver (implicit QL with Wilkinson shifts);
+// eigenvalues ascending; `vecs` (row-major m×m, columns = eigenvectors) only
+// when want_vectors.
+void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<double>& beta, std::vector<double>& values,
+                       std::vector<double>* vecs);
+
+// Index selection per spectral target (QMBED src/solve/mod.rs:1002-1036).
+std::vector<int> select_indices(const std::vector<double>& values, SpectrumTarget target, int count, double shift = 0.0);
+
+// Deterministic start (QMBED src/solve/mod.rs:729-742), written on the device.
+void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s);
+
+int device_count();
+
+}  // namespace qmbed

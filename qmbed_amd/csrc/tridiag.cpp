@@ -1,0 +1,167 @@
+// Host solvers for the small projected problems of the Krylov loops:
+// symmetric tridiagonal eigenvalues by the implicit QL method with Wilkinson
+// shifts (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11 (1968) 293,
+// the "tql1/tql2" procedures), eigenvectors by accumulating the rotations, and
+// single eigenvectors by inverse iteration on the shifted tridiagonal.
+// Sizes are the Krylov window (m <= a few hundred), so this is O(m^2)-O(m^3)
+// host work beside the O(m·N) device work.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "qmbed.hpp"
+
+namespace qmbed {
+
+namespace {
+
+// QL with implicit shifts. d: diagonal (in/out -> eigenvalues, unsorted),
+// e: subdiagonal e[0..n-2] (destroyed). z (optional, n×n row-major) receives
+// eigenvectors as columns when non-null (initialised to identity by caller).
+void tql(int n, std::vector<double>& d, std::vector<double> e, double* z) {
+  if (n <= 1) return;
+  e.resize(n);
+  e[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 60) break;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[i], b = c * e[i];
+          r = std::hypot(f, g);
+          e[i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - b;
+          if (z) {
+            for (int k = 0; k < n; ++k) {
+              double zk1 = z[size_t(k) * n + i + 1];
+              double zk = z[size_t(k) * n + i];
+              z[size_t(k) * n + i + 1] = s * zk + c * zk1;
+              z[size_t(k) * n + i] = c * zk - s * zk1;
+            }
+          }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[l] = g;
+        e[m] = 0.0;
+      }
+    } while (m != l);
+  }
+}
+
+}  // namespace
+
+void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<double>& beta, std::vector<double>& values,
+                       std::vector<double>* vecs) {
+  const int n = int(alpha.size());
+  std::vector<double> d = alpha;
+  std::vector<double> e(beta.begin(), beta.begin() + std::max(0, n - 1));
+  std::vector<double> z;
+  if (vecs) {
+    z.assign(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) z[size_t(i) * n + i] = 1.0;
+  }
+  tql(n, d, e, vecs ? z.data() : nullptr);
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return d[a] < d[b]; });
+  values.resize(n);
+  for (int j = 0; j < n; ++j) values[j] = d[order[j]];
+  if (vecs) {
+    vecs->assign(size_t(n) * n, 0.0);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) (*vecs)[size_t(i) * n + j] = z[size_t(i) * n + order[j]];
+  }
+}
+
+// Eigenvector of the symmetric tridiagonal (alpha, beta) for an eigenvalue
+// lam, by inverse iteration with partial pivoting (LAPACK dstein-style).
+std::vector<double> tridiagonal_eigvec(const std::vector<double>& alpha, const std::vector<double>& beta, double lam) {
+  const int n = int(alpha.size());
+  std::vector<double> x(n, 1.0 / std::sqrt(double(std::max(n, 1))));
+  if (n == 1) return {1.0};
+  double scale = 0;
+  for (int i = 0; i < n; ++i) scale = std::max(scale, std::fabs(alpha[i]));
+  for (int i = 0; i + 1 < n; ++i) scale = std::max(scale, std::fabs(beta[i]));
+  const double pert = std::max(scale, 1.0) * 1e-14;
+  // LU of (T - lam I) with partial pivoting: rows i, i+1 can swap; store
+  // U with up to two superdiagonals.
+  std::vector<double> dl(n), dd(n), du(n), du2(n);
+  std::vector<int> piv(n);
+  for (int i = 0; i < n; ++i) dd[i] = alpha[i] - lam;
+  for (int i = 0; i + 1 < n; ++i) {
+    dl[i] = beta[i];
+    du[i] = beta[i];
+  }
+  for (int i = 0; i < n; ++i) du2[i] = 0.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+      piv[i] = 0;
+      if (dd[i] == 0.0) dd[i] = pert;
+      double f = dl[i] / dd[i];
+      dl[i] = f;
+      dd[i + 1] -= f * du[i];
+      du2[i] = 0.0;
+    } else {
+      piv[i] = 1;
+      double f = dd[i] / dl[i];
+      dd[i] = dl[i];
+      dl[i] = f;
+      double t = du[i];
+      du[i] = dd[i + 1];
+      dd[i + 1] = t - f * dd[i + 1];
+      if (i + 2 < n) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du[i + 1];
+      }
+    }
+  }
+  if (dd[n - 1] == 0.0) dd[n - 1] = pert;
+  for (int i = 0; i < n; ++i)
+    if (std::fabs(dd[i]) < pert) dd[i] = dd[i] < 0 ? -pert : pert;
+  for (int it = 0; it < 3; ++it) {
+    // forward (L)
+    for (int i = 0; i + 1 < n; ++i) {
+      if (piv[i] == 0) {
+        x[i + 1] -= dl[i] * x[i];
+      } else {
+        double t = x[i];
+        x[i] = x[i + 1];
+        x[i + 1] = t - dl[i] * x[i + 1];
+      }
+    }
+    // backward (U)
+    x[n - 1] /= dd[n - 1];
+    if (n >= 2) x[n - 2] = (x[n - 2] - du[n - 2] * x[n - 1]) / dd[n - 2];
+    for (int i = n - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / dd[i];
+    double nrm = 0;
+    for (double v : x) nrm += v * v;
+    nrm = std::sqrt(nrm);
+    for (double& v : x) v /= nrm;
+  }
+  return x;
+}
+
+}  // namespace qmbed
