@@ -66,6 +66,9 @@ long qmbed_hip_model_dimension(const qmbed_hip_model* model);
 int qmbed_hip_model_is_real(const qmbed_hip_model* model);
 /* host copies of the sorted representatives and their orbit sizes */
 int qmbed_hip_model_states(const qmbed_hip_model* model, uint32_t* states, uint8_t* orbits);
+/* entries the reference's assembly would store for this operator (merged
+ * per (row, column), |v| > machine epsilon, diagonal included); -1 on error */
+long qmbed_hip_model_nnz(const qmbed_hip_model* model);
 /* the model's HIP stream (hipStream_t) */
 void* qmbed_hip_model_stream(const qmbed_hip_model* model);
 /* y = H x on device pointers (dtype QMBED_*); stream NULL = model stream */

@@ -58,6 +58,8 @@ def library():
     L.qmbed_hip_model_dimension.argtypes = [c_void_p]
     L.qmbed_hip_model_is_real.argtypes = [c_void_p]
     L.qmbed_hip_model_states.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.qmbed_hip_model_nnz.restype = c_long
+    L.qmbed_hip_model_nnz.argtypes = [c_void_p]
     L.qmbed_hip_model_stream.restype = c_void_p
     L.qmbed_hip_model_stream.argtypes = [c_void_p]
     L.qmbed_hip_apply_device.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
@@ -203,6 +205,29 @@ class Model:
         if library().qmbed_hip_model_states(self._h, s.ctypes.data, o.ctypes.data) != 0:
             raise QmbedError("state copy failed")
         return s, o
+
+    def nnz(self):
+        v = int(library().qmbed_hip_model_nnz(self._h))
+        if v < 0:
+            raise QmbedError("entry count failed")
+        return v
+
+    def time_apply(self, dtype, iters=20, seed=0):
+        """Average ms per H·v on device-resident random vectors (HIP events)."""
+        dt = np.dtype(dtype)
+        rng = np.random.default_rng(seed)
+        x = rng.standard_normal(self.dim * (2 if dt.kind == "c" else 1)).astype(
+            np.float32 if dt in (np.dtype(np.float32), np.dtype(np.complex64)) else np.float64)
+        bx, by = DeviceBuffer(x.nbytes), DeviceBuffer(x.nbytes)
+        bx.upload(x)
+        L = library()
+        ms = L.qmbed_hip_time_apply(self._h, _DTYPES[dt], bx.ptr, by.ptr, 2)  # warm
+        ms = L.qmbed_hip_time_apply(self._h, _DTYPES[dt], bx.ptr, by.ptr, iters)
+        bx.free()
+        by.free()
+        if ms < 0:
+            raise QmbedError("timed apply failed")
+        return ms
 
     def apply(self, x):
         x = np.ascontiguousarray(x)

@@ -643,6 +643,15 @@ int qmbed_hip_model_states(const qmbed_hip_model* m, uint32_t* states, uint8_t* 
   }
 }
 
+long qmbed_hip_model_nnz(const qmbed_hip_model* m) {
+  try {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    return long(tm->m->op->stored_entries(tm->m->stream));
+  } catch (...) {
+    return -1;
+  }
+}
+
 void* qmbed_hip_model_stream(const qmbed_hip_model* m) { return reinterpret_cast<const TypedModel*>(m)->m->stream; }
 
 int qmbed_hip_apply_device(const qmbed_hip_model* m, int dtype, const void* x, void* y, void* stream) {

@@ -285,7 +285,8 @@ constexpr int kMaxRowEntries = 128;
 
 __global__ void __launch_bounds__(kBlock) row_entries_kernel(HvArgs a, int sym, int shift_needed, int complex_coef,
                                                              cplx* dense, double* rowsum, double* diag_re,
-                                                             double* diag_im, int* overflow, cplx shift) {
+                                                             double* diag_im, int* overflow, cplx shift,
+                                                             uint32_t* rowcount) {
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;
   uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_amp + 2 * a.n_amp);
@@ -356,6 +357,14 @@ __global__ void __launch_bounds__(kBlock) row_entries_kernel(HvArgs a, int sym, 
     }
     vre[k] += cre;
     vim[k] += cim;
+  }
+  if (rowcount) {
+    // stored entries of the reference matrix: merged values with |v| > eps
+    uint32_t c = (sqrt(dre * dre + dim_ * dim_) > 2.220446049250313e-16) ? 1u : 0u;
+    for (int k = 0; k < cnt; ++k)
+      if (sqrt(vre[k] * vre[k] + vim[k] * vim[k]) > 2.220446049250313e-16) ++c;
+    rowcount[row] = c;
+    return;
   }
   if (dense) {
     cplx* drow = dense + size_t(row) * a.n;
@@ -583,7 +592,7 @@ void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) cons
 }
 
 static void run_row_entries(const Operator& op, cplx* dense, double* rowsum, double* dre, double* dim_, cplx shift,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t* rowcount = nullptr) {
   const SpinBasis1D& b = op.basis();
   HvArgs a = op.tables_->args(b);
   if (a.n == 0) return;
@@ -592,12 +601,24 @@ static void run_row_entries(const Operator& op, cplx* dense, double* rowsum, dou
   size_t sm = sizeof(double) * 2 * size_t(a.n_amp) + sizeof(uint32_t) * (4 * size_t(a.n_diag + a.n_off) + a.nspl);
   dim3 grid((a.n + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(row_entries_kernel, grid, dim3(kBlock), sm, s, a, sym_code(b), b.geometry().kmom > 0 ? 1 : 0,
-                     op.tables_->complex_coef ? 1 : 0, dense, rowsum, dre, dim_, overflow.data(), shift);
+                     op.tables_->complex_coef ? 1 : 0, dense, rowsum, dre, dim_, overflow.data(), shift, rowcount);
   QMBED_HIP(hipGetLastError());
   int of = 0;
   QMBED_HIP(hipMemcpyAsync(&of, overflow.data(), sizeof(int), hipMemcpyDeviceToHost, s));
   QMBED_HIP(hipStreamSynchronize(s));
   if (of) throw unsupported_backend("a row has more distinct off-diagonal entries than the row-entry kernel holds");
+}
+
+size_t Operator::stored_entries(hipStream_t s) const {
+  size_t n = dim();
+  if (n == 0) return 0;
+  DeviceArray<uint32_t> cnt(n);
+  run_row_entries(*this, nullptr, nullptr, nullptr, nullptr, cplx(0, 0), s, cnt.data());
+  std::vector<uint32_t> h(n);
+  QMBED_HIP(hipMemcpy(h.data(), cnt.data(), n * 4, hipMemcpyDeviceToHost));
+  size_t tot = 0;
+  for (uint32_t c : h) tot += c;
+  return tot;
 }
 
 void Operator::materialize_dense(cplx* dense, hipStream_t s) const {

@@ -220,6 +220,10 @@ class Operator {
   // returns (trace, max row sum) — ‖A-μI‖₁ for Hermitian A
   std::pair<cplx, double> shifted_trace_and_one_norm(hipStream_t s) const;
 
+  // number of entries the reference's assembly would store (merged per
+  // (row, column), |v| > f64::EPSILON, diagonal included)
+  size_t stored_entries(hipStream_t s) const;
+
   // dense materialization (row-major, complex) on the device
   void materialize_dense(cplx* dense, hipStream_t s) const;
 

@@ -1,0 +1,99 @@
+"""CPU-only checks of the native library: it loads, exports every symbol the
+public header declares, and answers protocol errors (before any GPU work)
+with the reference's envelopes and texts."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "qmbed_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(qmbed_\w+)\s*\(", text, flags=re.M):
+        names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_symbols_exported(qa):
+    lib = ctypes.CDLL(qa.LIBRARY_PATH)
+    names = declared_symbols()
+    assert "qmbed_run_json" in names and "qmbed_string_free" in names and len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_reference_exports_present(qa):
+    # the five symbols the reference's Python loader resolves (bindings/python/src/qmbed/_ffi.py:154-185)
+    lib = ctypes.CDLL(qa.LIBRARY_PATH)
+    for n in ("qmbed_run_json", "qmbed_command_json", "qmbed_string_free",
+              "qmbed_evolve_model_with_drive_json", "qmbed_register_user_basis_32_json",
+              "qmbed_register_user_basis_64_json"):
+        assert hasattr(lib, n)
+
+
+def test_null_request(qa):
+    # bindings/capi/src/lib.rs:6093-6095
+    assert qa.raw_call("qmbed_run_json", None) == {"status": "error", "error": "request pointer is null"}
+    assert qa.raw_call("qmbed_command_json", None) == {"status": "error", "error": "request pointer is null"}
+
+
+def test_protocol_errors(qa):
+    r = qa.raw_call("qmbed_run_json", "{bad")
+    assert r["status"] == "error" and r["error"].startswith("invalid options: invalid binding request")
+    r = qa.raw_call("qmbed_command_json", {"operation": "release_model", "handle": "0"})
+    assert r == {"status": "error", "error": "invalid options: model handle must be positive"}
+    r = qa.raw_call("qmbed_command_json", {"operation": "release_model", "handle": "abc"})
+    assert r["error"] == 'invalid options: model handle "abc" is not a positive integer'
+    r = qa.raw_call("qmbed_command_json", {"operation": "release_model", "handle": "17"})
+    assert r["error"] == "invalid options: model handle 17 is not registered"
+    r = qa.raw_call("qmbed_command_json", {"operation": "apply_expm_action", "expm_handle": "7", "vectors": []})
+    assert r["error"] == 'invalid options: exponential-action handle "7" does not use the expm: prefix'
+    r = qa.raw_call("qmbed_command_json", {"operation": "release_expm_action", "expm_handle": "expm:0"})
+    assert r["error"] == "invalid options: exponential-action handle must be positive"
+    r = qa.raw_call("qmbed_command_json", {"operation": "release_expm_action", "expm_handle": "expm:9"})
+    assert r["error"] == "invalid options: exponential-action handle expm:9 is not registered"
+
+
+def test_out_of_path_requests_error(qa):
+    r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis",
+                                           "basis": {"kind": "boson", "sites": 3, "states_per_site": 2}})
+    assert r["status"] == "error" and "unsupported backend" in r["error"]
+    r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis",
+                                           "basis": {"kind": "spin", "sites": 4, "reverse": True}})
+    assert r["status"] == "error"
+    r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis",
+                                           "basis": {"kind": "spin", "sites": 4, "up": 2, "up_sectors": [1, 2]}})
+    assert r["error"] == "invalid options: spin basis accepts either up or up_sectors, not both"
+
+
+def test_bad_terms_rejected_before_device_work(qa):
+    bad_arity = {"basis": {"kind": "spin", "sites": 4},
+                 "terms": [{"product": {"local": ["z", "z"]}, "couplings": [{"coefficient": [1, 0], "sites": [0]}]}],
+                 "solver": {"eigenpairs": 1, "target": {"kind": "smallest_algebraic"}}}
+    r = qa.raw_call("qmbed_run_json", bad_arity)
+    assert r["status"] == "error" and r["error"].startswith("invalid coupling: operator \"zz\" has arity 2")
+    bad_name = dict(bad_arity)
+    bad_name["terms"] = [{"product": {"local": ["spin"]}, "couplings": []}]
+    r = qa.raw_call("qmbed_run_json", bad_name)
+    assert r["error"] == "invalid or unregistered operator: spin"
+
+
+def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
+    import importlib
+
+    import qmbed_amd
+
+    monkeypatch.setenv("QMBED_HIP_LIBRARY", str(tmp_path / "missing.so"))
+    mod = importlib.reload(qmbed_amd)
+    try:
+        with pytest.raises(RuntimeError, match="native library not found"):
+            mod.library()
+    finally:
+        monkeypatch.delenv("QMBED_HIP_LIBRARY")
+        importlib.reload(qmbed_amd)

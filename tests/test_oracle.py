@@ -1,0 +1,349 @@
+"""Pins the CPU oracle (CPU only).
+
+Every check is independent of the oracle's own formulas: the reference's
+inline anchors, closed-form sector counts (necklace / bracelet counting),
+spectra of Hamiltonians built here from Kronecker products, explicit symmetry
+projectors, scipy's dense exponential, and a brute-force search of the
+Taylor degree/scaling table.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from tests import cases
+
+# ----------------------------------------------------------- helpers -------
+
+SX = np.array([[0, 1], [1, 0]], dtype=complex) / 2
+SY = np.array([[0, 1j], [-1j, 0]], dtype=complex) / 2  # (down, up) ordering: Sy|down> = -i/2 |up>
+SZ = np.array([[-1, 0], [0, 1]], dtype=complex) / 2  # digit 0 = down (m=-1/2)
+SP = np.array([[0, 0], [1, 0]], dtype=complex)       # raises digit 0 -> 1
+SM = SP.T.copy()
+ID = np.eye(2, dtype=complex)
+
+
+def local_matrix(sym, norm="angular"):
+    m = {"x": SX, "y": SY, "z": SZ, "+": SP, "-": SM, "I": ID}[sym]
+    if norm == "pauli_cartesian" and sym in "xyz":
+        m = 2 * m
+    if norm == "pauli" and sym != "I":
+        m = 2 * m
+    return m
+
+
+def full_hamiltonian(L, spec, norm="angular"):
+    """Dense H on all 2^L states, basis index = integer state (bit i = site i)."""
+    dim = 2 ** L
+    H = np.zeros((dim, dim), dtype=complex)
+    for sym, couplings in spec:
+        for coeff, sites in couplings:
+            op = np.eye(dim, dtype=complex)
+            # product O_0 O_1 ... O_{n-1}; O_{n-1} acts first
+            for s, site in zip(sym, sites):
+                mats = [ID] * L
+                mats[site] = local_matrix(s, norm)
+                full = np.array([[1.0 + 0j]])
+                for q in range(L - 1, -1, -1):  # site L-1 is the most significant bit
+                    full = np.kron(full, mats[q])
+                op = op @ full
+            H += coeff * op
+    return H
+
+
+def rotate(s, k, L):
+    k %= L
+    mask = (1 << L) - 1
+    return ((s << k) & mask) | (s >> (L - k)) if k else s
+
+
+def reflect(s, L):
+    return int(format(s, f"0{L}b")[::-1], 2)
+
+
+def mobius(n):
+    res, p, m = 1, 2, n
+    while p * p <= m:
+        if m % p == 0:
+            m //= p
+            if m % p == 0:
+                return 0
+            res = -res
+        p += 1
+    return -res if m > 1 else res
+
+
+def primitive_necklaces(d, w):
+    """Aperiodic binary necklaces of length d and weight w."""
+    tot = 0
+    for e in range(1, d + 1):
+        if d % e == 0 and w % e == 0:
+            tot += mobius(e) * math.comb(d // e, w // e)
+    return tot // d
+
+
+def momentum_sector_dim(L, w, k):
+    """Orbits of period d with (w·d) ≡ 0 (mod L) and k·d ≡ 0 (mod L)."""
+    tot = 0
+    for d in range(1, L + 1):
+        if L % d or (w * d) % L or (k * d) % L:
+            continue
+        tot += primitive_necklaces(d, w * d // L)
+    return tot
+
+
+def bracelets(n):
+    neck = sum(_phi(d) * 2 ** (n // d) for d in range(1, n + 1) if n % d == 0) // n
+    if n % 2 == 0:
+        return (neck + 3 * 2 ** (n // 2 - 1)) // 2
+    return (neck + 2 ** ((n + 1) // 2)) // 2
+
+
+def _phi(n):
+    return sum(1 for i in range(1, n + 1) if math.gcd(i, n) == 1)
+
+
+# ------------------------------------------------- reference anchors -------
+
+def test_reference_sector_anchors(oracle):
+    # tests/visible_contract.rs:114-121, :158-163
+    assert oracle.Model(4, [], up=2).dim == 6
+    assert oracle.Model(4, [], up=2, momentum=0).dim == 2
+    assert oracle.Model(4, [], up=2, momentum=1).dim == 1
+    # tests/full_api_foundation.rs:160-167
+    assert oracle.Model(4, [], up=2, parity=1).dim == 4
+    assert oracle.Model(4, [], up=2, parity=-1).dim == 2
+    # tests/full_api_foundation.rs:191-197
+    with pytest.raises(oracle.OracleError, match="incompatible symmetry request"):
+        oracle.Model(4, [], up=2, momentum=1, parity=1)
+    # tests/visible_contract.rs:192-194
+    assert oracle.Model(18, [], up=9, momentum=0).dim == 2704
+
+
+def test_reference_reduction_metadata(oracle):
+    # bindings/capi/src/lib.rs:8700-8735: L=4 up=2 k=1
+    m = oracle.Model(4, [], up=2, momentum=1)
+    rep, phase, orbit = m.image(3)
+    assert (rep, orbit) == (3, 4) and abs(phase - 1) < 1e-15
+    rep, phase, orbit = m.image(6)
+    assert rep == 3 and orbit == 4
+    assert abs(phase - (-1j)) < 1e-15  # exp(-i pi/2): state 6 = T^1 3
+    assert abs(0.5 * phase - (-0.5j)) < 1e-15  # amplitude 1/sqrt(4) * phase
+    assert m.image(5) is None and m.image(0) is None
+
+
+def test_reference_energy_anchors(oracle):
+    # dimer -0.75 (tests/visible_contract.rs:991-1007), L=4 k=0 Heisenberg -2 (:171-179)
+    dimer = [("zz", [(1.0, [0, 1])]), ("+-", [(0.5, [0, 1])]), ("-+", [(0.5, [0, 1])])]
+    ev = np.linalg.eigvalsh(oracle.Model(2, dimer).dense())
+    np.testing.assert_allclose(ev, [-0.75, 0.25, 0.25, 0.25], atol=1e-14)
+    ev = np.linalg.eigvalsh(oracle.Model(4, cases.heisenberg(4), up=2, momentum=0).dense())
+    assert abs(ev[0] + 2) < 1e-13
+
+
+# -------------------------------------------------- closed-form counts -----
+
+@pytest.mark.parametrize("L,w,k", [(16, 8, 0), (12, 6, 0), (12, 6, 6), (12, 5, 3), (10, 5, 1), (14, 7, 7), (9, 3, 0),
+                                   (18, 9, 0), (15, 6, 5)])
+def test_momentum_dims_closed_form(oracle, L, w, k):
+    assert oracle.Model(L, [], up=w, momentum=k).dim == momentum_sector_dim(L, w, k)
+
+
+def test_config_dims_closed_form():
+    # the benchmark sectors; these are too large for the oracle's hash map
+    assert momentum_sector_dim(16, 8, 0) == 810
+    assert momentum_sector_dim(24, 12, 0) == 112720
+    assert momentum_sector_dim(28, 14, 0) == 1432860
+    dims = [momentum_sector_dim(24, w, k) for w in (11, 12, 13) for k in range(24)]
+    assert (min(dims), max(dims), sum(dims)) == (104006, 112720, 7696444)
+    assert bracelets(26) == 1296858
+
+
+@pytest.mark.parametrize("L", [8, 10, 12, 13])
+def test_parity_k0_is_bracelets(oracle, L):
+    # k=0, p=+1 on the unconstrained space: every dihedral orbit survives
+    assert oracle.Model(L, [], momentum=0, parity=1).dim == bracelets(L)
+
+
+@pytest.mark.parametrize("L,up,k,p", cases.SECTORS)
+def test_basis_brute_force(oracle, L, up, k, p):
+    """Representatives = orbit minima whose projected coefficient survives."""
+    m = oracle.Model(L, [], up=up, momentum=k, parity=p)
+    states, orbits = m.states()
+    expect, exp_orb = [], []
+    for s in range(2 ** L):
+        if up is not None and bin(s).count("1") != up:
+            continue
+        group = []
+        shifts = range(L) if k is not None else [0]
+        for t in shifts:
+            r = rotate(s, t, L)
+            group.append((r, t, 1))
+            if p is not None:
+                group.append((reflect(r, L), t, p))
+        if min(g[0] for g in group) != s:
+            continue
+        coeff = {}
+        for r, t, sign in group:
+            ch = np.exp(-2j * np.pi * (k or 0) * t / L)
+            coeff[r] = coeff.get(r, 0) + sign * ch
+        alive = {r: c for r, c in coeff.items() if abs(c) > 1e-9}
+        if alive:
+            expect.append(s)
+            exp_orb.append(len(alive))
+    np.testing.assert_array_equal(states, np.array(expect, dtype=np.uint64))
+    np.testing.assert_array_equal(orbits, np.array(exp_orb))
+
+
+# --------------------------------------------------- matrix elements -------
+
+@pytest.mark.parametrize("name", ["heisenberg", "j1j2", "dm", "xyfield"])
+def test_unreduced_matrix_is_kron_hamiltonian(oracle, name):
+    L = 6
+    spec = {"heisenberg": cases.heisenberg, "j1j2": cases.j1j2, "dm": cases.dm_chain, "xyfield": cases.xy_field}[name](L)
+    m = oracle.Model(L, spec)
+    np.testing.assert_allclose(m.dense(), full_hamiltonian(L, spec), atol=1e-14)
+    if name != "xyfield":
+        sub = oracle.Model(L, spec, up=3)
+        idx = [s for s in range(2 ** L) if bin(s).count("1") == 3]
+        np.testing.assert_allclose(sub.dense(), full_hamiltonian(L, spec)[np.ix_(idx, idx)], atol=1e-14)
+
+
+def test_pauli_normalizations(oracle):
+    L = 4
+    spec = [("zz", [(0.7, [0, 1]), (0.3, [2, 3])]), ("x", [(0.4, [1])]), ("+-", [(0.2, [1, 2])]), ("-+", [(0.2, [1, 2])])]
+    for norm in ("angular", "pauli", "pauli_cartesian"):
+        m = oracle.Model(L, spec, norm=norm)
+        np.testing.assert_allclose(m.dense(), full_hamiltonian(L, spec, norm), atol=1e-14)
+
+
+@pytest.mark.parametrize("name,L,up", [("heisenberg", 8, 4), ("j1j2", 8, 4), ("dm", 8, 3), ("xyfield", 7, None)])
+def test_momentum_sectors_union_is_full_spectrum(oracle, name, L, up):
+    spec = {"heisenberg": cases.heisenberg, "j1j2": cases.j1j2, "dm": cases.dm_chain, "xyfield": cases.xy_field}[name](L)
+    full = oracle.Model(L, spec, up=up)
+    ref = np.linalg.eigvalsh(full.dense())
+    parts = []
+    for k in range(L):
+        mk = oracle.Model(L, spec, up=up, momentum=k)
+        if mk.dim:
+            d = mk.dense()
+            assert np.allclose(d, d.conj().T, atol=1e-13)
+            parts.append(np.linalg.eigvalsh(d))
+    np.testing.assert_allclose(np.sort(np.concatenate(parts)), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("L,up,k", [(8, 4, 0), (8, 4, 4), (10, 5, 0), (9, None, 0)])
+def test_parity_sectors_split_momentum_sector(oracle, L, up, k):
+    spec = cases.j1j2(L)
+    whole = np.linalg.eigvalsh(oracle.Model(L, spec, up=up, momentum=k).dense())
+    parts = [np.linalg.eigvalsh(oracle.Model(L, spec, up=up, momentum=k, parity=p).dense()) for p in (1, -1)]
+    np.testing.assert_allclose(np.sort(np.concatenate(parts)), whole, atol=1e-12)
+
+
+def test_reduced_matrix_equals_projector_matrix(oracle):
+    """H_k[i,j] = <ψ_i|H|ψ_j> with ψ_r built from explicit orbit vectors."""
+    L, up, k = 8, 4, 3
+    spec = cases.dm_chain(L)
+    m = oracle.Model(L, spec, up=up, momentum=k)
+    H = full_hamiltonian(L, spec)
+    states, orbits = m.states()
+    vecs = []
+    for r in states:
+        v = np.zeros(2 ** L, dtype=complex)
+        for s in range(L):
+            v[rotate(int(r), s, L)] += np.exp(-2j * np.pi * k * s / L)
+        v /= np.linalg.norm(v)
+        vecs.append(v)
+    B = np.array(vecs).T  # columns = symmetric states
+    # the reference's element H[s,r] = Σ_t a_t sqrt(N_r/N_s) conj(phase_t) with phase(T^s r) = χ(s) is
+    # <s~|H|r~> for the symmetric states |r~> = N^-1/2 Σ_s χ(s) |T^s r>
+    Hk = B.conj().T @ H @ B
+    np.testing.assert_allclose(m.dense(), Hk, atol=1e-13)
+
+
+# ----------------------------------------------------- solvers -------------
+
+def test_deterministic_start_closed_form(oracle):
+    # src/solve/mod.rs:729-742, evaluated with Python integers
+    for seed in (0, 1, 12345, 2 ** 63 + 7):
+        state = seed | 1
+        vals = []
+        for _ in range(50):
+            state = (state * 6364136223846793005 + 1442695040888963407) % 2 ** 64
+            vals.append((state >> 11) / 2 ** 53 - 0.5)
+        v = np.array(vals)
+        v /= np.linalg.norm(v)
+        np.testing.assert_allclose(oracle.deterministic_start(50, seed), v, rtol=0, atol=1e-16)
+
+
+def test_hermitian_eig_against_lapack(oracle):
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40))
+    a = a + a.conj().T
+    np.testing.assert_allclose(oracle.hermitian_eigvals(a), np.linalg.eigvalsh(a), atol=1e-11)
+
+
+@pytest.mark.parametrize("L,up,k,name,nev", [(16, 8, 0, "heisenberg", 4), (12, 6, 3, "dm", 3), (12, 6, 6, "j1j2", 2)])
+def test_oracle_eigsh_against_dense(oracle, L, up, k, name, nev):
+    spec = {"heisenberg": cases.heisenberg, "j1j2": cases.j1j2, "dm": cases.dm_chain}[name](L)
+    m = oracle.Model(L, spec, up=up, momentum=k)
+    r = m.eigsh(nev)
+    dense = np.linalg.eigvalsh(m.dense())
+    np.testing.assert_allclose(r["eigenvalues"], dense[:nev], rtol=1e-10, atol=1e-10)
+    assert all(res <= 1e-10 * max(abs(v), 1) for v, res in zip(r["eigenvalues"], r["residuals"]))
+    # the reference's non-restarted window always runs in full (src/solve/mod.rs:1344-1377)
+    if m.dim <= 256:  # dense route (src/solve/mod.rs:364-372)
+        assert r["iterations"] == 1
+    else:
+        assert r["iterations"] == min(max(8 * nev + 64, 256), m.dim)
+
+
+def test_oracle_evolve_against_expm(oracle):
+    L = 10
+    spec = cases.xxz(L, 1.0)
+    m = oracle.Model(L, spec, up=5, momentum=0)
+    H = m.dense()
+    rng = np.random.default_rng(1)
+    psi = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    psi /= np.linalg.norm(psi)
+    times = [0.1, 1.0, 10.0]
+    out, stats = m.evolve(psi, times, krylov_dimension=30)
+    for t, got in zip(times, out):
+        exact = scipy.linalg.expm(-1j * t * H) @ psi
+        assert np.linalg.norm(got - exact) <= 1e-9
+    assert stats["matrix_vector_products"] >= stats["lanczos_projections"]
+
+
+def test_oracle_taylor_against_expm(oracle):
+    L = 8
+    spec = cases.dm_chain(L)
+    m = oracle.Model(L, spec, up=4, momentum=2)
+    H = m.dense()
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(m.dim) + 1j * rng.standard_normal(m.dim)
+    for coef in (-0.5j, 0.3, -2.0 + 1j):
+        out, info = m.expm(coef, [v])
+        exact = scipy.linalg.expm(coef * H) @ v
+        assert np.linalg.norm(out[0] - exact) <= 1e-12 * np.linalg.norm(exact)
+        # degree/scaling = argmin m*s over the theta table (brute force here)
+        theta = [(1, 2.29e-16), (2, 2.58e-8), (3, 1.39e-5), (4, 3.40e-4), (5, 2.40e-3), (6, 9.07e-3), (7, 2.38e-2),
+                 (8, 5.00e-2), (9, 8.96e-2), (10, 1.44e-1), (11, 2.14e-1), (12, 3.00e-1), (13, 4.00e-1),
+                 (14, 5.14e-1), (15, 6.41e-1), (16, 7.81e-1), (17, 9.31e-1), (18, 1.09), (19, 1.26), (20, 1.44),
+                 (21, 1.62), (22, 1.82), (23, 2.01), (24, 2.22), (25, 2.43), (26, 2.64), (27, 2.86), (28, 3.08),
+                 (29, 3.31), (30, 3.54), (35, 4.7), (40, 6.0), (45, 7.2), (50, 8.5), (55, 9.9)]
+        mu = np.trace(H) / m.dim
+        norm1 = np.abs(H - mu * np.eye(m.dim)).sum(axis=0).max()
+        assert abs(info["one_norm"] - norm1) <= 1e-12 * norm1
+        x = abs(coef) * norm1
+        best = min(theta, key=lambda t: t[0] * max(math.ceil(x / t[1]), 1))
+        assert (info["degree"], info["scaling"]) == (best[0], max(math.ceil(x / best[1]), 1))
+
+
+def test_oracle_nilpotent_taylor(oracle):
+    # a single raising operator is nilpotent: exp(A)·|down> = |down> + |up>
+    # (nilpotent Taylor anchor, tests/solver_extensions.rs:364-444)
+    m = oracle.Model(1, [("+", [(1.0, [0])])], check_hermiticity=False, check_particles=False)
+    out, info = m.expm(1.0, [np.array([1.0, 0.0])])
+    np.testing.assert_allclose(out[0], [1.0, 1.0], atol=1e-15)
