@@ -358,9 +358,11 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
   DeviceArray<int> info(1);
   // row-major A is the column-major conjugate: eigenvectors come back
   // conjugated, eigenvalues unchanged
-  rocblas_status st = rocsolver_zheevd(h, rocblas_evect_original, rocblas_fill_upper, rocblas_int(n),
-                                       reinterpret_cast<rocblas_double_complex*>(A.data()), rocblas_int(n), D.data(),
-                                       E.data(), info.data());
+  // QR-based heev: the divide-and-conquer heevd returned non-finite
+  // eigenpairs for n = 246 on gfx950 with this ROCm release
+  rocblas_status st = rocsolver_zheev(h, rocblas_evect_original, rocblas_fill_upper, rocblas_int(n),
+                                      reinterpret_cast<rocblas_double_complex*>(A.data()), rocblas_int(n), D.data(),
+                                      E.data(), info.data());
   QMBED_HIP(hipStreamSynchronize(s));
   rocblas_destroy_handle(h);
   if (st != rocblas_status_success) throw internal_state("rocsolver_zheevd failed");

@@ -279,125 +279,250 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   }
 }
 
-// Row-wise exact entries for small dense materialization and the one-norm:
-// each lane gathers its row's (column, value) pairs, merged per column.
-constexpr int kMaxRowEntries = 128;
-
-__global__ void __launch_bounds__(kBlock) row_entries_kernel(HvArgs a, int sym, int shift_needed, int complex_coef,
-                                                             cplx* dense, double* rowsum, double* diag_re,
-                                                             double* diag_im, int* overflow, cplx shift,
-                                                             uint32_t* rowcount) {
-  extern __shared__ double smem_d[];
-  double* s_amp = smem_d;
-  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_amp + 2 * a.n_amp);
-  uint32_t* s_spl = s_meta + 4 * (a.n_diag + a.n_off);
-  const int G = a.n_diag + a.n_off;
-  for (int i = threadIdx.x; i < (complex_coef ? 2 : 1) * a.n_amp; i += kBlock) s_amp[i] = a.amp[i];
-  for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
-  for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
-  __syncthreads();
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= a.n) return;
-  const uint32_t r = a.states[row];
-  const double nr = sym ? double(a.orbits[row]) : 1.0;
-  int cols[kMaxRowEntries];
-  double vre[kMaxRowEntries], vim[kMaxRowEntries];
+// Exact column entries of the sector matrix: lane r applies every coupling
+// to its representative and merges the contributions per target row, which
+// is the reference's column-streaming assembly for column r
+// (src/operator/mod.rs:2119-2170). `amp` holds the forward amplitudes c·a as
+// (re, im) pairs; the character enters conjugated (conj(phase_t)).
+template <int CAP>
+struct ColList {
+  int rows[CAP];
+  double vre[CAP], vim[CAP];
   int cnt = 0;
-  double dre = 0, dim_ = 0;
-  for (int g = 0; g < a.n_diag; ++g) {
-    const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
-    dre += complex_coef ? s_amp[2 * off] : s_amp[off];
-    dim_ += complex_coef ? s_amp[2 * off + 1] : 0.0;
+  double dre = 0, dim = 0;
+  bool overflow = false;
+};
+
+__device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_needed, uint32_t t, const uint32_t* spl,
+                                          int& idx, int& orbit, int& ci, int& sign) {
+  uint32_t rep = t;
+  orbit = 1;
+  ci = 0;
+  sign = 1;
+  if (sym == 1) {
+    if (shift_needed) walk<1, true>(t, a, rep, orbit, ci, sign);
+    else walk<1, false>(t, a, rep, orbit, ci, sign);
+  } else if (sym == 2) {
+    walk<2, false>(t, a, rep, orbit, ci, sign);
+  } else if (sym == 3) {
+    if (shift_needed) walk<3, true>(t, a, rep, orbit, ci, sign);
+    else walk<3, false>(t, a, rep, orbit, ci, sign);
   }
+  idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.n);
+  return idx >= 0;
+}
+
+// contribution value c·a·sqrt(N_src/N_t)·conj(sign·χ(ci))
+__device__ __forceinline__ void column_value(const HvArgs& a, const double* famp, int off, int sym, double nsrc,
+                                             int orbit, int ci, int sign, double& re, double& im) {
+  double cre = famp[2 * off], cim = famp[2 * off + 1];
+  const double scale = double(sign) * (sym ? sqrt(nsrc / double(orbit)) : 1.0);
+  cre *= scale;
+  cim *= scale;
+  if (sym == 1 || sym == 3) {
+    const double pr = a.chars[2 * ci], pi = -a.chars[2 * ci + 1];
+    re = cre * pr - cim * pi;
+    im = cre * pi + cim * pr;
+  } else {
+    re = cre;
+    im = cim;
+  }
+}
+
+template <int CAP>
+__device__ void collect_column(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
+                               const uint32_t* spl, uint32_t col, ColList<CAP>& out) {
+  const uint32_t r = a.states[col];
+  const double ns = sym ? double(a.orbits[col]) : 1.0;
+  for (int g = 0; g < a.n_diag; ++g) {
+    const int off = int(meta[4 * g + 3]) + pattern_of(r, meta[4 * g + 1]);
+    out.dre += famp[2 * off];
+    out.dim += famp[2 * off + 1];
+  }
+  const int G = a.n_diag + a.n_off;
   for (int g = a.n_diag; g < G; ++g) {
-    const int pat = pattern_of(r, s_meta[4 * g + 1]);
-    if (!((s_meta[4 * g + 2] >> pat) & 1u)) continue;
-    const uint32_t t = r ^ s_meta[4 * g];
-    const int off = int(s_meta[4 * g + 3]) + pat;
-    uint32_t rep = t;
-    int orbit = 1, ci = 0, sign = 1;
-    if (sym == 1) {
-      if (shift_needed) walk<1, true>(t, a, rep, orbit, ci, sign);
-      else walk<1, false>(t, a, rep, orbit, ci, sign);
-    } else if (sym == 2) {
-      walk<2, false>(t, a, rep, orbit, ci, sign);
-    } else if (sym == 3) {
-      if (shift_needed) walk<3, true>(t, a, rep, orbit, ci, sign);
-      else walk<3, false>(t, a, rep, orbit, ci, sign);
-    }
-    const int idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.n);
-    if (idx < 0) continue;
-    double cre = complex_coef ? s_amp[2 * off] : s_amp[off];
-    double cim = complex_coef ? s_amp[2 * off + 1] : 0.0;
-    double scale = double(sign) * (sym ? sqrt(nr / double(orbit)) : 1.0);
-    cre *= scale;
-    cim *= scale;
-    if (sym == 1 || sym == 3) {
-      const double pr = a.chars[2 * ci], pi = a.chars[2 * ci + 1];
-      const double re = cre * pr - cim * pi;
-      cim = cre * pi + cim * pr;
-      cre = re;
-    }
-    if (uint32_t(idx) == row) {
-      dre += cre;
-      dim_ += cim;
+    const int pat = pattern_of(r, meta[4 * g + 1]);
+    if (!((meta[4 * g + 2] >> pat) & 1u)) continue;
+    int idx, orbit, ci, sign;
+    if (!target_of(a, sym, shift_needed, r ^ meta[4 * g], spl, idx, orbit, ci, sign)) continue;
+    double re, im;
+    column_value(a, famp, int(meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    if (uint32_t(idx) == col) {
+      out.dre += re;
+      out.dim += im;
       continue;
     }
     int k = 0;
-    while (k < cnt && cols[k] != idx) ++k;
-    if (k == cnt) {
-      if (cnt == kMaxRowEntries) {
-        *overflow = 1;
+    while (k < out.cnt && out.rows[k] != idx) ++k;
+    if (k == out.cnt) {
+      if (out.cnt == CAP) {
+        out.overflow = true;
         return;
       }
-      cols[cnt] = idx;
-      vre[cnt] = 0;
-      vim[cnt] = 0;
-      ++cnt;
+      out.rows[out.cnt] = idx;
+      out.vre[out.cnt] = 0;
+      out.vim[out.cnt] = 0;
+      ++out.cnt;
     }
-    vre[k] += cre;
-    vim[k] += cim;
+    out.vre[k] += re;
+    out.vim[k] += im;
   }
-  if (rowcount) {
-    // stored entries of the reference matrix: merged values with |v| > eps
-    uint32_t c = (sqrt(dre * dre + dim_ * dim_) > 2.220446049250313e-16) ? 1u : 0u;
-    for (int k = 0; k < cnt; ++k)
-      if (sqrt(vre[k] * vre[k] + vim[k] * vim[k]) > 2.220446049250313e-16) ++c;
-    rowcount[row] = c;
+}
+
+// merged entry H[row, col] (all contributions of column col landing on row)
+__device__ void entry_at(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
+                         const uint32_t* spl, uint32_t col, uint32_t row, double& vre, double& vim) {
+  const uint32_t r = a.states[col];
+  const double ns = sym ? double(a.orbits[col]) : 1.0;
+  vre = 0;
+  vim = 0;
+  if (row == col) {
+    for (int g = 0; g < a.n_diag; ++g) {
+      const int off = int(meta[4 * g + 3]) + pattern_of(r, meta[4 * g + 1]);
+      vre += famp[2 * off];
+      vim += famp[2 * off + 1];
+    }
+  }
+  const int G = a.n_diag + a.n_off;
+  for (int g = a.n_diag; g < G; ++g) {
+    const int pat = pattern_of(r, meta[4 * g + 1]);
+    if (!((meta[4 * g + 2] >> pat) & 1u)) continue;
+    int idx, orbit, ci, sign;
+    if (!target_of(a, sym, shift_needed, r ^ meta[4 * g], spl, idx, orbit, ci, sign)) continue;
+    if (uint32_t(idx) != row) continue;
+    double re, im;
+    column_value(a, famp, int(meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    vre += re;
+    vim += im;
+  }
+}
+
+constexpr int kMaxColEntries = 128;
+constexpr double kEps = 2.220446049250313e-16;
+
+// dense column write / entry counts / diagonal / absolute column sums
+__global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
+                                                                cplx* dense, double* colsum, double* diag_re,
+                                                                double* diag_im, int* overflow, cplx shift,
+                                                                uint32_t* count) {
+  const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= a.n) return;
+  ColList<kMaxColEntries> C;
+  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  if (C.overflow) {
+    *overflow = 1;
     return;
   }
-  if (dense) {
-    cplx* drow = dense + size_t(row) * a.n;
-    drow[row] = cplx(dre, dim_);
-    for (int k = 0; k < cnt; ++k) drow[cols[k]] = cplx(vre[k], vim[k]);
+  if (count) {
+    // stored entries of the reference matrix: merged values with |v| > eps
+    uint32_t c = (hypot(C.dre, C.dim) > kEps) ? 1u : 0u;
+    for (int k = 0; k < C.cnt; ++k)
+      if (hypot(C.vre[k], C.vim[k]) > kEps) ++c;
+    count[col] = c;
+    return;
+  }
+  if (dense) {  // row-major: dense[row * n + col]
+    dense[size_t(col) * a.n + col] = cplx(C.dre, C.dim);
+    for (int k = 0; k < C.cnt; ++k) dense[size_t(C.rows[k]) * a.n + col] = cplx(C.vre[k], C.vim[k]);
     return;
   }
   if (diag_re) {
-    diag_re[row] = dre;
-    diag_im[row] = dim_;
+    diag_re[col] = C.dre;
+    diag_im[col] = C.dim;
   }
-  if (rowsum) {
-    // entries with |v| <= eps are dropped by the reference's assembly
+  if (colsum) {
+    // shifted_trace_and_one_norm (src/solve/mod.rs:59-104): off-diagonal
+    // stored entries |v| plus |diag - shift| for every column
     double s = 0;
-    for (int k = 0; k < cnt; ++k) {
-      double m = sqrt(vre[k] * vre[k] + vim[k] * vim[k]);
-      if (m > 2.220446049250313e-16) s += m;
+    for (int k = 0; k < C.cnt; ++k) {
+      double m = hypot(C.vre[k], C.vim[k]);
+      if (m > kEps) s += m;
     }
-    double ddre = dre - shift.real(), ddim = dim_ - shift.imag();
-    s += sqrt(ddre * ddre + ddim * ddim);
-    rowsum[row] = s;
+    double dr = (hypot(C.dre, C.dim) > kEps) ? C.dre : 0.0, di = (hypot(C.dre, C.dim) > kEps) ? C.dim : 0.0;
+    s += hypot(dr - shift.real(), di - shift.imag());
+    colsum[col] = s;
   }
+}
+
+// Hermiticity of the sector matrix, entry by entry as the reference checks
+// it: every stored entry (j, r, v) needs |v - conj(H[r, j])| <= 1e-12
+// (src/operator/mod.rs:1240-1242, 1313-1398, 2212-2214). Lane r lists column
+// r and re-evaluates column j for each of its entries.
+__global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
+                                                             int* flags) {
+  const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= a.n) return;
+  ColList<kMaxColEntries> C;
+  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  if (C.overflow) {
+    flags[1] = 1;
+    return;
+  }
+  const double tol = 1.0e-12;
+  bool bad = false;
+  if (hypot(C.dre, C.dim) > kEps && 2.0 * fabs(C.dim) > tol) bad = true;
+  for (int k = 0; k < C.cnt && !bad; ++k) {
+    if (hypot(C.vre[k], C.vim[k]) <= kEps) continue;  // not stored
+    double wr, wi;
+    entry_at(a, a.meta, famp, sym, shift_needed, a.spl, uint32_t(C.rows[k]), col, wr, wi);
+    if (hypot(wr, wi) <= kEps) wr = wi = 0.0;
+    if (hypot(C.vre[k] - wr, C.vim[k] + wi) > tol) bad = true;
+  }
+  if (bad) flags[0] = 1;
+}
+
+// y = H x by columns with atomic accumulation, for sector matrices that are
+// not Hermitian (assembly checks disabled); y must be zeroed first.
+__global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
+                                                         const double2* __restrict__ x, double* y) {
+  const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= a.n) return;
+  const uint32_t r = a.states[col];
+  const double ns = sym ? double(a.orbits[col]) : 1.0;
+  const double2 xc = x[col];
+  double dre = 0, dim_ = 0;
+  for (int g = 0; g < a.n_diag; ++g) {
+    const int off = int(a.meta[4 * g + 3]) + pattern_of(r, a.meta[4 * g + 1]);
+    dre += famp[2 * off];
+    dim_ += famp[2 * off + 1];
+  }
+  atomicAdd(y + 2 * col, dre * xc.x - dim_ * xc.y);
+  atomicAdd(y + 2 * col + 1, dre * xc.y + dim_ * xc.x);
+  const int G = a.n_diag + a.n_off;
+  for (int g = a.n_diag; g < G; ++g) {
+    const int pat = pattern_of(r, a.meta[4 * g + 1]);
+    if (!((a.meta[4 * g + 2] >> pat) & 1u)) continue;
+    int idx, orbit, ci, sign;
+    if (!target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign)) continue;
+    double re, im;
+    column_value(a, famp, int(a.meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    atomicAdd(y + 2 * idx, re * xc.x - im * xc.y);
+    atomicAdd(y + 2 * idx + 1, re * xc.y + im * xc.x);
+  }
+}
+
+int sym_code(const SpinBasis1D& b) {
+  const SectorGeometry& g = b.geometry();
+  if (g.kmom >= 0 && g.parity) return 3;
+  if (g.kmom >= 0) return 1;
+  if (g.parity) return 2;
+  return 0;
 }
 
 }  // namespace
 
 // Device copies of the compiled flip groups and symmetry tables.
+// amp: gather amplitudes conj(c·a) (real or (re, im)), read by the H·v
+// kernels; famp: forward amplitudes c·a as (re, im), read by the column
+// kernels.
 struct Operator::DeviceTables {
   DeviceArray<uint32_t> meta;
   DeviceArray<double> amp;
+  DeviceArray<double> famp;
   DeviceArray<double> chars;
   int n_diag = 0, n_off = 0, n_amp = 0;
   bool complex_coef = false;  // amplitudes or characters complex
+  bool hermitian = true;      // sector matrix Hermitian: gather form valid
   HvArgs args(const SpinBasis1D& b) const {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
@@ -424,6 +549,22 @@ struct Operator::DeviceTables {
   }
 };
 
+static bool sector_matrix_hermitian(const Operator& op, hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  HvArgs a = op.tables_->args(b);
+  if (a.n == 0) return true;
+  DeviceArray<int> flags(2);
+  QMBED_HIP(hipMemsetAsync(flags.data(), 0, 2 * sizeof(int), s));
+  hipLaunchKernelGGL(hermiticity_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
+                     op.tables_->famp.data(), sym_code(b), b.geometry().kmom > 0 ? 1 : 0, flags.data());
+  QMBED_HIP(hipGetLastError());
+  int h[2] = {0, 0};
+  QMBED_HIP(hipMemcpyAsync(h, flags.data(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  if (h[1]) throw unsupported_backend("a column has more distinct entries than the Hermiticity check holds");
+  return h[0] == 0;
+}
+
 std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   if (!basis_) throw invalid_options("operator builder has no basis");
   auto op = std::shared_ptr<Operator>(new Operator());
@@ -440,7 +581,7 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
     throw unsupported_backend("the operator has more distinct local actions than the H·v kernel tables hold");
   t->complex_coef = !op->real_;
   std::vector<uint32_t> meta;
-  std::vector<double> amp;
+  std::vector<double> amp, famp;
   int n_amp = 0;
   for (auto* v : {&op->diag_, &op->off_}) {
     for (auto& g : *v) {
@@ -455,7 +596,9 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
       meta.push_back(uint32_t(n_amp));
       for (auto& c : g.amp) {
         amp.push_back(c.real());
-        if (t->complex_coef) amp.push_back(c.imag());
+        if (t->complex_coef) amp.push_back(-c.imag());  // gather form reads conj(c·a)
+        famp.push_back(c.real());
+        famp.push_back(c.imag());
       }
       n_amp += int(g.amp.size());
     }
@@ -482,11 +625,24 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   }
   t->meta.resize(std::max<size_t>(meta.size(), 4));
   t->amp.resize(std::max<size_t>(amp.size(), 2));
+  t->famp.resize(std::max<size_t>(famp.size(), 2));
   t->chars.resize(64);
   if (!meta.empty()) QMBED_HIP(hipMemcpy(t->meta.data(), meta.data(), meta.size() * 4, hipMemcpyHostToDevice));
   if (!amp.empty()) QMBED_HIP(hipMemcpy(t->amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice));
+  if (!famp.empty()) QMBED_HIP(hipMemcpy(t->famp.data(), famp.data(), famp.size() * 8, hipMemcpyHostToDevice));
   QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), 64 * 8, hipMemcpyHostToDevice));
   op->tables_ = t;
+  // The gather-form kernels read row r as conj(column r), which is the
+  // sector matrix exactly when it is Hermitian; the reference's assembly
+  // checks exactly that (1e-12 per entry) unless checks are disabled.
+  if (basis_->len() > 0) {
+    if (checks_.hermiticity) {
+      if (!sector_matrix_hermitian(*op, nullptr)) throw non_hermitian();
+    } else {
+      t->hermitian = sector_matrix_hermitian(*op, nullptr);
+      if (!t->hermitian) op->real_ = false;
+    }
+  }
   return op;
 }
 
@@ -526,15 +682,18 @@ void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream
   QMBED_HIP(hipGetLastError());
 }
 
-int sym_code(const SpinBasis1D& b) {
-  const SectorGeometry& g = b.geometry();
-  if (g.kmom >= 0 && g.parity) return 3;
-  if (g.kmom >= 0) return 1;
-  if (g.parity) return 2;
-  return 0;
-}
-
 }  // namespace
+
+static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  HvArgs a = op.tables_->args(b);
+  if (a.n == 0) return;
+  QMBED_HIP(hipMemsetAsync(y, 0, size_t(a.n) * sizeof(cplx), s));
+  hipLaunchKernelGGL(scatter_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
+                     op.tables_->famp.data(), sym_code(b), b.geometry().kmom > 0 ? 1 : 0,
+                     reinterpret_cast<const double2*>(x), reinterpret_cast<double*>(y));
+  QMBED_HIP(hipGetLastError());
+}
 
 void Operator::apply(const double* x, double* y, hipStream_t s) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
@@ -549,6 +708,10 @@ void Operator::apply(const float* x, float* y, hipStream_t s) const {
 }
 
 void Operator::apply(const cplx* x, cplx* y, hipStream_t s) const {
+  if (!tables_->hermitian) {
+    scatter_apply(*this, x, y, s);
+    return;
+  }
   HvArgs a = tables_->args(*basis_);
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
@@ -557,6 +720,8 @@ void Operator::apply(const cplx* x, cplx* y, hipStream_t s) const {
 }
 
 void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s) const {
+  if (!tables_->hermitian)
+    throw unsupported_backend("single-precision action of a non-Hermitian sector matrix is not provided");
   HvArgs a = tables_->args(*basis_);
   const float* xf = reinterpret_cast<const float*>(x);
   float* yf = reinterpret_cast<float*>(y);
@@ -565,6 +730,10 @@ void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipSt
 }
 
 void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const {
+  if (!tables_->hermitian) {
+    for (int v = 0; v < nvec; ++v) scatter_apply(*this, x + size_t(v) * dim(), y + size_t(v) * dim(), s);
+    return;
+  }
   HvArgs a = tables_->args(*basis_);
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
@@ -591,29 +760,28 @@ void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) cons
   }
 }
 
-static void run_row_entries(const Operator& op, cplx* dense, double* rowsum, double* dre, double* dim_, cplx shift,
-                            hipStream_t s, uint32_t* rowcount = nullptr) {
+static void run_columns(const Operator& op, cplx* dense, double* colsum, double* dre, double* dim_, cplx shift,
+                        hipStream_t s, uint32_t* count = nullptr) {
   const SpinBasis1D& b = op.basis();
   HvArgs a = op.tables_->args(b);
   if (a.n == 0) return;
   DeviceArray<int> overflow(1);
   QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
-  size_t sm = sizeof(double) * 2 * size_t(a.n_amp) + sizeof(uint32_t) * (4 * size_t(a.n_diag + a.n_off) + a.nspl);
   dim3 grid((a.n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(row_entries_kernel, grid, dim3(kBlock), sm, s, a, sym_code(b), b.geometry().kmom > 0 ? 1 : 0,
-                     op.tables_->complex_coef ? 1 : 0, dense, rowsum, dre, dim_, overflow.data(), shift, rowcount);
+  hipLaunchKernelGGL(column_entries_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
+                     b.geometry().kmom > 0 ? 1 : 0, dense, colsum, dre, dim_, overflow.data(), shift, count);
   QMBED_HIP(hipGetLastError());
   int of = 0;
   QMBED_HIP(hipMemcpyAsync(&of, overflow.data(), sizeof(int), hipMemcpyDeviceToHost, s));
   QMBED_HIP(hipStreamSynchronize(s));
-  if (of) throw unsupported_backend("a row has more distinct off-diagonal entries than the row-entry kernel holds");
+  if (of) throw unsupported_backend("a column has more distinct entries than the column-entry kernel holds");
 }
 
 size_t Operator::stored_entries(hipStream_t s) const {
   size_t n = dim();
   if (n == 0) return 0;
   DeviceArray<uint32_t> cnt(n);
-  run_row_entries(*this, nullptr, nullptr, nullptr, nullptr, cplx(0, 0), s, cnt.data());
+  run_columns(*this, nullptr, nullptr, nullptr, nullptr, cplx(0, 0), s, cnt.data());
   std::vector<uint32_t> h(n);
   QMBED_HIP(hipMemcpy(h.data(), cnt.data(), n * 4, hipMemcpyDeviceToHost));
   size_t tot = 0;
@@ -624,29 +792,29 @@ size_t Operator::stored_entries(hipStream_t s) const {
 void Operator::materialize_dense(cplx* dense, hipStream_t s) const {
   size_t n = dim();
   QMBED_HIP(hipMemsetAsync(dense, 0, n * n * sizeof(cplx), s));
-  run_row_entries(*this, dense, nullptr, nullptr, nullptr, cplx(0, 0), s);
+  run_columns(*this, dense, nullptr, nullptr, nullptr, cplx(0, 0), s);
 }
 
 std::pair<cplx, double> Operator::shifted_trace_and_one_norm(hipStream_t s) const {
   size_t n = dim();
   if (n == 0) return {cplx(0, 0), 0.0};
-  DeviceArray<double> dre(n), dimv(n), rowsum(n);
-  run_row_entries(*this, nullptr, nullptr, dre.data(), dimv.data(), cplx(0, 0), s);
+  DeviceArray<double> dre(n), dimv(n), colsum(n);
+  run_columns(*this, nullptr, nullptr, dre.data(), dimv.data(), cplx(0, 0), s);
   std::vector<double> hre(n), him(n);
   QMBED_HIP(hipMemcpy(hre.data(), dre.data(), n * 8, hipMemcpyDeviceToHost));
   QMBED_HIP(hipMemcpy(him.data(), dimv.data(), n * 8, hipMemcpyDeviceToHost));
   cplx trace(0, 0);
   for (size_t i = 0; i < n; ++i) {
-    // diagonal entries with |v| <= eps are dropped by the reference's assembly
+    // diagonal entries with |v| <= eps are not stored by the reference
     cplx d(hre[i], him[i]);
-    if (std::abs(d) > 2.220446049250313e-16) trace += d;
+    if (std::abs(d) > kEps) trace += d;
   }
   cplx shift = trace / double(n);
-  run_row_entries(*this, nullptr, rowsum.data(), nullptr, nullptr, shift, s);
-  std::vector<double> rs(n);
-  QMBED_HIP(hipMemcpy(rs.data(), rowsum.data(), n * 8, hipMemcpyDeviceToHost));
+  run_columns(*this, nullptr, colsum.data(), nullptr, nullptr, shift, s);
+  std::vector<double> cs(n);
+  QMBED_HIP(hipMemcpy(cs.data(), colsum.data(), n * 8, hipMemcpyDeviceToHost));
   double m = 0;
-  for (double v : rs) m = std::max(m, v);
+  for (double v : cs) m = std::max(m, v);
   return {shift, m};
 }
 

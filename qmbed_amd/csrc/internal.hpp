@@ -6,6 +6,7 @@
 
 namespace qmbed {
 
+// flip groups holding the column entries c·a of every coupling
 void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& terms, const AssemblyChecks& checks,
                     bool nonempty, std::vector<FlipGroup>& diag, std::vector<FlipGroup>& off, bool& real);
 

@@ -5,12 +5,11 @@
 // For spin one-half every local product maps a basis state to at most one
 // state, and the set of flipped bits does not depend on the state: '+', '-',
 // 'x', 'y' flip their site, 'z' and 'I' do not. Couplings that flip the same
-// bits are merged into one FlipGroup whose amplitude table is indexed by the
-// bits of the row state at the group's sites. The kernels apply the gather
-// (row) form y[r] = Σ_j H[r,j] x[j]; the row entries H[r,·] are the complex
-// conjugates of the column entries the reference's column-streaming assembly
-// generates for the adjoint product (reversed order, '+' <-> '-',
-// conjugated coefficient), so each table stores conj(c' · a'(pattern)).
+// bits are merged into one FlipGroup whose amplitude table, indexed by the
+// bits of the source state at the group's sites, holds c·a — the column
+// entries of the reference's column-streaming assembly. For a Hermitian
+// sector matrix the kernels read row r as the conjugate of column r
+// (gather form).
 #include <algorithm>
 #include <cmath>
 #include <map>
@@ -103,14 +102,6 @@ static bool local_apply(const SpinFactors& f, const std::string& sym, const std:
   return true;
 }
 
-static char adjoint_symbol(char c) {
-  switch (c) {
-    case '+': return '-';
-    case '-': return '+';
-    default: return c;
-  }
-}
-
 // operator_sum_preserves_particle_sector with the spin basis' sector test
 // (QMBED src/operator/mod.rs:1963-2033, src/basis/mod.rs:294-371).
 static bool preserves_particle_sector(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& terms) {
@@ -162,77 +153,11 @@ static bool preserves_particle_sector(const SpinBasisSpec& spec, const std::vect
   return true;
 }
 
-// Hermiticity of the operator sum, tested exactly on its expansion in Pauli
-// strings: every coefficient of Σ c·Π O must be real (within 1e-12) because
-// Pauli strings are Hermitian and linearly independent.
-static bool terms_hermitian(const std::vector<OperatorSpec>& terms, const SpinFactors& f) {
-  // local 2x2 matrix of one symbol: M[out][in]
-  auto local = [&](char c, cplx M[2][2]) {
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) M[a][b] = 0.0;
-    switch (c) {
-      case 'I': M[0][0] = M[1][1] = 1.0; break;
-      case 'z': M[0][0] = f.z[0]; M[1][1] = f.z[1]; break;
-      case '+': M[1][0] = f.raise[0]; break;
-      case '-': M[0][1] = f.lower[1]; break;
-      case 'x': M[1][0] = cplx(f.xy, 0) * f.raise[0]; M[0][1] = cplx(f.xy, 0) * f.lower[1]; break;
-      case 'y': M[1][0] = cplx(0, -f.xy) * f.raise[0]; M[0][1] = cplx(0, f.xy) * f.lower[1]; break;
-      default: throw invalid_operator(std::string(1, c));
-    }
-  };
-  // Pauli basis {1, X, Y, Z}: M = Σ_p w_p σ_p with w_p = tr(σ_p M)/2
-  auto pauli_coeffs = [](const cplx M[2][2], cplx w[4]) {
-    w[0] = 0.5 * (M[0][0] + M[1][1]);
-    w[1] = 0.5 * (M[0][1] + M[1][0]);
-    w[2] = 0.5 * cplx(0, 1) * (M[0][1] - M[1][0]);  // tr(σy M)/2 with σy = [[0,-i],[i,0]]
-    w[3] = 0.5 * (M[0][0] - M[1][1]);
-  };
-  std::map<std::vector<std::pair<int, int>>, cplx> strings;
-  for (const auto& t : terms) {
-    std::string sym = strip_bars(t.symbols);
-    for (const auto& c : t.couplings) {
-      // per-site product, applied right to left: M_site = O_first ... O_last
-      std::map<int, std::array<std::array<cplx, 2>, 2>> per_site;
-      for (int pos = int(sym.size()) - 1; pos >= 0; --pos) {
-        cplx L[2][2];
-        local(sym[pos], L);
-        int site = c.sites[pos];
-        auto it = per_site.find(site);
-        if (it == per_site.end()) {
-          std::array<std::array<cplx, 2>, 2> m{};
-          for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) m[a][b] = L[a][b];
-          per_site[site] = m;
-        } else {
-          auto& m = it->second;
-          std::array<std::array<cplx, 2>, 2> r{};
-          for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) r[a][b] = L[a][0] * m[0][b] + L[a][1] * m[1][b];
-          m = r;
-        }
-      }
-      std::vector<std::pair<std::vector<std::pair<int, int>>, cplx>> acc{{{}, c.coefficient}};
-      for (auto& kv : per_site) {
-        cplx M[2][2] = {{kv.second[0][0], kv.second[0][1]}, {kv.second[1][0], kv.second[1][1]}};
-        cplx w[4];
-        pauli_coeffs(M, w);
-        std::vector<std::pair<std::vector<std::pair<int, int>>, cplx>> next;
-        for (auto& a : acc)
-          for (int p = 0; p < 4; ++p) {
-            if (w[p] == cplx(0, 0)) continue;
-            auto key = a.first;
-            if (p != 0) key.push_back({kv.first, p});
-            next.push_back({key, a.second * w[p]});
-          }
-        acc.swap(next);
-      }
-      for (auto& a : acc) strings[a.first] += a.second;
-    }
-  }
-  for (auto& kv : strings)
-    if (std::abs(kv.second.imag()) > 1.0e-12) return false;
-  return true;
-}
+// Group the couplings by flip mask; each table holds c·a(pattern) for the
+// source state's bits, i.e. the column entries the reference's
+// column-streaming assembly generates.
+static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFactors& f, std::vector<FlipGroup>& diag,
+                         std::vector<FlipGroup>& off);
 
 void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& terms, const AssemblyChecks& checks,
                     bool nonempty, std::vector<FlipGroup>& diag, std::vector<FlipGroup>& off, bool& real) {
@@ -256,19 +181,25 @@ void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& 
       }
     }
   }
-  if (checks.hermiticity && nonempty && !terms_hermitian(terms, f)) throw non_hermitian();
+  build_groups(terms, f, diag, off);
+  real = true;
+  for (auto* v : {&diag, &off})
+    for (auto& g : *v)
+      for (auto& a : g.amp)
+        if (a.imag() != 0.0) real = false;
+}
 
+static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFactors& f, std::vector<FlipGroup>& diag,
+                         std::vector<FlipGroup>& off) {
   diag.clear();
   off.clear();
   std::map<std::vector<int>, size_t> diag_index;  // site set -> group
   for (const auto& t : terms) {
     std::string sym = strip_bars(t.symbols);
-    // adjoint product: reversed order, '+' <-> '-', conjugated coefficient
-    std::string adj(sym.rbegin(), sym.rend());
-    for (char& ch : adj) ch = adjoint_symbol(ch);
+    const std::string& adj = sym;
     for (const auto& c : t.couplings) {
       if (c.coefficient == cplx(0, 0)) continue;
-      std::vector<int> sites(c.sites.rbegin(), c.sites.rend());
+      const std::vector<int>& sites = c.sites;
       std::vector<int> uniq = sites;
       std::sort(uniq.begin(), uniq.end());
       uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
@@ -278,8 +209,6 @@ void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& 
         slot[i] = int(std::lower_bound(uniq.begin(), uniq.end(), sites[i]) - uniq.begin());
       // flip mask (pattern independent for spin one-half)
       uint32_t flip_local = 0;
-      for (char ch : adj)
-        (void)ch;
       for (size_t i = 0; i < adj.size(); ++i)
         if (adj[i] == '+' || adj[i] == '-' || adj[i] == 'x' || adj[i] == 'y') flip_local ^= 1u << slot[i];
       uint32_t flip = 0;
@@ -292,7 +221,7 @@ void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& 
         uint32_t out;
         cplx a;
         if (local_apply(f, adj, slot, uint32_t(pat), out, a)) {
-          local_amp[pat] = std::conj(std::conj(c.coefficient) * a);
+          local_amp[pat] = c.coefficient * a;
         }
       }
       // merge into a group
@@ -365,11 +294,6 @@ void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& 
   };
   diag.erase(std::remove_if(diag.begin(), diag.end(), empty), diag.end());
   off.erase(std::remove_if(off.begin(), off.end(), empty), off.end());
-  real = true;
-  for (auto* v : {&diag, &off})
-    for (auto& g : *v)
-      for (auto& a : g.amp)
-        if (a.imag() != 0.0) real = false;
 }
 
 }  // namespace qmbed

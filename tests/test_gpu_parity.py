@@ -60,7 +60,14 @@ def test_hv_fp64(qa, oracle, name, L, up, k, p):
         pytest.skip("term set does not conserve the sector")
     spec = HAMS[name](L)
     print(f"[hv] {name} L={L} up={up} k={k} p={p}", flush=True)
-    om = oracle.Model(L, spec, up=up, momentum=k, parity=p)
+    try:
+        om = oracle.Model(L, spec, up=up, momentum=k, parity=p)
+    except oracle.OracleError as e:
+        # e.g. the DM term breaks reflection symmetry: the sector matrix is not Hermitian
+        with pytest.raises(qa.QmbedError) as g:
+            qa.Model(_basis_json(L, up, k, p), spec)
+        assert str(g.value) == str(e)
+        return
     if om.dim == 0:
         pytest.skip("empty sector")
     m = qa.Model(_basis_json(L, up, k, p), spec)
@@ -205,3 +212,29 @@ def test_expm_plan_matches_oracle(qa, oracle):
         assert np.linalg.norm(got - ref[c]) <= 1e-12 * np.linalg.norm(ref[c])
     qa.command({"operation": "release_expm_action", "expm_handle": plan["handle"]})
     qa.command({"operation": "release_model", "handle": h})
+
+
+def test_symmetry_incompatible_operator_is_rejected(qa, oracle):
+    # an extra exchange on one bond breaks translation symmetry: the reduced
+    # matrix is not Hermitian, and both sides report it with the reference's text
+    L = 8
+    spec = cases.heisenberg(L) + [("+-", [(0.3, [2, 3])]), ("-+", [(0.3, [2, 3])])]
+    with pytest.raises(oracle.OracleError, match="a Hermitian operator is required"):
+        oracle.Model(L, spec, up=4, momentum=1)
+    with pytest.raises(qa.QmbedError, match="^a Hermitian operator is required$"):
+        qa.Model(_basis_json(L, 4, 1, None), spec)
+    # without the check the operator is still built (checks.hermiticity = false)
+    m = qa.Model(_basis_json(L, 4, 1, None), spec, checks={"hermiticity": False})
+    om = oracle.Model(L, spec, up=4, momentum=1, check_hermiticity=False)
+    x = cases.rand_vec(om.dim, seed=2)
+    np.testing.assert_allclose(m.apply(x), om.apply(x), atol=1e-13 * np.linalg.norm(om.apply(x)))
+
+
+def test_non_hermitian_terms_rejected(qa, oracle):
+    L = 6
+    spec = [("+-", [(1.0, [i, (i + 1) % L]) for i in range(L)])]
+    with pytest.raises(oracle.OracleError, match="a Hermitian operator is required"):
+        oracle.Model(L, spec, up=3)
+    r = qa.raw_call("qmbed_run_json", {"basis": _basis_json(L, 3, None, None), "terms": qa.terms_from_spec(spec),
+                                       "solver": {"eigenpairs": 1, "target": {"kind": "smallest_algebraic"}}})
+    assert r == {"status": "error", "error": "a Hermitian operator is required"}
