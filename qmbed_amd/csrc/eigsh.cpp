@@ -160,6 +160,8 @@ class LanczosWindow {
       na = std::sqrt(host[1]);
       second = true;
     }
+    if (!std::isfinite(alpha_host_) || !std::isfinite(na) || !std::isfinite(nb))
+      throw internal_state("non-finite Lanczos coefficient at step " + std::to_string(j));
     passes_ += 1 + (second ? 1 : 0);
     seconds_ += second ? 1 : 0;
     alpha_.push_back(alpha_host_);
@@ -195,7 +197,7 @@ class LanczosWindow {
     for (int idx : sel) {
       auto s = tridiagonal_eigvec(a, b, vals[idx]);
       double est = last_beta_ * std::fabs(s[sz - 1]);
-      if (est > safety * tol * std::max(std::fabs(vals[idx]), 1.0)) return false;
+      if (!std::isfinite(est) || est > safety * tol * std::max(std::fabs(vals[idx]), 1.0)) return false;
     }
     return true;
   }
@@ -453,6 +455,7 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
     } else {
       QMBED_HIP(hipMemcpyAsync(start.data(), re.data(), n * 8, hipMemcpyDeviceToDevice, s));
     }
+    QMBED_HIP(hipStreamSynchronize(s));  // re dies here
   }
   {
     lk::Workspace ws;
@@ -463,6 +466,7 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
     QMBED_HIP(hipStreamSynchronize(s));
     if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) throw non_convergence(0, std::sqrt(nn));
     lk::normalize(c, start.data(), sl.data(), n, start.data(), nullptr, nullptr, s);
+    QMBED_HIP(hipStreamSynchronize(s));  // ws and sl die here
   }
 
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;

@@ -406,6 +406,7 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
         QMBED_HIP(hipMemcpyAsync(dactive.data(), cont.data(), nc * 4, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(kb_scale, B.grid(), dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()),
                            dscale.data(), dactive.data(), n, reinterpret_cast<double2*>(Vc + size_t(j + 1) * ldj));
+        QMBED_HIP(hipStreamSynchronize(s));  // inv / cont are host staging for the copies above
       }
     }
     // ---- per-column step control (host, m×m) ----
