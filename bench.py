@@ -47,7 +47,7 @@ def hv_roofline(qa, model, dtype, vec_bytes, iters=50):
     algo = model.dim * (4 + vec_bytes) + nnz * (vec_bytes + 4)
     gbs = algo / (ms * 1e-3) / 1e9
     return {"dim": model.dim, "nnz": nnz, "bytes_per_launch": algo, "ms": ms, "achieved_GBs": gbs,
-            "frac": gbs / HBM_PEAK_GBS}
+            "frac": gbs / HBM_PEAK_GBS, **model.hv_config()}
 
 
 def cpu_baseline(delta=1.5):

@@ -60,6 +60,7 @@ def library():
     L.qmbed_hip_model_states.argtypes = [c_void_p, c_void_p, c_void_p]
     L.qmbed_hip_model_nnz.restype = c_long
     L.qmbed_hip_model_nnz.argtypes = [c_void_p]
+    L.qmbed_hip_model_hv_config.argtypes = [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     L.qmbed_hip_model_stream.restype = c_void_p
     L.qmbed_hip_model_stream.argtypes = [c_void_p]
     L.qmbed_hip_apply_device.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p]
@@ -205,6 +206,11 @@ class Model:
         if library().qmbed_hip_model_states(self._h, s.ctypes.data, o.ctypes.data) != 0:
             raise QmbedError("state copy failed")
         return s, o
+
+    def hv_config(self):
+        lpr, img = ctypes.c_int(), ctypes.c_int()
+        library().qmbed_hip_model_hv_config(self._h, ctypes.byref(lpr), ctypes.byref(img))
+        return {"lanes_per_row": lpr.value, "mode": "image" if img.value else "walk"}
 
     def nnz(self):
         v = int(library().qmbed_hip_model_nnz(self._h))

@@ -175,7 +175,7 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   if (spec.spin_twice != 1)
     throw unsupported_backend("the MI355X spin engine implements spin one-half sectors");
   const int L = spec.sites;
-  if (L > 32) throw unsupported_backend("the MI355X spin engine packs states in 32 bits (at most 32 sites)");
+  if (L > 31) throw unsupported_backend("the MI355X spin engine packs states in 32 bits (at most 31 sites)");
   if (spec.up && *spec.up > L) throw invalid_sector("particle count exceeds site count");
   for (int u : spec.up_sectors)
     if (u < 0 || u > L) throw invalid_sector("particle sector exceeds the spin capacity");
@@ -247,7 +247,9 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
     total += counts[i];
   }
   b->n_ = total;
-  b->states_.resize(std::max<uint64_t>(total, 1));
+  const uint64_t padded = std::max<uint64_t>(16, (total + 15) / 16 * 16);
+  b->states_.resize(padded);
+  QMBED_HIP(hipMemsetAsync(b->states_.data(), 0xFF, padded * 4, stream));
   b->orbits_.resize(std::max<uint64_t>(total, 1));
   QMBED_HIP(hipMemcpyAsync(d_offsets.data(), offsets.data(), blocks * 8, hipMemcpyHostToDevice, stream));
   hipLaunchKernelGGL(enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, nullptr,
@@ -265,6 +267,11 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   b->splitters_.resize(cnt);
   hipLaunchKernelGGL(splitter_kernel, dim3((cnt + 255) / 256), dim3(256), 0, stream, b->states_.data(), total, shift,
                      cnt, b->splitters_.data());
+  QMBED_HIP(hipGetLastError());
+  const uint64_t nheads = std::max<uint64_t>(16, (padded / 16 + 15) / 16 * 16);
+  b->heads_.resize(nheads);
+  hipLaunchKernelGGL(splitter_kernel, dim3(unsigned((nheads + 255) / 256)), dim3(256), 0, stream, b->states_.data(),
+                     padded, 4, int(nheads), b->heads_.data());
   QMBED_HIP(hipGetLastError());
   QMBED_HIP(hipStreamSynchronize(stream));
   return b;

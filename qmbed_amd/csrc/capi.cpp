@@ -652,6 +652,17 @@ long qmbed_hip_model_nnz(const qmbed_hip_model* m) {
   }
 }
 
+int qmbed_hip_model_hv_config(const qmbed_hip_model* m, int* lanes_per_row, int* image_table) {
+  try {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    if (lanes_per_row) *lanes_per_row = tm->m->op->lanes_per_row();
+    if (image_table) *image_table = tm->m->op->uses_image_table() ? 1 : 0;
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 void* qmbed_hip_model_stream(const qmbed_hip_model* m) { return reinterpret_cast<const TypedModel*>(m)->m->stream; }
 
 int qmbed_hip_apply_device(const qmbed_hip_model* m, int dtype, const void* x, void* y, void* stream) {

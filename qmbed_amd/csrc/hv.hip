@@ -16,7 +16,10 @@
 // row-wise: the row entries are the conjugated column entries of the adjoint
 // product, which is what the flip-group tables hold.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 
 #include "internal.hpp"
 
@@ -41,6 +44,19 @@ struct HvArgs {
   const uint32_t* spl;
   int nspl;   // power of two
   int sh;
+  const uint32_t* heads;
+  uint32_t nheads;
+  // lanes per row (power of two) and the mask of flip groups a lane owns
+  int lpr;
+  uint32_t lane_groups;
+  // image-table mode: per parent-state rank, a packed (index, orbit code,
+  // reflection sign, character index) word; 0xFFFFFFFF = not in the sector
+  const uint32_t* img;
+  int img_bits;          // bits of the representative index
+  int up;                // fixed up count (colex ranks) or -1 (rank = state)
+  const uint32_t* hop;   // per group: p | q << 8 | 1 << 16 for two-site hops
+  const uint32_t* binom; // C(n, k) at [n * 32 + k], n, k < 32
+  const double* code_rsq;  // 1/sqrt(orbit size) per orbit code (16)
   // groups (diag first, then off-diagonal)
   const uint32_t* meta;  // 4 words per group: flip, packed sites, nzmask, amp offset
   const double* amp;     // real: 1 double per entry; complex: 2 doubles per entry
@@ -66,17 +82,81 @@ __device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
   return pat;
 }
 
+// Count of the 16 sorted entries of one aligned 64-byte line that are
+// <= key (and whether key occurs), read as four 16-byte loads.
+__device__ __forceinline__ int line_count(const uint32_t* __restrict__ line, uint32_t key, bool& found) {
+  const uint4* p = reinterpret_cast<const uint4*>(line);
+  const uint4 a = __ldg(p), b = __ldg(p + 1), c = __ldg(p + 2), d = __ldg(p + 3);
+  int cnt = int(a.x <= key) + int(a.y <= key) + int(a.z <= key) + int(a.w <= key) + int(b.x <= key) +
+            int(b.y <= key) + int(b.z <= key) + int(b.w <= key) + int(c.x <= key) + int(c.y <= key) +
+            int(c.z <= key) + int(c.w <= key) + int(d.x <= key) + int(d.y <= key) + int(d.z <= key) +
+            int(d.w <= key);
+  found = (a.x == key) | (a.y == key) | (a.z == key) | (a.w == key) | (b.x == key) | (b.y == key) | (b.z == key) |
+          (b.w == key) | (c.x == key) | (c.y == key) | (c.z == key) | (c.w == key) | (d.x == key) | (d.y == key) |
+          (d.z == key) | (d.w == key);
+  return cnt;
+}
+
+// Index of `key` in the sorted state table, or -1. Three levels:
+//  1. LDS splitters (one per 2^sh states): binary search, no global traffic;
+//  2. the aligned 64-byte line of line heads covering the bucket (heads[i] =
+//     states[16 i], a small table that stays in L2);
+//  3. the aligned 64-byte line of states holding the answer.
+// Levels 2 and 3 are single line reads counted without dependent loads.
 __device__ __forceinline__ int find_state(uint32_t key, const uint32_t* spl, int nspl, int sh,
-                                          const uint32_t* __restrict__ states, uint32_t n) {
+                                          const uint32_t* __restrict__ states, const uint32_t* __restrict__ heads,
+                                          uint32_t nheads) {
   int pos = 0;
   for (int step = nspl >> 1; step > 0; step >>= 1)
     if (spl[pos + step] <= key) pos += step;
-  uint32_t base = uint32_t(pos) << sh;
-  for (uint32_t step = (1u << sh) >> 1; step > 0; step >>= 1) {
-    uint32_t c = base + step;
-    if (c < n && __ldg(states + c) <= key) base = c;
+  uint32_t line;
+  bool found;
+  if (sh <= 4) {
+    line = (uint32_t(pos) << sh) >> 4;
+  } else if (sh <= 8) {
+    const uint32_t hl = (uint32_t(pos) << (sh - 4)) & ~15u;
+    const int c = line_count(heads + hl, key, found);
+    if (c == 0) return -1;
+    line = hl + uint32_t(c) - 1;
+  } else {
+    uint32_t h0 = uint32_t(pos) << (sh - 4);
+    for (uint32_t step = (1u << (sh - 4)) >> 1; step > 0; step >>= 1)
+      if (h0 + step < nheads && __ldg(heads + h0 + step) <= key) h0 += step;
+    line = h0;
   }
-  return (base < n && __ldg(states + base) == key) ? int(base) : -1;
+  const int c = line_count(states + 16 * size_t(line), key, found);
+  return found ? int(16 * line + uint32_t(c) - 1) : -1;
+}
+
+// Colex (Gosper-order) rank of a fixed-weight state: Σ C(position, ordinal)
+// over its set bits (QMBED src/basis/mod.rs:640-659).
+__device__ __forceinline__ uint32_t rank_colex(uint32_t s, const uint32_t* binom) {
+  uint32_t rank = 0;
+  int ord = 1;
+  while (s) {
+    const int pos = __ffs(s) - 1;
+    rank += binom[pos * 32 + ord];
+    s &= s - 1;
+    ++ord;
+  }
+  return rank;
+}
+
+// rank(s') - rank(s) when the set bit at p moves to the empty position q > p:
+// bits strictly between p and q lose one ordinal, the moved bit gains them.
+__device__ __forceinline__ int move_delta(uint32_t s, int p, int q, const uint32_t* binom) {
+  const int a = __popc(s & ((1u << p) - 1u)) + 1;
+  uint32_t mid = s & (((q >= 32) ? 0xFFFFFFFFu : ((1u << q) - 1u)) & ~((2u << p) - 1u));
+  int m = __popc(mid);
+  int d = int(binom[q * 32 + a + m]) - int(binom[p * 32 + a]);
+  int k = a;
+  while (mid) {
+    const int b = __ffs(mid) - 1;
+    d += int(binom[b * 32 + k]) - int(binom[b * 32 + k + 1]);
+    mid &= mid - 1;
+    ++k;
+  }
+  return d;
 }
 
 // Orbit walk of a parent state t.
@@ -115,9 +195,12 @@ __device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep,
     }
   }
   int d = L;
-  for (int i = a.n_div - 1; i >= 0; --i) {
-    int dv = a.divisors[i];
-    if (rotl_l(y, L, dv, a.mask) == t) d = dv;
+#pragma unroll
+  for (int i = 15; i >= 0; --i) {
+    if (i < a.n_div) {
+      const int dv = a.divisors[i];
+      if (rotl_l(y, L, dv, a.mask) == t) d = dv;
+    }
   }
   if (SYM == 1) {
     rep = mn;
@@ -160,21 +243,31 @@ struct Cx {
 // CMODE 0: real vectors, real coefficients
 // CMODE 1: complex vectors, real coefficients (real characters)
 // CMODE 2: complex vectors, complex coefficients/characters
-template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV, bool IMG>
 __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
   double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
   double* s_sq = s_chr + 2 * 32;                           // 65
   double* s_rsq = s_sq + 65;                               // 65
-  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_rsq + 65);
-  uint32_t* s_spl = s_meta + 4 * (a.n_diag + a.n_off);
+  double* s_crsq = s_rsq + 65;                             // 16 (image mode)
+  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_crsq + 16);
+  uint32_t* s_hop = s_meta + 4 * (a.n_diag + a.n_off);     // G (image mode)
+  uint32_t* s_bin = s_hop + (a.n_diag + a.n_off);          // 1024 (image mode)
+  uint32_t* s_spl = IMG ? s_bin + 1024 : s_hop;            // nspl (walk mode)
 
   const int G = a.n_diag + a.n_off;
   const int namp = (CMODE == 2 ? 2 : 1) * a.n_amp;
   for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
   for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
-  for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
+  if (IMG) {
+    for (int i = threadIdx.x; i < G; i += kBlock) s_hop[i] = a.hop[i];
+    if (a.up >= 0)
+      for (int i = threadIdx.x; i < 1024; i += kBlock) s_bin[i] = a.binom[i];
+    for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
+  } else {
+    for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
+  }
   if (SYM == 1 || SYM == 3)
     for (int i = threadIdx.x; i < 2 * a.L; i += kBlock) s_chr[i] = a.chars[i];
   for (int i = threadIdx.x; i < 65; i += kBlock) {
@@ -183,15 +276,20 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   }
   __syncthreads();
 
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= a.n) return;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t row = gid / uint32_t(a.lpr);
+  const int sub = int(gid % uint32_t(a.lpr));
+  if (row >= a.n) return;  // all lanes of a row leave together
   const uint32_t r = a.states[row];
   S sq_r = 1;
   if (SYM) sq_r = S(s_sq[a.orbits[row]]);
+  uint32_t rank_r = 0;
+  if (IMG && a.up >= 0) rank_r = rank_colex(r, s_bin);
 
-  // diagonal
+  // diagonal (first lane of the row)
   Cx<S> dsum{0, 0};
-  for (int g = 0; g < a.n_diag; ++g) {
+  const int ndiag = sub == 0 ? a.n_diag : 0;
+  for (int g = 0; g < ndiag; ++g) {
     const int pat = pattern_of(r, s_meta[4 * g + 1]);
     const int off = int(s_meta[4 * g + 3]) + pat;
     if (CMODE == 2) {
@@ -223,17 +321,43 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
       const int pat = pattern_of(r, s_meta[4 * g + 1]);
       act |= ((s_meta[4 * g + 2] >> pat) & 1u) << gi;
     }
+    act &= a.lane_groups << sub;
     while (act) {
       const int gi = __ffs(act) - 1;
       act &= act - 1;
       const int g = a.n_diag + w0 + gi;
       const uint32_t t = r ^ s_meta[4 * g];
       const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
-      uint32_t rep = t;
-      int orbit = 1, ci = 0, sign = 1;
-      if (SYM) walk<SYM, SHIFT>(t, a, rep, orbit, ci, sign);
-      const int idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.n);
-      if (idx < 0) continue;
+      int idx, ci = 0, sign = 1;
+      S rsq_t = 1;
+      if (IMG) {
+        uint32_t rt = t;
+        if (a.up >= 0) {
+          const uint32_t hp = s_hop[g];
+          if (hp >> 16) {
+            const int p = hp & 255, q = (hp >> 8) & 255;
+            rt = ((r >> p) & 1u) ? rank_r + uint32_t(move_delta(r, p, q, s_bin))
+                                 : rank_r - uint32_t(move_delta(t, p, q, s_bin));
+          } else {
+            if (__popc(t) != a.up) continue;  // leaves the fixed-magnetization sector
+            rt = rank_colex(t, s_bin);
+          }
+        }
+        const uint32_t e = __ldg(a.img + rt);
+        if (e == 0xFFFFFFFFu) continue;
+        idx = int(e & ((1u << a.img_bits) - 1u));
+        const int code = int((e >> a.img_bits) & 15u);
+        sign = ((e >> (a.img_bits + 4)) & 1u) ? -1 : 1;
+        ci = int((e >> (a.img_bits + 5)) & 31u);
+        if (SYM) rsq_t = S(s_crsq[code]);
+      } else {
+        uint32_t rep = t;
+        int orbit = 1;
+        if (SYM) walk<SYM, SHIFT>(t, a, rep, orbit, ci, sign);
+        idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
+        if (idx < 0) continue;
+        if (SYM) rsq_t = S(s_rsq[orbit]);
+      }
       Cx<S> c;
       if (CMODE == 2) {
         c.re = S(s_amp[2 * off]);
@@ -243,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
         c.im = 0;
       }
       S scale = S(sign);
-      if (SYM) scale *= sq_r * S(s_rsq[orbit]);
+      if (SYM) scale *= sq_r * rsq_t;
       c.re *= scale;
       c.im *= scale;
       if ((SYM == 1 || SYM == 3) && SHIFT) {
@@ -268,6 +392,14 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
       }
     }
   }
+  for (int o = 1; o < a.lpr; o <<= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      acc[v].re += __shfl_xor(acc[v].re, o, 64);
+      if (CMODE != 0) acc[v].im += __shfl_xor(acc[v].im, o, 64);
+    }
+  }
+  if (sub != 0) return;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     if (CMODE == 0) {
@@ -308,7 +440,7 @@ __device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_ne
     if (shift_needed) walk<3, true>(t, a, rep, orbit, ci, sign);
     else walk<3, false>(t, a, rep, orbit, ci, sign);
   }
-  idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.n);
+  idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
   return idx >= 0;
 }
 
@@ -501,6 +633,55 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double*
   }
 }
 
+__device__ uint32_t unrank_colex32(uint32_t rank, int L, int up, const uint32_t* binom) {
+  uint32_t s = 0;
+  int hi = L - 1;
+  for (int k = up; k >= 1; --k) {
+    int lo = k - 1, h = hi;
+    while (lo < h) {
+      const int mid = (lo + h + 1) >> 1;
+      if (binom[mid * 32 + k] <= rank) lo = mid;
+      else h = mid - 1;
+    }
+    s |= 1u << lo;
+    rank -= binom[lo * 32 + k];
+    hi = lo - 1;
+  }
+  return s;
+}
+
+// Image table: for every parent state (by colex rank for a fixed up count,
+// by value otherwise) the orbit walk + sorted-table lookup done once:
+// idx | orbit code << bits | (sign < 0) << (bits + 4) | ci << (bits + 5).
+constexpr int kImgChunk = 32;
+
+__global__ void __launch_bounds__(kBlock) image_kernel(HvArgs a, int sym, int shift_needed, uint64_t popmask,
+                                                       uint32_t n_parents, const uint8_t* orbit_code,
+                                                       uint32_t* out) {
+  const uint32_t first = (blockIdx.x * kBlock + threadIdx.x) * uint32_t(kImgChunk);
+  if (first >= n_parents) return;
+  const int todo = int(min<uint32_t>(kImgChunk, n_parents - first));
+  uint32_t s = a.up >= 0 ? unrank_colex32(first, a.L, a.up, a.binom) : first;
+  for (int i = 0; i < todo; ++i) {
+    uint32_t e = 0xFFFFFFFFu;
+    if (a.up >= 0 || ((popmask >> __popc(s)) & 1ull)) {
+      int idx, orbit, ci, sign;
+      if (target_of(a, sym, shift_needed, s, a.spl, idx, orbit, ci, sign))
+        e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) |
+            (uint32_t(sign < 0) << (a.img_bits + 4)) | (uint32_t(ci) << (a.img_bits + 5));
+    }
+    out[first + i] = e;
+    if (a.up >= 0) {
+      if (i + 1 < todo) {
+        const uint32_t low = s & (0u - s), ripple = s + low;
+        s = (((ripple ^ s) >> 2) / low) | ripple;
+      }
+    } else {
+      ++s;
+    }
+  }
+}
+
 int sym_code(const SpinBasis1D& b) {
   const SectorGeometry& g = b.geometry();
   if (g.kmom >= 0 && g.parity) return 3;
@@ -523,6 +704,12 @@ struct Operator::DeviceTables {
   int n_diag = 0, n_off = 0, n_amp = 0;
   bool complex_coef = false;  // amplitudes or characters complex
   bool hermitian = true;      // sector matrix Hermitian: gather form valid
+  int lpr = 1;                // lanes per row of the H·v kernels (autotuned)
+  // image-table mode (autotuned against the orbit-walk mode)
+  bool use_img = false;
+  DeviceArray<uint32_t> img, hop, binom;
+  DeviceArray<double> code_rsq;
+  int img_bits = 0;
   HvArgs args(const SpinBasis1D& b) const {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
@@ -537,6 +724,18 @@ struct Operator::DeviceTables {
     a.spl = b.splitters();
     a.nspl = b.split_count();
     a.sh = b.split_shift();
+    a.heads = b.heads();
+    a.nheads = uint32_t(b.heads_count());
+    a.lpr = lpr;
+    uint32_t mask = 0;
+    for (int i = 0; i < 32; i += lpr) mask |= 1u << i;
+    a.lane_groups = mask;
+    a.img = use_img ? img.data() : nullptr;
+    a.img_bits = img_bits;
+    a.up = (b.spec().up && b.spec().up_sectors.empty()) ? *b.spec().up : -1;
+    a.hop = hop.data();
+    a.binom = binom.data();
+    a.code_rsq = code_rsq.data();
     a.meta = meta.data();
     a.amp = amp.data();
     a.n_diag = n_diag;
@@ -564,6 +763,8 @@ static bool sector_matrix_hermitian(const Operator& op, hipStream_t s) {
   if (h[1]) throw unsupported_backend("a column has more distinct entries than the Hermiticity check holds");
   return h[0] == 0;
 }
+
+static void tune_lanes(Operator& op);
 
 std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   if (!basis_) throw invalid_options("operator builder has no basis");
@@ -642,24 +843,26 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
       t->hermitian = sector_matrix_hermitian(*op, nullptr);
       if (!t->hermitian) op->real_ = false;
     }
+    if (t->hermitian) tune_lanes(*op);
   }
   return op;
 }
 
 namespace {
 
-size_t hv_smem(const HvArgs& a, int cmode) {
-  return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 130) +
-         sizeof(uint32_t) * (4 * size_t(a.n_diag + a.n_off) + size_t(a.nspl));
+size_t hv_smem(const HvArgs& a, int cmode, bool img) {
+  const size_t G = size_t(a.n_diag + a.n_off);
+  return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 130 + 16) +
+         sizeof(uint32_t) * (4 * G + G + (img ? 1024 : size_t(a.nspl)));
 }
 
-template <typename S, int CMODE, int NV>
-void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
-  if (a.n == 0) return;
-  dim3 grid((a.n + kBlock - 1) / kBlock), block(kBlock);
-  size_t sm = hv_smem(a, CMODE);
-#define QMBED_HV_CASE(SYMV, SH)                                                                        \
-  hipLaunchKernelGGL((hv_kernel<S, CMODE, SYMV, SH, NV>), grid, block, sm, s, a, x, y);
+template <typename S, int CMODE, int NV, bool IMG>
+void launch_hv_mode(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
+  const size_t lanes = size_t(a.n) * size_t(a.lpr);
+  dim3 grid(unsigned((lanes + kBlock - 1) / kBlock)), block(kBlock);
+  size_t sm = hv_smem(a, CMODE, IMG);
+#define QMBED_HV_CASE(SYMV, SH) \
+  hipLaunchKernelGGL((hv_kernel<S, CMODE, SYMV, SH, NV, IMG>), grid, block, sm, s, a, x, y);
   switch (sym) {
     case 0: QMBED_HV_CASE(0, false); break;
     case 1:
@@ -682,7 +885,170 @@ void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream
   QMBED_HIP(hipGetLastError());
 }
 
+template <typename S, int CMODE, int NV>
+void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
+  if (a.n == 0) return;
+  if (a.img) launch_hv_mode<S, CMODE, NV, true>(a, sym, shift, x, y, s);
+  else launch_hv_mode<S, CMODE, NV, false>(a, sym, shift, x, y, s);
+}
+
 }  // namespace
+
+// Autotuner: lanes per row. Small sectors leave most of the 256 CUs idle
+// with one lane per row; splitting a row's flip groups over 2-8 lanes (and
+// reducing with cross-lane shuffles) restores parallelism. The FP64 kernel is
+// timed with HIP events for each candidate and the fastest is kept.
+// Build the image table of the operator's sector (one pass of the orbit walk
+// and sorted-table lookup over every parent state). Returns false when the
+// sector does not fit the packed format or the memory budget.
+static bool build_image_table(Operator& op, hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  const SpinBasisSpec& spec = b.spec();
+  const SectorGeometry& g = b.geometry();
+  auto& t = *op.tables_;
+  // host tables: binomials, hops, orbit codes
+  std::vector<uint32_t> binom(1024, 0);
+  for (int n = 0; n < 32; ++n) {
+    binom[n * 32] = 1;
+    for (int k = 1; k < 32; ++k) binom[n * 32 + k] = n == 0 ? 0 : binom[(n - 1) * 32 + k - 1] + binom[(n - 1) * 32 + k];
+  }
+  std::vector<uint32_t> hop;
+  for (auto* v : {&op.diagonal_groups(), &op.offdiagonal_groups()})
+    for (auto& grp : *v) {
+      uint32_t h = 0;
+      if (grp.flip && __builtin_popcount(grp.flip) == 2) {
+        const int p = __builtin_ctz(grp.flip), q = 31 - __builtin_clz(grp.flip);
+        // a two-site hop: amplitudes only where exactly one of p, q is up
+        bool hopping = true;
+        for (size_t pat = 0; pat < grp.amp.size(); ++pat) {
+          if (grp.amp[pat] == cplx(0, 0)) continue;
+          int bp = -1, bq = -1;
+          for (int i = 0; i < grp.nsites; ++i) {
+            if (grp.sites[i] == p) bp = int((pat >> i) & 1);
+            if (grp.sites[i] == q) bq = int((pat >> i) & 1);
+          }
+          if (bp < 0 || bq < 0 || bp == bq) hopping = false;
+        }
+        if (hopping) h = uint32_t(p) | (uint32_t(q) << 8) | (1u << 16);
+      }
+      hop.push_back(h);
+    }
+  std::vector<int> sizes;
+  for (int d = 1; d <= g.L; ++d)
+    if (g.L % d == 0) {
+      sizes.push_back(d);
+      if (g.parity) sizes.push_back(2 * d);
+    }
+  if (!b.symmetric()) sizes = {1};
+  if (g.kmom < 0 && g.parity) sizes = {1, 2};
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  if (sizes.size() > 16) return false;
+  std::vector<uint8_t> code(64, 0);
+  std::vector<double> crsq(16, 0.0);
+  for (size_t i = 0; i < sizes.size(); ++i) {
+    code[sizes[i]] = uint8_t(i);
+    crsq[i] = 1.0 / std::sqrt(double(sizes[i]));
+  }
+  int bits = 1;
+  while ((size_t(1) << bits) < b.len()) ++bits;
+  if (bits + 10 > 32) return false;
+  const bool fixed_up = spec.up.has_value() && spec.up_sectors.empty();
+  uint64_t parents = 0, popmask = ~0ull;
+  if (fixed_up) {
+    parents = 1;
+    for (int i = 1; i <= *spec.up; ++i) parents = parents * uint64_t(g.L - *spec.up + i) / uint64_t(i);
+  } else {
+    parents = uint64_t(1) << g.L;
+    if (!spec.up_sectors.empty()) {
+      popmask = 0;
+      for (int u : spec.up_sectors) popmask |= 1ull << u;
+    }
+  }
+  if (parents >= (uint64_t(1) << 32)) return false;
+  size_t free_b = 0, total_b = 0;
+  QMBED_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (parents * 4 > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
+  t.binom.resize(1024);
+  t.hop.resize(std::max<size_t>(hop.size(), 1));
+  t.code_rsq.resize(16);
+  DeviceArray<uint8_t> dcode(64);
+  QMBED_HIP(hipMemcpy(t.binom.data(), binom.data(), 4096, hipMemcpyHostToDevice));
+  if (!hop.empty()) QMBED_HIP(hipMemcpy(t.hop.data(), hop.data(), hop.size() * 4, hipMemcpyHostToDevice));
+  QMBED_HIP(hipMemcpy(t.code_rsq.data(), crsq.data(), 16 * 8, hipMemcpyHostToDevice));
+  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 64, hipMemcpyHostToDevice));
+  t.img_bits = bits;
+  t.img.resize(parents);
+  HvArgs a = t.args(b);
+  const uint64_t threads = (parents + kImgChunk - 1) / kImgChunk;
+  hipLaunchKernelGGL(image_kernel, dim3(unsigned((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, sym_code(b),
+                     g.kmom > 0 ? 1 : 0, popmask, uint32_t(parents), dcode.data(), t.img.data());
+  QMBED_HIP(hipGetLastError());
+  QMBED_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
+// Autotuner: H·v mode (orbit walk + sorted-table search, or image table) and
+// lanes per row. Small sectors leave most of the 256 CUs idle with one lane
+// per row; splitting a row's flip groups over 2-8 lanes (reduced with
+// cross-lane shuffles) restores parallelism. Each candidate FP64 kernel is
+// timed with HIP events and the fastest is kept; an unused image table is
+// released.
+static void tune_lanes(Operator& op) {
+  const SpinBasis1D& b = op.basis();
+  const size_t n = b.len();
+  if (n == 0) return;
+  hipStream_t s;
+  QMBED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  // QMBED_HV_MODE = walk | image | auto (default) restricts the candidates
+  const char* mode_env = std::getenv("QMBED_HV_MODE");
+  const std::string mode = mode_env ? mode_env : "auto";
+  const bool have_img = mode != "walk" && build_image_table(op, s);
+  DeviceArray<double> x(2 * n), y(2 * n);
+  QMBED_HIP(hipMemsetAsync(x.data(), 0, 2 * n * 8, s));
+  hipEvent_t e0, e1;
+  QMBED_HIP(hipEventCreate(&e0));
+  QMBED_HIP(hipEventCreate(&e1));
+  const int sym = sym_code(b);
+  const bool sh = b.geometry().kmom > 0;
+  float best = 1e30f;
+  int best_lpr = 1;
+  bool best_img = false;
+  for (int img = (have_img && mode == "image") ? 1 : 0; img < (have_img ? 2 : 1); ++img) {
+    for (int lpr : {1, 2, 4, 8}) {
+      op.tables_->lpr = lpr;
+      op.tables_->use_img = img != 0;
+      HvArgs a = op.tables_->args(b);
+      auto launch = [&]() {
+        if (op.is_real()) launch_hv<double, 0, 1>(a, sym, sh, x.data(), y.data(), s);
+        else launch_hv<double, 2, 1>(a, sym, sh, x.data(), y.data(), s);
+      };
+      launch();
+      QMBED_HIP(hipEventRecord(e0, s));
+      for (int it = 0; it < 3; ++it) launch();
+      QMBED_HIP(hipEventRecord(e1, s));
+      QMBED_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      QMBED_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_lpr = lpr;
+        best_img = img != 0;
+      }
+    }
+  }
+  op.tables_->lpr = best_lpr;
+  op.tables_->use_img = best_img;
+  if (!best_img) op.tables_->img.release();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  QMBED_HIP(hipStreamSynchronize(s));
+  (void)hipStreamDestroy(s);
+}
+
+bool Operator::uses_image_table() const { return tables_->use_img; }
+
+int Operator::lanes_per_row() const { return tables_->lpr; }
 
 static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_t s) {
   const SpinBasis1D& b = op.basis();

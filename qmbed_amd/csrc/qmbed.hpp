@@ -150,6 +150,10 @@ class SpinBasis1D {
   // Top of the sorted-state search tree kept in LDS by the H·v kernels:
   // splitters[i] = states[i << split_shift], padded to a power of two.
   const uint32_t* splitters() const { return splitters_.data(); }
+  // line heads: heads[i] = states[16 i]; states and heads are padded with
+  // 0xFFFFFFFF to whole 64-byte lines so a lookup reads aligned lines
+  const uint32_t* heads() const { return heads_.data(); }
+  size_t heads_count() const { return heads_.size(); }
   int split_count() const { return split_count_; }   // padded (power of two)
   int split_shift() const { return split_shift_; }
 
@@ -160,6 +164,7 @@ class SpinBasis1D {
   DeviceArray<uint32_t> states_;
   DeviceArray<uint8_t> orbits_;
   DeviceArray<uint32_t> splitters_;
+  DeviceArray<uint32_t> heads_;
   int split_count_ = 0;
   int split_shift_ = 0;
 };
@@ -223,6 +228,9 @@ class Operator {
   // number of entries the reference's assembly would store (merged per
   // (row, column), |v| > f64::EPSILON, diagonal included)
   size_t stored_entries(hipStream_t s) const;
+  // lanes per row and H·v mode chosen by the autotuner
+  int lanes_per_row() const;
+  bool uses_image_table() const;
 
   // dense materialization (row-major, complex) on the device
   void materialize_dense(cplx* dense, hipStream_t s) const;

@@ -53,11 +53,13 @@ def _ham_ok(name, up):
     return not (name == "xyfield" and up is not None)
 
 
+@pytest.mark.parametrize("mode", ["walk", "image"])
 @pytest.mark.parametrize("name", sorted(HAMS))
 @pytest.mark.parametrize("L,up,k,p", cases.SECTORS[:18])
-def test_hv_fp64(qa, oracle, name, L, up, k, p):
+def test_hv_fp64(qa, oracle, monkeypatch, name, L, up, k, p, mode):
     if not _ham_ok(name, up):
         pytest.skip("term set does not conserve the sector")
+    monkeypatch.setenv("QMBED_HV_MODE", mode)
     spec = HAMS[name](L)
     print(f"[hv] {name} L={L} up={up} k={k} p={p}", flush=True)
     try:
