@@ -75,8 +75,8 @@ def main():
     ap.add_argument("--no-extended", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (every sector and scan point is "
-              "independent; sharding is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (sectors, scan points and right-hand sides "
+              "are independent units that shard without communication). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import qmbed_amd as qa
