@@ -221,6 +221,11 @@ SolverReq parse_solver(const Value& s) {
     else bad("matvec_precision must be \"fp64\" or \"fp32\"");
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
+  if (auto* v = s.get("reorthogonalization_eta"); v && !v->is_null()) {
+    const double eta = as_double(*v, "reorthogonalization_eta");
+    if (!(eta > 0.0 && eta <= 1.0)) bad("reorthogonalization_eta must lie in (0, 1]");
+    r.ctl.dgks_eta = eta;
+  }
   return r;
 }
 

@@ -92,11 +92,20 @@ struct Candidate {
   int column;  // column in the Ritz buffer
 };
 
-// One Lanczos window on the device.
+// One Lanczos window on the device. A step is four launches and no host
+// round trip: H·v_j, the per-row-block overlaps of w with V (and ‖w‖²), the
+// subtraction w −= V h, and a finishing kernel that takes α_j, decides the
+// DGKS second pass from the overlaps and normalizes w into v_{j+1}. Steps are
+// launched speculatively on the assumption that no second pass is needed
+// (the usual case); when one is, the finishing kernel raises a device halt
+// flag that turns every later launch into a no-op, and the host, at its next
+// read-back, runs the second pass for that step and relaunches from there.
+// The host reads coefficients back only at these read-backs (`sync`), where
+// breakdown (β ≤ 1e-14) truncates the window to the steps before it.
 class LanczosWindow {
  public:
-  LanczosWindow(const Operator& op, bool complex, int m, bool fp32, hipStream_t s)
-      : op_(op), c_(complex), n_(op.dim()), m_(m), fp32_(fp32), s_(s) {
+  LanczosWindow(const Operator& op, bool complex, int m, bool fp32, hipStream_t s, double eta = DGKS)
+      : op_(op), c_(complex), n_(op.dim()), m_(m), fp32_(fp32), eta_(eta), s_(s) {
     const size_t esz = c_ ? 2 : 1;
     V_.resize(esz * n_ * size_t(m_));
     w_.resize(esz * n_);
@@ -104,10 +113,13 @@ class LanczosWindow {
       x32_.resize(esz * n_);
       y32_.resize(esz * n_);
     }
-    // slots: [0,m) alpha, [m,2m) beta, 2m: before, 2m+1: after, 2m+2..: h (2*m doubles)
-    slots_.resize(2 * size_t(m_) + 8 + 2 * size_t(m_) + 64);
+    // slots: [0,m) alpha, [m,2m) beta, then h (2(m+1) doubles), then scratch
+    slots_.resize(2 * size_t(m_) + 8 + 2 * size_t(m_ + 1) + 64);
     h_ = slots_.data() + 2 * m_ + 8;
-    scratch_ = h_ + 2 * m_;
+    scratch_ = h_ + 2 * (m_ + 1);
+    // flags: [0,m) per-step second-pass record, m: halt
+    flags_.resize(size_t(m_) + 1);
+    QMBED_HIP(hipMemsetAsync(flags_.data(), 0, (size_t(m_) + 1) * sizeof(int), s_));
   }
 
   size_t esz() const { return c_ ? 2 : 1; }
@@ -119,63 +131,98 @@ class LanczosWindow {
     if (fp32_) lk::to_f32(c_, col(0), n_, x32_.data(), s_);
     alpha_.clear();
     beta_.clear();
-    size_ = 1;
+    launched_ = 0;
+    size_ = 0;
   }
 
-  void matvec(int j) {
+  void matvec(int j, const int* gate) {
     if (fp32_) {
       if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32_.data()),
-                        reinterpret_cast<std::complex<float>*>(y32_.data()), s_);
-      else op_.apply(x32_.data(), y32_.data(), s_);
-      lk::from_f32(c_, y32_.data(), n_, w_.data(), s_);
+                        reinterpret_cast<std::complex<float>*>(y32_.data()), s_, gate);
+      else op_.apply(x32_.data(), y32_.data(), s_, gate);
+      lk::from_f32(c_, y32_.data(), n_, w_.data(), s_, gate);
     } else {
-      if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w_.data()), s_);
-      else op_.apply(col(j), w_.data(), s_);
+      if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w_.data()), s_, gate);
+      else op_.apply(col(j), w_.data(), s_, gate);
     }
     ++matvecs_;
   }
 
-  // Advance one step from column j (= size_-1). Returns false when the
-  // window is exhausted or broke down (β <= 1e-14).
-  bool step() {
-    const int j = size_ - 1;
-    double* a_slot = slots_.data() + j;
-    double* b_prev = j > 0 ? slots_.data() + m_ + (j - 1) : nullptr;
-    double* before = slots_.data() + 2 * m_;
-    double* after = before + 1;
-    matvec(j);
-    lk::redot_to_slot(c_, col(j), w_.data(), n_, a_slot, ws_, s_);
-    lk::three_term(c_, w_.data(), col(j), j > 0 ? col(j - 1) : col(j), a_slot, b_prev, n_, before, ws_, s_);
-    lk::project_out(c_, V_.data(), n_, j + 1, w_.data(), h_, after, ws_, s_);
-    double host[2];
-    QMBED_HIP(hipMemcpyAsync(host, before, 2 * sizeof(double), hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipMemcpyAsync(&alpha_host_, a_slot, sizeof(double), hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipStreamSynchronize(s_));
-    double nb = std::sqrt(host[0]), na = std::sqrt(host[1]);
-    bool second = false;
-    if (nb > F64_EPS && na <= DGKS * nb) {
-      lk::project_out(c_, V_.data(), n_, j + 1, w_.data(), h_, after, ws_, s_);
-      QMBED_HIP(hipMemcpyAsync(host + 1, after, sizeof(double), hipMemcpyDeviceToHost, s_));
-      QMBED_HIP(hipStreamSynchronize(s_));
-      na = std::sqrt(host[1]);
-      second = true;
-    }
-    if (!std::isfinite(alpha_host_) || !std::isfinite(na) || !std::isfinite(nb))
-      throw internal_state("non-finite Lanczos coefficient at step " + std::to_string(j));
-    passes_ += 1 + (second ? 1 : 0);
-    seconds_ += second ? 1 : 0;
-    alpha_.push_back(alpha_host_);
-    last_beta_ = na;
-    if (j + 1 == m_ || na <= 1.0e-14) {
-      broke_ = na <= 1.0e-14;
-      return false;
-    }
-    beta_.push_back(na);
-    lk::normalize(c_, w_.data(), after, n_, col(j + 1), fp32_ ? x32_.data() : nullptr, slots_.data() + m_ + j, s_);
-    ++size_;
-    return true;
+  int* halt() const { return flags_.data() + m_; }
+
+  // Launch step j = launched() (requires launched() < m).
+  void step() {
+    const int j = launched_;
+    const int* gate = halt();
+    matvec(j, gate);
+    const lk::ThreeTerm three{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots_.data() + m_ + (j - 1) : nullptr,
+                              slots_.data() + j};
+    lk::lanczos_alpha(c_, col(j), w_.data(), n_, ws_, s_, gate);
+    lk::lanczos_overlaps(c_, V_.data(), n_, j + 1, w_.data(), ws_, s_, true, gate, &three);
+    lk::lanczos_subtract(c_, V_.data(), n_, j + 1, w_.data(), h_, ws_, s_, true, gate, &three);
+    finish(j, false);
+    ++launched_;
   }
 
+  // Read back the coefficients of the launched steps. A pending second pass
+  // (halt flag set at step h) is run here and the window rewinds to h + 1
+  // (the voided steps are relaunched by the caller). Otherwise `ended` is
+  // true when the window is full or broke down (then truncated to the step
+  // whose β fell to 1e-14 or below, as the reference stops there).
+  struct Sync {
+    bool ended = false;
+    bool rewound = false;
+  };
+  Sync sync() {
+    const int L = launched_;
+    std::vector<double> hb(2 * size_t(m_));
+    std::vector<int> hf(static_cast<size_t>(m_) + 1);
+    QMBED_HIP(hipMemcpyAsync(hb.data(), slots_.data(), 2 * size_t(m_) * 8, hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipMemcpyAsync(hf.data(), flags_.data(), (size_t(m_) + 1) * sizeof(int), hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipStreamSynchronize(s_));
+    const int halted_at = hf[size_t(m_)] - 1;  // -1: none
+    if (halted_at >= L) throw internal_state("Lanczos halt flag beyond the launched steps");
+    const int valid = halted_at >= 0 ? halted_at : L;  // steps with final α and β
+    alpha_.clear();
+    beta_.clear();
+    Sync r;
+    for (int j = 0; j < valid; ++j) {
+      const double a = hb[size_t(j)], b = hb[size_t(m_ + j)];
+      if (!std::isfinite(a) || !std::isfinite(b))
+        throw internal_state("non-finite Lanczos coefficient at step " + std::to_string(j));
+      if (b <= 1.0e-14) {
+        alpha_.push_back(a);
+        last_beta_ = b;
+        broke_ = true;
+        size_ = j + 1;
+        count_passes(hf, size_);
+        r.ended = true;
+        return r;
+      }
+    }
+    if (halted_at >= 0) {
+      // second pass for step h, then resume from h + 1
+      const int h = halted_at;
+      lk::lanczos_overlaps(c_, V_.data(), n_, h + 1, w_.data(), ws_, s_, false, nullptr, nullptr);
+      lk::lanczos_subtract(c_, V_.data(), n_, h + 1, w_.data(), h_, ws_, s_, false, nullptr, nullptr);
+      finish(h, true);
+      launched_ = h + 1;
+      r.rewound = true;
+      return r;
+    }
+    for (int j = 0; j < L; ++j) {
+      alpha_.push_back(hb[size_t(j)]);
+      if (j + 1 < L) beta_.push_back(hb[size_t(m_ + j)]);
+    }
+    last_beta_ = hb[size_t(m_ + L - 1)];
+    size_ = L;
+    count_passes(hf, L);
+    r.ended = L == m_;
+    return r;
+  }
+
+  int launched() const { return launched_; }
+  // completed steps known on the host (= columns spanned by T)
   int size() const { return size_; }
   const std::vector<double>& alpha() const { return alpha_; }
   const std::vector<double>& beta() const { return beta_; }
@@ -248,19 +295,36 @@ class LanczosWindow {
   long matvecs64() const { return matvecs64_; }
 
  private:
+  void finish(int j, bool second_pass) {
+    const bool next = j + 1 < m_;
+    lk::lanczos_finish(c_, w_.data(), n_, h_, j, eta_, slots_.data() + m_ + j, halt(), flags_.data() + j,
+                       next ? col(j + 1) : nullptr, (next && fp32_) ? x32_.data() : nullptr, ws_, s_, second_pass);
+  }
+  void count_passes(const std::vector<int>& hf, int steps) {
+    passes_ = 0;
+    seconds_ = 0;
+    for (int j = 0; j < steps; ++j) {
+      passes_ += 1 + (hf[size_t(j)] ? 1 : 0);
+      seconds_ += hf[size_t(j)] ? 1 : 0;
+    }
+  }
+
   const Operator& op_;
   bool c_;
   size_t n_;
   int m_;
   bool fp32_;
+  double eta_;
   hipStream_t s_;
   DeviceArray<double> V_, w_, slots_;
   DeviceArray<float> x32_, y32_;
+  DeviceArray<int> flags_;  // [0,m) per-step second-pass record, m: halt (step + 1, 0 = none)
   double* h_ = nullptr;
   double* scratch_ = nullptr;
   lk::Workspace ws_;
   std::vector<double> alpha_, beta_;
-  double alpha_host_ = 0, last_beta_ = 0;
+  double last_beta_ = 0;
+  int launched_ = 0;
   int size_ = 0;
   long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0;
   bool broke_ = false;
@@ -279,16 +343,32 @@ struct WindowResult {
 // does not converge.
 void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
                 const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s) {
-  LanczosWindow win(op, c, m, fp32, s);
+  LanczosWindow win(op, c, m, fp32, s, ctl.dgks_eta);
   win.start_from(d_start);
   const int k = o.eigenpairs;
   int next_check = std::max(k + 1, ctl.check_every);
+  // read-backs: at convergence checks, at the window's end, and otherwise
+  // every `interval` steps (halved after a rewind, so frequent second passes
+  // void few speculative steps)
+  int interval = ctl.check_every;
+  int next_sync = interval;
   for (;;) {
-    bool more = win.step();
+    if (win.launched() < m) win.step();
+    const int launched = win.launched();
+    const bool full = launched == m;
+    const bool conv = ctl.early_stop && launched >= next_check && launched > k;
+    if (!(full || conv || launched >= next_sync)) continue;
+    const auto st = win.sync();
+    if (st.rewound) {
+      interval = std::max(1, interval / 2);
+      next_sync = win.launched() + interval;
+      continue;
+    }
+    interval = std::min(ctl.check_every, interval * 2);
+    next_sync = launched + interval;
+    const bool at_end = st.ended;
+    if (!(at_end || conv)) continue;
     const int sz = win.size();
-    bool at_end = !more;
-    bool check = at_end || (ctl.early_stop && sz >= next_check && sz > k);
-    if (!check) continue;
     if (!at_end) {
       if (!win.estimates_converged(k, o.target, o.shift, accept_tol, 0.5)) {
         next_check = sz + ctl.check_every;

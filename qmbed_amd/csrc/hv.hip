@@ -65,6 +65,9 @@ struct HvArgs {
   // batched columns
   int nvec;
   size_t ld;
+  // optional device flag: a nonzero value makes the launch a no-op (a
+  // speculative Lanczos step voided by a pending reorthogonalization)
+  const int* gate;
 };
 
 __device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t mask) {
@@ -245,6 +248,7 @@ struct Cx {
 // CMODE 2: complex vectors, complex coefficients/characters
 template <typename S, int CMODE, int SYM, bool SHIFT, int NV, bool IMG>
 __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
+  if (a.gate && *a.gate) return;
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
   double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
@@ -744,6 +748,7 @@ struct Operator::DeviceTables {
     a.chars = chars.data();
     a.nvec = 1;
     a.ld = b.len();
+    a.gate = nullptr;
     return a;
   }
 };
@@ -1061,34 +1066,38 @@ static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_
   QMBED_HIP(hipGetLastError());
 }
 
-void Operator::apply(const double* x, double* y, hipStream_t s) const {
+void Operator::apply(const double* x, double* y, hipStream_t s, const int* gate) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
   HvArgs a = tables_->args(*basis_);
+  a.gate = gate;
   launch_hv<double, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
 }
 
-void Operator::apply(const float* x, float* y, hipStream_t s) const {
+void Operator::apply(const float* x, float* y, hipStream_t s, const int* gate) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
   HvArgs a = tables_->args(*basis_);
+  a.gate = gate;
   launch_hv<float, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
 }
 
-void Operator::apply(const cplx* x, cplx* y, hipStream_t s) const {
+void Operator::apply(const cplx* x, cplx* y, hipStream_t s, const int* gate) const {
   if (!tables_->hermitian) {
     scatter_apply(*this, x, y, s);
     return;
   }
   HvArgs a = tables_->args(*basis_);
+  a.gate = gate;
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
   if (real_) launch_hv<double, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
   else launch_hv<double, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
 }
 
-void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s) const {
+void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s, const int* gate) const {
   if (!tables_->hermitian)
     throw unsupported_backend("single-precision action of a non-Hermitian sector matrix is not provided");
   HvArgs a = tables_->args(*basis_);
+  a.gate = gate;
   const float* xf = reinterpret_cast<const float*>(x);
   float* yf = reinterpret_cast<float*>(y);
   if (real_) launch_hv<float, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);

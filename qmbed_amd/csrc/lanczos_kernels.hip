@@ -1,7 +1,13 @@
 // Device-scalar Lanczos building blocks for real (double) and complex
 // (double2) Krylov vectors. Scalars (α_j, β_j, norms, overlap vectors) stay in
-// device slots so an iteration needs no host round trip except the DGKS
-// decision; every pass streams whole columns of V (column-major, ld = n).
+// device slots, so a Lanczos step runs without any host round trip; every
+// pass streams whole columns of V (column-major, ld = n).
+//
+// Grid-wide sums are finished by the consuming kernel: the producer writes
+// one partial per block and every block of the consumer adds them again in
+// the same fixed order, so all blocks hold bit-identical totals and no
+// in-kernel fence between blocks is needed (an agent-scope release writes back
+// the whole L2 of an XCD, which costs more than the redundant adds from L2).
 #include <algorithm>
 
 #include "internal.hpp"
@@ -14,8 +20,8 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kMaxBlocks = 1024;
-constexpr int kRowsT = 2048;
-constexpr int kTile = 8;
+constexpr int kRPT = 2;  // rows per thread in the overlap kernel
+constexpr int kMaxCols = 512;
 
 __device__ __forceinline__ double re_of(double a) { return a; }
 __device__ __forceinline__ double re_of(double2 a) { return a.x; }
@@ -52,6 +58,7 @@ __device__ __forceinline__ double2 shfl_down(double2 v, int o) {
   return make_double2(__shfl_down(v.x, o, 64), __shfl_down(v.y, o, 64));
 }
 
+// block-wide sum; the result is valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -64,63 +71,141 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// Σ partial[0..nb) in a fixed order; valid in thread 0
+__device__ __forceinline__ double sum_partials(const double* partial, int nb, double* sh) {
+  double t = 0;
+  for (int b = threadIdx.x; b < nb; b += kT) t += partial[b];
+  return block_sum(t, sh);
+}
+
 int nblocks(size_t n) { return int(std::min<size_t>(kMaxBlocks, std::max<size_t>(1, (n + kT * 4 - 1) / (kT * 4)))); }
 
-// partial = Re Σ conj(x) y
+// partial[block] = Re Σ conj(x) y over the block's rows
 template <typename E>
-__global__ void k_redot(const E* __restrict__ x, const E* __restrict__ y, size_t n, double* partial) {
-  __shared__ double sh[4];
+__global__ __launch_bounds__(kT) void k_redot(const E* __restrict__ x, const E* __restrict__ y, size_t n,
+                                              double* partial) {
+  __shared__ double sh[kT / 64];
   double s = 0;
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) s += re_of(cmul(x[i], y[i]));
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-__global__ void k_sum(const double* partial, int nb, double* out, int mode) {
-  if (threadIdx.x != 0) return;
-  double s = 0;
-  for (int b = 0; b < nb; ++b) s += partial[b];
-  out[0] = mode ? sqrt(s) : s;
-}
-
-// w -= a v + b u  (a, b real device scalars), partial |w|^2
+// partial[block] = Σ |y − λ x|² over the block's rows (λ from a device slot)
 template <typename E>
-__global__ void k_three(E* w, const E* __restrict__ v, const E* __restrict__ u, const double* a_slot,
-                        const double* b_slot, size_t n, double* partial) {
-  __shared__ double sh[4];
-  const double a = *a_slot;
-  const double b = b_slot ? *b_slot : 0.0;
+__global__ __launch_bounds__(kT) void k_resid(const E* __restrict__ y, const E* __restrict__ x, const double* lam_slot,
+                                              size_t n, double* partial) {
+  __shared__ double sh[kT / 64];
+  const double lam = *lam_slot;
   double s = 0;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    E x = sub(w[i], scale(v[i], a));
-    if (b_slot) x = sub(x, scale(u[i], b));
-    w[i] = x;
-    s += sq(x);
-  }
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT)
+    s += sq(sub(y[i], scale(x[i], lam)));
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// partial[rb*m + c] = Σ_rows conj(V_c) w
+// slot = Σ partial (its square root when root), one block
+__global__ __launch_bounds__(kT) void k_sum(const double* partial, int nb, double* slot, int root) {
+  __shared__ double sh[kT / 64];
+  const double t = sum_partials(partial, nb, sh);
+  if (threadIdx.x == 0) *slot = root ? sqrt(t) : t;
+}
+
+// partial[block] = Re Σ conj(v_j) w over the block's rows (α_j of a step)
 template <typename E>
-__global__ void k_gemvt(const E* __restrict__ V, size_t n, int m, const E* __restrict__ w, E* partial) {
-  __shared__ E sh[kTile][kT / 64];
-  const size_t r0 = size_t(blockIdx.x) * kRowsT;
-  const int c0 = blockIdx.y * kTile;
-  const int nc = min(kTile, m - c0);
-  E acc[kTile];
+__global__ __launch_bounds__(kT) void k_alpha(const E* __restrict__ vj, const E* __restrict__ w, size_t n,
+                                              double* partial, const int* gate) {
+  if (gate && *gate) return;
+  __shared__ double sh[kT / 64];
+  double s = 0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT)
+    s += re_of(cmul(vj[i], w[i]));
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// The three-term part of a first pass (QMBED src/solve/mod.rs:1040-1048):
+// w₀ = w − α_j v_j − β_{j−1} v_{j−1}, α_j = Σ pa (the k_alpha partials, added
+// by every block in the same order). vj == nullptr: no three-term part.
+template <typename E>
+struct Three {
+  const E* vj = nullptr;
+  const E* vjm1 = nullptr;           // null at j = 0
+  const double* beta_prev = nullptr;  // β_{j−1}
+  const double* pa = nullptr;
+  int nba = 0;
+  double* alpha = nullptr;  // α_j slot (written by block 0 of the subtract)
+};
+
+// α_j and β_{j−1} into (alpha, beta) in every thread of the block
+template <typename E>
+__device__ __forceinline__ void three_coefs(const Three<E>& t, double* sh, double* bc, double& alpha, double& beta) {
+  const double a = sum_partials(t.pa, t.nba, sh);
+  if (threadIdx.x == 0) *bc = a;
+  __syncthreads();
+  alpha = *bc;
+  beta = t.vjm1 ? *t.beta_prev : 0.0;
+}
+
+template <typename E>
+__device__ __forceinline__ E three_apply(const Three<E>& t, E x, size_t i, double alpha, double beta) {
+  x = sub(x, scale(t.vj[i], alpha));
+  if (t.vjm1) x = sub(x, scale(t.vjm1[i], beta));
+  return x;
+}
+
+// Overlaps of one classical Gram–Schmidt pass, per row block: for columns
+// c < m, Σ_i conj(V[c][i]) w₀[i], and when mm == m + 1 also Σ_i |w₀[i]|² at
+// c = m (‖w₀‖, the DGKS reference norm). w₀ is w after the three-term part
+// (first pass) or w itself. Block (x, y) takes the row tiles x, x + gridDim.x,
+// … of kT·kRPT rows and the columns [y·CT, y·CT + CT); every load of a tile
+// is unconditional (row and column indices are clamped, out-of-range rows
+// carry w₀ = 0), so the CT·kRPT loads of a thread are in flight together.
+// Output: partial[x·ld + col_off + c].
+template <typename E, int CT>
+__global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, int m, int mm, const E* __restrict__ w,
+                                             E* __restrict__ partial, int ld, int col_off, const int* gate,
+                                             Three<E> t) {
+  if (gate && *gate) return;
+  __shared__ E sh[CT][kT / 64];
+  __shared__ double shd[kT / 64];
+  __shared__ double bc;
+  double alpha = 0, beta = 0;
+  if (t.vj) three_coefs(t, shd, &bc, alpha, beta);
+  const int c0 = blockIdx.y * CT;
+  const int nc = min(CT, mm - c0);
+  const size_t tiles = (n + size_t(kT * kRPT) - 1) / size_t(kT * kRPT);
+  E acc[CT];
 #pragma unroll
-  for (int c = 0; c < kTile; ++c) acc[c] = zero<E>();
-  const size_t r1 = min(n, r0 + kRowsT);
-  for (size_t i = r0 + threadIdx.x; i < r1; i += kT) {
-    const E wi = w[i];
+  for (int c = 0; c < CT; ++c) acc[c] = zero<E>();
+  for (size_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+    const size_t r0 = tt * (kT * kRPT) + threadIdx.x;
+    size_t ic[kRPT];
+    E wv[kRPT];
 #pragma unroll
-    for (int c = 0; c < kTile; ++c)
-      if (c < nc) acc[c] = add(acc[c], cmul(V[size_t(c0 + c) * n + i], wi));
+    for (int rr = 0; rr < kRPT; ++rr) {
+      const size_t i = r0 + size_t(rr) * kT;
+      ic[rr] = i < n ? i : n - 1;
+      wv[rr] = w[ic[rr]];
+      if (t.vj) wv[rr] = three_apply(t, wv[rr], ic[rr], alpha, beta);
+      if (i >= n) wv[rr] = zero<E>();
+    }
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = c0 + c;
+      const bool is_w = col >= m;
+      const size_t base = size_t(min(col, m - 1)) * n;
+#pragma unroll
+      for (int rr = 0; rr < kRPT; ++rr) {
+        E v = V[base + ic[rr]];
+        if (is_w) v = wv[rr];
+        acc[c] = add(acc[c], cmul(v, wv[rr]));
+      }
+    }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < kTile; ++c) {
+  for (int c = 0; c < CT; ++c) {
     E v = acc[c];
     for (int o = 32; o > 0; o >>= 1) v = add(v, shfl_down(v, o));
     if (lane == 0) sh[c][wid] = v;
@@ -129,53 +214,124 @@ __global__ void k_gemvt(const E* __restrict__ V, size_t n, int m, const E* __res
   if (threadIdx.x < nc) {
     E s = zero<E>();
     for (int w2 = 0; w2 < kT / 64; ++w2) s = add(s, sh[threadIdx.x][w2]);
-    partial[size_t(blockIdx.x) * m + c0 + threadIdx.x] = s;
+    partial[size_t(blockIdx.x) * ld + col_off + c0 + threadIdx.x] = s;
   }
 }
 
+// h[c] = Σ_b partial[b·ld + c] over the rb row blocks of the overlap kernel,
+// one wave per column (lane-strided, then a shuffle tree: a fixed order).
 template <typename E>
-__global__ void k_reduce_cols(const E* partial, int nb, int m, E* out) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m) return;
+__global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int rb, int ld, int ncols, E* h,
+                                             const int* gate) {
+  if (gate && *gate) return;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
+  if (c >= ncols) return;
   E s = zero<E>();
-  for (int b = 0; b < nb; ++b) s = add(s, partial[size_t(b) * m + c]);
-  out[c] = s;
+  for (int b = lane; b < rb; b += 64) s = add(s, partial[size_t(b) * ld + c]);
+  for (int o = 32; o > 0; o >>= 1) s = add(s, shfl_down(s, o));
+  if (lane == 0) h[c] = s;
 }
 
-constexpr int kMaxCols = 512;
-
-// w -= V h (columns [0, m)), partial |w|^2 ; h in device memory
+// One column chunk of the subtraction w ← w₀ − V h (hc: the chunk's
+// overlaps). The three-term part is applied by the first chunk only, whose
+// block 0 also stores α_j. With partial2 (last chunk) the per-block ‖w‖²
+// partials of the result follow.
 template <typename E>
-__global__ void k_gemvn_sub(const E* __restrict__ V, size_t n, int m, const E* __restrict__ h, E* w, double* partial) {
+__global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, int mc, const E* __restrict__ hc,
+                                            E* __restrict__ w, double* __restrict__ partial2, const int* gate,
+                                            Three<E> t) {
+  if (gate && *gate) return;
   __shared__ E sh_h[kMaxCols];
-  __shared__ double sh[4];
-  for (int c = threadIdx.x; c < m; c += kT) sh_h[c] = h[c];
+  __shared__ double sh[kT / 64];
+  __shared__ double bc;
+  double alpha = 0, beta = 0;
+  if (t.vj) {
+    three_coefs(t, sh, &bc, alpha, beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *t.alpha = alpha;
+  }
+  for (int c = threadIdx.x; c < mc; c += kT) sh_h[c] = hc[c];
   __syncthreads();
   double s = 0;
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     E acc = zero<E>();
     int c = 0;
-    for (; c + 4 <= m; c += 4) {
+    for (; c + 4 <= mc; c += 4) {
       acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
       acc = add(acc, mul(V[size_t(c + 1) * n + i], sh_h[c + 1]));
       acc = add(acc, mul(V[size_t(c + 2) * n + i], sh_h[c + 2]));
       acc = add(acc, mul(V[size_t(c + 3) * n + i], sh_h[c + 3]));
     }
-    for (; c < m; ++c) acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
-    E v = sub(w[i], acc);
+    for (; c < mc; ++c) acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
+    E x = w[i];
+    if (t.vj) x = three_apply(t, x, i, alpha, beta);
+    const E v = sub(x, acc);
     w[i] = v;
     s += sq(v);
   }
+  if (!partial2) return;
   s = block_sum(s, sh);
-  if (threadIdx.x == 0 && partial) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial2[blockIdx.x] = s;
 }
 
-// out = w / sqrt(*nrm2); beta_slot (optional) = sqrt(*nrm2); optional FP32 copy
+// End of a Lanczos step. Every block forms ‖w‖² = Σ partial2 (fixed order).
+// First pass (pass2 == 0): the DGKS decision — a second pass iff
+// ‖w‖ <= η‖w₀‖ with ‖w₀‖ > eps (QMBED src/solve/mod.rs:1050-1073, η = 1/√2
+// there), ‖w₀‖² = h[j+1] from the overlaps. A needed second pass sets
+// *halt = j + 1 and leaves w as it is; every later gated launch is then a
+// no-op until the host has run the second pass. Otherwise (and after a second
+// pass, pass2 == 1, which clears *halt) β_j = ‖w‖ and out = w/β_j (+ FP32
+// copy); out == nullptr writes β_j only. A first pass is skipped while *halt
+// is set.
 template <typename E, typename F>
-__global__ void k_normalize(const E* w, const double* nrm2, size_t n, E* out, F* out32, double* beta_slot) {
+__global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n, const double* __restrict__ partial2,
+                                               int nb2, const E* __restrict__ h, int j, double eta, double* beta,
+                                               int* halt, int* hist, E* out, F* out32, int pass2) {
+  if (!pass2 && *halt) return;
+  __shared__ double sh[kT / 64];
+  __shared__ double s_after;
+  const double t = sum_partials(partial2, nb2, sh);
+  if (threadIdx.x == 0) s_after = t;
+  __syncthreads();
+  const double after2 = s_after;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!pass2) {
+    const double nb = sqrt(re_of(h[j + 1])), na = sqrt(after2);
+    const bool second = nb > 2.220446049250313e-16 && na <= eta * nb;
+    if (lead) {
+      *hist = second ? 1 : 0;
+      if (second) *halt = j + 1;
+    }
+    if (second) return;
+  } else if (lead) {
+    *halt = 0;
+  }
+  const double b = sqrt(after2);
+  if (lead) *beta = b;
+  if (!out) return;
+  const double inv = 1.0 / b;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    const E v = scale(w[i], inv);
+    out[i] = v;
+    if (out32) {
+      if constexpr (sizeof(E) == 8) out32[i] = float(re_of(v));
+      else {
+        out32[2 * i] = float(v.x);
+        out32[2 * i + 1] = float(v.y);
+      }
+    }
+  }
+}
+
+// out = w / sqrt(*nrm2); beta_slot (optional) = sqrt(*nrm2); optional FP32
+// copy; out == nullptr writes beta_slot only
+template <typename E, typename F>
+__global__ __launch_bounds__(kT) void k_normalize(const E* w, const double* nrm2, size_t n, E* out, F* out32,
+                                                  double* beta_slot) {
   const double b = sqrt(*nrm2);
   const double inv = 1.0 / b;
   if (beta_slot && blockIdx.x == 0 && threadIdx.x == 0) *beta_slot = b;
+  if (!out) return;
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     E v = scale(w[i], inv);
     out[i] = v;
@@ -191,7 +347,8 @@ __global__ void k_normalize(const E* w, const double* nrm2, size_t n, E* out, F*
 
 // Y[:, q] = V C[:, q], q < K; C column-major m×K with element type E
 template <typename E, int K>
-__global__ void k_gemm_vc(const E* __restrict__ V, size_t n, int m, const E* __restrict__ C, E* Y) {
+__global__ __launch_bounds__(kT) void k_gemm_vc(const E* __restrict__ V, size_t n, int m, const E* __restrict__ C,
+                                                E* Y) {
   extern __shared__ double shc_raw[];
   E* shc = reinterpret_cast<E*>(shc_raw);
   for (int i = threadIdx.x; i < m * K; i += kT) shc[i] = C[i];
@@ -211,7 +368,7 @@ __global__ void k_gemm_vc(const E* __restrict__ V, size_t n, int m, const E* __r
 }
 
 template <typename E>
-__global__ void k_to32(const E* x, size_t n, float* y) {
+__global__ __launch_bounds__(kT) void k_to32(const E* x, size_t n, float* y) {
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     if constexpr (sizeof(E) == 8) y[i] = float(re_of(x[i]));
     else {
@@ -221,24 +378,12 @@ __global__ void k_to32(const E* x, size_t n, float* y) {
   }
 }
 template <typename E>
-__global__ void k_from32(const float* y, size_t n, E* x) {
+__global__ __launch_bounds__(kT) void k_from32(const float* y, size_t n, E* x, const int* gate) {
+  if (gate && *gate) return;
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     if constexpr (sizeof(E) == 8) x[i] = double(y[i]);
     else x[i] = make_double2(double(y[2 * i]), double(y[2 * i + 1]));
   }
-}
-
-// r = y - lam * x ; partial |r|^2 (lam from device slot)
-template <typename E>
-__global__ void k_resid(const E* __restrict__ y, const E* __restrict__ x, const double* lam_slot, size_t n,
-                        double* partial) {
-  __shared__ double sh[4];
-  const double lam = *lam_slot;
-  double s = 0;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT)
-    s += sq(sub(y[i], scale(x[i], lam)));
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // deterministic_start (QMBED src/solve/mod.rs:729-742) with LCG jump-ahead:
@@ -263,9 +408,15 @@ __global__ void k_det_start(double* out, size_t n, uint64_t seed) {
   }
 }
 
+// column tile of the overlap kernel: 16 real or 8 complex columns
+template <typename E>
+constexpr int dots_tile() {
+  return sizeof(E) == 8 ? 16 : 8;
+}
+
 }  // namespace
 
-Workspace::Workspace() : partial(64 * 1024), slots(4096) {}
+Workspace::Workspace() : partial(64 * 1024), partial2(kMaxBlocks), partial3(kMaxBlocks), slots(4096) {}
 
 template <typename E>
 static const E* cE(const void* p) {
@@ -276,65 +427,27 @@ static E* mE(void* p) {
   return static_cast<E*>(p);
 }
 
-#define QMBED_DISPATCH(cplx_flag, ...)                                            \
-  do {                                                                             \
-    if (cplx_flag) {                                                               \
-      using E = double2;                                                           \
-      __VA_ARGS__;                                                                 \
-    } else {                                                                       \
-      using E = double;                                                            \
-      __VA_ARGS__;                                                                 \
-    }                                                                              \
+#define QMBED_DISPATCH(cplx_flag, ...) \
+  do {                                 \
+    if (cplx_flag) {                   \
+      using E = double2;               \
+      __VA_ARGS__;                     \
+    } else {                           \
+      using E = double;                \
+      __VA_ARGS__;                     \
+    }                                  \
   } while (0)
 
-void redot_to_slot(bool c, const void* x, const void* y, size_t n, double* slot, Workspace& ws, hipStream_t s) {
-  int nb = nblocks(n);
-  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_redot<E>, dim3(nb), dim3(kT), 0, s, cE<E>(x), cE<E>(y), n, ws.partial.data()));
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, s, ws.partial.data(), nb, slot, 0);
+void redot_to_slot(bool c, const void* x, const void* y, size_t n, double* slot, Workspace& ws, hipStream_t s,
+                   bool root) {
+  const int nb = nblocks(n);
+  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_redot<E>, dim3(nb), dim3(kT), 0, s, cE<E>(x), cE<E>(y), n,
+                                       ws.partial2.data()));
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kT), 0, s, ws.partial2.data(), nb, slot, root ? 1 : 0);
 }
 
 void nrm2sq_to_slot(bool c, const void* x, size_t n, double* slot, Workspace& ws, hipStream_t s) {
-  redot_to_slot(c, x, x, n, slot, ws, s);
-}
-
-void three_term(bool c, void* w, const void* v, const void* u, const double* a_slot, const double* b_slot, size_t n,
-                double* nrm2_slot, Workspace& ws, hipStream_t s) {
-  int nb = nblocks(n);
-  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_three<E>, dim3(nb), dim3(kT), 0, s, mE<E>(w), cE<E>(v), cE<E>(u), a_slot,
-                                       b_slot, n, ws.partial.data()));
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, s, ws.partial.data(), nb, nrm2_slot, 0);
-}
-
-void project_out(bool c, const void* V, size_t n, int m, void* w, void* d_h, double* nrm2_slot, Workspace& ws,
-                 hipStream_t s) {
-  // h = V^H w ; w -= V h ; nrm2 = |w|^2, in column chunks of kMaxCols
-  const int rb = int((n + kRowsT - 1) / kRowsT);
-  const size_t esz = c ? 16 : 8;
-  size_t need = size_t(rb) * std::min(m, kMaxCols) * esz / 8;
-  if (ws.partial.size() < need) ws.partial.resize(need * 2);
-  for (int c0 = 0; c0 < m; c0 += kMaxCols) {
-    int mc = std::min(kMaxCols, m - c0);
-    bool last = c0 + mc >= m;
-    QMBED_DISPATCH(c, {
-      const E* Vc = cE<E>(V) + size_t(c0) * n;
-      E* hp = mE<E>(d_h) + c0;
-      E* part = reinterpret_cast<E*>(ws.partial.data());
-      hipLaunchKernelGGL(k_gemvt<E>, dim3(rb, (mc + kTile - 1) / kTile), dim3(kT), 0, s, Vc, n, mc, cE<E>(w), part);
-      hipLaunchKernelGGL(k_reduce_cols<E>, dim3((mc + 63) / 64), dim3(64), 0, s, part, rb, mc, hp);
-    });
-    (void)last;
-  }
-  // subtract after all overlaps are known (classical Gram-Schmidt pass)
-  int nb = nblocks(n);
-  DeviceArray<double>& part2 = ws.partial2;
-  if (part2.size() < size_t(nb)) part2.resize(kMaxBlocks);
-  for (int c0 = 0; c0 < m; c0 += kMaxCols) {
-    int mc = std::min(kMaxCols, m - c0);
-    bool last = c0 + mc >= m;
-    QMBED_DISPATCH(c, hipLaunchKernelGGL(k_gemvn_sub<E>, dim3(nb), dim3(kT), 0, s, cE<E>(V) + size_t(c0) * n, n, mc,
-                                         cE<E>(d_h) + c0, mE<E>(w), last ? part2.data() : nullptr));
-  }
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, s, part2.data(), nb, nrm2_slot, 0);
+  redot_to_slot(c, x, x, n, slot, ws, s, false);
 }
 
 void normalize(bool c, const void* w, const double* nrm2_slot, size_t n, void* out, float* out32, double* beta_slot,
@@ -365,16 +478,107 @@ void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, voi
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s) {
   QMBED_DISPATCH(c, hipLaunchKernelGGL(k_to32<E>, dim3(nblocks(n)), dim3(kT), 0, s, cE<E>(x), n, y));
 }
-void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s) {
-  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_from32<E>, dim3(nblocks(n)), dim3(kT), 0, s, y, n, mE<E>(x)));
+void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s, const int* gate) {
+  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_from32<E>, dim3(nblocks(n)), dim3(kT), 0, s, y, n, mE<E>(x), gate));
 }
 
 void residual_to_slot(bool c, const void* y, const void* x, const double* lam_slot, size_t n, double* slot,
                       Workspace& ws, hipStream_t s) {
-  int nb = nblocks(n);
+  const int nb = nblocks(n);
   QMBED_DISPATCH(c, hipLaunchKernelGGL(k_resid<E>, dim3(nb), dim3(kT), 0, s, cE<E>(y), cE<E>(x), lam_slot, n,
-                                       ws.partial.data()));
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, s, ws.partial.data(), nb, slot, 1);
+                                       ws.partial2.data()));
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kT), 0, s, ws.partial2.data(), nb, slot, 1);
+}
+
+// row blocks of the overlap kernel: about 1024 blocks in all, at most 256,
+// never more than there are row tiles
+static int dots_row_blocks(size_t n, int groups) {
+  const size_t tiles = (n + size_t(kT * kRPT) - 1) / size_t(kT * kRPT);
+  const int want = std::min(256, std::max(16, 1024 / std::max(groups, 1)));
+  return int(std::min<size_t>(tiles, size_t(want)));
+}
+
+template <typename E>
+static int chunk_groups(int mm) {
+  return (mm + dots_tile<E>() - 1) / dots_tile<E>();
+}
+
+// the layout shared by overlaps and subtract: rb from the widest chunk
+template <typename E>
+static void pass_layout(size_t n, int m, bool with_ww, int& rb, int& ld) {
+  const int widest = std::min(m, kMaxCols) + (with_ww ? 1 : 0);
+  rb = dots_row_blocks(n, chunk_groups<E>(widest));
+  ld = m + 1;
+}
+
+template <typename E>
+static Three<E> three_of(const ThreeTerm* tt, Workspace& ws, size_t n) {
+  Three<E> t;
+  if (!tt) return t;
+  t.vj = cE<E>(tt->vj);
+  t.vjm1 = cE<E>(tt->vjm1);
+  t.beta_prev = tt->beta_prev;
+  t.pa = ws.partial3.data();
+  t.nba = nblocks(n);
+  t.alpha = tt->alpha;
+  return t;
+}
+
+void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate) {
+  QMBED_DISPATCH(c, hipLaunchKernelGGL(k_alpha<E>, dim3(nblocks(n)), dim3(kT), 0, s, cE<E>(vj), cE<E>(w), n,
+                                       ws.partial3.data(), gate));
+}
+
+void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Workspace& ws, hipStream_t s,
+                      bool with_ww, const int* gate, const ThreeTerm* three) {
+  QMBED_DISPATCH(c, {
+    int rb, ld;
+    pass_layout<E>(n, m, with_ww, rb, ld);
+    const size_t need = size_t(rb) * size_t(ld) * (sizeof(E) / 8);
+    if (ws.partial.size() < need) ws.partial.resize(need * 2);
+    const Three<E> t = three_of<E>(three, ws, n);
+    for (int c0 = 0; c0 < m; c0 += kMaxCols) {
+      const int mc = std::min(kMaxCols, m - c0);
+      const bool last = c0 + mc >= m;
+      const int mm = mc + ((last && with_ww) ? 1 : 0);
+      constexpr int CT = dots_tile<E>();
+      hipLaunchKernelGGL((k_dots<E, CT>), dim3(rb, chunk_groups<E>(mm)), dim3(kT), 0, s, cE<E>(V) + size_t(c0) * n, n,
+                         mc, mm, cE<E>(w), mE<E>(ws.partial.data()), ld, c0, gate, t);
+    }
+  });
+}
+
+// blocks of the subtract kernel: one row per thread up to 1024 blocks (its
+// per-block norm partials are re-added by every block of the finishing
+// kernel)
+static int sub_blocks(size_t n) {
+  return int(std::min<size_t>(kMaxBlocks, std::max<size_t>(1, (n + kT - 1) / kT)));
+}
+
+void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h, Workspace& ws, hipStream_t s,
+                      bool with_ww, const int* gate, const ThreeTerm* three) {
+  const int nb = sub_blocks(n);
+  QMBED_DISPATCH(c, {
+    int rb, ld;
+    pass_layout<E>(n, m, with_ww, rb, ld);
+    const int ncols = m + (with_ww ? 1 : 0);
+    hipLaunchKernelGGL(k_hred<E>, dim3((ncols + kT / 64 - 1) / (kT / 64)), dim3(kT), 0, s, cE<E>(ws.partial.data()),
+                       rb, ld, ncols, mE<E>(d_h), gate);
+    const Three<E> t = three_of<E>(three, ws, n);
+    for (int c0 = 0; c0 < m; c0 += kMaxCols) {
+      const int mc = std::min(kMaxCols, m - c0);
+      const bool last = c0 + mc >= m;
+      hipLaunchKernelGGL(k_sub<E>, dim3(nb), dim3(kT), 0, s, cE<E>(V) + size_t(c0) * n, n, mc, cE<E>(d_h) + c0,
+                         mE<E>(w), last ? ws.partial2.data() : nullptr, gate, c0 == 0 ? t : Three<E>());
+    }
+  });
+}
+
+void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,
+                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass) {
+  QMBED_DISPATCH(c, hipLaunchKernelGGL((k_finish<E, float>), dim3(nblocks(n)), dim3(kT), 0, s, cE<E>(w), n,
+                                       ws.partial2.data(), sub_blocks(n), cE<E>(d_h), j, eta, beta, halt, hist,
+                                       mE<E>(out), out32, second_pass ? 1 : 0));
 }
 
 }  // namespace lk

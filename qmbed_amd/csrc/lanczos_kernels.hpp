@@ -8,31 +8,63 @@ namespace qmbed {
 namespace lk {
 
 struct Workspace {
-  DeviceArray<double> partial;
-  DeviceArray<double> partial2;
+  DeviceArray<double> partial;   // overlap partials, one row of columns per row block
+  DeviceArray<double> partial2;  // per-block scalar partials
+  DeviceArray<double> partial3;  // per-block α_j partials of a Lanczos step
   DeviceArray<double> slots;
   Workspace();
 };
 
-// slot = Re <x, y>
-void redot_to_slot(bool c, const void* x, const void* y, size_t n, double* slot, Workspace& ws, hipStream_t s);
+// slot = Re <x, y> (its square root when root)
+void redot_to_slot(bool c, const void* x, const void* y, size_t n, double* slot, Workspace& ws, hipStream_t s,
+                   bool root = false);
 void nrm2sq_to_slot(bool c, const void* x, size_t n, double* slot, Workspace& ws, hipStream_t s);
-// w -= a v + b u ; nrm2 = |w|^2
-void three_term(bool c, void* w, const void* v, const void* u, const double* a_slot, const double* b_slot, size_t n,
-                double* nrm2_slot, Workspace& ws, hipStream_t s);
-// one classical Gram-Schmidt pass against V[:, 0..m): h = V^H w, w -= V h; nrm2 = |w|^2
-void project_out(bool c, const void* V, size_t n, int m, void* w, void* d_h, double* nrm2_slot, Workspace& ws,
-                 hipStream_t s);
-// out = w / sqrt(nrm2); beta_slot = sqrt(nrm2); optional FP32 copy
+// out = w / sqrt(nrm2); beta_slot = sqrt(nrm2); optional FP32 copy; out may be
+// null to write beta_slot only
 void normalize(bool c, const void* w, const double* nrm2_slot, size_t n, void* out, float* out32, double* beta_slot,
                hipStream_t s);
 // Y = V C (n×m · m×k)
 void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, void* Y, hipStream_t s);
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s);
-void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s);
+void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s, const int* gate = nullptr);
 // slot = |y - lam x|
 void residual_to_slot(bool c, const void* y, const void* x, const double* lam_slot, size_t n, double* slot,
                       Workspace& ws, hipStream_t s);
+
+// One host-free Lanczos step j (m = j + 1 columns) follows the reference's
+// order (QMBED src/solve/mod.rs:1030-1080): w = H·v_j; α_j = <v_j, w>
+// (lanczos_alpha); w₀ = w − α_j v_j − β_{j−1} v_{j−1} is formed on the fly by
+// lanczos_overlaps (overlaps with V and ‖w₀‖²) and lanczos_subtract
+// (w ← w₀ − V h); lanczos_finish takes the DGKS decision against ‖w₀‖ and
+// normalizes. The launches of a first pass take `gate` = the window's halt
+// flag and are no-ops while it is set. When lanczos_finish finds that a
+// second pass is needed it sets *halt = j + 1 instead of normalizing; the
+// host then runs lanczos_overlaps/lanczos_subtract on w (no three-term part,
+// no ‖w‖², ungated) and lanczos_finish(second_pass), which clears *halt, and
+// relaunches the voided steps.
+
+// per-block partials of α_j = Re <v_j, w>
+void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate);
+
+// the three-term part of a first pass
+struct ThreeTerm {
+  const void* vj;
+  const void* vjm1;         // null at j = 0
+  const double* beta_prev;  // β_{j−1} slot (unused at j = 0)
+  double* alpha;            // α_j slot, written by lanczos_subtract
+};
+
+// per-row-block overlaps of w₀ with V[:, 0..m) (and ‖w₀‖² when with_ww)
+void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Workspace& ws, hipStream_t s,
+                      bool with_ww, const int* gate, const ThreeTerm* three);
+// h = Σ partials (into d_h, m entries, + ‖w₀‖² at d_h[m] when with_ww;
+// one reduction launch), then w ← w₀ − V h and per-block ‖w‖² partials
+void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h, Workspace& ws, hipStream_t s,
+                      bool with_ww, const int* gate, const ThreeTerm* three);
+// DGKS decision with threshold eta (first pass), β_j = ‖w‖, out = w / β_j
+// (+ FP32 copy)
+void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,
+                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass);
 
 }  // namespace lk
 }  // namespace qmbed

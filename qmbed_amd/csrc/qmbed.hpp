@@ -214,10 +214,12 @@ class Operator {
 
   // y = H x on `stream`. Pointers are device pointers. Complex vectors are
   // interleaved (re, im) pairs. FP32 variants compute in single precision.
-  void apply(const double* x, double* y, hipStream_t s) const;          // real vectors (requires is_real)
-  void apply(const cplx* x, cplx* y, hipStream_t s) const;              // complex vectors
-  void apply(const float* x, float* y, hipStream_t s) const;            // real FP32
-  void apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s) const;
+  // `gate` (optional device int): a nonzero value at launch time makes the
+  // action a no-op (used by speculative Lanczos steps)
+  void apply(const double* x, double* y, hipStream_t s, const int* gate = nullptr) const;  // real (requires is_real)
+  void apply(const cplx* x, cplx* y, hipStream_t s, const int* gate = nullptr) const;      // complex vectors
+  void apply(const float* x, float* y, hipStream_t s, const int* gate = nullptr) const;    // real FP32
+  void apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s, const int* gate = nullptr) const;
   // nvec columns (stride = dim), complex; one lookup shared across columns
   void apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const;
 
@@ -327,6 +329,10 @@ struct EigshControls {
   // when true, stop inside the window as soon as the requested pairs have
   // converged (the reference always runs the full window)
   bool early_stop = true;
+  // DGKS threshold η: a second reorthogonalization pass is taken when the
+  // first leaves ‖w‖ <= η‖w₀‖ (the reference's constant is 1/√2; η = 1 takes
+  // the second-pass path on nearly every step, which exercises it)
+  double dgks_eta = 0.7071067811865476;
 };
 
 struct Eigensystem {

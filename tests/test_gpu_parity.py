@@ -157,6 +157,68 @@ def test_eigsh_fp32_matvec(qa, oracle):
     assert r["fp32_iterations"] > 0
 
 
+def _zz_field(L, eps):
+    return [("zz", [(1.0, [i, (i + 1) % L]) for i in range(L)]), ("x", [(eps, [i]) for i in range(L)])]
+
+
+def _check_spectrum(r, exact):
+    assert abs(r["eigenvalues"][0] - exact[0]) <= 1e-10 * abs(exact[0])
+    for lam, res in zip(r["eigenvalues"], r["residuals"]):
+        assert np.min(np.abs(exact - lam)) <= 1e-10 * max(abs(lam), 1.0)
+        assert res <= 1e-10 * max(abs(lam), 1.0)
+
+
+@pytest.mark.parametrize("eps", [1e-4, 1e-5])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_eigsh_near_invariant_subspace(qa, oracle, eps, prec):
+    # ZZ chain with a tiny transverse field: the Krylov space brushes past
+    # near-invariant subspaces, where ‖H·v_j‖ exceeds β_j by orders of
+    # magnitude. The overlaps must be taken after the three-term part (as the
+    # reference does): one Gram-Schmidt pass on the raw H·v_j loses
+    # orthogonality completely here (full-window Ritz residuals ~1e2).
+    L = 10
+    spec = _zz_field(L, eps)
+    exact = np.linalg.eigvalsh(oracle.Model(L, spec).dense())
+    print(f"[near-invariant] eps={eps} prec={prec}", flush=True)
+    m = qa.Model({"kind": "spin", "sites": L}, spec)
+    _check_spectrum(m.eigsh(3, matvec_precision=prec, early_stop=False), exact)
+    _check_spectrum(m.eigsh(1, matvec_precision=prec), exact)
+
+
+@pytest.mark.parametrize("case", ["zz_field", "xxz_k3"])
+def test_eigsh_second_pass_rewind(qa, oracle, case):
+    # With the DGKS threshold raised to η = 1 nearly every step asks for a
+    # second pass, so the speculative steps launched after each are voided
+    # and relaunched; the spectrum must not change.
+    eta = 1.0
+    if case == "zz_field":
+        L, spec, basis = 10, _zz_field(10, 1e-4), {"kind": "spin", "sites": 10}
+        om = oracle.Model(L, spec)
+    else:
+        L = 16  # complex sector, dimension 810 (Lanczos, not the dense route)
+        spec = cases.xxz(L, 1.5)
+        basis = {"kind": "spin", "sites": L, "up": 8, "momentum": 3}
+        om = oracle.Model(L, spec, up=8, momentum=3)
+    exact = np.linalg.eigvalsh(om.dense())
+    print(f"[second-pass] case={case} eta={eta}", flush=True)
+    m = qa.Model(basis, spec)
+    ref = m.eigsh(3, early_stop=False)
+    r = m.eigsh(3, early_stop=False, reorthogonalization_eta=eta)
+    assert r["conditional_second_passes"] > 0
+    assert ref["conditional_second_passes"] < r["conditional_second_passes"]
+    _check_spectrum(r, exact)
+    np.testing.assert_allclose(r["eigenvalues"], ref["eigenvalues"], rtol=1e-10, atol=0)
+    r2 = m.eigsh(2, reorthogonalization_eta=eta, matvec_precision="fp32")
+    _check_spectrum(r2, exact)
+
+
+def test_eigsh_rejects_bad_eta(qa):
+    m = qa.Model({"kind": "spin", "sites": 8, "up": 4}, cases.heisenberg(8))
+    for bad in (0.0, 1.5, -1.0):
+        with pytest.raises(qa.QmbedError, match="reorthogonalization_eta"):
+            m.eigsh(1, reorthogonalization_eta=bad)
+
+
 def test_dimer_and_l4_anchors(qa):
     # reference anchors: dimer ground state -0.75 (tests/visible_contract.rs:991-1007,
     # capi :6312-6329); L=4 k=0 Heisenberg ground state -2 (visible_contract.rs:171-179)
