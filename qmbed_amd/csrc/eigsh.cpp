@@ -19,6 +19,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
+#include <mutex>
 #include <numeric>
 
 #include "internal.hpp"
@@ -92,43 +94,117 @@ struct Candidate {
   int column;  // column in the Ritz buffer
 };
 
-// One Lanczos window on the device. A step is four launches and no host
-// round trip: H·v_j, the per-row-block overlaps of w with V (and ‖w‖²), the
-// subtraction w −= V h, and a finishing kernel that takes α_j, decides the
-// DGKS second pass from the overlaps and normalizes w into v_{j+1}. Steps are
-// launched speculatively on the assumption that no second pass is needed
-// (the usual case); when one is, the finishing kernel raises a device halt
-// flag that turns every later launch into a no-op, and the host, at its next
-// read-back, runs the second pass for that step and relaunches from there.
-// The host reads coefficients back only at these read-backs (`sync`), where
-// breakdown (β ≤ 1e-14) truncates the window to the steps before it.
+// Device and pinned host buffers of a Lanczos window, kept in a small
+// per-device pool between solves so that a solve does not allocate (and free,
+// which synchronizes the device) its Krylov basis every time. Buffers only
+// grow; the pool keeps at most two sets per device.
+struct WindowBuffers {
+  int device = 0;
+  DeviceArray<double> V, w, slots;
+  DeviceArray<float> x32, y32;
+  lk::Workspace ws;
+  double* pin = nullptr;  // pinned read-back area
+  size_t pin_n = 0;
+  hipEvent_t ev = nullptr;
+  WindowBuffers() {
+    QMBED_HIP(hipGetDevice(&device));
+    QMBED_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  ~WindowBuffers() {
+    if (pin) (void)hipHostFree(pin);
+    if (ev) (void)hipEventDestroy(ev);
+  }
+  void ensure_pin(size_t n) {
+    if (pin_n >= n) return;
+    if (pin) QMBED_HIP(hipHostFree(pin));
+    pin = nullptr;
+    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin), n * sizeof(double), hipHostMallocDefault));
+    pin_n = n;
+  }
+};
+
+template <typename T>
+static void grow(DeviceArray<T>& a, size_t n) {
+  if (a.size() < n) a.resize(n);
+}
+
+static std::mutex g_pool_mu;
+static std::vector<std::unique_ptr<WindowBuffers>> g_pool;
+
+static std::unique_ptr<WindowBuffers> acquire_buffers() {
+  int dev = 0;
+  QMBED_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  for (size_t i = 0; i < g_pool.size(); ++i) {
+    if (g_pool[i]->device == dev) {
+      auto b = std::move(g_pool[i]);
+      g_pool.erase(g_pool.begin() + long(i));
+      return b;
+    }
+  }
+  return std::make_unique<WindowBuffers>();
+}
+
+// the caller has synchronized the stream that used b
+static void release_buffers(std::unique_ptr<WindowBuffers> b) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  int same = 0;
+  for (auto& e : g_pool) same += e->device == b->device ? 1 : 0;
+  if (same < 2) g_pool.push_back(std::move(b));
+}
+
+// One Lanczos window on the device. A step is six launches and no host
+// round trip: H·v_j, the partials of α_j = <v_j, H·v_j>, the overlaps of
+// w₀ = H·v_j − α_j v_j − β_{j−1} v_{j−1} with V (and ‖w₀‖²), their
+// reduction, the subtraction w ← w₀ − V h, and a finishing kernel that takes
+// the DGKS decision and normalizes w into v_{j+1}. Steps are launched
+// speculatively on the assumption that no second pass is needed (the usual
+// case); when one is, the finishing kernel raises a device halt flag that
+// turns every later launch into a no-op, and the host, when it collects the
+// next read-back, runs the second pass for that step and relaunches from
+// there. Coefficients reach the host only through read-backs: `post` queues
+// an asynchronous copy into pinned memory at the current step, `collect`
+// waits for it (by then the device is already working on later steps) and
+// parses the coefficients of the steps before it; breakdown (β ≤ 1e-14)
+// truncates the window to the steps before it.
 class LanczosWindow {
  public:
   LanczosWindow(const Operator& op, bool complex, int m, bool fp32, hipStream_t s, double eta = DGKS)
-      : op_(op), c_(complex), n_(op.dim()), m_(m), fp32_(fp32), eta_(eta), s_(s) {
+      : op_(op), c_(complex), n_(op.dim()), m_(m), fp32_(fp32), eta_(eta), s_(s), buf_(acquire_buffers()) {
     const size_t esz = c_ ? 2 : 1;
-    V_.resize(esz * n_ * size_t(m_));
-    w_.resize(esz * n_);
+    grow(buf_->V, esz * n_ * size_t(m_));
+    grow(buf_->w, esz * n_);
     if (fp32_) {
-      x32_.resize(esz * n_);
-      y32_.resize(esz * n_);
+      grow(buf_->x32, esz * n_);
+      grow(buf_->y32, esz * n_);
     }
-    // slots: [0,m) alpha, [m,2m) beta, then h (2(m+1) doubles), then scratch
-    slots_.resize(2 * size_t(m_) + 8 + 2 * size_t(m_ + 1) + 64);
-    h_ = slots_.data() + 2 * m_ + 8;
+    // slots: [0,m) α, [m,2m) β, [2m, 2m+F) flags (m+1 ints: per-step
+    // second-pass record, then halt), then h (2(m+1)), then scratch
+    flag_doubles_ = (size_t(m_) + 2) / 2;
+    const size_t head = 2 * size_t(m_) + flag_doubles_;
+    grow(buf_->slots, head + 2 * size_t(m_ + 1) + 3 * size_t(m_ + 1) + 64);
+    h_ = buf_->slots.data() + head;
     scratch_ = h_ + 2 * (m_ + 1);
-    // flags: [0,m) per-step second-pass record, m: halt
-    flags_.resize(size_t(m_) + 1);
-    QMBED_HIP(hipMemsetAsync(flags_.data(), 0, (size_t(m_) + 1) * sizeof(int), s_));
+    flags_ = reinterpret_cast<int*>(buf_->slots.data() + 2 * m_);
+    QMBED_HIP(hipMemsetAsync(flags_, 0, flag_doubles_ * sizeof(double), s_));
+    buf_->ensure_pin(head);
   }
+  ~LanczosWindow() {
+    if (hipStreamSynchronize(s_) == hipSuccess) release_buffers(std::move(buf_));
+  }
+  LanczosWindow(const LanczosWindow&) = delete;
+  LanczosWindow& operator=(const LanczosWindow&) = delete;
 
   size_t esz() const { return c_ ? 2 : 1; }
-  double* col(int j) const { return V_.data() + esz() * n_ * size_t(j); }
+  double* col(int j) const { return buf_->V.data() + esz() * n_ * size_t(j); }
+  double* V() const { return buf_->V.data(); }
+  double* w() const { return buf_->w.data(); }
+  double* slots() const { return buf_->slots.data(); }
 
   // Install the start vector (device, FP64, already normalized) as column 0.
   void start_from(const double* d_v0) {
     QMBED_HIP(hipMemcpyAsync(col(0), d_v0, esz() * n_ * 8, hipMemcpyDeviceToDevice, s_));
-    if (fp32_) lk::to_f32(c_, col(0), n_, x32_.data(), s_);
+    if (fp32_) lk::to_f32(c_, col(0), n_, buf_->x32.data(), s_);
     alpha_.clear();
     beta_.clear();
     launched_ = 0;
@@ -137,50 +213,59 @@ class LanczosWindow {
 
   void matvec(int j, const int* gate) {
     if (fp32_) {
-      if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32_.data()),
-                        reinterpret_cast<std::complex<float>*>(y32_.data()), s_, gate);
-      else op_.apply(x32_.data(), y32_.data(), s_, gate);
-      lk::from_f32(c_, y32_.data(), n_, w_.data(), s_, gate);
+      float* x32 = buf_->x32.data();
+      float* y32 = buf_->y32.data();
+      if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32), reinterpret_cast<std::complex<float>*>(y32),
+                        s_, gate);
+      else op_.apply(x32, y32, s_, gate);
+      lk::from_f32(c_, y32, n_, w(), s_, gate);
     } else {
-      if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w_.data()), s_, gate);
-      else op_.apply(col(j), w_.data(), s_, gate);
+      if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w()), s_, gate);
+      else op_.apply(col(j), w(), s_, gate);
     }
     ++matvecs_;
   }
 
-  int* halt() const { return flags_.data() + m_; }
+  int* halt() const { return flags_ + m_; }
 
   // Launch step j = launched() (requires launched() < m).
   void step() {
     const int j = launched_;
     const int* gate = halt();
     matvec(j, gate);
-    const lk::ThreeTerm three{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots_.data() + m_ + (j - 1) : nullptr,
-                              slots_.data() + j};
-    lk::lanczos_alpha(c_, col(j), w_.data(), n_, ws_, s_, gate);
-    lk::lanczos_overlaps(c_, V_.data(), n_, j + 1, w_.data(), ws_, s_, true, gate, &three);
-    lk::lanczos_subtract(c_, V_.data(), n_, j + 1, w_.data(), h_, ws_, s_, true, gate, &three);
+    const lk::ThreeTerm three{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr,
+                              slots() + j};
+    lk::lanczos_alpha(c_, col(j), w(), n_, buf_->ws, s_, gate);
+    lk::lanczos_overlaps(c_, V(), n_, j + 1, w(), buf_->ws, s_, true, gate, &three);
+    lk::lanczos_subtract(c_, V(), n_, j + 1, w(), h_, buf_->ws, s_, true, gate, &three);
     finish(j, false);
     ++launched_;
   }
 
-  // Read back the coefficients of the launched steps. A pending second pass
-  // (halt flag set at step h) is run here and the window rewinds to h + 1
-  // (the voided steps are relaunched by the caller). Otherwise `ended` is
-  // true when the window is full or broke down (then truncated to the step
-  // whose β fell to 1e-14 or below, as the reference stops there).
+  // Queue the read-back of the coefficients of the launched steps.
+  void post() {
+    QMBED_HIP(hipMemcpyAsync(buf_->pin, slots(), (2 * size_t(m_) + flag_doubles_) * 8, hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipEventRecord(buf_->ev, s_));
+    posted_ = launched_;
+  }
+  int posted() const { return posted_; }
+
+  // Wait for the posted read-back and parse it. A pending second pass (halt
+  // flag set at step h) is queued here and the window rewinds to h + 1 (the
+  // voided steps are relaunched by the caller). Otherwise `ended` is true
+  // when the read-back covered the full window or a breakdown (the window is
+  // then truncated to the step whose β fell to 1e-14 or below, as the
+  // reference stops there).
   struct Sync {
     bool ended = false;
     bool rewound = false;
   };
-  Sync sync() {
-    const int L = launched_;
-    std::vector<double> hb(2 * size_t(m_));
-    std::vector<int> hf(static_cast<size_t>(m_) + 1);
-    QMBED_HIP(hipMemcpyAsync(hb.data(), slots_.data(), 2 * size_t(m_) * 8, hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipMemcpyAsync(hf.data(), flags_.data(), (size_t(m_) + 1) * sizeof(int), hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipStreamSynchronize(s_));
-    const int halted_at = hf[size_t(m_)] - 1;  // -1: none
+  Sync collect() {
+    QMBED_HIP(hipEventSynchronize(buf_->ev));
+    const int L = posted_;
+    const double* hb = buf_->pin;
+    const int* hf = reinterpret_cast<const int*>(buf_->pin + 2 * m_);
+    const int halted_at = hf[m_] - 1;  // -1: none
     if (halted_at >= L) throw internal_state("Lanczos halt flag beyond the launched steps");
     const int valid = halted_at >= 0 ? halted_at : L;  // steps with final α and β
     alpha_.clear();
@@ -191,7 +276,8 @@ class LanczosWindow {
       if (!std::isfinite(a) || !std::isfinite(b))
         throw internal_state("non-finite Lanczos coefficient at step " + std::to_string(j));
       if (b <= 1.0e-14) {
-        alpha_.push_back(a);
+        for (int i = 0; i <= j; ++i) alpha_.push_back(hb[size_t(i)]);
+        for (int i = 0; i < j; ++i) beta_.push_back(hb[size_t(m_ + i)]);
         last_beta_ = b;
         broke_ = true;
         size_ = j + 1;
@@ -203,8 +289,8 @@ class LanczosWindow {
     if (halted_at >= 0) {
       // second pass for step h, then resume from h + 1
       const int h = halted_at;
-      lk::lanczos_overlaps(c_, V_.data(), n_, h + 1, w_.data(), ws_, s_, false, nullptr, nullptr);
-      lk::lanczos_subtract(c_, V_.data(), n_, h + 1, w_.data(), h_, ws_, s_, false, nullptr, nullptr);
+      lk::lanczos_overlaps(c_, V(), n_, h + 1, w(), buf_->ws, s_, false, nullptr, nullptr);
+      lk::lanczos_subtract(c_, V(), n_, h + 1, w(), h_, buf_->ws, s_, false, nullptr, nullptr);
       finish(h, true);
       launched_ = h + 1;
       r.rewound = true;
@@ -249,7 +335,7 @@ class LanczosWindow {
     return true;
   }
 
-  // Assemble Ritz vectors of the selected pairs into `ritz` (k columns),
+  // Assemble Ritz vectors of the selected pairs into `ritz_buf` (k columns),
   // normalize, and compute FP64 Rayleigh quotients and residuals.
   std::vector<Candidate> ritz(int k, SpectrumTarget target, double shift, DeviceArray<double>& ritz_buf) {
     const int sz = size_;
@@ -265,31 +351,35 @@ class LanczosWindow {
     DeviceArray<double> dC(C.size());
     QMBED_HIP(hipMemcpyAsync(dC.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
     ritz_buf.resize(e * n_ * sel.size());
-    lk::gemm_vc(c_, V_.data(), n_, sz, dC.data(), int(sel.size()), ritz_buf.data(), s_);
+    lk::gemm_vc(c_, V(), n_, sz, dC.data(), int(sel.size()), ritz_buf.data(), s_);
+    auto rr = rayleigh(ritz_buf.data(), int(sel.size()));
     std::vector<Candidate> out;
-    for (size_t q = 0; q < sel.size(); ++q) {
-      double* y = ritz_buf.data() + e * n_ * q;
-      auto rr = rayleigh(y);
-      out.push_back({rr.first, rr.second, int(q)});
-    }
-    QMBED_HIP(hipStreamSynchronize(s_));
+    for (size_t q = 0; q < sel.size(); ++q) out.push_back({rr[q].first, rr[q].second, int(q)});
     return out;
   }
 
-  // normalize y in place; FP64 Rayleigh quotient and residual norm
-  std::pair<double, double> rayleigh(double* y) {
-    double* sl = scratch_;
-    lk::nrm2sq_to_slot(c_, y, n_, sl, ws_, s_);
-    lk::normalize(c_, y, sl, n_, y, nullptr, nullptr, s_);
-    if (c_) op_.apply(reinterpret_cast<const cplx*>(y), reinterpret_cast<cplx*>(w_.data()), s_);
-    else op_.apply(y, w_.data(), s_);
-    ++matvecs64_;
-    lk::redot_to_slot(c_, y, w_.data(), n_, sl + 1, ws_, s_);
-    lk::residual_to_slot(c_, w_.data(), y, sl + 1, n_, sl + 2, ws_, s_);
-    double host[3];
-    QMBED_HIP(hipMemcpyAsync(host, sl, 3 * sizeof(double), hipMemcpyDeviceToHost, s_));
+  // Normalize the nv columns of Y in place; FP64 Rayleigh quotients and
+  // residual norms, read back with one synchronization.
+  std::vector<std::pair<double, double>> rayleigh(double* Y, int nv) {
+    if (3 * size_t(nv) > 3 * size_t(m_ + 1) + 64) throw internal_state("Rayleigh batch exceeds the slot area");
+    const size_t e = esz();
+    for (int q = 0; q < nv; ++q) {
+      double* y = Y + e * n_ * size_t(q);
+      double* sl = scratch_ + 3 * q;
+      lk::nrm2sq_to_slot(c_, y, n_, sl, buf_->ws, s_);
+      lk::normalize(c_, y, sl, n_, y, nullptr, nullptr, s_);
+      if (c_) op_.apply(reinterpret_cast<const cplx*>(y), reinterpret_cast<cplx*>(w()), s_);
+      else op_.apply(y, w(), s_);
+      ++matvecs64_;
+      lk::redot_to_slot(c_, y, w(), n_, sl + 1, buf_->ws, s_);
+      lk::residual_to_slot(c_, w(), y, sl + 1, n_, sl + 2, buf_->ws, s_);
+    }
+    std::vector<double> host(3 * size_t(nv));
+    QMBED_HIP(hipMemcpyAsync(host.data(), scratch_, host.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
     QMBED_HIP(hipStreamSynchronize(s_));
-    return {host[1], host[2]};
+    std::vector<std::pair<double, double>> out;
+    for (int q = 0; q < nv; ++q) out.push_back({host[3 * size_t(q) + 1], host[3 * size_t(q) + 2]});
+    return out;
   }
 
   long matvecs64() const { return matvecs64_; }
@@ -297,10 +387,10 @@ class LanczosWindow {
  private:
   void finish(int j, bool second_pass) {
     const bool next = j + 1 < m_;
-    lk::lanczos_finish(c_, w_.data(), n_, h_, j, eta_, slots_.data() + m_ + j, halt(), flags_.data() + j,
-                       next ? col(j + 1) : nullptr, (next && fp32_) ? x32_.data() : nullptr, ws_, s_, second_pass);
+    lk::lanczos_finish(c_, w(), n_, h_, j, eta_, slots() + m_ + j, halt(), flags_ + j, next ? col(j + 1) : nullptr,
+                       (next && fp32_) ? buf_->x32.data() : nullptr, buf_->ws, s_, second_pass);
   }
-  void count_passes(const std::vector<int>& hf, int steps) {
+  void count_passes(const int* hf, int steps) {
     passes_ = 0;
     seconds_ = 0;
     for (int j = 0; j < steps; ++j) {
@@ -316,15 +406,15 @@ class LanczosWindow {
   bool fp32_;
   double eta_;
   hipStream_t s_;
-  DeviceArray<double> V_, w_, slots_;
-  DeviceArray<float> x32_, y32_;
-  DeviceArray<int> flags_;  // [0,m) per-step second-pass record, m: halt (step + 1, 0 = none)
+  std::unique_ptr<WindowBuffers> buf_;
+  size_t flag_doubles_ = 0;
+  int* flags_ = nullptr;  // [0,m) per-step second-pass record, m: halt (step + 1, 0 = none)
   double* h_ = nullptr;
   double* scratch_ = nullptr;
-  lk::Workspace ws_;
   std::vector<double> alpha_, beta_;
   double last_beta_ = 0;
   int launched_ = 0;
+  int posted_ = 0;
   int size_ = 0;
   long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0;
   bool broke_ = false;
@@ -347,27 +437,40 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
   win.start_from(d_start);
   const int k = o.eigenpairs;
   int next_check = std::max(k + 1, ctl.check_every);
-  // read-backs: at convergence checks, at the window's end, and otherwise
-  // every `interval` steps (halved after a rewind, so frequent second passes
-  // void few speculative steps)
-  int interval = ctl.check_every;
+  // Read-backs are posted at convergence checks, at the window's end, and
+  // otherwise every `interval` steps (halved after a rewind, so frequent
+  // second passes void few speculative steps). A posted read-back is
+  // collected kLag steps later, when the device has long finished the copy,
+  // so the device never waits for the host; a convergence found at a
+  // collected point costs at most kLag extra steps.
+  constexpr int kLag = 2;
+  int interval = std::max(1, ctl.check_every);
   int next_sync = interval;
+  bool pending = false, pending_conv = false;
   for (;;) {
     if (win.launched() < m) win.step();
     const int launched = win.launched();
     const bool full = launched == m;
-    const bool conv = ctl.early_stop && launched >= next_check && launched > k;
-    if (!(full || conv || launched >= next_sync)) continue;
-    const auto st = win.sync();
+    if (!pending) {
+      const bool conv = ctl.early_stop && launched >= next_check && launched > k;
+      if (full || conv || launched >= next_sync) {
+        win.post();
+        pending = true;
+        pending_conv = conv;
+      }
+    }
+    if (!pending || (launched < win.posted() + kLag && !full)) continue;
+    pending = false;
+    const auto st = win.collect();
     if (st.rewound) {
       interval = std::max(1, interval / 2);
       next_sync = win.launched() + interval;
       continue;
     }
-    interval = std::min(ctl.check_every, interval * 2);
-    next_sync = launched + interval;
+    interval = std::min(std::max(1, ctl.check_every), interval * 2);
+    next_sync = win.posted() + interval;
     const bool at_end = st.ended;
-    if (!(at_end || conv)) continue;
+    if (!(at_end || pending_conv)) continue;
     const int sz = win.size();
     if (!at_end) {
       if (!win.estimates_converged(k, o.target, o.shift, accept_tol, 0.5)) {
@@ -475,7 +578,7 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
     }
     double* y = vec_out.data() + e * n * q;
     QMBED_HIP(hipMemcpyAsync(y, col.data(), e * n * 8, hipMemcpyHostToDevice, s));
-    auto r = rq.rayleigh(y);
+    auto r = rq.rayleigh(y, 1)[0];
     es.eigenvalues.push_back(r.first);
     es.residuals.push_back(r.second);
   }
