@@ -56,8 +56,10 @@ typedef struct qmbed_hip_model qmbed_hip_model;
 
 enum { QMBED_F64 = 0, QMBED_F32 = 1, QMBED_C128 = 2, QMBED_C64 = 3 };
 
-/* model_json: {"basis":{...},"terms":[...],"checks":{...}?} as in
- * create_model. Builds the sector on the GPU and compiles the operator.
+/* model_json: {"basis":{...},"terms":[...],"components":[...]?,
+ * "checks":{...}?} as in create_model (components: [{"name", "terms",
+ * "default": [re, im]?}], bindings/capi/src/lib.rs:783-787). Builds the
+ * sector on the GPU and compiles the operator.
  * On failure returns NULL and stores an error string (free with
  * qmbed_string_free) in *error when error is non-NULL. */
 qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error);
@@ -86,6 +88,15 @@ double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void*
  * the extra solver keys "matvec_precision" ("fp64"|"fp32") and
  * "early_stop" (bool). */
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
+
+/* Evaluate a parameterized model (created with "components") at
+ * {"name": [re, im], ...}: H = H_static + sum_p c_p H_p, defaults filling
+ * absent names. Later applies and solves use these values. Answers the
+ * coefficients in component order, or the reference's error text (unknown
+ * or missing parameter, non-finite value, non-Hermitian evaluation). No
+ * counterpart in the reference's ABI (its commands take "parameters" per
+ * call, which eigsh_json also accepts). */
+char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* parameters_json);
 
 void* qmbed_hip_malloc(size_t bytes);
 void qmbed_hip_free(void* ptr);

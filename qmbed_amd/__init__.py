@@ -69,6 +69,8 @@ def library():
     L.qmbed_hip_time_apply.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int]
     L.qmbed_hip_eigsh_json.restype = c_void_p
     L.qmbed_hip_eigsh_json.argtypes = [c_void_p, c_char_p]
+    L.qmbed_hip_model_set_parameters.restype = c_void_p
+    L.qmbed_hip_model_set_parameters.argtypes = [c_void_p, c_char_p]
     L.qmbed_hip_malloc.restype = c_void_p
     L.qmbed_hip_malloc.argtypes = [ctypes.c_size_t]
     L.qmbed_hip_free.argtypes = [c_void_p]
@@ -168,14 +170,27 @@ def ising_terms(L, hx, hz, j=1.0):
     return out
 
 
+def _terms_json(terms):
+    return terms if terms and isinstance(terms[0], dict) else terms_from_spec(terms)
+
+
 class Model:
     """Device-resident sector + matrix-free operator (typed fast path)."""
 
-    def __init__(self, basis: dict, terms, checks=None):
+    def __init__(self, basis: dict, terms, checks=None, components=None):
+        """components: [(name, terms, default or None), ...] — a family
+        H = H_static + Σ_p c_p H_p (the reference's create_model components)."""
         L = library()
-        req = {"basis": basis, "terms": terms if terms and isinstance(terms[0], dict) else terms_from_spec(terms)}
+        req = {"basis": basis, "terms": _terms_json(terms)}
         if checks is not None:
             req["checks"] = checks
+        if components:
+            req["components"] = []
+            for name, cterms, default in components:
+                c = {"name": name, "terms": _terms_json(cterms)}
+                if default is not None:
+                    c["default"] = [complex(default).real, complex(default).imag]
+                req["components"].append(c)
         err = ctypes.c_void_p()
         h = L.qmbed_hip_model_create(json.dumps(req).encode(), ctypes.byref(err))
         if not h:
@@ -245,8 +260,19 @@ class Model:
             raise QmbedError(f"apply failed (code {rc})")
         return y
 
-    def eigsh(self, k, target="smallest_algebraic", **solver):
+    def set_parameters(self, **values):
+        """Evaluate a family at the named coefficients (complex allowed)."""
+        req = {k: [complex(v).real, complex(v).imag] for k, v in values.items()}
+        response = json.loads(_take(library().qmbed_hip_model_set_parameters(self._h, json.dumps(req).encode())))
+        if response.get("status") != "ok":
+            raise QmbedError(response.get("error", "unknown error"))
+        self.is_real = bool(response["result"]["real"])
+        return [complex(a, b) for a, b in response["result"]["values"]]
+
+    def eigsh(self, k, target="smallest_algebraic", parameters=None, **solver):
         req = {"eigenpairs": k, "target": {"kind": target}, **solver}
+        if parameters is not None:
+            req["parameters"] = {n: [complex(v).real, complex(v).imag] for n, v in parameters.items()}
         return _call_model(self._h, req)
 
 

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <set>
 #include <sstream>
 
 #include "../../include/qmbed_hip.h"
@@ -233,6 +234,7 @@ SolverReq parse_solver(const Value& s) {
 struct Model {
   SpinBasisSpec spec;
   std::vector<OperatorSpec> terms;
+  std::vector<ComponentSpec> components;
   AssemblyChecks checks;
   std::shared_ptr<SpinBasis1D> basis;
   std::shared_ptr<Operator> op;
@@ -241,16 +243,66 @@ struct Model {
   ~Model() {
     if (stream) (void)hipStreamDestroy(stream);
   }
-  const Operator& materialized() {
-    if (!op) op = OperatorBuilder::on(basis).terms(terms).checks(checks).build(MatrixFormat::MatrixFree);
+  Operator& materialized() {
+    if (!op)
+      op = OperatorBuilder::on(basis).terms(terms).components(components).checks(checks).build(
+          MatrixFormat::MatrixFree);
     return *op;
+  }
+  // Evaluate a family at the request's parameters (caller holds mu). A model
+  // without components accepts no parameters (src/interop.rs:964-973).
+  Operator& bind(const Value* parameters) {
+    Operator& o = materialized();
+    std::vector<std::pair<std::string, cplx>> named;
+    if (parameters && !parameters->is_null()) {
+      if (parameters->kind != Value::Obj) throw invalid_options("parameters must be an object");
+      for (auto& kv : *parameters->obj) named.emplace_back(kv.first, as_complex(kv.second, "parameters"));
+    }
+    if (!o.parameterized()) {
+      if (!named.empty()) throw invalid_options("unknown operator parameter " + debug_string(named[0].first));
+      return o;
+    }
+    o.evaluate_coefficients(o.resolve_coefficients(named), stream);
+    // PackedOperatorModel checks the evaluated operator (src/interop.rs:589-640)
+    if (!o.hermitian()) throw non_hermitian();
+    return o;
   }
 };
 
-std::shared_ptr<Model> make_model(const Value& basis, const Value& terms, const Value* checks) {
+// TermComponentRequest (capi :783-787)
+std::vector<ComponentSpec> parse_components(const Value* v) {
+  std::vector<ComponentSpec> out;
+  if (!v || v->is_null()) return out;
+  for (auto& c : v->array()) {
+    if (c.kind != Value::Obj) bad("components: expected objects");
+    ComponentSpec cs;
+    const Value& name = need(c, "name", "component");
+    if (name.kind != Value::String) bad("component name must be a string");
+    cs.name = name.raw;
+    cs.terms = parse_terms(need(c, "terms", "component"));
+    if (auto* d = c.get("default"); d && !d->is_null()) {
+      cs.has_default = true;
+      cs.default_value = as_complex(*d, "default");
+    }
+    out.push_back(std::move(cs));
+  }
+  // validate_term_components (src/interop.rs:1308-1326), at model creation
+  std::set<std::string> names;
+  for (const auto& c : out) {
+    if (c.name.empty() || !names.insert(c.name).second)
+      throw invalid_options("component names must be nonempty and unique");
+    if (c.has_default && (!std::isfinite(c.default_value.real()) || !std::isfinite(c.default_value.imag())))
+      throw invalid_options("component defaults must be finite");
+  }
+  return out;
+}
+
+std::shared_ptr<Model> make_model(const Value& basis, const Value& terms, const Value* checks,
+                                  const Value* components = nullptr) {
   auto m = std::make_shared<Model>();
   m->spec = parse_basis(basis);
   m->terms = parse_terms(terms);
+  m->components = parse_components(components);
   m->checks = parse_checks(checks);
   QMBED_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   m->basis = SpinBasis1D::build(m->spec, m->stream);
@@ -345,9 +397,9 @@ std::string eigensystem_body(size_t dim, const Eigensystem& es, bool vectors, bo
   return o;
 }
 
-Eigensystem solve_model(Model& m, const SolverReq& r) {
+Eigensystem solve_model(Model& m, const SolverReq& r, const Value* parameters = nullptr) {
   std::lock_guard<std::mutex> g(m.mu);
-  const Operator& op = m.materialized();
+  const Operator& op = m.bind(parameters);
   if (r.has_initial && r.initial.size() != op.dim())
     throw dimension_mismatch("eigsh initial vector does not match the operator");
   return eigsh(op, r.opts, r.has_initial ? r.initial.data() : nullptr, r.ctl, r.eigenvectors, m.stream);
@@ -395,11 +447,10 @@ std::string run_command(const std::string& req) {
     return ok(o);
   }
   if (op == "create_model") {
-    if (auto* c = v.get("components"); c && !c->array().empty())
-      throw unsupported_backend("parameterized model components are not built on the MI355X path");
     if (auto* p = v.get("site_permutation"); p && !p->is_null())
       throw unsupported_backend("site permutations are not built on the MI355X path");
-    auto m = make_model(need(v, "basis", "create_model"), need(v, "terms", "create_model"), v.get("checks"));
+    auto m = make_model(need(v, "basis", "create_model"), need(v, "terms", "create_model"), v.get("checks"),
+                        v.get("components"));
     uint64_t id = g_next_model.fetch_add(1);
     size_t dim = m->basis->len();
     {
@@ -414,10 +465,8 @@ std::string run_command(const std::string& req) {
     if (h.kind != Value::String) bad("handle must be a string");
     auto m = registered_model(h.raw);
     (void)parse_format(v.get("format"));
-    if (auto* p = v.get("parameters"); p && p->kind == Value::Obj && !p->obj->empty())
-      throw unsupported_backend("parameterized models are not built on the MI355X path");
     SolverReq r = parse_solver(need(v, "solver", "eigsh_model"));
-    Eigensystem es = solve_model(*m, r);
+    Eigensystem es = solve_model(*m, r, v.get("parameters"));
     return ok(eigensystem_body(m->basis->len(), es, r.eigenvectors, true));
   }
   if (op == "evolve_model") {
@@ -436,7 +485,7 @@ std::string run_command(const std::string& req) {
     std::vector<std::vector<cplx>> cols;
     for (auto& c : need(v, "vectors", "evolve_model").array()) cols.push_back(as_cvec(c, "vectors"));
     std::lock_guard<std::mutex> g(m->mu);
-    const Operator& H = m->materialized();
+    const Operator& H = m->bind(v.get("parameters"));
     auto traj = evolve_batch(H, cols, eo, nullptr, m->stream);
     std::string o = "{\"kind\":\"trajectory\",\"dimension\":" + std::to_string(H.dim()) + ",\"times\":";
     write_dvec(o, eo.times);
@@ -464,8 +513,12 @@ std::string run_command(const std::string& req) {
     if (auto* x = v.get("max_degree"); x && !x->is_null()) max_degree = as_long(*x, "max_degree");
     if (auto* x = v.get("tolerance"); x && !x->is_null()) tol = as_double(*x, "tolerance");
     if (auto* x = v.get("max_substeps"); x && !x->is_null()) max_sub = as_long(*x, "max_substeps");
-    if (auto* p = v.get("parameters"); p && p->kind == Value::Obj && !p->obj->empty())
-      throw unsupported_backend("parameterized models are not built on the MI355X path");
+    {
+      std::lock_guard<std::mutex> g(m->mu);
+      if (m->materialized().parameterized())
+        throw unsupported_backend("exponential-action plans of parameterized models are not built on the MI355X path");
+      m->bind(v.get("parameters"));
+    }
     std::shared_ptr<ExpmEntry> e = std::make_shared<ExpmEntry>();
     e->model = m;
     {
@@ -607,7 +660,7 @@ qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error) {
     }
     auto tm = new TypedModel();
     try {
-      tm->m = make_model(need(v, "basis", "model"), need(v, "terms", "model"), v.get("checks"));
+      tm->m = make_model(need(v, "basis", "model"), need(v, "terms", "model"), v.get("checks"), v.get("components"));
       tm->m->materialized();
     } catch (...) {
       delete tm;
@@ -750,13 +803,39 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
       throw invalid_options(std::string("invalid binding request: ") + e.what());
     }
     SolverReq r = parse_solver(v);
-    Eigensystem es = solve_model(*tm->m, r);
+    Eigensystem es = solve_model(*tm->m, r, v.get("parameters"));
     std::string body = eigensystem_body(tm->m->basis->len(), es, r.eigenvectors, false);
     body.pop_back();
     body += ",\"fp32_iterations\":" + std::to_string(es.fp32_iterations) +
             ",\"fp64_iterations\":" + std::to_string(es.fp64_iterations) +
             ",\"reorthogonalization_passes\":" + std::to_string(es.reorthogonalization_passes) +
             ",\"conditional_second_passes\":" + std::to_string(es.conditional_second_passes) + "}";
+    return ok(body);
+  });
+}
+
+char* qmbed_hip_model_set_parameters(qmbed_hip_model* m, const char* parameters_json) {
+  return guarded(parameters_json, [&](const std::string& req) {
+    auto* tm = reinterpret_cast<TypedModel*>(m);
+    json::Value v;
+    try {
+      v = json::parse(req);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    std::lock_guard<std::mutex> g(tm->m->mu);
+    Operator& op = tm->m->bind(&v);
+    std::string body = "{\"kind\":\"coefficients\",\"values\":[";
+    const auto& c = op.coefficients();
+    for (size_t i = 0; i < c.size(); ++i) {
+      if (i) body += ',';
+      body += '[';
+      json::write_number(body, c[i].real());
+      body += ',';
+      json::write_number(body, c[i].imag());
+      body += ']';
+    }
+    body += "],\"real\":" + std::string(op.is_real() ? "true" : "false") + "}";
     return ok(body);
   });
 }

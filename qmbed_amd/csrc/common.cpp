@@ -26,6 +26,29 @@ Error incompatible_symmetry(const std::string& s) {
 }
 Error non_hermitian() { return Error(ErrorKind::NonHermitian, "a Hermitian operator is required"); }
 Error invalid_options(const std::string& s) { return Error(ErrorKind::InvalidOptions, "invalid options: " + s); }
+std::string debug_string(const std::string& s) {
+  std::string out = "\"";
+  for (unsigned char ch : s) {
+    switch (ch) {
+      case '"': out += "\\\""; break;
+      case '\\': out += "\\\\"; break;
+      case '\n': out += "\\n"; break;
+      case '\r': out += "\\r"; break;
+      case '\t': out += "\\t"; break;
+      case '\0': out += "\\0"; break;
+      default:
+        if (ch < 0x20 || ch == 0x7f) {
+          char buf[16];
+          std::snprintf(buf, sizeof buf, "\\u{%x}", unsigned(ch));
+          out += buf;
+        } else {
+          out += char(ch);
+        }
+    }
+  }
+  return out + "\"";
+}
+
 Error dimension_mismatch(const std::string& s) {
   return Error(ErrorKind::DimensionMismatch, "dimension mismatch: " + s);
 }

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <set>
 #include <string>
 
 #include "internal.hpp"
@@ -708,6 +709,7 @@ struct Operator::DeviceTables {
   int n_diag = 0, n_off = 0, n_amp = 0;
   bool complex_coef = false;  // amplitudes or characters complex
   bool hermitian = true;      // sector matrix Hermitian: gather form valid
+  bool family = false;        // parameterized: amplitudes are re-evaluated
   int lpr = 1;                // lanes per row of the H·v kernels (autotuned)
   // image-table mode (autotuned against the orbit-walk mode)
   bool use_img = false;
@@ -771,41 +773,88 @@ static bool sector_matrix_hermitian(const Operator& op, hipStream_t s) {
 
 static void tune_lanes(Operator& op);
 
+// Amplitude tables of the current groups: the gather-form table conj(c·a)
+// (one double per entry for real coefficients, (re, im) otherwise) and the
+// forward table c·a as (re, im). For families the device arrays are sized
+// for (re, im) so that an evaluation may switch between the layouts.
+static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, const std::vector<FlipGroup>& off,
+                              hipStream_t s) {
+  auto& t = *op.tables_;
+  std::vector<double> amp, famp;
+  for (auto* v : {&diag, &off}) {
+    for (auto& g : *v) {
+      for (auto& c : g.amp) {
+        amp.push_back(c.real());
+        if (t.complex_coef) amp.push_back(-c.imag());  // gather form reads conj(c·a)
+        famp.push_back(c.real());
+        famp.push_back(c.imag());
+      }
+    }
+  }
+  const size_t cap = t.family ? 2 * size_t(std::max(t.n_amp, 1)) : std::max<size_t>(amp.size(), 2);
+  if (t.amp.size() < cap) t.amp.resize(cap);
+  if (t.famp.size() < std::max<size_t>(famp.size(), 2)) t.famp.resize(std::max<size_t>(famp.size(), 2));
+  if (!amp.empty()) QMBED_HIP(hipMemcpyAsync(t.amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice, s));
+  if (!famp.empty()) QMBED_HIP(hipMemcpyAsync(t.famp.data(), famp.data(), famp.size() * 8, hipMemcpyHostToDevice, s));
+  QMBED_HIP(hipStreamSynchronize(s));  // the host vectors die here
+}
+
+// The sector matrix of the current tables is Hermitian (checked on the
+// device) — with a family, of each part in turn when `part` >= 0.
+static bool current_hermitian(Operator& op, hipStream_t s) { return sector_matrix_hermitian(op, s); }
+
 std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   if (!basis_) throw invalid_options("operator builder has no basis");
   auto op = std::shared_ptr<Operator>(new Operator());
   op->basis_ = basis_;
   op->format_ = format;
+  const bool family = !components_.empty();
+  if (family) {
+    // QuantumOperator::new (src/operator/mod.rs:2717-2745)
+    std::set<std::string> names;
+    for (const auto& c : components_) {
+      if (c.name.empty() || !names.insert(c.name).second)
+        throw invalid_options("component names must be nonempty and unique");
+      if (c.has_default && (!std::isfinite(c.default_value.real()) || !std::isfinite(c.default_value.imag())))
+        throw invalid_options("component defaults must be finite");
+    }
+  }
   bool real_amp = true;
-  compile_groups(basis_->spec(), terms_, checks_, basis_->len() > 0, op->diag_, op->off_, real_amp);
+  if (family) {
+    std::vector<std::vector<OperatorSpec>> parts{terms_};
+    for (const auto& c : components_) parts.push_back(c.terms);
+    compile_family_groups(basis_->spec(), parts, checks_, basis_->len() > 0, op->diag_, op->off_, real_amp);
+  } else {
+    compile_groups(basis_->spec(), terms_, checks_, basis_->len() > 0, op->diag_, op->off_, real_amp);
+  }
   const SectorGeometry& geo = basis_->geometry();
   op->real_ = real_amp && basis_->real_characters();
+  op->parts_real_ = real_amp;
   auto t = std::make_shared<Operator::DeviceTables>();
+  t->family = family;
   t->n_diag = int(op->diag_.size());
   t->n_off = int(op->off_.size());
   if (t->n_diag + t->n_off > kMaxGroups)
     throw unsupported_backend("the operator has more distinct local actions than the H·v kernel tables hold");
   t->complex_coef = !op->real_;
   std::vector<uint32_t> meta;
-  std::vector<double> amp, famp;
   int n_amp = 0;
   for (auto* v : {&op->diag_, &op->off_}) {
     for (auto& g : *v) {
       uint32_t packed = uint32_t(g.nsites);
       for (int i = 0; i < g.nsites; ++i) packed |= uint32_t(g.sites[i]) << (3 + 5 * i);
+      // nonzero patterns: of the table, or of any part of a family (the
+      // structure must not depend on the coefficients)
       uint32_t nz = 0;
-      for (size_t p = 0; p < g.amp.size(); ++p)
-        if (g.amp[p] != cplx(0, 0)) nz |= 1u << p;
+      for (size_t p = 0; p < g.amp.size(); ++p) {
+        bool on = g.amp[p] != cplx(0, 0);
+        for (auto& part : g.parts) on = on || part[p] != cplx(0, 0);
+        if (on) nz |= 1u << p;
+      }
       meta.push_back(g.flip);
       meta.push_back(packed);
       meta.push_back(nz);
       meta.push_back(uint32_t(n_amp));
-      for (auto& c : g.amp) {
-        amp.push_back(c.real());
-        if (t->complex_coef) amp.push_back(-c.imag());  // gather form reads conj(c·a)
-        famp.push_back(c.real());
-        famp.push_back(c.imag());
-      }
       n_amp += int(g.amp.size());
     }
   }
@@ -830,28 +879,118 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
     chars[0] = 1.0;
   }
   t->meta.resize(std::max<size_t>(meta.size(), 4));
-  t->amp.resize(std::max<size_t>(amp.size(), 2));
-  t->famp.resize(std::max<size_t>(famp.size(), 2));
   t->chars.resize(64);
   if (!meta.empty()) QMBED_HIP(hipMemcpy(t->meta.data(), meta.data(), meta.size() * 4, hipMemcpyHostToDevice));
-  if (!amp.empty()) QMBED_HIP(hipMemcpy(t->amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice));
-  if (!famp.empty()) QMBED_HIP(hipMemcpy(t->famp.data(), famp.data(), famp.size() * 8, hipMemcpyHostToDevice));
   QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), 64 * 8, hipMemcpyHostToDevice));
   op->tables_ = t;
-  // The gather-form kernels read row r as conj(column r), which is the
-  // sector matrix exactly when it is Hermitian; the reference's assembly
-  // checks exactly that (1e-12 per entry) unless checks are disabled.
-  if (basis_->len() > 0) {
-    if (checks_.hermiticity) {
-      if (!sector_matrix_hermitian(*op, nullptr)) throw non_hermitian();
-    } else {
-      t->hermitian = sector_matrix_hermitian(*op, nullptr);
-      if (!t->hermitian) op->real_ = false;
+  if (!family) {
+    upload_amplitudes(*op, op->diag_, op->off_, nullptr);
+    // The gather-form kernels read row r as conj(column r), which is the
+    // sector matrix exactly when it is Hermitian; the reference's assembly
+    // checks exactly that (1e-12 per entry) unless checks are disabled.
+    if (basis_->len() > 0) {
+      if (checks_.hermiticity) {
+        if (!current_hermitian(*op, nullptr)) throw non_hermitian();
+      } else {
+        t->hermitian = current_hermitian(*op, nullptr);
+        if (!t->hermitian) op->real_ = false;
+      }
+      if (t->hermitian) {
+        tune_lanes(*op);
+        op->tuned_ = true;
+      }
     }
-    if (t->hermitian) tune_lanes(*op);
+    return op;
   }
+  // Family: the static part is checked like a plain operator (the reference
+  // materializes it with the model's checks, src/interop.rs:1277-1280); each
+  // component's Hermiticity is recorded (components are built unchecked,
+  // :1256-1260) so that real-coefficient sums of Hermitian parts need no
+  // check when evaluated.
+  op->components_ = components_;
+  const size_t nparts = components_.size() + 1;
+  op->part_hermitian_.assign(nparts, true);
+  if (basis_->len() > 0) {
+    for (size_t part = 0; part < nparts; ++part) {
+      std::vector<FlipGroup> d = op->diag_, o = op->off_;
+      for (auto* v : {&d, &o})
+        for (auto& g : *v) g.amp = g.parts[part];
+      t->complex_coef = !(op->parts_real_ && basis_->real_characters());
+      upload_amplitudes(*op, d, o, nullptr);
+      op->part_hermitian_[part] = current_hermitian(*op, nullptr);
+    }
+    if (checks_.hermiticity && !op->part_hermitian_[0]) throw non_hermitian();
+  }
+  // initial evaluation: defaults, and zero for required components (an
+  // evaluation with their values must come before any use)
+  std::vector<cplx> c(components_.size(), cplx(0, 0));
+  for (size_t p = 0; p < components_.size(); ++p)
+    if (components_[p].has_default) c[p] = components_[p].default_value;
+  op->evaluate_coefficients(c, nullptr);
   return op;
 }
+
+std::vector<cplx> Operator::resolve_coefficients(const std::vector<std::pair<std::string, cplx>>& parameters) const {
+  // src/operator/mod.rs:2775-2812 (and src/interop.rs:964-973 without components)
+  for (const auto& kv : parameters) {
+    bool known = false;
+    for (const auto& c : components_) known = known || c.name == kv.first;
+    if (!known) throw invalid_options("unknown operator parameter " + debug_string(kv.first));
+  }
+  std::vector<cplx> out;
+  for (const auto& c : components_) {
+    const cplx* found = nullptr;
+    for (const auto& kv : parameters)
+      if (kv.first == c.name) found = &kv.second;
+    if (!found && !c.has_default)
+      throw invalid_options("missing required operator parameter " + debug_string(c.name));
+    const cplx v = found ? *found : c.default_value;
+    if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
+      throw invalid_options("operator parameter " + debug_string(c.name) + " must be finite");
+    out.push_back(v);
+  }
+  return out;
+}
+
+void Operator::evaluate_coefficients(const std::vector<cplx>& c, hipStream_t s) {
+  if (!parameterized()) {
+    if (!c.empty()) throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for 0 components");
+    return;
+  }
+  if (c.size() != components_.size())
+    throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for " +
+                             std::to_string(components_.size()) + " components");
+  for (const auto& v : c)
+    if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
+      throw invalid_options("operator coefficients must be finite");
+  bool real_c = true;
+  for (const auto& v : c) real_c = real_c && v.imag() == 0.0;
+  for (auto* v : {&diag_, &off_}) {
+    for (auto& g : *v) {
+      for (size_t pat = 0; pat < g.amp.size(); ++pat) {
+        cplx a = g.parts[0][pat];
+        for (size_t p = 0; p < c.size(); ++p) a += c[p] * g.parts[p + 1][pat];
+        g.amp[pat] = a;
+      }
+    }
+  }
+  coefficients_ = c;
+  auto& t = *tables_;
+  real_ = parts_real_ && real_c && basis_->real_characters();
+  t.complex_coef = !real_;
+  upload_amplitudes(*this, diag_, off_, s);
+  if (basis_->len() == 0) return;
+  bool all_parts = real_c;
+  for (bool h : part_hermitian_) all_parts = all_parts && h;
+  t.hermitian = all_parts ? true : current_hermitian(*this, s);
+  if (!t.hermitian) real_ = false;
+  if (t.hermitian && !tuned_) {
+    tune_lanes(*this);
+    tuned_ = true;
+  }
+}
+
+bool Operator::hermitian() const { return tables_->hermitian; }
 
 namespace {
 

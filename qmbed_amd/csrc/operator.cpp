@@ -156,8 +156,58 @@ static bool preserves_particle_sector(const SpinBasisSpec& spec, const std::vect
 // Group the couplings by flip mask; each table holds c·a(pattern) for the
 // source state's bits, i.e. the column entries the reference's
 // column-streaming assembly generates.
-static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFactors& f, std::vector<FlipGroup>& diag,
-                         std::vector<FlipGroup>& off);
+static void build_groups(const std::vector<OperatorSpec>& terms, const std::vector<int>& part_of, int nparts,
+                         const SpinFactors& f, std::vector<FlipGroup>& diag, std::vector<FlipGroup>& off);
+
+static void check_sites_and_symbols(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& terms) {
+  for (const auto& t : terms) {
+    std::string sym = strip_bars(t.symbols);
+    for (const auto& c : t.couplings) {
+      for (int pos = int(sym.size()) - 1; pos >= 0; --pos) {
+        int site = c.sites[pos];
+        if (site < 0 || site >= spec.sites) throw invalid_site(site, spec.sites);
+        char ch = sym[pos];
+        if (ch != 'I' && ch != 'z' && ch != '+' && ch != '-' && ch != 'x' && ch != 'y')
+          throw invalid_operator(std::string(1, ch));
+      }
+    }
+  }
+}
+
+static bool groups_real(const std::vector<FlipGroup>& diag, const std::vector<FlipGroup>& off) {
+  for (auto* v : {&diag, &off})
+    for (auto& g : *v) {
+      for (auto& a : g.amp)
+        if (a.imag() != 0.0) return false;
+      for (auto& t : g.parts)
+        for (auto& a : t)
+          if (a.imag() != 0.0) return false;
+    }
+  return true;
+}
+
+void compile_family_groups(const SpinBasisSpec& spec, const std::vector<std::vector<OperatorSpec>>& parts,
+                           const AssemblyChecks& checks, bool nonempty, std::vector<FlipGroup>& diag,
+                           std::vector<FlipGroup>& off, bool& real) {
+  std::vector<OperatorSpec> all;
+  std::vector<int> part_of;
+  for (size_t p = 0; p < parts.size(); ++p)
+    for (const auto& t : parts[p]) {
+      all.push_back(t);
+      part_of.push_back(int(p));
+    }
+  validate_terms(all, spec.sites);
+  // the reference checks particle conservation on the static operator and on
+  // every component separately (each is built with the model's checks)
+  if (checks.particle_conservation)
+    for (const auto& terms : parts)
+      if (!preserves_particle_sector(spec, terms))
+        throw invalid_sector("operator sum does not preserve the selected particle sector");
+  SpinFactors f(spec.normalization);
+  if (nonempty) check_sites_and_symbols(spec, all);
+  build_groups(all, part_of, int(parts.size()), f, diag, off);
+  real = groups_real(diag, off);
+}
 
 void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& terms, const AssemblyChecks& checks,
                     bool nonempty, std::vector<FlipGroup>& diag, std::vector<FlipGroup>& off, bool& real) {
@@ -167,34 +217,21 @@ void compile_groups(const SpinBasisSpec& spec, const std::vector<OperatorSpec>& 
   SpinFactors f(spec.normalization);
   // symbol alphabet and sites are validated where the reference's local
   // action would meet them (only when the sector is non-empty)
-  if (nonempty) {
-    for (const auto& t : terms) {
-      std::string sym = strip_bars(t.symbols);
-      for (const auto& c : t.couplings) {
-        for (int pos = int(sym.size()) - 1; pos >= 0; --pos) {
-          int site = c.sites[pos];
-          if (site < 0 || site >= spec.sites) throw invalid_site(site, spec.sites);
-          char ch = sym[pos];
-          if (ch != 'I' && ch != 'z' && ch != '+' && ch != '-' && ch != 'x' && ch != 'y')
-            throw invalid_operator(std::string(1, ch));
-        }
-      }
-    }
-  }
-  build_groups(terms, f, diag, off);
-  real = true;
-  for (auto* v : {&diag, &off})
-    for (auto& g : *v)
-      for (auto& a : g.amp)
-        if (a.imag() != 0.0) real = false;
+  if (nonempty) check_sites_and_symbols(spec, terms);
+  build_groups(terms, std::vector<int>(terms.size(), 0), 0, f, diag, off);
+  real = groups_real(diag, off);
 }
 
-static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFactors& f, std::vector<FlipGroup>& diag,
-                         std::vector<FlipGroup>& off) {
+// nparts > 0 (families): couplings of term i belong to part part_of[i], and
+// every group also keeps one table per part.
+static void build_groups(const std::vector<OperatorSpec>& terms, const std::vector<int>& part_of, int nparts,
+                         const SpinFactors& f, std::vector<FlipGroup>& diag, std::vector<FlipGroup>& off) {
   diag.clear();
   off.clear();
   std::map<std::vector<int>, size_t> diag_index;  // site set -> group
-  for (const auto& t : terms) {
+  for (size_t ti = 0; ti < terms.size(); ++ti) {
+    const auto& t = terms[ti];
+    const int part = part_of[ti];
     std::string sym = strip_bars(t.symbols);
     const std::string& adj = sym;
     for (const auto& c : t.couplings) {
@@ -236,6 +273,7 @@ static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFacto
           g->nsites = int(uniq.size());
           for (size_t u = 0; u < uniq.size(); ++u) g->sites[u] = uniq[u];
           g->amp.assign(size_t(1) << uniq.size(), cplx(0, 0));
+          g->parts.assign(size_t(nparts), g->amp);
           diag_index[uniq] = diag.size() - 1;
         }
       } else {
@@ -247,16 +285,21 @@ static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFacto
           un.erase(std::unique(un.begin(), un.end()), un.end());
           if (un.size() > 5) continue;
           if (int(un.size()) != cand.nsites) {
-            // widen the candidate's table to the union of sites
-            std::vector<cplx> wide(size_t(1) << un.size(), cplx(0, 0));
-            for (size_t pat = 0; pat < wide.size(); ++pat) {
-              int old = 0;
-              for (int i = 0; i < cand.nsites; ++i) {
-                int pos = int(std::lower_bound(un.begin(), un.end(), cand.sites[i]) - un.begin());
-                old |= int((pat >> pos) & 1) << i;
+            // widen the candidate's tables to the union of sites
+            auto widen = [&](const std::vector<cplx>& narrow) {
+              std::vector<cplx> wide(size_t(1) << un.size(), cplx(0, 0));
+              for (size_t pat = 0; pat < wide.size(); ++pat) {
+                int old = 0;
+                for (int i = 0; i < cand.nsites; ++i) {
+                  int pos = int(std::lower_bound(un.begin(), un.end(), cand.sites[i]) - un.begin());
+                  old |= int((pat >> pos) & 1) << i;
+                }
+                wide[pat] = narrow[old];
               }
-              wide[pat] = cand.amp[old];
-            }
+              return wide;
+            };
+            std::vector<cplx> wide = widen(cand.amp);
+            for (auto& t : cand.parts) t = widen(t);
             cand.nsites = int(un.size());
             for (size_t u = 0; u < un.size(); ++u) cand.sites[u] = un[u];
             cand.amp.swap(wide);
@@ -271,6 +314,7 @@ static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFacto
           g->nsites = int(uniq.size());
           for (size_t u = 0; u < uniq.size(); ++u) g->sites[u] = uniq[u];
           g->amp.assign(size_t(1) << uniq.size(), cplx(0, 0));
+          g->parts.assign(size_t(nparts), g->amp);
         }
       }
       // accumulate this coupling's amplitudes into the group's table
@@ -283,13 +327,17 @@ static void build_groups(const std::vector<OperatorSpec>& terms, const SpinFacto
           lp |= int((pat >> pos) & 1) << u;
         }
         g->amp[pat] += local_amp[lp];
+        if (nparts > 0) g->parts[size_t(part)][pat] += local_amp[lp];
       }
     }
   }
-  // drop groups whose amplitudes all vanish
+  // drop groups whose amplitudes all vanish (in every part, for families)
   auto empty = [](const FlipGroup& g) {
     for (auto& a : g.amp)
       if (a != cplx(0, 0)) return false;
+    for (auto& t : g.parts)
+      for (auto& a : t)
+        if (a != cplx(0, 0)) return false;
     return true;
   };
   diag.erase(std::remove_if(diag.begin(), diag.end(), empty), diag.end());

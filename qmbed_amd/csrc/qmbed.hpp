@@ -56,6 +56,8 @@ Error incompatible_symmetry(const std::string& s);
 Error non_hermitian();
 Error invalid_options(const std::string& s);
 Error dimension_mismatch(const std::string& s);
+// a string as Rust's `{:?}` prints it (quoted, with escapes) for error texts
+std::string debug_string(const std::string& s);
 Error unsupported_backend(const std::string& s);
 Error internal_state(const std::string& s);
 Error non_convergence(long iterations, double residual);
@@ -182,6 +184,17 @@ struct OperatorSpec {
   std::vector<Coupling> couplings;
 };
 
+// A named, parameterized part of an operator family: H(c) = H_static +
+// Σ_p c_p H_p (QMBED bindings/capi/src/lib.rs:783-787 TermComponentRequest,
+// src/operator/mod.rs:2663-2690 QuantumComponent). Without a default the
+// component's coefficient is required at every evaluation.
+struct ComponentSpec {
+  std::string name;
+  std::vector<OperatorSpec> terms;
+  bool has_default = false;
+  cplx default_value{0.0, 0.0};
+};
+
 struct AssemblyChecks {
   bool hermiticity = true;
   bool particle_conservation = true;
@@ -198,6 +211,9 @@ struct FlipGroup {
   int nsites = 0;
   int sites[5] = {0, 0, 0, 0, 0};
   std::vector<cplx> amp;  // 1 << nsites entries
+  // operator families: the table of every part (static first, then one per
+  // component) with unit coefficients; amp = Σ_part coefficient · parts[part]
+  std::vector<std::vector<cplx>> parts;
 };
 
 // Matrix-free Hermitian operator on one spin sector, applied on the fly by
@@ -237,6 +253,23 @@ class Operator {
   // dense materialization (row-major, complex) on the device
   void materialize_dense(cplx* dense, hipStream_t s) const;
 
+  // Operator families (QMBED src/operator/mod.rs:2710-2830, 3140-3230).
+  bool parameterized() const { return !components_.empty(); }
+  const std::vector<ComponentSpec>& components() const { return components_; }
+  // named parameters → coefficients in component order (the reference's
+  // resolve_coefficients: unknown names, missing required ones and
+  // non-finite values are errors)
+  std::vector<cplx> resolve_coefficients(const std::vector<std::pair<std::string, cplx>>& parameters) const;
+  // Rebind the family to H_static + Σ_p c_p H_p: only the amplitude tables
+  // change (basis, lookup and image tables are shared). Hermiticity of the
+  // sum is known without a check when every part was Hermitian at build and
+  // all c_p are real; otherwise the sector matrix is checked exactly.
+  // Applies already queued on other streams must have completed.
+  void evaluate_coefficients(const std::vector<cplx>& c, hipStream_t s);
+  const std::vector<cplx>& coefficients() const { return coefficients_; }
+  // the current sector matrix is Hermitian (the gather-form kernels apply)
+  bool hermitian() const;
+
   friend class OperatorBuilder;
   struct DeviceTables;  // defined in hv_kernels.hip
   std::shared_ptr<DeviceTables> tables_;
@@ -246,6 +279,12 @@ class Operator {
   std::vector<FlipGroup> diag_, off_;
   bool real_ = true;
   MatrixFormat format_ = MatrixFormat::MatrixFree;
+  // families
+  std::vector<ComponentSpec> components_;
+  std::vector<cplx> coefficients_;
+  std::vector<bool> part_hermitian_;  // static, then components
+  bool parts_real_ = true;            // every part table real
+  bool tuned_ = false;
 };
 
 // OperatorBuilder: QMBED src/operator/mod.rs:1941-2216.
@@ -268,11 +307,17 @@ class OperatorBuilder {
     checks_ = c;
     return *this;
   }
+  // parameterized components (names nonempty and unique, defaults finite)
+  OperatorBuilder& components(const std::vector<ComponentSpec>& c) {
+    components_.insert(components_.end(), c.begin(), c.end());
+    return *this;
+  }
   std::shared_ptr<Operator> build(MatrixFormat format = MatrixFormat::MatrixFree) const;
 
  private:
   std::shared_ptr<const SpinBasis1D> basis_;
   std::vector<OperatorSpec> terms_;
+  std::vector<ComponentSpec> components_;
   AssemblyChecks checks_;
 };
 
