@@ -98,6 +98,42 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json
  * call, which eigsh_json also accepts). */
 char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* parameters_json);
 
+/* ---- Learned Lanczos warm start for parameter scans ----
+ * No counterpart in the reference (its scan reuse is EigshWorkspace's
+ * balanced subspace start, src/solve/mod.rs:477-526 and :2419-2444). A scan
+ * is bound to a model created with "components" (one real coefficient per
+ * component is one scan coordinate).
+ * options_json: {"snapshots":32,"hidden":[64,64],"epochs":4000,
+ * "learning_rate":0.002,"seed":0}, any subset, NULL for all defaults. */
+typedef struct qmbed_hip_scan qmbed_hip_scan;
+qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
+void qmbed_hip_scan_free(qmbed_hip_scan* scan);
+/* Solve the n training points (row-major n × components) with the plain
+ * start and solver_json (as qmbed_hip_eigsh_json; eigenpairs is taken as 1),
+ * select the snapshot ground states and train the FP32 network on the GPU.
+ * Answers a JSON report (energies, iterations, snapshot indices, timings,
+ * first/last training loss). */
+char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json);
+/* Ground states of n points, from the network's start (warm != 0) or the
+ * plain deterministic start. Answers energies, residuals, iterations
+ * (total, FP32-matvec and FP64), per-point seconds and fallback count. */
+char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);
+/* Network outputs (n × snapshots, row-major) of n points; returns the
+ * snapshot count, or -1 on failure. */
+int qmbed_hip_scan_predict(qmbed_hip_scan* scan, const double* points, int n, double* coefficients);
+
+/* The FP32 network kernels themselves (host buffers; for checks against a
+ * CPU implementation). Shape in → h1 (tanh) → h2 (tanh) → out, parameters
+ * W1[h1][in], b1, W2[h2][h1], b2, W3[out][h2], b3; loss 1/(2n) Σ‖y − t‖².
+ * mlp_train with epochs == 0 writes one gradient into grad and the loss
+ * into loss_history[0]; with epochs > 0 it runs full-batch Adam, updates
+ * params in place and writes every epoch's loss into loss_history
+ * (optional). Return 0 on success, -1 on failure. */
+int qmbed_hip_mlp_parameter_count(int in, int h1, int h2, int out);
+int qmbed_hip_mlp_forward(int in, int h1, int h2, int out, const float* params, const float* x, int n, float* y);
+int qmbed_hip_mlp_train(int in, int h1, int h2, int out, float* params, const float* x, const float* t, int n,
+                        int epochs, float learning_rate, float* grad, float* loss_history);
+
 void* qmbed_hip_malloc(size_t bytes);
 void qmbed_hip_free(void* ptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device */

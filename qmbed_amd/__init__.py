@@ -71,6 +71,18 @@ def library():
     L.qmbed_hip_eigsh_json.argtypes = [c_void_p, c_char_p]
     L.qmbed_hip_model_set_parameters.restype = c_void_p
     L.qmbed_hip_model_set_parameters.argtypes = [c_void_p, c_char_p]
+    L.qmbed_hip_scan_create.restype = c_void_p
+    L.qmbed_hip_scan_create.argtypes = [c_void_p, c_char_p, ctypes.POINTER(c_void_p)]
+    L.qmbed_hip_scan_free.argtypes = [c_void_p]
+    for f in (L.qmbed_hip_scan_train, L.qmbed_hip_scan_solve):
+        f.restype = c_void_p
+    L.qmbed_hip_scan_train.argtypes = [c_void_p, c_void_p, c_int, c_char_p]
+    L.qmbed_hip_scan_solve.argtypes = [c_void_p, c_void_p, c_int, c_char_p, c_int]
+    L.qmbed_hip_scan_predict.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
+    L.qmbed_hip_mlp_parameter_count.argtypes = [c_int, c_int, c_int, c_int]
+    L.qmbed_hip_mlp_forward.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p]
+    L.qmbed_hip_mlp_train.argtypes = [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                      ctypes.c_float, c_void_p, c_void_p]
     L.qmbed_hip_malloc.restype = c_void_p
     L.qmbed_hip_malloc.argtypes = [ctypes.c_size_t]
     L.qmbed_hip_free.argtypes = [c_void_p]
@@ -282,6 +294,96 @@ def _call_model(h, req):
     if response.get("status") != "ok":
         raise QmbedError(response.get("error", "unknown error"))
     return response["result"]
+
+
+def _envelope(ptr):
+    response = json.loads(_take(ptr))
+    if response.get("status") != "ok":
+        raise QmbedError(response.get("error", "unknown error"))
+    return response["result"]
+
+
+class Scan:
+    """Learned Lanczos warm start over a family's parameter space (typed path):
+    train on points solved with the plain start, then solve points from the
+    network's prediction (warm=True) or the plain start (warm=False)."""
+
+    def __init__(self, model: "Model", **options):
+        err = ctypes.c_void_p()
+        h = library().qmbed_hip_scan_create(model.handle, json.dumps(options).encode(), ctypes.byref(err))
+        if not h:
+            raise QmbedError(_take(err.value) if err.value else "scan creation failed")
+        self._h = h
+        self._model = model  # the scan re-evaluates this model
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qmbed_hip_scan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def train(self, points, **solver):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        req = {"eigenpairs": 1, "target": {"kind": "smallest_algebraic"}, **solver}
+        return _envelope(library().qmbed_hip_scan_train(self._h, pts.ctypes.data, pts.shape[0],
+                                                        json.dumps(req).encode()))
+
+    def solve(self, points, warm=True, **solver):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        req = {"eigenpairs": 1, "target": {"kind": "smallest_algebraic"}, **solver}
+        return _envelope(library().qmbed_hip_scan_solve(self._h, pts.ctypes.data, pts.shape[0],
+                                                        json.dumps(req).encode(), 1 if warm else 0))
+
+    def predict(self, points):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros((pts.shape[0], 64))
+        k = library().qmbed_hip_scan_predict(self._h, pts.ctypes.data, pts.shape[0], out.ctypes.data)
+        if k < 0:
+            raise QmbedError("prediction failed")
+        return out.ravel()[: pts.shape[0] * k].reshape(pts.shape[0], k)
+
+
+def mlp_forward(shape, params, x):
+    """Network outputs on the GPU (FP32)."""
+    i, h1, h2, o = shape
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((x.shape[0], o), dtype=np.float32)
+    if library().qmbed_hip_mlp_forward(i, h1, h2, o, p.ctypes.data, x.ctypes.data, x.shape[0], y.ctypes.data) != 0:
+        raise QmbedError("mlp forward failed")
+    return y
+
+
+def mlp_gradient(shape, params, x, t):
+    """(gradient, loss) of one evaluation by the GPU training kernel."""
+    i, h1, h2, o = shape
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros_like(p)
+    loss = np.zeros(1, dtype=np.float32)
+    if library().qmbed_hip_mlp_train(i, h1, h2, o, p.ctypes.data, x.ctypes.data, t.ctypes.data, x.shape[0], 0,
+                                     0.0, g.ctypes.data, loss.ctypes.data) != 0:
+        raise QmbedError("mlp gradient failed")
+    return g, float(loss[0])
+
+
+def mlp_train(shape, params, x, t, epochs, lr):
+    """(trained parameters, loss of every epoch) from the GPU training kernel."""
+    i, h1, h2, o = shape
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    hist = np.zeros(epochs, dtype=np.float32)
+    if library().qmbed_hip_mlp_train(i, h1, h2, o, p.ctypes.data, x.ctypes.data, t.ctypes.data, x.shape[0], epochs,
+                                     float(lr), None, hist.ctypes.data) != 0:
+        raise QmbedError("mlp training failed")
+    return p, hist
 
 
 class DeviceBuffer:

@@ -16,6 +16,7 @@
 #include "../../include/qmbed_hip.h"
 #include "internal.hpp"
 #include "json.hpp"
+#include "scan.hpp"
 
 namespace qmbed {
 
@@ -612,6 +613,28 @@ struct TypedModel {
   std::shared_ptr<Model> m;
 };
 
+struct TypedScan {
+  std::shared_ptr<Model> m;
+  std::unique_ptr<WarmStartScan> scan;
+};
+
+json::Value parse_or_throw(const std::string& req) {
+  try {
+    return json::parse(req);
+  } catch (json::ParseError& e) {
+    throw invalid_options(std::string("invalid binding request: ") + e.what());
+  }
+}
+
+void write_long_array(std::string& o, const std::vector<long>& v) {
+  o += '[';
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (i) o += ',';
+    o += std::to_string(v[i]);
+  }
+  o += ']';
+}
+
 }  // namespace qmbed
 
 using namespace qmbed;
@@ -838,6 +861,168 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* m, const char* parameters_
     body += "],\"real\":" + std::string(op.is_real() ? "true" : "false") + "}";
     return ok(body);
   });
+}
+
+qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error) {
+  try {
+    if (!model) throw invalid_options("model pointer is null");
+    auto* tm = reinterpret_cast<TypedModel*>(model);
+    ScanOptions o;
+    if (options_json) {
+      json::Value v = parse_or_throw(options_json);
+      if (v.kind != json::Value::Obj) bad("scan options must be an object");
+      if (auto* x = v.get("snapshots"); x && !x->is_null()) o.snapshots = int(as_long(*x, "snapshots"));
+      if (auto* x = v.get("hidden"); x && !x->is_null()) {
+        auto& a = x->array();
+        if (a.size() != 2) bad("hidden must hold two layer widths");
+        o.hidden1 = int(as_long(a[0], "hidden"));
+        o.hidden2 = int(as_long(a[1], "hidden"));
+      }
+      if (auto* x = v.get("epochs"); x && !x->is_null()) o.epochs = int(as_long(*x, "epochs"));
+      if (auto* x = v.get("learning_rate"); x && !x->is_null()) o.learning_rate = float(as_double(*x, "learning_rate"));
+      if (auto* x = v.get("seed"); x && !x->is_null()) o.seed = as_u64(*x, "seed");
+    }
+    auto ts = std::make_unique<TypedScan>();
+    ts->m = tm->m;
+    std::lock_guard<std::mutex> g(ts->m->mu);
+    ts->scan = std::make_unique<WarmStartScan>(ts->m->materialized(), ts->m->stream, o);
+    if (error) *error = nullptr;
+    return reinterpret_cast<qmbed_hip_scan*>(ts.release());
+  } catch (Error& e) {
+    if (error) *error = dup_c(e.message);
+  } catch (std::exception& e) {
+    if (error) *error = dup_c(std::string("internal state error: ") + e.what());
+  }
+  return nullptr;
+}
+
+void qmbed_hip_scan_free(qmbed_hip_scan* scan) { delete reinterpret_cast<TypedScan*>(scan); }
+
+char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json) {
+  return guarded(solver_json, [&](const std::string& req) {
+    auto* ts = reinterpret_cast<TypedScan*>(scan);
+    if (!points) throw invalid_options("points pointer is null");
+    SolverReq r = parse_solver(parse_or_throw(req));
+    std::lock_guard<std::mutex> g(ts->m->mu);
+    auto rep = ts->scan->train(points, n, r.opts, r.ctl);
+    std::string o = "{\"kind\":\"scan_training\",\"points\":" + std::to_string(n) +
+                    ",\"snapshots\":" + std::to_string(rep.snapshots) + ",\"energies\":";
+    write_dvec(o, rep.energies);
+    o += ",\"iterations\":";
+    write_long_array(o, rep.iterations);
+    o += ",\"snapshot_indices\":";
+    write_long_array(o, std::vector<long>(rep.snapshot_indices.begin(), rep.snapshot_indices.end()));
+    o += ",\"solve_seconds\":";
+    json::write_number(o, rep.solve_seconds);
+    o += ",\"gram_seconds\":";
+    json::write_number(o, rep.gram_seconds);
+    o += ",\"train_seconds\":";
+    json::write_number(o, rep.train_seconds);
+    o += ",\"loss_first\":";
+    json::write_number(o, rep.loss_first);
+    o += ",\"loss_last\":";
+    json::write_number(o, rep.loss_last);
+    o += ",\"target_norm2\":";
+    json::write_number(o, rep.target_norm2);
+    o += "}";
+    return ok(o);
+  });
+}
+
+char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm) {
+  return guarded(solver_json, [&](const std::string& req) {
+    auto* ts = reinterpret_cast<TypedScan*>(scan);
+    if (!points) throw invalid_options("points pointer is null");
+    SolverReq r = parse_solver(parse_or_throw(req));
+    std::lock_guard<std::mutex> g(ts->m->mu);
+    std::vector<double> e, res, secs;
+    std::vector<long> it, it32, it64;
+    long warm_n = 0, fallbacks = 0;
+    for (int i = 0; i < n; ++i) {
+      auto pr = ts->scan->solve(points + size_t(i) * ts->scan->parameters(), r.opts, r.ctl, warm != 0);
+      e.push_back(pr.es.eigenvalues[0]);
+      res.push_back(pr.es.residuals[0]);
+      secs.push_back(pr.seconds);
+      it.push_back(pr.es.iterations);
+      it32.push_back(pr.es.fp32_iterations);
+      it64.push_back(pr.es.fp64_iterations);
+      warm_n += pr.warm ? 1 : 0;
+      fallbacks += pr.fallback ? 1 : 0;
+    }
+    std::string o = "{\"kind\":\"scan_solve\",\"points\":" + std::to_string(n) + ",\"energies\":";
+    write_dvec(o, e);
+    o += ",\"residuals\":";
+    write_dvec(o, res);
+    o += ",\"seconds\":";
+    write_dvec(o, secs);
+    o += ",\"iterations\":";
+    write_long_array(o, it);
+    o += ",\"fp32_iterations\":";
+    write_long_array(o, it32);
+    o += ",\"fp64_iterations\":";
+    write_long_array(o, it64);
+    o += ",\"warm_starts\":" + std::to_string(warm_n) + ",\"fallbacks\":" + std::to_string(fallbacks) + "}";
+    return ok(o);
+  });
+}
+
+int qmbed_hip_scan_predict(qmbed_hip_scan* scan, const double* points, int n, double* coefficients) {
+  try {
+    auto* ts = reinterpret_cast<TypedScan*>(scan);
+    std::lock_guard<std::mutex> g(ts->m->mu);
+    auto y = ts->scan->predict(points, n);
+    std::memcpy(coefficients, y.data(), y.size() * sizeof(double));
+    return ts->scan->snapshots();
+  } catch (...) {
+    return -1;
+  }
+}
+
+int qmbed_hip_mlp_parameter_count(int in, int h1, int h2, int out) {
+  try {
+    return mlp_parameter_count(in, h1, h2, out);
+  } catch (...) {
+    return -1;
+  }
+}
+
+int qmbed_hip_mlp_forward(int in, int h1, int h2, int out, const float* params, const float* x, int n, float* y) {
+  try {
+    const int np = mlp_parameter_count(in, h1, h2, out);
+    DeviceArray<float> dp(static_cast<size_t>(np)), dx(static_cast<size_t>(n) * in), dy(static_cast<size_t>(n) * out);
+    QMBED_HIP(hipMemcpy(dp.data(), params, size_t(np) * 4, hipMemcpyHostToDevice));
+    QMBED_HIP(hipMemcpy(dx.data(), x, size_t(n) * in * 4, hipMemcpyHostToDevice));
+    mlp_forward_device(in, h1, h2, out, dp.data(), dx.data(), n, dy.data(), nullptr);
+    QMBED_HIP(hipMemcpy(y, dy.data(), size_t(n) * out * 4, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int qmbed_hip_mlp_train(int in, int h1, int h2, int out, float* params, const float* x, const float* t, int n,
+                        int epochs, float learning_rate, float* grad, float* loss_history) {
+  try {
+    const int np = mlp_parameter_count(in, h1, h2, out);
+    DeviceArray<float> dp(static_cast<size_t>(np)), dx(static_cast<size_t>(n) * in), dt(static_cast<size_t>(n) * out),
+        dg(static_cast<size_t>(np)), dl(1), dh(static_cast<size_t>(std::max(epochs, 1)));
+    QMBED_HIP(hipMemcpy(dp.data(), params, size_t(np) * 4, hipMemcpyHostToDevice));
+    QMBED_HIP(hipMemcpy(dx.data(), x, size_t(n) * in * 4, hipMemcpyHostToDevice));
+    QMBED_HIP(hipMemcpy(dt.data(), t, size_t(n) * out * 4, hipMemcpyHostToDevice));
+    mlp_train_device(in, h1, h2, out, dp.data(), dx.data(), dt.data(), n, epochs, learning_rate,
+                     epochs == 0 ? dg.data() : nullptr, dl.data(), loss_history ? dh.data() : nullptr, nullptr);
+    QMBED_HIP(hipDeviceSynchronize());
+    if (epochs == 0) {
+      if (grad) QMBED_HIP(hipMemcpy(grad, dg.data(), size_t(np) * 4, hipMemcpyDeviceToHost));
+      if (loss_history) QMBED_HIP(hipMemcpy(loss_history, dl.data(), 4, hipMemcpyDeviceToHost));
+    } else {
+      QMBED_HIP(hipMemcpy(params, dp.data(), size_t(np) * 4, hipMemcpyDeviceToHost));
+      if (loss_history) QMBED_HIP(hipMemcpy(loss_history, dh.data(), size_t(epochs) * 4, hipMemcpyDeviceToHost));
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
 }
 
 void* qmbed_hip_malloc(size_t bytes) {

@@ -768,6 +768,12 @@ Eigensystem eigsh(const Operator& op, const EigshOptions& o, const cplx* initial
   return es;
 }
 
+Eigensystem eigsh_on_device(const Operator& op, const EigshOptions& o, const double* d_initial, bool initial_complex,
+                            const EigshControls& ctl, DeviceArray<double>& vectors, bool& complex_vectors,
+                            hipStream_t s) {
+  return eigsh_impl(op, o, d_initial, initial_complex, ctl, vectors, complex_vectors, s);
+}
+
 Eigensystem eigsh_device(const Operator& op, const EigshOptions& o, const double* d_initial, const EigshControls& ctl,
                          double* d_vectors_out, hipStream_t s) {
   DeviceArray<double> vecs;

@@ -403,6 +403,14 @@ Eigensystem eigsh(const Operator& op, const EigshOptions& opts, const cplx* init
 Eigensystem eigsh_device(const Operator& op, const EigshOptions& opts, const double* d_initial,
                          const EigshControls& ctl, double* d_vectors_out, hipStream_t s);
 
+// Device start vector (interleaved complex when initial_complex or the
+// operator is complex; nullptr: deterministic start); the eigenvectors stay
+// on the device in `vectors` (eigenpairs columns, complex when
+// complex_vectors).
+Eigensystem eigsh_on_device(const Operator& op, const EigshOptions& opts, const double* d_initial,
+                            bool initial_complex, const EigshControls& ctl, DeviceArray<double>& vectors,
+                            bool& complex_vectors, hipStream_t s);
+
 // Krylov time evolution: QMBED src/solve/mod.rs:2454-3454.
 struct EvolutionOptions {
   std::vector<double> times;

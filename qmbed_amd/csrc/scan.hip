@@ -1,0 +1,365 @@
+// Learned Lanczos warm start for parameter scans of an operator family
+// H(c) = H_static + Σ_p c_p H_p (no counterpart in the reference, whose scan
+// reuse is EigshWorkspace's balanced subspace start, src/solve/mod.rs:477-526
+// and :2419-2444).
+//
+// Training: the training points are solved by plain Lanczos; their ground
+// states stay on the device, gauge-fixed (real: sign of Σ_i ψ_i, or of the
+// largest entry when that sum vanishes). One GEMM forms the Gram matrix
+// G = Ψᵀ Ψ. Pivoted Cholesky of G (host, n × n) selects the snapshot states
+// greedily (each pivot is the state farthest from the span of those before)
+// and gives G ≈ L Lᵀ; the snapshot basis S = Ψ_sel R⁻¹ (R = L[sel]ᵀ) is
+// orthonormal and row i of L holds the coefficients of ψ_i over S, which are
+// the MLP's targets. The MLP (mlp.hip) maps the normalized coefficients c to
+// those snapshot coefficients and is trained on the device.
+//
+// Solving a point: MLP(c) → coefficients y, start vector v₀ = S y, Lanczos
+// from v₀. A vanishing prediction falls back to the plain deterministic
+// start; the Lanczos residual test decides convergence either way.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "internal.hpp"
+#include "lanczos_kernels.hpp"
+#include "scan.hpp"
+
+namespace qmbed {
+namespace {
+
+constexpr int kT = 256;
+
+// partial[b] = Σ x over block b's rows
+__global__ __launch_bounds__(kT) void k_sum_blocks(const double* __restrict__ x, size_t n, double* partial) {
+  __shared__ double sh[kT / 64];
+  double s = 0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) s += x[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < kT / 64; ++w) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// per-block (max |x|, signed value at that max) for the fallback gauge
+__global__ __launch_bounds__(kT) void k_absmax_blocks(const double* __restrict__ x, size_t n, double* amax,
+                                                      double* aval) {
+  __shared__ double shm[kT], shv[kT];
+  double m = -1.0, v = 0.0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    const double a = fabs(x[i]);
+    if (a > m) {
+      m = a;
+      v = x[i];
+    }
+  }
+  shm[threadIdx.x] = m;
+  shv[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = -1.0, bv = 0.0;
+    for (int t = 0; t < kT; ++t)
+      if (shm[t] > bm) {
+        bm = shm[t];
+        bv = shv[t];
+      }
+    amax[blockIdx.x] = bm;
+    aval[blockIdx.x] = bv;
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_scale_copy(const double* __restrict__ x, size_t n, double f,
+                                                   double* __restrict__ y) {
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) y[i] = f * x[i];
+}
+
+__global__ void k_f2d(const float* __restrict__ a, int n, double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double(a[i]);
+}
+
+int grid_for(size_t n) { return int(std::min<size_t>(1024, std::max<size_t>(1, (n + kT * 4 - 1) / (kT * 4)))); }
+
+// deterministic uniform numbers in [-1, 1) (64-bit LCG, top 53 bits)
+struct Lcg {
+  uint64_t x;
+  explicit Lcg(uint64_t seed) : x(seed * 2 + 1) {}
+  double next() {
+    x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+    return double(x >> 11) / 4503599627370496.0 - 1.0;
+  }
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+WarmStartScan::WarmStartScan(Operator& family, hipStream_t s, const ScanOptions& o) : op_(family), s_(s), opt_(o) {
+  if (!op_.parameterized()) throw invalid_options("a parameter scan needs a model with components");
+  if (o.snapshots < 1 || o.snapshots > 64) throw invalid_options("snapshots must lie in [1, 64]");
+  if (o.epochs < 1) throw invalid_options("epochs must be positive");
+  if (!(o.learning_rate > 0.0f)) throw invalid_options("learning_rate must be positive");
+  P_ = int(op_.components().size());
+  if (P_ > 8) throw invalid_options("the warm-start network takes at most 8 parameters");
+  (void)mlp_parameter_count(P_, o.hidden1, o.hidden2, o.snapshots);
+}
+
+std::vector<cplx> WarmStartScan::coefficients_of(const double* point) const {
+  std::vector<cplx> c(static_cast<size_t>(P_));
+  for (int p = 0; p < P_; ++p) {
+    if (!std::isfinite(point[p])) throw invalid_options("scan points must be finite");
+    c[size_t(p)] = cplx(point[p], 0.0);
+  }
+  return c;
+}
+
+// gauge of a real ground state: sign of Σ ψ_i, or of the largest entry
+double WarmStartScan::gauge_sign(const double* d_psi) {
+  const size_t n = op_.dim();
+  const int nb = grid_for(n);
+  DeviceArray<double> part(nb), amax(nb), aval(nb);
+  hipLaunchKernelGGL(k_sum_blocks, dim3(nb), dim3(kT), 0, s_, d_psi, n, part.data());
+  hipLaunchKernelGGL(k_absmax_blocks, dim3(nb), dim3(kT), 0, s_, d_psi, n, amax.data(), aval.data());
+  std::vector<double> hp(nb), hm(nb), hv(nb);
+  QMBED_HIP(hipMemcpyAsync(hp.data(), part.data(), nb * 8, hipMemcpyDeviceToHost, s_));
+  QMBED_HIP(hipMemcpyAsync(hm.data(), amax.data(), nb * 8, hipMemcpyDeviceToHost, s_));
+  QMBED_HIP(hipMemcpyAsync(hv.data(), aval.data(), nb * 8, hipMemcpyDeviceToHost, s_));
+  QMBED_HIP(hipStreamSynchronize(s_));
+  double sum = 0;
+  for (double v : hp) sum += v;
+  if (std::fabs(sum) > 1.0e-6) return sum < 0 ? -1.0 : 1.0;
+  double bm = -1.0, bv = 1.0;
+  for (int b = 0; b < nb; ++b)
+    if (hm[size_t(b)] > bm) {
+      bm = hm[size_t(b)];
+      bv = hv[size_t(b)];
+    }
+  return bv < 0 ? -1.0 : 1.0;
+}
+
+WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, const EigshOptions& o,
+                                                const EigshControls& ctl) {
+  if (n < 1) throw invalid_options("training needs at least one point");
+  if (!op_.is_real() || !op_.basis().real_characters())
+    throw unsupported_backend("the learned warm start is built for real-valued families");
+  TrainReport rep;
+  const size_t N = op_.dim();
+  auto t0 = std::chrono::steady_clock::now();
+  // 1. ground states of the training points (plain start)
+  DeviceArray<double> psi(N * size_t(n));
+  for (int i = 0; i < n; ++i) {
+    op_.evaluate_coefficients(coefficients_of(points + size_t(i) * P_), s_);
+    if (!op_.hermitian()) throw non_hermitian();
+    DeviceArray<double> vecs;
+    bool cv = false;
+    EigshOptions oo = o;
+    oo.eigenpairs = 1;
+    Eigensystem es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
+    if (cv) throw unsupported_backend("the learned warm start is built for real-valued families");
+    const double sign = gauge_sign(vecs.data());
+    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(N)), dim3(kT), 0, s_, vecs.data(), N, sign,
+                       psi.data() + N * size_t(i));
+    QMBED_HIP(hipStreamSynchronize(s_));  // vecs dies here
+    rep.energies.push_back(es.eigenvalues[0]);
+    rep.iterations.push_back(es.iterations);
+  }
+  rep.solve_seconds = seconds_since(t0);
+  // 2. Gram matrix G = Ψᵀ Ψ
+  auto t1 = std::chrono::steady_clock::now();
+  DeviceArray<double> dG(size_t(n) * size_t(n));
+  {
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
+    rocblas_set_stream(h, s_);
+    const double one = 1.0, zero = 0.0;
+    rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, rocblas_int(N),
+                                      &one, psi.data(), rocblas_int(N), psi.data(), rocblas_int(N), &zero, dG.data(), n);
+    QMBED_HIP(hipStreamSynchronize(s_));
+    rocblas_destroy_handle(h);
+    if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
+  }
+  std::vector<double> G(size_t(n) * size_t(n));
+  QMBED_HIP(hipMemcpy(G.data(), dG.data(), G.size() * 8, hipMemcpyDeviceToHost));
+  // 3. pivoted Cholesky: G ≈ L Lᵀ over the selected pivots (column-major G)
+  const int want = std::min(opt_.snapshots, n);
+  std::vector<double> d(static_cast<size_t>(n)), Lm(size_t(n) * size_t(want), 0.0);  // Lm[i + n*k]
+  double dmax0 = 0;
+  for (int i = 0; i < n; ++i) {
+    d[size_t(i)] = G[size_t(i) + size_t(n) * size_t(i)];
+    dmax0 = std::max(dmax0, d[size_t(i)]);
+  }
+  std::vector<int> sel;
+  std::vector<char> used(static_cast<size_t>(n), 0);
+  for (int k = 0; k < want; ++k) {
+    int j = -1;
+    double best = -1.0;
+    for (int i = 0; i < n; ++i)
+      if (!used[size_t(i)] && d[size_t(i)] > best) {
+        best = d[size_t(i)];
+        j = i;
+      }
+    if (j < 0 || best <= 1.0e-10 * dmax0) break;
+    used[size_t(j)] = 1;
+    sel.push_back(j);
+    const double piv = std::sqrt(best);
+    for (int i = 0; i < n; ++i) {
+      if (used[size_t(i)] && i != j) continue;
+      double v = G[size_t(i) + size_t(n) * size_t(j)];
+      for (int l = 0; l < k; ++l) v -= Lm[size_t(i) + size_t(n) * size_t(l)] * Lm[size_t(j) + size_t(n) * size_t(l)];
+      Lm[size_t(i) + size_t(n) * size_t(k)] = (i == j) ? piv : v / piv;
+    }
+    for (int i = 0; i < n; ++i) {
+      if (used[size_t(i)]) continue;
+      const double l = Lm[size_t(i) + size_t(n) * size_t(k)];
+      d[size_t(i)] -= l * l;
+    }
+  }
+  ns_ = int(sel.size());
+  if (ns_ == 0) throw internal_state("the training ground states span no snapshot");
+  // R (ns × ns, upper): R[a][b] = L[sel[b]][a]; S = Ψ_sel R⁻¹
+  std::vector<double> R(size_t(ns_) * size_t(ns_), 0.0), Rinv(size_t(ns_) * size_t(ns_), 0.0);  // column-major
+  for (int b = 0; b < ns_; ++b)
+    for (int a = 0; a <= b; ++a) R[size_t(a) + size_t(ns_) * size_t(b)] = Lm[size_t(sel[size_t(b)]) + size_t(n) * size_t(a)];
+  for (int col = 0; col < ns_; ++col) {  // solve R x = e_col (back substitution)
+    for (int row = ns_ - 1; row >= 0; --row) {
+      double v = (row == col) ? 1.0 : 0.0;
+      for (int k = row + 1; k < ns_; ++k) v -= R[size_t(row) + size_t(ns_) * size_t(k)] * Rinv[size_t(k) + size_t(ns_) * size_t(col)];
+      Rinv[size_t(row) + size_t(ns_) * size_t(col)] = v / R[size_t(row) + size_t(ns_) * size_t(row)];
+    }
+  }
+  {
+    DeviceArray<double> Psel(N * size_t(ns_)), dR(Rinv.size());
+    for (int b = 0; b < ns_; ++b)
+      QMBED_HIP(hipMemcpyAsync(Psel.data() + N * size_t(b), psi.data() + N * size_t(sel[size_t(b)]), N * 8,
+                               hipMemcpyDeviceToDevice, s_));
+    QMBED_HIP(hipMemcpyAsync(dR.data(), Rinv.data(), Rinv.size() * 8, hipMemcpyHostToDevice, s_));
+    S_.resize(N * size_t(ns_));
+    lk::gemm_vc(false, Psel.data(), N, ns_, dR.data(), ns_, S_.data(), s_);
+    QMBED_HIP(hipStreamSynchronize(s_));
+  }
+  rep.gram_seconds = seconds_since(t1);
+  // 4. the network: inputs normalized to [-1, 1] per parameter, targets L
+  auto t2 = std::chrono::steady_clock::now();
+  center_.assign(size_t(P_), 0.0);
+  scale_.assign(size_t(P_), 1.0);
+  for (int p = 0; p < P_; ++p) {
+    double lo = points[size_t(p)], hi = points[size_t(p)];
+    for (int i = 0; i < n; ++i) {
+      lo = std::min(lo, points[size_t(i) * P_ + p]);
+      hi = std::max(hi, points[size_t(i) * P_ + p]);
+    }
+    center_[size_t(p)] = 0.5 * (lo + hi);
+    scale_[size_t(p)] = hi > lo ? 0.5 * (hi - lo) : 1.0;
+  }
+  std::vector<float> X(size_t(n) * P_), T(size_t(n) * ns_);
+  for (int i = 0; i < n; ++i) {
+    for (int p = 0; p < P_; ++p)
+      X[size_t(i) * P_ + p] = float((points[size_t(i) * P_ + p] - center_[size_t(p)]) / scale_[size_t(p)]);
+    for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(Lm[size_t(i) + size_t(n) * size_t(k)]);
+  }
+  const int np = mlp_parameter_count(P_, opt_.hidden1, opt_.hidden2, ns_);
+  std::vector<float> init(static_cast<size_t>(np), 0.0f);
+  {
+    // Glorot-uniform weights, zero biases
+    Lcg rng(opt_.seed);
+    const int fan[3][2] = {{P_, opt_.hidden1}, {opt_.hidden1, opt_.hidden2}, {opt_.hidden2, ns_}};
+    size_t off = 0;
+    for (int layer = 0; layer < 3; ++layer) {
+      const int fin = fan[layer][0], fout = fan[layer][1];
+      const double a = std::sqrt(6.0 / double(fin + fout));
+      for (int i = 0; i < fin * fout; ++i) init[off + size_t(i)] = float(a * rng.next());
+      off += size_t(fin) * size_t(fout) + size_t(fout);
+    }
+  }
+  params_.resize(static_cast<size_t>(np));
+  DeviceArray<float> dX(X.size()), dT(T.size()), dLoss(1), dHist(size_t(opt_.epochs));
+  QMBED_HIP(hipMemcpyAsync(params_.data(), init.data(), init.size() * 4, hipMemcpyHostToDevice, s_));
+  QMBED_HIP(hipMemcpyAsync(dX.data(), X.data(), X.size() * 4, hipMemcpyHostToDevice, s_));
+  QMBED_HIP(hipMemcpyAsync(dT.data(), T.data(), T.size() * 4, hipMemcpyHostToDevice, s_));
+  mlp_train_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dX.data(), dT.data(), n, opt_.epochs,
+                   opt_.learning_rate, nullptr, dLoss.data(), dHist.data(), s_);
+  std::vector<float> hist(size_t(opt_.epochs));
+  QMBED_HIP(hipMemcpyAsync(hist.data(), dHist.data(), hist.size() * 4, hipMemcpyDeviceToHost, s_));
+  QMBED_HIP(hipStreamSynchronize(s_));
+  rep.train_seconds = seconds_since(t2);
+  rep.loss_first = hist.front();
+  rep.loss_last = hist.back();
+  // relative fit of the training targets: Σ|T|² for reference
+  double tn = 0;
+  for (float v : T) tn += double(v) * double(v);
+  rep.target_norm2 = tn / double(n);
+  rep.snapshot_indices = sel;
+  rep.snapshots = ns_;
+  trained_ = true;
+  return rep;
+}
+
+std::vector<double> WarmStartScan::predict(const double* points, int n) {
+  if (!trained_) throw invalid_options("the warm-start network has not been trained");
+  std::vector<float> X(size_t(n) * P_);
+  for (int i = 0; i < n; ++i)
+    for (int p = 0; p < P_; ++p) {
+      if (!std::isfinite(points[size_t(i) * P_ + p])) throw invalid_options("scan points must be finite");
+      X[size_t(i) * P_ + p] = float((points[size_t(i) * P_ + p] - center_[size_t(p)]) / scale_[size_t(p)]);
+    }
+  DeviceArray<float> dX(X.size()), dY(size_t(n) * ns_);
+  QMBED_HIP(hipMemcpyAsync(dX.data(), X.data(), X.size() * 4, hipMemcpyHostToDevice, s_));
+  mlp_forward_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dX.data(), n, dY.data(), s_);
+  std::vector<float> Y(size_t(n) * ns_);
+  QMBED_HIP(hipMemcpyAsync(Y.data(), dY.data(), Y.size() * 4, hipMemcpyDeviceToHost, s_));
+  QMBED_HIP(hipStreamSynchronize(s_));
+  return std::vector<double>(Y.begin(), Y.end());
+}
+
+WarmStartScan::PointResult WarmStartScan::solve(const double* point, const EigshOptions& o, const EigshControls& ctl,
+                                                bool warm) {
+  PointResult r;
+  auto t0 = std::chrono::steady_clock::now();
+  op_.evaluate_coefficients(coefficients_of(point), s_);
+  if (!op_.hermitian()) throw non_hermitian();
+  EigshOptions oo = o;
+  oo.eigenpairs = 1;
+  DeviceArray<double> vecs;
+  bool cv = false;
+  const size_t N = op_.dim();
+  if (warm) {
+    if (!trained_) throw invalid_options("the warm-start network has not been trained");
+    // network on the device; the start vector v₀ = S y
+    std::vector<float> x(static_cast<size_t>(P_));
+    for (int p = 0; p < P_; ++p) x[size_t(p)] = float((point[p] - center_[size_t(p)]) / scale_[size_t(p)]);
+    DeviceArray<float> dx(static_cast<size_t>(P_)), dy(static_cast<size_t>(ns_));
+    DeviceArray<double> dc(static_cast<size_t>(ns_)), v0(N);
+    QMBED_HIP(hipMemcpyAsync(dx.data(), x.data(), x.size() * 4, hipMemcpyHostToDevice, s_));
+    mlp_forward_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dx.data(), 1, dy.data(), s_);
+    hipLaunchKernelGGL(k_f2d, dim3(1), dim3(64), 0, s_, dy.data(), ns_, dc.data());
+    lk::gemm_vc(false, S_.data(), N, ns_, dc.data(), 1, v0.data(), s_);
+    std::vector<double> c(static_cast<size_t>(ns_));
+    QMBED_HIP(hipMemcpyAsync(c.data(), dc.data(), c.size() * 8, hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipStreamSynchronize(s_));
+    double cn = 0;
+    for (double v : c) cn += v * v;
+    // S is orthonormal, so ‖v₀‖ = ‖y‖; a vanishing or non-finite prediction
+    // falls back to the plain start
+    if (std::isfinite(cn) && cn > 1.0e-6) {
+      r.es = eigsh_on_device(op_, oo, v0.data(), false, ctl, vecs, cv, s_);
+      r.warm = true;
+    } else {
+      r.es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
+      r.fallback = true;
+    }
+    QMBED_HIP(hipStreamSynchronize(s_));  // dx, dy, dc, v0 die here
+  } else {
+    r.es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
+  }
+  r.seconds = seconds_since(t0);
+  return r;
+}
+
+}  // namespace qmbed

@@ -1,0 +1,117 @@
+"""Learned warm start: the GPU network kernels against the CPU restatement
+(oracle/mlp.py), and a parameter scan whose warm-started solves agree with the
+oracle and need fewer Lanczos iterations than the plain start."""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from tests import cases
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle"))
+import mlp  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2, 64, 64, 32), (3, 7, 5, 4)]
+
+
+def _params(shape, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(mlp.offsets(shape)["count"]) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_forward_matches_cpu(qa, shape):
+    p = _params(shape, 1)
+    x = np.random.default_rng(2).uniform(-1, 1, (50, shape[0])).astype(np.float32)
+    y = qa.mlp_forward(shape, p, x)
+    ref, _ = mlp.forward(shape, p.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(y - ref)) <= 2e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("n", [1, 16, 37])
+def test_gradient_matches_cpu(qa, shape, n):
+    p = _params(shape, 3)
+    rng = np.random.default_rng(4 + n)
+    x = rng.uniform(-1, 1, (n, shape[0])).astype(np.float32)
+    t = (rng.standard_normal((n, shape[3])) * 0.3).astype(np.float32)
+    g, loss = qa.mlp_gradient(shape, p, x, t)
+    ref = mlp.gradient(shape, p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    lref = mlp.loss(shape, p.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    assert np.linalg.norm(g - ref) <= 1e-5 * np.linalg.norm(ref)
+    assert abs(loss - lref) <= 1e-5 * lref
+
+
+def test_training_follows_cpu_adam(qa):
+    shape = SHAPES[0]
+    p = _params(shape, 5, scale=0.2)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, (40, 2)).astype(np.float32)
+    t = np.tanh(x @ rng.standard_normal((2, 32))).astype(np.float32) * 0.5
+    pg, hg = qa.mlp_train(shape, p, x, t, 60, 3e-3)
+    pc, hc = mlp.adam(shape, p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), 60, 3e-3)
+    np.testing.assert_allclose(hg, hc, rtol=2e-3)
+    assert np.max(np.abs(pg - pc)) <= 2e-3
+    assert hg[-1] < 0.5 * hg[0]
+
+
+L = 14
+
+
+def _ising_scan_model(qa):
+    static = [("zz", [(-1.0, b) for b in cases.bonds(L)])]
+    comps = [("hx", [("x", [(-1.0, [i]) for i in range(L)])], None),
+             ("hz", [("z", [(-1.0, [i]) for i in range(L)])], None)]
+    return qa.Model({"kind": "spin", "sites": L, "pauli": True, "momentum": 0, "parity": 1}, static,
+                    components=comps)
+
+
+def test_learned_warm_start_scan(qa, oracle):
+    # 8×8 grid over hx ∈ [0.2, 2], hz ∈ [0, 0.5]: 24 seeded training points,
+    # 8 held-out points solved from the network's start and from the plain start
+    hx = np.linspace(0.2, 2.0, 8)
+    hz = np.linspace(0.0, 0.5, 8)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    perm = np.random.default_rng(0).permutation(len(grid))
+    train, held = grid[perm[:24]], grid[perm[24:32]]
+    m = _ising_scan_model(qa)
+    scan = qa.Scan(m, snapshots=8, epochs=3000, learning_rate=3e-3, seed=0)
+    rep = scan.train(train)
+    print("[learned] training", {k: rep[k] for k in ("snapshots", "loss_first", "loss_last", "target_norm2")},
+          flush=True)
+    assert rep["snapshots"] == 8
+    assert rep["loss_last"] < 0.05 * rep["loss_first"]
+    warm = scan.solve(held, warm=True)
+    plain = scan.solve(held, warm=False)
+    warm32 = scan.solve(held, warm=True, matvec_precision="fp32")
+    assert warm["warm_starts"] == len(held) and warm["fallbacks"] == 0
+    for i, (a, b) in enumerate(held):
+        om = oracle.Model(L, cases.ising(L, a, b), momentum=0, parity=1, norm="pauli_cartesian")
+        e0 = np.linalg.eigvalsh(om.dense())[0]
+        for r in (warm, plain, warm32):
+            assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
+            assert r["residuals"][i] <= 1e-10 * abs(e0)
+    wi, pi = np.mean(warm["iterations"]), np.mean(plain["iterations"])
+    print(f"[learned] mean Lanczos iterations: warm {wi:.1f}, plain {pi:.1f}, warm+fp32 "
+          f"{np.mean(warm32['iterations']):.1f}", flush=True)
+    assert wi < pi
+    # the network reproduces its training targets' scale: predictions at the
+    # training points have unit-ish norm (coefficients of unit vectors)
+    y = scan.predict(train)
+    assert np.all(np.abs(np.linalg.norm(y, axis=1) - 1.0) < 0.2)
+
+
+def test_scan_errors(qa):
+    plain = qa.Model({"kind": "spin", "sites": 8, "up": 4}, cases.heisenberg(8))
+    with pytest.raises(qa.QmbedError, match="needs a model with components"):
+        qa.Scan(plain)
+    m = _ising_scan_model(qa)
+    with pytest.raises(qa.QmbedError, match="snapshots must lie"):
+        qa.Scan(m, snapshots=0)
+    scan = qa.Scan(m, snapshots=4)
+    with pytest.raises(qa.QmbedError, match="has not been trained"):
+        scan.solve(np.array([[1.0, 0.1]]), warm=True)
