@@ -204,21 +204,35 @@ class LanczosWindow {
   // Install the start vector (device, FP64, already normalized) as column 0.
   void start_from(const double* d_v0) {
     QMBED_HIP(hipMemcpyAsync(col(0), d_v0, esz() * n_ * 8, hipMemcpyDeviceToDevice, s_));
-    if (fp32_) lk::to_f32(c_, col(0), n_, buf_->x32.data(), s_);
+    use32_ = false;
     alpha_.clear();
     beta_.clear();
     launched_ = 0;
     size_ = 0;
   }
 
+  // FP32 matvecs from the next step on (the caller decides when; see
+  // run_window): the next column is converted once, and every later
+  // normalization also writes the FP32 copy
+  void set_fp32(bool on) {
+    if (on == use32_) return;
+    if (on) {
+      if (!fp32_) throw internal_state("FP32 matvecs were not enabled for this window");
+      if (launched_ < m_) lk::to_f32(c_, col(launched_), n_, buf_->x32.data(), s_);
+    }
+    use32_ = on;
+  }
+  bool fp32_active() const { return use32_; }
+
   void matvec(int j, const int* gate) {
-    if (fp32_) {
+    if (use32_) {
       float* x32 = buf_->x32.data();
       float* y32 = buf_->y32.data();
       if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32), reinterpret_cast<std::complex<float>*>(y32),
                         s_, gate);
       else op_.apply(x32, y32, s_, gate);
       lk::from_f32(c_, y32, n_, w(), s_, gate);
+      ++matvecs32_;
     } else {
       if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w()), s_, gate);
       else op_.apply(col(j), w(), s_, gate);
@@ -316,7 +330,26 @@ class LanczosWindow {
   long passes() const { return passes_; }
   long seconds() const { return seconds_; }
   long matvecs() const { return matvecs_; }
+  long matvecs32() const { return matvecs32_; }
   bool broke() const { return broke_; }
+
+  // largest residual estimate β|s_last| / max(|θ|, 1) of the selected pairs
+  double max_estimate(int k, SpectrumTarget target, double shift) {
+    const int sz = size_;
+    if (sz <= k) return INFINITY;
+    std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
+    std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
+    std::vector<double> vals;
+    tridiagonal_eigen(a, b, vals, nullptr);
+    auto sel = select_indices(vals, target, k, shift);
+    double worst = 0;
+    for (int idx : sel) {
+      auto v = tridiagonal_eigvec(a, b, vals[idx]);
+      const double est = last_beta_ * std::fabs(v[sz - 1]) / std::max(std::fabs(vals[idx]), 1.0);
+      worst = std::isfinite(est) ? std::max(worst, est) : INFINITY;
+    }
+    return worst;
+  }
 
   // Ritz residual estimates for the selected pairs of the current T.
   bool estimates_converged(int k, SpectrumTarget target, double shift, double tol, double safety) {
@@ -388,7 +421,7 @@ class LanczosWindow {
   void finish(int j, bool second_pass) {
     const bool next = j + 1 < m_;
     lk::lanczos_finish(c_, w(), n_, h_, j, eta_, slots() + m_ + j, halt(), flags_ + j, next ? col(j + 1) : nullptr,
-                       (next && fp32_) ? buf_->x32.data() : nullptr, buf_->ws, s_, second_pass);
+                       (next && use32_) ? buf_->x32.data() : nullptr, buf_->ws, s_, second_pass);
   }
   void count_passes(const int* hf, int steps) {
     passes_ = 0;
@@ -416,8 +449,9 @@ class LanczosWindow {
   int launched_ = 0;
   int posted_ = 0;
   int size_ = 0;
-  long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0;
+  long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0, matvecs32_ = 0;
   bool broke_ = false;
+  bool use32_ = false;
 };
 
 struct WindowResult {
@@ -426,6 +460,7 @@ struct WindowResult {
   bool ok = false;
   long iterations = 0;
   double fail_residual = 0;
+  long fp32_matvecs = 0;  // of a window that failed after FP32 products
 };
 
 // One Lanczos window with in-window convergence checks.
@@ -447,6 +482,7 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
   int interval = std::max(1, ctl.check_every);
   int next_sync = interval;
   bool pending = false, pending_conv = false;
+  bool fp32_off = false;
   for (;;) {
     if (win.launched() < m) win.step();
     const int launched = win.launched();
@@ -470,6 +506,13 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
     interval = std::min(std::max(1, ctl.check_every), interval * 2);
     next_sync = win.posted() + interval;
     const bool at_end = st.ended;
+    // FP32 matvecs once the wanted pairs have converged far enough that the
+    // single-precision product errors, weighted by the small components of
+    // the Ritz vectors on the remaining Krylov vectors, stay well below the
+    // tolerance (relaxed inexact Krylov)
+    if (fp32 && !fp32_off && !win.fp32_active() && !at_end &&
+        win.max_estimate(k, o.target, o.shift) <= ctl.fp32_switch * accept_tol)
+      win.set_fp32(true);
     if (!(at_end || pending_conv)) continue;
     const int sz = win.size();
     if (!at_end) {
@@ -492,10 +535,17 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
       if (cd.residual > accept_tol * std::max(std::fabs(cd.value), 1.0)) fail = std::max(fail, cd.residual);
     if (fail > 0 && !at_end) {
       next_check = sz + ctl.check_every;
+      // a Ritz pair that fails the FP64 test after the switch: the rest of
+      // the window runs in FP64
+      if (win.fp32_active()) {
+        win.set_fp32(false);
+        fp32_off = true;
+      }
       continue;
     }
     out.iterations = win.matvecs();
     if (fail > 0) {
+      out.fp32_matvecs = win.matvecs32();
       Error e = non_convergence(sz, fail);
       throw e;
     }
@@ -516,9 +566,8 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
     out.es.reorthogonalization_passes = win.passes();
     out.es.conditional_second_passes = win.seconds();
     out.es.converged = true;
-    if (fp32) out.es.fp32_iterations = win.matvecs();
-    else out.es.fp64_iterations = win.matvecs();
-    out.es.fp64_iterations += win.matvecs64();
+    out.es.fp32_iterations = win.matvecs32();
+    out.es.fp64_iterations = win.matvecs() - win.matvecs32() + win.matvecs64();
     out.ok = true;
     return;
   }
@@ -587,6 +636,9 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
   return es;
 }
 
+
+}  // namespace
+
 // Balanced combination Σ_j e^{2πij/J}/√J v_j (src/solve/mod.rs:708-727),
 // real part kept for real problems.
 void balanced_start(const DeviceArray<double>& vecs, int k, size_t n, bool c, DeviceArray<double>& out,
@@ -608,8 +660,6 @@ void balanced_start(const DeviceArray<double>& vecs, int k, size_t n, bool c, De
   lk::normalize(c, out.data(), sl.data(), n, out.data(), nullptr, nullptr, s);
   QMBED_HIP(hipStreamSynchronize(s));
 }
-
-}  // namespace
 
 Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* d_initial, bool initial_complex,
                        const EigshControls& ctl, DeviceArray<double>& vec_out, bool& complex_vectors, hipStream_t s) {
@@ -653,75 +703,62 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
   }
 
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
+  // Windows: without krylov_dimension, min(max(8k+64, 256), N, budget),
+  // doubled on non-convergence, dense fallback at the full space for
+  // N <= 2048 (src/solve/mod.rs:2242-2328); with it, one window. A window
+  // that fails after switching to FP32 products is repeated at the same size
+  // in FP64 from the balanced combination of its Ritz vectors (the FP32
+  // errors left a residual floor above the tolerance).
+  DeviceArray<double> polish_start;
   auto run_expanding = [&](const double* d_start, bool use_fp32, double accept_tol, long budget) -> WindowResult {
-    WindowResult wr;
     EigshOptions oo = o;
-    if (!o.krylov_dimension) {
-      long sub = std::min<long>(std::min<long>(std::max<long>(8L * o.eigenpairs + 64, 256), long(n)), budget);
-      long spent = 0;
-      for (;;) {
-        try {
-          WindowResult w2;
-          run_window(op, c, oo, int(sub), d_start, use_fp32, ctl, accept_tol, w2, s);
-          w2.es.iterations += spent;
-          return w2;
-        } catch (Error& err) {
-          if (err.kind != ErrorKind::NonConvergence) throw;
-          spent += err.iterations;
-          if (sub == long(n) && n <= 2048) {
-            WindowResult w3;
-            bool cv = false;
-            w3.es = dense_eigsh(op, o, w3.vectors, cv, s);
-            w3.es.iterations += spent;
-            w3.ok = true;
-            return w3;
+    const bool fixed = bool(o.krylov_dimension);
+    long sub = fixed ? std::min<long>(std::min<long>(*o.krylov_dimension, o.max_iterations), long(n))
+                     : std::min<long>(std::min<long>(std::max<long>(8L * o.eigenpairs + 64, 256), long(n)), budget);
+    if (fixed && sub <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
+    long spent = 0, fp32_spent = 0;
+    for (;;) {
+      WindowResult w2;
+      try {
+        run_window(op, c, oo, int(sub), d_start, use_fp32, ctl, accept_tol, w2, s);
+        w2.es.iterations += spent;
+        w2.es.fp32_iterations += fp32_spent;
+        return w2;
+      } catch (Error& err) {
+        if (err.kind != ErrorKind::NonConvergence) throw;
+        spent += err.iterations;
+        if (use_fp32 && w2.fp32_matvecs > 0 && w2.vectors.size() > 0) {
+          fp32_spent += w2.fp32_matvecs;
+          balanced_start(w2.vectors, o.eigenpairs, n, c, polish_start, s);
+          d_start = polish_start.data();
+          use_fp32 = false;
+          if (!fixed) {
+            const long remaining = budget > spent ? budget - spent : 0;
+            sub = std::min(sub, remaining);
+            if (sub <= o.eigenpairs) throw non_convergence(spent, err.residual);
           }
-          long remaining = budget > spent ? budget - spent : 0;
-          long next = std::min(std::min(sub * 2, long(n)), remaining);
-          if (next <= sub) throw non_convergence(spent, err.residual);
-          sub = next;
+          continue;
         }
+        if (fixed) throw;
+        if (sub == long(n) && n <= 2048) {
+          WindowResult w3;
+          bool cv = false;
+          w3.es = dense_eigsh(op, o, w3.vectors, cv, s);
+          w3.es.iterations += spent;
+          w3.ok = true;
+          return w3;
+        }
+        long remaining = budget > spent ? budget - spent : 0;
+        long next = std::min(std::min(sub * 2, long(n)), remaining);
+        if (next <= sub) throw non_convergence(spent, err.residual);
+        sub = next;
       }
     }
-    long kd = std::min<long>(std::min<long>(*o.krylov_dimension, o.max_iterations), long(n));
-    if (kd <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
-    run_window(op, c, oo, int(kd), d_start, use_fp32, ctl, accept_tol, wr, s);
-    return wr;
   };
 
-  if (!fp32) {
-    WindowResult wr = run_expanding(start.data(), false, o.tolerance, o.max_iterations);
-    vec_out = std::move(wr.vectors);
-    return wr.es;
-  }
-  // FP32 stage: converge to the single-precision floor, then refine in FP64
-  const double tol32 = std::max(o.tolerance, 2.0e-6);
-  WindowResult w1;
-  try {
-    w1 = run_expanding(start.data(), true, tol32, o.max_iterations);
-  } catch (Error& err) {
-    if (err.kind != ErrorKind::NonConvergence) throw;
-    WindowResult wr = run_expanding(start.data(), false, o.tolerance, o.max_iterations);
-    wr.es.iterations += err.iterations;
-    wr.es.fp32_iterations = err.iterations;
-    vec_out = std::move(wr.vectors);
-    return wr.es;
-  }
-  bool done = true;
-  for (size_t i = 0; i < w1.es.residuals.size(); ++i)
-    if (w1.es.residuals[i] > o.tolerance * std::max(std::fabs(w1.es.eigenvalues[i]), 1.0)) done = false;
-  if (done) {
-    vec_out = std::move(w1.vectors);
-    return w1.es;
-  }
-  DeviceArray<double> warm;
-  balanced_start(w1.vectors, o.eigenpairs, n, c, warm, s);
-  long spent32 = w1.es.iterations;
-  WindowResult w2 = run_expanding(warm.data(), false, o.tolerance, std::max<long>(o.max_iterations - spent32, 1));
-  w2.es.fp32_iterations = w1.es.fp32_iterations;
-  w2.es.iterations += spent32;
-  vec_out = std::move(w2.vectors);
-  return w2.es;
+  WindowResult wr = run_expanding(start.data(), fp32, o.tolerance, o.max_iterations);
+  vec_out = std::move(wr.vectors);
+  return wr.es;
 }
 
 Eigensystem eigsh(const Operator& op, const EigshOptions& o, const cplx* initial, const EigshControls& ctl,

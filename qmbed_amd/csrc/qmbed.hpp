@@ -362,9 +362,13 @@ struct EigshOptions {
   void validate(size_t dimension) const;
 };
 
-// Precision of the matrix-vector product inside the Krylov loop. The Lanczos
-// coefficients, reorthogonalization, tridiagonal solve and the final
-// Rayleigh-quotient/residual polish always stay FP64.
+// Precision of the matrix-vector product inside the Krylov loop. FP32 is a
+// relaxed schedule: FP64 products until the wanted pairs' residual estimates
+// fall below fp32_switch × tolerance, FP32 products after that (their errors
+// enter the Ritz vectors through components of that size), back to FP64 if
+// a Ritz pair then fails the test. The Lanczos coefficients,
+// reorthogonalization, tridiagonal solve and the final Rayleigh-quotient /
+// residual test always stay FP64.
 enum class MatvecPrecision { FP64 = 0, FP32 = 1 };
 
 struct EigshControls {
@@ -374,6 +378,9 @@ struct EigshControls {
   // when true, stop inside the window as soon as the requested pairs have
   // converged (the reference always runs the full window)
   bool early_stop = true;
+  // FP32 matvecs start once every wanted pair's residual estimate is below
+  // fp32_switch × tolerance (relative to max(|θ|, 1))
+  double fp32_switch = 1.0e4;
   // DGKS threshold η: a second reorthogonalization pass is taken when the
   // first leaves ‖w‖ <= η‖w₀‖ (the reference's constant is 1/√2; η = 1 takes
   // the second-pass path on nearly every step, which exercises it)
