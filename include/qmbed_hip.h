@@ -71,8 +71,9 @@ int qmbed_hip_model_states(const qmbed_hip_model* model, uint32_t* states, uint8
 /* entries the reference's assembly would store for this operator (merged
  * per (row, column), |v| > machine epsilon, diagonal included); -1 on error */
 long qmbed_hip_model_nnz(const qmbed_hip_model* model);
-/* H·v configuration picked by the autotuner: lanes per row and whether the
- * image-table mode (1) or the orbit-walk mode (0) is used */
+/* H·v configuration picked by the autotuner: lanes per row and the mode in
+ * *image_table: 0 orbit walk + sorted-table search, 1 image table,
+ * 2 compiled index stream */
 int qmbed_hip_model_hv_config(const qmbed_hip_model* model, int* lanes_per_row, int* image_table);
 /* the model's HIP stream (hipStream_t) */
 void* qmbed_hip_model_stream(const qmbed_hip_model* model);

@@ -237,7 +237,7 @@ class Model:
     def hv_config(self):
         lpr, img = ctypes.c_int(), ctypes.c_int()
         library().qmbed_hip_model_hv_config(self._h, ctypes.byref(lpr), ctypes.byref(img))
-        return {"lanes_per_row": lpr.value, "mode": "image" if img.value else "walk"}
+        return {"lanes_per_row": lpr.value, "mode": ("walk", "image", "stream")[img.value]}
 
     def nnz(self):
         v = int(library().qmbed_hip_model_nnz(self._h))

@@ -737,7 +737,7 @@ int qmbed_hip_model_hv_config(const qmbed_hip_model* m, int* lanes_per_row, int*
   try {
     auto* tm = reinterpret_cast<const TypedModel*>(m);
     if (lanes_per_row) *lanes_per_row = tm->m->op->lanes_per_row();
-    if (image_table) *image_table = tm->m->op->uses_image_table() ? 1 : 0;
+    if (image_table) *image_table = tm->m->op->uses_index_stream() ? 2 : (tm->m->op->uses_image_table() ? 1 : 0);
     return 0;
   } catch (...) {
     return -1;

@@ -482,7 +482,6 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
   int interval = std::max(1, ctl.check_every);
   int next_sync = interval;
   bool pending = false, pending_conv = false;
-  bool fp32_off = false;
   for (;;) {
     if (win.launched() < m) win.step();
     const int launched = win.launched();
@@ -510,7 +509,7 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
     // single-precision product errors, weighted by the small components of
     // the Ritz vectors on the remaining Krylov vectors, stay well below the
     // tolerance (relaxed inexact Krylov)
-    if (fp32 && !fp32_off && !win.fp32_active() && !at_end &&
+    if (fp32 && !win.fp32_active() && !at_end &&
         win.max_estimate(k, o.target, o.shift) <= ctl.fp32_switch * accept_tol)
       win.set_fp32(true);
     if (!(at_end || pending_conv)) continue;
@@ -534,13 +533,17 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
     for (auto& cd : cands)
       if (cd.residual > accept_tol * std::max(std::fabs(cd.value), 1.0)) fail = std::max(fail, cd.residual);
     if (fail > 0 && !at_end) {
-      next_check = sz + ctl.check_every;
-      // a Ritz pair that fails the FP64 test after the switch: the rest of
-      // the window runs in FP64
-      if (win.fp32_active()) {
-        win.set_fp32(false);
-        fp32_off = true;
+      // A Ritz pair whose estimate says converged but which fails the FP64
+      // test after FP32 products: the single-precision errors already sit
+      // in this window's tridiagonal matrix and more steps cannot remove
+      // them, so the window ends here and the caller restarts in FP64 from
+      // these Ritz vectors (accurate to about the FP32 floor).
+      if (win.matvecs32() > 0) {
+        out.iterations = win.matvecs();
+        out.fp32_matvecs = win.matvecs32();
+        throw non_convergence(sz, fail);
       }
+      next_check = sz + ctl.check_every;
       continue;
     }
     out.iterations = win.matvecs();

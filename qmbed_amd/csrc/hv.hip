@@ -69,6 +69,9 @@ struct HvArgs {
   // optional device flag: a nonzero value makes the launch a no-op (a
   // speculative Lanczos step voided by a pending reorthogonalization)
   const int* gate;
+  // compiled index stream (sliced ELL of packed target words), or null
+  const uint32_t* stream;
+  const uint32_t* slice_ptr;
 };
 
 __device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t mask) {
@@ -318,6 +321,110 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   }
 
   // off-diagonal
+  if constexpr (IMG) {
+    // Batched: the image words of up to B targets are loaded together, then
+    // the B input entries they name, then the B products are accumulated —
+    // two memory round trips per batch instead of two per target.
+    constexpr int B = NV > 1 ? 4 : 8;
+    const uint32_t imask = (1u << a.img_bits) - 1u;
+    for (int w0 = 0; w0 < a.n_off; w0 += 32) {
+      uint32_t act = 0;
+      const int gend = min(32, a.n_off - w0);
+      for (int gi = 0; gi < gend; ++gi) {
+        const int g = a.n_diag + w0 + gi;
+        const int pat = pattern_of(r, s_meta[4 * g + 1]);
+        act |= ((s_meta[4 * g + 2] >> pat) & 1u) << gi;
+      }
+      act &= a.lane_groups << sub;
+      while (act) {
+        // ranks of up to B targets (ALU and LDS only), then their image words
+        uint32_t ent[B], rts[B];
+        int offs[B];
+        bool in[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          in[b] = false;
+          rts[b] = 0;
+          offs[b] = 0;
+          if (act) {
+            const int gi = __ffs(act) - 1;
+            act &= act - 1;
+            const int g = a.n_diag + w0 + gi;
+            const uint32_t t = r ^ s_meta[4 * g];
+            offs[b] = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+            uint32_t rt = t;
+            bool ok = true;
+            if (a.up >= 0) {
+              const uint32_t hp = s_hop[g];
+              if (hp >> 16) {
+                const int p = hp & 255, q = (hp >> 8) & 255;
+                rt = ((r >> p) & 1u) ? rank_r + uint32_t(move_delta(r, p, q, s_bin))
+                                     : rank_r - uint32_t(move_delta(t, p, q, s_bin));
+              } else {
+                ok = __popc(t) == a.up;  // else it leaves the fixed-magnetization sector
+                rt = ok ? rank_colex(t, s_bin) : 0u;
+              }
+            }
+            in[b] = ok;
+            rts[b] = rt;
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) ent[b] = in[b] ? __ldg(a.img + rts[b]) : 0xFFFFFFFFu;
+        S xr[B][NV], xi[B][NV];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          const int idx = ent[b] != 0xFFFFFFFFu ? int(ent[b] & imask) : int(row);
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            if (CMODE == 0) {
+              xr[b][v] = x[v * a.ld + idx];
+              xi[b][v] = 0;
+            } else {
+              xr[b][v] = x[2 * (v * a.ld + idx)];
+              xi[b][v] = x[2 * (v * a.ld + idx) + 1];
+            }
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          if (ent[b] == 0xFFFFFFFFu) continue;
+          const uint32_t e = ent[b];
+          const int code = int((e >> a.img_bits) & 15u);
+          const int ci = int((e >> (a.img_bits + 5)) & 31u);
+          S scale = ((e >> (a.img_bits + 4)) & 1u) ? S(-1) : S(1);
+          if (SYM) scale *= sq_r * S(s_crsq[code]);
+          Cx<S> c;
+          if (CMODE == 2) {
+            c.re = S(s_amp[2 * offs[b]]) * scale;
+            c.im = S(s_amp[2 * offs[b] + 1]) * scale;
+          } else {
+            c.re = S(s_amp[offs[b]]) * scale;
+            c.im = 0;
+          }
+          if ((SYM == 1 || SYM == 3) && SHIFT) {
+            const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+            if (CMODE == 2) {
+              const S re = c.re * pr - c.im * pi;
+              c.im = c.re * pi + c.im * pr;
+              c.re = re;
+            } else {
+              c.re *= pr;  // real characters (k = L/2): pi == 0
+            }
+          }
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            if (CMODE == 0) {
+              acc[v].re += c.re * xr[b][v];
+            } else {
+              acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
+              acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+            }
+          }
+        }
+      }
+    }
+  } else {
   for (int w0 = 0; w0 < a.n_off; w0 += 32) {
     uint32_t act = 0;
     const int gend = min(32, a.n_off - w0);
@@ -396,6 +503,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
         }
       }
     }
+  }
   }
   for (int o = 1; o < a.lpr; o <<= 1) {
 #pragma unroll
@@ -687,6 +795,222 @@ __global__ void __launch_bounds__(kBlock) image_kernel(HvArgs a, int sym, int sh
   }
 }
 
+// Compiled index stream: for every representative, the packed target word
+// (as in the image table: idx | orbit code << bits | (sign < 0) << (bits+4) |
+// ci << (bits+5), or 0xFFFFFFFF outside the sector) of each of its active
+// off-diagonal flip groups, in group order. Sliced ELL over 64-row slices
+// (one wavefront): slice s holds 64·width_s words and word k of its row l
+// sits at slice_ptr[s] + 64·k + l, so the words of one ordinal are read by a
+// wavefront as one contiguous 256-byte line; shorter rows are padded with
+// 0xFFFFFFFF. The stream depends only on the sector and the flip-group
+// structure, so an operator family keeps it across evaluations.
+constexpr int kSlice = 64;
+
+// the active off-diagonal groups of representative r in group order
+struct ActiveGroups {
+  const uint32_t* meta;
+  int n_diag, n_off;
+  uint32_t r;
+  int w0;
+  uint32_t act;
+  __device__ ActiveGroups(const uint32_t* m, int nd, int no, uint32_t rr)
+      : meta(m), n_diag(nd), n_off(no), r(rr), w0(-32), act(0) {}
+  __device__ uint32_t window(int w) const {
+    uint32_t out = 0;
+    const int gend = min(32, n_off - w);
+    for (int gi = 0; gi < gend; ++gi) {
+      const int g = n_diag + w + gi;
+      const int pat = pattern_of(r, meta[4 * g + 1]);
+      out |= ((meta[4 * g + 2] >> pat) & 1u) << gi;
+    }
+    return out;
+  }
+  // next active group, -1 when none is left
+  __device__ int next() {
+    while (!act) {
+      w0 += 32;
+      if (w0 >= n_off) return -1;
+      act = window(w0);
+    }
+    const int gi = __ffs(act) - 1;
+    act &= act - 1;
+    return n_diag + w0 + gi;
+  }
+};
+
+// width_s = the largest active-group count among the slice's rows
+__global__ void __launch_bounds__(kBlock) stream_count_kernel(HvArgs a, uint32_t n_slices, uint32_t* width) {
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  int cnt = 0;
+  if (row < a.n) {
+    ActiveGroups ag(a.meta, a.n_diag, a.n_off, a.states[row]);
+    for (int w = 0; w < a.n_off; w += 32) cnt += __popc(ag.window(w));
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o, 64));
+  if ((threadIdx.x & 63) == 0 && row / kSlice < n_slices) width[row / kSlice] = uint32_t(cnt);
+}
+
+__global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, int shift_needed, uint32_t n_slices,
+                                                             const uint8_t* orbit_code, const uint32_t* slice_ptr,
+                                                             uint32_t* out) {
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  if (slice >= n_slices) return;
+  const uint32_t base = slice_ptr[slice];
+  const uint32_t width = (slice_ptr[slice + 1] - base) / kSlice;
+  uint32_t k = 0;
+  if (row < a.n) {
+    const uint32_t r = a.states[row];
+    ActiveGroups ag(a.meta, a.n_diag, a.n_off, r);
+    for (int g = ag.next(); g >= 0; g = ag.next()) {
+      uint32_t e = 0xFFFFFFFFu;
+      int idx, orbit, ci, sign;
+      if (target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign))
+        e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) | (uint32_t(sign < 0) << (a.img_bits + 4)) |
+            (uint32_t(ci) << (a.img_bits + 5));
+      out[base + kSlice * k + lane] = e;
+      ++k;
+    }
+  }
+  for (; k < width; ++k) out[base + kSlice * k + lane] = 0xFFFFFFFFu;
+}
+
+// H·v from the compiled stream: one lane per row; the words of B ordinals
+// are loaded together (coalesced over the wavefront), then the B input
+// entries they name, then the B products are accumulated.
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
+__global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
+  if (a.gate && *a.gate) return;
+  extern __shared__ double smem_d[];
+  double* s_amp = smem_d;                                  // n_amp or 2*n_amp
+  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
+  double* s_sq = s_chr + 2 * 32;                           // 65
+  double* s_crsq = s_sq + 65;                              // 16
+  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_crsq + 16);
+  const int G = a.n_diag + a.n_off;
+  const int namp = (CMODE == 2 ? 2 : 1) * a.n_amp;
+  for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
+  for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
+  for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
+  if (SYM == 1 || SYM == 3)
+    for (int i = threadIdx.x; i < 2 * a.L; i += kBlock) s_chr[i] = a.chars[i];
+  for (int i = threadIdx.x; i < 65; i += kBlock) s_sq[i] = sqrt(double(i));
+  __syncthreads();
+
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t n_slices = (a.n + kSlice - 1) / kSlice;
+  if (slice >= n_slices) return;  // whole wavefronts leave together
+  const bool live = row < a.n;
+  const uint32_t base = a.slice_ptr[slice];
+  const uint32_t width = (a.slice_ptr[slice + 1] - base) / kSlice;
+  const uint32_t r = live ? a.states[row] : 0u;
+  S sq_r = 1;
+  if (SYM && live) sq_r = S(s_sq[a.orbits[row]]);
+
+  Cx<S> acc[NV];
+  {
+    Cx<S> dsum{0, 0};
+    const int ndiag = live ? a.n_diag : 0;
+    for (int g = 0; g < ndiag; ++g) {
+      const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+      if (CMODE == 2) {
+        dsum.re += S(s_amp[2 * off]);
+        dsum.im += S(s_amp[2 * off + 1]);
+      } else {
+        dsum.re += S(s_amp[off]);
+      }
+    }
+    const uint32_t own = live ? row : 0u;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (CMODE == 0) {
+        acc[v].re = dsum.re * x[v * a.ld + own];
+        acc[v].im = 0;
+      } else {
+        const S xr = x[2 * (v * a.ld + own)], xi = x[2 * (v * a.ld + own) + 1];
+        acc[v].re = dsum.re * xr - dsum.im * xi;
+        acc[v].im = dsum.re * xi + dsum.im * xr;
+      }
+    }
+  }
+  constexpr int B = NV > 1 ? 4 : 8;
+  const uint32_t imask = (1u << a.img_bits) - 1u;
+  ActiveGroups ag(s_meta, a.n_diag, a.n_off, r);
+  const uint32_t* words = a.stream + base + lane;
+  for (uint32_t k0 = 0; k0 < width; k0 += B) {
+    uint32_t ent[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) ent[b] = (k0 + b < width) ? words[size_t(kSlice) * (k0 + b)] : 0xFFFFFFFFu;
+    int grp[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) grp[b] = (live && k0 + b < width) ? ag.next() : -1;
+    S xr[B][NV], xi[B][NV];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int idx = ent[b] != 0xFFFFFFFFu ? int(ent[b] & imask) : 0;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (CMODE == 0) {
+          xr[b][v] = x[v * a.ld + idx];
+          xi[b][v] = 0;
+        } else {
+          xr[b][v] = x[2 * (v * a.ld + idx)];
+          xi[b][v] = x[2 * (v * a.ld + idx) + 1];
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      if (ent[b] == 0xFFFFFFFFu || grp[b] < 0) continue;
+      const uint32_t e = ent[b];
+      const int g = grp[b];
+      const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+      const int code = int((e >> a.img_bits) & 15u);
+      const int ci = int((e >> (a.img_bits + 5)) & 31u);
+      S scale = ((e >> (a.img_bits + 4)) & 1u) ? S(-1) : S(1);
+      if (SYM) scale *= sq_r * S(s_crsq[code]);
+      Cx<S> c;
+      if (CMODE == 2) {
+        c.re = S(s_amp[2 * off]) * scale;
+        c.im = S(s_amp[2 * off + 1]) * scale;
+      } else {
+        c.re = S(s_amp[off]) * scale;
+        c.im = 0;
+      }
+      if ((SYM == 1 || SYM == 3) && SHIFT) {
+        const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+        if (CMODE == 2) {
+          const S re = c.re * pr - c.im * pi;
+          c.im = c.re * pi + c.im * pr;
+          c.re = re;
+        } else {
+          c.re *= pr;  // real characters (k = L/2): pi == 0
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        if (CMODE == 0) {
+          acc[v].re += c.re * xr[b][v];
+        } else {
+          acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
+          acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+        }
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (CMODE == 0) {
+      y[v * a.ld + row] = acc[v].re;
+    } else {
+      y[2 * (v * a.ld + row)] = acc[v].re;
+      y[2 * (v * a.ld + row) + 1] = acc[v].im;
+    }
+  }
+}
+
 int sym_code(const SpinBasis1D& b) {
   const SectorGeometry& g = b.geometry();
   if (g.kmom >= 0 && g.parity) return 3;
@@ -716,6 +1040,9 @@ struct Operator::DeviceTables {
   DeviceArray<uint32_t> img, hop, binom;
   DeviceArray<double> code_rsq;
   int img_bits = 0;
+  // compiled index stream (autotuned against both)
+  bool use_stream = false;
+  DeviceArray<uint32_t> stream, slice_ptr;
   HvArgs args(const SpinBasis1D& b) const {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
@@ -751,6 +1078,8 @@ struct Operator::DeviceTables {
     a.nvec = 1;
     a.ld = b.len();
     a.gate = nullptr;
+    a.stream = use_stream ? stream.data() : nullptr;
+    a.slice_ptr = slice_ptr.data();
     return a;
   }
 };
@@ -1030,9 +1359,40 @@ void launch_hv_mode(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipS
 }
 
 template <typename S, int CMODE, int NV>
+void launch_hv_stream(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
+  const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
+  dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
+  const size_t G = size_t(a.n_diag + a.n_off);
+  const size_t sm = sizeof(double) * ((CMODE == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G;
+#define QMBED_HVS_CASE(SYMV, SH) \
+  hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV>), grid, block, sm, s, a, x, y);
+  switch (sym) {
+    case 0: QMBED_HVS_CASE(0, false); break;
+    case 1:
+      if (shift) {
+        QMBED_HVS_CASE(1, true);
+      } else {
+        QMBED_HVS_CASE(1, false);
+      }
+      break;
+    case 2: QMBED_HVS_CASE(2, false); break;
+    default:
+      if (shift) {
+        QMBED_HVS_CASE(3, true);
+      } else {
+        QMBED_HVS_CASE(3, false);
+      }
+      break;
+  }
+#undef QMBED_HVS_CASE
+  QMBED_HIP(hipGetLastError());
+}
+
+template <typename S, int CMODE, int NV>
 void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
   if (a.n == 0) return;
-  if (a.img) launch_hv_mode<S, CMODE, NV, true>(a, sym, shift, x, y, s);
+  if (a.stream) launch_hv_stream<S, CMODE, NV>(a, sym, shift, x, y, s);
+  else if (a.img) launch_hv_mode<S, CMODE, NV, true>(a, sym, shift, x, y, s);
   else launch_hv_mode<S, CMODE, NV, false>(a, sym, shift, x, y, s);
 }
 
@@ -1042,6 +1402,81 @@ void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream
 // with one lane per row; splitting a row's flip groups over 2-8 lanes (and
 // reducing with cross-lane shuffles) restores parallelism. The FP64 kernel is
 // timed with HIP events for each candidate and the fastest is kept.
+// Orbit codes of the packed target words (image table and stream): every
+// possible orbit size gets a 4-bit code, 1/sqrt(size) per code goes to the
+// device, and the representative index takes the low img_bits bits. Returns
+// false when the sector does not fit the packing.
+static bool prepare_codes(Operator& op, std::vector<uint8_t>& code) {
+  const SpinBasis1D& b = op.basis();
+  const SectorGeometry& g = b.geometry();
+  auto& t = *op.tables_;
+  std::vector<int> sizes;
+  for (int d = 1; d <= g.L; ++d)
+    if (g.L % d == 0) {
+      sizes.push_back(d);
+      if (g.parity) sizes.push_back(2 * d);
+    }
+  if (!b.symmetric()) sizes = {1};
+  if (g.kmom < 0 && g.parity) sizes = {1, 2};
+  std::sort(sizes.begin(), sizes.end());
+  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+  if (sizes.size() > 16) return false;
+  code.assign(64, 0);
+  std::vector<double> crsq(16, 0.0);
+  for (size_t i = 0; i < sizes.size(); ++i) {
+    code[size_t(sizes[i])] = uint8_t(i);
+    crsq[i] = 1.0 / std::sqrt(double(sizes[i]));
+  }
+  int bits = 1;
+  while ((size_t(1) << bits) < b.len()) ++bits;
+  if (bits + 10 > 32) return false;
+  t.code_rsq.resize(16);
+  QMBED_HIP(hipMemcpy(t.code_rsq.data(), crsq.data(), 16 * 8, hipMemcpyHostToDevice));
+  t.img_bits = bits;
+  return true;
+}
+
+// Build the compiled index stream (count pass, slice offsets on the host,
+// fill pass). Returns false when it does not fit the memory budget.
+static bool build_stream(Operator& op, hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  auto& t = *op.tables_;
+  std::vector<uint8_t> code;
+  if (!prepare_codes(op, code)) return false;
+  const size_t n = b.len();
+  const uint32_t n_slices = uint32_t((n + kSlice - 1) / kSlice);
+  HvArgs a = t.args(b);
+  DeviceArray<uint32_t> width(n_slices);
+  const unsigned grid = unsigned((size_t(n_slices) * kSlice + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(stream_count_kernel, dim3(grid), dim3(kBlock), 0, s, a, n_slices, width.data());
+  QMBED_HIP(hipGetLastError());
+  std::vector<uint32_t> w(n_slices);
+  QMBED_HIP(hipMemcpyAsync(w.data(), width.data(), size_t(n_slices) * 4, hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  std::vector<uint32_t> ptr(size_t(n_slices) + 1, 0);
+  uint64_t total = 0;
+  for (uint32_t sl = 0; sl < n_slices; ++sl) {
+    ptr[sl] = uint32_t(total);
+    total += uint64_t(kSlice) * w[sl];
+    if (total >= (uint64_t(1) << 32)) return false;
+  }
+  ptr[n_slices] = uint32_t(total);
+  size_t free_b = 0, total_b = 0;
+  QMBED_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (total * 4 > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
+  t.slice_ptr.resize(ptr.size());
+  QMBED_HIP(hipMemcpy(t.slice_ptr.data(), ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
+  t.stream.resize(std::max<uint64_t>(total, 1));
+  DeviceArray<uint8_t> dcode(64);
+  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 64, hipMemcpyHostToDevice));
+  a = t.args(b);
+  hipLaunchKernelGGL(stream_fill_kernel, dim3(grid), dim3(kBlock), 0, s, a, sym_code(b),
+                     b.geometry().kmom > 0 ? 1 : 0, n_slices, dcode.data(), t.slice_ptr.data(), t.stream.data());
+  QMBED_HIP(hipGetLastError());
+  QMBED_HIP(hipStreamSynchronize(s));
+  return true;
+}
+
 // Build the image table of the operator's sector (one pass of the orbit walk
 // and sorted-table lookup over every parent state). Returns false when the
 // sector does not fit the packed format or the memory budget.
@@ -1077,26 +1512,8 @@ static bool build_image_table(Operator& op, hipStream_t s) {
       }
       hop.push_back(h);
     }
-  std::vector<int> sizes;
-  for (int d = 1; d <= g.L; ++d)
-    if (g.L % d == 0) {
-      sizes.push_back(d);
-      if (g.parity) sizes.push_back(2 * d);
-    }
-  if (!b.symmetric()) sizes = {1};
-  if (g.kmom < 0 && g.parity) sizes = {1, 2};
-  std::sort(sizes.begin(), sizes.end());
-  sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
-  if (sizes.size() > 16) return false;
-  std::vector<uint8_t> code(64, 0);
-  std::vector<double> crsq(16, 0.0);
-  for (size_t i = 0; i < sizes.size(); ++i) {
-    code[sizes[i]] = uint8_t(i);
-    crsq[i] = 1.0 / std::sqrt(double(sizes[i]));
-  }
-  int bits = 1;
-  while ((size_t(1) << bits) < b.len()) ++bits;
-  if (bits + 10 > 32) return false;
+  std::vector<uint8_t> code;
+  if (!prepare_codes(op, code)) return false;
   const bool fixed_up = spec.up.has_value() && spec.up_sectors.empty();
   uint64_t parents = 0, popmask = ~0ull;
   if (fixed_up) {
@@ -1115,13 +1532,10 @@ static bool build_image_table(Operator& op, hipStream_t s) {
   if (parents * 4 > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
   t.binom.resize(1024);
   t.hop.resize(std::max<size_t>(hop.size(), 1));
-  t.code_rsq.resize(16);
   DeviceArray<uint8_t> dcode(64);
   QMBED_HIP(hipMemcpy(t.binom.data(), binom.data(), 4096, hipMemcpyHostToDevice));
   if (!hop.empty()) QMBED_HIP(hipMemcpy(t.hop.data(), hop.data(), hop.size() * 4, hipMemcpyHostToDevice));
-  QMBED_HIP(hipMemcpy(t.code_rsq.data(), crsq.data(), 16 * 8, hipMemcpyHostToDevice));
   QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 64, hipMemcpyHostToDevice));
-  t.img_bits = bits;
   t.img.resize(parents);
   HvArgs a = t.args(b);
   const uint64_t threads = (parents + kImgChunk - 1) / kImgChunk;
@@ -1144,10 +1558,13 @@ static void tune_lanes(Operator& op) {
   if (n == 0) return;
   hipStream_t s;
   QMBED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  // QMBED_HV_MODE = walk | image | auto (default) restricts the candidates
+  // QMBED_HV_MODE = walk | image | stream | auto (default) restricts the
+  // candidates
   const char* mode_env = std::getenv("QMBED_HV_MODE");
   const std::string mode = mode_env ? mode_env : "auto";
-  const bool have_img = mode != "walk" && build_image_table(op, s);
+  const bool want_walk = mode == "auto" || mode == "walk";
+  const bool have_img = (mode == "auto" || mode == "image") && build_image_table(op, s);
+  const bool have_stream = (mode == "auto" || mode == "stream") && build_stream(op, s);
   DeviceArray<double> x(2 * n), y(2 * n);
   QMBED_HIP(hipMemsetAsync(x.data(), 0, 2 * n * 8, s));
   hipEvent_t e0, e1;
@@ -1156,12 +1573,14 @@ static void tune_lanes(Operator& op) {
   const int sym = sym_code(b);
   const bool sh = b.geometry().kmom > 0;
   float best = 1e30f;
-  int best_lpr = 1;
-  bool best_img = false;
-  for (int img = (have_img && mode == "image") ? 1 : 0; img < (have_img ? 2 : 1); ++img) {
+  int best_lpr = 1, best_mode = 0;  // 0 walk, 1 image, 2 stream
+  for (int md = 0; md < 3; ++md) {
+    if ((md == 0 && !want_walk) || (md == 1 && !have_img) || (md == 2 && !have_stream)) continue;
     for (int lpr : {1, 2, 4, 8}) {
+      if (md == 2 && lpr > 1) break;  // the stream kernel runs one lane per row
       op.tables_->lpr = lpr;
-      op.tables_->use_img = img != 0;
+      op.tables_->use_img = md == 1;
+      op.tables_->use_stream = md == 2;
       HvArgs a = op.tables_->args(b);
       auto launch = [&]() {
         if (op.is_real()) launch_hv<double, 0, 1>(a, sym, sh, x.data(), y.data(), s);
@@ -1177,13 +1596,19 @@ static void tune_lanes(Operator& op) {
       if (ms < best) {
         best = ms;
         best_lpr = lpr;
-        best_img = img != 0;
+        best_mode = md;
       }
     }
   }
+  if (best == 1e30f) throw unsupported_backend("QMBED_HV_MODE names an H·v mode this sector cannot use");
   op.tables_->lpr = best_lpr;
-  op.tables_->use_img = best_img;
-  if (!best_img) op.tables_->img.release();
+  op.tables_->use_img = best_mode == 1;
+  op.tables_->use_stream = best_mode == 2;
+  if (best_mode != 1) op.tables_->img.release();
+  if (best_mode != 2) {
+    op.tables_->stream.release();
+    op.tables_->slice_ptr.release();
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   QMBED_HIP(hipStreamSynchronize(s));
@@ -1191,6 +1616,8 @@ static void tune_lanes(Operator& op) {
 }
 
 bool Operator::uses_image_table() const { return tables_->use_img; }
+
+bool Operator::uses_index_stream() const { return tables_->use_stream; }
 
 int Operator::lanes_per_row() const { return tables_->lpr; }
 

@@ -249,6 +249,7 @@ class Operator {
   // lanes per row and H·v mode chosen by the autotuner
   int lanes_per_row() const;
   bool uses_image_table() const;
+  bool uses_index_stream() const;
 
   // dense materialization (row-major, complex) on the device
   void materialize_dense(cplx* dense, hipStream_t s) const;
@@ -365,8 +366,9 @@ struct EigshOptions {
 // Precision of the matrix-vector product inside the Krylov loop. FP32 is a
 // relaxed schedule: FP64 products until the wanted pairs' residual estimates
 // fall below fp32_switch × tolerance, FP32 products after that (their errors
-// enter the Ritz vectors through components of that size), back to FP64 if
-// a Ritz pair then fails the test. The Lanczos coefficients,
+// enter the Ritz vectors through components of that size); a Ritz pair that
+// then fails the FP64 test ends the window, which restarts in FP64 from its
+// Ritz vectors. The Lanczos coefficients,
 // reorthogonalization, tridiagonal solve and the final Rayleigh-quotient /
 // residual test always stay FP64.
 enum class MatvecPrecision { FP64 = 0, FP32 = 1 };
@@ -374,7 +376,7 @@ enum class MatvecPrecision { FP64 = 0, FP32 = 1 };
 struct EigshControls {
   MatvecPrecision precision = MatvecPrecision::FP64;
   // check Ritz residual estimates every `check_every` steps inside a window
-  int check_every = 8;
+  int check_every = 4;
   // when true, stop inside the window as soon as the requested pairs have
   // converged (the reference always runs the full window)
   bool early_stop = true;

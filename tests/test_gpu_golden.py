@@ -38,7 +38,7 @@ def terms_of(d):
     return out
 
 
-@pytest.mark.parametrize("mode", ["walk", "image"])
+@pytest.mark.parametrize("mode", ["walk", "image", "stream"])
 @pytest.mark.parametrize("name", CASES)
 def test_fingerprint(qa, monkeypatch, name, mode):
     d = load(name)
@@ -46,8 +46,8 @@ def test_fingerprint(qa, monkeypatch, name, mode):
     print(f"[golden] {name} mode={mode}", flush=True)
     m = qa.Model(basis_of(d), terms_of(d))
     assert m.dim == d["dimension"]
-    if mode == "image":
-        assert m.hv_config()["mode"] == "image"
+    if mode != "walk":
+        assert m.hv_config()["mode"] == mode
     s, o = m.states()
     assert hashlib.sha256(s.astype("<u4").tobytes()).hexdigest() == d["states_sha256"]
     assert hashlib.sha256(o.astype("u1").tobytes()).hexdigest() == d["orbits_sha256"]

@@ -53,7 +53,7 @@ def _ham_ok(name, up):
     return not (name == "xyfield" and up is not None)
 
 
-@pytest.mark.parametrize("mode", ["walk", "image"])
+@pytest.mark.parametrize("mode", ["walk", "image", "stream"])
 @pytest.mark.parametrize("name", sorted(HAMS))
 @pytest.mark.parametrize("L,up,k,p", cases.SECTORS[:18])
 def test_hv_fp64(qa, oracle, monkeypatch, name, L, up, k, p, mode):
@@ -74,6 +74,8 @@ def test_hv_fp64(qa, oracle, monkeypatch, name, L, up, k, p, mode):
         pytest.skip("empty sector")
     m = qa.Model(_basis_json(L, up, k, p), spec)
     assert m.dim == om.dim
+    if mode != "walk":
+        assert m.hv_config()["mode"] == mode
     x = cases.rand_vec(om.dim, seed=L * 100 + (k or 0))
     y_ref = om.apply(x)
     y = m.apply(x.astype(np.complex128))
@@ -233,7 +235,10 @@ def test_dimer_and_l4_anchors(qa):
     assert abs(r["eigenvalues"][0] + 2.0) <= 1e-12
 
 
-def test_evolve_matches_oracle(qa, oracle):
+@pytest.mark.parametrize("mode", ["auto", "walk", "image", "stream"])
+def test_evolve_matches_oracle(qa, oracle, monkeypatch, mode):
+    # several columns in lockstep: the multi-vector H·v kernels of each mode
+    monkeypatch.setenv("QMBED_HV_MODE", mode)
     L = 12
     spec = cases.xxz(L, 1.0)
     om = oracle.Model(L, spec, up=6, momentum=0)
