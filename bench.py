@@ -38,6 +38,11 @@ import numpy as np  # noqa: E402
 
 METRIC = "GS solves/s @ resid<1e-8, warm-start+FP32 vs plain FP64; H·v HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP32 stream H·v at the headline sector (hx = 1,
+# hz = 0.25), from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+# `bench.py --hv-profile` calibrated on a 256 MiB device copy (summary and
+# commands in profiles/hv_stream_ising_L26.md)
+HV_FP32_TRAFFIC_BYTES = 265954527
 L_SCAN = 26
 BATCH = 16
 N_TRAIN = 256
@@ -111,6 +116,30 @@ def cpu_baseline(n_target):
         return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
 
 
+def hv_profile(qa):
+    """Kernels for counter runs: the FP32 and FP64 H·v of the headline sector
+    at hx = 1, hz = 0.25 (20 timed launches each after the autotuner's own),
+    and 5 device-to-device copies of 256 MiB, whose counter readings
+    calibrate the bytes the H·v counters report."""
+    static, comps = ising_family(L_SCAN)
+    model = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
+                     components=comps)
+    model.set_parameters(hx=1.0, hz=0.25)
+    out = {"hv_fp32": hv_roofline(model, np.float32, 4, iters=20),
+           "hv_fp64": hv_roofline(model, np.float64, 8, iters=20)}
+    L = qa.library()
+    nbytes = 256 << 20
+    a, b = qa.DeviceBuffer(nbytes), qa.DeviceBuffer(nbytes)
+    for _ in range(5):
+        L.qmbed_hip_memcpy(b.ptr, a.ptr, nbytes, 3)
+    L.qmbed_hip_synchronize()
+    a.free()
+    b.free()
+    out["copy_bytes"] = nbytes
+    model.close()
+    print(json.dumps(out))
+
+
 def mean(xs):
     return float(np.mean(xs)) if len(xs) else None
 
@@ -122,6 +151,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extended", action="store_true")
+    ap.add_argument("--hv-profile", action="store_true",
+                    help="only the headline sector's FP32 and FP64 H·v (20 launches each) and 5 device copies of "
+                         "256 MiB, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (sectors, scan points and right-hand sides "
@@ -129,6 +161,10 @@ def main():
         sys.exit(2)
 
     import qmbed_amd as qa
+
+    if args.hv_profile:
+        hv_profile(qa)
+        return
 
     # ---- setup: family, training scan, network ----
     static, comps = ising_family(L_SCAN)
@@ -268,7 +304,8 @@ def main():
                                "<= 1e-8*max(|E|,1); 16 points per step",
                    "global_batch": BATCH},
         "roofline": {"bound": "hbm", "achieved": hv32["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hv32["frac"], "traffic": None, "kernel": "hv_kernel (FP32 H·v, L=26 Ising sector)"},
+                     "frac": hv32["frac"], "traffic": HV_FP32_TRAFFIC_BYTES if hv32["mode"] == "stream" else None,
+                     "kernel": f"FP32 H·v, L=26 Ising sector ({hv32['mode']} mode)"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(model.dim),
         "extras": extras,
     }
