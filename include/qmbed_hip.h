@@ -105,7 +105,9 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * is bound to a model created with "components" (one real coefficient per
  * component is one scan coordinate).
  * options_json: {"snapshots":32,"hidden":[64,64],"epochs":4000,
- * "learning_rate":0.002,"seed":0}, any subset, NULL for all defaults. */
+ * "learning_rate":0.002,"final_learning_rate":<learning_rate>,"seed":0},
+ * any subset, NULL for all defaults (the learning rate decays
+ * exponentially to final_learning_rate; equal values keep it constant). */
 typedef struct qmbed_hip_scan qmbed_hip_scan;
 qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
 void qmbed_hip_scan_free(qmbed_hip_scan* scan);

@@ -880,6 +880,9 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
       }
       if (auto* x = v.get("epochs"); x && !x->is_null()) o.epochs = int(as_long(*x, "epochs"));
       if (auto* x = v.get("learning_rate"); x && !x->is_null()) o.learning_rate = float(as_double(*x, "learning_rate"));
+      o.final_learning_rate = o.learning_rate;  // constant unless given
+      if (auto* x = v.get("final_learning_rate"); x && !x->is_null())
+        o.final_learning_rate = float(as_double(*x, "final_learning_rate"));
       if (auto* x = v.get("seed"); x && !x->is_null()) o.seed = as_u64(*x, "seed");
     }
     auto ts = std::make_unique<TypedScan>();
@@ -1009,7 +1012,7 @@ int qmbed_hip_mlp_train(int in, int h1, int h2, int out, float* params, const fl
     QMBED_HIP(hipMemcpy(dp.data(), params, size_t(np) * 4, hipMemcpyHostToDevice));
     QMBED_HIP(hipMemcpy(dx.data(), x, size_t(n) * in * 4, hipMemcpyHostToDevice));
     QMBED_HIP(hipMemcpy(dt.data(), t, size_t(n) * out * 4, hipMemcpyHostToDevice));
-    mlp_train_device(in, h1, h2, out, dp.data(), dx.data(), dt.data(), n, epochs, learning_rate,
+    mlp_train_device(in, h1, h2, out, dp.data(), dx.data(), dt.data(), n, epochs, learning_rate, 1.0f,
                      epochs == 0 ? dg.data() : nullptr, dl.data(), loss_history ? dh.data() : nullptr, nullptr);
     QMBED_HIP(hipDeviceSynchronize());
     if (epochs == 0) {

@@ -21,10 +21,12 @@ int mlp_parameter_count(int in, int h1, int h2, int out);
 // Y [n][out] = MLP(X [n][in]) on the device
 void mlp_forward_device(int in, int h1, int h2, int out, const float* d_params, const float* d_x, int n, float* d_y,
                         hipStream_t s);
-// epochs > 0: full-batch Adam on (X, T), parameters updated in place, the
-// loss of every epoch into d_loss_hist (optional); epochs == 0: one
-// gradient into d_grad and the loss into d_loss
+// epochs > 0: full-batch Adam on (X, T) with the learning rate multiplied
+// by lr_decay after every epoch, parameters updated in place, the loss of
+// every epoch into d_loss_hist (optional); epochs == 0: one gradient into
+// d_grad and the loss into d_loss
 void mlp_train_device(int in, int h1, int h2, int out, float* d_params, const float* d_x, const float* d_t, int n,
-                      int epochs, float lr, float* d_grad, float* d_loss, float* d_loss_hist, hipStream_t s);
+                      int epochs, float lr, float lr_decay, float* d_grad, float* d_loss, float* d_loss_hist,
+                      hipStream_t s);
 
 }  // namespace qmbed

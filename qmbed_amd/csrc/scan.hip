@@ -106,6 +106,8 @@ WarmStartScan::WarmStartScan(Operator& family, hipStream_t s, const ScanOptions&
   if (o.snapshots < 1 || o.snapshots > 64) throw invalid_options("snapshots must lie in [1, 64]");
   if (o.epochs < 1) throw invalid_options("epochs must be positive");
   if (!(o.learning_rate > 0.0f)) throw invalid_options("learning_rate must be positive");
+  if (!(o.final_learning_rate > 0.0f && o.final_learning_rate <= o.learning_rate))
+    throw invalid_options("final_learning_rate must be positive and at most learning_rate");
   P_ = int(op_.components().size());
   if (P_ > 8) throw invalid_options("the warm-start network takes at most 8 parameters");
   (void)mlp_parameter_count(P_, o.hidden1, o.hidden2, o.snapshots);
@@ -283,8 +285,12 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   QMBED_HIP(hipMemcpyAsync(params_.data(), init.data(), init.size() * 4, hipMemcpyHostToDevice, s_));
   QMBED_HIP(hipMemcpyAsync(dX.data(), X.data(), X.size() * 4, hipMemcpyHostToDevice, s_));
   QMBED_HIP(hipMemcpyAsync(dT.data(), T.data(), T.size() * 4, hipMemcpyHostToDevice, s_));
+  const float decay =
+      opt_.epochs > 1 ? float(std::pow(double(opt_.final_learning_rate) / double(opt_.learning_rate),
+                                       1.0 / double(opt_.epochs - 1)))
+                      : 1.0f;
   mlp_train_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dX.data(), dT.data(), n, opt_.epochs,
-                   opt_.learning_rate, nullptr, dLoss.data(), dHist.data(), s_);
+                   opt_.learning_rate, decay, nullptr, dLoss.data(), dHist.data(), s_);
   std::vector<float> hist(size_t(opt_.epochs));
   QMBED_HIP(hipMemcpyAsync(hist.data(), dHist.data(), hist.size() * 4, hipMemcpyDeviceToHost, s_));
   QMBED_HIP(hipStreamSynchronize(s_));

@@ -10,6 +10,9 @@ struct ScanOptions {
   int hidden1 = 64, hidden2 = 64;
   int epochs = 4000;  // full-batch Adam epochs
   float learning_rate = 2.0e-3f;
+  // learning rate of the last epoch (exponential decay from learning_rate;
+  // equal to it: constant)
+  float final_learning_rate = 2.0e-3f;
   uint64_t seed = 0;  // network initialization
 };
 
