@@ -90,6 +90,18 @@ double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void*
  * "early_stop" (bool). */
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
 
+/* Solve `count` models (typically the symmetry sectors of one Hamiltonian)
+ * with one SolverRequest in a lockstep batch: every Lanczos launch covers all
+ * unconverged sectors. Response: {"kind":"eigensystems","results":[one
+ * qmbed_hip_eigsh_json result per model, in order],"batch":{"sectors",
+ * "batched","fallback","steps","ritz_rounds","window","seconds"}}. Every
+ * sector starts from the deterministic start vector (no initial_vector);
+ * FP64 matvecs. No counterpart in the reference's ABI: its callers loop over
+ * eigsh per sector (the same results, within the solver tolerance); the
+ * JSON command {"operation":"eigsh_models","handles":[...],"solver":{...}}
+ * is the same call on registered handles. */
+char* qmbed_hip_eigsh_batch_json(qmbed_hip_model* const* models, int count, const char* solver_json);
+
 /* Evaluate a parameterized model (created with "components") at
  * {"name": [re, im], ...}: H = H_static + sum_p c_p H_p, defaults filling
  * absent names. Later applies and solves use these values. Answers the

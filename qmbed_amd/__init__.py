@@ -69,6 +69,8 @@ def library():
     L.qmbed_hip_time_apply.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int]
     L.qmbed_hip_eigsh_json.restype = c_void_p
     L.qmbed_hip_eigsh_json.argtypes = [c_void_p, c_char_p]
+    L.qmbed_hip_eigsh_batch_json.restype = c_void_p
+    L.qmbed_hip_eigsh_batch_json.argtypes = [ctypes.POINTER(c_void_p), c_int, c_char_p]
     L.qmbed_hip_model_set_parameters.restype = c_void_p
     L.qmbed_hip_model_set_parameters.argtypes = [c_void_p, c_char_p]
     L.qmbed_hip_scan_create.restype = c_void_p
@@ -286,6 +288,15 @@ class Model:
         if parameters is not None:
             req["parameters"] = {n: [complex(v).real, complex(v).imag] for n, v in parameters.items()}
         return _call_model(self._h, req)
+
+
+def eigsh_batch(models, k, target="smallest_algebraic", **solver):
+    """Solve several models (e.g. the symmetry sectors of one Hamiltonian) in
+    one lockstep batch; returns {"results": [one eigsh result per model],
+    "batch": {...}}."""
+    req = {"eigenpairs": k, "target": {"kind": target}, **solver}
+    handles = (ctypes.c_void_p * len(models))(*[m.handle for m in models])
+    return _envelope(library().qmbed_hip_eigsh_batch_json(handles, len(models), json.dumps(req).encode()))
 
 
 def _call_model(h, req):

@@ -6,6 +6,7 @@
 // Every entry point catches all exceptions and answers
 // {"status":"error","error":"..."}; responses are malloc'd C strings owned by
 // the caller until qmbed_string_free.
+#include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <map>
@@ -406,6 +407,45 @@ Eigensystem solve_model(Model& m, const SolverReq& r, const Value* parameters = 
   return eigsh(op, r.opts, r.has_initial ? r.initial.data() : nullptr, r.ctl, r.eigenvectors, m.stream);
 }
 
+std::string counters_body(size_t dim, const Eigensystem& es, bool vectors) {
+  std::string body = eigensystem_body(dim, es, vectors, false);
+  body.pop_back();
+  body += ",\"fp32_iterations\":" + std::to_string(es.fp32_iterations) +
+          ",\"fp64_iterations\":" + std::to_string(es.fp64_iterations) +
+          ",\"reorthogonalization_passes\":" + std::to_string(es.reorthogonalization_passes) +
+          ",\"conditional_second_passes\":" + std::to_string(es.conditional_second_passes) + "}";
+  return body;
+}
+
+// Many models in one lockstep batch (eigsh_sectors). Each model is used at
+// its current evaluation; the batch runs on the first model's stream.
+std::string solve_models(const std::vector<std::shared_ptr<Model>>& models, const SolverReq& r) {
+  if (models.empty()) bad("the batch names no models");
+  if (r.has_initial) bad("the sector batch starts every sector from the deterministic start vector");
+  std::vector<Model*> order;
+  for (auto& m : models) order.push_back(m.get());
+  std::sort(order.begin(), order.end());
+  if (std::adjacent_find(order.begin(), order.end()) != order.end()) bad("a model appears twice in the batch");
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (Model* m : order) locks.emplace_back(m->mu);
+  std::vector<Operator*> ops;
+  for (auto& m : models) ops.push_back(&m->materialized());
+  SectorBatchStats st;
+  auto res = eigsh_sectors(ops, r.opts, r.ctl, r.eigenvectors, models[0]->stream, &st);
+  std::string o = "{\"kind\":\"eigensystems\",\"results\":[";
+  for (size_t i = 0; i < res.size(); ++i) {
+    if (i) o += ',';
+    o += counters_body(ops[i]->dim(), res[i], r.eigenvectors);
+  }
+  o += "],\"batch\":{\"sectors\":" + std::to_string(st.sectors) + ",\"batched\":" + std::to_string(st.batched) +
+       ",\"fallback\":" + std::to_string(st.fallback) + ",\"steps\":" + std::to_string(st.steps) +
+       ",\"ritz_rounds\":" + std::to_string(st.ritz_rounds) + ",\"window\":" + std::to_string(st.window) +
+       ",\"seconds\":";
+  json::write_number(o, st.seconds);
+  o += "}}";
+  return o;
+}
+
 std::string run_solve(const std::string& req) {
   Value v;
   try {
@@ -469,6 +509,16 @@ std::string run_command(const std::string& req) {
     SolverReq r = parse_solver(need(v, "solver", "eigsh_model"));
     Eigensystem es = solve_model(*m, r, v.get("parameters"));
     return ok(eigensystem_body(m->basis->len(), es, r.eigenvectors, true));
+  }
+  if (op == "eigsh_models") {
+    // extension: several registered models (sectors) solved in one batch
+    std::vector<std::shared_ptr<Model>> ms;
+    for (auto& h : need(v, "handles", "eigsh_models").array()) {
+      if (h.kind != Value::String) bad("handle must be a string");
+      ms.push_back(registered_model(h.raw));
+    }
+    SolverReq r = parse_solver(need(v, "solver", "eigsh_models"));
+    return ok(solve_models(ms, r));
   }
   if (op == "evolve_model") {
     const Value& h = need(v, "handle", "evolve_model");
@@ -827,13 +877,25 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
     }
     SolverReq r = parse_solver(v);
     Eigensystem es = solve_model(*tm->m, r, v.get("parameters"));
-    std::string body = eigensystem_body(tm->m->basis->len(), es, r.eigenvectors, false);
-    body.pop_back();
-    body += ",\"fp32_iterations\":" + std::to_string(es.fp32_iterations) +
-            ",\"fp64_iterations\":" + std::to_string(es.fp64_iterations) +
-            ",\"reorthogonalization_passes\":" + std::to_string(es.reorthogonalization_passes) +
-            ",\"conditional_second_passes\":" + std::to_string(es.conditional_second_passes) + "}";
-    return ok(body);
+    return ok(counters_body(tm->m->basis->len(), es, r.eigenvectors));
+  });
+}
+
+char* qmbed_hip_eigsh_batch_json(qmbed_hip_model* const* models, int count, const char* solver_json) {
+  return guarded(solver_json, [&](const std::string& req) {
+    if (count <= 0 || !models) bad("the batch names no models");
+    std::vector<std::shared_ptr<Model>> ms;
+    for (int i = 0; i < count; ++i) {
+      if (!models[i]) bad("the batch holds a null model");
+      ms.push_back(reinterpret_cast<TypedModel*>(models[i])->m);
+    }
+    json::Value v;
+    try {
+      v = json::parse(req);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    return ok(solve_models(ms, parse_solver(v)));
   });
 }
 

@@ -878,9 +878,11 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
 // H·v from the compiled stream: one lane per row; the words of B ordinals
 // are loaded together (coalesced over the wavefront), then the B input
 // entries they name, then the B products are accumulated.
+// The rows of block `blk` of one sector (shared by the single-sector kernel
+// and the sector-batched kernel).
 template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
-__global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
-  if (a.gate && *a.gate) return;
+__device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict__ x, S* __restrict__ y,
+                                            uint32_t blk) {
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
   double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
@@ -897,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __
   for (int i = threadIdx.x; i < 65; i += kBlock) s_sq[i] = sqrt(double(i));
   __syncthreads();
 
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t row = blk * kBlock + threadIdx.x;
   const uint32_t slice = row / kSlice, lane = row % kSlice;
   const uint32_t n_slices = (a.n + kSlice - 1) / kSlice;
   if (slice >= n_slices) return;  // whole wavefronts leave together
@@ -1008,6 +1010,41 @@ __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __
       y[2 * (v * a.ld + row)] = acc[v].re;
       y[2 * (v * a.ld + row) + 1] = acc[v].im;
     }
+  }
+}
+
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
+__global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
+  if (a.gate && *a.gate) return;
+  stream_rows<S, CMODE, SYM, SHIFT, NV>(a, x, y, blockIdx.x);
+}
+
+// One sector of a sector-batched action: its tables and where its rows sit
+// in the concatenated (complex) vectors.
+struct BatchSector {
+  HvArgs a;
+  size_t offset;  // first row in the concatenated vectors
+  int cmode;      // 1: real amplitudes, 2: complex amplitudes
+  int shift;      // character phases needed (k != 0)
+};
+
+// y = H x on many sectors in one launch: block b applies the rows of block
+// (item & 0xFFFF) of sector (item >> 16), item = blocks[b].
+template <int SYM>
+__global__ void __launch_bounds__(kBlock) hv_stream_batch_kernel(const BatchSector* __restrict__ sec,
+                                                                 const uint32_t* __restrict__ blocks,
+                                                                 const double* __restrict__ x, double* __restrict__ y) {
+  const uint32_t item = blocks[blockIdx.x];
+  const BatchSector& bs = sec[item >> 16];
+  const uint32_t blk = item & 0xFFFFu;
+  const double* xs = x + 2 * bs.offset;
+  double* ys = y + 2 * bs.offset;
+  if (bs.cmode == 2) {
+    if (bs.shift) stream_rows<double, 2, SYM, true, 1>(bs.a, xs, ys, blk);
+    else stream_rows<double, 2, SYM, false, 1>(bs.a, xs, ys, blk);
+  } else {
+    if (bs.shift) stream_rows<double, 1, SYM, true, 1>(bs.a, xs, ys, blk);
+    else stream_rows<double, 1, SYM, false, 1>(bs.a, xs, ys, blk);
   }
 }
 
@@ -1728,6 +1765,57 @@ size_t Operator::stored_entries(hipStream_t s) const {
   size_t tot = 0;
   for (uint32_t c : h) tot += c;
   return tot;
+}
+
+static_assert(SectorBatchAction::kRows == kBlock, "batched blocks are H·v blocks");
+
+SectorBatchAction::SectorBatchAction(const std::vector<Operator*>& ops, const std::vector<size_t>& offsets,
+                                     hipStream_t s) {
+  if (ops.size() != offsets.size()) throw dimension_mismatch("sector batch: one offset per operator is required");
+  if (ops.size() >= (size_t(1) << 16)) throw unsupported_backend("sector batch: at most 65535 sectors");
+  count_ = int(ops.size());
+  std::vector<BatchSector> host(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) {
+    Operator& op = *ops[i];
+    const SpinBasis1D& b = op.basis();
+    if (!op.hermitian()) throw non_hermitian();
+    const int sym = sym_code(b);
+    if (i == 0) sym_ = sym;
+    if (sym != sym_) throw unsupported_backend("sector batch: every sector must use the same symmetry kind");
+    auto& t = *op.tables_;
+    if (!t.use_stream && !build_stream(op, s))
+      throw unsupported_backend("sector batch: a sector does not fit the compiled index stream");
+    const uint32_t nb = uint32_t((b.len() + kRows - 1) / kRows);
+    if (nb > 0xFFFFu) throw unsupported_backend("sector batch: a sector exceeds 65535 blocks");
+    blocks_.push_back(nb);
+    BatchSector bs{};
+    bs.a = t.args(b);
+    bs.a.stream = t.stream.data();
+    bs.a.slice_ptr = t.slice_ptr.data();
+    bs.a.img = nullptr;
+    bs.a.gate = nullptr;
+    bs.offset = offsets[i];
+    bs.cmode = op.is_real() ? 1 : 2;
+    bs.shift = b.geometry().kmom > 0 ? 1 : 0;
+    host[i] = bs;
+    const size_t G = size_t(bs.a.n_diag + bs.a.n_off);
+    smem_ = std::max(smem_, sizeof(double) * (2 * size_t(bs.a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G);
+  }
+  sec_.resize(std::max<size_t>(1, host.size() * sizeof(BatchSector)));
+  if (!host.empty())
+    QMBED_HIP(hipMemcpy(sec_.data(), host.data(), host.size() * sizeof(BatchSector), hipMemcpyHostToDevice));
+}
+
+void SectorBatchAction::apply(const uint32_t* d_items, int nitems, const double* x, double* y, hipStream_t s) const {
+  if (nitems <= 0) return;
+  const BatchSector* sec = reinterpret_cast<const BatchSector*>(sec_.data());
+  switch (sym_) {
+    case 0: hipLaunchKernelGGL(hv_stream_batch_kernel<0>, dim3(nitems), dim3(kBlock), smem_, s, sec, d_items, x, y); break;
+    case 1: hipLaunchKernelGGL(hv_stream_batch_kernel<1>, dim3(nitems), dim3(kBlock), smem_, s, sec, d_items, x, y); break;
+    case 2: hipLaunchKernelGGL(hv_stream_batch_kernel<2>, dim3(nitems), dim3(kBlock), smem_, s, sec, d_items, x, y); break;
+    default: hipLaunchKernelGGL(hv_stream_batch_kernel<3>, dim3(nitems), dim3(kBlock), smem_, s, sec, d_items, x, y); break;
+  }
+  QMBED_HIP(hipGetLastError());
 }
 
 void Operator::materialize_dense(cplx* dense, hipStream_t s) const {

@@ -15,6 +15,29 @@ void compile_family_groups(const SpinBasisSpec& spec, const std::vector<std::vec
                            const AssemblyChecks& checks, bool nonempty, std::vector<FlipGroup>& diag,
                            std::vector<FlipGroup>& off, bool& real);
 
+// ---- sector-batched action (hv.hip) ----
+// One launch applies many Hermitian sector operators, each to its own segment
+// of concatenated complex FP64 vectors (sector i owns rows [offsets[i],
+// offsets[i] + dim_i)), over the sectors' compiled index streams. The
+// operators must stay alive; all must share one symmetry kind (translation,
+// parity, both or none).
+class SectorBatchAction {
+ public:
+  static constexpr int kRows = 256;  // rows per block
+  SectorBatchAction(const std::vector<Operator*>& ops, const std::vector<size_t>& offsets, hipStream_t s);
+  int sectors() const { return count_; }
+  // block count of one sector (kRows rows each)
+  uint32_t blocks_of(int sector) const { return blocks_[size_t(sector)]; }
+  // y = H x over the listed blocks: d_items[b] = sector << 16 | block
+  void apply(const uint32_t* d_items, int nitems, const double* x, double* y, hipStream_t s) const;
+
+ private:
+  int count_ = 0;
+  int sym_ = 0;
+  size_t smem_ = 0;
+  std::vector<uint32_t> blocks_;
+  DeviceArray<uint8_t> sec_;  // per-sector kernel arguments
+};
 
 // ---- FP32 MLP of the learned warm start (mlp.hip) ----
 int mlp_parameter_count(int in, int h1, int h2, int out);

@@ -10,6 +10,7 @@
 // the whole L2 of an XCD, which costs more than the redundant adds from L2).
 #include <algorithm>
 
+#include "device_common.hpp"
 #include "internal.hpp"
 #include "lanczos_kernels.hpp"
 
@@ -386,26 +387,10 @@ __global__ __launch_bounds__(kT) void k_from32(const float* y, size_t n, E* x, c
   }
 }
 
-// deterministic_start (QMBED src/solve/mod.rs:729-742) with LCG jump-ahead:
-// element i holds the (i+1)-th state of x <- a x + c from x0 = seed | 1.
+// deterministic_start (QMBED src/solve/mod.rs:729-742)
 __global__ void k_det_start(double* out, size_t n, uint64_t seed) {
-  const uint64_t A = 6364136223846793005ULL, C = 1442695040888963407ULL;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    uint64_t steps = uint64_t(i) + 1;
-    uint64_t ra = 1, rc = 0;  // accumulated affine map
-    uint64_t pa = A, pc = C;  // current power
-    while (steps) {
-      if (steps & 1) {  // r = p ∘ r
-        rc = pa * rc + pc;
-        ra = pa * ra;
-      }
-      pc = pa * pc + pc;
-      pa = pa * pa;
-      steps >>= 1;
-    }
-    uint64_t x = ra * (seed | 1ULL) + rc;
-    out[i] = double(x >> 11) / 9007199254740992.0 - 0.5;
-  }
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT)
+    out[i] = deterministic_entry(uint64_t(i), seed);
 }
 
 // column tile of the overlap kernel: 16 real or 8 complex columns

@@ -406,6 +406,25 @@ struct Eigensystem {
 Eigensystem eigsh(const Operator& op, const EigshOptions& opts, const cplx* initial = nullptr,
                   const EigshControls& ctl = EigshControls(), bool want_vectors = false, hipStream_t s = nullptr);
 
+// Many sectors at once (sector_batch.hip): the Krylov-route Hermitian
+// sectors run one fully reorthogonalized Lanczos window each, in lockstep, so
+// every launch of a step covers all of them; sectors that do not converge
+// within the common window, dense-route and non-Hermitian sectors are solved
+// one by one afterwards. Results are in input order; each sector's pairs pass
+// the same FP64 residual test as eigsh. FP64 matvecs only.
+struct SectorBatchStats {
+  int sectors = 0;      // sectors requested
+  int batched = 0;      // converged inside the lockstep batch
+  int fallback = 0;     // batched sectors finished by the single-sector solver
+  long steps = 0;       // lockstep steps launched
+  int ritz_rounds = 0;  // Ritz tests (each one synchronization)
+  int window = 0;       // common Krylov window
+  double seconds = 0;   // wall time of the call
+};
+std::vector<Eigensystem> eigsh_sectors(const std::vector<Operator*>& ops, const EigshOptions& opts,
+                                       const EigshControls& ctl, bool want_vectors, hipStream_t s,
+                                       SectorBatchStats* stats = nullptr);
+
 // Device-side variant: initial vector already on the device (real, FP64);
 // returned eigenvectors (if requested) are written to `vectors_out` (device,
 // real FP64, eigenpairs × dim) for real operators.
