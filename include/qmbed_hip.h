@@ -102,6 +102,18 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json
  * is the same call on registered handles. */
 char* qmbed_hip_eigsh_batch_json(qmbed_hip_model* const* models, int count, const char* solver_json);
 
+/* Krylov evolution of device-resident states: `states` holds ncols complex128
+ * columns of length dim ([column][row]); `out` receives
+ * [time][column][row] (times × ncols × dim complex128). evolution_json:
+ * {"times":[...], "krylov_dimension" (default 30, EvolutionOptions::new),
+ * "tolerance" (1e-10), "max_substeps" (10000), "parameters" (families)}.
+ * Answers {"kind":"evolution","dimension","times"} plus the reference's
+ * EvolutionDiagnostics counters. Replaces the vector transfer of
+ * evolve_model (QMBED capi evolve_model_json; evolve_batch,
+ * src/solve/mod.rs:4141-4170) for callers whose states live on the GPU. */
+char* qmbed_hip_evolve_device(qmbed_hip_model* model, const void* states, int ncols, void* out,
+                              const char* evolution_json);
+
 /* Evaluate a parameterized model (created with "components") at
  * {"name": [re, im], ...}: H = H_static + sum_p c_p H_p, defaults filling
  * absent names. Later applies and solves use these values. Answers the

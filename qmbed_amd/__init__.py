@@ -70,6 +70,8 @@ def library():
     L.qmbed_hip_eigsh_json.restype = c_void_p
     L.qmbed_hip_eigsh_json.argtypes = [c_void_p, c_char_p]
     L.qmbed_hip_eigsh_batch_json.restype = c_void_p
+    L.qmbed_hip_evolve_device.restype = c_void_p
+    L.qmbed_hip_evolve_device.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_char_p]
     L.qmbed_hip_eigsh_batch_json.argtypes = [ctypes.POINTER(c_void_p), c_int, c_char_p]
     L.qmbed_hip_model_set_parameters.restype = c_void_p
     L.qmbed_hip_model_set_parameters.argtypes = [c_void_p, c_char_p]
@@ -273,6 +275,28 @@ class Model:
         if rc != 0:
             raise QmbedError(f"apply failed (code {rc})")
         return y
+
+    def evolve(self, states, times, **options):
+        """Krylov evolution exp(-iHt)ψ of the columns of `states` (ncols × dim,
+        complex) through the device-resident entry point; returns
+        (outputs [time][column][dim], diagnostics)."""
+        x = np.ascontiguousarray(np.atleast_2d(states), dtype=np.complex128)
+        nc = x.shape[0]
+        if x.shape[1] != self.dim:
+            raise QmbedError("states do not match the model dimension")
+        bx = DeviceBuffer(max(x.nbytes, 16))
+        bo = DeviceBuffer(max(x.nbytes * len(times), 16))
+        try:
+            bx.upload(x)
+            req = {"times": list(map(float, times)), **options}
+            res = _envelope(library().qmbed_hip_evolve_device(self._h, bx.ptr, nc, bo.ptr,
+                                                              json.dumps(req).encode()))
+            out = np.empty((len(times), nc, self.dim), dtype=np.complex128)
+            bo.download(out)
+        finally:
+            bx.free()
+            bo.free()
+        return out, res
 
     def set_parameters(self, **values):
         """Evaluate a family at the named coefficients (complex allowed)."""

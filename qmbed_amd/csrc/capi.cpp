@@ -881,6 +881,42 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
   });
 }
 
+char* qmbed_hip_evolve_device(qmbed_hip_model* m, const void* states, int ncols, void* out,
+                              const char* evolution_json) {
+  return guarded(evolution_json, [&](const std::string& req) {
+    auto* tm = reinterpret_cast<TypedModel*>(m);
+    json::Value v;
+    try {
+      v = json::parse(req);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    if (ncols < 0) bad("ncols must not be negative");
+    if (ncols > 0 && (!states || !out)) bad("states and out must be device pointers");
+    EvolutionOptions eo;  // EvolutionOptions::new defaults (Krylov 30)
+    for (auto& t : need(v, "times", "evolution").array()) eo.times.push_back(as_double(t, "times"));
+    if (auto* x = v.get("krylov_dimension"); x && !x->is_null()) eo.krylov_dimension = as_long(*x, "krylov_dimension");
+    if (auto* x = v.get("tolerance"); x && !x->is_null()) eo.tolerance = as_double(*x, "tolerance");
+    if (auto* x = v.get("max_substeps"); x && !x->is_null()) eo.max_substeps = as_long(*x, "max_substeps");
+    std::lock_guard<std::mutex> g(tm->m->mu);
+    const Operator& H = tm->m->bind(v.get("parameters"));
+    EvolutionDiagnostics dg;
+    evolve_batch_device(H, static_cast<const cplx*>(states), ncols, eo, &dg, static_cast<cplx*>(out), tm->m->stream);
+    std::string o = "{\"kind\":\"evolution\",\"dimension\":" + std::to_string(H.dim()) + ",\"times\":";
+    write_dvec(o, eo.times);
+    o += ",\"lanczos_projections\":" + std::to_string(dg.lanczos_projections) +
+         ",\"matrix_vector_products\":" + std::to_string(dg.matrix_vector_products) +
+         ",\"real_lanczos_projections\":" + std::to_string(dg.real_lanczos_projections) +
+         ",\"real_matrix_vector_products\":" + std::to_string(dg.real_matrix_vector_products) +
+         ",\"accepted_substeps\":" + std::to_string(dg.accepted_substeps) +
+         ",\"rejected_trial_intervals\":" + std::to_string(dg.rejected_trial_intervals) +
+         ",\"maximum_estimated_error\":";
+    json::write_number(o, dg.maximum_estimated_error);
+    o += "}";
+    return ok(o);
+  });
+}
+
 char* qmbed_hip_eigsh_batch_json(qmbed_hip_model* const* models, int count, const char* solver_json) {
   return guarded(solver_json, [&](const std::string& req) {
     if (count <= 0 || !models) bad("the batch names no models");

@@ -101,28 +101,6 @@ __global__ void kb_reduce(const double* partial, int nb, int w, int ncol, double
   out[t] = s;
 }
 
-// W -= alpha V + beta U (per column real coefficients; beta may be null)
-__global__ void kb_three(double2* W, const double2* __restrict__ V, const double2* __restrict__ U, const double* alpha,
-                         const double* beta, const int* active, size_t n) {
-  const int col = blockIdx.y;
-  if (!active[col]) return;
-  const double a = alpha[col], b = beta ? beta[col] : 0.0;
-  double2* w = W + size_t(col) * n;
-  const double2* v = V + size_t(col) * n;
-  const double2* u = U ? U + size_t(col) * n : nullptr;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    double2 x = w[i], y = v[i];
-    x.x -= a * y.x;
-    x.y -= a * y.y;
-    if (u) {
-      double2 z = u[i];
-      x.x -= b * z.x;
-      x.y -= b * z.y;
-    }
-    w[i] = x;
-  }
-}
-
 // out = scale[col] * X (per column real scale; zero when scale is 0)
 __global__ void kb_scale(const double2* X, const double* scale, const int* active, size_t n, double2* out) {
   const int col = blockIdx.y;
@@ -133,22 +111,6 @@ __global__ void kb_scale(const double2* X, const double* scale, const int* activ
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     double2 a = x[i];
     o[i] = make_double2(s * a.x, s * a.y);
-  }
-}
-
-// y = Σ_j c[j] V[j][col] for one column (V stride between j blocks = ldj)
-__global__ void kb_combine(const double2* __restrict__ V, size_t ldj, int m, const double* c, size_t n, double2* y) {
-  __shared__ double shc[2 * 128];
-  for (int j = threadIdx.x; j < 2 * m; j += kT) shc[j] = c[j];
-  __syncthreads();
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    double ar = 0, ai = 0;
-    for (int j = 0; j < m; ++j) {
-      double2 v = V[size_t(j) * ldj + i];
-      ar += v.x * shc[2 * j] - v.y * shc[2 * j + 1];
-      ai += v.x * shc[2 * j + 1] + v.y * shc[2 * j];
-    }
-    y[i] = make_double2(ar, ai);
   }
 }
 
@@ -282,55 +244,235 @@ void EvolutionOptions::validate() const {
     throw invalid_options("evolution controls must be positive");
 }
 
-std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, const std::vector<std::vector<cplx>>& cols,
-                                                         const EvolutionOptions& opts, EvolutionDiagnostics* diag,
-                                                         hipStream_t s) {
+namespace {
+
+// ------------------------------------------- device-resident projections ---
+// One projection of all columns runs without a host round trip: α_j, β_j and
+// the stop of each column (window end or breakdown β <= 1e-14) are decided on
+// the device and kept in a per-column history that the host reads once, at
+// the end of the projection.
+
+// ‖cur_c‖ from the kb_norm partials: init_norm, running = act && norm > eps,
+// inv = 1/norm (0 when not running), size = 0
+__global__ void kb_proj_init(const double* partial, int nb, int nc, const int* act, double* init_norm, int* running,
+                             double* inv, int* size) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += partial[size_t(c) * nb + b];
+  const double nrm = sqrt(s);
+  init_norm[c] = nrm;
+  const bool run = act[c] && nrm > F64_EPS;
+  running[c] = run ? 1 : 0;
+  inv[c] = run ? 1.0 / nrm : 0.0;
+  size[c] = 0;
+}
+
+// α_j (real and imaginary part) of the running columns into the history
+__global__ void kb_alpha_hist(const double* partial, int nb, int nc, const int* running, int m, int j, double* are,
+                              double* aim) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !running[c]) return;
+  double sr = 0, si = 0;
+  for (int b = 0; b < nb; ++b) {
+    sr += partial[(size_t(c) * nb + b) * 2];
+    si += partial[(size_t(c) * nb + b) * 2 + 1];
+  }
+  are[size_t(c) * m + j] = sr;
+  aim[size_t(c) * m + j] = si;
+}
+
+// W -= α_j V_j + β_{j−1} V_{j−1} with the coefficients from the history
+__global__ void kb_three_hist(double2* W, const double2* __restrict__ V, const double2* __restrict__ U,
+                              const double* are, const double* bh, const int* running, int m, int j, size_t n) {
+  const int col = blockIdx.y;
+  if (!running[col]) return;
+  const double a = are[size_t(col) * m + j];
+  const double b = j > 0 ? bh[size_t(col) * m + j - 1] : 0.0;
+  double2* w = W + size_t(col) * n;
+  const double2* v = V + size_t(col) * n;
+  const double2* u = U ? U + size_t(col) * n : nullptr;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    double2 x = w[i], y = v[i];
+    x.x -= a * y.x;
+    x.y -= a * y.y;
+    if (u) {
+      double2 z = u[i];
+      x.x -= b * z.x;
+      x.y -= b * z.y;
+    }
+    w[i] = x;
+  }
+}
+
+// β_j = ‖w‖ into the history; the column stops at the window's end or at a
+// breakdown (src/solve/mod.rs:2909-3054), with size = j + 1
+__global__ void kb_beta_hist(const double* partial, int nb, int nc, int* running, int m, int j, double* bh,
+                             int* size, double* inv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc || !running[c]) return;
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += partial[size_t(c) * nb + b];
+  const double beta = sqrt(s);
+  bh[size_t(c) * m + j] = beta;
+  if (j + 1 == m || beta <= 1.0e-14) {
+    running[c] = 0;
+    size[c] = j + 1;
+    inv[c] = 0.0;
+  } else {
+    inv[c] = 1.0 / beta;
+  }
+}
+
+// one output or state update: dst = Σ_j coef[j] V[j] over `size` Krylov
+// blocks of column `col` (V: that column in block 0), renormalized to the
+// column's initial norm; copy actions (coef < 0) copy V to dst
+struct Act {
+  const double2* V;
+  double2* dst;
+  int size;
+  int coef;  // offset (complex entries) into the coefficient array
+  int col;
+  int pad;
+};
+
+__global__ void kb_combine_acts(const Act* __restrict__ acts, size_t ldj, const double* __restrict__ coefs, size_t n) {
+  __shared__ double shc[2 * 128];
+  const Act a = acts[blockIdx.y];
+  for (int j = threadIdx.x; j < 2 * a.size; j += kT) shc[j] = coefs[2 * size_t(a.coef) + j];
+  __syncthreads();
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    double ar = 0, ai = 0;
+    for (int j = 0; j < a.size; ++j) {
+      const double2 v = a.V[size_t(j) * ldj + i];
+      ar += v.x * shc[2 * j] - v.y * shc[2 * j + 1];
+      ai += v.x * shc[2 * j + 1] + v.y * shc[2 * j];
+    }
+    a.dst[i] = make_double2(ar, ai);
+  }
+}
+
+__global__ void kb_norm_acts(const Act* __restrict__ acts, size_t n, double* partial) {
+  __shared__ double sh[4];
+  const Act a = acts[blockIdx.y];
+  double s = 0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    const double2 v = a.dst[i];
+    s += v.x * v.x + v.y * v.y;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// dst *= init_norm / ‖dst‖ when that norm is finite and above eps
+// (src/solve/mod.rs:3104-3112); every block re-adds the partials in order
+__global__ void kb_renorm_acts(const Act* __restrict__ acts, const double* partial, int nb,
+                               const double* init_norm, size_t n) {
+  const Act a = acts[blockIdx.y];
+  double t = 0;
+  for (int b = 0; b < nb; ++b) t += partial[size_t(blockIdx.y) * nb + b];
+  const double on = sqrt(t);
+  if (!(on > F64_EPS && isfinite(on))) return;
+  const double sc = init_norm[a.col] / on;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    const double2 v = a.dst[i];
+    a.dst[i] = make_double2(sc * v.x, sc * v.y);
+  }
+}
+
+__global__ void kb_copy_acts(const Act* __restrict__ acts, size_t n) {
+  const Act a = acts[blockIdx.y];
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) a.dst[i] = a.V[i];
+}
+
+// per-column Σ |Im x| partials (a column with none is a real state)
+__global__ void kb_imag_abs(const double2* __restrict__ X, size_t n, double* partial) {
+  __shared__ double sh[4];
+  const double2* x = X + size_t(blockIdx.y) * n;
+  double s = 0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) s += fabs(x[i].y);
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+template <typename T>
+struct Pinned {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t want) {
+    if (n >= want) return;
+    if (p) QMBED_HIP(hipHostFree(p));
+    p = nullptr;
+    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault));
+    n = want;
+  }
+  ~Pinned() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+}  // namespace
+
+void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const EvolutionOptions& opts,
+                         EvolutionDiagnostics* diag, cplx* d_out, hipStream_t s) {
   opts.validate();
   const size_t n = op.dim();
-  for (auto& c : cols)
-    if (c.size() != n) throw dimension_mismatch("evolution operator and initial state do not match");
   if (!opts.hamiltonian) throw unsupported_backend("direct-generator evolution is outside the MI355X path");
-  const int nc = int(cols.size());
   const int T = int(opts.times.size());
-  std::vector<std::vector<std::vector<cplx>>> out(T, std::vector<std::vector<cplx>>(nc));
-  if (nc == 0) return out;
   EvolutionDiagnostics dg;
+  if (nc <= 0 || n == 0) {
+    if (diag) *diag = dg;
+    return;
+  }
   const int m = int(std::max<long>(std::min<long>(opts.krylov_dimension, long(n)), 1));
   if (m > 128) throw unsupported_backend("Krylov dimension above 128 exceeds the batched projection kernels");
   const double local_tol = opts.tolerance / double(std::max(T, 1));
+  const int nb = row_blocks(n);
+  const dim3 grid(nb, nc);
 
-  DeviceArray<double> cur(2 * n * nc), W(2 * n * nc), V(2 * n * size_t(nc) * m), tmp(2 * n);
-  DeviceArray<double> dalpha(nc), dbeta(nc), dscale(nc), dcoef(2 * 128);
-  DeviceArray<int> dactive(nc);
-  Batch B(n, nc, s);
-  for (int c = 0; c < nc; ++c)
-    QMBED_HIP(hipMemcpyAsync(cur.data() + 2 * n * c, cols[c].data(), n * 16, hipMemcpyHostToDevice, s));
+  DeviceArray<double> cur(2 * n * nc), W(2 * n * nc), V(2 * n * size_t(nc) * m);
+  DeviceArray<double> partial(size_t(nc) * nb * std::max(2, T + 1) + 64), are(size_t(nc) * m), aim(size_t(nc) * m),
+      bh(size_t(nc) * m), init_norm(nc), inv(nc), coefs(2 * size_t(nc) * (T + 1) * m + 2);
+  DeviceArray<int> act_d(nc), running(nc), size_d(nc);
+  DeviceArray<uint8_t> acts_d(sizeof(Act) * size_t(nc) * (T + 1) + sizeof(Act));
+  Pinned<double> hist;  // are | aim | bh | init_norm | size (as int pairs)
+  hist.ensure(3 * size_t(nc) * m + 2 * size_t(nc) + 8);
+  Pinned<int> act_h;
+  act_h.ensure(nc);
+  Pinned<Act> acts_h;
+  acts_h.ensure(size_t(nc) * (T + 1) + 1);
+  Pinned<double> coef_h;
+  coef_h.ensure(2 * size_t(nc) * (T + 1) * m + 2);
+
+  QMBED_HIP(hipMemcpyAsync(cur.data(), d_cols, 16 * n * nc, hipMemcpyDeviceToDevice, s));
+  // real initial states (for the real-projection counters and the α check)
+  std::vector<int> real_state(nc, 1);
+  {
+    hipLaunchKernelGGL(kb_imag_abs, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), n,
+                       partial.data());
+    std::vector<double> h(size_t(nc) * nb);
+    QMBED_HIP(hipMemcpyAsync(h.data(), partial.data(), h.size() * 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < nc; ++c)
+      for (int b = 0; b < nb; ++b)
+        if (h[size_t(c) * nb + b] != 0.0) real_state[c] = 0;
+  }
+  auto out_col = [&](int q, int c) { return reinterpret_cast<double2*>(d_out) + (size_t(q) * nc + c) * n; };
+  auto cur_col = [&](int c) { return reinterpret_cast<double2*>(cur.data()) + size_t(c) * n; };
   std::vector<double> t(nc, 0.0);
   std::vector<int> oi(nc, 0), substeps(nc, 0), done(nc, 0);
-  std::vector<int> real_state(nc, 0);
-  for (int c = 0; c < nc; ++c) {
-    bool r = true;
-    for (auto& z : cols[c])
-      if (z.imag() != 0.0) r = false;
-    real_state[c] = r;
-  }
-  auto fetch = [&](int c, std::vector<cplx>& dst) {
-    dst.resize(n);
-    QMBED_HIP(hipMemcpyAsync(dst.data(), cur.data() + 2 * n * c, n * 16, hipMemcpyDeviceToHost, s));
-    QMBED_HIP(hipStreamSynchronize(s));
-  };
-  cplx* Vc = reinterpret_cast<cplx*>(V.data());
+  double2* Vc = reinterpret_cast<double2*>(V.data());
   const size_t ldj = n * nc;  // complex elements per Krylov block
   for (;;) {
     // emit outputs whose time is reached; check budgets
-    std::vector<int> act(nc, 0);
     int nact = 0;
     for (int c = 0; c < nc; ++c) {
+      act_h.p[c] = 0;
       if (done[c]) continue;
       while (oi[c] < T) {
-        double target = opts.times[oi[c]];
+        const double target = opts.times[oi[c]];
         if (std::fabs(target - t[c]) <= 16.0 * F64_EPS * std::max(std::fabs(target), 1.0)) {
-          fetch(c, out[oi[c]][c]);
+          QMBED_HIP(hipMemcpyAsync(out_col(oi[c], c), cur_col(c), n * 16, hipMemcpyDeviceToDevice, s));
           t[c] = target;
           ++oi[c];
         } else {
@@ -341,83 +483,95 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
         done[c] = 1;
         continue;
       }
-      if (substeps[c] >= opts.max_substeps)
-        throw non_convergence(substeps[c], std::fabs(opts.times[oi[c]] - t[c]));
-      act[c] = 1;
+      if (substeps[c] >= opts.max_substeps) throw non_convergence(substeps[c], std::fabs(opts.times[oi[c]] - t[c]));
+      act_h.p[c] = 1;
       ++nact;
     }
     if (nact == 0) break;
     // ---- batched Lanczos projection (no reorthogonalization) ----
-    B.norms(reinterpret_cast<const cplx*>(cur.data()), 0);
-    std::vector<Projection> P(nc);
-    std::vector<double> scale(nc, 0.0);
-    std::vector<int> running(nc, 0);
-    for (int c = 0; c < nc; ++c) {
-      P[c].init_norm = std::sqrt(B.host[c]);
-      if (act[c] && P[c].init_norm > F64_EPS) {
-        scale[c] = 1.0 / P[c].init_norm;
-        running[c] = 1;
+    QMBED_HIP(hipMemcpyAsync(act_d.data(), act_h.p, size_t(nc) * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), n,
+                       partial.data());
+    hipLaunchKernelGGL(kb_proj_init, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, act_d.data(),
+                       init_norm.data(), running.data(), inv.data(), size_d.data());
+    hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), inv.data(),
+                       nullptr, n, Vc);
+    // runs of consecutive active columns share one multi-column H·v
+    std::vector<std::pair<int, int>> runs;
+    for (int c = 0; c < nc;) {
+      if (!act_h.p[c]) {
+        ++c;
+        continue;
       }
+      int e = c;
+      while (e < nc && act_h.p[e]) ++e;
+      runs.push_back({c, e - c});
+      c = e;
     }
-    QMBED_HIP(hipMemcpyAsync(dscale.data(), scale.data(), nc * 8, hipMemcpyHostToDevice, s));
-    QMBED_HIP(hipMemcpyAsync(dactive.data(), running.data(), nc * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(kb_scale, B.grid(), dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()),
-                       dscale.data(), nullptr, n, reinterpret_cast<double2*>(V.data()));
-    std::vector<std::vector<double>> alphas(nc), betas(nc);
-    int active_cols = 0;
-    for (int c = 0; c < nc; ++c) active_cols += running[c];
-    for (int j = 0; j < m && active_cols > 0; ++j) {
-      cplx* Vj = Vc + size_t(j) * ldj;
-      op.apply_batch(Vj, reinterpret_cast<cplx*>(W.data()), nc, s);
-      B.zdot(Vj, reinterpret_cast<const cplx*>(W.data()));
-      std::vector<double> a(nc, 0.0), b(nc, 0.0);
-      for (int c = 0; c < nc; ++c) {
-        if (!running[c]) continue;
-        double ai = B.host[2 * c + 1];
-        if (!(op.is_real() && real_state[c]) && std::fabs(ai) > 1.0e-10) throw non_hermitian();
-        a[c] = B.host[2 * c];
-        alphas[c].push_back(a[c]);
-        if (j > 0) b[c] = betas[c][j - 1];
-      }
-      QMBED_HIP(hipMemcpyAsync(dalpha.data(), a.data(), nc * 8, hipMemcpyHostToDevice, s));
-      QMBED_HIP(hipMemcpyAsync(dbeta.data(), b.data(), nc * 8, hipMemcpyHostToDevice, s));
-      QMBED_HIP(hipMemcpyAsync(dactive.data(), running.data(), nc * 4, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(kb_three, B.grid(), dim3(kT), 0, s, reinterpret_cast<double2*>(W.data()),
-                         reinterpret_cast<const double2*>(Vj), j > 0 ? reinterpret_cast<const double2*>(Vj - ldj) : nullptr,
-                         dalpha.data(), j > 0 ? dbeta.data() : nullptr, dactive.data(), n);
-      B.norms(reinterpret_cast<const cplx*>(W.data()), 0);
-      std::vector<double> inv(nc, 0.0);
-      std::vector<int> cont(nc, 0);
-      for (int c = 0; c < nc; ++c) {
-        if (!running[c]) continue;
-        double beta = std::sqrt(B.host[c]);
-        P[c].residual_beta = beta;
-        if (j + 1 == m || beta <= 1.0e-14) {
-          running[c] = 0;
-          --active_cols;
-          continue;
-        }
-        betas[c].push_back(beta);
-        inv[c] = 1.0 / beta;
-        cont[c] = 1;
-      }
-      if (j + 1 < m && active_cols > 0) {
-        QMBED_HIP(hipMemcpyAsync(dscale.data(), inv.data(), nc * 8, hipMemcpyHostToDevice, s));
-        QMBED_HIP(hipMemcpyAsync(dactive.data(), cont.data(), nc * 4, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(kb_scale, B.grid(), dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()),
-                           dscale.data(), dactive.data(), n, reinterpret_cast<double2*>(Vc + size_t(j + 1) * ldj));
-        QMBED_HIP(hipStreamSynchronize(s));  // inv / cont are host staging for the copies above
-      }
+    for (int j = 0; j < m; ++j) {
+      double2* Vj = Vc + size_t(j) * ldj;
+      for (auto& r : runs)
+        op.apply_batch(reinterpret_cast<const cplx*>(Vj + size_t(r.first) * n),
+                       reinterpret_cast<cplx*>(W.data()) + size_t(r.first) * n, r.second, s);
+      hipLaunchKernelGGL(kb_zdot, grid, dim3(kT), 0, s, Vj, reinterpret_cast<const double2*>(W.data()), n,
+                         partial.data());
+      hipLaunchKernelGGL(kb_alpha_hist, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, running.data(),
+                         m, j, are.data(), aim.data());
+      hipLaunchKernelGGL(kb_three_hist, grid, dim3(kT), 0, s, reinterpret_cast<double2*>(W.data()), Vj,
+                         j > 0 ? Vj - ldj : nullptr, are.data(), bh.data(), running.data(), m, j, n);
+      hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), n,
+                         partial.data());
+      hipLaunchKernelGGL(kb_beta_hist, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, running.data(),
+                         m, j, bh.data(), size_d.data(), inv.data());
+      if (j + 1 < m)
+        hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), inv.data(),
+                           running.data(), n, Vj + ldj);
     }
+    QMBED_HIP(hipGetLastError());
+    double* hp = hist.p;
+    const size_t cm = size_t(nc) * m;
+    QMBED_HIP(hipMemcpyAsync(hp, are.data(), cm * 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipMemcpyAsync(hp + cm, aim.data(), cm * 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipMemcpyAsync(hp + 2 * cm, bh.data(), cm * 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipMemcpyAsync(hp + 3 * cm, init_norm.data(), size_t(nc) * 8, hipMemcpyDeviceToHost, s));
+    int* sizes = reinterpret_cast<int*>(hp + 3 * cm + nc);
+    QMBED_HIP(hipMemcpyAsync(sizes, size_d.data(), size_t(nc) * 4, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipStreamSynchronize(s));
     // ---- per-column step control (host, m×m) ----
-    for (int c = 0; c < nc; ++c) {
-      if (!act[c]) continue;
-      Projection& p = P[c];
-      p.size = int(alphas[c].size());
-      if (p.init_norm <= F64_EPS) p.size = 0;
+    int nacts = 0, ncoef = 0;
+    std::vector<Act> copies;
+    auto add_action = [&](int c, const Projection& p, double interval, double2* dst) {
+      Act a{};
+      a.V = Vc + size_t(c) * n;
+      a.dst = dst;
+      a.size = p.size;
+      a.coef = ncoef;
+      a.col = c;
       if (p.size > 0) {
-        std::vector<double> bb(betas[c].begin(), betas[c].begin() + (p.size - 1));
-        tridiagonal_eigen(alphas[c], bb, p.evals, &p.evecs);
+        const auto coef = projected_coefficients(p, interval);
+        for (int jj = 0; jj < p.size; ++jj) {
+          coef_h.p[2 * size_t(ncoef + jj)] = coef[jj].real();
+          coef_h.p[2 * size_t(ncoef + jj) + 1] = coef[jj].imag();
+        }
+        ncoef += p.size;
+      }
+      acts_h.p[nacts++] = a;
+    };
+    for (int c = 0; c < nc; ++c) {
+      if (!act_h.p[c]) continue;
+      Projection p;
+      p.init_norm = hp[3 * cm + c];
+      p.size = p.init_norm <= F64_EPS ? 0 : sizes[c];
+      const double* ac = hp + size_t(c) * m;
+      const double* ic = hp + cm + size_t(c) * m;
+      const double* bc = hp + 2 * cm + size_t(c) * m;
+      if (!(op.is_real() && real_state[c]))
+        for (int j = 0; j < p.size; ++j)
+          if (std::fabs(ic[j]) > 1.0e-10) throw non_hermitian();
+      if (p.size > 0) {
+        std::vector<double> aa(ac, ac + p.size), bb(bc, bc + (p.size - 1));
+        p.residual_beta = bc[p.size - 1];
+        tridiagonal_eigen(aa, bb, p.evals, &p.evecs);
       }
       ++dg.lanczos_projections;
       dg.matrix_vector_products += p.size;
@@ -425,57 +579,27 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
         ++dg.real_lanczos_projections;
         dg.real_matrix_vector_products += p.size;
       }
-      // accepted output: y = V_c · coef, renormalized to init_norm
-      auto action = [&](double interval, double* dst) {
-        if (p.size == 0) {
-          QMBED_HIP(hipMemsetAsync(dst, 0, n * 16, s));
-          return;
-        }
-        auto coef = projected_coefficients(p, interval);
-        std::vector<double> ch(2 * p.size);
-        for (int jj = 0; jj < p.size; ++jj) {
-          ch[2 * jj] = coef[jj].real();
-          ch[2 * jj + 1] = coef[jj].imag();
-        }
-        QMBED_HIP(hipMemcpyAsync(dcoef.data(), ch.data(), ch.size() * 8, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(kb_combine, dim3(row_blocks(n)), dim3(kT), 0, s,
-                           reinterpret_cast<const double2*>(Vc + size_t(c) * n), ldj, p.size, dcoef.data(), n,
-                           reinterpret_cast<double2*>(dst));
-        // renormalize to ‖ψ‖ (src/solve/mod.rs:3104-3112)
-        Batch one(n, 1, s);
-        one.norms(reinterpret_cast<const cplx*>(dst), 0);
-        double on = std::sqrt(one.host[0]);
-        if (on > F64_EPS && std::isfinite(on)) {
-          double sc = p.init_norm / on;
-          QMBED_HIP(hipMemcpyAsync(dscale.data(), &sc, 8, hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(kb_scale, dim3(row_blocks(n), 1), dim3(kT), 0, s, reinterpret_cast<const double2*>(dst),
-                             dscale.data(), nullptr, n, reinterpret_cast<double2*>(dst));
-        }
-        QMBED_HIP(hipStreamSynchronize(s));
-      };
-      const double sc = std::max(P[c].init_norm, 1.0);
-      const double thr = local_tol * sc;
-      double* curc = cur.data() + 2 * n * c;
+      const double thr = local_tol * std::max(p.init_norm, 1.0);
       int accepted = 0;
       double last_time = t[c];
       for (int q = oi[c]; q < T; ++q) {
-        double iv = opts.times[q] - t[c];
-        double err = error_bound(p, iv, n);
+        const double iv = opts.times[q] - t[c];
+        const double err = error_bound(p, iv, n);
         dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
         if (err > thr) {
           ++dg.rejected_trial_intervals;
           break;
         }
-        action(iv, tmp.data());
-        out[q][c].resize(n);
-        QMBED_HIP(hipMemcpyAsync(out[q][c].data(), tmp.data(), n * 16, hipMemcpyDeviceToHost, s));
-        QMBED_HIP(hipStreamSynchronize(s));
+        add_action(c, p, iv, out_col(q, c));
         ++accepted;
         last_time = opts.times[q];
       }
       if (accepted) {
-        // current state = last accepted candidate
-        QMBED_HIP(hipMemcpyAsync(curc, tmp.data(), n * 16, hipMemcpyDeviceToDevice, s));
+        // current state = last accepted output
+        Act cp{};
+        cp.V = out_col(oi[c] + accepted - 1, c);
+        cp.dst = cur_col(c);
+        copies.push_back(cp);
         t[c] = last_time;
         oi[c] += accepted;
         ++substeps[c];
@@ -489,7 +613,7 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
       const double min_iv = 16.0 * F64_EPS * std::max(std::fabs(target), 1.0);
       double rej = std::fabs(tiv), acc = rej;
       for (;;) {
-        double err = error_bound(p, dir * acc, n);
+        const double err = error_bound(p, dir * acc, n);
         dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
         if (err <= thr || acc <= min_iv) break;
         ++dg.rejected_trial_intervals;
@@ -498,8 +622,8 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
       }
       if (acc > min_iv && rej > acc) {
         for (int r = 0; r < 8; ++r) {
-          double trial = 0.5 * (acc + rej);
-          double err = error_bound(p, dir * trial, n);
+          const double trial = 0.5 * (acc + rej);
+          const double err = error_bound(p, dir * trial, n);
           dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
           if (err <= thr) acc = trial;
           else {
@@ -508,17 +632,60 @@ std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, con
           }
         }
       }
-      double iv = dir * acc;
+      const double iv = dir * acc;
       t[c] += iv;
-      action(iv, tmp.data());
-      QMBED_HIP(hipMemcpyAsync(curc, tmp.data(), n * 16, hipMemcpyDeviceToDevice, s));
+      add_action(c, p, iv, cur_col(c));
       ++substeps[c];
       ++dg.accepted_substeps;
       real_state[c] = 0;
     }
+    if (nacts > 0) {
+      const int ncopy = int(copies.size());
+      for (int i = 0; i < ncopy; ++i) acts_h.p[nacts + i] = copies[size_t(i)];
+      QMBED_HIP(hipMemcpyAsync(acts_d.data(), acts_h.p, sizeof(Act) * size_t(nacts + ncopy), hipMemcpyHostToDevice,
+                               s));
+      if (ncoef > 0)
+        QMBED_HIP(hipMemcpyAsync(coefs.data(), coef_h.p, 16 * size_t(ncoef), hipMemcpyHostToDevice, s));
+      const Act* ad = reinterpret_cast<const Act*>(acts_d.data());
+      hipLaunchKernelGGL(kb_combine_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, ldj, coefs.data(), n);
+      hipLaunchKernelGGL(kb_norm_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, n, partial.data());
+      hipLaunchKernelGGL(kb_renorm_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, partial.data(), nb, init_norm.data(),
+                         n);
+      if (ncopy > 0) hipLaunchKernelGGL(kb_copy_acts, dim3(nb, ncopy), dim3(kT), 0, s, ad + nacts, n);
+      QMBED_HIP(hipGetLastError());
+    }
   }
   QMBED_HIP(hipStreamSynchronize(s));
   if (diag) *diag = dg;
+}
+
+std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, const std::vector<std::vector<cplx>>& cols,
+                                                         const EvolutionOptions& opts, EvolutionDiagnostics* diag,
+                                                         hipStream_t s) {
+  opts.validate();
+  const size_t n = op.dim();
+  for (auto& c : cols)
+    if (c.size() != n) throw dimension_mismatch("evolution operator and initial state do not match");
+  if (!opts.hamiltonian) throw unsupported_backend("direct-generator evolution is outside the MI355X path");
+  const int nc = int(cols.size());
+  const int T = int(opts.times.size());
+  std::vector<std::vector<std::vector<cplx>>> out(T, std::vector<std::vector<cplx>>(nc));
+  if (nc == 0) {
+    if (diag) *diag = EvolutionDiagnostics();
+    return out;
+  }
+  DeviceArray<double> in(2 * n * nc), res(2 * n * size_t(nc) * T);
+  for (int c = 0; c < nc; ++c)
+    QMBED_HIP(hipMemcpyAsync(in.data() + 2 * n * c, cols[c].data(), n * 16, hipMemcpyHostToDevice, s));
+  evolve_batch_device(op, reinterpret_cast<const cplx*>(in.data()), nc, opts, diag, reinterpret_cast<cplx*>(res.data()),
+                      s);
+  for (int q = 0; q < T; ++q)
+    for (int c = 0; c < nc; ++c) {
+      out[q][c].resize(n);
+      QMBED_HIP(hipMemcpyAsync(out[q][c].data(), res.data() + 2 * n * (size_t(q) * nc + c), n * 16,
+                               hipMemcpyDeviceToHost, s));
+    }
+  QMBED_HIP(hipStreamSynchronize(s));
   return out;
 }
 

@@ -459,6 +459,12 @@ struct EvolutionDiagnostics {
   double maximum_estimated_error = 0.0;
 };
 
+// Device-resident evolution: d_cols holds ncol complex columns of length
+// dim ([column][row]); d_out receives [time][column][row]. One host
+// synchronization per batched Lanczos projection.
+void evolve_batch_device(const Operator& op, const cplx* d_cols, int ncol, const EvolutionOptions& opts,
+                         EvolutionDiagnostics* diag, cplx* d_out, hipStream_t s);
+
 // Evolve `ncol` host columns (column-major, dim each); output is
 // [time][column][dim] on the host.
 std::vector<std::vector<std::vector<cplx>>> evolve_batch(const Operator& op, const std::vector<std::vector<cplx>>& columns,

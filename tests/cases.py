@@ -60,3 +60,12 @@ SECTORS = [
     (8, 4, 4, -1), (10, 5, 3, None), (10, None, 0, 1), (12, 6, 0, None), (12, 6, 5, None), (12, 6, 6, -1),
     (12, 5, 0, 1), (14, 7, 0, None), (14, None, 7, None), (16, 8, 0, None), (9, 4, 2, None), (9, None, 0, -1),
 ]
+
+
+def seeded_state(n, seed):
+    """Normalized complex Gaussian state (g_re + i g_im)/‖g‖, g ~ N(0, 1) from
+    numpy's default generator seeded with `seed` (the 64 evolution states of
+    the batched-expm workload use seed = column index)."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return g / np.linalg.norm(g)
