@@ -17,8 +17,10 @@ solves per second over the timed steps. Beside it (extras): the plain FP64
 solve (deterministic start) of the same points, mean Lanczos iterations of
 both, per-point energy differences, the amortized throughput of the whole
 4,096-point scan including setup, the H·v bandwidth at this sector, and the
-FP64 6-eigenpair solve of the XXZ L=24 sector and the FP32/FP64 H·v of the
-J1-J2 L=28 sector.
+FP64 6-eigenpair solve of the XXZ L=24 sector, the FP32/FP64 H·v of the
+J1-J2 L=28 sector, the 72 XXZ L=24 momentum/magnetization sectors solved in
+one lockstep batch beside one-by-one solves, and exp(-iHt)ψ of 64 seeded
+states at t ∈ {0.1, 1, 10} on the XXZ L=24 up=12 k=0 sector.
 
 The roofline object is the FP32 H·v of the headline sector: algorithmic bytes
 N·(4+4) + nnz·(4+4) per launch over the kernel time measured with HIP events
@@ -138,6 +140,70 @@ def hv_profile(qa):
     out["copy_bytes"] = nbytes
     model.close()
     print(json.dumps(out))
+
+
+def sector_batch(qa):
+    """XXZ Δ=1, L=24, up ∈ {11,12,13} × k ∈ 0..23: the 72 sectors' ground
+    states (and 6 lowest pairs) in one lockstep batch, beside the same solves
+    one sector at a time."""
+    L = 24
+    terms = xxz_terms(L, 1.0)
+    keys = [(up, k) for up in (11, 12, 13) for k in range(L)]
+    t0 = time.perf_counter()
+    models = [qa.Model({"kind": "spin", "sites": L, "up": up, "momentum": k}, terms) for up, k in keys]
+    build_s = time.perf_counter() - t0
+    dims = [m.dim for m in models]
+    out = {"c4_sectors72": {"build_s": build_s, "sectors": len(models), "dim_min": min(dims), "dim_max": max(dims),
+                            "dim_sum": sum(dims)}}
+    rec = out["c4_sectors72"]
+    for k in (1, 6):
+        qa.eigsh_batch(models, k)  # warm-up (allocations, index streams)
+        t0 = time.perf_counter()
+        b = qa.eigsh_batch(models, k)
+        tb = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        single = [m.eigsh(k) for m in models]
+        ts = time.perf_counter() - t0
+        diff = max(abs(x - y) / abs(y) for r, q in zip(b["results"], single)
+                   for x, y in zip(r["eigenvalues"], q["eigenvalues"]))
+        rec[f"eigenpairs_{k}"] = {
+            "batch_s": tb, "batch_sectors_per_s": len(models) / tb, "steps": b["batch"]["steps"],
+            "ritz_rounds": b["batch"]["ritz_rounds"], "fallback": b["batch"]["fallback"],
+            "mean_iterations": mean([r["iterations"] for r in b["results"]]),
+            "one_by_one_s": ts, "one_by_one_sectors_per_s": len(models) / ts, "speedup": ts / tb,
+            "max_rel_diff_vs_one_by_one": diff,
+            "max_residual": max(x for r in b["results"] for x in r["residuals"])}
+        if k == 1:
+            e = [r["eigenvalues"][0] for r in b["results"]]
+            rec["lowest_sector"] = {"up": keys[int(np.argmin(e))][0], "k": keys[int(np.argmin(e))][1],
+                                    "E0": float(min(e))}
+    for m in models:
+        m.close()
+    return out
+
+
+def expm_batch(qa):
+    """exp(-iHt)ψ for 64 seeded normalized complex states of the XXZ Δ=1, L=24,
+    up=12, k=0 sector (N = 112,720) at t ∈ {0.1, 1, 10}, Krylov 30, tolerance
+    1e-10, device-resident."""
+    from tests import cases
+
+    m = qa.Model({"kind": "spin", "sites": 24, "up": 12, "momentum": 0}, xxz_terms(24, 1.0))
+    states = np.array([cases.seeded_state(m.dim, c) for c in range(64)])
+    times = [0.1, 1.0, 10.0]
+    m.evolve(states[:4], times, krylov_dimension=30)  # warm-up
+    t0 = time.perf_counter()
+    outs, diag = m.evolve(states, times, krylov_dimension=30)
+    dt = time.perf_counter() - t0
+    norm_dev = float(np.max(np.abs(np.linalg.norm(outs, axis=2) - 1.0)))
+    m.close()
+    return {"c4_expm64": {"seconds": dt, "states_per_s": 64 / dt, "sector_dim": int(states.shape[1]),
+                          "times": times, "krylov_dimension": 30, "tolerance": 1e-10,
+                          "lanczos_projections": diag["lanczos_projections"],
+                          "matrix_vector_products": diag["matrix_vector_products"],
+                          "rejected_trial_intervals": diag["rejected_trial_intervals"],
+                          "max_norm_deviation": norm_dev,
+                          "note": "includes the host upload of the 64 states and the download of 192 outputs"}}
 
 
 def mean(xs):
@@ -283,6 +349,8 @@ def main():
         e64 = extras["j1j2_L28_gs_fp64"]["E0"]
         extras["j1j2_L28_gs_fp32"]["rel_err_vs_fp64"] = abs(extras["j1j2_L28_gs_fp32"]["E0"] - e64) / abs(e64)
         big.close()
+        extras.update(sector_batch(qa))
+        extras.update(expm_batch(qa))
 
     scan.close()
     line = {

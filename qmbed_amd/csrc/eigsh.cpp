@@ -76,6 +76,51 @@ std::vector<int> select_indices(const std::vector<double>& v, SpectrumTarget tar
   return idx;
 }
 
+std::vector<double> selected_ritz_values(const std::vector<double>& a, const std::vector<double>& b,
+                                         SpectrumTarget target, int k, double shift,
+                                         std::vector<std::vector<double>>* vectors) {
+  const int n = int(a.size());
+  const bool low = target == SpectrumTarget::SmallestAlgebraic || target == SpectrumTarget::BothEnds ||
+                   target == SpectrumTarget::LargestMagnitude;
+  const bool high = target == SpectrumTarget::LargestAlgebraic || target == SpectrumTarget::BothEnds ||
+                    target == SpectrumTarget::LargestMagnitude;
+  std::vector<double> out;
+  if ((low || high) && 2 * k < n) {
+    // candidates: the k lowest and/or k highest eigenvalues, ascending
+    std::vector<double> cand;
+    if (low) cand = tridiagonal_eigenvalue_range(a, b, 0, k);
+    if (high) {
+      auto h = tridiagonal_eigenvalue_range(a, b, n - k, n);
+      cand.insert(cand.end(), h.begin(), h.end());
+    }
+    for (int i : select_indices(cand, target, k, shift)) out.push_back(cand[size_t(i)]);
+    if (!vectors) return out;
+    bool close = false;
+    for (size_t i = 0; i < out.size() && !close; ++i)
+      for (size_t j = i + 1; j < out.size(); ++j)
+        if (std::fabs(out[i] - out[j]) <= 1.0e-9 * std::max(std::fabs(out[i]), 1.0)) close = true;
+    if (!close) {
+      vectors->clear();
+      for (double v : out) vectors->push_back(tridiagonal_eigvec(a, b, v));
+      return out;
+    }
+    out.clear();
+  }
+  std::vector<double> vals, vecs;
+  tridiagonal_eigen(a, b, vals, vectors ? &vecs : nullptr);
+  const auto sel = select_indices(vals, target, k, shift);
+  if (vectors) vectors->clear();
+  for (int i : sel) {
+    out.push_back(vals[size_t(i)]);
+    if (vectors) {
+      std::vector<double> v(static_cast<size_t>(n));
+      for (int r = 0; r < n; ++r) v[size_t(r)] = vecs[size_t(r) * n + i];
+      vectors->push_back(std::move(v));
+    }
+  }
+  return out;
+}
+
 static bool target_before(SpectrumTarget t, double a, double b, double shift) {
   switch (t) {
     case SpectrumTarget::LargestAlgebraic: return b < a;
@@ -339,13 +384,10 @@ class LanczosWindow {
     if (sz <= k) return INFINITY;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
     std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
-    std::vector<double> vals;
-    tridiagonal_eigen(a, b, vals, nullptr);
-    auto sel = select_indices(vals, target, k, shift);
     double worst = 0;
-    for (int idx : sel) {
-      auto v = tridiagonal_eigvec(a, b, vals[idx]);
-      const double est = last_beta_ * std::fabs(v[sz - 1]) / std::max(std::fabs(vals[idx]), 1.0);
+    for (double val : selected_ritz_values(a, b, target, k, shift)) {
+      auto v = tridiagonal_eigvec(a, b, val);
+      const double est = last_beta_ * std::fabs(v[sz - 1]) / std::max(std::fabs(val), 1.0);
       worst = std::isfinite(est) ? std::max(worst, est) : INFINITY;
     }
     return worst;
@@ -356,14 +398,12 @@ class LanczosWindow {
     const int sz = size_;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
     std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
-    std::vector<double> vals;
-    tridiagonal_eigen(a, b, vals, nullptr);
-    auto sel = select_indices(vals, target, k, shift);
-    if (int(sel.size()) < k) return false;
-    for (int idx : sel) {
-      auto s = tridiagonal_eigvec(a, b, vals[idx]);
+    const auto vals = selected_ritz_values(a, b, target, k, shift);
+    if (int(vals.size()) < k) return false;
+    for (double val : vals) {
+      auto s = tridiagonal_eigvec(a, b, val);
       double est = last_beta_ * std::fabs(s[sz - 1]);
-      if (!std::isfinite(est) || est > safety * tol * std::max(std::fabs(vals[idx]), 1.0)) return false;
+      if (!std::isfinite(est) || est > safety * tol * std::max(std::fabs(val), 1.0)) return false;
     }
     return true;
   }
@@ -374,20 +414,19 @@ class LanczosWindow {
     const int sz = size_;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
     std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
-    std::vector<double> vals, vecs;
-    tridiagonal_eigen(a, b, vals, &vecs);
-    auto sel = select_indices(vals, target, k, shift);
+    std::vector<std::vector<double>> vecs;
+    const auto vals = selected_ritz_values(a, b, target, k, shift, &vecs);
     const size_t e = esz();
-    std::vector<double> C(e * size_t(sz) * sel.size(), 0.0);
-    for (size_t q = 0; q < sel.size(); ++q)
-      for (int jj = 0; jj < sz; ++jj) C[e * (q * sz + jj)] = vecs[size_t(jj) * sz + sel[q]];
+    std::vector<double> C(e * size_t(sz) * vals.size(), 0.0);
+    for (size_t q = 0; q < vals.size(); ++q)
+      for (int jj = 0; jj < sz; ++jj) C[e * (q * sz + jj)] = vecs[q][size_t(jj)];
     DeviceArray<double> dC(C.size());
     QMBED_HIP(hipMemcpyAsync(dC.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
-    ritz_buf.resize(e * n_ * sel.size());
-    lk::gemm_vc(c_, V(), n_, sz, dC.data(), int(sel.size()), ritz_buf.data(), s_);
-    auto rr = rayleigh(ritz_buf.data(), int(sel.size()));
+    ritz_buf.resize(e * n_ * vals.size());
+    lk::gemm_vc(c_, V(), n_, sz, dC.data(), int(vals.size()), ritz_buf.data(), s_);
+    auto rr = rayleigh(ritz_buf.data(), int(vals.size()));
     std::vector<Candidate> out;
-    for (size_t q = 0; q < sel.size(); ++q) out.push_back({rr[q].first, rr[q].second, int(q)});
+    for (size_t q = 0; q < vals.size(); ++q) out.push_back({rr[q].first, rr[q].second, int(q)});
     return out;
   }
 

@@ -14,6 +14,7 @@
 // interleaved), Krylov basis V is [j][col][row].
 #include <algorithm>
 #include <cmath>
+#include <thread>
 
 #include "internal.hpp"
 
@@ -252,14 +253,23 @@ namespace {
 // the device and kept in a per-column history that the host reads once, at
 // the end of the projection.
 
+// Per-column reductions of the row-block partials run one wave per column:
+// lane-strided sums, then a shuffle tree (a fixed order).
+__device__ __forceinline__ double wave_total(const double* p, int nb, int stride, int lane) {
+  double s = 0;
+  for (int b = lane; b < nb; b += 64) s += p[size_t(b) * stride];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  return s;
+}
+
 // ‖cur_c‖ from the kb_norm partials: init_norm, running = act && norm > eps,
 // inv = 1/norm (0 when not running), size = 0
 __global__ void kb_proj_init(const double* partial, int nb, int nc, const int* act, double* init_norm, int* running,
                              double* inv, int* size) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (c >= nc) return;
-  double s = 0;
-  for (int b = 0; b < nb; ++b) s += partial[size_t(c) * nb + b];
+  const double s = wave_total(partial + size_t(c) * nb, nb, 1, lane);
+  if (lane) return;
   const double nrm = sqrt(s);
   init_norm[c] = nrm;
   const bool run = act[c] && nrm > F64_EPS;
@@ -271,13 +281,11 @@ __global__ void kb_proj_init(const double* partial, int nb, int nc, const int* a
 // α_j (real and imaginary part) of the running columns into the history
 __global__ void kb_alpha_hist(const double* partial, int nb, int nc, const int* running, int m, int j, double* are,
                               double* aim) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (c >= nc || !running[c]) return;
-  double sr = 0, si = 0;
-  for (int b = 0; b < nb; ++b) {
-    sr += partial[(size_t(c) * nb + b) * 2];
-    si += partial[(size_t(c) * nb + b) * 2 + 1];
-  }
+  const double sr = wave_total(partial + size_t(c) * nb * 2, nb, 2, lane);
+  const double si = wave_total(partial + size_t(c) * nb * 2 + 1, nb, 2, lane);
+  if (lane) return;
   are[size_t(c) * m + j] = sr;
   aim[size_t(c) * m + j] = si;
 }
@@ -309,10 +317,10 @@ __global__ void kb_three_hist(double2* W, const double2* __restrict__ V, const d
 // breakdown (src/solve/mod.rs:2909-3054), with size = j + 1
 __global__ void kb_beta_hist(const double* partial, int nb, int nc, int* running, int m, int j, double* bh,
                              int* size, double* inv) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (c >= nc || !running[c]) return;
-  double s = 0;
-  for (int b = 0; b < nb; ++b) s += partial[size_t(c) * nb + b];
+  const double s = wave_total(partial + size_t(c) * nb, nb, 1, lane);
+  if (lane) return;
   const double beta = sqrt(s);
   bh[size_t(c) * m + j] = beta;
   if (j + 1 == m || beta <= 1.0e-14) {
@@ -393,6 +401,13 @@ __global__ void kb_imag_abs(const double2* __restrict__ X, size_t n, double* par
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) s += fabs(x[i].y);
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// host threads for the per-column step control: the hardware threads, at
+// most 16
+int control_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return int(std::max(1u, std::min(16u, h ? h : 1u)));
 }
 
 template <typename T>
@@ -492,7 +507,7 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
     QMBED_HIP(hipMemcpyAsync(act_d.data(), act_h.p, size_t(nc) * 4, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), n,
                        partial.data());
-    hipLaunchKernelGGL(kb_proj_init, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, act_d.data(),
+    hipLaunchKernelGGL(kb_proj_init, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, act_d.data(),
                        init_norm.data(), running.data(), inv.data(), size_d.data());
     hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), inv.data(),
                        nullptr, n, Vc);
@@ -515,13 +530,13 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
                        reinterpret_cast<cplx*>(W.data()) + size_t(r.first) * n, r.second, s);
       hipLaunchKernelGGL(kb_zdot, grid, dim3(kT), 0, s, Vj, reinterpret_cast<const double2*>(W.data()), n,
                          partial.data());
-      hipLaunchKernelGGL(kb_alpha_hist, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, running.data(),
+      hipLaunchKernelGGL(kb_alpha_hist, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, running.data(),
                          m, j, are.data(), aim.data());
       hipLaunchKernelGGL(kb_three_hist, grid, dim3(kT), 0, s, reinterpret_cast<double2*>(W.data()), Vj,
                          j > 0 ? Vj - ldj : nullptr, are.data(), bh.data(), running.data(), m, j, n);
       hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), n,
                          partial.data());
-      hipLaunchKernelGGL(kb_beta_hist, dim3((nc + 63) / 64), dim3(64), 0, s, partial.data(), nb, nc, running.data(),
+      hipLaunchKernelGGL(kb_beta_hist, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, running.data(),
                          m, j, bh.data(), size_d.data(), inv.data());
       if (j + 1 < m)
         hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), inv.data(),
@@ -538,28 +553,30 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
     QMBED_HIP(hipMemcpyAsync(sizes, size_d.data(), size_t(nc) * 4, hipMemcpyDeviceToHost, s));
     QMBED_HIP(hipStreamSynchronize(s));
     // ---- per-column step control (host, m×m) ----
-    int nacts = 0, ncoef = 0;
-    std::vector<Act> copies;
-    auto add_action = [&](int c, const Projection& p, double interval, double2* dst) {
-      Act a{};
-      a.V = Vc + size_t(c) * n;
-      a.dst = dst;
-      a.size = p.size;
-      a.coef = ncoef;
-      a.col = c;
-      if (p.size > 0) {
-        const auto coef = projected_coefficients(p, interval);
-        for (int jj = 0; jj < p.size; ++jj) {
-          coef_h.p[2 * size_t(ncoef + jj)] = coef[jj].real();
-          coef_h.p[2 * size_t(ncoef + jj) + 1] = coef[jj].imag();
-        }
-        ncoef += p.size;
-      }
-      acts_h.p[nacts++] = a;
-    };
-    for (int c = 0; c < nc; ++c) {
-      if (!act_h.p[c]) continue;
+    // Columns are independent: their plans are made on several host threads
+    // and assembled in column order (the same actions and diagnostics for
+    // any thread count).
+    struct Plan {
       Projection p;
+      std::vector<std::pair<double2*, double>> outs;  // (destination, interval)
+      double2* copy_src = nullptr;                    // state <- last output
+      double t = 0;
+      int oi = 0;
+      bool stepped = false;
+      bool nonherm = false;
+      EvolutionDiagnostics dg;
+    };
+    std::vector<int> cols_act;
+    for (int c = 0; c < nc; ++c)
+      if (act_h.p[c]) cols_act.push_back(c);
+    std::vector<Plan> plans(cols_act.size());
+    auto plan_column = [&](size_t idx) {
+      const int c = cols_act[idx];
+      Plan& pl = plans[idx];
+      Projection& p = pl.p;
+      EvolutionDiagnostics& d = pl.dg;
+      pl.t = t[c];
+      pl.oi = oi[c];
       p.init_norm = hp[3 * cm + c];
       p.size = p.init_norm <= F64_EPS ? 0 : sizes[c];
       const double* ac = hp + size_t(c) * m;
@@ -567,56 +584,54 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       const double* bc = hp + 2 * cm + size_t(c) * m;
       if (!(op.is_real() && real_state[c]))
         for (int j = 0; j < p.size; ++j)
-          if (std::fabs(ic[j]) > 1.0e-10) throw non_hermitian();
+          if (std::fabs(ic[j]) > 1.0e-10) {
+            pl.nonherm = true;
+            return;
+          }
       if (p.size > 0) {
         std::vector<double> aa(ac, ac + p.size), bb(bc, bc + (p.size - 1));
         p.residual_beta = bc[p.size - 1];
         tridiagonal_eigen(aa, bb, p.evals, &p.evecs);
       }
-      ++dg.lanczos_projections;
-      dg.matrix_vector_products += p.size;
+      ++d.lanczos_projections;
+      d.matrix_vector_products += p.size;
       if (op.is_real() && real_state[c]) {
-        ++dg.real_lanczos_projections;
-        dg.real_matrix_vector_products += p.size;
+        ++d.real_lanczos_projections;
+        d.real_matrix_vector_products += p.size;
       }
       const double thr = local_tol * std::max(p.init_norm, 1.0);
       int accepted = 0;
-      double last_time = t[c];
-      for (int q = oi[c]; q < T; ++q) {
-        const double iv = opts.times[q] - t[c];
+      double last_time = pl.t;
+      for (int q = pl.oi; q < T; ++q) {
+        const double iv = opts.times[q] - pl.t;
         const double err = error_bound(p, iv, n);
-        dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
+        d.maximum_estimated_error = std::max(d.maximum_estimated_error, err);
         if (err > thr) {
-          ++dg.rejected_trial_intervals;
+          ++d.rejected_trial_intervals;
           break;
         }
-        add_action(c, p, iv, out_col(q, c));
+        pl.outs.push_back({out_col(q, c), iv});
         ++accepted;
         last_time = opts.times[q];
       }
+      pl.stepped = true;
+      ++d.accepted_substeps;
       if (accepted) {
-        // current state = last accepted output
-        Act cp{};
-        cp.V = out_col(oi[c] + accepted - 1, c);
-        cp.dst = cur_col(c);
-        copies.push_back(cp);
-        t[c] = last_time;
-        oi[c] += accepted;
-        ++substeps[c];
-        ++dg.accepted_substeps;
-        real_state[c] = 0;
-        continue;
+        pl.copy_src = out_col(pl.oi + accepted - 1, c);
+        pl.t = last_time;
+        pl.oi += accepted;
+        return;
       }
-      const double target = opts.times[oi[c]];
-      const double tiv = target - t[c];
+      const double target = opts.times[pl.oi];
+      const double tiv = target - pl.t;
       const double dir = tiv > 0 ? 1.0 : -1.0;
       const double min_iv = 16.0 * F64_EPS * std::max(std::fabs(target), 1.0);
       double rej = std::fabs(tiv), acc = rej;
       for (;;) {
         const double err = error_bound(p, dir * acc, n);
-        dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
+        d.maximum_estimated_error = std::max(d.maximum_estimated_error, err);
         if (err <= thr || acc <= min_iv) break;
-        ++dg.rejected_trial_intervals;
+        ++d.rejected_trial_intervals;
         rej = acc;
         acc *= 0.5;
       }
@@ -624,20 +639,71 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
         for (int r = 0; r < 8; ++r) {
           const double trial = 0.5 * (acc + rej);
           const double err = error_bound(p, dir * trial, n);
-          dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, err);
+          d.maximum_estimated_error = std::max(d.maximum_estimated_error, err);
           if (err <= thr) acc = trial;
           else {
-            ++dg.rejected_trial_intervals;
+            ++d.rejected_trial_intervals;
             rej = trial;
           }
         }
       }
       const double iv = dir * acc;
-      t[c] += iv;
-      add_action(c, p, iv, cur_col(c));
-      ++substeps[c];
-      ++dg.accepted_substeps;
-      real_state[c] = 0;
+      pl.t += iv;
+      pl.outs.push_back({cur_col(c), iv});
+    };
+    {
+      const size_t np = plans.size();
+      const size_t nth = std::min<size_t>(np, size_t(control_threads()));
+      std::vector<std::thread> pool;
+      auto work = [&](size_t tid) {
+        for (size_t i = tid; i < np; i += nth) plan_column(i);
+      };
+      for (size_t tid = 1; tid < nth; ++tid) pool.emplace_back(work, tid);
+      if (nth > 0) work(0);
+      for (auto& th : pool) th.join();
+    }
+    int nacts = 0, ncoef = 0;
+    std::vector<Act> copies;
+    for (size_t idx = 0; idx < plans.size(); ++idx) {
+      const int c = cols_act[idx];
+      Plan& pl = plans[idx];
+      if (pl.nonherm) throw non_hermitian();
+      dg.lanczos_projections += pl.dg.lanczos_projections;
+      dg.matrix_vector_products += pl.dg.matrix_vector_products;
+      dg.real_lanczos_projections += pl.dg.real_lanczos_projections;
+      dg.real_matrix_vector_products += pl.dg.real_matrix_vector_products;
+      dg.accepted_substeps += pl.dg.accepted_substeps;
+      dg.rejected_trial_intervals += pl.dg.rejected_trial_intervals;
+      dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, pl.dg.maximum_estimated_error);
+      for (auto& o : pl.outs) {
+        Act a{};
+        a.V = Vc + size_t(c) * n;
+        a.dst = o.first;
+        a.size = pl.p.size;
+        a.coef = ncoef;
+        a.col = c;
+        if (pl.p.size > 0) {
+          const auto coef = projected_coefficients(pl.p, o.second);
+          for (int jj = 0; jj < pl.p.size; ++jj) {
+            coef_h.p[2 * size_t(ncoef + jj)] = coef[jj].real();
+            coef_h.p[2 * size_t(ncoef + jj) + 1] = coef[jj].imag();
+          }
+          ncoef += pl.p.size;
+        }
+        acts_h.p[nacts++] = a;
+      }
+      if (pl.copy_src) {
+        Act cp{};
+        cp.V = pl.copy_src;
+        cp.dst = cur_col(c);
+        copies.push_back(cp);
+      }
+      t[c] = pl.t;
+      oi[c] = pl.oi;
+      if (pl.stepped) {
+        ++substeps[c];
+        real_state[c] = 0;
+      }
     }
     if (nacts > 0) {
       const int ncopy = int(copies.size());

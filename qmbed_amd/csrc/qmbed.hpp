@@ -503,6 +503,19 @@ void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<doubl
 
 // Index selection per spectral target (QMBED src/solve/mod.rs:1002-1036).
 std::vector<int> select_indices(const std::vector<double>& values, SpectrumTarget target, int count, double shift = 0.0);
+// eigenvalues lo..hi-1 (ascending, 0 = smallest) of a symmetric tridiagonal
+// by bisection
+std::vector<double> tridiagonal_eigenvalue_range(const std::vector<double>& alpha, const std::vector<double>& beta,
+                                                 int lo, int hi);
+// The Ritz values a target selects from the tridiagonal (alpha, beta), in
+// select_indices order; with `vectors`, their unit eigenvectors too.
+// Extreme targets (smallest/largest algebraic, both ends, largest magnitude)
+// find only the k lowest and/or highest values by bisection and their
+// vectors by inverse iteration (the full QL decomposition when selected
+// values nearly coincide); the other targets take the full QL spectrum.
+std::vector<double> selected_ritz_values(const std::vector<double>& alpha, const std::vector<double>& beta,
+                                         SpectrumTarget target, int k, double shift,
+                                         std::vector<std::vector<double>>* vectors = nullptr);
 
 // Deterministic start (QMBED src/solve/mod.rs:729-742), written on the device.
 void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s);

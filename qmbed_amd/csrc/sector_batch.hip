@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <numeric>
+#include <thread>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -485,6 +486,25 @@ __global__ __launch_bounds__(kT) void kb_ritz(const uint32_t* __restrict__ tiles
 
 int groups(int items, int per_block) { return (items + per_block - 1) / per_block; }
 
+// f(i) for i < count on up to 16 host threads (the per-sector tridiagonal
+// work of a read-back; each i touches only its own sector's state)
+template <typename F>
+void parallel_for(int count, F f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nth = std::min(count, int(std::max(1u, std::min(16u, hw ? hw : 1u))));
+  if (nth <= 1) {
+    for (int i = 0; i < count; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> pool;
+  auto work = [&](int tid) {
+    for (int i = tid; i < count; i += nth) f(i);
+  };
+  for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+}
+
 // Host side of one batch.
 class Batch {
  public:
@@ -714,7 +734,7 @@ class Batch {
   // decide which sectors take the Ritz test now.
   void collect(int L) {
     QMBED_HIP(hipEventSynchronize(ev_));
-    std::vector<int> test;
+    std::vector<int> test, check;
     std::vector<char> ended(S_, 0);
     for (int i : running_) {
       SectorState& st = state_[i];
@@ -754,9 +774,16 @@ class Batch {
         continue;
       }
       if (!ctl_.early_stop || size < st.next_check || size <= k_) continue;
-      if (estimates_converged(st)) test.push_back(i);
-      else st.next_check = size + ctl_.check_every;
+      check.push_back(i);
     }
+    // residual estimates of the sectors due for a check, in parallel
+    std::vector<char> conv(check.size(), 0);
+    parallel_for(int(check.size()), [&](int q) { conv[size_t(q)] = estimates_converged(state_[check[q]]) ? 1 : 0; });
+    for (size_t q = 0; q < check.size(); ++q) {
+      if (conv[q]) test.push_back(check[q]);
+      else state_[check[q]].next_check = state_[check[q]].size + ctl_.check_every;
+    }
+    std::sort(test.begin(), test.end());
     std::vector<int> keep;
     if (!test.empty()) ritz(test);
     for (int i : running_) {
@@ -781,14 +808,12 @@ class Batch {
 
   bool estimates_converged(const SectorState& st) const {
     const int sz = st.size;
-    std::vector<double> vals;
-    tridiagonal_eigen(st.alpha, st.beta, vals, nullptr);
-    auto sel = select_indices(vals, o_.target, k_, o_.shift);
-    if (int(sel.size()) < k_) return false;
-    for (int idx : sel) {
-      auto v = tridiagonal_eigvec(st.alpha, st.beta, vals[idx]);
+    const auto vals = selected_ritz_values(st.alpha, st.beta, o_.target, k_, o_.shift);
+    if (int(vals.size()) < k_) return false;
+    for (double val : vals) {
+      auto v = tridiagonal_eigvec(st.alpha, st.beta, val);
       const double est = st.last_beta * std::fabs(v[sz - 1]);
-      if (!std::isfinite(est) || est > 0.5 * o_.tolerance * std::max(std::fabs(vals[idx]), 1.0)) return false;
+      if (!std::isfinite(est) || est > 0.5 * o_.tolerance * std::max(std::fabs(val), 1.0)) return false;
     }
     return true;
   }
@@ -799,20 +824,21 @@ class Batch {
     ++ritz_rounds_;
     std::vector<uint32_t> offs(S_, 0);
     std::vector<int> sizes(S_, 0);
-    std::vector<std::vector<int>> sel(S_);
     std::vector<std::vector<double>> vals(S_);
+    std::vector<std::vector<std::vector<double>>> vecs(S_);
+    // selected Ritz pairs of the tridiagonal matrices, in parallel
+    parallel_for(int(secs.size()), [&](int q) {
+      const int i = secs[size_t(q)];
+      vals[i] = selected_ritz_values(state_[i].alpha, state_[i].beta, o_.target, k_, o_.shift, &vecs[i]);
+    });
     size_t cursor = 0;
     for (int i : secs) {
-      const SectorState& st = state_[i];
-      const int sz = st.size;
-      std::vector<double> vecs;
-      tridiagonal_eigen(st.alpha, st.beta, vals[i], &vecs);
-      sel[i] = select_indices(vals[i], o_.target, k_, o_.shift);
+      const int sz = state_[i].size;
       offs[i] = uint32_t(cursor);
       sizes[i] = sz;
       for (int q = 0; q < k_; ++q) {
-        const int e = q < int(sel[i].size()) ? sel[i][q] : 0;
-        for (int c = 0; c < sz; ++c) ritz_stage_[cursor + size_t(q) * sz + c] = vecs[size_t(c) * sz + e];
+        const bool have = q < int(vecs[i].size());
+        for (int c = 0; c < sz; ++c) ritz_stage_[cursor + size_t(q) * sz + c] = have ? vecs[i][q][size_t(c)] : 0.0;
       }
       cursor += size_t(k_) * sz;
     }
@@ -880,7 +906,7 @@ class Batch {
         int column;
       };
       std::vector<Cand> cands;
-      for (int q = 0; q < int(sel[i].size()); ++q)
+      for (int q = 0; q < int(vals[i].size()); ++q)
         cands.push_back({slot_host_[size_t(S_) * k_ + size_t(i) * k_ + q],
                          slot_host_[2 * size_t(S_) * k_ + size_t(i) * k_ + q], q});
       const SpectrumTarget t = o_.target == SpectrumTarget::BothEnds ? SpectrumTarget::SmallestAlgebraic : o_.target;

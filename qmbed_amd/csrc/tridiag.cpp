@@ -96,6 +96,69 @@ void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<doubl
   }
 }
 
+namespace {
+
+// number of eigenvalues below x: negative pivots of the LDL^T factorization
+// of T - xI (Sturm sequence), a pivot within pivmin of zero taken negative
+int count_below(const std::vector<double>& a, const std::vector<double>& b2, double x, double pivmin) {
+  int cnt = 0;
+  double q = 1.0;
+  for (size_t i = 0; i < a.size(); ++i) {
+    q = (a[i] - x) - (i ? b2[i - 1] / q : 0.0);
+    if (std::fabs(q) <= pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+}  // namespace
+
+// Eigenvalues lo..hi-1 (ascending order, 0 = smallest) of the symmetric
+// tridiagonal (alpha, beta) by bisection on Sturm counts (Barth, Martin &
+// Wilkinson, Numer. Math. 9 (1967) 386), each to a few ulps of the
+// spectrum's scale: O(n) per bisection step instead of the O(n^2) of a full
+// QL sweep when only the extreme Ritz values are wanted.
+std::vector<double> tridiagonal_eigenvalue_range(const std::vector<double>& alpha, const std::vector<double>& beta,
+                                                 int lo, int hi) {
+  const int n = int(alpha.size());
+  lo = std::max(lo, 0);
+  hi = std::min(hi, n);
+  std::vector<double> out;
+  if (lo >= hi) return out;
+  std::vector<double> b2(size_t(std::max(n - 1, 0)));
+  double bmax = 0;
+  for (int i = 0; i + 1 < n; ++i) {
+    b2[i] = beta[i] * beta[i];
+    bmax = std::max(bmax, b2[i]);
+  }
+  const double pivmin = 2.2250738585072014e-308 * std::max(1.0, bmax);
+  double gl = INFINITY, gu = -INFINITY;
+  for (int i = 0; i < n; ++i) {
+    const double r = (i ? std::fabs(beta[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(beta[i]) : 0.0);
+    gl = std::min(gl, alpha[i] - r);
+    gu = std::max(gu, alpha[i] + r);
+  }
+  const double eps = 2.220446049250313e-16;
+  const double tnorm = std::max(std::fabs(gl), std::fabs(gu));
+  gl -= 2.0 * eps * tnorm * n + 2.0 * pivmin;
+  gu += 2.0 * eps * tnorm * n + 2.0 * pivmin;
+  double prev = gl;
+  for (int j = lo; j < hi; ++j) {
+    double l = prev, u = gu;  // eigenvalue j is not below eigenvalue j - 1
+    for (int it = 0; it < 200; ++it) {
+      if (u - l <= 2.0 * eps * std::max(std::fabs(l), std::fabs(u)) + 4.0 * pivmin) break;
+      const double mid = 0.5 * (l + u);
+      if (mid <= l || mid >= u) break;
+      if (count_below(alpha, b2, mid, pivmin) > j) u = mid;
+      else l = mid;
+    }
+    const double v = 0.5 * (l + u);
+    out.push_back(v);
+    prev = l;
+  }
+  return out;
+}
+
 // Eigenvector of the symmetric tridiagonal (alpha, beta) for an eigenvalue
 // lam, by inverse iteration with partial pivoting (LAPACK dstein-style).
 std::vector<double> tridiagonal_eigvec(const std::vector<double>& alpha, const std::vector<double>& beta, double lam) {
