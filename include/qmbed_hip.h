@@ -129,9 +129,14 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * is bound to a model created with "components" (one real coefficient per
  * component is one scan coordinate).
  * options_json: {"snapshots":32,"hidden":[64,64],"epochs":4000,
- * "learning_rate":0.002,"final_learning_rate":<learning_rate>,"seed":0},
- * any subset, NULL for all defaults (the learning rate decays
- * exponentially to final_learning_rate; equal values keep it constant). */
+ * "learning_rate":0.002,"final_learning_rate":<learning_rate>,"seed":0,
+ * "targets":"projections","samples":512}, any subset, NULL for all
+ * defaults (the learning rate decays exponentially to final_learning_rate;
+ * equal values keep it constant). targets "projections": the network learns
+ * the coefficients of the training ground states over the snapshots;
+ * "continuation": it learns the lowest eigenvector of the family projected
+ * on the snapshots at the training points and `samples` seeded points of
+ * their bounding box. */
 typedef struct qmbed_hip_scan qmbed_hip_scan;
 qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
 void qmbed_hip_scan_free(qmbed_hip_scan* scan);
@@ -141,9 +146,12 @@ void qmbed_hip_scan_free(qmbed_hip_scan* scan);
  * Answers a JSON report (energies, iterations, snapshot indices, timings,
  * first/last training loss). */
 char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json);
-/* Ground states of n points, from the network's start (warm != 0) or the
- * plain deterministic start. Answers energies, residuals, iterations
- * (total, FP32-matvec and FP64), per-point seconds and fallback count. */
+/* Ground states of n points. warm = 0: the plain deterministic start;
+ * 1: the network's start S·MLP(c); 2: the snapshot Rayleigh-Ritz start
+ * S·y, y the lowest eigenvector of Sᵀ H(c) S (eigenvector continuation over
+ * the learned snapshots; the projected parts are formed once at training).
+ * Answers energies, residuals, iterations (total, FP32-matvec and FP64),
+ * per-point seconds and fallback count. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);
 /* Network outputs (n × snapshots, row-major) of n points; returns the
  * snapshot count, or -1 on failure. */

@@ -369,10 +369,13 @@ class Scan:
                                                         json.dumps(req).encode()))
 
     def solve(self, points, warm=True, **solver):
+        """warm: False (plain start), True (network start) or "rayleigh_ritz"
+        (the lowest eigenvector of the family projected on the snapshots)."""
         pts = np.ascontiguousarray(points, dtype=np.float64)
         req = {"eigenpairs": 1, "target": {"kind": "smallest_algebraic"}, **solver}
+        mode = 2 if warm == "rayleigh_ritz" else (1 if warm else 0)
         return _envelope(library().qmbed_hip_scan_solve(self._h, pts.ctypes.data, pts.shape[0],
-                                                        json.dumps(req).encode(), 1 if warm else 0))
+                                                        json.dumps(req).encode(), mode))
 
     def predict(self, points):
         pts = np.ascontiguousarray(points, dtype=np.float64)

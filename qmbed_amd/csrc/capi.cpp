@@ -982,6 +982,16 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
       if (auto* x = v.get("final_learning_rate"); x && !x->is_null())
         o.final_learning_rate = float(as_double(*x, "final_learning_rate"));
       if (auto* x = v.get("seed"); x && !x->is_null()) o.seed = as_u64(*x, "seed");
+      if (auto* x = v.get("targets"); x && !x->is_null()) {
+        if (x->kind != json::Value::String) bad("targets must be a string");
+        if (x->raw == "projections") o.targets = ScanOptions::Targets::Projections;
+        else if (x->raw == "continuation") o.targets = ScanOptions::Targets::Continuation;
+        else bad("targets must be \"projections\" or \"continuation\"");
+      }
+      if (auto* x = v.get("samples"); x && !x->is_null()) {
+        o.samples = int(as_long(*x, "samples"));
+        if (o.samples < 0 || o.samples > 65536) bad("samples must lie in [0, 65536]");
+      }
     }
     auto ts = std::make_unique<TypedScan>();
     ts->m = tm->m;
@@ -1040,7 +1050,9 @@ char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, co
     std::vector<long> it, it32, it64;
     long warm_n = 0, fallbacks = 0;
     for (int i = 0; i < n; ++i) {
-      auto pr = ts->scan->solve(points + size_t(i) * ts->scan->parameters(), r.opts, r.ctl, warm != 0);
+      const auto start = warm == 2 ? WarmStartScan::Start::SnapshotRR
+                                   : (warm ? WarmStartScan::Start::Network : WarmStartScan::Start::Plain);
+      auto pr = ts->scan->solve(points + size_t(i) * ts->scan->parameters(), r.opts, r.ctl, start);
       e.push_back(pr.es.eigenvalues[0]);
       res.push_back(pr.es.residuals[0]);
       secs.push_back(pr.seconds);

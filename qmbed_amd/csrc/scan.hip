@@ -16,6 +16,12 @@
 // Solving a point: MLP(c) → coefficients y, start vector v₀ = S y, Lanczos
 // from v₀. A vanishing prediction falls back to the plain deterministic
 // start; the Lanczos residual test decides convergence either way.
+//
+// Snapshot Rayleigh-Ritz start (eigenvector continuation): training also
+// forms the projected parts Sᵀ H_part S (ns H·v and one GEMM per part), so Sᵀ H(c) S = Σ_part c_part Sᵀ H_part S costs nothing per
+// point; its lowest eigenvector y (ns × ns Jacobi on the host) gives the
+// start S y, the best start the snapshot span holds for the Rayleigh
+// quotient. The same map can serve as the network's targets.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -78,6 +84,10 @@ __global__ __launch_bounds__(kT) void k_scale_copy(const double* __restrict__ x,
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) y[i] = f * x[i];
 }
 
+__global__ __launch_bounds__(kT) void k_fill(double* __restrict__ x, size_t n, double v) {
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) x[i] = v;
+}
+
 __global__ void k_f2d(const float* __restrict__ a, int n, double* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = double(a[i]);
@@ -94,6 +104,47 @@ struct Lcg {
     return double(x >> 11) / 4503599627370496.0 - 1.0;
   }
 };
+
+// Eigen-decomposition of a symmetric n × n matrix (column-major) by cyclic
+// Jacobi rotations; returns the eigenvector of the lowest eigenvalue
+std::vector<double> lowest_eigenvector(std::vector<double> a, int n) {
+  std::vector<double> v(size_t(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) v[size_t(i) * n + i] = 1.0;
+  auto A = [&](int i, int j) -> double& { return a[size_t(j) * n + i]; };
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, diag = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) (i == j ? diag : off) += A(i, j) * A(i, j);
+    if (off <= 1e-30 * std::max(diag, 1e-300)) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A(p, q);
+        if (apq == 0.0) continue;
+        const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = A(k, p), akq = A(k, q);
+          A(k, p) = c * akp - sn * akq;
+          A(k, q) = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = A(p, k), aqk = A(q, k);
+          A(p, k) = c * apk - sn * aqk;
+          A(q, k) = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = v[size_t(p) * n + k], vkq = v[size_t(q) * n + k];
+          v[size_t(p) * n + k] = c * vkp - sn * vkq;
+          v[size_t(q) * n + k] = sn * vkp + c * vkq;
+        }
+      }
+  }
+  int lo = 0;
+  for (int i = 1; i < n; ++i)
+    if (A(i, i) < A(lo, lo)) lo = i;
+  return std::vector<double>(v.begin() + size_t(lo) * n, v.begin() + size_t(lo + 1) * n);
+}
 
 double seconds_since(std::chrono::steady_clock::time_point t0) {
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -246,8 +297,52 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
     lk::gemm_vc(false, Psel.data(), N, ns_, dR.data(), ns_, S_.data(), s_);
     QMBED_HIP(hipStreamSynchronize(s_));
   }
+  // projected parts Sᵀ H_part S: H(0) S, then H(e_p) S − H(0) S per component
+  {
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
+    rocblas_set_stream(h, s_);
+    DeviceArray<double> HS(N * size_t(ns_)), dP(size_t(ns_) * ns_);
+    proj_.assign(size_t(P_) + 1, std::vector<double>(size_t(ns_) * ns_, 0.0));
+    for (int part = 0; part <= P_; ++part) {
+      std::vector<cplx> c(size_t(P_), cplx(0, 0));
+      if (part > 0) c[size_t(part - 1)] = cplx(1, 0);
+      op_.evaluate_coefficients(c, s_);
+      for (int b = 0; b < ns_; ++b) op_.apply(S_.data() + N * size_t(b), HS.data() + N * size_t(b), s_);
+      const double one = 1.0, zero = 0.0;
+      rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ns_, ns_,
+                                        rocblas_int(N), &one, S_.data(), rocblas_int(N), HS.data(), rocblas_int(N),
+                                        &zero, dP.data(), ns_);
+      if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
+      QMBED_HIP(hipMemcpyAsync(proj_[size_t(part)].data(), dP.data(), size_t(ns_) * ns_ * 8, hipMemcpyDeviceToHost,
+                               s_));
+      QMBED_HIP(hipStreamSynchronize(s_));
+      auto& m = proj_[size_t(part)];
+      for (int i = 0; i < ns_; ++i)  // symmetrize the rounding
+        for (int j = i + 1; j < ns_; ++j) {
+          const double a = 0.5 * (m[size_t(j) * ns_ + i] + m[size_t(i) * ns_ + j]);
+          m[size_t(j) * ns_ + i] = m[size_t(i) * ns_ + j] = a;
+        }
+      if (part > 0)
+        for (size_t e = 0; e < m.size(); ++e) m[e] -= proj_[0][e];
+    }
+    // Sᵀ 1 for the gauge of continuation targets (the training states' own
+    // gauge: the sign of Σ_i ψ_i)
+    {
+      DeviceArray<double> ones(N), dg(static_cast<size_t>(ns_));
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(kT), 0, s_, ones.data(), N, 1.0);
+      const double one = 1.0, zero = 0.0;
+      rocblas_status st = rocblas_dgemv(h, rocblas_operation_transpose, rocblas_int(N), ns_, &one, S_.data(),
+                                        rocblas_int(N), ones.data(), 1, &zero, dg.data(), 1);
+      if (st != rocblas_status_success) throw internal_state("rocblas_dgemv failed");
+      gauge_.assign(size_t(ns_), 0.0);
+      QMBED_HIP(hipMemcpyAsync(gauge_.data(), dg.data(), size_t(ns_) * 8, hipMemcpyDeviceToHost, s_));
+      QMBED_HIP(hipStreamSynchronize(s_));
+    }
+    rocblas_destroy_handle(h);
+  }
   rep.gram_seconds = seconds_since(t1);
-  // 4. the network: inputs normalized to [-1, 1] per parameter, targets L
+  // 4. the network: inputs normalized to [-1, 1] per parameter
   auto t2 = std::chrono::steady_clock::now();
   center_.assign(size_t(P_), 0.0);
   scale_.assign(size_t(P_), 1.0);
@@ -260,11 +355,25 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
     center_[size_t(p)] = 0.5 * (lo + hi);
     scale_[size_t(p)] = hi > lo ? 0.5 * (hi - lo) : 1.0;
   }
-  std::vector<float> X(size_t(n) * P_), T(size_t(n) * ns_);
-  for (int i = 0; i < n; ++i) {
+  // samples: the training points, then (continuation targets) seeded
+  // uniform points of their bounding box
+  std::vector<double> pts(points, points + size_t(n) * P_);
+  if (opt_.targets == ScanOptions::Targets::Continuation) {
+    Lcg rng(opt_.seed + 1);
+    for (int q = 0; q < opt_.samples; ++q)
+      for (int p = 0; p < P_; ++p) pts.push_back(center_[size_t(p)] + scale_[size_t(p)] * rng.next());
+  }
+  const int nt = int(pts.size() / size_t(P_));
+  std::vector<float> X(size_t(nt) * P_), T(size_t(nt) * ns_);
+  for (int i = 0; i < nt; ++i) {
     for (int p = 0; p < P_; ++p)
-      X[size_t(i) * P_ + p] = float((points[size_t(i) * P_ + p] - center_[size_t(p)]) / scale_[size_t(p)]);
-    for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(Lm[size_t(i) + size_t(n) * size_t(k)]);
+      X[size_t(i) * P_ + p] = float((pts[size_t(i) * P_ + p] - center_[size_t(p)]) / scale_[size_t(p)]);
+    if (opt_.targets == ScanOptions::Targets::Continuation) {
+      const auto y = continuation(pts.data() + size_t(i) * P_);
+      for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(y[size_t(k)]);
+    } else {
+      for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(Lm[size_t(i) + size_t(n) * size_t(k)]);
+    }
   }
   const int np = mlp_parameter_count(P_, opt_.hidden1, opt_.hidden2, ns_);
   std::vector<float> init(static_cast<size_t>(np), 0.0f);
@@ -289,7 +398,7 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
       opt_.epochs > 1 ? float(std::pow(double(opt_.final_learning_rate) / double(opt_.learning_rate),
                                        1.0 / double(opt_.epochs - 1)))
                       : 1.0f;
-  mlp_train_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dX.data(), dT.data(), n, opt_.epochs,
+  mlp_train_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dX.data(), dT.data(), nt, opt_.epochs,
                    opt_.learning_rate, decay, nullptr, dLoss.data(), dHist.data(), s_);
   std::vector<float> hist(size_t(opt_.epochs));
   QMBED_HIP(hipMemcpyAsync(hist.data(), dHist.data(), hist.size() * 4, hipMemcpyDeviceToHost, s_));
@@ -300,11 +409,23 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   // relative fit of the training targets: Σ|T|² for reference
   double tn = 0;
   for (float v : T) tn += double(v) * double(v);
-  rep.target_norm2 = tn / double(n);
+  rep.target_norm2 = tn / double(nt);
   rep.snapshot_indices = sel;
   rep.snapshots = ns_;
   trained_ = true;
   return rep;
+}
+
+std::vector<double> WarmStartScan::continuation(const double* point) const {
+  std::vector<double> pm = proj_[0];
+  for (int p = 0; p < P_; ++p)
+    for (size_t e = 0; e < pm.size(); ++e) pm[e] += point[p] * proj_[size_t(p) + 1][e];
+  auto y = lowest_eigenvector(pm, ns_);
+  double g = 0;
+  for (int k = 0; k < ns_; ++k) g += y[size_t(k)] * gauge_[size_t(k)];
+  if (g < 0)
+    for (double& v : y) v = -v;
+  return y;
 }
 
 std::vector<double> WarmStartScan::predict(const double* points, int n) {
@@ -325,7 +446,8 @@ std::vector<double> WarmStartScan::predict(const double* points, int n) {
 }
 
 WarmStartScan::PointResult WarmStartScan::solve(const double* point, const EigshOptions& o, const EigshControls& ctl,
-                                                bool warm) {
+                                                Start start) {
+  const bool warm = start != Start::Plain;
   PointResult r;
   auto t0 = std::chrono::steady_clock::now();
   op_.evaluate_coefficients(coefficients_of(point), s_);
@@ -345,9 +467,14 @@ WarmStartScan::PointResult WarmStartScan::solve(const double* point, const Eigsh
     QMBED_HIP(hipMemcpyAsync(dx.data(), x.data(), x.size() * 4, hipMemcpyHostToDevice, s_));
     mlp_forward_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dx.data(), 1, dy.data(), s_);
     hipLaunchKernelGGL(k_f2d, dim3(1), dim3(64), 0, s_, dy.data(), ns_, dc.data());
-    lk::gemm_vc(false, S_.data(), N, ns_, dc.data(), 1, v0.data(), s_);
     std::vector<double> c(static_cast<size_t>(ns_));
-    QMBED_HIP(hipMemcpyAsync(c.data(), dc.data(), c.size() * 8, hipMemcpyDeviceToHost, s_));
+    if (start == Start::SnapshotRR) {
+      c = continuation(point);
+      QMBED_HIP(hipMemcpyAsync(dc.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice, s_));
+    }
+    lk::gemm_vc(false, S_.data(), N, ns_, dc.data(), 1, v0.data(), s_);
+    if (start != Start::SnapshotRR)
+      QMBED_HIP(hipMemcpyAsync(c.data(), dc.data(), c.size() * 8, hipMemcpyDeviceToHost, s_));
     QMBED_HIP(hipStreamSynchronize(s_));
     double cn = 0;
     for (double v : c) cn += v * v;

@@ -14,6 +14,14 @@ struct ScanOptions {
   // equal to it: constant)
   float final_learning_rate = 2.0e-3f;
   uint64_t seed = 0;  // network initialization
+  // Network targets. Projections: the coefficients of each training ground
+  // state over the snapshots. Continuation: the lowest eigenvector of the
+  // family projected on the snapshots, Sᵀ H(c) S (eigenvector continuation),
+  // at the training points and at `samples` seeded points of their bounding
+  // box, so the network learns that map over the whole domain.
+  enum class Targets { Projections, Continuation };
+  Targets targets = Targets::Projections;
+  int samples = 512;
 };
 
 class WarmStartScan {
@@ -40,9 +48,14 @@ class WarmStartScan {
     bool fallback = false;  // prediction vanished: plain start used
     double seconds = 0;
   };
-  // ground state of one point, from the learned start (warm) or the plain
-  // deterministic start
-  PointResult solve(const double* point, const EigshOptions& o, const EigshControls& ctl, bool warm);
+  // How a point's Lanczos start is made.
+  enum class Start {
+    Plain,        // the deterministic start
+    Network,      // v₀ = S·MLP(c)
+    SnapshotRR,   // v₀ = S·y, y the lowest eigenvector of Sᵀ H(c) S
+  };
+  // ground state of one point from the chosen start
+  PointResult solve(const double* point, const EigshOptions& o, const EigshControls& ctl, Start start);
   // network outputs (snapshot coefficients) of n points
   std::vector<double> predict(const double* points, int n);
 
@@ -52,6 +65,8 @@ class WarmStartScan {
 
  private:
   std::vector<cplx> coefficients_of(const double* point) const;
+  // gauge-fixed lowest eigenvector of Sᵀ H(c) S
+  std::vector<double> continuation(const double* point) const;
   double gauge_sign(const double* d_psi);
 
   Operator& op_;
@@ -60,6 +75,10 @@ class WarmStartScan {
   int P_ = 0, ns_ = 0;
   bool trained_ = false;
   DeviceArray<double> S_;      // N × ns orthonormal snapshot basis
+  // Sᵀ H_part S (ns × ns, column-major) of the static part, then of each
+  // component: Sᵀ H(c) S = proj_[0] + Σ_p c_p proj_[1 + p]
+  std::vector<std::vector<double>> proj_;
+  std::vector<double> gauge_;  // Sᵀ 1: the gauge sign of S·y is that of y·gauge_
   DeviceArray<float> params_;  // network parameters
   std::vector<double> center_, scale_;
 };

@@ -103,6 +103,54 @@ def test_learned_warm_start_scan(qa, oracle):
     # training points have unit-ish norm (coefficients of unit vectors)
     y = scan.predict(train)
     assert np.all(np.abs(np.linalg.norm(y, axis=1) - 1.0) < 0.2)
+    # the snapshot Rayleigh-Ritz start: same accuracy, fewer iterations than
+    # the network's start (it is the best start the snapshot span holds)
+    rr = scan.solve(held, warm="rayleigh_ritz")
+    rr32 = scan.solve(held, warm="rayleigh_ritz", matvec_precision="fp32")
+    assert rr["warm_starts"] == len(held) and rr["fallbacks"] == 0
+    for i, (a, b) in enumerate(held):
+        e0 = np.linalg.eigvalsh(oracle.Model(L, cases.ising(L, a, b), momentum=0, parity=1,
+                                             norm="pauli_cartesian").dense())[0]
+        for r in (rr, rr32):
+            assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
+            assert r["residuals"][i] <= 1e-10 * abs(e0)
+    ri = np.mean(rr["iterations"])
+    print(f"[learned] snapshot Rayleigh-Ritz start: {ri:.1f} iterations", flush=True)
+    assert ri < wi
+
+
+def test_projected_family_matches_oracle(qa, oracle):
+    # the continuation start at a training point reproduces that point's
+    # ground state: with every training state a snapshot, S·y is ψ itself
+    # (to the subspace's rounding), so Lanczos from it converges at once
+    m = _ising_scan_model(qa)
+    train = np.array([(0.5, 0.1), (1.0, 0.2), (1.5, 0.3), (1.2, 0.05)])
+    scan = qa.Scan(m, snapshots=4, epochs=200, seed=1)
+    rep = scan.train(train)
+    assert rep["snapshots"] == 4
+    r = scan.solve(train, warm="rayleigh_ritz")
+    for i, (a, b) in enumerate(train):
+        e0 = np.linalg.eigvalsh(oracle.Model(L, cases.ising(L, a, b), momentum=0, parity=1,
+                                             norm="pauli_cartesian").dense())[0]
+        assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
+        assert r["iterations"][i] <= 8
+
+
+def test_continuation_targets(qa):
+    m = _ising_scan_model(qa)
+    hx = np.linspace(0.2, 2.0, 6)
+    hz = np.linspace(0.0, 0.5, 6)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    scan = qa.Scan(m, snapshots=8, epochs=1500, learning_rate=3e-3, seed=0, targets="continuation", samples=64)
+    rep = scan.train(grid[::3])
+    assert rep["loss_last"] < 0.05 * rep["loss_first"]
+    held = grid[1::5]
+    warm = scan.solve(held, warm=True)
+    plain = scan.solve(held, warm=False)
+    assert warm["fallbacks"] == 0
+    np.testing.assert_allclose(warm["energies"], plain["energies"], rtol=1e-10)
+    with pytest.raises(qa.QmbedError, match="targets must be"):
+        qa.Scan(m, targets="other")
 
 
 def test_scan_errors(qa):
