@@ -224,6 +224,11 @@ SolverReq parse_solver(const Value& s) {
     else bad("matvec_precision must be \"fp64\" or \"fp32\"");
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
+  if (auto* v = s.get("check_every"); v && !v->is_null()) {
+    const long c = as_long(*v, "check_every");
+    if (c < 1 || c > 64) bad("check_every must lie in [1, 64]");
+    r.ctl.check_every = int(c);
+  }
   if (auto* v = s.get("reorthogonalization_eta"); v && !v->is_null()) {
     const double eta = as_double(*v, "reorthogonalization_eta");
     if (!(eta > 0.0 && eta <= 1.0)) bad("reorthogonalization_eta must lie in (0, 1]");

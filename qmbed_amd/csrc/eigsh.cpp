@@ -146,6 +146,7 @@ struct Candidate {
 struct WindowBuffers {
   int device = 0;
   DeviceArray<double> V, w, slots;
+  DeviceArray<double> start, coef;  // eigsh start vector; Ritz coefficients
   DeviceArray<float> x32, y32;
   lk::Workspace ws;
   double* pin = nullptr;  // pinned read-back area
@@ -197,6 +198,18 @@ static void release_buffers(std::unique_ptr<WindowBuffers> b) {
   for (auto& e : g_pool) same += e->device == b->device ? 1 : 0;
   if (same < 2) g_pool.push_back(std::move(b));
 }
+
+// A pooled buffer set held for a scope; returned after the stream drains.
+struct PooledBuffers {
+  std::unique_ptr<WindowBuffers> b;
+  hipStream_t s;
+  explicit PooledBuffers(hipStream_t st) : b(acquire_buffers()), s(st) {}
+  ~PooledBuffers() {
+    if (hipStreamSynchronize(s) == hipSuccess) release_buffers(std::move(b));
+  }
+  PooledBuffers(const PooledBuffers&) = delete;
+  PooledBuffers& operator=(const PooledBuffers&) = delete;
+};
 
 // One Lanczos window on the device. A step is six launches and no host
 // round trip: H·v_j, the partials of α_j = <v_j, H·v_j>, the overlaps of
@@ -420,10 +433,10 @@ class LanczosWindow {
     std::vector<double> C(e * size_t(sz) * vals.size(), 0.0);
     for (size_t q = 0; q < vals.size(); ++q)
       for (int jj = 0; jj < sz; ++jj) C[e * (q * sz + jj)] = vecs[q][size_t(jj)];
-    DeviceArray<double> dC(C.size());
-    QMBED_HIP(hipMemcpyAsync(dC.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
+    grow(buf_->coef, C.size());  // rayleigh() below synchronizes before C dies
+    QMBED_HIP(hipMemcpyAsync(buf_->coef.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
     ritz_buf.resize(e * n_ * vals.size());
-    lk::gemm_vc(c_, V(), n_, sz, dC.data(), int(vals.size()), ritz_buf.data(), s_);
+    lk::gemm_vc(c_, V(), n_, sz, buf_->coef.data(), int(vals.size()), ritz_buf.data(), s_);
     auto rr = rayleigh(ritz_buf.data(), int(vals.size()));
     std::vector<Candidate> out;
     for (size_t q = 0; q < vals.size(); ++q) out.push_back({rr[q].first, rr[q].second, int(q)});
@@ -592,14 +605,18 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
       throw e;
     }
     // reorder the Ritz columns to the candidate order
-    const size_t e = c ? 2 : 1;
-    const size_t n = op.dim();
-    DeviceArray<double> ordered(e * n * cands.size());
-    for (size_t q = 0; q < cands.size(); ++q)
-      QMBED_HIP(hipMemcpyAsync(ordered.data() + e * n * q, out.vectors.data() + e * n * cands[q].column, e * n * 8,
-                               hipMemcpyDeviceToDevice, s));
-    QMBED_HIP(hipStreamSynchronize(s));
-    out.vectors = std::move(ordered);
+    bool identity = true;
+    for (size_t q = 0; q < cands.size(); ++q) identity = identity && cands[q].column == int(q);
+    if (!identity) {
+      const size_t e = c ? 2 : 1;
+      const size_t n = op.dim();
+      DeviceArray<double> ordered(e * n * cands.size());
+      for (size_t q = 0; q < cands.size(); ++q)
+        QMBED_HIP(hipMemcpyAsync(ordered.data() + e * n * q, out.vectors.data() + e * n * cands[q].column, e * n * 8,
+                                 hipMemcpyDeviceToDevice, s));
+      QMBED_HIP(hipStreamSynchronize(s));
+      out.vectors = std::move(ordered);
+    }
     for (auto& cd : cands) {
       out.es.eigenvalues.push_back(cd.value);
       out.es.residuals.push_back(cd.residual);
@@ -714,34 +731,21 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
   const bool c = !op.is_real() || initial_complex;
   complex_vectors = c;
   const size_t e = c ? 2 : 1;
-  // start vector
-  DeviceArray<double> start(e * n);
-  if (d_initial) {
-    QMBED_HIP(hipMemcpyAsync(start.data(), d_initial, e * n * 8, hipMemcpyDeviceToDevice, s));
-  } else {
-    DeviceArray<double> re(n);
-    deterministic_start_device(re.data(), n, o.seed, s);
-    if (c) {
-      std::vector<double> h(n), z(2 * n, 0.0);
-      QMBED_HIP(hipMemcpyAsync(h.data(), re.data(), n * 8, hipMemcpyDeviceToHost, s));
-      QMBED_HIP(hipStreamSynchronize(s));
-      for (size_t i = 0; i < n; ++i) z[2 * i] = h[i];
-      QMBED_HIP(hipMemcpyAsync(start.data(), z.data(), 2 * n * 8, hipMemcpyHostToDevice, s));
-    } else {
-      QMBED_HIP(hipMemcpyAsync(start.data(), re.data(), n * 8, hipMemcpyDeviceToDevice, s));
-    }
-    QMBED_HIP(hipStreamSynchronize(s));  // re dies here
-  }
+  // start vector (pooled scratch: no allocation per solve)
+  PooledBuffers scratch(s);
+  grow(scratch.b->start, e * n);
+  grow(scratch.b->slots, 2);
+  double* start = scratch.b->start.data();
+  double* sl = scratch.b->slots.data();
+  if (d_initial) QMBED_HIP(hipMemcpyAsync(start, d_initial, e * n * 8, hipMemcpyDeviceToDevice, s));
+  else deterministic_start_device(start, n, o.seed, s, c);
   {
-    lk::Workspace ws;
-    DeviceArray<double> sl(2);
-    lk::nrm2sq_to_slot(c, start.data(), n, sl.data(), ws, s);
+    lk::nrm2sq_to_slot(c, start, n, sl, scratch.b->ws, s);
     double nn = 0;
-    QMBED_HIP(hipMemcpyAsync(&nn, sl.data(), 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipMemcpyAsync(&nn, sl, 8, hipMemcpyDeviceToHost, s));
     QMBED_HIP(hipStreamSynchronize(s));
     if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) throw non_convergence(0, std::sqrt(nn));
-    lk::normalize(c, start.data(), sl.data(), n, start.data(), nullptr, nullptr, s);
-    QMBED_HIP(hipStreamSynchronize(s));  // ws and sl die here
+    lk::normalize(c, start, sl, n, start, nullptr, nullptr, s);
   }
 
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
@@ -798,7 +802,7 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
     }
   };
 
-  WindowResult wr = run_expanding(start.data(), fp32, o.tolerance, o.max_iterations);
+  WindowResult wr = run_expanding(start, fp32, o.tolerance, o.max_iterations);
   vec_out = std::move(wr.vectors);
   return wr.es;
 }

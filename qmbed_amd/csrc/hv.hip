@@ -72,7 +72,19 @@ struct HvArgs {
   // compiled index stream (sliced ELL of packed target words), or null
   const uint32_t* stream;
   const uint32_t* slice_ptr;
+  // compact stream words (compact != 0): idx | amplitude entry (relative to
+  // off_base) << c_b | orbit code << (c_b + c_ob) | (sign < 0) << (c_b +
+  // c_ob + c_cb) | χ-index << (c_b + c_ob + c_cb + c_sb); no group walk
+  int compact;
+  int c_b, c_ob, c_cb, c_sb, c_off_base;
+  // per-row diagonal (re[n], then im[n]) of the current coefficients, or null
+  const double* diag;
 };
+
+// `bits` wide field of e at `shift` (0 when bits == 0)
+__device__ __forceinline__ uint32_t field(uint32_t e, int shift, int bits) {
+  return bits ? (e >> shift) & ((1u << bits) - 1u) : 0u;
+}
 
 __device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t mask) {
   return uint32_t(y >> (L - j)) & mask;
@@ -865,9 +877,22 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
     for (int g = ag.next(); g >= 0; g = ag.next()) {
       uint32_t e = 0xFFFFFFFFu;
       int idx, orbit, ci, sign;
-      if (target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign))
-        e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) | (uint32_t(sign < 0) << (a.img_bits + 4)) |
-            (uint32_t(ci) << (a.img_bits + 5));
+      if (target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign)) {
+        if (a.compact) {
+          const uint32_t off = a.meta[4 * g + 3] + uint32_t(pattern_of(r, a.meta[4 * g + 1])) - uint32_t(a.c_off_base);
+          int sh = a.c_b;
+          e = uint32_t(idx) | (off << sh);
+          sh += a.c_ob;
+          if (a.c_cb) e |= uint32_t(orbit_code[orbit]) << sh;
+          sh += a.c_cb;
+          if (a.c_sb) e |= uint32_t(sign < 0) << sh;
+          sh += a.c_sb;
+          if (shift_needed) e |= uint32_t(ci) << sh;
+        } else {
+          e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) | (uint32_t(sign < 0) << (a.img_bits + 4)) |
+              (uint32_t(ci) << (a.img_bits + 5));
+        }
+      }
       out[base + kSlice * k + lane] = e;
       ++k;
     }
@@ -906,21 +931,30 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
   const bool live = row < a.n;
   const uint32_t base = a.slice_ptr[slice];
   const uint32_t width = (a.slice_ptr[slice + 1] - base) / kSlice;
-  const uint32_t r = live ? a.states[row] : 0u;
+  // the state itself is needed only to walk the groups
+  const bool walk = !(a.compact && a.diag);
+  const uint32_t r = (live && walk) ? a.states[row] : 0u;
   S sq_r = 1;
   if (SYM && live) sq_r = S(s_sq[a.orbits[row]]);
 
   Cx<S> acc[NV];
   {
     Cx<S> dsum{0, 0};
-    const int ndiag = live ? a.n_diag : 0;
-    for (int g = 0; g < ndiag; ++g) {
-      const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
-      if (CMODE == 2) {
-        dsum.re += S(s_amp[2 * off]);
-        dsum.im += S(s_amp[2 * off + 1]);
-      } else {
-        dsum.re += S(s_amp[off]);
+    if (a.diag) {
+      if (live) {
+        dsum.re = S(a.diag[row]);
+        if (CMODE == 2) dsum.im = S(a.diag[a.n + row]);
+      }
+    } else {
+      const int ndiag = live ? a.n_diag : 0;
+      for (int g = 0; g < ndiag; ++g) {
+        const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
+        if (CMODE == 2) {
+          dsum.re += S(s_amp[2 * off]);
+          dsum.im += S(s_amp[2 * off + 1]);
+        } else {
+          dsum.re += S(s_amp[off]);
+        }
       }
     }
     const uint32_t own = live ? row : 0u;
@@ -937,9 +971,85 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
     }
   }
   constexpr int B = NV > 1 ? 4 : 8;
+  const uint32_t* words = a.stream + base + lane;
+  if (a.compact) {
+    // every entry names its amplitude, orbit code, sign and character: no
+    // group walk; loads are unconditional (clamped ordinal, then masked)
+    const int so = a.c_b, sc = so + a.c_ob, ss = sc + a.c_cb, sh = ss + a.c_sb;
+    const uint32_t imask = (1u << a.c_b) - 1u;
+    for (uint32_t k0 = 0; k0 < width; k0 += B) {
+      uint32_t ent[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const uint32_t e = words[size_t(kSlice) * min(k0 + b, width - 1)];
+        ent[b] = k0 + b < width ? e : 0xFFFFFFFFu;
+      }
+      S xr[B][NV], xi[B][NV];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int idx = ent[b] != 0xFFFFFFFFu ? int(ent[b] & imask) : 0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          if (CMODE == 0) {
+            xr[b][v] = x[v * a.ld + idx];
+            xi[b][v] = 0;
+          } else {
+            xr[b][v] = x[2 * (v * a.ld + idx)];
+            xi[b][v] = x[2 * (v * a.ld + idx) + 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const uint32_t e = ent[b];
+        if (e == 0xFFFFFFFFu) continue;
+        const int off = a.c_off_base + int(field(e, so, a.c_ob));
+        S scale = field(e, ss, a.c_sb) ? S(-1) : S(1);
+        if (SYM) scale *= sq_r * S(s_crsq[field(e, sc, a.c_cb)]);
+        Cx<S> c;
+        if (CMODE == 2) {
+          c.re = S(s_amp[2 * off]) * scale;
+          c.im = S(s_amp[2 * off + 1]) * scale;
+        } else {
+          c.re = S(s_amp[off]) * scale;
+          c.im = 0;
+        }
+        if ((SYM == 1 || SYM == 3) && SHIFT) {
+          const int ci = int(e >> sh);
+          const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+          if (CMODE == 2) {
+            const S re = c.re * pr - c.im * pi;
+            c.im = c.re * pi + c.im * pr;
+            c.re = re;
+          } else {
+            c.re *= pr;  // real characters (k = L/2): pi == 0
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          if (CMODE == 0) {
+            acc[v].re += c.re * xr[b][v];
+          } else {
+            acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
+            acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+          }
+        }
+      }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (CMODE == 0) {
+        y[v * a.ld + row] = acc[v].re;
+      } else {
+        y[2 * (v * a.ld + row)] = acc[v].re;
+        y[2 * (v * a.ld + row) + 1] = acc[v].im;
+      }
+    }
+    return;
+  }
   const uint32_t imask = (1u << a.img_bits) - 1u;
   ActiveGroups ag(s_meta, a.n_diag, a.n_off, r);
-  const uint32_t* words = a.stream + base + lane;
   for (uint32_t k0 = 0; k0 < width; k0 += B) {
     uint32_t ent[B];
 #pragma unroll
@@ -1048,6 +1158,27 @@ __global__ void __launch_bounds__(kBlock) hv_stream_batch_kernel(const BatchSect
   }
 }
 
+// Per-row diagonal of the current amplitude tables (gather layout: one
+// double per entry, or (re, im) when complex): out[row] = Σ re, out[n + row]
+// = Σ im over the diagonal groups, in group order.
+__global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp, double* out) {
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= a.n) return;
+  const uint32_t r = a.states[row];
+  double re = 0, im = 0;
+  for (int g = 0; g < a.n_diag; ++g) {
+    const int off = int(a.meta[4 * g + 3]) + pattern_of(r, a.meta[4 * g + 1]);
+    if (complex_amp) {
+      re += a.amp[2 * off];
+      im += a.amp[2 * off + 1];
+    } else {
+      re += a.amp[off];
+    }
+  }
+  out[row] = re;
+  out[a.n + row] = im;
+}
+
 int sym_code(const SpinBasis1D& b) {
   const SectorGeometry& g = b.geometry();
   if (g.kmom >= 0 && g.parity) return 3;
@@ -1080,6 +1211,12 @@ struct Operator::DeviceTables {
   // compiled index stream (autotuned against both)
   bool use_stream = false;
   DeviceArray<uint32_t> stream, slice_ptr;
+  // compact stream layout (field widths; see HvArgs)
+  bool compact = false;
+  int c_b = 0, c_ob = 0, c_cb = 0, c_sb = 0, c_off_base = 0;
+  int n_codes = 1;
+  // per-row diagonal of the current coefficients (re[n], im[n])
+  DeviceArray<double> diag;
   HvArgs args(const SpinBasis1D& b) const {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
@@ -1117,6 +1254,13 @@ struct Operator::DeviceTables {
     a.gate = nullptr;
     a.stream = use_stream ? stream.data() : nullptr;
     a.slice_ptr = slice_ptr.data();
+    a.compact = compact ? 1 : 0;
+    a.c_b = c_b;
+    a.c_ob = c_ob;
+    a.c_cb = c_cb;
+    a.c_sb = c_sb;
+    a.c_off_base = c_off_base;
+    a.diag = diag.size() >= 2 * b.len() && b.len() > 0 ? diag.data() : nullptr;
     return a;
   }
 };
@@ -1162,6 +1306,17 @@ static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, 
   if (t.famp.size() < std::max<size_t>(famp.size(), 2)) t.famp.resize(std::max<size_t>(famp.size(), 2));
   if (!amp.empty()) QMBED_HIP(hipMemcpyAsync(t.amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice, s));
   if (!famp.empty()) QMBED_HIP(hipMemcpyAsync(t.famp.data(), famp.data(), famp.size() * 8, hipMemcpyHostToDevice, s));
+  // the per-row diagonal read by the stream kernel (one pass per evaluation
+  // instead of a walk over the diagonal groups in every H·v)
+  const SpinBasis1D& b = op.basis();
+  const size_t n = b.len();
+  if (n > 0 && n <= (size_t(1) << 28)) {
+    if (t.diag.size() != 2 * n) t.diag.resize(2 * n);
+    HvArgs a = t.args(b);
+    hipLaunchKernelGGL(diag_kernel, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a,
+                       t.complex_coef ? 1 : 0, t.diag.data());
+    QMBED_HIP(hipGetLastError());
+  }
   QMBED_HIP(hipStreamSynchronize(s));  // the host vectors die here
 }
 
@@ -1470,17 +1625,39 @@ static bool prepare_codes(Operator& op, std::vector<uint8_t>& code) {
   t.code_rsq.resize(16);
   QMBED_HIP(hipMemcpy(t.code_rsq.data(), crsq.data(), 16 * 8, hipMemcpyHostToDevice));
   t.img_bits = bits;
+  t.n_codes = int(sizes.size());
   return true;
 }
 
 // Build the compiled index stream (count pass, slice offsets on the host,
 // fill pass). Returns false when it does not fit the memory budget.
+static int bits_for(size_t values) {
+  int bits = 0;
+  while ((size_t(1) << bits) < values) ++bits;
+  return bits;
+}
+
 static bool build_stream(Operator& op, hipStream_t s) {
   const SpinBasis1D& b = op.basis();
   auto& t = *op.tables_;
   std::vector<uint8_t> code;
   if (!prepare_codes(op, code)) return false;
   const size_t n = b.len();
+  // compact words when every field fits in 32 bits: index (all ones is never
+  // an index), amplitude entry among the off-diagonal tables, orbit code,
+  // reflection sign (parity sectors), χ-index (k != 0)
+  {
+    const SectorGeometry& g = b.geometry();
+    int base = 0;  // the off-diagonal tables follow the diagonal ones
+    for (auto& grp : op.diagonal_groups()) base += int(grp.amp.size());
+    t.c_off_base = base;
+    t.c_b = bits_for(n + 1);
+    t.c_ob = bits_for(size_t(std::max(t.n_amp - base, 1)));
+    t.c_cb = bits_for(size_t(t.n_codes));
+    t.c_sb = g.parity ? 1 : 0;
+    const int chb = g.kmom > 0 ? bits_for(size_t(g.L)) : 0;
+    t.compact = t.c_b + t.c_ob + t.c_cb + t.c_sb + chb <= 32;
+  }
   const uint32_t n_slices = uint32_t((n + kSlice - 1) / kSlice);
   HvArgs a = t.args(b);
   DeviceArray<uint32_t> width(n_slices);

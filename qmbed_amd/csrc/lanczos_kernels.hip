@@ -388,9 +388,16 @@ __global__ __launch_bounds__(kT) void k_from32(const float* y, size_t n, E* x, c
 }
 
 // deterministic_start (QMBED src/solve/mod.rs:729-742)
-__global__ void k_det_start(double* out, size_t n, uint64_t seed) {
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT)
-    out[i] = deterministic_entry(uint64_t(i), seed);
+__global__ void k_det_start(double* out, size_t n, uint64_t seed, int complex_pairs) {
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    const double v = deterministic_entry(uint64_t(i), seed);
+    if (complex_pairs) {
+      out[2 * i] = v;
+      out[2 * i + 1] = 0.0;
+    } else {
+      out[i] = v;
+    }
+  }
 }
 
 // column tile of the overlap kernel: 16 real or 8 complex columns
@@ -568,8 +575,8 @@ void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, dou
 
 }  // namespace lk
 
-void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s) {
-  hipLaunchKernelGGL(lk::k_det_start, dim3(lk::nblocks(n)), dim3(lk::kT), 0, s, d_out, n, seed);
+void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s, bool complex) {
+  hipLaunchKernelGGL(lk::k_det_start, dim3(lk::nblocks(n)), dim3(lk::kT), 0, s, d_out, n, seed, complex ? 1 : 0);
   QMBED_HIP(hipGetLastError());
 }
 

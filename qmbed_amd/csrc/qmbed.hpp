@@ -518,7 +518,8 @@ std::vector<double> selected_ritz_values(const std::vector<double>& alpha, const
                                          std::vector<std::vector<double>>* vectors = nullptr);
 
 // Deterministic start (QMBED src/solve/mod.rs:729-742), written on the device.
-void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s);
+// complex: interleaved (value, 0) pairs
+void deterministic_start_device(double* d_out, size_t n, uint64_t seed, hipStream_t s, bool complex = false);
 
 int device_count();
 

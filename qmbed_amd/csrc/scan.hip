@@ -459,35 +459,38 @@ WarmStartScan::PointResult WarmStartScan::solve(const double* point, const Eigsh
   const size_t N = op_.dim();
   if (warm) {
     if (!trained_) throw invalid_options("the warm-start network has not been trained");
-    // network on the device; the start vector v₀ = S y
-    std::vector<float> x(static_cast<size_t>(P_));
-    for (int p = 0; p < P_; ++p) x[size_t(p)] = float((point[p] - center_[size_t(p)]) / scale_[size_t(p)]);
-    DeviceArray<float> dx(static_cast<size_t>(P_)), dy(static_cast<size_t>(ns_));
-    DeviceArray<double> dc(static_cast<size_t>(ns_)), v0(N);
-    QMBED_HIP(hipMemcpyAsync(dx.data(), x.data(), x.size() * 4, hipMemcpyHostToDevice, s_));
-    mlp_forward_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dx.data(), 1, dy.data(), s_);
-    hipLaunchKernelGGL(k_f2d, dim3(1), dim3(64), 0, s_, dy.data(), ns_, dc.data());
+    // coefficients y (network on the device, or the projected family on the
+    // host); the start vector v₀ = S y
+    if (v0_.size() != N) v0_.resize(N);
+    if (dc_.size() != size_t(ns_)) dc_.resize(static_cast<size_t>(ns_));
     std::vector<double> c(static_cast<size_t>(ns_));
     if (start == Start::SnapshotRR) {
       c = continuation(point);
-      QMBED_HIP(hipMemcpyAsync(dc.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice, s_));
+      QMBED_HIP(hipMemcpyAsync(dc_.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice, s_));
+      lk::gemm_vc(false, S_.data(), N, ns_, dc_.data(), 1, v0_.data(), s_);
+    } else {
+      std::vector<float> x(static_cast<size_t>(P_));
+      for (int p = 0; p < P_; ++p) x[size_t(p)] = float((point[p] - center_[size_t(p)]) / scale_[size_t(p)]);
+      if (dx_.size() != size_t(P_)) dx_.resize(static_cast<size_t>(P_));
+      if (dy_.size() != size_t(ns_)) dy_.resize(static_cast<size_t>(ns_));
+      QMBED_HIP(hipMemcpyAsync(dx_.data(), x.data(), x.size() * 4, hipMemcpyHostToDevice, s_));
+      mlp_forward_device(P_, opt_.hidden1, opt_.hidden2, ns_, params_.data(), dx_.data(), 1, dy_.data(), s_);
+      hipLaunchKernelGGL(k_f2d, dim3(1), dim3(64), 0, s_, dy_.data(), ns_, dc_.data());
+      lk::gemm_vc(false, S_.data(), N, ns_, dc_.data(), 1, v0_.data(), s_);
+      QMBED_HIP(hipMemcpyAsync(c.data(), dc_.data(), c.size() * 8, hipMemcpyDeviceToHost, s_));
+      QMBED_HIP(hipStreamSynchronize(s_));  // c (host) and x die here
     }
-    lk::gemm_vc(false, S_.data(), N, ns_, dc.data(), 1, v0.data(), s_);
-    if (start != Start::SnapshotRR)
-      QMBED_HIP(hipMemcpyAsync(c.data(), dc.data(), c.size() * 8, hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipStreamSynchronize(s_));
     double cn = 0;
     for (double v : c) cn += v * v;
     // S is orthonormal, so ‖v₀‖ = ‖y‖; a vanishing or non-finite prediction
     // falls back to the plain start
     if (std::isfinite(cn) && cn > 1.0e-6) {
-      r.es = eigsh_on_device(op_, oo, v0.data(), false, ctl, vecs, cv, s_);
+      r.es = eigsh_on_device(op_, oo, v0_.data(), false, ctl, vecs, cv, s_);
       r.warm = true;
     } else {
       r.es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
       r.fallback = true;
     }
-    QMBED_HIP(hipStreamSynchronize(s_));  // dx, dy, dc, v0 die here
   } else {
     r.es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
   }

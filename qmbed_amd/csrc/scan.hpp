@@ -79,6 +79,9 @@ class WarmStartScan {
   // component: Sᵀ H(c) S = proj_[0] + Σ_p c_p proj_[1 + p]
   std::vector<std::vector<double>> proj_;
   std::vector<double> gauge_;  // Sᵀ 1: the gauge sign of S·y is that of y·gauge_
+  // per-solve device buffers, kept between solves
+  DeviceArray<float> dx_, dy_;
+  DeviceArray<double> dc_, v0_;
   DeviceArray<float> params_;  // network parameters
   std::vector<double> center_, scale_;
 };
