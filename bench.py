@@ -96,9 +96,16 @@ def scan_grid():
 
 
 def hv_roofline(model, dtype, vec_bytes, iters=50):
+    """dtype "mixed": FP32 input gathered, FP64 output (the FP32 schedule's
+    product); vec_bytes is then the input element size."""
     nnz = model.nnz()
-    ms = model.time_apply(dtype, iters=iters)
-    algo = model.dim * (4 + vec_bytes) + nnz * (vec_bytes + 4)
+    if dtype == "mixed":
+        ms = model.time_apply_mixed(complex_=not model.is_real, iters=iters)
+        out_bytes = 2 * vec_bytes
+    else:
+        ms = model.time_apply(dtype, iters=iters)
+        out_bytes = vec_bytes
+    algo = model.dim * (4 + out_bytes) + nnz * (vec_bytes + 4)
     gbs = algo / (ms * 1e-3) / 1e9
     return {"dim": model.dim, "nnz": nnz, "bytes_per_launch": algo, "ms": ms, "achieved_GBs": gbs,
             "frac": gbs / HBM_PEAK_GBS, **model.hv_config()}
@@ -136,7 +143,8 @@ def hv_profile(qa):
                      components=comps)
     model.set_parameters(hx=1.0, hz=0.25)
     out = {"hv_fp32": hv_roofline(model, np.float32, 4, iters=20),
-           "hv_fp64": hv_roofline(model, np.float64, 8, iters=20)}
+           "hv_fp64": hv_roofline(model, np.float64, 8, iters=20),
+           "hv_mixed": hv_roofline(model, "mixed", 4, iters=20)}
     L = qa.library()
     nbytes = 256 << 20
     a, b = qa.DeviceBuffer(nbytes), qa.DeviceBuffer(nbytes)
@@ -228,6 +236,9 @@ def main():
     ap.add_argument("--hv-profile", action="store_true",
                     help="only the headline sector's FP32 and FP64 H·v (20 launches each) and 5 device copies of "
                          "256 MiB, for hardware-counter runs")
+    ap.add_argument("--batch-profile", action="store_true",
+                    help="only the 72-sector XXZ L=24 ground-state batch and the 64-state evolution (one timed run "
+                         "each after a warm-up), for kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (sectors, scan points and right-hand sides "
@@ -238,6 +249,18 @@ def main():
 
     if args.hv_profile:
         hv_profile(qa)
+        return
+    if args.batch_profile:
+        L = 24
+        keys = [(up, k) for up in (11, 12, 13) for k in range(L)]
+        models = [qa.Model({"kind": "spin", "sites": L, "up": up, "momentum": k}, xxz_terms(L, 1.0))
+                  for up, k in keys]
+        qa.eigsh_batch(models, 1)
+        b = qa.eigsh_batch(models, 1)
+        for m in models:
+            m.close()
+        e = expm_batch(qa)
+        print(json.dumps({"batch": b["batch"], "expm": e["c4_expm64"]}))
         return
 
     # ---- setup: family, training scan, network ----
@@ -300,6 +323,7 @@ def main():
     model.set_parameters(hx=1.0, hz=0.25)
     hv32 = hv_roofline(model, np.float32, 4)
     hv64 = hv_roofline(model, np.float64, 8)
+    hvmix = hv_roofline(model, "mixed", 4)
 
     extras = {
         "scan": {
@@ -339,6 +363,7 @@ def main():
         },
         "hv_c5_fp32": hv32,
         "hv_c5_fp64": hv64,
+        "hv_c5_fp32_in_fp64_out": hvmix,
     }
 
     if not args.no_extended:

@@ -55,6 +55,11 @@ void qmbed_string_free(char* response);
 typedef struct qmbed_hip_model qmbed_hip_model;
 
 enum { QMBED_F64 = 0, QMBED_F32 = 1, QMBED_C128 = 2, QMBED_C64 = 3 };
+/* Mixed-precision actions (compiled-stream mode only; other modes answer
+ * -3): FP32 input, FP64 accumulation and output, real (x float, y double)
+ * or complex (x complex64, y complex128). The FP32 Lanczos schedule's
+ * product. */
+enum { QMBED_F32_TO_F64 = 4, QMBED_C64_TO_C128 = 5 };
 
 /* model_json: {"basis":{...},"terms":[...],"components":[...]?,
  * "checks":{...}?} as in create_model (components: [{"name", "terms",
@@ -87,7 +92,8 @@ double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void*
 /* eigsh with a SolverRequest JSON; response as qmbed_run_json's result plus
  * fp32_iterations / fp64_iterations / reorthogonalization counters. Accepts
  * the extra solver keys "matvec_precision" ("fp64"|"fp32") and
- * "early_stop" (bool). */
+ * "early_stop" (bool), "check_every" (steps between Ritz-estimate checks,
+ * 1-64, default 2) and "reorthogonalization_eta" (DGKS threshold, (0, 1]). */
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
 
 /* Solve `count` models (typically the symmetry sectors of one Hamiltonian)

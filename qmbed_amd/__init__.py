@@ -266,6 +266,38 @@ class Model:
             raise QmbedError("timed apply failed")
         return ms
 
+    def apply_mixed(self, x):
+        """FP32 input, FP64 accumulation and output (compiled-stream mode):
+        float32 → float64, or complex64 → complex128."""
+        x = np.ascontiguousarray(x)
+        if x.dtype not in (np.dtype(np.float32), np.dtype(np.complex64)):
+            x = x.astype(np.complex64 if np.iscomplexobj(x) else np.float32)
+        code = 5 if x.dtype == np.dtype(np.complex64) else 4
+        y = np.empty(x.shape, dtype=np.complex128 if code == 5 else np.float64)
+        rc = library().qmbed_hip_apply_host(self._h, code, x.ctypes.data, y.ctypes.data)
+        if rc == -3:
+            raise QmbedError("the mixed-precision action needs the compiled-stream mode")
+        if rc != 0:
+            raise QmbedError(f"apply failed (code {rc})")
+        return y
+
+    def time_apply_mixed(self, complex_=False, iters=20, seed=0):
+        """Average ms per mixed-precision H·v (FP32 in, FP64 out) on device data."""
+        rng = np.random.default_rng(seed)
+        n_in = self.dim * (2 if complex_ else 1)
+        x = rng.standard_normal(n_in).astype(np.float32)
+        bx, by = DeviceBuffer(x.nbytes), DeviceBuffer(x.nbytes * 2)
+        bx.upload(x)
+        L = library()
+        code = 5 if complex_ else 4
+        ms = L.qmbed_hip_time_apply(self._h, code, bx.ptr, by.ptr, 2)  # warm
+        ms = L.qmbed_hip_time_apply(self._h, code, bx.ptr, by.ptr, iters)
+        bx.free()
+        by.free()
+        if ms < 0:
+            raise QmbedError("timed apply failed")
+        return ms
+
     def apply(self, x):
         x = np.ascontiguousarray(x)
         if x.dtype not in _DTYPES:

@@ -813,6 +813,11 @@ int qmbed_hip_apply_device(const qmbed_hip_model* m, int dtype, const void* x, v
       case QMBED_C64:
         op.apply(static_cast<const std::complex<float>*>(x), static_cast<std::complex<float>*>(y), s);
         break;
+      case QMBED_F32_TO_F64:
+      case QMBED_C64_TO_C128:
+        if (!op.apply_mixed(static_cast<const float*>(x), static_cast<double*>(y), dtype == QMBED_C64_TO_C128, s))
+          return -3;
+        break;
       default: return -2;
     }
     return 0;
@@ -821,12 +826,15 @@ int qmbed_hip_apply_device(const qmbed_hip_model* m, int dtype, const void* x, v
   }
 }
 
-static size_t dtype_size(int dtype) {
+// element bytes of the input (out = false) or output vector
+static size_t dtype_size(int dtype, bool out = false) {
   switch (dtype) {
     case QMBED_F64: return 8;
     case QMBED_F32: return 4;
     case QMBED_C128: return 16;
     case QMBED_C64: return 8;
+    case QMBED_F32_TO_F64: return out ? 8 : 4;
+    case QMBED_C64_TO_C128: return out ? 16 : 8;
     default: return 0;
   }
 }
@@ -835,14 +843,14 @@ int qmbed_hip_apply_host(const qmbed_hip_model* m, int dtype, const void* x, voi
   try {
     auto* tm = reinterpret_cast<const TypedModel*>(m);
     size_t n = tm->m->basis->len();
-    size_t bytes = dtype_size(dtype) * n;
+    size_t bytes = dtype_size(dtype) * n, out_bytes = dtype_size(dtype, true) * n;
     if (!bytes) return dtype_size(dtype) ? 0 : -2;
-    DeviceArray<uint8_t> dx(bytes), dy(bytes);
+    DeviceArray<uint8_t> dx(bytes), dy(out_bytes);
     QMBED_HIP(hipMemcpy(dx.data(), x, bytes, hipMemcpyHostToDevice));
     int rc = qmbed_hip_apply_device(m, dtype, dx.data(), dy.data(), nullptr);
     if (rc) return rc;
     QMBED_HIP(hipStreamSynchronize(tm->m->stream));
-    QMBED_HIP(hipMemcpy(y, dy.data(), bytes, hipMemcpyDeviceToHost));
+    QMBED_HIP(hipMemcpy(y, dy.data(), out_bytes, hipMemcpyDeviceToHost));
     return 0;
   } catch (...) {
     return -1;

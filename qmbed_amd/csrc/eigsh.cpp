@@ -285,11 +285,15 @@ class LanczosWindow {
   void matvec(int j, const int* gate) {
     if (use32_) {
       float* x32 = buf_->x32.data();
-      float* y32 = buf_->y32.data();
-      if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32), reinterpret_cast<std::complex<float>*>(y32),
-                        s_, gate);
-      else op_.apply(x32, y32, s_, gate);
-      lk::from_f32(c_, y32, n_, w(), s_, gate);
+      // FP32 input into FP64 accumulation and output where the operator has
+      // that kernel; otherwise an FP32 product converted to FP64
+      if (!op_.apply_mixed(x32, w(), c_, s_, gate)) {
+        float* y32 = buf_->y32.data();
+        if (c_) op_.apply(reinterpret_cast<const std::complex<float>*>(x32),
+                          reinterpret_cast<std::complex<float>*>(y32), s_, gate);
+        else op_.apply(x32, y32, s_, gate);
+        lk::from_f32(c_, y32, n_, w(), s_, gate);
+      }
       ++matvecs32_;
     } else {
       if (c_) op_.apply(reinterpret_cast<const cplx*>(col(j)), reinterpret_cast<cplx*>(w()), s_, gate);

@@ -905,8 +905,11 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
 // entries they name, then the B products are accumulated.
 // The rows of block `blk` of one sector (shared by the single-sector kernel
 // and the sector-batched kernel).
-template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
-__device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict__ x, S* __restrict__ y,
+// S: accumulation and output type; SX: input type (float input with double
+// accumulation and output is the mixed-precision product of the Lanczos
+// FP32 schedule)
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S>
+__device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restrict__ x, S* __restrict__ y,
                                             uint32_t blk) {
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
@@ -961,10 +964,10 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       if (CMODE == 0) {
-        acc[v].re = dsum.re * x[v * a.ld + own];
+        acc[v].re = dsum.re * S(x[v * a.ld + own]);
         acc[v].im = 0;
       } else {
-        const S xr = x[2 * (v * a.ld + own)], xi = x[2 * (v * a.ld + own) + 1];
+        const S xr = S(x[2 * (v * a.ld + own)]), xi = S(x[2 * (v * a.ld + own) + 1]);
         acc[v].re = dsum.re * xr - dsum.im * xi;
         acc[v].im = dsum.re * xi + dsum.im * xr;
       }
@@ -991,11 +994,11 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           if (CMODE == 0) {
-            xr[b][v] = x[v * a.ld + idx];
+            xr[b][v] = S(x[v * a.ld + idx]);
             xi[b][v] = 0;
           } else {
-            xr[b][v] = x[2 * (v * a.ld + idx)];
-            xi[b][v] = x[2 * (v * a.ld + idx) + 1];
+            xr[b][v] = S(x[2 * (v * a.ld + idx)]);
+            xi[b][v] = S(x[2 * (v * a.ld + idx) + 1]);
           }
         }
       }
@@ -1064,11 +1067,11 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         if (CMODE == 0) {
-          xr[b][v] = x[v * a.ld + idx];
+          xr[b][v] = S(x[v * a.ld + idx]);
           xi[b][v] = 0;
         } else {
-          xr[b][v] = x[2 * (v * a.ld + idx)];
-          xi[b][v] = x[2 * (v * a.ld + idx) + 1];
+          xr[b][v] = S(x[2 * (v * a.ld + idx)]);
+          xi[b][v] = S(x[2 * (v * a.ld + idx) + 1]);
         }
       }
     }
@@ -1123,10 +1126,10 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const S* __restrict
   }
 }
 
-template <typename S, int CMODE, int SYM, bool SHIFT, int NV>
-__global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const S* __restrict__ x, S* __restrict__ y) {
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S>
+__global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const SX* __restrict__ x, S* __restrict__ y) {
   if (a.gate && *a.gate) return;
-  stream_rows<S, CMODE, SYM, SHIFT, NV>(a, x, y, blockIdx.x);
+  stream_rows<S, CMODE, SYM, SHIFT, NV, SX>(a, x, y, blockIdx.x);
 }
 
 // One sector of a sector-batched action: its tables and where its rows sit
@@ -1550,14 +1553,14 @@ void launch_hv_mode(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipS
   QMBED_HIP(hipGetLastError());
 }
 
-template <typename S, int CMODE, int NV>
-void launch_hv_stream(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
+template <typename S, int CMODE, int NV, typename SX = S>
+void launch_hv_stream(const HvArgs& a, int sym, bool shift, const SX* x, S* y, hipStream_t s) {
   const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
   dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t G = size_t(a.n_diag + a.n_off);
   const size_t sm = sizeof(double) * ((CMODE == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G;
 #define QMBED_HVS_CASE(SYMV, SH) \
-  hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV>), grid, block, sm, s, a, x, y);
+  hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV, SX>), grid, block, sm, s, a, x, y);
   switch (sym) {
     case 0: QMBED_HVS_CASE(0, false); break;
     case 1:
@@ -1656,7 +1659,10 @@ static bool build_stream(Operator& op, hipStream_t s) {
     t.c_cb = bits_for(size_t(t.n_codes));
     t.c_sb = g.parity ? 1 : 0;
     const int chb = g.kmom > 0 ? bits_for(size_t(g.L)) : 0;
-    t.compact = t.c_b + t.c_ob + t.c_cb + t.c_sb + chb <= 32;
+    // QMBED_HV_STREAM_FORMAT=groups keeps the group-walk words everywhere
+    const char* fmt = std::getenv("QMBED_HV_STREAM_FORMAT");
+    const bool force_groups = fmt && std::string(fmt) == "groups";
+    t.compact = !force_groups && t.c_b + t.c_ob + t.c_cb + t.c_sb + chb <= 32;
   }
   const uint32_t n_slices = uint32_t((n + kSlice - 1) / kSlice);
   HvArgs a = t.args(b);
@@ -1882,6 +1888,24 @@ void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipSt
   float* yf = reinterpret_cast<float*>(y);
   if (real_) launch_hv<float, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
   else launch_hv<float, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
+}
+
+bool Operator::apply_mixed(const float* x, double* y, bool complex_vectors, hipStream_t s, const int* gate) const {
+  if (!tables_->use_stream || !tables_->hermitian) return false;
+  HvArgs a = tables_->args(*basis_);
+  a.gate = gate;
+  const int sym = sym_code(*basis_);
+  const bool sh = basis_->geometry().kmom > 0;
+  if (a.n == 0) return true;
+  if (!complex_vectors) {
+    if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+    launch_hv_stream<double, 0, 1, float>(a, sym, sh, x, y, s);
+  } else if (real_) {
+    launch_hv_stream<double, 1, 1, float>(a, sym, sh, x, y, s);
+  } else {
+    launch_hv_stream<double, 2, 1, float>(a, sym, sh, x, y, s);
+  }
+  return true;
 }
 
 void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const {

@@ -236,6 +236,10 @@ class Operator {
   void apply(const cplx* x, cplx* y, hipStream_t s, const int* gate = nullptr) const;      // complex vectors
   void apply(const float* x, float* y, hipStream_t s, const int* gate = nullptr) const;    // real FP32
   void apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s, const int* gate = nullptr) const;
+  // Mixed precision: FP32 input (real, or interleaved complex when
+  // complex_vectors), FP64 accumulation and output. Only the compiled-stream
+  // mode has this kernel: returns false (nothing launched) otherwise.
+  bool apply_mixed(const float* x, double* y, bool complex_vectors, hipStream_t s, const int* gate = nullptr) const;
   // nvec columns (stride = dim), complex; one lookup shared across columns
   void apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const;
 
@@ -376,7 +380,7 @@ enum class MatvecPrecision { FP64 = 0, FP32 = 1 };
 struct EigshControls {
   MatvecPrecision precision = MatvecPrecision::FP64;
   // check Ritz residual estimates every `check_every` steps inside a window
-  int check_every = 4;
+  int check_every = 2;
   // when true, stop inside the window as soon as the requested pairs have
   // converged (the reference always runs the full window)
   bool early_stop = true;

@@ -53,12 +53,15 @@ def _ham_ok(name, up):
     return not (name == "xyfield" and up is not None)
 
 
-@pytest.mark.parametrize("mode", ["walk", "image", "stream"])
+@pytest.mark.parametrize("mode", ["walk", "image", "stream", "stream-groups"])
 @pytest.mark.parametrize("name", sorted(HAMS))
 @pytest.mark.parametrize("L,up,k,p", cases.SECTORS[:18])
 def test_hv_fp64(qa, oracle, monkeypatch, name, L, up, k, p, mode):
     if not _ham_ok(name, up):
         pytest.skip("term set does not conserve the sector")
+    # stream: compact words where they fit; stream-groups: the group-walk words
+    monkeypatch.setenv("QMBED_HV_STREAM_FORMAT", "groups" if mode == "stream-groups" else "compact")
+    mode = "stream" if mode == "stream-groups" else mode
     monkeypatch.setenv("QMBED_HV_MODE", mode)
     spec = HAMS[name](L)
     print(f"[hv] {name} L={L} up={up} k={k} p={p}", flush=True)
@@ -89,9 +92,22 @@ def test_hv_fp64(qa, oracle, monkeypatch, name, L, up, k, p, mode):
         assert np.max(np.abs(yr.imag)) <= 1e-14 * max(np.linalg.norm(yr), 1.0)
         y32 = m.apply(xr.astype(np.float32))
         assert np.linalg.norm(y32 - yr.real) <= 2e-6 * max(np.linalg.norm(yr), 1e-300)
+        if mode == "stream":
+            # FP32 input, FP64 accumulation: exact up to the input's rounding
+            x32 = xr.astype(np.float32)
+            ym = m.apply_mixed(x32)
+            ymr = om.apply(x32.astype(np.complex128)).real
+            assert ym.dtype == np.float64
+            assert np.linalg.norm(ym - ymr) <= 1e-13 * max(np.linalg.norm(ymr), 1e-300)
     else:
         y64 = m.apply(x.astype(np.complex64))
         assert np.linalg.norm(y64 - y_ref) <= 2e-6 * scale
+    if mode == "stream":
+        x64 = x.astype(np.complex64)
+        ym = m.apply_mixed(x64)
+        ymr = om.apply(x64.astype(np.complex128))
+        assert ym.dtype == np.complex128
+        assert np.linalg.norm(ym - ymr) <= 1e-13 * max(np.linalg.norm(ymr), 1e-300)
 
 
 def test_hv_pauli_ising(qa, oracle):
