@@ -26,9 +26,11 @@ J1-J2 L=28 sector, the 72 XXZ L=24 momentum/magnetization sectors solved in
 one lockstep batch beside one-by-one solves, and exp(-iHt)ψ of 64 seeded
 states at t ∈ {0.1, 1, 10} on the XXZ L=24 up=12 k=0 sector.
 
-The roofline object is the FP32 H·v of the headline sector: algorithmic bytes
-N·(4+4) + nnz·(4+4) per launch over the kernel time measured with HIP events
-on the stream the kernel runs on.
+The roofline object is the H·v of the FP32 schedule at the headline sector
+(FP32 input, FP64 accumulation and output): algorithmic bytes
+N·(4+8) + nnz·(4+4) per launch over the kernel time measured with HIP events
+on the stream the kernel runs on. The pure FP32 H·v (N·(4+4) + nnz·(4+4))
+is reported beside it.
 """
 
 import argparse
@@ -44,11 +46,13 @@ import numpy as np  # noqa: E402
 
 METRIC = "GS solves/s @ resid<1e-8, warm-start+FP32 vs plain FP64; H·v HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the FP32 stream H·v at the headline sector (hx = 1,
-# hz = 0.25), from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
-# `bench.py --hv-profile` calibrated on a 256 MiB device copy (summary and
-# commands in profiles/hv_stream_ising_L26.md)
-HV_FP32_TRAFFIC_BYTES = 265954527
+# HBM bytes per launch of the stream H·v at the headline sector (hx = 1,
+# hz = 0.25): FP32 input with FP64 accumulation and output (the FP32
+# schedule's product, the roofline kernel) and pure FP32, from the rocprofv3
+# FETCH_SIZE / WRITE_SIZE runs of `bench.py --hv-profile` calibrated on a
+# 256 MiB device copy (summary and commands in profiles/hv_stream_ising_L26.md)
+HV_MIXED_TRAFFIC_BYTES = 275021548
+HV_FP32_TRAFFIC_BYTES = 267536152
 L_SCAN = 26
 BATCH = 16
 # the headline start: the lowest eigenvector of the family projected on the 32
@@ -361,7 +365,7 @@ def main():
                                                   "plain_fp64_incl_build": amort_plain,
                                                   "speedup": amort_warm / amort_plain},
         },
-        "hv_c5_fp32": hv32,
+        "hv_c5_fp32": dict(hv32, traffic=HV_FP32_TRAFFIC_BYTES if hv32["mode"] == "stream" else None),
         "hv_c5_fp64": hv64,
         "hv_c5_fp32_in_fp64_out": hvmix,
     }
@@ -418,9 +422,11 @@ def main():
                                "extras.scan.network_start_fp32) + FP32-matvec schedule; ground state to residual "
                                "<= 1e-8*max(|E|,1); 16 points per step",
                    "global_batch": BATCH},
-        "roofline": {"bound": "hbm", "achieved": hv32["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hv32["frac"], "traffic": HV_FP32_TRAFFIC_BYTES if hv32["mode"] == "stream" else None,
-                     "kernel": f"FP32 H·v, L=26 Ising sector ({hv32['mode']} mode)"},
+        "roofline": {"bound": "hbm", "achieved": hvmix["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": hvmix["frac"], "traffic": HV_MIXED_TRAFFIC_BYTES if hvmix["mode"] == "stream" else None,
+                     "kernel": f"H·v of the FP32 schedule (FP32 input gathered, FP64 accumulation and output), "
+                               f"L=26 Ising sector ({hvmix['mode']} mode); algorithmic bytes N·(4+8) + nnz·(4+4) "
+                               f"= {hvmix['bytes_per_launch']} per launch; pure FP32 H·v in extras.hv_c5_fp32"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(model.dim),
         "extras": extras,
     }
