@@ -105,7 +105,9 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json
  * FP64 matvecs. No counterpart in the reference's ABI: its callers loop over
  * eigsh per sector (the same results, within the solver tolerance); the
  * JSON command {"operation":"eigsh_models","handles":[...],"solver":{...}}
- * is the same call on registered handles. */
+ * is the same call on registered handles. Batches are sized to the free
+ * device memory (the solver key "batch_rows" caps the concatenated rows of
+ * one batch; "batches" in the answer counts them). */
 char* qmbed_hip_eigsh_batch_json(qmbed_hip_model* const* models, int count, const char* solver_json);
 
 /* Krylov evolution of device-resident states: `states` holds ncols complex128

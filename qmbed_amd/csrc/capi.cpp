@@ -224,6 +224,11 @@ SolverReq parse_solver(const Value& s) {
     else bad("matvec_precision must be \"fp64\" or \"fp32\"");
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
+  if (auto* v = s.get("batch_rows"); v && !v->is_null()) {
+    const long c = as_long(*v, "batch_rows");
+    if (c < 0) bad("batch_rows must not be negative");
+    r.ctl.batch_rows = c;
+  }
   if (auto* v = s.get("check_every"); v && !v->is_null()) {
     const long c = as_long(*v, "check_every");
     if (c < 1 || c > 64) bad("check_every must lie in [1, 64]");
@@ -445,6 +450,7 @@ std::string solve_models(const std::vector<std::shared_ptr<Model>>& models, cons
   o += "],\"batch\":{\"sectors\":" + std::to_string(st.sectors) + ",\"batched\":" + std::to_string(st.batched) +
        ",\"fallback\":" + std::to_string(st.fallback) + ",\"steps\":" + std::to_string(st.steps) +
        ",\"ritz_rounds\":" + std::to_string(st.ritz_rounds) + ",\"window\":" + std::to_string(st.window) +
+       ",\"batches\":" + std::to_string(st.batches) +
        ",\"seconds\":";
   json::write_number(o, st.seconds);
   o += "}}";

@@ -391,6 +391,9 @@ struct EigshControls {
   // first leaves ‖w‖ <= η‖w₀‖ (the reference's constant is 1/√2; η = 1 takes
   // the second-pass path on nearly every step, which exercises it)
   double dgks_eta = 0.7071067811865476;
+  // sector batches: at most this many concatenated (tile-padded) rows per
+  // lockstep batch; 0 sizes the batches to the free device memory
+  long batch_rows = 0;
 };
 
 struct Eigensystem {
@@ -423,6 +426,7 @@ struct SectorBatchStats {
   long steps = 0;       // lockstep steps launched
   int ritz_rounds = 0;  // Ritz tests (each one synchronization)
   int window = 0;       // common Krylov window
+  int batches = 0;      // consecutive lockstep batches (device memory)
   double seconds = 0;   // wall time of the call
 };
 std::vector<Eigensystem> eigsh_sectors(const std::vector<Operator*>& ops, const EigshOptions& opts,

@@ -1012,21 +1012,49 @@ std::vector<Eigensystem> eigsh_sectors(const std::vector<Operator*>& ops, const 
   std::vector<Eigensystem> out(ops.size());
   SectorBatchStats st;
   st.sectors = int(ops.size());
+  // Batches sized to the device: the Krylov basis of a batch (window + 2 + k
+  // concatenated columns of 16 bytes per padded row, plus its partials) must
+  // fit in 80% of the free memory; larger sets run as consecutive batches of
+  // whole sectors, in input order.
+  std::vector<std::vector<int>> chunks;
   if (!batched.empty()) {
+    size_t free_b = 0, total_b = 0;
+    QMBED_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_row = (size_t(window) + 2 + size_t(o.eigenpairs)) * 16 + (size_t(window) + 1) * 16 / kTile;
+    size_t budget = free_b / 10 * 8;
+    if (ctl.batch_rows > 0) budget = std::min(budget, size_t(ctl.batch_rows) * per_row);
+    std::vector<int> cur;
+    size_t rows = 0;
+    for (int i : batched) {
+      const size_t padded = (ops[size_t(i)]->dim() + kTile - 1) / kTile * kTile;
+      if (!cur.empty() && (rows + padded) * per_row > budget) {
+        chunks.push_back(cur);
+        cur.clear();
+        rows = 0;
+      }
+      if (padded * per_row > budget)
+        throw unsupported_backend("sector batch: the Krylov window of one sector does not fit in device memory");
+      cur.push_back(i);
+      rows += padded;
+    }
+    if (!cur.empty()) chunks.push_back(cur);
+  }
+  for (const auto& chunk : chunks) {
     std::vector<Operator*> bops;
-    for (int i : batched) bops.push_back(ops[size_t(i)]);
+    for (int i : chunk) bops.push_back(ops[size_t(i)]);
     Batch b(bops, o, ctl, int(window), s);
     b.run(want_vectors);
-    st.steps = b.steps();
-    st.ritz_rounds = b.ritz_rounds();
+    st.steps += b.steps();
+    st.ritz_rounds += b.ritz_rounds();
     st.window = int(window);
-    for (size_t q = 0; q < batched.size(); ++q) {
+    ++st.batches;
+    for (size_t q = 0; q < chunk.size(); ++q) {
       const auto& ss = b.state(int(q));
       if (ss.accepted) {
-        out[size_t(batched[q])] = ss.es;
+        out[size_t(chunk[q])] = ss.es;
         ++st.batched;
       } else {
-        single.push_back(batched[q]);
+        single.push_back(chunk[q]);
         ++st.fallback;
       }
     }

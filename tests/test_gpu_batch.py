@@ -111,6 +111,29 @@ def test_batch_mixed_routes(qa, oracle):
         assert abs(r["eigenvalues"][0] - e0) <= 1e-10 * abs(e0)
 
 
+def test_batch_chunks_and_targets(qa, oracle):
+    # consecutive batches of whole sectors give the same results as one batch
+    L = 16
+    spec, keys, models = _sector_models(qa, L, (8,))
+    one = qa.eigsh_batch(models, 2)
+    many = qa.eigsh_batch(models, 2, batch_rows=3 * 2048)
+    assert one["batch"]["batches"] == 1 and many["batch"]["batches"] == 6
+    for a, b in zip(one["results"], many["results"]):
+        assert a["eigenvalues"] == b["eigenvalues"]
+    # other extreme targets
+    for target in ("largest_algebraic", "both_ends", "largest_magnitude"):
+        out = qa.eigsh_batch(models[:4], 2, target=target)
+        for (up, k), r in zip(keys[:4], out["results"]):
+            ev = np.linalg.eigvalsh(oracle.Model(L, spec, up=up, momentum=k).dense())
+            if target == "largest_algebraic":
+                ref = sorted(ev[-2:], reverse=True)
+            elif target == "both_ends":
+                ref = [ev[0], ev[-1]]
+            else:
+                ref = sorted(ev, key=lambda x: -abs(x))[:2]
+            assert np.allclose(sorted(r["eigenvalues"]), sorted(ref), rtol=1e-10, atol=1e-10), (target, k)
+
+
 def test_batch_errors(qa):
     spec = cases.xxz(16, 1.0)
     a = qa.Model({"kind": "spin", "sites": 16, "up": 8, "momentum": 0}, spec)
