@@ -330,6 +330,14 @@ class Model:
             bo.free()
         return out, res
 
+    def evolve_device(self, states_ptr, ncols, out_ptr, times, **options):
+        """Device-resident evolution: `states_ptr` holds ncols complex128 columns
+        ([column][row]), `out_ptr` receives [time][column][row]; returns the
+        diagnostics."""
+        req = {"times": list(map(float, times)), **options}
+        return _envelope(library().qmbed_hip_evolve_device(self._h, states_ptr, int(ncols), out_ptr,
+                                                           json.dumps(req).encode()))
+
     def set_parameters(self, **values):
         """Evaluate a family at the named coefficients (complex allowed)."""
         req = {k: [complex(v).real, complex(v).imag] for k, v in values.items()}

@@ -14,6 +14,7 @@
 // interleaved), Krylov basis V is [j][col][row].
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <thread>
 
 #include "internal.hpp"
@@ -655,12 +656,19 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       const size_t np = plans.size();
       const size_t nth = std::min<size_t>(np, size_t(control_threads()));
       std::vector<std::thread> pool;
+      std::vector<std::exception_ptr> err(nth);
       auto work = [&](size_t tid) {
-        for (size_t i = tid; i < np; i += nth) plan_column(i);
+        try {
+          for (size_t i = tid; i < np; i += nth) plan_column(i);
+        } catch (...) {
+          err[tid] = std::current_exception();
+        }
       };
       for (size_t tid = 1; tid < nth; ++tid) pool.emplace_back(work, tid);
       if (nth > 0) work(0);
       for (auto& th : pool) th.join();
+      for (auto& e : err)
+        if (e) std::rethrow_exception(e);
     }
     int nacts = 0, ncoef = 0;
     std::vector<Act> copies;

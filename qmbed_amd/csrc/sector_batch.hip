@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <exception>
 #include <numeric>
 #include <thread>
 
@@ -497,12 +498,19 @@ void parallel_for(int count, F f) {
     return;
   }
   std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> err(static_cast<size_t>(nth));
   auto work = [&](int tid) {
-    for (int i = tid; i < count; i += nth) f(i);
+    try {
+      for (int i = tid; i < count; i += nth) f(i);
+    } catch (...) {
+      err[size_t(tid)] = std::current_exception();
+    }
   };
   for (int t = 1; t < nth; ++t) pool.emplace_back(work, t);
   work(0);
   for (auto& th : pool) th.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
 }
 
 // Host side of one batch.
