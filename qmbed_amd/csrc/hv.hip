@@ -908,9 +908,13 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
 // S: accumulation and output type; SX: input type (float input with double
 // accumulation and output is the mixed-precision product of the Lanczos
 // FP32 schedule)
-template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S>
+// IL: interleaved columns (element (row, v) at row · ld + v, the columns of
+// a row contiguous, so the NV inputs a target names share cache lines);
+// otherwise column-major (v · ld + row).
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S, bool IL = false>
 __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restrict__ x, S* __restrict__ y,
                                             uint32_t blk) {
+  auto at = [&](int v, uint32_t i) -> size_t { return IL ? size_t(i) * a.ld + v : size_t(v) * a.ld + i; };
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
   double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
@@ -964,10 +968,10 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       if (CMODE == 0) {
-        acc[v].re = dsum.re * S(x[v * a.ld + own]);
+        acc[v].re = dsum.re * S(x[at(v, own)]);
         acc[v].im = 0;
       } else {
-        const S xr = S(x[2 * (v * a.ld + own)]), xi = S(x[2 * (v * a.ld + own) + 1]);
+        const S xr = S(x[2 * at(v, own)]), xi = S(x[2 * at(v, own) + 1]);
         acc[v].re = dsum.re * xr - dsum.im * xi;
         acc[v].im = dsum.re * xi + dsum.im * xr;
       }
@@ -994,11 +998,11 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
           if (CMODE == 0) {
-            xr[b][v] = S(x[v * a.ld + idx]);
+            xr[b][v] = S(x[at(v, idx)]);
             xi[b][v] = 0;
           } else {
-            xr[b][v] = S(x[2 * (v * a.ld + idx)]);
-            xi[b][v] = S(x[2 * (v * a.ld + idx) + 1]);
+            xr[b][v] = S(x[2 * at(v, idx)]);
+            xi[b][v] = S(x[2 * at(v, idx) + 1]);
           }
         }
       }
@@ -1043,10 +1047,10 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       if (CMODE == 0) {
-        y[v * a.ld + row] = acc[v].re;
+        y[at(v, row)] = acc[v].re;
       } else {
-        y[2 * (v * a.ld + row)] = acc[v].re;
-        y[2 * (v * a.ld + row) + 1] = acc[v].im;
+        y[2 * at(v, row)] = acc[v].re;
+        y[2 * at(v, row) + 1] = acc[v].im;
       }
     }
     return;
@@ -1067,11 +1071,11 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         if (CMODE == 0) {
-          xr[b][v] = S(x[v * a.ld + idx]);
+          xr[b][v] = S(x[at(v, idx)]);
           xi[b][v] = 0;
         } else {
-          xr[b][v] = S(x[2 * (v * a.ld + idx)]);
-          xi[b][v] = S(x[2 * (v * a.ld + idx) + 1]);
+          xr[b][v] = S(x[2 * at(v, idx)]);
+          xi[b][v] = S(x[2 * at(v, idx) + 1]);
         }
       }
     }
@@ -1118,18 +1122,18 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     if (CMODE == 0) {
-      y[v * a.ld + row] = acc[v].re;
+      y[at(v, row)] = acc[v].re;
     } else {
-      y[2 * (v * a.ld + row)] = acc[v].re;
-      y[2 * (v * a.ld + row) + 1] = acc[v].im;
+      y[2 * at(v, row)] = acc[v].re;
+      y[2 * at(v, row) + 1] = acc[v].im;
     }
   }
 }
 
-template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S>
+template <typename S, int CMODE, int SYM, bool SHIFT, int NV, typename SX = S, bool IL = false>
 __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const SX* __restrict__ x, S* __restrict__ y) {
   if (a.gate && *a.gate) return;
-  stream_rows<S, CMODE, SYM, SHIFT, NV, SX>(a, x, y, blockIdx.x);
+  stream_rows<S, CMODE, SYM, SHIFT, NV, SX, IL>(a, x, y, blockIdx.x);
 }
 
 // One sector of a sector-batched action: its tables and where its rows sit
@@ -1214,6 +1218,9 @@ struct Operator::DeviceTables {
   // compiled index stream (autotuned against both)
   bool use_stream = false;
   DeviceArray<uint32_t> stream, slice_ptr;
+  // the stream is kept for the interleaved multi-column action even when
+  // the single-vector autotuner prefers another mode
+  bool use_stream_interleaved = false;
   // compact stream layout (field widths; see HvArgs)
   bool compact = false;
   int c_b = 0, c_ob = 0, c_cb = 0, c_sb = 0, c_off_base = 0;
@@ -1553,14 +1560,14 @@ void launch_hv_mode(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipS
   QMBED_HIP(hipGetLastError());
 }
 
-template <typename S, int CMODE, int NV, typename SX = S>
+template <typename S, int CMODE, int NV, typename SX = S, bool IL = false>
 void launch_hv_stream(const HvArgs& a, int sym, bool shift, const SX* x, S* y, hipStream_t s) {
   const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
   dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t G = size_t(a.n_diag + a.n_off);
   const size_t sm = sizeof(double) * ((CMODE == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G;
 #define QMBED_HVS_CASE(SYMV, SH) \
-  hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV, SX>), grid, block, sm, s, a, x, y);
+  hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV, SX, IL>), grid, block, sm, s, a, x, y);
   switch (sym) {
     case 0: QMBED_HVS_CASE(0, false); break;
     case 1:
@@ -1908,6 +1915,52 @@ bool Operator::apply_mixed(const float* x, double* y, bool complex_vectors, hipS
   return true;
 }
 
+bool Operator::apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nvec, hipStream_t s) const {
+  if (!tables_->use_stream_interleaved || !tables_->hermitian) return false;
+  HvArgs a = tables_->args(*basis_);
+  a.stream = tables_->stream.data();
+  a.slice_ptr = tables_->slice_ptr.data();
+  a.img = nullptr;
+  a.ld = ld;
+  if (a.n == 0) return true;
+  const int sym = sym_code(*basis_);
+  const bool sh = basis_->geometry().kmom > 0;
+  const double* xd = reinterpret_cast<const double*>(x);
+  double* yd = reinterpret_cast<double*>(y);
+  int done = 0;
+  while (done < nvec) {
+    const int left = nvec - done;
+    const double* xs = xd + 2 * size_t(done);
+    double* ys = yd + 2 * size_t(done);
+    // eight columns of complex FP64 fill one 128-byte line per gathered
+    // target row; the remainder goes by four and one
+    if (left >= 8) {
+      if (real_) launch_hv_stream<double, 1, 8, double, true>(a, sym, sh, xs, ys, s);
+      else launch_hv_stream<double, 2, 8, double, true>(a, sym, sh, xs, ys, s);
+      done += 8;
+    } else if (left >= 4) {
+      if (real_) launch_hv_stream<double, 1, 4, double, true>(a, sym, sh, xs, ys, s);
+      else launch_hv_stream<double, 2, 4, double, true>(a, sym, sh, xs, ys, s);
+      done += 4;
+    } else {
+      if (real_) launch_hv_stream<double, 1, 1, double, true>(a, sym, sh, xs, ys, s);
+      else launch_hv_stream<double, 2, 1, double, true>(a, sym, sh, xs, ys, s);
+      done += 1;
+    }
+  }
+  return true;
+}
+
+bool prepare_interleaved_action(const Operator& op, hipStream_t s) {
+  auto& t = *op.tables_;
+  if (!t.hermitian || op.dim() == 0) return false;
+  if (!t.use_stream_interleaved) {
+    if (t.stream.size() == 0 && !build_stream(const_cast<Operator&>(op), s)) return false;
+    t.use_stream_interleaved = true;
+  }
+  return true;
+}
+
 void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const {
   if (!tables_->hermitian) {
     for (int v = 0; v < nvec; ++v) scatter_apply(*this, x + size_t(v) * dim(), y + size_t(v) * dim(), s);
@@ -1984,7 +2037,7 @@ SectorBatchAction::SectorBatchAction(const std::vector<Operator*>& ops, const st
     if (i == 0) sym_ = sym;
     if (sym != sym_) throw unsupported_backend("sector batch: every sector must use the same symmetry kind");
     auto& t = *op.tables_;
-    if (!t.use_stream && !build_stream(op, s))
+    if (t.stream.size() == 0 && !build_stream(op, s))
       throw unsupported_backend("sector batch: a sector does not fit the compiled index stream");
     const uint32_t nb = uint32_t((b.len() + kRows - 1) / kRows);
     if (nb > 0xFFFFu) throw unsupported_backend("sector batch: a sector exceeds 65535 blocks");

@@ -253,6 +253,122 @@ namespace {
 // the stop of each column (window end or breakdown β <= 1e-14) are decided on
 // the device and kept in a per-column history that the host reads once, at
 // the end of the projection.
+//
+// The columns are interleaved: element (row r, column c) of a block of nc
+// columns at r·nc + c, so the H·v's gather of a target row fetches the inputs
+// of several columns in one cache line. Element-wise kernels map a block of
+// 256 threads onto 4 rows × 64 columns (consecutive threads, consecutive
+// columns of one row: coalesced); per-column sums combine the 4 row lanes in
+// LDS and leave one partial per row block, partial[c · nb + block].
+
+constexpr int kCols = 64;              // column lanes of an element-wise block
+constexpr int kRowLanes = kT / kCols;  // row lanes
+
+int il_row_blocks(size_t n) {
+  return int(std::min<size_t>(1024, std::max<size_t>(1, (n + size_t(kRowLanes) * 16 - 1) / (size_t(kRowLanes) * 16))));
+}
+dim3 il_grid(size_t n, int cols) { return dim3(il_row_blocks(n), (cols + kCols - 1) / kCols); }
+
+// per-column sum of the 4 row lanes into partial[c · nb + block]
+__device__ __forceinline__ void il_partial(double v, int nc, double* sh, double* partial, int stride, int off) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x < kCols) {
+    const int c = blockIdx.y * kCols + threadIdx.x;
+    double t = sh[threadIdx.x];
+    for (int q = 1; q < kRowLanes; ++q) t += sh[q * kCols + threadIdx.x];
+    if (c < nc) partial[(size_t(c) * gridDim.x + blockIdx.x) * stride + off] = t;
+  }
+  __syncthreads();
+}
+
+// Σ_r |X[r, c]|² per column (active == nullptr: every column)
+__global__ __launch_bounds__(kT) void il_norm(const double2* __restrict__ X, size_t n, int nc, const int* active,
+                                              double* partial) {
+  __shared__ double sh[kT];
+  const int c = blockIdx.y * kCols + (threadIdx.x % kCols);
+  double acc = 0;
+  if (c < nc && (!active || active[c]))
+    for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+      const double2 v = X[r * nc + c];
+      acc += v.x * v.x + v.y * v.y;
+    }
+  il_partial(acc, nc, sh, partial, 1, 0);
+}
+
+// Σ_r conj(X[r, c]) Y[r, c] per column: (re, im) pairs in the partials
+__global__ __launch_bounds__(kT) void il_zdot(const double2* __restrict__ X, const double2* __restrict__ Y, size_t n,
+                                              int nc, const int* active, double* partial) {
+  __shared__ double sh[kT];
+  const int c = blockIdx.y * kCols + (threadIdx.x % kCols);
+  double sr = 0, si = 0;
+  if (c < nc && (!active || active[c]))
+    for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+      const double2 a = X[r * nc + c], b = Y[r * nc + c];
+      sr += a.x * b.x + a.y * b.y;
+      si += a.x * b.y - a.y * b.x;
+    }
+  il_partial(sr, nc, sh, partial, 2, 0);
+  il_partial(si, nc, sh, partial, 2, 1);
+}
+
+// out[r, c] = scale[c] · X[r, c] for the active columns (active == nullptr:
+// every column)
+__global__ __launch_bounds__(kT) void il_scale(const double2* X, const double* scale, const int* active, size_t n,
+                                               int nc, double2* out) {
+  const int c = blockIdx.y * kCols + (threadIdx.x % kCols);
+  if (c >= nc || (active && !active[c])) return;
+  const double f = scale[c];
+  for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+    const double2 v = X[r * nc + c];
+    out[r * nc + c] = make_double2(f * v.x, f * v.y);
+  }
+}
+
+// W -= α_j V_j + β_{j−1} V_{j−1} with the coefficients from the history, and
+// the partials of ‖W‖² of the result
+__global__ __launch_bounds__(kT) void il_three(double2* W, const double2* __restrict__ V, const double2* __restrict__ U,
+                                               const double* are, const double* bh, const int* running, int m, int j,
+                                               size_t n, int nc, double* partial) {
+  __shared__ double sh[kT];
+  const int c = blockIdx.y * kCols + (threadIdx.x % kCols);
+  double acc = 0;
+  if (c < nc && running[c]) {
+    const double a = are[size_t(c) * m + j];
+    const double b = j > 0 ? bh[size_t(c) * m + j - 1] : 0.0;
+    for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+      const size_t i = r * nc + c;
+      double2 x = W[i];
+      const double2 y = V[i];
+      x.x -= a * y.x;
+      x.y -= a * y.y;
+      if (U) {
+        const double2 z = U[i];
+        x.x -= b * z.x;
+        x.y -= b * z.y;
+      }
+      W[i] = x;
+      acc += x.x * x.x + x.y * x.y;
+    }
+  }
+  il_partial(acc, nc, sh, partial, 1, 0);
+}
+
+// [col][row] <-> interleaved (rows r of columns c0.., count columns)
+__global__ __launch_bounds__(kT) void il_from_cols(const double2* __restrict__ cols, size_t n, int nc, int c0,
+                                                   int count, double2* __restrict__ il) {
+  const int v = blockIdx.y * kCols + (threadIdx.x % kCols);
+  if (v >= count) return;
+  for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes)
+    il[r * nc + c0 + v] = cols[size_t(v) * n + r];
+}
+__global__ __launch_bounds__(kT) void il_to_cols(const double2* __restrict__ il, size_t n, int nc, int c0, int count,
+                                                 double2* __restrict__ cols) {
+  const int v = blockIdx.y * kCols + (threadIdx.x % kCols);
+  if (v >= count) return;
+  for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes)
+    cols[size_t(v) * n + r] = il[r * nc + c0 + v];
+}
 
 // Per-column reductions of the row-block partials run one wave per column:
 // lane-strided sums, then a shuffle tree (a fixed order).
@@ -263,7 +379,7 @@ __device__ __forceinline__ double wave_total(const double* p, int nb, int stride
   return s;
 }
 
-// ‖cur_c‖ from the kb_norm partials: init_norm, running = act && norm > eps,
+// ‖cur_c‖ from the norm partials: init_norm, running = act && norm > eps,
 // inv = 1/norm (0 when not running), size = 0
 __global__ void kb_proj_init(const double* partial, int nb, int nc, const int* act, double* init_norm, int* running,
                              double* inv, int* size) {
@@ -291,29 +407,6 @@ __global__ void kb_alpha_hist(const double* partial, int nb, int nc, const int* 
   aim[size_t(c) * m + j] = si;
 }
 
-// W -= α_j V_j + β_{j−1} V_{j−1} with the coefficients from the history
-__global__ void kb_three_hist(double2* W, const double2* __restrict__ V, const double2* __restrict__ U,
-                              const double* are, const double* bh, const int* running, int m, int j, size_t n) {
-  const int col = blockIdx.y;
-  if (!running[col]) return;
-  const double a = are[size_t(col) * m + j];
-  const double b = j > 0 ? bh[size_t(col) * m + j - 1] : 0.0;
-  double2* w = W + size_t(col) * n;
-  const double2* v = V + size_t(col) * n;
-  const double2* u = U ? U + size_t(col) * n : nullptr;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    double2 x = w[i], y = v[i];
-    x.x -= a * y.x;
-    x.y -= a * y.y;
-    if (u) {
-      double2 z = u[i];
-      x.x -= b * z.x;
-      x.y -= b * z.y;
-    }
-    w[i] = x;
-  }
-}
-
 // β_j = ‖w‖ into the history; the column stops at the window's end or at a
 // breakdown (src/solve/mod.rs:2909-3054), with size = j + 1
 __global__ void kb_beta_hist(const double* partial, int nb, int nc, int* running, int m, int j, double* bh,
@@ -333,68 +426,81 @@ __global__ void kb_beta_hist(const double* partial, int nb, int nc, int* running
   }
 }
 
-// one output or state update: dst = Σ_j coef[j] V[j] over `size` Krylov
-// blocks of column `col` (V: that column in block 0), renormalized to the
-// column's initial norm; copy actions (coef < 0) copy V to dst
+// One output or state update of column `col`: dst[r, col] = Σ_j coef[j]
+// V[j][r, col] over `size` Krylov blocks, renormalized to the column's
+// initial norm (a separate pass once the norms are known). Copy actions
+// (size < 0) copy src[r, col] instead.
 struct Act {
-  const double2* V;
-  double2* dst;
+  double2* dst;        // interleaved block of nc columns
+  const double2* src;  // copy actions: interleaved block to copy from
+  int col;
   int size;
   int coef;  // offset (complex entries) into the coefficient array
-  int col;
   int pad;
 };
 
-__global__ void kb_combine_acts(const Act* __restrict__ acts, size_t ldj, const double* __restrict__ coefs, size_t n) {
-  __shared__ double shc[2 * 128];
-  const Act a = acts[blockIdx.y];
-  for (int j = threadIdx.x; j < 2 * a.size; j += kT) shc[j] = coefs[2 * size_t(a.coef) + j];
-  __syncthreads();
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    double ar = 0, ai = 0;
-    for (int j = 0; j < a.size; ++j) {
-      const double2 v = a.V[size_t(j) * ldj + i];
-      ar += v.x * shc[2 * j] - v.y * shc[2 * j + 1];
-      ai += v.x * shc[2 * j + 1] + v.y * shc[2 * j];
+// Thread lanes run over actions (consecutive actions: consecutive columns of
+// one row, in practice), with the per-action partials of ‖dst‖²
+__global__ __launch_bounds__(kT) void il_combine_acts(const Act* __restrict__ acts, int nacts,
+                                                      const double2* __restrict__ V, size_t ldj, int nc,
+                                                      const double* __restrict__ coefs, size_t n, double* partial) {
+  __shared__ double sh[kT];
+  const int ai = blockIdx.y * kCols + (threadIdx.x % kCols);
+  double acc = 0;
+  if (ai < nacts) {
+    const Act a = acts[ai];
+    const double* cf = coefs + 2 * size_t(a.coef);
+    for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+      const size_t i = r * nc + a.col;
+      double ar = 0, aim = 0;
+      for (int j = 0; j < a.size; ++j) {
+        const double2 v = V[size_t(j) * ldj + i];
+        ar += v.x * cf[2 * j] - v.y * cf[2 * j + 1];
+        aim += v.x * cf[2 * j + 1] + v.y * cf[2 * j];
+      }
+      a.dst[i] = make_double2(ar, aim);
+      acc += ar * ar + aim * aim;
     }
-    a.dst[i] = make_double2(ar, ai);
   }
+  il_partial(acc, nacts, sh, partial, 1, 0);
 }
 
-__global__ void kb_norm_acts(const Act* __restrict__ acts, size_t n, double* partial) {
-  __shared__ double sh[4];
-  const Act a = acts[blockIdx.y];
-  double s = 0;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
-    const double2 v = a.dst[i];
-    s += v.x * v.x + v.y * v.y;
-  }
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
-}
-
-// dst *= init_norm / ‖dst‖ when that norm is finite and above eps
-// (src/solve/mod.rs:3104-3112); every block re-adds the partials in order
-__global__ void kb_renorm_acts(const Act* __restrict__ acts, const double* partial, int nb,
-                               const double* init_norm, size_t n) {
-  const Act a = acts[blockIdx.y];
-  double t = 0;
-  for (int b = 0; b < nb; ++b) t += partial[size_t(blockIdx.y) * nb + b];
+// per-action factor init_norm[col] / ‖dst‖ (1 when that norm is not finite
+// and above eps: src/solve/mod.rs:3104-3112), one wave per action
+__global__ void kb_act_scale(const Act* __restrict__ acts, int nacts, const double* partial, int nb,
+                             const double* init_norm, double* scale) {
+  const int ai = blockIdx.x * (kT / 64) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (ai >= nacts) return;
+  const double t = wave_total(partial + size_t(ai) * nb, nb, 1, lane);
+  if (lane) return;
   const double on = sqrt(t);
-  if (!(on > F64_EPS && isfinite(on))) return;
-  const double sc = init_norm[a.col] / on;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+  scale[ai] = (on > F64_EPS && isfinite(on)) ? init_norm[acts[ai].col] / on : 1.0;
+}
+
+__global__ __launch_bounds__(kT) void il_scale_acts(const Act* __restrict__ acts, int nacts, const double* scale,
+                                                    size_t n, int nc) {
+  const int ai = blockIdx.y * kCols + (threadIdx.x % kCols);
+  if (ai >= nacts) return;
+  const Act a = acts[ai];
+  const double f = scale[ai];
+  if (f == 1.0) return;
+  for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes) {
+    const size_t i = r * nc + a.col;
     const double2 v = a.dst[i];
-    a.dst[i] = make_double2(sc * v.x, sc * v.y);
+    a.dst[i] = make_double2(f * v.x, f * v.y);
   }
 }
 
-__global__ void kb_copy_acts(const Act* __restrict__ acts, size_t n) {
-  const Act a = acts[blockIdx.y];
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) a.dst[i] = a.V[i];
+__global__ __launch_bounds__(kT) void il_copy_acts(const Act* __restrict__ acts, int nacts, size_t n, int nc) {
+  const int ai = blockIdx.y * kCols + (threadIdx.x % kCols);
+  if (ai >= nacts) return;
+  const Act a = acts[ai];
+  for (size_t r = blockIdx.x * size_t(kRowLanes) + threadIdx.x / kCols; r < n; r += size_t(gridDim.x) * kRowLanes)
+    a.dst[r * nc + a.col] = a.src[r * nc + a.col];
 }
 
-// per-column Σ |Im x| partials (a column with none is a real state)
+// per-column Σ |Im x| partials of [col][row] input (a column with none is a
+// real state)
 __global__ void kb_imag_abs(const double2* __restrict__ X, size_t n, double* partial) {
   __shared__ double sh[4];
   const double2* x = X + size_t(blockIdx.y) * n;
@@ -443,52 +549,74 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
   const int m = int(std::max<long>(std::min<long>(opts.krylov_dimension, long(n)), 1));
   if (m > 128) throw unsupported_backend("Krylov dimension above 128 exceeds the batched projection kernels");
   const double local_tol = opts.tolerance / double(std::max(T, 1));
-  const int nb = row_blocks(n);
-  const dim3 grid(nb, nc);
+  const int nb = il_row_blocks(n);
+  const dim3 grid = il_grid(n, nc);
+  // the interleaved H·v (compiled stream); otherwise each column block is
+  // moved to column-major and back around the column-major multi-vector H·v
+  const bool il_action = prepare_interleaved_action(op, s);
+  const size_t blk = n * size_t(nc);  // complex elements per interleaved block
 
-  DeviceArray<double> cur(2 * n * nc), W(2 * n * nc), V(2 * n * size_t(nc) * m);
-  DeviceArray<double> partial(size_t(nc) * nb * std::max(2, T + 1) + 64), are(size_t(nc) * m), aim(size_t(nc) * m),
-      bh(size_t(nc) * m), init_norm(nc), inv(nc), coefs(2 * size_t(nc) * (T + 1) * m + 2);
+  DeviceArray<double> cur(2 * blk), W(2 * blk), V(2 * blk * size_t(m)), outs(2 * blk * size_t(T));
+  DeviceArray<double> tmp_x, tmp_y;
+  if (!il_action) {
+    tmp_x.resize(2 * blk);
+    tmp_y.resize(2 * blk);
+  }
+  const int max_acts = nc * (T + 1);
+  DeviceArray<double> partial(size_t(std::max(max_acts, 2 * nc)) * size_t(std::max(nb, 256)) + 64),
+      are(size_t(nc) * m), aim(size_t(nc) * m), bh(size_t(nc) * m), init_norm(nc), inv(nc), ascale(max_acts),
+      coefs(2 * size_t(nc) * (T + 1) * m + 2);
   DeviceArray<int> act_d(nc), running(nc), size_d(nc);
-  DeviceArray<uint8_t> acts_d(sizeof(Act) * size_t(nc) * (T + 1) + sizeof(Act));
+  DeviceArray<uint8_t> acts_d(sizeof(Act) * (size_t(max_acts) + size_t(nc)) + sizeof(Act));
   Pinned<double> hist;  // are | aim | bh | init_norm | size (as int pairs)
   hist.ensure(3 * size_t(nc) * m + 2 * size_t(nc) + 8);
   Pinned<int> act_h;
   act_h.ensure(nc);
   Pinned<Act> acts_h;
-  acts_h.ensure(size_t(nc) * (T + 1) + 1);
+  acts_h.ensure(size_t(max_acts) + size_t(nc) + 1);
+  Pinned<Act> emit_h;
+  emit_h.ensure(size_t(nc) * T + 1);
+  DeviceArray<uint8_t> emit_d(sizeof(Act) * (size_t(nc) * T + 1));
   Pinned<double> coef_h;
   coef_h.ensure(2 * size_t(nc) * (T + 1) * m + 2);
 
-  QMBED_HIP(hipMemcpyAsync(cur.data(), d_cols, 16 * n * nc, hipMemcpyDeviceToDevice, s));
+  double2* curp = reinterpret_cast<double2*>(cur.data());
+  double2* Wp = reinterpret_cast<double2*>(W.data());
+  double2* Vc = reinterpret_cast<double2*>(V.data());
+  auto out_blk = [&](int q) { return reinterpret_cast<double2*>(outs.data()) + blk * size_t(q); };
+
   // real initial states (for the real-projection counters and the α check)
   std::vector<int> real_state(nc, 1);
   {
-    hipLaunchKernelGGL(kb_imag_abs, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), n,
+    const int nbc = row_blocks(n);
+    hipLaunchKernelGGL(kb_imag_abs, dim3(nbc, nc), dim3(kT), 0, s, reinterpret_cast<const double2*>(d_cols), n,
                        partial.data());
-    std::vector<double> h(size_t(nc) * nb);
+    std::vector<double> h(size_t(nc) * nbc);
     QMBED_HIP(hipMemcpyAsync(h.data(), partial.data(), h.size() * 8, hipMemcpyDeviceToHost, s));
     QMBED_HIP(hipStreamSynchronize(s));
     for (int c = 0; c < nc; ++c)
-      for (int b = 0; b < nb; ++b)
-        if (h[size_t(c) * nb + b] != 0.0) real_state[c] = 0;
+      for (int b = 0; b < nbc; ++b)
+        if (h[size_t(c) * nbc + b] != 0.0) real_state[c] = 0;
   }
-  auto out_col = [&](int q, int c) { return reinterpret_cast<double2*>(d_out) + (size_t(q) * nc + c) * n; };
-  auto cur_col = [&](int c) { return reinterpret_cast<double2*>(cur.data()) + size_t(c) * n; };
+  hipLaunchKernelGGL(il_from_cols, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(d_cols), n, nc, 0, nc,
+                     curp);
   std::vector<double> t(nc, 0.0);
   std::vector<int> oi(nc, 0), substeps(nc, 0), done(nc, 0);
-  double2* Vc = reinterpret_cast<double2*>(V.data());
-  const size_t ldj = n * nc;  // complex elements per Krylov block
+  const size_t ldj = blk;  // complex elements per Krylov block
   for (;;) {
     // emit outputs whose time is reached; check budgets
-    int nact = 0;
+    int nact = 0, nemit = 0;
     for (int c = 0; c < nc; ++c) {
       act_h.p[c] = 0;
       if (done[c]) continue;
       while (oi[c] < T) {
         const double target = opts.times[oi[c]];
         if (std::fabs(target - t[c]) <= 16.0 * F64_EPS * std::max(std::fabs(target), 1.0)) {
-          QMBED_HIP(hipMemcpyAsync(out_col(oi[c], c), cur_col(c), n * 16, hipMemcpyDeviceToDevice, s));
+          Act e{};
+          e.dst = out_blk(oi[c]);
+          e.src = curp;
+          e.col = c;
+          emit_h.p[nemit++] = e;
           t[c] = target;
           ++oi[c];
         } else {
@@ -503,15 +631,19 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       act_h.p[c] = 1;
       ++nact;
     }
+    if (nemit > 0) {
+      QMBED_HIP(hipMemcpyAsync(emit_d.data(), emit_h.p, sizeof(Act) * size_t(nemit), hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(il_copy_acts, il_grid(n, nemit), dim3(kT), 0, s, reinterpret_cast<const Act*>(emit_d.data()),
+                         nemit, n, nc);
+      // the emission list is rewritten only after the next synchronization
+    }
     if (nact == 0) break;
     // ---- batched Lanczos projection (no reorthogonalization) ----
     QMBED_HIP(hipMemcpyAsync(act_d.data(), act_h.p, size_t(nc) * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), n,
-                       partial.data());
+    hipLaunchKernelGGL(il_norm, grid, dim3(kT), 0, s, curp, n, nc, nullptr, partial.data());
     hipLaunchKernelGGL(kb_proj_init, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, act_d.data(),
                        init_norm.data(), running.data(), inv.data(), size_d.data());
-    hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(cur.data()), inv.data(),
-                       nullptr, n, Vc);
+    hipLaunchKernelGGL(il_scale, grid, dim3(kT), 0, s, curp, inv.data(), nullptr, n, nc, Vc);
     // runs of consecutive active columns share one multi-column H·v
     std::vector<std::pair<int, int>> runs;
     for (int c = 0; c < nc;) {
@@ -526,22 +658,28 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
     }
     for (int j = 0; j < m; ++j) {
       double2* Vj = Vc + size_t(j) * ldj;
-      for (auto& r : runs)
-        op.apply_batch(reinterpret_cast<const cplx*>(Vj + size_t(r.first) * n),
-                       reinterpret_cast<cplx*>(W.data()) + size_t(r.first) * n, r.second, s);
-      hipLaunchKernelGGL(kb_zdot, grid, dim3(kT), 0, s, Vj, reinterpret_cast<const double2*>(W.data()), n,
-                         partial.data());
+      for (auto& r : runs) {
+        if (il_action) {
+          op.apply_batch_interleaved(reinterpret_cast<const cplx*>(Vj + r.first), reinterpret_cast<cplx*>(Wp + r.first),
+                                     size_t(nc), r.second, s);
+        } else {
+          const dim3 g2 = il_grid(n, r.second);
+          double2* tx = reinterpret_cast<double2*>(tmp_x.data());
+          double2* ty = reinterpret_cast<double2*>(tmp_y.data());
+          hipLaunchKernelGGL(il_to_cols, g2, dim3(kT), 0, s, Vj, n, nc, r.first, r.second, tx);
+          op.apply_batch(reinterpret_cast<const cplx*>(tx), reinterpret_cast<cplx*>(ty), r.second, s);
+          hipLaunchKernelGGL(il_from_cols, g2, dim3(kT), 0, s, ty, n, nc, r.first, r.second, Wp);
+        }
+      }
+      hipLaunchKernelGGL(il_zdot, grid, dim3(kT), 0, s, Vj, Wp, n, nc, running.data(), partial.data());
       hipLaunchKernelGGL(kb_alpha_hist, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, running.data(),
                          m, j, are.data(), aim.data());
-      hipLaunchKernelGGL(kb_three_hist, grid, dim3(kT), 0, s, reinterpret_cast<double2*>(W.data()), Vj,
-                         j > 0 ? Vj - ldj : nullptr, are.data(), bh.data(), running.data(), m, j, n);
-      hipLaunchKernelGGL(kb_norm<0>, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), n,
-                         partial.data());
+      hipLaunchKernelGGL(il_three, grid, dim3(kT), 0, s, Wp, Vj, j > 0 ? Vj - ldj : nullptr, are.data(), bh.data(),
+                         running.data(), m, j, n, nc, partial.data());
       hipLaunchKernelGGL(kb_beta_hist, dim3((nc + 3) / 4), dim3(kT), 0, s, partial.data(), nb, nc, running.data(),
                          m, j, bh.data(), size_d.data(), inv.data());
       if (j + 1 < m)
-        hipLaunchKernelGGL(kb_scale, grid, dim3(kT), 0, s, reinterpret_cast<const double2*>(W.data()), inv.data(),
-                           running.data(), n, Vj + ldj);
+        hipLaunchKernelGGL(il_scale, grid, dim3(kT), 0, s, Wp, inv.data(), running.data(), n, nc, Vj + ldj);
     }
     QMBED_HIP(hipGetLastError());
     double* hp = hist.p;
@@ -559,8 +697,8 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
     // any thread count).
     struct Plan {
       Projection p;
-      std::vector<std::pair<double2*, double>> outs;  // (destination, interval)
-      double2* copy_src = nullptr;                    // state <- last output
+      std::vector<std::pair<double2*, double>> outs;  // (destination block, interval)
+      const double2* copy_src = nullptr;              // state <- last output
       double t = 0;
       int oi = 0;
       bool stepped = false;
@@ -611,14 +749,14 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
           ++d.rejected_trial_intervals;
           break;
         }
-        pl.outs.push_back({out_col(q, c), iv});
+        pl.outs.push_back({out_blk(q), iv});
         ++accepted;
         last_time = opts.times[q];
       }
       pl.stepped = true;
       ++d.accepted_substeps;
       if (accepted) {
-        pl.copy_src = out_col(pl.oi + accepted - 1, c);
+        pl.copy_src = out_blk(pl.oi + accepted - 1);
         pl.t = last_time;
         pl.oi += accepted;
         return;
@@ -650,7 +788,7 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       }
       const double iv = dir * acc;
       pl.t += iv;
-      pl.outs.push_back({cur_col(c), iv});
+      pl.outs.push_back({curp, iv});
     };
     {
       const size_t np = plans.size();
@@ -685,11 +823,10 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       dg.maximum_estimated_error = std::max(dg.maximum_estimated_error, pl.dg.maximum_estimated_error);
       for (auto& o : pl.outs) {
         Act a{};
-        a.V = Vc + size_t(c) * n;
         a.dst = o.first;
+        a.col = c;
         a.size = pl.p.size;
         a.coef = ncoef;
-        a.col = c;
         if (pl.p.size > 0) {
           const auto coef = projected_coefficients(pl.p, o.second);
           for (int jj = 0; jj < pl.p.size; ++jj) {
@@ -702,8 +839,9 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       }
       if (pl.copy_src) {
         Act cp{};
-        cp.V = pl.copy_src;
-        cp.dst = cur_col(c);
+        cp.src = pl.copy_src;
+        cp.dst = curp;
+        cp.col = c;
         copies.push_back(cp);
       }
       t[c] = pl.t;
@@ -721,14 +859,21 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       if (ncoef > 0)
         QMBED_HIP(hipMemcpyAsync(coefs.data(), coef_h.p, 16 * size_t(ncoef), hipMemcpyHostToDevice, s));
       const Act* ad = reinterpret_cast<const Act*>(acts_d.data());
-      hipLaunchKernelGGL(kb_combine_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, ldj, coefs.data(), n);
-      hipLaunchKernelGGL(kb_norm_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, n, partial.data());
-      hipLaunchKernelGGL(kb_renorm_acts, dim3(nb, nacts), dim3(kT), 0, s, ad, partial.data(), nb, init_norm.data(),
-                         n);
-      if (ncopy > 0) hipLaunchKernelGGL(kb_copy_acts, dim3(nb, ncopy), dim3(kT), 0, s, ad + nacts, n);
+      const dim3 ga = il_grid(n, nacts);
+      hipLaunchKernelGGL(il_combine_acts, ga, dim3(kT), 0, s, ad, nacts, Vc, ldj, nc, coefs.data(), n,
+                         partial.data());
+      hipLaunchKernelGGL(kb_act_scale, dim3((nacts + 3) / 4), dim3(kT), 0, s, ad, nacts, partial.data(), nb,
+                         init_norm.data(), ascale.data());
+      hipLaunchKernelGGL(il_scale_acts, ga, dim3(kT), 0, s, ad, nacts, ascale.data(), n, nc);
+      if (ncopy > 0) hipLaunchKernelGGL(il_copy_acts, il_grid(n, ncopy), dim3(kT), 0, s, ad + nacts, ncopy, n, nc);
       QMBED_HIP(hipGetLastError());
     }
   }
+  // outputs to the caller's [time][column][row] layout
+  for (int q = 0; q < T; ++q)
+    hipLaunchKernelGGL(il_to_cols, grid, dim3(kT), 0, s, out_blk(q), n, nc, 0, nc,
+                       reinterpret_cast<double2*>(d_out) + blk * size_t(q));
+  QMBED_HIP(hipGetLastError());
   QMBED_HIP(hipStreamSynchronize(s));
   if (diag) *diag = dg;
 }

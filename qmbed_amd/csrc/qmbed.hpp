@@ -242,6 +242,11 @@ class Operator {
   bool apply_mixed(const float* x, double* y, bool complex_vectors, hipStream_t s, const int* gate = nullptr) const;
   // nvec columns (stride = dim), complex; one lookup shared across columns
   void apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const;
+  // nvec complex columns interleaved: element (row, v) at row · ld + v (the
+  // columns of a row contiguous, so one gathered cache line serves several
+  // columns). Needs prepare_interleaved_action (compiled stream); returns
+  // false (nothing launched) otherwise.
+  bool apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nvec, hipStream_t s) const;
 
   // per-row Σ|H_ij| over j != i merged per column, plus |H_ii - shift|;
   // returns (trace, max row sum) — ‖A-μI‖₁ for Hermitian A
@@ -466,6 +471,10 @@ struct EvolutionDiagnostics {
   long rejected_trial_intervals = 0;
   double maximum_estimated_error = 0.0;
 };
+
+// Builds (once) the compiled index stream an operator needs for
+// apply_batch_interleaved; false when the sector does not fit it.
+bool prepare_interleaved_action(const Operator& op, hipStream_t s);
 
 // Device-resident evolution: d_cols holds ncol complex columns of length
 // dim ([column][row]); d_out receives [time][column][row]. One host
