@@ -306,17 +306,25 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   uint32_t rank_r = 0;
   if (IMG && a.up >= 0) rank_r = rank_colex(r, s_bin);
 
-  // diagonal (first lane of the row)
+  // diagonal (first lane of the row): the per-row table of the current
+  // coefficients when there is one, else a walk over the diagonal groups
   Cx<S> dsum{0, 0};
-  const int ndiag = sub == 0 ? a.n_diag : 0;
-  for (int g = 0; g < ndiag; ++g) {
-    const int pat = pattern_of(r, s_meta[4 * g + 1]);
-    const int off = int(s_meta[4 * g + 3]) + pat;
-    if (CMODE == 2) {
-      dsum.re += S(s_amp[2 * off]);
-      dsum.im += S(s_amp[2 * off + 1]);
-    } else {
-      dsum.re += S(s_amp[off]);
+  if (a.diag) {
+    if (sub == 0) {
+      dsum.re = S(a.diag[row]);
+      if (CMODE == 2) dsum.im = S(a.diag[a.n + row]);
+    }
+  } else {
+    const int ndiag = sub == 0 ? a.n_diag : 0;
+    for (int g = 0; g < ndiag; ++g) {
+      const int pat = pattern_of(r, s_meta[4 * g + 1]);
+      const int off = int(s_meta[4 * g + 3]) + pat;
+      if (CMODE == 2) {
+        dsum.re += S(s_amp[2 * off]);
+        dsum.im += S(s_amp[2 * off + 1]);
+      } else {
+        dsum.re += S(s_amp[off]);
+      }
     }
   }
   Cx<S> acc[NV];
