@@ -138,13 +138,15 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * component is one scan coordinate).
  * options_json: {"snapshots":32,"hidden":[64,64],"epochs":4000,
  * "learning_rate":0.002,"final_learning_rate":<learning_rate>,"seed":0,
- * "targets":"projections","samples":512}, any subset, NULL for all
+ * "targets":"projections","samples":512,"provisional":true}, any subset, NULL for all
  * defaults (the learning rate decays exponentially to final_learning_rate;
  * equal values keep it constant). targets "projections": the network learns
  * the coefficients of the training ground states over the snapshots;
  * "continuation": it learns the lowest eigenvector of the family projected
  * on the snapshots at the training points and `samples` seeded points of
- * their bounding box. */
+ * their bounding box. provisional: training points after the first
+ * `snapshots` start from the projection on the snapshot space of those
+ * already solved. */
 typedef struct qmbed_hip_scan qmbed_hip_scan;
 qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
 void qmbed_hip_scan_free(qmbed_hip_scan* scan);

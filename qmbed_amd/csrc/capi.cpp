@@ -1007,6 +1007,7 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
         else if (x->raw == "continuation") o.targets = ScanOptions::Targets::Continuation;
         else bad("targets must be \"projections\" or \"continuation\"");
       }
+      if (auto* x = v.get("provisional"); x && x->kind == json::Value::Bool) o.provisional = x->b;
       if (auto* x = v.get("samples"); x && !x->is_null()) {
         o.samples = int(as_long(*x, "samples"));
         if (o.samples < 0 || o.samples > 65536) bad("samples must lie in [0, 65536]");
@@ -1040,6 +1041,7 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
     write_dvec(o, rep.energies);
     o += ",\"iterations\":";
     write_long_array(o, rep.iterations);
+    o += ",\"provisional_starts\":" + std::to_string(rep.provisional_starts);
     o += ",\"snapshot_indices\":";
     write_long_array(o, std::vector<long>(rep.snapshot_indices.begin(), rep.snapshot_indices.end()));
     o += ",\"solve_seconds\":";

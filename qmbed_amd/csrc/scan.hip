@@ -33,6 +33,7 @@
 #include "scan.hpp"
 
 namespace qmbed {
+
 namespace {
 
 constexpr int kT = 256;
@@ -197,35 +198,12 @@ double WarmStartScan::gauge_sign(const double* d_psi) {
   return bv < 0 ? -1.0 : 1.0;
 }
 
-WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, const EigshOptions& o,
-                                                const EigshControls& ctl) {
-  if (n < 1) throw invalid_options("training needs at least one point");
-  if (!op_.is_real() || !op_.basis().real_characters())
-    throw unsupported_backend("the learned warm start is built for real-valued families");
-  TrainReport rep;
+// Snapshot space of the first n training states (device, N × n): the Gram
+// matrix G = Ψᵀ Ψ, pivoted Cholesky of G selecting up to `snapshots` states,
+// the orthonormal basis S = Ψ_sel R⁻¹ and the family's projection on it.
+// Lm (n × ns, column-major) receives the coefficients of every state over S.
+void WarmStartScan::build_space(const double* psi, int n, std::vector<double>& Lm, std::vector<int>& sel) {
   const size_t N = op_.dim();
-  auto t0 = std::chrono::steady_clock::now();
-  // 1. ground states of the training points (plain start)
-  DeviceArray<double> psi(N * size_t(n));
-  for (int i = 0; i < n; ++i) {
-    op_.evaluate_coefficients(coefficients_of(points + size_t(i) * P_), s_);
-    if (!op_.hermitian()) throw non_hermitian();
-    DeviceArray<double> vecs;
-    bool cv = false;
-    EigshOptions oo = o;
-    oo.eigenpairs = 1;
-    Eigensystem es = eigsh_on_device(op_, oo, nullptr, false, ctl, vecs, cv, s_);
-    if (cv) throw unsupported_backend("the learned warm start is built for real-valued families");
-    const double sign = gauge_sign(vecs.data());
-    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(N)), dim3(kT), 0, s_, vecs.data(), N, sign,
-                       psi.data() + N * size_t(i));
-    QMBED_HIP(hipStreamSynchronize(s_));  // vecs dies here
-    rep.energies.push_back(es.eigenvalues[0]);
-    rep.iterations.push_back(es.iterations);
-  }
-  rep.solve_seconds = seconds_since(t0);
-  // 2. Gram matrix G = Ψᵀ Ψ
-  auto t1 = std::chrono::steady_clock::now();
   DeviceArray<double> dG(size_t(n) * size_t(n));
   {
     rocblas_handle h;
@@ -233,7 +211,7 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
     rocblas_set_stream(h, s_);
     const double one = 1.0, zero = 0.0;
     rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, rocblas_int(N),
-                                      &one, psi.data(), rocblas_int(N), psi.data(), rocblas_int(N), &zero, dG.data(), n);
+                                      &one, psi, rocblas_int(N), psi, rocblas_int(N), &zero, dG.data(), n);
     QMBED_HIP(hipStreamSynchronize(s_));
     rocblas_destroy_handle(h);
     if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
@@ -242,13 +220,14 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   QMBED_HIP(hipMemcpy(G.data(), dG.data(), G.size() * 8, hipMemcpyDeviceToHost));
   // 3. pivoted Cholesky: G ≈ L Lᵀ over the selected pivots (column-major G)
   const int want = std::min(opt_.snapshots, n);
-  std::vector<double> d(static_cast<size_t>(n)), Lm(size_t(n) * size_t(want), 0.0);  // Lm[i + n*k]
+  std::vector<double> d(static_cast<size_t>(n));
+  Lm.assign(size_t(n) * size_t(want), 0.0);  // Lm[i + n*k]
   double dmax0 = 0;
   for (int i = 0; i < n; ++i) {
     d[size_t(i)] = G[size_t(i) + size_t(n) * size_t(i)];
     dmax0 = std::max(dmax0, d[size_t(i)]);
   }
-  std::vector<int> sel;
+  sel.clear();
   std::vector<char> used(static_cast<size_t>(n), 0);
   for (int k = 0; k < want; ++k) {
     int j = -1;
@@ -290,7 +269,7 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   {
     DeviceArray<double> Psel(N * size_t(ns_)), dR(Rinv.size());
     for (int b = 0; b < ns_; ++b)
-      QMBED_HIP(hipMemcpyAsync(Psel.data() + N * size_t(b), psi.data() + N * size_t(sel[size_t(b)]), N * 8,
+      QMBED_HIP(hipMemcpyAsync(Psel.data() + N * size_t(b), psi + N * size_t(sel[size_t(b)]), N * 8,
                                hipMemcpyDeviceToDevice, s_));
     QMBED_HIP(hipMemcpyAsync(dR.data(), Rinv.data(), Rinv.size() * 8, hipMemcpyHostToDevice, s_));
     S_.resize(N * size_t(ns_));
@@ -341,6 +320,64 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
     }
     rocblas_destroy_handle(h);
   }
+}
+
+WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, const EigshOptions& o,
+                                                const EigshControls& ctl) {
+  if (n < 1) throw invalid_options("training needs at least one point");
+  if (!op_.is_real() || !op_.basis().real_characters())
+    throw unsupported_backend("the learned warm start is built for real-valued families");
+  TrainReport rep;
+  const size_t N = op_.dim();
+  auto t0 = std::chrono::steady_clock::now();
+  // 1. ground states of the training points: the first `snapshots` from the
+  // plain start; once they are solved (and span a space), the rest start
+  // from the lowest eigenvector of the family projected on that provisional
+  // snapshot space, so the training solves reuse their own results
+  DeviceArray<double> psi(N * size_t(n));
+  const int n_plain = opt_.provisional ? std::min(n, opt_.snapshots) : n;
+  bool provisional = false;
+  DeviceArray<double> v0;
+  for (int i = 0; i < n; ++i) {
+    if (i == n_plain && n_plain < n) {
+      std::vector<double> Lp;
+      std::vector<int> sp;
+      build_space(psi.data(), n_plain, Lp, sp);
+      provisional = ns_ > 0;
+      if (provisional) v0.resize(N);
+    }
+    const double* point = points + size_t(i) * P_;
+    const double* start = nullptr;
+    if (provisional) {
+      const auto y = continuation(point);
+      DeviceArray<double> dy(static_cast<size_t>(ns_));
+      QMBED_HIP(hipMemcpyAsync(dy.data(), y.data(), y.size() * 8, hipMemcpyHostToDevice, s_));
+      lk::gemm_vc(false, S_.data(), N, ns_, dy.data(), 1, v0.data(), s_);
+      QMBED_HIP(hipStreamSynchronize(s_));  // y and dy die here
+      start = v0.data();
+      ++rep.provisional_starts;
+    }
+    op_.evaluate_coefficients(coefficients_of(point), s_);
+    if (!op_.hermitian()) throw non_hermitian();
+    DeviceArray<double> vecs;
+    bool cv = false;
+    EigshOptions oo = o;
+    oo.eigenpairs = 1;
+    Eigensystem es = eigsh_on_device(op_, oo, start, false, ctl, vecs, cv, s_);
+    if (cv) throw unsupported_backend("the learned warm start is built for real-valued families");
+    const double sign = gauge_sign(vecs.data());
+    hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(N)), dim3(kT), 0, s_, vecs.data(), N, sign,
+                       psi.data() + N * size_t(i));
+    QMBED_HIP(hipStreamSynchronize(s_));  // vecs dies here
+    rep.energies.push_back(es.eigenvalues[0]);
+    rep.iterations.push_back(es.iterations);
+  }
+  rep.solve_seconds = seconds_since(t0);
+  // 2. the snapshot space of all training states
+  auto t1 = std::chrono::steady_clock::now();
+  std::vector<double> Lm;
+  std::vector<int> sel;
+  build_space(psi.data(), n, Lm, sel);
   rep.gram_seconds = seconds_since(t1);
   // 4. the network: inputs normalized to [-1, 1] per parameter
   auto t2 = std::chrono::steady_clock::now();

@@ -22,6 +22,9 @@ struct ScanOptions {
   enum class Targets { Projections, Continuation };
   Targets targets = Targets::Projections;
   int samples = 512;
+  // training points after the first `snapshots` start from the projection
+  // on the snapshot space of those already solved
+  bool provisional = true;
 };
 
 class WarmStartScan {
@@ -35,6 +38,7 @@ class WarmStartScan {
     std::vector<long> iterations;
     std::vector<int> snapshot_indices;
     int snapshots = 0;
+    int provisional_starts = 0;  // training solves started from the provisional space
     double solve_seconds = 0, gram_seconds = 0, train_seconds = 0;
     float loss_first = 0, loss_last = 0;
     double target_norm2 = 0;  // mean ‖target‖² (scale of the loss)
@@ -67,6 +71,9 @@ class WarmStartScan {
   std::vector<cplx> coefficients_of(const double* point) const;
   // gauge-fixed lowest eigenvector of Sᵀ H(c) S
   std::vector<double> continuation(const double* point) const;
+  // snapshot space of the first n training states (sets S_, ns_, proj_,
+  // gauge_); Lm and sel receive the coefficients and the selected states
+  void build_space(const double* psi, int n, std::vector<double>& Lm, std::vector<int>& sel);
   double gauge_sign(const double* d_psi);
 
   Operator& op_;

@@ -85,6 +85,12 @@ def test_learned_warm_start_scan(qa, oracle):
           flush=True)
     assert rep["snapshots"] == 8
     assert rep["loss_last"] < 0.05 * rep["loss_first"]
+    # training points after the first 8 start from the provisional snapshot
+    # space: same ground states as from the plain start, fewer iterations
+    assert rep["provisional_starts"] == 16
+    plain_train = scan.solve(train, warm=False)
+    np.testing.assert_allclose(rep["energies"], plain_train["energies"], rtol=1e-10)
+    assert np.mean(rep["iterations"][8:]) < np.mean(plain_train["iterations"][8:])
     warm = scan.solve(held, warm=True)
     plain = scan.solve(held, warm=False)
     warm32 = scan.solve(held, warm=True, matvec_precision="fp32")
