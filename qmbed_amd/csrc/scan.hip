@@ -34,6 +34,8 @@
 
 namespace qmbed {
 
+std::vector<double> tridiagonal_eigvec(const std::vector<double>& alpha, const std::vector<double>& beta, double lam);
+
 namespace {
 
 constexpr int kT = 256;
@@ -106,45 +108,65 @@ struct Lcg {
   }
 };
 
-// Eigen-decomposition of a symmetric n × n matrix (column-major) by cyclic
-// Jacobi rotations; returns the eigenvector of the lowest eigenvalue
+// Eigenvector of the lowest eigenvalue of a symmetric n × n matrix
+// (column-major): Householder reduction to tridiagonal form, the lowest
+// eigenvalue by bisection, its tridiagonal eigenvector by inverse iteration,
+// and the reflectors applied back (O(n³) with small constants: tens of
+// microseconds at n = 32).
 std::vector<double> lowest_eigenvector(std::vector<double> a, int n) {
-  std::vector<double> v(size_t(n) * n, 0.0);
-  for (int i = 0; i < n; ++i) v[size_t(i) * n + i] = 1.0;
   auto A = [&](int i, int j) -> double& { return a[size_t(j) * n + i]; };
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, diag = 0;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) (i == j ? diag : off) += A(i, j) * A(i, j);
-    if (off <= 1e-30 * std::max(diag, 1e-300)) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = A(p, q);
-        if (apq == 0.0) continue;
-        const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p, q
-          const double akp = A(k, p), akq = A(k, q);
-          A(k, p) = c * akp - sn * akq;
-          A(k, q) = sn * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // rows p, q
-          const double apk = A(p, k), aqk = A(q, k);
-          A(p, k) = c * apk - sn * aqk;
-          A(q, k) = sn * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double vkp = v[size_t(p) * n + k], vkq = v[size_t(q) * n + k];
-          v[size_t(p) * n + k] = c * vkp - sn * vkq;
-          v[size_t(q) * n + k] = sn * vkp + c * vkq;
-        }
-      }
+  std::vector<std::vector<double>> refl;  // v_k for rows k+1.. (unit, or empty)
+  std::vector<double> p(static_cast<size_t>(n)), q(static_cast<size_t>(n));
+  for (int k = 0; k + 2 < n; ++k) {
+    const int m = n - k - 1;
+    std::vector<double> v(static_cast<size_t>(m));
+    double xn = 0;
+    for (int i = 0; i < m; ++i) {
+      v[size_t(i)] = A(k + 1 + i, k);
+      xn += v[size_t(i)] * v[size_t(i)];
+    }
+    xn = std::sqrt(xn);
+    const double alpha = v[0] >= 0 ? -xn : xn;
+    v[0] -= alpha;
+    double vn = 0;
+    for (double x : v) vn += x * x;
+    vn = std::sqrt(vn);
+    if (vn == 0.0 || xn == 0.0) {
+      refl.emplace_back();
+      continue;
+    }
+    for (double& x : v) x /= vn;
+    // A22 ← H A22 H, H = I − 2 v vᵀ: p = A22 v, q = p − (vᵀp) v,
+    // A22 −= 2 (v qᵀ + q vᵀ)
+    for (int i = 0; i < m; ++i) {
+      double t = 0;
+      for (int j = 0; j < m; ++j) t += A(k + 1 + i, k + 1 + j) * v[size_t(j)];
+      p[size_t(i)] = t;
+    }
+    double kk = 0;
+    for (int i = 0; i < m; ++i) kk += v[size_t(i)] * p[size_t(i)];
+    for (int i = 0; i < m; ++i) q[size_t(i)] = p[size_t(i)] - kk * v[size_t(i)];
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i)
+        A(k + 1 + i, k + 1 + j) -= 2.0 * (v[size_t(i)] * q[size_t(j)] + q[size_t(i)] * v[size_t(j)]);
+    A(k + 1, k) = alpha;
+    A(k, k + 1) = alpha;
+    for (int i = 1; i < m; ++i) A(k + 1 + i, k) = A(k, k + 1 + i) = 0.0;
+    refl.push_back(std::move(v));
   }
-  int lo = 0;
-  for (int i = 1; i < n; ++i)
-    if (A(i, i) < A(lo, lo)) lo = i;
-  return std::vector<double>(v.begin() + size_t(lo) * n, v.begin() + size_t(lo + 1) * n);
+  std::vector<double> d(static_cast<size_t>(n)), e(static_cast<size_t>(std::max(n - 1, 0)));
+  for (int i = 0; i < n; ++i) d[size_t(i)] = A(i, i);
+  for (int i = 0; i + 1 < n; ++i) e[size_t(i)] = A(i + 1, i);
+  const double lam = tridiagonal_eigenvalue_range(d, e, 0, 1)[0];
+  std::vector<double> z = tridiagonal_eigvec(d, e, lam);
+  for (int k = int(refl.size()) - 1; k >= 0; --k) {
+    const auto& v = refl[size_t(k)];
+    if (v.empty()) continue;
+    double t = 0;
+    for (size_t i = 0; i < v.size(); ++i) t += v[i] * z[size_t(k) + 1 + i];
+    for (size_t i = 0; i < v.size(); ++i) z[size_t(k) + 1 + i] -= 2.0 * t * v[i];
+  }
+  return z;
 }
 
 double seconds_since(std::chrono::steady_clock::time_point t0) {
