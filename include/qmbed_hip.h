@@ -93,7 +93,9 @@ double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void*
  * fp32_iterations / fp64_iterations / reorthogonalization counters. Accepts
  * the extra solver keys "matvec_precision" ("fp64"|"fp32") and
  * "early_stop" (bool), "check_every" (steps between Ritz-estimate checks,
- * 1-64, default 2) and "reorthogonalization_eta" (DGKS threshold, (0, 1]). */
+ * 1-64, default 2), "read_lag" (steps launched between posting a read-back
+ * and collecting it, 0-8; 0, the default, picks 1 from 2^20 rows on and 2
+ * below) and "reorthogonalization_eta" (DGKS threshold, (0, 1]). */
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
 
 /* Solve `count` models (typically the symmetry sectors of one Hamiltonian)
@@ -138,7 +140,8 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * component is one scan coordinate).
  * options_json: {"snapshots":32,"hidden":[64,64],"epochs":4000,
  * "learning_rate":0.002,"final_learning_rate":<learning_rate>,"seed":0,
- * "targets":"projections","samples":512,"provisional":true}, any subset, NULL for all
+ * "targets":"projections","samples":512,"provisional":true,"space":"greedy"},
+ * any subset, NULL for all
  * defaults (the learning rate decays exponentially to final_learning_rate;
  * equal values keep it constant). targets "projections": the network learns
  * the coefficients of the training ground states over the snapshots;
@@ -146,7 +149,9 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * on the snapshots at the training points and `samples` seeded points of
  * their bounding box. provisional: training points after the first
  * `snapshots` start from the projection on the snapshot space of those
- * already solved. */
+ * already solved. space "greedy": `snapshots` training ground states chosen
+ * by pivoted Cholesky of their Gram matrix; "principal": the `snapshots`
+ * leading principal components of all training ground states. */
 typedef struct qmbed_hip_scan qmbed_hip_scan;
 qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
 void qmbed_hip_scan_free(qmbed_hip_scan* scan);

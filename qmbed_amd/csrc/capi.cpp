@@ -234,6 +234,11 @@ SolverReq parse_solver(const Value& s) {
     if (c < 1 || c > 64) bad("check_every must lie in [1, 64]");
     r.ctl.check_every = int(c);
   }
+  if (auto* v = s.get("read_lag"); v && !v->is_null()) {
+    const long c = as_long(*v, "read_lag");
+    if (c < 0 || c > 8) bad("read_lag must lie in [0, 8]");
+    r.ctl.read_lag = int(c);
+  }
   if (auto* v = s.get("reorthogonalization_eta"); v && !v->is_null()) {
     const double eta = as_double(*v, "reorthogonalization_eta");
     if (!(eta > 0.0 && eta <= 1.0)) bad("reorthogonalization_eta must lie in (0, 1]");
@@ -1008,6 +1013,12 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
         else bad("targets must be \"projections\" or \"continuation\"");
       }
       if (auto* x = v.get("provisional"); x && x->kind == json::Value::Bool) o.provisional = x->b;
+      if (auto* x = v.get("space"); x && !x->is_null()) {
+        if (x->kind != json::Value::String) bad("space must be a string");
+        if (x->raw == "greedy") o.space = ScanOptions::Space::Greedy;
+        else if (x->raw == "principal") o.space = ScanOptions::Space::Principal;
+        else bad("space must be \"greedy\" or \"principal\"");
+      }
       if (auto* x = v.get("samples"); x && !x->is_null()) {
         o.samples = int(as_long(*x, "samples"));
         if (o.samples < 0 || o.samples > 65536) bad("samples must lie in [0, 65536]");

@@ -534,7 +534,7 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
   // collected kLag steps later, when the device has long finished the copy,
   // so the device never waits for the host; a convergence found at a
   // collected point costs at most kLag extra steps.
-  constexpr int kLag = 2;
+  const int kLag = ctl.read_lag > 0 ? ctl.read_lag : (op.dim() >= (size_t(1) << 20) ? 1 : 2);
   int interval = std::max(1, ctl.check_every);
   int next_sync = interval;
   bool pending = false, pending_conv = false;

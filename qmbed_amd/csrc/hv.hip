@@ -1177,21 +1177,32 @@ __global__ void __launch_bounds__(kBlock) hv_stream_batch_kernel(const BatchSect
 // double per entry, or (re, im) when complex): out[row] = Σ re, out[n + row]
 // = Σ im over the diagonal groups, in group order.
 __global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp, double* out) {
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= a.n) return;
-  const uint32_t r = a.states[row];
-  double re = 0, im = 0;
-  for (int g = 0; g < a.n_diag; ++g) {
-    const int off = int(a.meta[4 * g + 3]) + pattern_of(r, a.meta[4 * g + 1]);
-    if (complex_amp) {
-      re += a.amp[2 * off];
-      im += a.amp[2 * off + 1];
-    } else {
-      re += a.amp[off];
-    }
+  // the amplitude table and the diagonal groups' (sites, offset) words in
+  // LDS: every row reads n_diag of them at row-dependent offsets
+  __shared__ double s_amp[2 * kMaxAmps];
+  __shared__ uint32_t s_meta[2 * kMaxGroups];
+  const int namp = (complex_amp ? 2 : 1) * a.n_amp;
+  for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
+  for (int g = threadIdx.x; g < a.n_diag; g += kBlock) {
+    s_meta[2 * g] = a.meta[4 * g + 1];
+    s_meta[2 * g + 1] = a.meta[4 * g + 3];
   }
-  out[row] = re;
-  out[a.n + row] = im;
+  __syncthreads();
+  for (size_t row = size_t(blockIdx.x) * kBlock + threadIdx.x; row < a.n; row += size_t(gridDim.x) * kBlock) {
+    const uint32_t r = a.states[row];
+    double re = 0, im = 0;
+    for (int g = 0; g < a.n_diag; ++g) {
+      const int off = int(s_meta[2 * g + 1]) + pattern_of(r, s_meta[2 * g]);
+      if (complex_amp) {
+        re += s_amp[2 * off];
+        im += s_amp[2 * off + 1];
+      } else {
+        re += s_amp[off];
+      }
+    }
+    out[row] = re;
+    out[a.n + row] = im;
+  }
 }
 
 int sym_code(const SpinBasis1D& b) {
@@ -1235,6 +1246,12 @@ struct Operator::DeviceTables {
   int n_codes = 1;
   // per-row diagonal of the current coefficients (re[n], im[n])
   DeviceArray<double> diag;
+  // pinned staging of the amplitude uploads and the event after their copies
+  PinnedArray<double> stage;
+  hipEvent_t staged = nullptr;
+  ~DeviceTables() {
+    if (staged) (void)hipEventDestroy(staged);
+  }
   HvArgs args(const SpinBasis1D& b) const {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
@@ -1322,8 +1339,15 @@ static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, 
   const size_t cap = t.family ? 2 * size_t(std::max(t.n_amp, 1)) : std::max<size_t>(amp.size(), 2);
   if (t.amp.size() < cap) t.amp.resize(cap);
   if (t.famp.size() < std::max<size_t>(famp.size(), 2)) t.famp.resize(std::max<size_t>(famp.size(), 2));
-  if (!amp.empty()) QMBED_HIP(hipMemcpyAsync(t.amp.data(), amp.data(), amp.size() * 8, hipMemcpyHostToDevice, s));
-  if (!famp.empty()) QMBED_HIP(hipMemcpyAsync(t.famp.data(), famp.data(), famp.size() * 8, hipMemcpyHostToDevice, s));
+  // staged through pinned memory, so the copies are asynchronous; the
+  // staging is rewritten only after the previous upload's copies are done
+  if (t.staged) QMBED_HIP(hipEventSynchronize(t.staged));
+  t.stage.ensure(amp.size() + famp.size());
+  std::copy(amp.begin(), amp.end(), t.stage.p);
+  std::copy(famp.begin(), famp.end(), t.stage.p + amp.size());
+  if (!amp.empty()) QMBED_HIP(hipMemcpyAsync(t.amp.data(), t.stage.p, amp.size() * 8, hipMemcpyHostToDevice, s));
+  if (!famp.empty())
+    QMBED_HIP(hipMemcpyAsync(t.famp.data(), t.stage.p + amp.size(), famp.size() * 8, hipMemcpyHostToDevice, s));
   // the per-row diagonal read by the stream kernel (one pass per evaluation
   // instead of a walk over the diagonal groups in every H·v)
   const SpinBasis1D& b = op.basis();
@@ -1331,11 +1355,13 @@ static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, 
   if (n > 0 && n <= (size_t(1) << 28)) {
     if (t.diag.size() != 2 * n) t.diag.resize(2 * n);
     HvArgs a = t.args(b);
-    hipLaunchKernelGGL(diag_kernel, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a,
-                       t.complex_coef ? 1 : 0, t.diag.data());
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(diag_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, a, t.complex_coef ? 1 : 0,
+                       t.diag.data());
     QMBED_HIP(hipGetLastError());
   }
-  QMBED_HIP(hipStreamSynchronize(s));  // the host vectors die here
+  if (!t.staged) QMBED_HIP(hipEventCreateWithFlags(&t.staged, hipEventDisableTiming));
+  QMBED_HIP(hipEventRecord(t.staged, s));
 }
 
 // The sector matrix of the current tables is Hermitian (checked on the

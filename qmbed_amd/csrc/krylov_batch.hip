@@ -517,22 +517,6 @@ int control_threads() {
   return int(std::max(1u, std::min(16u, h ? h : 1u)));
 }
 
-template <typename T>
-struct Pinned {
-  T* p = nullptr;
-  size_t n = 0;
-  void ensure(size_t want) {
-    if (n >= want) return;
-    if (p) QMBED_HIP(hipHostFree(p));
-    p = nullptr;
-    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault));
-    n = want;
-  }
-  ~Pinned() {
-    if (p) (void)hipHostFree(p);
-  }
-};
-
 }  // namespace
 
 void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const EvolutionOptions& opts,
@@ -568,16 +552,16 @@ void evolve_batch_device(const Operator& op, const cplx* d_cols, int nc, const E
       coefs(2 * size_t(nc) * (T + 1) * m + 2);
   DeviceArray<int> act_d(nc), running(nc), size_d(nc);
   DeviceArray<uint8_t> acts_d(sizeof(Act) * (size_t(max_acts) + size_t(nc)) + sizeof(Act));
-  Pinned<double> hist;  // are | aim | bh | init_norm | size (as int pairs)
+  PinnedArray<double> hist;  // are | aim | bh | init_norm | size (as int pairs)
   hist.ensure(3 * size_t(nc) * m + 2 * size_t(nc) + 8);
-  Pinned<int> act_h;
+  PinnedArray<int> act_h;
   act_h.ensure(nc);
-  Pinned<Act> acts_h;
+  PinnedArray<Act> acts_h;
   acts_h.ensure(size_t(max_acts) + size_t(nc) + 1);
-  Pinned<Act> emit_h;
+  PinnedArray<Act> emit_h;
   emit_h.ensure(size_t(nc) * T + 1);
   DeviceArray<uint8_t> emit_d(sizeof(Act) * (size_t(nc) * T + 1));
-  Pinned<double> coef_h;
+  PinnedArray<double> coef_h;
   coef_h.ensure(2 * size_t(nc) * (T + 1) * m + 2);
 
   double2* curp = reinterpret_cast<double2*>(cur.data());

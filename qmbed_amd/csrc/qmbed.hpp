@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <complex>
 #include <cstdint>
 #include <memory>
@@ -105,6 +106,28 @@ class DeviceArray {
  private:
   T* p_ = nullptr;
   size_t n_ = 0;
+};
+
+// Page-locked host buffer (asynchronous copies), grown on demand.
+template <typename T>
+struct PinnedArray {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedArray() = default;
+  PinnedArray(const PinnedArray&) = delete;
+  PinnedArray& operator=(const PinnedArray&) = delete;
+  void ensure(size_t want) {
+    if (n >= want) return;
+    if (p) QMBED_HIP(hipHostFree(p));
+    p = nullptr;
+    n = 0;
+    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(want, 1) * sizeof(T),
+                            hipHostMallocDefault));
+    n = want;
+  }
+  ~PinnedArray() {
+    if (p) (void)hipHostFree(p);
+  }
 };
 
 // ------------------------------------------------------------------ basis --
@@ -399,6 +422,11 @@ struct EigshControls {
   // sector batches: at most this many concatenated (tile-padded) rows per
   // lockstep batch; 0 sizes the batches to the free device memory
   long batch_rows = 0;
+  // steps launched between posting a read-back of the window's coefficients
+  // and collecting it (the host then never waits for the device); 0 picks 1
+  // for operators of at least 2^20 rows, whose steps outlast the host's work
+  // on a read-back, and 2 below that
+  int read_lag = 0;
 };
 
 struct Eigensystem {

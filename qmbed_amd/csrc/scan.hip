@@ -23,6 +23,7 @@
 // start S y, the best start the snapshot span holds for the Rayleigh
 // quotient. The same map can serve as the network's targets.
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -238,10 +239,76 @@ void WarmStartScan::build_space(const double* psi, int n, std::vector<double>& L
     rocblas_destroy_handle(h);
     if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
   }
+  const int want = std::min(opt_.snapshots, n);
+  if (opt_.space == ScanOptions::Space::Principal) {
+    principal_space(psi, n, dG, want, Lm);
+    sel.clear();
+  } else {
+    greedy_space(psi, n, dG, want, Lm, sel);
+  }
+  project_family();
+}
+
+// Principal components: G = U Λ Uᵀ (rocSOLVER, ascending); the leading
+// `want` pairs with λ > 1e-10 λ_max give S = Ψ U_k Λ_k^{-1/2} (orthonormal)
+// and the coefficients of ψ_i over S, Lm[i + n·k] = U[i][k] √λ_k.
+void WarmStartScan::principal_space(const double* psi, int n, DeviceArray<double>& dG, int want,
+                                    std::vector<double>& Lm) {
+  const size_t N = op_.dim();
+  std::vector<double> U(size_t(n) * size_t(n)), W(static_cast<size_t>(n));
+  {
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
+    rocblas_set_stream(h, s_);
+    DeviceArray<double> dW(static_cast<size_t>(n)), dE(static_cast<size_t>(n));
+    DeviceArray<int> dInfo(1);
+    rocblas_status st = rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, dG.data(), n, dW.data(),
+                                         dE.data(), dInfo.data());
+    int info = 0;
+    QMBED_HIP(hipMemcpyAsync(&info, dInfo.data(), sizeof(int), hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipMemcpyAsync(U.data(), dG.data(), U.size() * 8, hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipMemcpyAsync(W.data(), dW.data(), W.size() * 8, hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipStreamSynchronize(s_));
+    rocblas_destroy_handle(h);
+    if (st != rocblas_status_success || info != 0) throw internal_state("rocsolver_dsyevd failed on the Gram matrix");
+  }
+  std::vector<int> cols;
+  for (int k = n - 1; k >= 0 && int(cols.size()) < want; --k)
+    if (W[size_t(k)] > 1.0e-10 * W[size_t(n) - 1]) cols.push_back(k);
+  ns_ = int(cols.size());
+  if (ns_ == 0) throw internal_state("the training ground states span no snapshot");
+  std::vector<double> C(size_t(n) * size_t(ns_));
+  Lm.assign(size_t(n) * size_t(ns_), 0.0);
+  for (int k = 0; k < ns_; ++k) {
+    const double lam = W[size_t(cols[size_t(k)])];
+    const double* u = U.data() + size_t(n) * size_t(cols[size_t(k)]);
+    for (int i = 0; i < n; ++i) {
+      C[size_t(i) + size_t(n) * size_t(k)] = u[i] / std::sqrt(lam);
+      Lm[size_t(i) + size_t(n) * size_t(k)] = u[i] * std::sqrt(lam);
+    }
+  }
+  DeviceArray<double> dC(C.size());
+  QMBED_HIP(hipMemcpyAsync(dC.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
+  S_.resize(N * size_t(ns_));
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
+  rocblas_set_stream(h, s_);
+  const double one = 1.0, zero = 0.0;
+  rocblas_status st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, rocblas_int(N), ns_, n, &one,
+                                    psi, rocblas_int(N), dC.data(), n, &zero, S_.data(), rocblas_int(N));
+  QMBED_HIP(hipStreamSynchronize(s_));
+  rocblas_destroy_handle(h);
+  if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
+}
+
+// Pivoted Cholesky of the Gram matrix: G ≈ L Lᵀ over the selected pivots;
+// S = Ψ_sel R⁻¹ with R = L[sel]ᵀ, and Lm the coefficients of ψ_i over S.
+void WarmStartScan::greedy_space(const double* psi, int n, DeviceArray<double>& dG, int want,
+                                 std::vector<double>& Lm, std::vector<int>& sel) {
+  const size_t N = op_.dim();
   std::vector<double> G(size_t(n) * size_t(n));
   QMBED_HIP(hipMemcpy(G.data(), dG.data(), G.size() * 8, hipMemcpyDeviceToHost));
-  // 3. pivoted Cholesky: G ≈ L Lᵀ over the selected pivots (column-major G)
-  const int want = std::min(opt_.snapshots, n);
+  // (column-major G)
   std::vector<double> d(static_cast<size_t>(n));
   Lm.assign(size_t(n) * size_t(want), 0.0);  // Lm[i + n*k]
   double dmax0 = 0;
@@ -298,7 +365,12 @@ void WarmStartScan::build_space(const double* psi, int n, std::vector<double>& L
     lk::gemm_vc(false, Psel.data(), N, ns_, dR.data(), ns_, S_.data(), s_);
     QMBED_HIP(hipStreamSynchronize(s_));
   }
-  // projected parts Sᵀ H_part S: H(0) S, then H(e_p) S − H(0) S per component
+}
+
+// Projected parts Sᵀ H_part S: H(0) S, then H(e_p) S − H(0) S per component,
+// and the gauge Sᵀ1.
+void WarmStartScan::project_family() {
+  const size_t N = op_.dim();
   {
     rocblas_handle h;
     if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");

@@ -25,6 +25,13 @@ struct ScanOptions {
   // training points after the first `snapshots` start from the projection
   // on the snapshot space of those already solved
   bool provisional = true;
+  // The snapshot space. Greedy: `snapshots` training ground states chosen by
+  // pivoted Cholesky of their Gram matrix (largest residual first).
+  // Principal: the `snapshots` leading principal components of all training
+  // ground states (eigenvectors of the Gram matrix), the space that best
+  // approximates them in the mean-square sense.
+  enum class Space { Greedy, Principal };
+  Space space = Space::Greedy;
 };
 
 class WarmStartScan {
@@ -74,6 +81,10 @@ class WarmStartScan {
   // snapshot space of the first n training states (sets S_, ns_, proj_,
   // gauge_); Lm and sel receive the coefficients and the selected states
   void build_space(const double* psi, int n, std::vector<double>& Lm, std::vector<int>& sel);
+  void principal_space(const double* psi, int n, DeviceArray<double>& dG, int want, std::vector<double>& Lm);
+  void greedy_space(const double* psi, int n, DeviceArray<double>& dG, int want, std::vector<double>& Lm,
+                    std::vector<int>& sel);
+  void project_family();
   double gauge_sign(const double* d_psi);
 
   Operator& op_;

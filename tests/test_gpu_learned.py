@@ -125,21 +125,51 @@ def test_learned_warm_start_scan(qa, oracle):
     assert ri < wi
 
 
-def test_projected_family_matches_oracle(qa, oracle):
+@pytest.mark.parametrize("space", ["greedy", "principal"])
+def test_projected_family_matches_oracle(qa, oracle, space):
     # the continuation start at a training point reproduces that point's
-    # ground state: with every training state a snapshot, S·y is ψ itself
-    # (to the subspace's rounding), so Lanczos from it converges at once
+    # ground state: with every training state in the snapshot space, S·y is
+    # ψ itself (to the subspace's rounding), so Lanczos from it converges at
+    # once
     m = _ising_scan_model(qa)
     train = np.array([(0.5, 0.1), (1.0, 0.2), (1.5, 0.3), (1.2, 0.05)])
-    scan = qa.Scan(m, snapshots=4, epochs=200, seed=1)
+    scan = qa.Scan(m, snapshots=4, epochs=200, seed=1, space=space)
     rep = scan.train(train)
     assert rep["snapshots"] == 4
+    assert len(rep["snapshot_indices"]) == (4 if space == "greedy" else 0)
     r = scan.solve(train, warm="rayleigh_ritz")
     for i, (a, b) in enumerate(train):
         e0 = np.linalg.eigvalsh(oracle.Model(L, cases.ising(L, a, b), momentum=0, parity=1,
                                              norm="pauli_cartesian").dense())[0]
         assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
         assert r["iterations"][i] <= 8
+
+
+def test_principal_space(qa):
+    # the leading principal components capture the most of the training
+    # states of any space of that dimension: Σ_i ‖Sᵀψ_i‖² (the network's
+    # projection targets) is at least the greedy selection's, and at most n
+    m = _ising_scan_model(qa)
+    hx = np.linspace(0.2, 2.0, 6)
+    hz = np.linspace(0.0, 0.5, 4)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    train = grid[::2]
+    greedy = qa.Scan(m, snapshots=4, epochs=50, seed=0)
+    principal = qa.Scan(m, snapshots=4, epochs=50, seed=0, space="principal")
+    g = greedy.train(train)
+    p = principal.train(train)
+    print("[principal] captured", g["target_norm2"], p["target_norm2"], "of", len(train), flush=True)
+    assert p["snapshots"] == g["snapshots"] == 4
+    assert p["target_norm2"] >= g["target_norm2"] * (1 - 1e-6)
+    assert p["target_norm2"] <= len(train) * (1 + 1e-6)
+    assert p["provisional_starts"] == len(train) - 4
+    held = grid[1::2]
+    warm = principal.solve(held, warm="rayleigh_ritz", matvec_precision="fp32")
+    plain = principal.solve(held, warm=False)
+    np.testing.assert_allclose(warm["energies"], plain["energies"], rtol=1e-10)
+    assert np.mean(warm["iterations"]) < np.mean(plain["iterations"])
+    with pytest.raises(qa.QmbedError, match="space must be"):
+        qa.Scan(m, space="other")
 
 
 def test_continuation_targets(qa):

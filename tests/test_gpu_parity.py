@@ -230,6 +230,28 @@ def test_eigsh_second_pass_rewind(qa, oracle, case):
     _check_spectrum(r2, exact)
 
 
+@pytest.mark.parametrize("eta", [None, 1.0])
+def test_eigsh_read_lag(qa, oracle, eta):
+    # the read-back lag decides how many steps are in flight when the host
+    # reads the window, and so the steps at which it is checked (and where
+    # FP32 products start): every lag passes the same acceptance test, and a
+    # given lag is deterministic (with η = 1 also through the voided and
+    # relaunched steps of second passes)
+    L = 16
+    spec = cases.j1j2(L)
+    exact = np.linalg.eigvalsh(oracle.Model(L, spec, up=8, momentum=0).dense())
+    m = qa.Model(_basis_json(L, 8, 0, None), spec)
+    kw = {} if eta is None else {"reorthogonalization_eta": eta}
+    for prec in ("fp64", "fp32"):
+        for lag in (1, 2, 3, 5):
+            r = m.eigsh(2, read_lag=lag, matvec_precision=prec, **kw)
+            _check_spectrum(r, exact)
+            again = m.eigsh(2, read_lag=lag, matvec_precision=prec, **kw)
+            assert again["eigenvalues"] == r["eigenvalues"] and again["iterations"] == r["iterations"]
+    with pytest.raises(qa.QmbedError, match="read_lag must lie"):
+        m.eigsh(1, read_lag=9)
+
+
 def test_eigsh_rejects_bad_eta(qa):
     m = qa.Model({"kind": "spin", "sites": 8, "up": 4}, cases.heisenberg(8))
     for bad in (0.0, 1.5, -1.0):
