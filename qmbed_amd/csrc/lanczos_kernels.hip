@@ -220,18 +220,25 @@ __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, 
 }
 
 // h[c] = Σ_b partial[b·ld + c] over the rb row blocks of the overlap kernel,
-// one wave per column (lane-strided, then a shuffle tree: a fixed order).
+// one block per column: thread-strided sums, then the waves' shuffle trees
+// and their four totals in order (a fixed order).
 template <typename E>
 __global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int rb, int ld, int ncols, E* h,
                                              const int* gate) {
   if (gate && *gate) return;
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * (kT / 64) + (threadIdx.x >> 6);
-  if (c >= ncols) return;
+  __shared__ E sh[kT / 64];
+  const int c = blockIdx.x;
   E s = zero<E>();
-  for (int b = lane; b < rb; b += 64) s = add(s, partial[size_t(b) * ld + c]);
+  for (int b = threadIdx.x; b < rb; b += kT) s = add(s, partial[size_t(b) * ld + c]);
   for (int o = 32; o > 0; o >>= 1) s = add(s, shfl_down(s, o));
-  if (lane == 0) h[c] = s;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    E t = zero<E>();
+    for (int w = 0; w < kT / 64; ++w) t = add(t, sh[w]);
+    h[c] = t;
+  }
 }
 
 // One column chunk of the subtraction w ← w₀ − V h (hc: the chunk's
@@ -484,15 +491,27 @@ void residual_to_slot(bool c, const void* y, const void* x, const double* lam_sl
 
 // row blocks of the overlap kernel: about 1024 blocks in all, at most 256,
 // never more than there are row tiles
+// Row blocks of the overlap kernel: about 1024 blocks in all (four per CU,
+// so the column loads of sixteen waves per CU are in flight), at least 16
+// per column group, at most one per row tile.
 static int dots_row_blocks(size_t n, int groups) {
   const size_t tiles = (n + size_t(kT * kRPT) - 1) / size_t(kT * kRPT);
-  const int want = std::min(256, std::max(16, 1024 / std::max(groups, 1)));
+  const int want = std::max(16, 1024 / std::max(groups, 1));
   return int(std::min<size_t>(tiles, size_t(want)));
+}
+
+// Columns per block of the overlap kernel for a chunk of mm columns: the
+// smallest tile that holds them (4, 8, then dots_tile), so short windows
+// issue no loads for absent columns.
+template <typename E>
+static int dots_ct(int mm) {
+  return mm <= 4 ? 4 : (mm <= 8 || dots_tile<E>() == 8) ? 8 : 16;
 }
 
 template <typename E>
 static int chunk_groups(int mm) {
-  return (mm + dots_tile<E>() - 1) / dots_tile<E>();
+  const int ct = dots_ct<E>(mm);
+  return (mm + ct - 1) / ct;
 }
 
 // the layout shared by overlaps and subtract: rb from the widest chunk
@@ -533,9 +552,22 @@ void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Wor
       const int mc = std::min(kMaxCols, m - c0);
       const bool last = c0 + mc >= m;
       const int mm = mc + ((last && with_ww) ? 1 : 0);
-      constexpr int CT = dots_tile<E>();
-      hipLaunchKernelGGL((k_dots<E, CT>), dim3(rb, chunk_groups<E>(mm)), dim3(kT), 0, s, cE<E>(V) + size_t(c0) * n, n,
-                         mc, mm, cE<E>(w), mE<E>(ws.partial.data()), ld, c0, gate, t);
+      const dim3 grid(rb, chunk_groups<E>(mm));
+      const E* Vc = cE<E>(V) + size_t(c0) * n;
+      switch (dots_ct<E>(mm)) {
+        case 4:
+          hipLaunchKernelGGL((k_dots<E, 4>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w), mE<E>(ws.partial.data()),
+                             ld, c0, gate, t);
+          break;
+        case 8:
+          hipLaunchKernelGGL((k_dots<E, 8>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w), mE<E>(ws.partial.data()),
+                             ld, c0, gate, t);
+          break;
+        default:
+          hipLaunchKernelGGL((k_dots<E, dots_tile<E>()>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w),
+                             mE<E>(ws.partial.data()), ld, c0, gate, t);
+          break;
+      }
     }
   });
 }
@@ -554,7 +586,8 @@ void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h
     int rb, ld;
     pass_layout<E>(n, m, with_ww, rb, ld);
     const int ncols = m + (with_ww ? 1 : 0);
-    hipLaunchKernelGGL(k_hred<E>, dim3((ncols + kT / 64 - 1) / (kT / 64)), dim3(kT), 0, s, cE<E>(ws.partial.data()),
+    if (ncols > 0)
+      hipLaunchKernelGGL(k_hred<E>, dim3(ncols), dim3(kT), 0, s, cE<E>(ws.partial.data()),
                        rb, ld, ncols, mE<E>(d_h), gate);
     const Three<E> t = three_of<E>(three, ws, n);
     for (int c0 = 0; c0 < m; c0 += kMaxCols) {
