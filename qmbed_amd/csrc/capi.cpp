@@ -1081,10 +1081,9 @@ char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, co
     std::vector<double> e, res, secs;
     std::vector<long> it, it32, it64;
     long warm_n = 0, fallbacks = 0;
-    for (int i = 0; i < n; ++i) {
-      const auto start = warm == 2 ? WarmStartScan::Start::SnapshotRR
-                                   : (warm ? WarmStartScan::Start::Network : WarmStartScan::Start::Plain);
-      auto pr = ts->scan->solve(points + size_t(i) * ts->scan->parameters(), r.opts, r.ctl, start);
+    const auto start = warm == 2 ? WarmStartScan::Start::SnapshotRR
+                                 : (warm ? WarmStartScan::Start::Network : WarmStartScan::Start::Plain);
+    for (const auto& pr : ts->scan->solve_many(points, n, r.opts, r.ctl, start)) {
       e.push_back(pr.es.eigenvalues[0]);
       res.push_back(pr.es.residuals[0]);
       secs.push_back(pr.seconds);

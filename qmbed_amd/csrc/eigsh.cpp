@@ -517,13 +517,21 @@ struct WindowResult {
   long iterations = 0;
   double fail_residual = 0;
   long fp32_matvecs = 0;  // of a window that failed after FP32 products
+  // the start's ‖·‖ read at the window's first read-back was zero or not
+  // finite (the window then stops there)
+  bool bad_start = false;
+  double start_norm = 0;
 };
 
 // One Lanczos window with in-window convergence checks.
 // Throws NonConvergence (as Error) like the reference when the full window
 // does not converge.
+// start_norm2: a pinned host copy of the start's ‖·‖², queued before the
+// window on the same stream and checked at its first read-back (nullptr:
+// already checked).
 void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
-                const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s) {
+                const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s,
+                const double* start_norm2 = nullptr) {
   LanczosWindow win(op, c, m, fp32, s, ctl.dgks_eta);
   win.start_from(d_start);
   const int k = o.eigenpairs;
@@ -553,6 +561,15 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
     if (!pending || (launched < win.posted() + kLag && !full)) continue;
     pending = false;
     const auto st = win.collect();
+    if (start_norm2) {
+      const double nn = *start_norm2;
+      start_norm2 = nullptr;
+      if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) {
+        out.bad_start = true;
+        out.start_norm = std::sqrt(nn);
+        return;
+      }
+    }
     if (st.rewound) {
       interval = std::max(1, interval / 2);
       next_sync = win.launched() + interval;
@@ -741,16 +758,19 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
   grow(scratch.b->slots, 2);
   double* start = scratch.b->start.data();
   double* sl = scratch.b->slots.data();
-  if (d_initial) QMBED_HIP(hipMemcpyAsync(start, d_initial, e * n * 8, hipMemcpyDeviceToDevice, s));
-  else deterministic_start_device(start, n, o.seed, s, c);
-  {
-    lk::nrm2sq_to_slot(c, start, n, sl, scratch.b->ws, s);
-    double nn = 0;
-    QMBED_HIP(hipMemcpyAsync(&nn, sl, 8, hipMemcpyDeviceToHost, s));
-    QMBED_HIP(hipStreamSynchronize(s));
-    if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) throw non_convergence(0, std::sqrt(nn));
-    lk::normalize(c, start, sl, n, start, nullptr, nullptr, s);
+  // the normalized start; its ‖·‖² is read back without a wait and checked
+  // at the first window's first read-back (a zero or non-finite start then
+  // fails as NonConvergence after 0 iterations)
+  const double* raw = d_initial;
+  if (!d_initial) {
+    deterministic_start_device(start, n, o.seed, s, c);
+    raw = start;
   }
+  lk::nrm2sq_to_slot(c, raw, n, sl, scratch.b->ws, s);
+  scratch.b->ensure_pin(1);
+  const double* start_norm2 = scratch.b->pin;
+  QMBED_HIP(hipMemcpyAsync(scratch.b->pin, sl, 8, hipMemcpyDeviceToHost, s));
+  lk::normalize(c, raw, sl, n, start, nullptr, nullptr, s);
 
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
   // Windows: without krylov_dimension, min(max(8k+64, 256), N, budget),
@@ -770,7 +790,9 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
     for (;;) {
       WindowResult w2;
       try {
-        run_window(op, c, oo, int(sub), d_start, use_fp32, ctl, accept_tol, w2, s);
+        run_window(op, c, oo, int(sub), d_start, use_fp32, ctl, accept_tol, w2, s, start_norm2);
+        start_norm2 = nullptr;
+        if (w2.bad_start) return w2;
         w2.es.iterations += spent;
         w2.es.fp32_iterations += fp32_spent;
         return w2;
@@ -807,6 +829,7 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
   };
 
   WindowResult wr = run_expanding(start, fp32, o.tolerance, o.max_iterations);
+  if (wr.bad_start) throw non_convergence(0, wr.start_norm);
   vec_out = std::move(wr.vectors);
   return wr.es;
 }

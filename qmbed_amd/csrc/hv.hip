@@ -1176,32 +1176,87 @@ __global__ void __launch_bounds__(kBlock) hv_stream_batch_kernel(const BatchSect
 // Per-row diagonal of the current amplitude tables (gather layout: one
 // double per entry, or (re, im) when complex): out[row] = Σ re, out[n + row]
 // = Σ im over the diagonal groups, in group order.
+// pattern_of without branches: the five site fields of the packed word (a
+// group has at most five sites; absent fields are zero) and a mask of the ns
+// low bits
+__device__ __forceinline__ int pattern_fixed(uint32_t r, uint32_t packed) {
+  int pat = 0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) pat |= int(__builtin_amdgcn_ubfe(r, (packed >> (3 + 5 * i)) & 31u, 1)) << i;
+  return pat & ((1 << (packed & 7u)) - 1);
+}
+
 __global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp, double* out) {
-  // the amplitude table and the diagonal groups' (sites, offset) words in
-  // LDS: every row reads n_diag of them at row-dependent offsets
+  // The amplitude table and the diagonal groups' (sites, offset) words in
+  // LDS. Every row reads n_diag entries at row-dependent offsets; the group
+  // words arrive as vector registers, so the pattern extraction runs on the
+  // vector ALU (the per-CU scalar unit would otherwise serialize it).
   __shared__ double s_amp[2 * kMaxAmps];
-  __shared__ uint32_t s_meta[2 * kMaxGroups];
+  __shared__ uint2 s_grp[kMaxGroups];
   const int namp = (complex_amp ? 2 : 1) * a.n_amp;
   for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
-  for (int g = threadIdx.x; g < a.n_diag; g += kBlock) {
-    s_meta[2 * g] = a.meta[4 * g + 1];
-    s_meta[2 * g + 1] = a.meta[4 * g + 3];
-  }
+  for (int g = threadIdx.x; g < a.n_diag; g += kBlock) s_grp[g] = make_uint2(a.meta[4 * g + 1], a.meta[4 * g + 3]);
   __syncthreads();
+  const int G = a.n_diag;
   for (size_t row = size_t(blockIdx.x) * kBlock + threadIdx.x; row < a.n; row += size_t(gridDim.x) * kBlock) {
     const uint32_t r = a.states[row];
-    double re = 0, im = 0;
-    for (int g = 0; g < a.n_diag; ++g) {
-      const int off = int(s_meta[2 * g + 1]) + pattern_of(r, s_meta[2 * g]);
+    double re0 = 0, re1 = 0, im0 = 0, im1 = 0;
+    int g = 0;
+    for (; g + 2 <= G; g += 2) {
+      const uint2 m0 = s_grp[g], m1 = s_grp[g + 1];
+      const int o0 = int(m0.y) + pattern_fixed(r, m0.x);
+      const int o1 = int(m1.y) + pattern_fixed(r, m1.x);
       if (complex_amp) {
-        re += s_amp[2 * off];
-        im += s_amp[2 * off + 1];
+        re0 += s_amp[2 * o0];
+        im0 += s_amp[2 * o0 + 1];
+        re1 += s_amp[2 * o1];
+        im1 += s_amp[2 * o1 + 1];
       } else {
-        re += s_amp[off];
+        re0 += s_amp[o0];
+        re1 += s_amp[o1];
       }
     }
-    out[row] = re;
-    out[a.n + row] = im;
+    if (g < G) {
+      const uint2 m0 = s_grp[g];
+      const int o0 = int(m0.y) + pattern_fixed(r, m0.x);
+      if (complex_amp) {
+        re0 += s_amp[2 * o0];
+        im0 += s_amp[2 * o0 + 1];
+      } else {
+        re0 += s_amp[o0];
+      }
+    }
+    out[row] = re0 + re1;
+    out[a.n + row] = im0 + im1;
+  }
+}
+
+// Per-row diagonal of a family at coefficients c from the stored diagonals
+// S_p of its parts (p = 0 the static part, then one per component; each S_p
+// as re[n], im[n] in the gather form the H·v kernels read, the conjugate of
+// the part's diagonal): the diagonal is linear in the coefficients, so the
+// stored form at c is S_0 + Σ_p conj(c_p) S_p. With real part tables the S_p
+// are real and only their real halves are read.
+constexpr int kMaxDiagParts = 8;
+struct DiagCoefs {
+  int parts = 0;
+  int real_parts = 0;   // every S_p real
+  int complex_out = 0;  // the stored imaginary half is used (else zero)
+  double re[kMaxDiagParts], im[kMaxDiagParts];  // conj(c_p), p >= 1
+};
+
+__global__ void __launch_bounds__(kBlock) diag_combine_kernel(const double* __restrict__ S, size_t n, DiagCoefs k,
+                                                              double* __restrict__ out) {
+  for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kBlock) {
+    double re = S[i], im = k.real_parts ? 0.0 : S[n + i];
+    for (int p = 1; p < k.parts; ++p) {
+      const double sr = S[2 * n * size_t(p) + i];
+      const double si = k.real_parts ? 0.0 : S[2 * n * size_t(p) + n + i];
+      re += k.re[p] * sr - k.im[p] * si;
+      im += k.re[p] * si + k.im[p] * sr;
+    }
+    out[i] = re;
+    out[n + i] = k.complex_out ? im : 0.0;
   }
 }
 
@@ -1246,6 +1301,10 @@ struct Operator::DeviceTables {
   int n_codes = 1;
   // per-row diagonal of the current coefficients (re[n], im[n])
   DeviceArray<double> diag;
+  // families: the stored per-row diagonal of every part (parts × (re[n],
+  // im[n])), from which each evaluation combines the diagonal
+  DeviceArray<double> part_diag;
+  int diag_parts = 0;
   // pinned staging of the amplitude uploads and the event after their copies
   PinnedArray<double> stage;
   hipEvent_t staged = nullptr;
@@ -1323,7 +1382,7 @@ static void tune_lanes(Operator& op);
 // forward table c·a as (re, im). For families the device arrays are sized
 // for (re, im) so that an evaluation may switch between the layouts.
 static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, const std::vector<FlipGroup>& off,
-                              hipStream_t s) {
+                              hipStream_t s, const std::vector<cplx>* coefs = nullptr, bool real_parts = false) {
   auto& t = *op.tables_;
   std::vector<double> amp, famp;
   for (auto* v : {&diag, &off}) {
@@ -1352,7 +1411,23 @@ static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, 
   // instead of a walk over the diagonal groups in every H·v)
   const SpinBasis1D& b = op.basis();
   const size_t n = b.len();
-  if (n > 0 && n <= (size_t(1) << 28)) {
+  if (n > 0 && n <= (size_t(1) << 28) && coefs && t.diag_parts == int(coefs->size()) + 1) {
+    if (t.diag.size() != 2 * n) t.diag.resize(2 * n);
+    DiagCoefs k;
+    k.parts = t.diag_parts;
+    k.real_parts = real_parts ? 1 : 0;
+    k.complex_out = t.complex_coef ? 1 : 0;
+    k.re[0] = 1.0;
+    k.im[0] = 0.0;
+    for (size_t p = 0; p < coefs->size(); ++p) {
+      k.re[p + 1] = (*coefs)[p].real();
+      k.im[p + 1] = -(*coefs)[p].imag();
+    }
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(diag_combine_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, t.part_diag.data(), n, k,
+                       t.diag.data());
+    QMBED_HIP(hipGetLastError());
+  } else if (n > 0 && n <= (size_t(1) << 28)) {
     if (t.diag.size() != 2 * n) t.diag.resize(2 * n);
     HvArgs a = t.args(b);
     const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
@@ -1476,14 +1551,25 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   const size_t nparts = components_.size() + 1;
   op->part_hermitian_.assign(nparts, true);
   if (basis_->len() > 0) {
+    // the parts' stored diagonals, kept when they fit (at most 8 parts and
+    // 8 GiB), so that an evaluation combines them instead of walking the
+    // diagonal groups of every row again
+    const size_t n = basis_->len();
+    const bool keep = nparts <= size_t(kMaxDiagParts) && n <= (size_t(1) << 28) &&
+                      nparts * 2 * n * sizeof(double) <= (size_t(8) << 30);
+    if (keep) t->part_diag.resize(nparts * 2 * n);
     for (size_t part = 0; part < nparts; ++part) {
       std::vector<FlipGroup> d = op->diag_, o = op->off_;
       for (auto* v : {&d, &o})
         for (auto& g : *v) g.amp = g.parts[part];
       t->complex_coef = !(op->parts_real_ && basis_->real_characters());
       upload_amplitudes(*op, d, o, nullptr);
+      if (keep)
+        QMBED_HIP(hipMemcpy(t->part_diag.data() + part * 2 * n, t->diag.data(), 2 * n * sizeof(double),
+                            hipMemcpyDeviceToDevice));
       op->part_hermitian_[part] = current_hermitian(*op, nullptr);
     }
+    if (keep) t->diag_parts = int(nparts);
     if (checks_.hermiticity && !op->part_hermitian_[0]) throw non_hermitian();
   }
   // initial evaluation: defaults, and zero for required components (an
@@ -1543,7 +1629,7 @@ void Operator::evaluate_coefficients(const std::vector<cplx>& c, hipStream_t s) 
   auto& t = *tables_;
   real_ = parts_real_ && real_c && basis_->real_characters();
   t.complex_coef = !real_;
-  upload_amplitudes(*this, diag_, off_, s);
+  upload_amplitudes(*this, diag_, off_, s, &c, parts_real_);
   if (basis_->len() == 0) return;
   bool all_parts = real_c;
   for (bool h : part_hermitian_) all_parts = all_parts && h;

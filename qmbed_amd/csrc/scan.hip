@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <future>
 #include <cmath>
 
 #include "internal.hpp"
@@ -576,8 +577,26 @@ std::vector<double> WarmStartScan::predict(const double* points, int n) {
   return std::vector<double>(Y.begin(), Y.end());
 }
 
+std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* points, int n, const EigshOptions& o,
+                                                                 const EigshControls& ctl, Start start) {
+  std::vector<PointResult> out;
+  out.reserve(size_t(std::max(n, 0)));
+  if (start != Start::SnapshotRR || !trained_ || n < 2) {
+    for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
+    return out;
+  }
+  auto ahead = [this](const double* p) { return std::async(std::launch::async, [this, p] { return continuation(p); }); };
+  auto next = ahead(points);
+  for (int i = 0; i < n; ++i) {
+    const std::vector<double> y = next.get();
+    if (i + 1 < n) next = ahead(points + size_t(i + 1) * P_);
+    out.push_back(solve(points + size_t(i) * P_, o, ctl, start, &y));
+  }
+  return out;
+}
+
 WarmStartScan::PointResult WarmStartScan::solve(const double* point, const EigshOptions& o, const EigshControls& ctl,
-                                                Start start) {
+                                                Start start, const std::vector<double>* y) {
   const bool warm = start != Start::Plain;
   PointResult r;
   auto t0 = std::chrono::steady_clock::now();
@@ -596,8 +615,13 @@ WarmStartScan::PointResult WarmStartScan::solve(const double* point, const Eigsh
     if (dc_.size() != size_t(ns_)) dc_.resize(static_cast<size_t>(ns_));
     std::vector<double> c(static_cast<size_t>(ns_));
     if (start == Start::SnapshotRR) {
-      c = continuation(point);
-      QMBED_HIP(hipMemcpyAsync(dc_.data(), c.data(), c.size() * 8, hipMemcpyHostToDevice, s_));
+      c = y ? *y : continuation(point);
+      if (c.size() != size_t(ns_)) throw internal_state("continuation coefficients of the wrong length");
+      // staged through pinned memory (the previous point's copy completed
+      // before its solve returned)
+      cpin_.ensure(c.size());
+      std::copy(c.begin(), c.end(), cpin_.p);
+      QMBED_HIP(hipMemcpyAsync(dc_.data(), cpin_.p, c.size() * 8, hipMemcpyHostToDevice, s_));
       lk::gemm_vc(false, S_.data(), N, ns_, dc_.data(), 1, v0_.data(), s_);
     } else {
       std::vector<float> x(static_cast<size_t>(P_));

@@ -65,8 +65,15 @@ class WarmStartScan {
     Network,      // v₀ = S·MLP(c)
     SnapshotRR,   // v₀ = S·y, y the lowest eigenvector of Sᵀ H(c) S
   };
-  // ground state of one point from the chosen start
-  PointResult solve(const double* point, const EigshOptions& o, const EigshControls& ctl, Start start);
+  // ground state of one point from the chosen start; `y`: the point's
+  // continuation coefficients when already computed (snapshot start)
+  PointResult solve(const double* point, const EigshOptions& o, const EigshControls& ctl, Start start,
+                    const std::vector<double>* y = nullptr);
+  // ground states of n points in turn; with the snapshot start the host
+  // eigensolve of point i + 1's projected family runs on a helper thread
+  // while point i is solved on the device
+  std::vector<PointResult> solve_many(const double* points, int n, const EigshOptions& o, const EigshControls& ctl,
+                                      Start start);
   // network outputs (snapshot coefficients) of n points
   std::vector<double> predict(const double* points, int n);
 
@@ -100,6 +107,7 @@ class WarmStartScan {
   // per-solve device buffers, kept between solves
   DeviceArray<float> dx_, dy_;
   DeviceArray<double> dc_, v0_;
+  PinnedArray<double> cpin_;  // staging of the start coefficients
   DeviceArray<float> params_;  // network parameters
   std::vector<double> center_, scale_;
 };
