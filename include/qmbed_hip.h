@@ -184,6 +184,13 @@ int qmbed_hip_mlp_forward(int in, int h1, int h2, int out, const float* params, 
 int qmbed_hip_mlp_train(int in, int h1, int h2, int out, float* params, const float* x, const float* t, int n,
                         int epochs, float learning_rate, float* grad, float* loss_history);
 
+/* C = Aᵀ B for column-major device arrays A (n × p) and B (n × q) of real
+ * doubles, n ≫ p, q: the tall-skinny product of the snapshot projections
+ * (no counterpart in the reference; exported for tests and device callers).
+ * C is p × q, column-major, on the device; the call synchronizes. Return 0
+ * on success, -1 on failure. */
+int qmbed_hip_gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C);
+
 void* qmbed_hip_malloc(size_t bytes);
 void qmbed_hip_free(void* ptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device */

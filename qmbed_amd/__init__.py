@@ -91,6 +91,8 @@ def library():
     L.qmbed_hip_malloc.argtypes = [ctypes.c_size_t]
     L.qmbed_hip_free.argtypes = [c_void_p]
     L.qmbed_hip_memcpy.argtypes = [c_void_p, c_void_p, ctypes.c_size_t, c_int]
+    L.qmbed_hip_gemm_atb.restype = c_int
+    L.qmbed_hip_gemm_atb.argtypes = [c_void_p, ctypes.c_size_t, c_int, c_void_p, c_int, c_void_p]
     L.qmbed_hip_device_count.restype = c_int
     _lib = L
     return L

@@ -16,6 +16,7 @@
 
 #include "../../include/qmbed_hip.h"
 #include "internal.hpp"
+#include "lanczos_kernels.hpp"
 #include "json.hpp"
 #include "scan.hpp"
 
@@ -1180,6 +1181,17 @@ int qmbed_hip_memcpy(void* dst, const void* src, size_t bytes, int kind) {
   return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : -1;
 }
 int qmbed_hip_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+
+int qmbed_hip_gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C) {
+  try {
+    if (!A || !B || !C || p < 0 || q < 0) return -1;
+    DeviceArray<double> part;
+    lk::gemm_atb(A, n, p, B, q, C, part, nullptr);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+  } catch (...) {
+    return -1;
+  }
+}
 int qmbed_hip_device_count(void) { return device_count(); }
 
 }  // extern "C"

@@ -375,6 +375,82 @@ __global__ __launch_bounds__(kT) void k_gemm_vc(const E* __restrict__ V, size_t 
   }
 }
 
+// Tall-skinny Aᵀ B, one 32 × 32 output tile per blockIdx.y: the block's
+// chunks of 64 rows of both operands are staged in LDS (coalesced column
+// loads; absent rows and columns read as zero); four groups of 64 threads
+// take every fourth row of a chunk, each thread a 4 × 4 register tile of the
+// output; the groups' sums meet in LDS and the block writes its partial
+// tile to partial[(blockIdx.y · gridDim.x + blockIdx.x) · 1024 + i·32 + j].
+constexpr int kAtbTile = 32, kAtbRows = 64;
+__global__ __launch_bounds__(kT) void k_atb(const double* __restrict__ A, size_t n, int p, const double* __restrict__ B,
+                                            int q, int tiles_j, double* __restrict__ partial) {
+  // the operand tiles, then (after the last chunk) the groups' sums
+  constexpr int kLd = kAtbTile + 1;
+  __shared__ double sm[2 * kAtbRows * kLd];
+  static_assert(2 * kAtbRows * kLd >= 4 * 16 * 64, "LDS of the group sums");
+  double* sa = sm;
+  double* sb = sm + kAtbRows * kLd;
+  const int ti = blockIdx.y / tiles_j, tj = blockIdx.y % tiles_j;
+  const int t = threadIdx.x, grp = t >> 6, lane = t & 63;
+  const int i0 = (lane >> 3) * 4, j0 = (lane & 7) * 4;
+  double acc[4][4] = {};
+  const size_t chunks = (n + kAtbRows - 1) / kAtbRows;
+  for (size_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const size_t r0 = ch * kAtbRows;
+    for (int e = t; e < kAtbRows * kAtbTile; e += kT) {
+      const int rr = e % kAtbRows, cc = e / kAtbRows;
+      const size_t r = r0 + size_t(rr);
+      const int ca = ti * kAtbTile + cc, cb = tj * kAtbTile + cc;
+      sa[rr * kLd + cc] = (r < n && ca < p) ? A[size_t(ca) * n + r] : 0.0;
+      sb[rr * kLd + cc] = (r < n && cb < q) ? B[size_t(cb) * n + r] : 0.0;
+    }
+    __syncthreads();
+    for (int rr = grp; rr < kAtbRows; rr += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = sa[rr * kLd + i0 + u];
+        b[u] = sb[rr * kLd + j0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+    __syncthreads();
+  }
+  double* red = sm;  // [grp][16][64]; the loop ended with a barrier
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) red[(grp * 16 + u * 4 + v) * 64 + lane] = acc[u][v];
+  __syncthreads();
+  if (grp == 0) {
+    double* out = partial + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * (kAtbTile * kAtbTile);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int k = u * 4 + v;
+        out[(i0 + u) * kAtbTile + j0 + v] =
+            ((red[k * 64 + lane] + red[(16 + k) * 64 + lane]) + red[(32 + k) * 64 + lane]) + red[(48 + k) * 64 + lane];
+      }
+  }
+}
+
+// C[i + p·j] = Σ_b partial of tile (i/32, j/32), block b (in block order)
+__global__ __launch_bounds__(kT) void k_atb_reduce(const double* __restrict__ partial, int blocks, int p, int q,
+                                                   int tiles_j, double* __restrict__ C) {
+  const int e = blockIdx.x * kT + threadIdx.x;
+  if (e >= p * q) return;
+  const int i = e % p, j = e / p;
+  const int tile = (i / kAtbTile) * tiles_j + j / kAtbTile;
+  const int k = (i % kAtbTile) * kAtbTile + j % kAtbTile;
+  double s = 0;
+  for (int b = 0; b < blocks; ++b) s += partial[(size_t(tile) * blocks + b) * (kAtbTile * kAtbTile) + k];
+  C[e] = s;
+}
+
 template <typename E>
 __global__ __launch_bounds__(kT) void k_to32(const E* x, size_t n, float* y) {
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
@@ -472,6 +548,20 @@ void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, voi
       }
     });
   }
+}
+
+void gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C, DeviceArray<double>& partial,
+              hipStream_t s) {
+  if (p <= 0 || q <= 0) return;
+  const int tiles_i = (p + kAtbTile - 1) / kAtbTile, tiles_j = (q + kAtbTile - 1) / kAtbTile;
+  const size_t chunks = std::max<size_t>(1, (n + kAtbRows - 1) / kAtbRows);
+  // about 512 blocks in all, at least one chunk each
+  const int bx = int(std::min<size_t>(chunks, size_t(std::max(1, 512 / (tiles_i * tiles_j)))));
+  const size_t need = size_t(tiles_i) * tiles_j * bx * kAtbTile * kAtbTile;
+  if (partial.size() < need) partial.resize(need);
+  hipLaunchKernelGGL(k_atb, dim3(bx, tiles_i * tiles_j), dim3(kT), 0, s, A, n, p, B, q, tiles_j, partial.data());
+  hipLaunchKernelGGL(k_atb_reduce, dim3((p * q + kT - 1) / kT), dim3(kT), 0, s, partial.data(), bx, p, q, tiles_j, C);
+  QMBED_HIP(hipGetLastError());
 }
 
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s) {

@@ -25,6 +25,11 @@ void normalize(bool c, const void* w, const double* nrm2_slot, size_t n, void* o
                hipStream_t s);
 // Y = V C (n×m · m×k)
 void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, void* Y, hipStream_t s);
+// C = Aᵀ B for real column-major A (n × p) and B (n × q), n ≫ p, q: the
+// p × q result (column-major, device) from per-block partial products
+// summed in a fixed order; `partial` is grown as needed.
+void gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C, DeviceArray<double>& partial,
+              hipStream_t s);
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s);
 void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s, const int* gate = nullptr);
 // slot = |y - lam x|

@@ -229,7 +229,13 @@ double WarmStartScan::gauge_sign(const double* d_psi) {
 void WarmStartScan::build_space(const double* psi, int n, std::vector<double>& Lm, std::vector<int>& sel) {
   const size_t N = op_.dim();
   DeviceArray<double> dG(size_t(n) * size_t(n));
-  {
+  if (n <= 64) {
+    // few states: the tall-skinny kernel (a library GEMM of this shape may
+    // run without splitting the long dimension)
+    DeviceArray<double> part;
+    lk::gemm_atb(psi, N, n, psi, n, dG.data(), part, s_);
+    QMBED_HIP(hipStreamSynchronize(s_));
+  } else {
     rocblas_handle h;
     if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
     rocblas_set_stream(h, s_);
@@ -373,21 +379,14 @@ void WarmStartScan::greedy_space(const double* psi, int n, DeviceArray<double>& 
 void WarmStartScan::project_family() {
   const size_t N = op_.dim();
   {
-    rocblas_handle h;
-    if (rocblas_create_handle(&h) != rocblas_status_success) throw internal_state("rocblas_create_handle failed");
-    rocblas_set_stream(h, s_);
-    DeviceArray<double> HS(N * size_t(ns_)), dP(size_t(ns_) * ns_);
+    DeviceArray<double> HS(N * size_t(ns_)), dP(size_t(ns_) * ns_), atb;
     proj_.assign(size_t(P_) + 1, std::vector<double>(size_t(ns_) * ns_, 0.0));
     for (int part = 0; part <= P_; ++part) {
       std::vector<cplx> c(size_t(P_), cplx(0, 0));
       if (part > 0) c[size_t(part - 1)] = cplx(1, 0);
       op_.evaluate_coefficients(c, s_);
       for (int b = 0; b < ns_; ++b) op_.apply(S_.data() + N * size_t(b), HS.data() + N * size_t(b), s_);
-      const double one = 1.0, zero = 0.0;
-      rocblas_status st = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ns_, ns_,
-                                        rocblas_int(N), &one, S_.data(), rocblas_int(N), HS.data(), rocblas_int(N),
-                                        &zero, dP.data(), ns_);
-      if (st != rocblas_status_success) throw internal_state("rocblas_dgemm failed");
+      lk::gemm_atb(S_.data(), N, ns_, HS.data(), ns_, dP.data(), atb, s_);
       QMBED_HIP(hipMemcpyAsync(proj_[size_t(part)].data(), dP.data(), size_t(ns_) * ns_ * 8, hipMemcpyDeviceToHost,
                                s_));
       QMBED_HIP(hipStreamSynchronize(s_));
@@ -405,15 +404,11 @@ void WarmStartScan::project_family() {
     {
       DeviceArray<double> ones(N), dg(static_cast<size_t>(ns_));
       hipLaunchKernelGGL(k_fill, dim3(grid_for(N)), dim3(kT), 0, s_, ones.data(), N, 1.0);
-      const double one = 1.0, zero = 0.0;
-      rocblas_status st = rocblas_dgemv(h, rocblas_operation_transpose, rocblas_int(N), ns_, &one, S_.data(),
-                                        rocblas_int(N), ones.data(), 1, &zero, dg.data(), 1);
-      if (st != rocblas_status_success) throw internal_state("rocblas_dgemv failed");
+      lk::gemm_atb(S_.data(), N, ns_, ones.data(), 1, dg.data(), atb, s_);
       gauge_.assign(size_t(ns_), 0.0);
       QMBED_HIP(hipMemcpyAsync(gauge_.data(), dg.data(), size_t(ns_) * 8, hipMemcpyDeviceToHost, s_));
       QMBED_HIP(hipStreamSynchronize(s_));
     }
-    rocblas_destroy_handle(h);
   }
 }
 

@@ -125,6 +125,24 @@ def test_learned_warm_start_scan(qa, oracle):
     assert ri < wi
 
 
+@pytest.mark.parametrize("n,p,q", [(1, 1, 1), (1000, 3, 5), (70000, 32, 32), (100001, 33, 65), (5000, 64, 1),
+                                   (4097, 70, 40)])
+def test_gemm_atb_matches_numpy(qa, n, p, q):
+    # the tall-skinny Aᵀ B of the snapshot projections and small Gram
+    # matrices, on ragged shapes (partial row chunks and output tiles)
+    rng = np.random.default_rng(n + 7 * p + 13 * q)
+    A = rng.standard_normal((p, n))  # column-major n × p: column c is A[c]
+    B = rng.standard_normal((q, n))
+    bufs = [qa.DeviceBuffer(x.nbytes) for x in (A, B)] + [qa.DeviceBuffer(p * q * 8)]
+    bufs[0].upload(A)
+    bufs[1].upload(B)
+    assert qa.library().qmbed_hip_gemm_atb(bufs[0].ptr, n, p, bufs[1].ptr, q, bufs[2].ptr) == 0
+    C = bufs[2].download(np.empty(p * q)).reshape(q, p).T  # column-major p × q
+    np.testing.assert_allclose(C, A @ B.T, rtol=0, atol=1e-14 * n + 1e-15)
+    for b in bufs:
+        b.free()
+
+
 @pytest.mark.parametrize("space", ["greedy", "principal"])
 def test_projected_family_matches_oracle(qa, oracle, space):
     # the continuation start at a training point reproduces that point's
