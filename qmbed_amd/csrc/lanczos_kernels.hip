@@ -159,11 +159,11 @@ __device__ __forceinline__ E three_apply(const Three<E>& t, E x, size_t i, doubl
 // c < m, Σ_i conj(V[c][i]) w₀[i], and when mm == m + 1 also Σ_i |w₀[i]|² at
 // c = m (‖w₀‖, the DGKS reference norm). w₀ is w after the three-term part
 // (first pass) or w itself. Block (x, y) takes the row tiles x, x + gridDim.x,
-// … of kT·kRPT rows and the columns [y·CT, y·CT + CT); every load of a tile
+// … of kT·RPT rows and the columns [y·CT, y·CT + CT); every load of a tile
 // is unconditional (row and column indices are clamped, out-of-range rows
-// carry w₀ = 0), so the CT·kRPT loads of a thread are in flight together.
+// carry w₀ = 0), so the CT·RPT loads of a thread are in flight together.
 // Output: partial[x·ld + col_off + c].
-template <typename E, int CT>
+template <typename E, int CT, int RPT = kRPT>
 __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, int m, int mm, const E* __restrict__ w,
                                              E* __restrict__ partial, int ld, int col_off, const int* gate,
                                              Three<E> t) {
@@ -175,33 +175,56 @@ __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, 
   if (t.vj) three_coefs(t, shd, &bc, alpha, beta);
   const int c0 = blockIdx.y * CT;
   const int nc = min(CT, mm - c0);
-  const size_t tiles = (n + size_t(kT * kRPT) - 1) / size_t(kT * kRPT);
+  const size_t tiles = (n + size_t(kT * RPT) - 1) / size_t(kT * RPT);
   E acc[CT];
 #pragma unroll
   for (int c = 0; c < CT; ++c) acc[c] = zero<E>();
+  // v_j and v_{j−1} are the columns m − 1 and m − 2 of V: when they lie in
+  // this block's tile the three-term part takes the loaded values instead of
+  // reading them again
+  const int cj = m - 1 - c0, cjm1 = m - 2 - c0;
+  const bool tile_j = t.vj && t.vj == V + size_t(m - 1) * n && cj >= 0 && cj < CT;
+  const bool tile_jm1 = t.vjm1 && m >= 2 && t.vjm1 == V + size_t(m - 2) * n && cjm1 >= 0 && cjm1 < CT;
   for (size_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
-    const size_t r0 = tt * (kT * kRPT) + threadIdx.x;
-    size_t ic[kRPT];
-    E wv[kRPT];
+    const size_t r0 = tt * (kT * RPT) + threadIdx.x;
+    size_t ic[RPT];
 #pragma unroll
-    for (int rr = 0; rr < kRPT; ++rr) {
+    for (int rr = 0; rr < RPT; ++rr) {
       const size_t i = r0 + size_t(rr) * kT;
       ic[rr] = i < n ? i : n - 1;
-      wv[rr] = w[ic[rr]];
-      if (t.vj) wv[rr] = three_apply(t, wv[rr], ic[rr], alpha, beta);
-      if (i >= n) wv[rr] = zero<E>();
+    }
+    E vv[CT][RPT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const size_t base = size_t(min(c0 + c, m - 1)) * n;
+#pragma unroll
+      for (int rr = 0; rr < RPT; ++rr) vv[c][rr] = V[base + ic[rr]];
+    }
+    E wv[RPT];
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      E x = w[ic[rr]];
+      if (t.vj) {
+        E vj = zero<E>(), vm = zero<E>();
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+          if (c == cj) vj = vv[c][rr];
+          if (c == cjm1) vm = vv[c][rr];
+        }
+        if (!tile_j) vj = t.vj[ic[rr]];
+        x = sub(x, scale(vj, alpha));
+        if (t.vjm1) {
+          if (!tile_jm1) vm = t.vjm1[ic[rr]];
+          x = sub(x, scale(vm, beta));
+        }
+      }
+      wv[rr] = r0 + size_t(rr) * kT < n ? x : zero<E>();
     }
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
-      const int col = c0 + c;
-      const bool is_w = col >= m;
-      const size_t base = size_t(min(col, m - 1)) * n;
+      const bool is_w = c0 + c >= m;
 #pragma unroll
-      for (int rr = 0; rr < kRPT; ++rr) {
-        E v = V[base + ic[rr]];
-        if (is_w) v = wv[rr];
-        acc[c] = add(acc[c], cmul(v, wv[rr]));
-      }
+      for (int rr = 0; rr < RPT; ++rr) acc[c] = add(acc[c], cmul(is_w ? wv[rr] : vv[c][rr], wv[rr]));
     }
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -260,19 +283,36 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
   }
   for (int c = threadIdx.x; c < mc; c += kT) sh_h[c] = hc[c];
   __syncthreads();
+  // v_j and v_{j−1} are normally the last two columns of this chunk: their
+  // loads serve both the subtraction and the three-term part
+  const bool last_j = t.vj && t.vj == V + size_t(mc - 1) * n;
+  const bool last_jm1 = last_j && t.vjm1 && mc >= 2 && t.vjm1 == V + size_t(mc - 2) * n;
+  const int cend = mc - (last_j ? (last_jm1 ? 2 : 1) : 0);
   double s = 0;
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
     E acc = zero<E>();
     int c = 0;
-    for (; c + 4 <= mc; c += 4) {
+    for (; c + 4 <= cend; c += 4) {
       acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
       acc = add(acc, mul(V[size_t(c + 1) * n + i], sh_h[c + 1]));
       acc = add(acc, mul(V[size_t(c + 2) * n + i], sh_h[c + 2]));
       acc = add(acc, mul(V[size_t(c + 3) * n + i], sh_h[c + 3]));
     }
-    for (; c < mc; ++c) acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
+    for (; c < cend; ++c) acc = add(acc, mul(V[size_t(c) * n + i], sh_h[c]));
     E x = w[i];
-    if (t.vj) x = three_apply(t, x, i, alpha, beta);
+    if (last_j) {
+      const E vj = V[size_t(mc - 1) * n + i];
+      E vm = zero<E>();
+      if (last_jm1) {
+        vm = V[size_t(mc - 2) * n + i];
+        acc = add(acc, mul(vm, sh_h[mc - 2]));
+      }
+      acc = add(acc, mul(vj, sh_h[mc - 1]));
+      x = sub(x, scale(vj, alpha));
+      if (t.vjm1) x = sub(x, scale(last_jm1 ? vm : t.vjm1[i], beta));
+    } else if (t.vj) {
+      x = three_apply(t, x, i, alpha, beta);
+    }
     const E v = sub(x, acc);
     w[i] = v;
     s += sq(v);
@@ -584,8 +624,8 @@ void residual_to_slot(bool c, const void* y, const void* x, const double* lam_sl
 // Row blocks of the overlap kernel: about 1024 blocks in all (four per CU,
 // so the column loads of sixteen waves per CU are in flight), at least 16
 // per column group, at most one per row tile.
-static int dots_row_blocks(size_t n, int groups) {
-  const size_t tiles = (n + size_t(kT * kRPT) - 1) / size_t(kT * kRPT);
+static int dots_row_blocks(size_t n, int groups, int rpt) {
+  const size_t tiles = (n + size_t(kT * rpt) - 1) / size_t(kT * rpt);
   const int want = std::max(16, 1024 / std::max(groups, 1));
   return int(std::min<size_t>(tiles, size_t(want)));
 }
@@ -595,8 +635,13 @@ static int dots_row_blocks(size_t n, int groups) {
 // issue no loads for absent columns.
 template <typename E>
 static int dots_ct(int mm) {
-  return mm <= 4 ? 4 : (mm <= 8 || dots_tile<E>() == 8) ? 8 : 16;
+  if (dots_tile<E>() == 8) return mm <= 4 ? 4 : 8;
+  return mm <= 4 ? 4 : mm <= 8 ? 8 : mm <= 12 ? 12 : 16;
 }
+
+// rows per thread of the overlap kernel (4 for the narrow tiles measured
+// slower: fewer blocks in flight)
+static int dots_rpt(int) { return kRPT; }
 
 template <typename E>
 static int chunk_groups(int mm) {
@@ -608,7 +653,7 @@ static int chunk_groups(int mm) {
 template <typename E>
 static void pass_layout(size_t n, int m, bool with_ww, int& rb, int& ld) {
   const int widest = std::min(m, kMaxCols) + (with_ww ? 1 : 0);
-  rb = dots_row_blocks(n, chunk_groups<E>(widest));
+  rb = dots_row_blocks(n, chunk_groups<E>(widest), dots_rpt(dots_ct<E>(widest)));
   ld = m + 1;
 }
 
@@ -646,11 +691,15 @@ void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Wor
       const E* Vc = cE<E>(V) + size_t(c0) * n;
       switch (dots_ct<E>(mm)) {
         case 4:
-          hipLaunchKernelGGL((k_dots<E, 4>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w), mE<E>(ws.partial.data()),
-                             ld, c0, gate, t);
+          hipLaunchKernelGGL((k_dots<E, 4>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w),
+                             mE<E>(ws.partial.data()), ld, c0, gate, t);
           break;
         case 8:
-          hipLaunchKernelGGL((k_dots<E, 8>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w), mE<E>(ws.partial.data()),
+          hipLaunchKernelGGL((k_dots<E, 8>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w),
+                             mE<E>(ws.partial.data()), ld, c0, gate, t);
+          break;
+        case 12:
+          hipLaunchKernelGGL((k_dots<E, 12>), grid, dim3(kT), 0, s, Vc, n, mc, mm, cE<E>(w), mE<E>(ws.partial.data()),
                              ld, c0, gate, t);
           break;
         default:
