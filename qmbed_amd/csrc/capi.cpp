@@ -235,6 +235,11 @@ SolverReq parse_solver(const Value& s) {
     if (c < 1 || c > 64) bad("check_every must lie in [1, 64]");
     r.ctl.check_every = int(c);
   }
+  if (auto* v = s.get("lockstep"); v && !v->is_null()) {
+    const long c = as_long(*v, "lockstep");
+    if (c < 0 || c > 8) bad("lockstep must lie in [0, 8]");
+    r.ctl.lockstep = int(c);
+  }
   if (auto* v = s.get("read_lag"); v && !v->is_null()) {
     const long c = as_long(*v, "read_lag");
     if (c < 0 || c > 8) bad("read_lag must lie in [0, 8]");

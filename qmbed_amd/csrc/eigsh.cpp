@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -147,6 +148,7 @@ struct WindowBuffers {
   int device = 0;
   DeviceArray<double> V, w, slots;
   DeviceArray<double> start, coef;  // eigsh start vector; Ritz coefficients
+  DeviceArray<double> ritz;         // Ritz vectors of a convergence test
   DeviceArray<float> x32, y32;
   lk::Workspace ws;
   double* pin = nullptr;  // pinned read-back area
@@ -196,7 +198,7 @@ static void release_buffers(std::unique_ptr<WindowBuffers> b) {
   std::lock_guard<std::mutex> lk(g_pool_mu);
   int same = 0;
   for (auto& e : g_pool) same += e->device == b->device ? 1 : 0;
-  if (same < 2) g_pool.push_back(std::move(b));
+  if (same < 12) g_pool.push_back(std::move(b));
 }
 
 // A pooled buffer set held for a scope; returned after the stream drains.
@@ -260,8 +262,27 @@ class LanczosWindow {
   double* slots() const { return buf_->slots.data(); }
 
   // Install the start vector (device, FP64, already normalized) as column 0.
+  // External matvec mode (windows solved in lockstep): the caller launches
+  // H·v_j into w() before each step, reading the window's interleaved lane
+  // of v_j, which this window keeps current (start and every normalization);
+  // apply64 computes the FP64 products of the Rayleigh quotients.
+  void set_external(const lk::Interleave& il, std::function<void(const double*, double*)> apply64) {
+    if (c_) throw internal_state("lockstep windows are real");
+    external_ = true;
+    il_ = il;
+    apply64_ = std::move(apply64);
+  }
+  bool external() const { return external_; }
+  // a matvec launched by the caller for this window's next step
+  void count_external(bool fp32) {
+    ++matvecs_;
+    if (fp32) ++matvecs32_;
+  }
+  const int* gate() const { return halt(); }
+
   void start_from(const double* d_v0) {
     QMBED_HIP(hipMemcpyAsync(col(0), d_v0, esz() * n_ * 8, hipMemcpyDeviceToDevice, s_));
+    if (external_ && (il_.x64 || il_.x32)) lk::scatter_lane(col(0), n_, il_, s_);
     use32_ = false;
     alpha_.clear();
     beta_.clear();
@@ -276,11 +297,12 @@ class LanczosWindow {
     if (on == use32_) return;
     if (on) {
       if (!fp32_) throw internal_state("FP32 matvecs were not enabled for this window");
-      if (launched_ < m_) lk::to_f32(c_, col(launched_), n_, buf_->x32.data(), s_);
+      if (launched_ < m_ && !external_) lk::to_f32(c_, col(launched_), n_, buf_->x32.data(), s_);
     }
     use32_ = on;
   }
   bool fp32_active() const { return use32_; }
+  bool fp32_enabled() const { return fp32_; }
 
   void matvec(int j, const int* gate) {
     if (use32_) {
@@ -308,7 +330,7 @@ class LanczosWindow {
   void step() {
     const int j = launched_;
     const int* gate = halt();
-    matvec(j, gate);
+    if (!external_) matvec(j, gate);
     const lk::ThreeTerm three{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr,
                               slots() + j};
     lk::lanczos_alpha(c_, col(j), w(), n_, buf_->ws, s_, gate);
@@ -425,9 +447,12 @@ class LanczosWindow {
     return true;
   }
 
-  // Assemble Ritz vectors of the selected pairs into `ritz_buf` (k columns),
-  // normalize, and compute FP64 Rayleigh quotients and residuals.
-  std::vector<Candidate> ritz(int k, SpectrumTarget target, double shift, DeviceArray<double>& ritz_buf) {
+  // Assemble Ritz vectors of the selected pairs (k columns, in the pooled
+  // buffer ritz_vectors()), normalize, and compute FP64 Rayleigh quotients
+  // and residuals.
+  const double* ritz_vectors() const { return buf_->ritz.data(); }
+  std::vector<Candidate> ritz(int k, SpectrumTarget target, double shift) {
+    DeviceArray<double>& ritz_buf = buf_->ritz;
     const int sz = size_;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
     std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
@@ -439,7 +464,7 @@ class LanczosWindow {
       for (int jj = 0; jj < sz; ++jj) C[e * (q * sz + jj)] = vecs[q][size_t(jj)];
     grow(buf_->coef, C.size());  // rayleigh() below synchronizes before C dies
     QMBED_HIP(hipMemcpyAsync(buf_->coef.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
-    ritz_buf.resize(e * n_ * vals.size());
+    grow(ritz_buf, e * n_ * vals.size());
     lk::gemm_vc(c_, V(), n_, sz, buf_->coef.data(), int(vals.size()), ritz_buf.data(), s_);
     auto rr = rayleigh(ritz_buf.data(), int(vals.size()));
     std::vector<Candidate> out;
@@ -458,6 +483,7 @@ class LanczosWindow {
       lk::nrm2sq_to_slot(c_, y, n_, sl, buf_->ws, s_);
       lk::normalize(c_, y, sl, n_, y, nullptr, nullptr, s_);
       if (c_) op_.apply(reinterpret_cast<const cplx*>(y), reinterpret_cast<cplx*>(w()), s_);
+      else if (apply64_) apply64_(y, w());
       else op_.apply(y, w(), s_);
       ++matvecs64_;
       lk::redot_to_slot(c_, y, w(), n_, sl + 1, buf_->ws, s_);
@@ -477,7 +503,8 @@ class LanczosWindow {
   void finish(int j, bool second_pass) {
     const bool next = j + 1 < m_;
     lk::lanczos_finish(c_, w(), n_, h_, j, eta_, slots() + m_ + j, halt(), flags_ + j, next ? col(j + 1) : nullptr,
-                       (next && use32_) ? buf_->x32.data() : nullptr, buf_->ws, s_, second_pass);
+                       (next && use32_ && !external_) ? buf_->x32.data() : nullptr, buf_->ws, s_, second_pass,
+                       (next && external_) ? &il_ : nullptr);
   }
   void count_passes(const int* hf, int steps) {
     passes_ = 0;
@@ -508,6 +535,9 @@ class LanczosWindow {
   long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0, matvecs32_ = 0;
   bool broke_ = false;
   bool use32_ = false;
+  bool external_ = false;
+  lk::Interleave il_;
+  std::function<void(const double*, double*)> apply64_;
 };
 
 struct WindowResult {
@@ -523,134 +553,171 @@ struct WindowResult {
   double start_norm = 0;
 };
 
-// One Lanczos window with in-window convergence checks.
-// Throws NonConvergence (as Error) like the reference when the full window
-// does not converge.
+// One Lanczos window with in-window convergence checks, as a state machine:
+// each tick launches the next step (when the window has room) and then
+// posts, collects and acts on read-backs. run_window ticks one window to the
+// end; windows solved in lockstep are ticked side by side after the caller
+// has launched their shared matvec. A window throws NonConvergence (as
+// Error) like the reference when the full window does not converge.
 // start_norm2: a pinned host copy of the start's ‖·‖², queued before the
 // window on the same stream and checked at its first read-back (nullptr:
 // already checked).
-void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
-                const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s,
-                const double* start_norm2 = nullptr) {
-  LanczosWindow win(op, c, m, fp32, s, ctl.dgks_eta);
-  win.start_from(d_start);
-  const int k = o.eigenpairs;
-  int next_check = std::max(k + 1, ctl.check_every);
-  // Read-backs are posted at convergence checks, at the window's end, and
-  // otherwise every `interval` steps (halved after a rewind, so frequent
-  // second passes void few speculative steps). A posted read-back is
-  // collected kLag steps later, when the device has long finished the copy,
-  // so the device never waits for the host; a convergence found at a
-  // collected point costs at most kLag extra steps.
-  const int kLag = ctl.read_lag > 0 ? ctl.read_lag : (op.dim() >= (size_t(1) << 20) ? 1 : 2);
-  int interval = std::max(1, ctl.check_every);
-  int next_sync = interval;
-  bool pending = false, pending_conv = false;
-  for (;;) {
-    if (win.launched() < m) win.step();
-    const int launched = win.launched();
-    const bool full = launched == m;
-    if (!pending) {
-      const bool conv = ctl.early_stop && launched >= next_check && launched > k;
-      if (full || conv || launched >= next_sync) {
-        win.post();
-        pending = true;
-        pending_conv = conv;
+class WindowRun {
+ public:
+  WindowRun(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
+            const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s,
+            const double* start_norm2 = nullptr, const lk::Interleave* il = nullptr,
+            std::function<void(const double*, double*)> apply64 = {})
+      : op_(op), c_(c), o_(o), m_(m), ctl_(ctl), accept_tol_(accept_tol), out_(out), s_(s),
+        start_norm2_(start_norm2), win_(op, c, m, fp32, s, ctl.dgks_eta) {
+    if (il) win_.set_external(*il, std::move(apply64));
+    win_.start_from(d_start);
+    k_ = o.eigenpairs;
+    next_check_ = std::max(k_ + 1, ctl.check_every);
+    // Read-backs are posted at convergence checks, at the window's end, and
+    // otherwise every `interval` steps (halved after a rewind, so frequent
+    // second passes void few speculative steps). A posted read-back is
+    // collected kLag steps later, when the device has long finished the copy,
+    // so the device never waits for the host; a convergence found at a
+    // collected point costs at most kLag extra steps.
+    lag_ = ctl.read_lag > 0 ? ctl.read_lag : (op.dim() >= (size_t(1) << 20) ? 1 : 2);
+    interval_ = std::max(1, ctl.check_every);
+    next_sync_ = interval_;
+  }
+  bool done() const { return done_; }
+  // the Ritz vectors of the window's test, in candidate order, into the
+  // result (the window's own buffer is pooled)
+  void copy_vectors(const std::vector<Candidate>& cands) {
+    const size_t e = c_ ? 2 : 1;
+    const size_t n = op_.dim();
+    out_.vectors.resize(e * n * cands.size());
+    for (size_t q = 0; q < cands.size(); ++q)
+      QMBED_HIP(hipMemcpyAsync(out_.vectors.data() + e * n * q, win_.ritz_vectors() + e * n * cands[q].column,
+                               e * n * 8, hipMemcpyDeviceToDevice, s_));
+    QMBED_HIP(hipStreamSynchronize(s_));
+  }
+  // this tick launches a step
+  bool stepping() const { return !done_ && win_.launched() < m_; }
+  LanczosWindow& window() { return win_; }
+
+  void tick() {
+    if (done_) return;
+    if (win_.launched() < m_) win_.step();
+    const int launched = win_.launched();
+    const bool full = launched == m_;
+    if (!pending_) {
+      const bool conv = ctl_.early_stop && launched >= next_check_ && launched > k_;
+      if (full || conv || launched >= next_sync_) {
+        win_.post();
+        pending_ = true;
+        pending_conv_ = conv;
       }
     }
-    if (!pending || (launched < win.posted() + kLag && !full)) continue;
-    pending = false;
-    const auto st = win.collect();
-    if (start_norm2) {
-      const double nn = *start_norm2;
-      start_norm2 = nullptr;
+    if (!pending_ || (launched < win_.posted() + lag_ && !full)) return;
+    pending_ = false;
+    const auto st = win_.collect();
+    if (start_norm2_) {
+      const double nn = *start_norm2_;
+      start_norm2_ = nullptr;
       if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) {
-        out.bad_start = true;
-        out.start_norm = std::sqrt(nn);
+        out_.bad_start = true;
+        out_.start_norm = std::sqrt(nn);
+        done_ = true;
         return;
       }
     }
     if (st.rewound) {
-      interval = std::max(1, interval / 2);
-      next_sync = win.launched() + interval;
-      continue;
+      interval_ = std::max(1, interval_ / 2);
+      next_sync_ = win_.launched() + interval_;
+      return;
     }
-    interval = std::min(std::max(1, ctl.check_every), interval * 2);
-    next_sync = win.posted() + interval;
+    interval_ = std::min(std::max(1, ctl_.check_every), interval_ * 2);
+    next_sync_ = win_.posted() + interval_;
     const bool at_end = st.ended;
     // FP32 matvecs once the wanted pairs have converged far enough that the
     // single-precision product errors, weighted by the small components of
     // the Ritz vectors on the remaining Krylov vectors, stay well below the
     // tolerance (relaxed inexact Krylov)
-    if (fp32 && !win.fp32_active() && !at_end &&
-        win.max_estimate(k, o.target, o.shift) <= ctl.fp32_switch * accept_tol)
-      win.set_fp32(true);
-    if (!(at_end || pending_conv)) continue;
-    const int sz = win.size();
+    if (win_.fp32_enabled() && !win_.fp32_active() && !at_end &&
+        win_.max_estimate(k_, o_.target, o_.shift) <= ctl_.fp32_switch * accept_tol_)
+      win_.set_fp32(true);
+    if (!(at_end || pending_conv_)) return;
+    const int sz = win_.size();
     if (!at_end) {
-      if (!win.estimates_converged(k, o.target, o.shift, accept_tol, 0.5)) {
-        next_check = sz + ctl.check_every;
-        continue;
+      if (!win_.estimates_converged(k_, o_.target, o_.shift, accept_tol_, 0.5)) {
+        next_check_ = sz + ctl_.check_every;
+        return;
       }
     }
-    if (sz <= k) {
-      out.iterations = win.matvecs();
+    if (sz <= k_) {
+      out_.iterations = win_.matvecs();
       throw non_convergence(sz, INFINITY);
     }
-    auto cands = win.ritz(k, o.target, o.shift, out.vectors);
+    auto cands = win_.ritz(k_, o_.target, o_.shift);
     std::stable_sort(cands.begin(), cands.end(), [&](const Candidate& a, const Candidate& b) {
-      return target_before(o.target == SpectrumTarget::BothEnds ? SpectrumTarget::SmallestAlgebraic : o.target,
-                           a.value, b.value, o.shift);
+      return target_before(o_.target == SpectrumTarget::BothEnds ? SpectrumTarget::SmallestAlgebraic : o_.target,
+                           a.value, b.value, o_.shift);
     });
     double fail = 0;
     for (auto& cd : cands)
-      if (cd.residual > accept_tol * std::max(std::fabs(cd.value), 1.0)) fail = std::max(fail, cd.residual);
+      if (cd.residual > accept_tol_ * std::max(std::fabs(cd.value), 1.0)) fail = std::max(fail, cd.residual);
     if (fail > 0 && !at_end) {
       // A Ritz pair whose estimate says converged but which fails the FP64
       // test after FP32 products: the single-precision errors already sit
       // in this window's tridiagonal matrix and more steps cannot remove
       // them, so the window ends here and the caller restarts in FP64 from
       // these Ritz vectors (accurate to about the FP32 floor).
-      if (win.matvecs32() > 0) {
-        out.iterations = win.matvecs();
-        out.fp32_matvecs = win.matvecs32();
+      if (win_.matvecs32() > 0) {
+        out_.iterations = win_.matvecs();
+        out_.fp32_matvecs = win_.matvecs32();
+        copy_vectors(cands);
         throw non_convergence(sz, fail);
       }
-      next_check = sz + ctl.check_every;
-      continue;
+      next_check_ = sz + ctl_.check_every;
+      return;
     }
-    out.iterations = win.matvecs();
+    out_.iterations = win_.matvecs();
     if (fail > 0) {
-      out.fp32_matvecs = win.matvecs32();
+      out_.fp32_matvecs = win_.matvecs32();
+      copy_vectors(cands);
       Error e = non_convergence(sz, fail);
       throw e;
     }
-    // reorder the Ritz columns to the candidate order
-    bool identity = true;
-    for (size_t q = 0; q < cands.size(); ++q) identity = identity && cands[q].column == int(q);
-    if (!identity) {
-      const size_t e = c ? 2 : 1;
-      const size_t n = op.dim();
-      DeviceArray<double> ordered(e * n * cands.size());
-      for (size_t q = 0; q < cands.size(); ++q)
-        QMBED_HIP(hipMemcpyAsync(ordered.data() + e * n * q, out.vectors.data() + e * n * cands[q].column, e * n * 8,
-                                 hipMemcpyDeviceToDevice, s));
-      QMBED_HIP(hipStreamSynchronize(s));
-      out.vectors = std::move(ordered);
-    }
+    if (ctl_.keep_vectors) copy_vectors(cands);
     for (auto& cd : cands) {
-      out.es.eigenvalues.push_back(cd.value);
-      out.es.residuals.push_back(cd.residual);
+      out_.es.eigenvalues.push_back(cd.value);
+      out_.es.residuals.push_back(cd.residual);
     }
-    out.es.iterations = sz;
-    out.es.reorthogonalization_passes = win.passes();
-    out.es.conditional_second_passes = win.seconds();
-    out.es.converged = true;
-    out.es.fp32_iterations = win.matvecs32();
-    out.es.fp64_iterations = win.matvecs() - win.matvecs32() + win.matvecs64();
-    out.ok = true;
-    return;
+    out_.es.iterations = sz;
+    out_.es.reorthogonalization_passes = win_.passes();
+    out_.es.conditional_second_passes = win_.seconds();
+    out_.es.converged = true;
+    out_.es.fp32_iterations = win_.matvecs32();
+    out_.es.fp64_iterations = win_.matvecs() - win_.matvecs32() + win_.matvecs64();
+    out_.ok = true;
+    done_ = true;
   }
+
+ private:
+  const Operator& op_;
+  bool c_;
+  const EigshOptions& o_;
+  int m_;
+  const EigshControls& ctl_;
+  double accept_tol_;
+  WindowResult& out_;
+  hipStream_t s_;
+  const double* start_norm2_;
+  LanczosWindow win_;
+  int k_ = 1, next_check_ = 0, lag_ = 2, interval_ = 1, next_sync_ = 1;
+  bool pending_ = false, pending_conv_ = false, done_ = false;
+};
+
+void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
+                const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s,
+                const double* start_norm2 = nullptr) {
+  WindowRun run(op, c, o, m, d_start, fp32, ctl, accept_tol, out, s, start_norm2);
+  while (!run.done()) run.tick();
 }
 
 // Dense route for small problems: GPU materialization + rocSOLVER heevd.
@@ -718,6 +785,97 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
 
 
 }  // namespace
+
+std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& o, const EigshControls& ctl,
+                                           const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
+                                           DeviceArray<float>& lanes32, hipStream_t s) {
+  const int nb = int(pts.size());
+  const size_t n = op.dim();
+  o.validate(n);
+  if (nb < 1 || nb > 8) throw internal_state("a lockstep batch holds 1 to 8 windows");
+  if (n <= 256 || o.target == SpectrumTarget::Shift) throw internal_state("lockstep windows need the Krylov route");
+  const long m = o.krylov_dimension
+                     ? std::min<long>(std::min<long>(*o.krylov_dimension, o.max_iterations), long(n))
+                     : std::min<long>(std::min<long>(std::max<long>(8L * o.eigenpairs + 64, 256), long(n)),
+                                      o.max_iterations);
+  if (m <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
+  const bool fp32 = ctl.precision == MatvecPrecision::FP32;
+  if (lanes64.size() < 8 * n) lanes64.resize(8 * n);
+  if (lanes32.size() < 8 * n) lanes32.resize(8 * n);
+  std::vector<WindowResult> res(static_cast<size_t>(nb));
+  std::vector<std::unique_ptr<WindowRun>> runs;
+  const lk::Interleave no_lanes;
+  for (int b = 0; b < nb; ++b) {
+    const LockstepPoint pt = pts[size_t(b)];
+    auto apply64 = [&op, pt, s](const double* x, double* y) {
+      const double* am[1] = {pt.amps};
+      const double* dg[1] = {pt.diag};
+      double* ys[1] = {y};
+      op.apply_scan_batch(x, false, 1, am, dg, ys, nullptr, s);
+    };
+    runs.emplace_back(new WindowRun(op, false, o, int(m), pt.start, fp32, ctl, o.tolerance, res[size_t(b)], s, nullptr,
+                                    &no_lanes, apply64));
+  }
+  std::vector<char> failed(static_cast<size_t>(nb), 0);
+  // every tick of a window launches a step or collects a read-back, so a
+  // window ends within a few ticks per step of its window size
+  const long max_ticks = 8 * (m + 8);
+  for (long tick = 0;; ++tick) {
+    if (tick > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
+    // the windows stepping now, packed into consecutive lanes
+    const double* src[8] = {};
+    const double* amps[8] = {};
+    const double* diags[8] = {};
+    double* ys[8] = {};
+    const int* gates[8] = {};
+    int k = 0;
+    bool active = false, all32 = true;
+    for (int b = 0; b < nb; ++b) {
+      auto& r = *runs[size_t(b)];
+      if (failed[size_t(b)] || r.done()) continue;
+      active = true;
+      if (!r.stepping()) continue;
+      auto& w = r.window();
+      src[k] = w.col(w.launched());
+      amps[k] = pts[size_t(b)].amps;
+      diags[k] = pts[size_t(b)].diag;
+      ys[k] = w.w();
+      gates[k] = w.gate();
+      all32 = all32 && w.fp32_active();
+      ++k;
+    }
+    if (!active) break;
+    if (k > 0) {
+      // one launch for every stepping window: FP32 lanes once all of them
+      // take FP32 products, FP64 lanes before
+      const int width = k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
+      lk::interleave(src, n, width, all32 ? nullptr : lanes64.data(), all32 ? lanes32.data() : nullptr, s);
+      op.apply_scan_batch(all32 ? static_cast<const void*>(lanes32.data()) : static_cast<const void*>(lanes64.data()),
+                          all32, width, amps, diags, ys, gates, s);
+      for (int b = 0; b < nb; ++b) {
+        auto& r = *runs[size_t(b)];
+        if (!failed[size_t(b)] && !r.done() && r.stepping()) r.window().count_external(all32);
+      }
+    }
+    for (int b = 0; b < nb; ++b) {
+      auto& r = *runs[size_t(b)];
+      if (failed[size_t(b)] || r.done()) continue;
+      try {
+        r.tick();
+      } catch (const Error&) {
+        failed[size_t(b)] = 1;  // the caller solves this point on its own
+      }
+    }
+  }
+  QMBED_HIP(hipStreamSynchronize(s));
+  std::vector<LockstepResult> out(static_cast<size_t>(nb));
+  for (int b = 0; b < nb; ++b) {
+    out[size_t(b)].ok = !failed[size_t(b)] && res[size_t(b)].ok;
+    if (out[size_t(b)].ok) out[size_t(b)].es = res[size_t(b)].es;
+  }
+  return out;
+}
+
 
 // Balanced combination Σ_j e^{2πij/J}/√J v_j (src/solve/mod.rs:708-727),
 // real part kept for real problems.

@@ -1144,6 +1144,91 @@ __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const SX* _
   stream_rows<S, CMODE, SYM, SHIFT, NV, SX, IL>(a, x, y, blockIdx.x);
 }
 
+// Scan points of one real family in lockstep: per point an amplitude table
+// (gather form, real), a per-row diagonal, an output vector and a gate.
+constexpr int kMaxScanBatch = 8;
+struct ScanBatch {
+  const double* amp[kMaxScanBatch];  // per lane: a table of n_amp entries
+  const double* diag[kMaxScanBatch];
+  double* y[kMaxScanBatch];
+  const int* gate[kMaxScanBatch];
+};
+
+// y_b = H(c_b) x_b for the B lanes of an interleaved input x[row·B + b], on
+// the compact stream (real amplitudes, no character phases). One lane per
+// row as in stream_rows: the decode of an entry (target, amplitude entry,
+// orbit code, sign) is shared by the B points, each point scales by its own
+// table (LDS) and the B inputs of a target row are contiguous, so one
+// gathered line serves all points.
+template <typename SX, int B, int SYM>
+__global__ void __launch_bounds__(kBlock) hv_scan_batch_kernel(HvArgs a, ScanBatch sb, const SX* __restrict__ x) {
+  extern __shared__ double smem_d[];
+  double* s_amp = smem_d;                // B tables of n_amp
+  double* s_sq = s_amp + B * a.n_amp;    // 65
+  double* s_crsq = s_sq + 65;            // 16
+  bool on[B];
+  bool any = false;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    on[b] = sb.y[b] != nullptr && !(sb.gate[b] && *sb.gate[b]);
+    any = any || on[b];
+  }
+  if (!any) return;
+  for (int i = threadIdx.x; i < B * a.n_amp; i += kBlock) {
+    const int b = i / a.n_amp;
+    s_amp[i] = on[b] ? sb.amp[b][i - b * a.n_amp] : 0.0;
+  }
+  for (int i = threadIdx.x; i < 65; i += kBlock) s_sq[i] = sqrt(double(i));
+  for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t n_slices = (a.n + kSlice - 1) / kSlice;
+  if (slice >= n_slices) return;  // whole wavefronts leave together
+  const bool live = row < a.n;
+  const uint32_t base = a.slice_ptr[slice];
+  const uint32_t width = (a.slice_ptr[slice + 1] - base) / kSlice;
+  double sq_r = 1;
+  if (SYM && live) sq_r = s_sq[a.orbits[row]];
+  const uint32_t own = live ? row : 0u;
+  double acc[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) acc[b] = on[b] ? sb.diag[b][own] * double(x[size_t(own) * B + b]) : 0.0;
+  constexpr int E = 4;
+  const uint32_t* words = a.stream + base + lane;
+  const int so = a.c_b, sc = so + a.c_ob, ss = sc + a.c_cb;
+  const uint32_t imask = (1u << a.c_b) - 1u;
+  for (uint32_t k0 = 0; k0 < width; k0 += E) {
+    uint32_t ent[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t w = words[size_t(kSlice) * min(k0 + e, width - 1)];
+      ent[e] = k0 + e < width ? w : 0xFFFFFFFFu;
+    }
+    SX xv[E][B];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const size_t idx = ent[e] != 0xFFFFFFFFu ? size_t(ent[e] & imask) : 0;
+#pragma unroll
+      for (int b = 0; b < B; ++b) xv[e][b] = x[idx * B + b];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t w = ent[e];
+      if (w == 0xFFFFFFFFu) continue;
+      const int off = a.c_off_base + int(field(w, so, a.c_ob));
+      double scale = field(w, ss, a.c_sb) ? -1.0 : 1.0;
+      if (SYM) scale *= sq_r * s_crsq[field(w, sc, a.c_cb)];
+#pragma unroll
+      for (int b = 0; b < B; ++b) acc[b] += s_amp[b * a.n_amp + off] * scale * double(xv[e][b]);
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int b = 0; b < B; ++b)
+    if (on[b]) sb.y[b][row] = acc[b];
+}
+
 // One sector of a sector-batched action: its tables and where its rows sit
 // in the concatenated (complex) vectors.
 struct BatchSector {
@@ -2069,6 +2154,112 @@ bool Operator::apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nv
     }
   }
   return true;
+}
+
+bool Operator::scan_batch_supported(hipStream_t s) const {
+  auto& t = *tables_;
+  if (!parameterized() || dim() == 0 || !parts_real_ || !basis_->real_characters() || basis_->geometry().kmom > 0)
+    return false;
+  for (bool h : part_hermitian_)
+    if (!h) return false;
+  if (t.diag_parts != int(components_.size()) + 1) return false;
+  if (t.stream.size() == 0 && !build_stream(const_cast<Operator&>(*this), s)) return false;
+  // LDS of the widest launch: the lane tables and the two small tables
+  return t.compact && sizeof(double) * (size_t(kMaxScanBatch) * size_t(t.n_amp) + 81) <= 48 * 1024;
+}
+
+int Operator::scan_batch_amp_count() const { return tables_->n_amp; }
+
+void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, double* d_amps, double* d_diags,
+                                 hipStream_t s) const {
+  auto& t = *tables_;
+  const size_t n = dim();
+  const int na = t.n_amp;
+  std::vector<double> amps(coefs.size() * size_t(na));
+  for (size_t b = 0; b < coefs.size(); ++b) {
+    const auto& c = coefs[b];
+    if (c.size() != components_.size())
+      throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for " +
+                               std::to_string(components_.size()) + " components");
+    for (const auto& v : c)
+      if (!std::isfinite(v.real()) || v.imag() != 0.0)
+        throw invalid_options("batched scan coefficients must be finite and real");
+    // the gather table in upload order (diagonal groups, then off-diagonal)
+    size_t k = 0;
+    for (auto* v : {&diag_, &off_})
+      for (auto& g : *v)
+        for (size_t pat = 0; pat < g.parts[0].size(); ++pat) {
+          cplx a = g.parts[0][pat];
+          for (size_t p = 0; p < c.size(); ++p) a += c[p] * g.parts[p + 1][pat];
+          amps[b * size_t(na) + k++] = a.real();
+        }
+    if (k != size_t(na)) throw internal_state("batched scan amplitude table of the wrong length");
+    DiagCoefs kc;
+    kc.parts = t.diag_parts;
+    kc.real_parts = 1;
+    kc.complex_out = 0;
+    kc.re[0] = 1.0;
+    kc.im[0] = 0.0;
+    for (size_t p = 0; p < c.size(); ++p) {
+      kc.re[p + 1] = c[p].real();
+      kc.im[p + 1] = 0.0;
+    }
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(diag_combine_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, t.part_diag.data(), n, kc,
+                       d_diags + b * 2 * n);
+    QMBED_HIP(hipGetLastError());
+  }
+  QMBED_HIP(hipMemcpyAsync(d_amps, amps.data(), amps.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  QMBED_HIP(hipStreamSynchronize(s));  // amps dies here
+}
+
+void Operator::apply_scan_batch(const void* x, bool fp32, int width, const double* const* amps,
+                                const double* const* diags, double* const* y, const int* const* gates,
+                                hipStream_t s) const {
+  auto& t = *tables_;
+  if (!(width == 1 || width == 2 || width == 4 || width == 8))
+    throw internal_state("batched scan width must be 1, 2, 4 or 8");
+  if (!t.compact || t.stream.size() == 0) throw internal_state("batched scan action without a compact stream");
+  HvArgs a = t.args(*basis_);
+  a.stream = t.stream.data();
+  a.slice_ptr = t.slice_ptr.data();
+  a.img = nullptr;
+  if (a.n == 0) return;
+  ScanBatch sb{};
+  for (int b = 0; b < width; ++b) {
+    sb.amp[b] = amps[b];
+    sb.diag[b] = diags[b];
+    sb.y[b] = y[b];
+    sb.gate[b] = gates ? gates[b] : nullptr;
+  }
+  const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
+  const dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
+  const size_t sm = sizeof(double) * (size_t(width) * size_t(a.n_amp) + 81);
+  const int sym = sym_code(*basis_);
+#define QMBED_SCAN_LAUNCH(SX, W)                                                                                   \
+  switch (sym) {                                                                                                  \
+    case 0: hipLaunchKernelGGL((hv_scan_batch_kernel<SX, W, 0>), grid, block, sm, s, a, sb, static_cast<const SX*>(x)); break; \
+    case 1: hipLaunchKernelGGL((hv_scan_batch_kernel<SX, W, 1>), grid, block, sm, s, a, sb, static_cast<const SX*>(x)); break; \
+    case 2: hipLaunchKernelGGL((hv_scan_batch_kernel<SX, W, 2>), grid, block, sm, s, a, sb, static_cast<const SX*>(x)); break; \
+    default: hipLaunchKernelGGL((hv_scan_batch_kernel<SX, W, 3>), grid, block, sm, s, a, sb, static_cast<const SX*>(x)); break; \
+  }
+  if (fp32) {
+    switch (width) {
+      case 1: QMBED_SCAN_LAUNCH(float, 1) break;
+      case 2: QMBED_SCAN_LAUNCH(float, 2) break;
+      case 4: QMBED_SCAN_LAUNCH(float, 4) break;
+      default: QMBED_SCAN_LAUNCH(float, 8) break;
+    }
+  } else {
+    switch (width) {
+      case 1: QMBED_SCAN_LAUNCH(double, 1) break;
+      case 2: QMBED_SCAN_LAUNCH(double, 2) break;
+      case 4: QMBED_SCAN_LAUNCH(double, 4) break;
+      default: QMBED_SCAN_LAUNCH(double, 8) break;
+    }
+  }
+#undef QMBED_SCAN_LAUNCH
+  QMBED_HIP(hipGetLastError());
 }
 
 bool prepare_interleaved_action(const Operator& op, hipStream_t s) {

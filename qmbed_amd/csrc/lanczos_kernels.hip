@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
 template <typename E, typename F>
 __global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n, const double* __restrict__ partial2,
                                                int nb2, const E* __restrict__ h, int j, double eta, double* beta,
-                                               int* halt, int* hist, E* out, F* out32, int pass2) {
+                                               int* halt, int* hist, E* out, F* out32, int pass2, Interleave il) {
   if (!pass2 && *halt) return;
   __shared__ double sh[kT / 64];
   __shared__ double s_after;
@@ -367,6 +367,10 @@ __global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n
         out32[2 * i] = float(v.x);
         out32[2 * i + 1] = float(v.y);
       }
+    }
+    if constexpr (sizeof(E) == 8) {
+      if (il.x64) il.x64[i * il.stride + il.lane] = re_of(v);
+      if (il.x32) il.x32[i * il.stride + il.lane] = float(re_of(v));
     }
   }
 }
@@ -739,10 +743,67 @@ void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h
 }
 
 void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,
-                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass) {
+                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass,
+                    const Interleave* il) {
+  const Interleave none;
+  if (il && c) throw internal_state("interleaved copies are for real vectors");
   QMBED_DISPATCH(c, hipLaunchKernelGGL((k_finish<E, float>), dim3(nblocks(n)), dim3(kT), 0, s, cE<E>(w), n,
                                        ws.partial2.data(), sub_blocks(n), cE<E>(d_h), j, eta, beta, halt, hist,
-                                       mE<E>(out), out32, second_pass ? 1 : 0));
+                                       mE<E>(out), out32, second_pass ? 1 : 0, il ? *il : none));
+}
+
+__global__ __launch_bounds__(kT) void k_scatter_lane(const double* __restrict__ x, size_t n, Interleave il) {
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    if (il.x64) il.x64[i * il.stride + il.lane] = x[i];
+    if (il.x32) il.x32[i * il.stride + il.lane] = float(x[i]);
+  }
+}
+
+struct Lanes8 {
+  const double* src[8];
+};
+// one row per thread: the W values of a row are gathered from the W
+// columns (coalesced reads) and stored side by side as whole vectors
+template <typename T, int W>
+__global__ __launch_bounds__(kT) void k_interleave(Lanes8 l, size_t n, T* __restrict__ x) {
+  struct alignas(sizeof(T) * W) Row {
+    T v[W];
+  };
+  Row* out = reinterpret_cast<Row*>(x);
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+    Row r;
+#pragma unroll
+    for (int b = 0; b < W; ++b) r.v[b] = l.src[b] ? T(l.src[b][i]) : T(0);
+    out[i] = r;
+  }
+}
+
+void interleave(const double* const* src, size_t n, int width, double* x64, float* x32, hipStream_t s) {
+  if (!(width == 1 || width == 2 || width == 4 || width == 8))
+    throw internal_state("interleave width must be 1, 2, 4 or 8");
+  if (!x64 == !x32) throw internal_state("interleave writes exactly one of the FP64 and FP32 lanes");
+  Lanes8 l{};
+  for (int b = 0; b < width; ++b) l.src[b] = src[b];
+  const dim3 g(nblocks(n)), t(kT);
+#define QMBED_IL(T, W, X) hipLaunchKernelGGL((k_interleave<T, W>), g, t, 0, s, l, n, X)
+  if (x64) {
+    if (width == 1) QMBED_IL(double, 1, x64);
+    else if (width == 2) QMBED_IL(double, 2, x64);
+    else if (width == 4) QMBED_IL(double, 4, x64);
+    else QMBED_IL(double, 8, x64);
+  } else {
+    if (width == 1) QMBED_IL(float, 1, x32);
+    else if (width == 2) QMBED_IL(float, 2, x32);
+    else if (width == 4) QMBED_IL(float, 4, x32);
+    else QMBED_IL(float, 8, x32);
+  }
+#undef QMBED_IL
+  QMBED_HIP(hipGetLastError());
+}
+
+void scatter_lane(const double* x, size_t n, const Interleave& il, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_lane, dim3(nblocks(n)), dim3(kT), 0, s, x, n, il);
+  QMBED_HIP(hipGetLastError());
 }
 
 }  // namespace lk

@@ -66,10 +66,27 @@ void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Wor
 // one reduction launch), then w ← w₀ − V h and per-block ‖w‖² partials
 void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h, Workspace& ws, hipStream_t s,
                       bool with_ww, const int* gate, const ThreeTerm* three);
+// One lane of interleaved copies of a real vector: x64[i·stride + lane] (FP64)
+// and x32[i·stride + lane] (FP32), either optional. Windows solved in
+// lockstep keep their next vectors side by side this way, so that one
+// gathered cache line serves every window's H·v.
+struct Interleave {
+  double* x64 = nullptr;
+  float* x32 = nullptr;
+  int stride = 1;
+  int lane = 0;
+};
+
 // DGKS decision with threshold eta (first pass), β_j = ‖w‖, out = w / β_j
-// (+ FP32 copy)
+// (+ FP32 copy, + interleaved copies of a real vector)
 void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,
-                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass);
+                    int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass,
+                    const Interleave* il = nullptr);
+// the interleaved copies of a real vector (one lane)
+void scatter_lane(const double* x, size_t n, const Interleave& il, hipStream_t s);
+// x[i·width + b] = src[b][i] for b < width (src[b] == nullptr: zero), as
+// FP64 (x64) or FP32 (x32); width ≤ 8
+void interleave(const double* const* src, size_t n, int width, double* x64, float* x32, hipStream_t s);
 
 }  // namespace lk
 }  // namespace qmbed

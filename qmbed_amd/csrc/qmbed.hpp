@@ -271,6 +271,24 @@ class Operator {
   // false (nothing launched) otherwise.
   bool apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nvec, hipStream_t s) const;
 
+  // Several evaluations of one real family at once (scan points solved in
+  // lockstep). Supported when the family is real with Hermitian parts, its
+  // sector needs no character phases, and the compiled compact stream and
+  // the parts' stored diagonals exist (the stream is built on demand).
+  bool scan_batch_supported(hipStream_t s) const;
+  // per point b < nb: the gather-form amplitude table (scan_batch_amp_count
+  // doubles at d_amps + b·count) and the per-row diagonal (re[N], im[N] at
+  // d_diags + b·2N) of the family at coefs[b] (real coefficients)
+  int scan_batch_amp_count() const;
+  void scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, double* d_amps, double* d_diags,
+                         hipStream_t s) const;
+  // y_b = H(c_b) x_b for the lanes b < width (1, 2, 4 or 8) of an input
+  // interleaved as x[row·width + b] (FP32 or FP64); y[b] == nullptr skips a
+  // lane, and a nonzero *gates[b] (optional) voids it like the single-vector
+  // gate. amps[b] / diags[b]: the lane's point tables from scan_batch_tables.
+  void apply_scan_batch(const void* x, bool fp32, int width, const double* const* amps, const double* const* diags,
+                        double* const* y, const int* const* gates, hipStream_t s) const;
+
   // per-row Σ|H_ij| over j != i merged per column, plus |H_ii - shift|;
   // returns (trace, max row sum) — ‖A-μI‖₁ for Hermitian A
   std::pair<cplx, double> shifted_trace_and_one_norm(hipStream_t s) const;
@@ -427,6 +445,12 @@ struct EigshControls {
   // for operators of at least 2^20 rows, whose steps outlast the host's work
   // on a read-back, and 2 below that
   int read_lag = 0;
+  // scans from the snapshot start: points solved in lockstep per batch (one
+  // shared H·v launch per step; 0 or 1 solves them one by one)
+  int lockstep = 8;
+  // return the eigenvectors (false: only eigenvalues and residuals, with no
+  // device allocation per solve)
+  bool keep_vectors = true;
 };
 
 struct Eigensystem {
@@ -440,6 +464,30 @@ struct Eigensystem {
   long fp32_iterations = 0;  // matvecs spent in the FP32 stage
   long fp64_iterations = 0;  // matvecs spent in FP64 (incl. polish)
 };
+
+// Lockstep Lanczos windows for several evaluations of one real family (scan
+// points): one window per point, all stepping together, with one
+// apply_scan_batch launch per step for every window's H·v (the windows'
+// next vectors interleaved, so the gathers of all points share cache lines).
+// Each window runs the single-window algorithm (same checks, FP32 schedule,
+// DGKS passes) at the default window size. Each step packs the next vectors
+// of the windows still stepping into consecutive lanes (launch width 1, 2, 4
+// or 8), so converged windows cost nothing. lanes64 / lanes32: the
+// interleaved inputs (grown as needed, kept by the caller between calls). A
+// point whose window fails comes back with ok == false, for the caller to
+// solve on its own.
+struct LockstepPoint {
+  const double* start;  // normalized FP64 start vector (device)
+  const double* amps;   // the point's amplitude table (device)
+  const double* diag;   // the point's per-row diagonal (device)
+};
+struct LockstepResult {
+  Eigensystem es;
+  bool ok = false;
+};
+std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& opts, const EigshControls& ctl,
+                                           const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
+                                           DeviceArray<float>& lanes32, hipStream_t s);
 
 // eigsh: QMBED src/solve/mod.rs:2335-2452 (dispatch), 1116-1456 (Lanczos).
 // `initial` is a host vector of length dim (nullptr: deterministic start).

@@ -580,6 +580,7 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
     return out;
   }
+  if (ctl.lockstep > 1 && op_.scan_batch_supported(s_)) return solve_lockstep(points, n, o, ctl);
   auto ahead = [this](const double* p) { return std::async(std::launch::async, [this, p] { return continuation(p); }); };
   auto next = ahead(points);
   for (int i = 0; i < n; ++i) {
@@ -590,8 +591,91 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
   return out;
 }
 
-WarmStartScan::PointResult WarmStartScan::solve(const double* point, const EigshOptions& o, const EigshControls& ctl,
+// Snapshot starts of up to ctl.lockstep points at a time, solved in lockstep
+// (eigsh_lockstep): the points' amplitude tables and diagonals side by side,
+// their starts S·y_b from one product, one shared H·v launch per step. A
+// point whose start vanishes or whose window fails is solved on its own.
+std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const double* points, int n, const EigshOptions& o,
+                                                                      const EigshControls& ctl) {
+  std::vector<PointResult> out(static_cast<size_t>(n));
+  const size_t N = op_.dim();
+  const int count = op_.scan_batch_amp_count();
+  const int W = std::min(ctl.lockstep, 8);
+  EigshOptions oo = o;
+  oo.eigenpairs = 1;
+  EigshControls ctl1 = ctl;
+  ctl1.keep_vectors = false;  // energies only
+  lk::Workspace ws;
+  DeviceArray<double> slot(2);
+  for (int i0 = 0; i0 < n; i0 += W) {
+    const int nb = std::min(W, n - i0);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::future<std::vector<double>>> fy;
+    for (int b = 0; b < nb; ++b) {
+      const double* p = points + size_t(i0 + b) * P_;
+      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+    }
+    std::vector<std::vector<double>> ys;
+    for (auto& f : fy) ys.push_back(f.get());
+    std::vector<int> lanes;  // points with a usable start
+    for (int b = 0; b < nb; ++b) {
+      double cn = 0;
+      for (double v : ys[size_t(b)]) cn += v * v;
+      if (std::isfinite(cn) && cn > 1.0e-6) lanes.push_back(b);
+    }
+    std::vector<char> solved(static_cast<size_t>(nb), 0);
+    if (lanes.size() >= 2) {
+      const int nl = int(lanes.size());
+      std::vector<std::vector<cplx>> coefs;
+      std::vector<double> Y(size_t(ns_) * size_t(nl));
+      for (int l = 0; l < nl; ++l) {
+        const int b = lanes[size_t(l)];
+        coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
+        std::copy(ys[size_t(b)].begin(), ys[size_t(b)].end(), Y.begin() + size_t(l) * ns_);
+      }
+      // the batch tables and starts live in grow-only members: no device
+      // allocation (and no synchronizing free) per batch
+      auto grow = [](DeviceArray<double>& d, size_t want) {
+        if (d.size() < want) d.resize(want);
+      };
+      grow(lk_amps_, size_t(nl) * size_t(count));
+      grow(lk_diags_, size_t(nl) * 2 * N);
+      grow(lk_starts_, size_t(nl) * N);
+      grow(lk_y_, Y.size());
+      op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
+      QMBED_HIP(hipMemcpyAsync(lk_y_.data(), Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s_));
+      lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
+      std::vector<LockstepPoint> pts;
+      for (int l = 0; l < nl; ++l) {
+        double* v = lk_starts_.data() + size_t(l) * N;
+        lk::nrm2sq_to_slot(false, v, N, slot.data(), ws, s_);
+        lk::normalize(false, v, slot.data(), N, v, nullptr, nullptr, s_);
+        pts.push_back({v, lk_amps_.data() + size_t(l) * count, lk_diags_.data() + size_t(l) * 2 * N});
+      }
+      const auto res = eigsh_lockstep(op_, oo, ctl1, pts, lk_lanes64_, lk_lanes32_, s_);
+      QMBED_HIP(hipStreamSynchronize(s_));  // Y dies here
+      const double dt = seconds_since(t0);
+      for (int l = 0; l < nl; ++l) {
+        if (!res[size_t(l)].ok) continue;
+        const int b = lanes[size_t(l)];
+        PointResult& r = out[size_t(i0 + b)];
+        r.es = res[size_t(l)].es;
+        r.warm = true;
+        r.seconds = dt / double(nl);
+        solved[size_t(b)] = 1;
+      }
+    }
+    for (int b = 0; b < nb; ++b)
+      if (!solved[size_t(b)])
+        out[size_t(i0 + b)] = solve(points + size_t(i0 + b) * P_, o, ctl, Start::SnapshotRR, &ys[size_t(b)]);
+  }
+  return out;
+}
+
+WarmStartScan::PointResult WarmStartScan::solve(const double* point, const EigshOptions& o, const EigshControls& ctl_in,
                                                 Start start, const std::vector<double>* y) {
+  EigshControls ctl = ctl_in;
+  ctl.keep_vectors = false;  // energies only
   const bool warm = start != Start::Plain;
   PointResult r;
   auto t0 = std::chrono::steady_clock::now();

@@ -74,6 +74,7 @@ class WarmStartScan {
   // while point i is solved on the device
   std::vector<PointResult> solve_many(const double* points, int n, const EigshOptions& o, const EigshControls& ctl,
                                       Start start);
+  std::vector<PointResult> solve_lockstep(const double* points, int n, const EigshOptions& o, const EigshControls& ctl);
   // network outputs (snapshot coefficients) of n points
   std::vector<double> predict(const double* points, int n);
 
@@ -108,6 +109,10 @@ class WarmStartScan {
   DeviceArray<float> dx_, dy_;
   DeviceArray<double> dc_, v0_;
   PinnedArray<double> cpin_;  // staging of the start coefficients
+  // lockstep batches: the points' tables, diagonals, starts and coefficients,
+  // and the interleaved lanes of the windows' next vectors
+  DeviceArray<double> lk_amps_, lk_diags_, lk_starts_, lk_y_, lk_lanes64_;
+  DeviceArray<float> lk_lanes32_;
   DeviceArray<float> params_;  // network parameters
   std::vector<double> center_, scale_;
 };

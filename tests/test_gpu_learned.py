@@ -190,6 +190,35 @@ def test_principal_space(qa):
         qa.Scan(m, space="other")
 
 
+@pytest.mark.parametrize("kw", [{}, {"matvec_precision": "fp32"}, {"reorthogonalization_eta": 1.0}])
+def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
+    # snapshot-start solves of 11 points in lockstep batches (8 + 3: launch
+    # widths 8 and 4, one shared H·v per step) against the same points solved
+    # one by one; η = 1 forces DGKS second passes, whose halted windows the
+    # shared launch must skip
+    m = _ising_scan_model(qa)
+    hx = np.linspace(0.2, 2.0, 6)
+    hz = np.linspace(0.0, 0.5, 5)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    scan = qa.Scan(m, snapshots=6, epochs=50, seed=0, space="principal")
+    scan.train(grid[::3])
+    held = grid[1::3][:11]
+    lock = scan.solve(held, warm="rayleigh_ritz", lockstep=8, **kw)
+    one = scan.solve(held, warm="rayleigh_ritz", lockstep=0, **kw)
+    print("[lockstep]", kw, lock["iterations"], one["iterations"], flush=True)
+    np.testing.assert_allclose(lock["energies"], one["energies"], rtol=1e-10, atol=0)
+    assert lock["warm_starts"] == len(held) and lock["fallbacks"] == 0
+    for (a, b), e, r in zip(held, lock["energies"], lock["residuals"]):
+        assert r <= 1e-10 * max(abs(e), 1.0)
+    e0 = np.linalg.eigvalsh(oracle.Model(L, cases.ising(L, *held[0]), momentum=0, parity=1,
+                                         norm="pauli_cartesian").dense())[0]
+    assert abs(lock["energies"][0] - e0) <= 1e-10 * abs(e0)
+    if "matvec_precision" in kw:
+        assert sum(lock["fp32_iterations"]) > 0
+    with pytest.raises(qa.QmbedError, match="lockstep must lie"):
+        scan.solve(held[:2], warm="rayleigh_ritz", lockstep=9)
+
+
 def test_continuation_targets(qa):
     m = _ising_scan_model(qa)
     hx = np.linspace(0.2, 2.0, 6)
