@@ -13,7 +13,9 @@ states (the 32 leading principal components) with the family's projection on
 it, and the 2-64-64-32 FP32 network trained on the GPU by the library's own
 kernels.
 
-One step = 16 held-out grid points, each solved to residual
+One step = 16 held-out grid points (two lockstep batches of 8: one shared
+H·v launch per Lanczos step for the points still running), each solved to
+residual
 ≤ 1e-8·max(|E|,1) with the FP32-matvec schedule from the snapshot start: the
 lowest eigenvector of the family projected on the 32-dimensional snapshot
 space.
@@ -439,7 +441,8 @@ def main():
                                "space (the 32 leading principal components of the ground states of 256 plain-Lanczos "
                                "training points): v0 = lowest eigenvector of the family projected on that space (the 2-64-64-32 FP32 MLP start is reported in "
                                "extras.scan.network_start_fp32) + FP32-matvec schedule; ground state to residual "
-                               "<= 1e-8*max(|E|,1); 16 points per step",
+                               "<= 1e-8*max(|E|,1); 16 points per step, solved in lockstep batches of 8 (one shared H.v launch "
+                               "per Lanczos step over the interleaved vectors of the points still running)",
                    "global_batch": BATCH},
         "roofline": {"bound": "hbm", "achieved": hvmix["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": hvmix["frac"], "traffic": HV_MIXED_TRAFFIC_BYTES if hvmix["mode"] == "stream" else None,
