@@ -167,9 +167,9 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * the learned snapshots; the projected parts are formed once at training).
  * Answers energies, residuals, iterations (total, FP32-matvec and FP64),
  * per-point seconds and fallback count. With warm = 2 the points are solved
- * in lockstep batches of up to "lockstep" (solver key, 0-8, default 8; 0 or
- * 1: one by one): one Lanczos window per point, one shared H·v launch per
- * step over the interleaved vectors of the windows still stepping (each
+ * in lockstep batches of up to "lockstep" (solver key, 0-16, default 16; 0
+ * or 1: one by one): one Lanczos window per point, one shared H·v launch per
+ * step and 8 windows over the interleaved vectors of the windows still stepping (each
  * point with its own couplings). Each window runs the single-window
  * algorithm; a point whose window fails is solved on its own. Needs a real
  * family with Hermitian parts, no momentum phases and the compact compiled

@@ -237,7 +237,7 @@ SolverReq parse_solver(const Value& s) {
   }
   if (auto* v = s.get("lockstep"); v && !v->is_null()) {
     const long c = as_long(*v, "lockstep");
-    if (c < 0 || c > 8) bad("lockstep must lie in [0, 8]");
+    if (c < 0 || c > 16) bad("lockstep must lie in [0, 16]");
     r.ctl.lockstep = int(c);
   }
   if (auto* v = s.get("read_lag"); v && !v->is_null()) {

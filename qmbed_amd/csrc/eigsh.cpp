@@ -198,7 +198,7 @@ static void release_buffers(std::unique_ptr<WindowBuffers> b) {
   std::lock_guard<std::mutex> lk(g_pool_mu);
   int same = 0;
   for (auto& e : g_pool) same += e->device == b->device ? 1 : 0;
-  if (same < 12) g_pool.push_back(std::move(b));
+  if (same < 20) g_pool.push_back(std::move(b));
 }
 
 // A pooled buffer set held for a scope; returned after the stream drains.
@@ -792,7 +792,7 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   const int nb = int(pts.size());
   const size_t n = op.dim();
   o.validate(n);
-  if (nb < 1 || nb > 8) throw internal_state("a lockstep batch holds 1 to 8 windows");
+  if (nb < 1 || nb > 16) throw internal_state("a lockstep batch holds 1 to 16 windows");
   if (n <= 256 || o.target == SpectrumTarget::Shift) throw internal_state("lockstep windows need the Krylov route");
   const long m = o.krylov_dimension
                      ? std::min<long>(std::min<long>(*o.krylov_dimension, o.max_iterations), long(n))
@@ -800,8 +800,9 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
                                       o.max_iterations);
   if (m <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
-  if (lanes64.size() < 8 * n) lanes64.resize(8 * n);
-  if (lanes32.size() < 8 * n) lanes32.resize(8 * n);
+  // lanes of up to two launches of 8 windows each
+  if (lanes64.size() < 16 * n) lanes64.resize(16 * n);
+  if (lanes32.size() < 16 * n) lanes32.resize(16 * n);
   std::vector<WindowResult> res(static_cast<size_t>(nb));
   std::vector<std::unique_ptr<WindowRun>> runs;
   const lk::Interleave no_lanes;
@@ -822,12 +823,12 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   const long max_ticks = 8 * (m + 8);
   for (long tick = 0;; ++tick) {
     if (tick > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
-    // the windows stepping now, packed into consecutive lanes
-    const double* src[8] = {};
-    const double* amps[8] = {};
-    const double* diags[8] = {};
-    double* ys[8] = {};
-    const int* gates[8] = {};
+    // the windows stepping now, packed into consecutive lanes (8 per launch)
+    const double* src[16] = {};
+    const double* amps[16] = {};
+    const double* diags[16] = {};
+    double* ys[16] = {};
+    const int* gates[16] = {};
     int k = 0;
     bool active = false, all32 = true;
     for (int b = 0; b < nb; ++b) {
@@ -845,13 +846,18 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       ++k;
     }
     if (!active) break;
+    // one launch per 8 stepping windows: FP32 lanes once all of them take
+    // FP32 products, FP64 lanes before
+    for (int g0 = 0; g0 < k; g0 += 8) {
+      const int kg = std::min(8, k - g0);
+      const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : 8;
+      double* l64 = lanes64.data() + size_t(g0) * n;
+      float* l32 = lanes32.data() + size_t(g0) * n;
+      lk::interleave(src + g0, n, width, all32 ? nullptr : l64, all32 ? l32 : nullptr, s);
+      op.apply_scan_batch(all32 ? static_cast<const void*>(l32) : static_cast<const void*>(l64), all32, width,
+                          amps + g0, diags + g0, ys + g0, gates + g0, s);
+    }
     if (k > 0) {
-      // one launch for every stepping window: FP32 lanes once all of them
-      // take FP32 products, FP64 lanes before
-      const int width = k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : 8;
-      lk::interleave(src, n, width, all32 ? nullptr : lanes64.data(), all32 ? lanes32.data() : nullptr, s);
-      op.apply_scan_batch(all32 ? static_cast<const void*>(lanes32.data()) : static_cast<const void*>(lanes64.data()),
-                          all32, width, amps, diags, ys, gates, s);
       for (int b = 0; b < nb; ++b) {
         auto& r = *runs[size_t(b)];
         if (!failed[size_t(b)] && !r.done() && r.stepping()) r.window().count_external(all32);

@@ -446,8 +446,8 @@ struct EigshControls {
   // on a read-back, and 2 below that
   int read_lag = 0;
   // scans from the snapshot start: points solved in lockstep per batch (one
-  // shared H·v launch per step; 0 or 1 solves them one by one)
-  int lockstep = 8;
+  // shared H·v launch per step and 8 windows; 0 or 1 solves them one by one)
+  int lockstep = 16;
   // return the eigenvectors (false: only eigenvalues and residuals, with no
   // device allocation per solve)
   bool keep_vectors = true;

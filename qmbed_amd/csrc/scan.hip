@@ -600,7 +600,7 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
   std::vector<PointResult> out(static_cast<size_t>(n));
   const size_t N = op_.dim();
   const int count = op_.scan_batch_amp_count();
-  const int W = std::min(ctl.lockstep, 8);
+  const int W = std::min(ctl.lockstep, 16);
   EigshOptions oo = o;
   oo.eigenpairs = 1;
   EigshControls ctl1 = ctl;

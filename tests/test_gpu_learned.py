@@ -192,10 +192,10 @@ def test_principal_space(qa):
 
 @pytest.mark.parametrize("kw", [{}, {"matvec_precision": "fp32"}, {"reorthogonalization_eta": 1.0}])
 def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
-    # snapshot-start solves of 11 points in lockstep batches (8 + 3: launch
-    # widths 8 and 4, one shared H·v per step) against the same points solved
-    # one by one; η = 1 forces DGKS second passes, whose halted windows the
-    # shared launch must skip
+    # snapshot-start solves of 10 points in lockstep batches of 8 (8 + 2:
+    # launch widths 8 and 2) and of 16 (one batch, two launches per step),
+    # against the same points solved one by one; η = 1 forces DGKS second
+    # passes, whose halted windows the shared launch must skip
     m = _ising_scan_model(qa)
     hx = np.linspace(0.2, 2.0, 6)
     hz = np.linspace(0.0, 0.5, 5)
@@ -205,6 +205,9 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
     held = grid[1::3][:11]
     lock = scan.solve(held, warm="rayleigh_ritz", lockstep=8, **kw)
     one = scan.solve(held, warm="rayleigh_ritz", lockstep=0, **kw)
+    # all 10 points in one lockstep batch: two launches per step (8 + 2 lanes)
+    lock16 = scan.solve(held, warm="rayleigh_ritz", lockstep=16, **kw)
+    np.testing.assert_allclose(lock16["energies"], one["energies"], rtol=1e-10, atol=0)
     print("[lockstep]", kw, lock["iterations"], one["iterations"], flush=True)
     np.testing.assert_allclose(lock["energies"], one["energies"], rtol=1e-10, atol=0)
     assert lock["warm_starts"] == len(held) and lock["fallbacks"] == 0
@@ -216,7 +219,7 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
     if "matvec_precision" in kw:
         assert sum(lock["fp32_iterations"]) > 0
     with pytest.raises(qa.QmbedError, match="lockstep must lie"):
-        scan.solve(held[:2], warm="rayleigh_ritz", lockstep=9)
+        scan.solve(held[:2], warm="rayleigh_ritz", lockstep=17)
 
 
 def test_continuation_targets(qa):
