@@ -326,6 +326,30 @@ class LanczosWindow {
 
   int* halt() const { return flags_ + m_; }
 
+  // The phases of step j = launched() for windows stepping in lockstep (the
+  // caller launched H·v_j into w(), and launches the α, reduction and
+  // finishing phases of all windows together): the same launches as step().
+  lk::ThreeTerm three_term() const {
+    const int j = launched_;
+    return lk::ThreeTerm{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr, slots() + j};
+  }
+  lk::AlphaJob alpha_job() { return {col(launched_), w(), &buf_->ws, halt()}; }
+  void overlaps_phase() {
+    const lk::ThreeTerm three = three_term();
+    lk::lanczos_overlaps(c_, V(), n_, launched_ + 1, w(), buf_->ws, s_, true, halt(), &three);
+  }
+  lk::HredJob hred_job() { return {&buf_->ws, launched_ + 1, h_, halt()}; }
+  void subtract_phase() {
+    const lk::ThreeTerm three = three_term();
+    lk::lanczos_subtract(c_, V(), n_, launched_ + 1, w(), h_, buf_->ws, s_, true, halt(), &three, false);
+  }
+  lk::FinishJob finish_job() {
+    const int j = launched_;
+    return {w(), &buf_->ws, h_, j, slots() + m_ + j, halt(), flags_ + j, j + 1 < m_ ? col(j + 1) : nullptr};
+  }
+  void advance() { ++launched_; }
+  double eta() const { return eta_; }
+
   // Launch step j = launched() (requires launched() < m).
   void step() {
     const int j = launched_;
@@ -600,9 +624,11 @@ class WindowRun {
   bool stepping() const { return !done_ && win_.launched() < m_; }
   LanczosWindow& window() { return win_; }
 
-  void tick() {
+  // stepped: the caller has launched this tick's step (lockstep phases)
+  void tick(bool stepped = false) {
     if (done_) return;
-    if (win_.launched() < m_) win_.step();
+    if (stepped) win_.advance();
+    else if (win_.launched() < m_) win_.step();
     const int launched = win_.launched();
     const bool full = launched == m_;
     if (!pending_) {
@@ -857,17 +883,37 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       op.apply_scan_batch(all32 ? static_cast<const void*>(l32) : static_cast<const void*>(l64), all32, width,
                           amps + g0, diags + g0, ys + g0, gates + g0, s);
     }
-    if (k > 0) {
-      for (int b = 0; b < nb; ++b) {
-        auto& r = *runs[size_t(b)];
-        if (!failed[size_t(b)] && !r.done() && r.stepping()) r.window().count_external(all32);
-      }
+    // the rest of the step: α of every stepping window in one launch, the
+    // overlaps per window, their reductions in one launch, the subtractions
+    // per window, the finishing kernels in one launch
+    std::vector<int> stepping;
+    for (int b = 0; b < nb; ++b) {
+      auto& r = *runs[size_t(b)];
+      if (!failed[size_t(b)] && !r.done() && r.stepping()) stepping.push_back(b);
     }
+    if (!stepping.empty()) {
+      std::vector<lk::AlphaJob> aj;
+      std::vector<lk::HredJob> hj;
+      std::vector<lk::FinishJob> fj;
+      for (int b : stepping) {
+        runs[size_t(b)]->window().count_external(all32);
+        aj.push_back(runs[size_t(b)]->window().alpha_job());
+      }
+      lk::lanczos_alpha_multi(aj.data(), int(aj.size()), n, s);
+      for (int b : stepping) runs[size_t(b)]->window().overlaps_phase();
+      for (int b : stepping) hj.push_back(runs[size_t(b)]->window().hred_job());
+      lk::lanczos_hred_multi(hj.data(), int(hj.size()), n, s);
+      for (int b : stepping) runs[size_t(b)]->window().subtract_phase();
+      for (int b : stepping) fj.push_back(runs[size_t(b)]->window().finish_job());
+      lk::lanczos_finish_multi(fj.data(), int(fj.size()), n, ctl.dgks_eta, s);
+    }
+    std::vector<char> stepped(static_cast<size_t>(nb), 0);
+    for (int b : stepping) stepped[size_t(b)] = 1;
     for (int b = 0; b < nb; ++b) {
       auto& r = *runs[size_t(b)];
       if (failed[size_t(b)] || r.done()) continue;
       try {
-        r.tick();
+        r.tick(stepped[size_t(b)] != 0);
       } catch (const Error&) {
         failed[size_t(b)] = 1;  // the caller solves this point on its own
       }

@@ -114,8 +114,8 @@ __global__ __launch_bounds__(kT) void k_sum(const double* partial, int nb, doubl
 
 // partial[block] = Re Σ conj(v_j) w over the block's rows (α_j of a step)
 template <typename E>
-__global__ __launch_bounds__(kT) void k_alpha(const E* __restrict__ vj, const E* __restrict__ w, size_t n,
-                                              double* partial, const int* gate) {
+__device__ __forceinline__ void alpha_body(const E* __restrict__ vj, const E* __restrict__ w, size_t n,
+                                           double* partial, const int* gate) {
   if (gate && *gate) return;
   __shared__ double sh[kT / 64];
   double s = 0;
@@ -123,6 +123,11 @@ __global__ __launch_bounds__(kT) void k_alpha(const E* __restrict__ vj, const E*
     s += re_of(cmul(vj[i], w[i]));
   s = block_sum(s, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+template <typename E>
+__global__ __launch_bounds__(kT) void k_alpha(const E* __restrict__ vj, const E* __restrict__ w, size_t n,
+                                              double* partial, const int* gate) {
+  alpha_body<E>(vj, w, n, partial, gate);
 }
 
 // The three-term part of a first pass (QMBED src/solve/mod.rs:1040-1048):
@@ -246,11 +251,12 @@ __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, 
 // one block per column: thread-strided sums, then the waves' shuffle trees
 // and their four totals in order (a fixed order).
 template <typename E>
-__global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int rb, int ld, int ncols, E* h,
-                                             const int* gate) {
+__device__ __forceinline__ void hred_body(const E* __restrict__ partial, int rb, int ld, int ncols, E* h,
+                                          const int* gate) {
   if (gate && *gate) return;
   __shared__ E sh[kT / 64];
   const int c = blockIdx.x;
+  if (c >= ncols) return;
   E s = zero<E>();
   for (int b = threadIdx.x; b < rb; b += kT) s = add(s, partial[size_t(b) * ld + c]);
   for (int o = 32; o > 0; o >>= 1) s = add(s, shfl_down(s, o));
@@ -262,6 +268,11 @@ __global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int 
     for (int w = 0; w < kT / 64; ++w) t = add(t, sh[w]);
     h[c] = t;
   }
+}
+template <typename E>
+__global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int rb, int ld, int ncols, E* h,
+                                             const int* gate) {
+  hred_body<E>(partial, rb, ld, ncols, h, gate);
 }
 
 // One column chunk of the subtraction w ← w₀ − V h (hc: the chunk's
@@ -332,9 +343,9 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
 // copy); out == nullptr writes β_j only. A first pass is skipped while *halt
 // is set.
 template <typename E, typename F>
-__global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n, const double* __restrict__ partial2,
-                                               int nb2, const E* __restrict__ h, int j, double eta, double* beta,
-                                               int* halt, int* hist, E* out, F* out32, int pass2, Interleave il) {
+__device__ __forceinline__ void finish_body(const E* __restrict__ w, size_t n, const double* __restrict__ partial2,
+                                            int nb2, const E* __restrict__ h, int j, double eta, double* beta,
+                                            int* halt, int* hist, E* out, F* out32, int pass2, const Interleave& il) {
   if (!pass2 && *halt) return;
   __shared__ double sh[kT / 64];
   __shared__ double s_after;
@@ -373,6 +384,53 @@ __global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n
       if (il.x32) il.x32[i * il.stride + il.lane] = float(re_of(v));
     }
   }
+}
+template <typename E, typename F>
+__global__ __launch_bounds__(kT) void k_finish(const E* __restrict__ w, size_t n, const double* __restrict__ partial2,
+                                               int nb2, const E* __restrict__ h, int j, double eta, double* beta,
+                                               int* halt, int* hist, E* out, F* out32, int pass2, Interleave il) {
+  finish_body<E, F>(w, n, partial2, nb2, h, j, eta, beta, halt, hist, out, out32, pass2, il);
+}
+
+// Lockstep windows (real): one launch of a step phase for several windows,
+// blockIdx.y = window, each block doing exactly what the single-window
+// kernel's block does.
+constexpr int kMaxJobs = 16;
+struct AlphaJobs {
+  const double* vj[kMaxJobs];
+  const double* w[kMaxJobs];
+  double* partial[kMaxJobs];
+  const int* gate[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_alpha_multi(AlphaJobs J, size_t n) {
+  const int q = blockIdx.y;
+  alpha_body<double>(J.vj[q], J.w[q], n, J.partial[q], J.gate[q]);
+}
+struct HredJobs {
+  const double* partial[kMaxJobs];
+  int rb[kMaxJobs], ld[kMaxJobs], ncols[kMaxJobs];
+  double* h[kMaxJobs];
+  const int* gate[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_hred_multi(HredJobs J) {
+  const int q = blockIdx.y;
+  hred_body<double>(J.partial[q], J.rb[q], J.ld[q], J.ncols[q], J.h[q], J.gate[q]);
+}
+struct FinishJobs {
+  const double* w[kMaxJobs];
+  const double* partial2[kMaxJobs];
+  const double* h[kMaxJobs];
+  int j[kMaxJobs];
+  double* beta[kMaxJobs];
+  int* halt[kMaxJobs];
+  int* hist[kMaxJobs];
+  double* out[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_finish_multi(FinishJobs J, size_t n, int nb2, double eta) {
+  const int q = blockIdx.y;
+  const Interleave none;
+  finish_body<double, float>(J.w[q], n, J.partial2[q], nb2, J.h[q], J.j[q], eta, J.beta[q], J.halt[q], J.hist[q],
+                             J.out[q], static_cast<float*>(nullptr), 0, none);
 }
 
 // out = w / sqrt(*nrm2); beta_slot (optional) = sqrt(*nrm2); optional FP32
@@ -723,13 +781,13 @@ static int sub_blocks(size_t n) {
 }
 
 void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h, Workspace& ws, hipStream_t s,
-                      bool with_ww, const int* gate, const ThreeTerm* three) {
+                      bool with_ww, const int* gate, const ThreeTerm* three, bool reduce) {
   const int nb = sub_blocks(n);
   QMBED_DISPATCH(c, {
     int rb, ld;
     pass_layout<E>(n, m, with_ww, rb, ld);
     const int ncols = m + (with_ww ? 1 : 0);
-    if (ncols > 0)
+    if (reduce && ncols > 0)
       hipLaunchKernelGGL(k_hred<E>, dim3(ncols), dim3(kT), 0, s, cE<E>(ws.partial.data()),
                        rb, ld, ncols, mE<E>(d_h), gate);
     const Three<E> t = three_of<E>(three, ws, n);
@@ -740,6 +798,55 @@ void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h
                          mE<E>(w), last ? ws.partial2.data() : nullptr, gate, c0 == 0 ? t : Three<E>());
     }
   });
+}
+
+void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep phase launch of 1 to 16 windows");
+  AlphaJobs J{};
+  for (int q = 0; q < nj; ++q) {
+    J.vj[q] = jobs[q].vj;
+    J.w[q] = jobs[q].w;
+    J.partial[q] = jobs[q].ws->partial3.data();
+    J.gate[q] = jobs[q].gate;
+  }
+  hipLaunchKernelGGL(k_alpha_multi, dim3(nblocks(n), nj), dim3(kT), 0, s, J, n);
+  QMBED_HIP(hipGetLastError());
+}
+
+void lanczos_hred_multi(const HredJob* jobs, int nj, size_t n, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep phase launch of 1 to 16 windows");
+  HredJobs J{};
+  int widest = 1;
+  for (int q = 0; q < nj; ++q) {
+    int rb, ld;
+    pass_layout<double>(n, jobs[q].m, true, rb, ld);
+    J.partial[q] = jobs[q].ws->partial.data();
+    J.rb[q] = rb;
+    J.ld[q] = ld;
+    J.ncols[q] = jobs[q].m + 1;
+    J.h[q] = jobs[q].h;
+    J.gate[q] = jobs[q].gate;
+    widest = std::max(widest, jobs[q].m + 1);
+  }
+  hipLaunchKernelGGL(k_hred_multi, dim3(widest, nj), dim3(kT), 0, s, J);
+  QMBED_HIP(hipGetLastError());
+}
+
+void lanczos_finish_multi(const FinishJob* jobs, int nj, size_t n, double eta, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep phase launch of 1 to 16 windows");
+  FinishJobs J{};
+  for (int q = 0; q < nj; ++q) {
+    J.w[q] = jobs[q].w;
+    J.partial2[q] = jobs[q].ws->partial2.data();
+    J.h[q] = jobs[q].h;
+    J.j[q] = jobs[q].j;
+    J.beta[q] = jobs[q].beta;
+    J.halt[q] = jobs[q].halt;
+    J.hist[q] = jobs[q].hist;
+    J.out[q] = jobs[q].out;
+  }
+  hipLaunchKernelGGL(k_finish_multi, dim3(nblocks(n), nj), dim3(kT), 0, s, J, n, sub_blocks(n), eta);
+  QMBED_HIP(hipGetLastError());
 }
 
 void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,

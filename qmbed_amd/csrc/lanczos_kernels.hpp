@@ -65,7 +65,37 @@ void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Wor
 // h = Σ partials (into d_h, m entries, + ‖w₀‖² at d_h[m] when with_ww;
 // one reduction launch), then w ← w₀ − V h and per-block ‖w‖² partials
 void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h, Workspace& ws, hipStream_t s,
-                      bool with_ww, const int* gate, const ThreeTerm* three);
+                      bool with_ww, const int* gate, const ThreeTerm* three, bool reduce = true);
+
+// Lockstep windows (real vectors, first passes): one launch of a step phase
+// for up to 16 windows, each window's blocks computing exactly what its own
+// launch would. reduce = false above leaves the reduction of the overlaps to
+// lanczos_hred_multi.
+struct AlphaJob {
+  const double* vj;
+  const double* w;
+  Workspace* ws;
+  const int* gate;
+};
+void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s);
+struct HredJob {
+  Workspace* ws;
+  int m;  // columns of the step's overlaps (‖w₀‖² follows at m)
+  double* h;
+  const int* gate;
+};
+void lanczos_hred_multi(const HredJob* jobs, int nj, size_t n, hipStream_t s);
+struct FinishJob {
+  const double* w;
+  Workspace* ws;
+  const double* h;
+  int j;
+  double* beta;
+  int* halt;
+  int* hist;
+  double* out;
+};
+void lanczos_finish_multi(const FinishJob* jobs, int nj, size_t n, double eta, hipStream_t s);
 // One lane of interleaved copies of a real vector: x64[i·stride + lane] (FP64)
 // and x32[i·stride + lane] (FP32), either optional. Windows solved in
 // lockstep keep their next vectors side by side this way, so that one
