@@ -166,7 +166,7 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * S·y, y the lowest eigenvector of Sᵀ H(c) S (eigenvector continuation over
  * the learned snapshots; the projected parts are formed once at training).
  * Answers energies, residuals, iterations (total, FP32-matvec and FP64),
- * per-point seconds and fallback count. With warm = 2 the points are solved
+ * per-point seconds and fallback count. The points are solved (any start)
  * in lockstep batches of up to "lockstep" (solver key, 0-16, default 16; 0
  * or 1: one by one): one Lanczos window per point, one shared H·v launch per
  * step and 8 windows over the interleaved vectors of the windows still stepping (each

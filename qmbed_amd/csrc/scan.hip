@@ -576,11 +576,15 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
                                                                  const EigshControls& ctl, Start start) {
   std::vector<PointResult> out;
   out.reserve(size_t(std::max(n, 0)));
-  if (start != Start::SnapshotRR || !trained_ || n < 2) {
+  if (n < 2 || (start != Start::Plain && !trained_)) {
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
     return out;
   }
-  if (ctl.lockstep > 1 && op_.scan_batch_supported(s_)) return solve_lockstep(points, n, o, ctl);
+  if (ctl.lockstep > 1 && op_.scan_batch_supported(s_)) return solve_lockstep(points, n, o, ctl, start);
+  if (start != Start::SnapshotRR) {
+    for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
+    return out;
+  }
   auto ahead = [this](const double* p) { return std::async(std::launch::async, [this, p] { return continuation(p); }); };
   auto next = ahead(points);
   for (int i = 0; i < n; ++i) {
@@ -596,7 +600,7 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
 // their starts S·y_b from one product, one shared H·v launch per step. A
 // point whose start vanishes or whose window fails is solved on its own.
 std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const double* points, int n, const EigshOptions& o,
-                                                                      const EigshControls& ctl) {
+                                                                      const EigshControls& ctl, Start start) {
   std::vector<PointResult> out(static_cast<size_t>(n));
   const size_t N = op_.dim();
   const int count = op_.scan_batch_amp_count();
@@ -610,28 +614,36 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
   for (int i0 = 0; i0 < n; i0 += W) {
     const int nb = std::min(W, n - i0);
     auto t0 = std::chrono::steady_clock::now();
-    std::vector<std::future<std::vector<double>>> fy;
-    for (int b = 0; b < nb; ++b) {
-      const double* p = points + size_t(i0 + b) * P_;
-      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+    // the start coefficients: the continuation (host threads) or the network
+    // (one forward pass for the batch); the plain start needs none
+    std::vector<std::vector<double>> ys(static_cast<size_t>(nb));
+    if (start == Start::SnapshotRR) {
+      std::vector<std::future<std::vector<double>>> fy;
+      for (int b = 0; b < nb; ++b) {
+        const double* p = points + size_t(i0 + b) * P_;
+        fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+      }
+      for (int b = 0; b < nb; ++b) ys[size_t(b)] = fy[size_t(b)].get();
+    } else if (start == Start::Network) {
+      const auto y = predict(points + size_t(i0) * P_, nb);
+      for (int b = 0; b < nb; ++b)
+        ys[size_t(b)].assign(y.begin() + size_t(b) * ns_, y.begin() + size_t(b + 1) * ns_);
     }
-    std::vector<std::vector<double>> ys;
-    for (auto& f : fy) ys.push_back(f.get());
     std::vector<int> lanes;  // points with a usable start
     for (int b = 0; b < nb; ++b) {
       double cn = 0;
       for (double v : ys[size_t(b)]) cn += v * v;
-      if (std::isfinite(cn) && cn > 1.0e-6) lanes.push_back(b);
+      if (start == Start::Plain || (std::isfinite(cn) && cn > 1.0e-6)) lanes.push_back(b);
     }
     std::vector<char> solved(static_cast<size_t>(nb), 0);
     if (lanes.size() >= 2) {
       const int nl = int(lanes.size());
       std::vector<std::vector<cplx>> coefs;
-      std::vector<double> Y(size_t(ns_) * size_t(nl));
+      std::vector<double> Y(start == Start::Plain ? 0 : size_t(ns_) * size_t(nl));
       for (int l = 0; l < nl; ++l) {
         const int b = lanes[size_t(l)];
         coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
-        std::copy(ys[size_t(b)].begin(), ys[size_t(b)].end(), Y.begin() + size_t(l) * ns_);
+        if (start != Start::Plain) std::copy(ys[size_t(b)].begin(), ys[size_t(b)].end(), Y.begin() + size_t(l) * ns_);
       }
       // the batch tables and starts live in grow-only members: no device
       // allocation (and no synchronizing free) per batch
@@ -643,8 +655,12 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
       grow(lk_starts_, size_t(nl) * N);
       grow(lk_y_, Y.size());
       op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
-      QMBED_HIP(hipMemcpyAsync(lk_y_.data(), Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s_));
-      lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
+      if (start == Start::Plain) {
+        for (int l = 0; l < nl; ++l) deterministic_start_device(lk_starts_.data() + size_t(l) * N, N, oo.seed, s_, false);
+      } else {
+        QMBED_HIP(hipMemcpyAsync(lk_y_.data(), Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s_));
+        lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
+      }
       std::vector<LockstepPoint> pts;
       for (int l = 0; l < nl; ++l) {
         double* v = lk_starts_.data() + size_t(l) * N;
@@ -660,14 +676,15 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
         const int b = lanes[size_t(l)];
         PointResult& r = out[size_t(i0 + b)];
         r.es = res[size_t(l)].es;
-        r.warm = true;
+        r.warm = start != Start::Plain;
         r.seconds = dt / double(nl);
         solved[size_t(b)] = 1;
       }
     }
     for (int b = 0; b < nb; ++b)
       if (!solved[size_t(b)])
-        out[size_t(i0 + b)] = solve(points + size_t(i0 + b) * P_, o, ctl, Start::SnapshotRR, &ys[size_t(b)]);
+        out[size_t(i0 + b)] = solve(points + size_t(i0 + b) * P_, o, ctl, start,
+                                    start == Start::SnapshotRR ? &ys[size_t(b)] : nullptr);
   }
   return out;
 }

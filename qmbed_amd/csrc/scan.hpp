@@ -69,12 +69,14 @@ class WarmStartScan {
   // continuation coefficients when already computed (snapshot start)
   PointResult solve(const double* point, const EigshOptions& o, const EigshControls& ctl, Start start,
                     const std::vector<double>* y = nullptr);
-  // ground states of n points in turn; with the snapshot start the host
-  // eigensolve of point i + 1's projected family runs on a helper thread
-  // while point i is solved on the device
+  // ground states of n points: in lockstep batches (ctl.lockstep, any
+  // start) where the family allows; otherwise in turn, and then with the
+  // snapshot start the host eigensolve of point i + 1's projected family runs
+  // on a helper thread while point i is solved on the device
   std::vector<PointResult> solve_many(const double* points, int n, const EigshOptions& o, const EigshControls& ctl,
                                       Start start);
-  std::vector<PointResult> solve_lockstep(const double* points, int n, const EigshOptions& o, const EigshControls& ctl);
+  std::vector<PointResult> solve_lockstep(const double* points, int n, const EigshOptions& o, const EigshControls& ctl,
+                                          Start start);
   // network outputs (snapshot coefficients) of n points
   std::vector<double> predict(const double* points, int n);
 

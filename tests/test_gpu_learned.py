@@ -220,6 +220,16 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
         assert sum(lock["fp32_iterations"]) > 0
     with pytest.raises(qa.QmbedError, match="lockstep must lie"):
         scan.solve(held[:2], warm="rayleigh_ritz", lockstep=17)
+    # the network's start and the plain start in lockstep too
+    for warm in (True, False):
+        a = scan.solve(held, warm=warm, lockstep=16, **kw)
+        b = scan.solve(held, warm=warm, lockstep=0, **kw)
+        np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-10, atol=0)
+        assert a["warm_starts"] == b["warm_starts"]
+        if "matvec_precision" not in kw:
+            # FP64 products either way (with FP32 products, a lockstep window
+            # that has switched may still get FP64 lanes while others have not)
+            assert a["iterations"] == b["iterations"]
 
 
 def test_continuation_targets(qa):
