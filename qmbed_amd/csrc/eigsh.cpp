@@ -849,40 +849,43 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   const long max_ticks = 8 * (m + 8);
   for (long tick = 0;; ++tick) {
     if (tick > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
-    // the windows stepping now, packed into consecutive lanes (8 per launch)
-    const double* src[16] = {};
-    const double* amps[16] = {};
-    const double* diags[16] = {};
-    double* ys[16] = {};
-    const int* gates[16] = {};
-    int k = 0;
-    bool active = false, all32 = true;
-    for (int b = 0; b < nb; ++b) {
-      auto& r = *runs[size_t(b)];
-      if (failed[size_t(b)] || r.done()) continue;
-      active = true;
-      if (!r.stepping()) continue;
-      auto& w = r.window();
-      src[k] = w.col(w.launched());
-      amps[k] = pts[size_t(b)].amps;
-      diags[k] = pts[size_t(b)].diag;
-      ys[k] = w.w();
-      gates[k] = w.gate();
-      all32 = all32 && w.fp32_active();
-      ++k;
+    // the windows stepping now, by the precision of their products (each
+    // window's own FP32 switch, as in a single solve), packed into
+    // consecutive lanes, 8 per launch
+    bool active = false;
+    for (int prec = 0; prec < 2; ++prec) {
+      const bool use32 = prec == 1;
+      const double* src[16] = {};
+      const double* amps[16] = {};
+      const double* diags[16] = {};
+      double* ys[16] = {};
+      const int* gates[16] = {};
+      int k = 0;
+      for (int b = 0; b < nb; ++b) {
+        auto& r = *runs[size_t(b)];
+        if (failed[size_t(b)] || r.done()) continue;
+        active = true;
+        if (!r.stepping() || r.window().fp32_active() != use32) continue;
+        auto& w = r.window();
+        src[k] = w.col(w.launched());
+        amps[k] = pts[size_t(b)].amps;
+        diags[k] = pts[size_t(b)].diag;
+        ys[k] = w.w();
+        gates[k] = w.gate();
+        w.count_external(use32);
+        ++k;
+      }
+      for (int g0 = 0; g0 < k; g0 += 8) {
+        const int kg = std::min(8, k - g0);
+        const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : 8;
+        double* l64 = lanes64.data() + size_t(g0) * n;
+        float* l32 = lanes32.data() + size_t(g0) * n;
+        lk::interleave(src + g0, n, width, use32 ? nullptr : l64, use32 ? l32 : nullptr, s);
+        op.apply_scan_batch(use32 ? static_cast<const void*>(l32) : static_cast<const void*>(l64), use32, width,
+                            amps + g0, diags + g0, ys + g0, gates + g0, s);
+      }
     }
     if (!active) break;
-    // one launch per 8 stepping windows: FP32 lanes once all of them take
-    // FP32 products, FP64 lanes before
-    for (int g0 = 0; g0 < k; g0 += 8) {
-      const int kg = std::min(8, k - g0);
-      const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : 8;
-      double* l64 = lanes64.data() + size_t(g0) * n;
-      float* l32 = lanes32.data() + size_t(g0) * n;
-      lk::interleave(src + g0, n, width, all32 ? nullptr : l64, all32 ? l32 : nullptr, s);
-      op.apply_scan_batch(all32 ? static_cast<const void*>(l32) : static_cast<const void*>(l64), all32, width,
-                          amps + g0, diags + g0, ys + g0, gates + g0, s);
-    }
     // the rest of the step: α of every stepping window in one launch, the
     // overlaps per window, their reductions in one launch, the subtractions
     // per window, the finishing kernels in one launch
@@ -895,10 +898,7 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       std::vector<lk::AlphaJob> aj;
       std::vector<lk::HredJob> hj;
       std::vector<lk::FinishJob> fj;
-      for (int b : stepping) {
-        runs[size_t(b)]->window().count_external(all32);
-        aj.push_back(runs[size_t(b)]->window().alpha_job());
-      }
+      for (int b : stepping) aj.push_back(runs[size_t(b)]->window().alpha_job());
       lk::lanczos_alpha_multi(aj.data(), int(aj.size()), n, s);
       for (int b : stepping) runs[size_t(b)]->window().overlaps_phase();
       for (int b : stepping) hj.push_back(runs[size_t(b)]->window().hred_job());

@@ -209,6 +209,7 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
     lock16 = scan.solve(held, warm="rayleigh_ritz", lockstep=16, **kw)
     np.testing.assert_allclose(lock16["energies"], one["energies"], rtol=1e-10, atol=0)
     print("[lockstep]", kw, lock["iterations"], one["iterations"], flush=True)
+    assert lock["iterations"] == one["iterations"] and lock["fp32_iterations"] == one["fp32_iterations"]
     np.testing.assert_allclose(lock["energies"], one["energies"], rtol=1e-10, atol=0)
     assert lock["warm_starts"] == len(held) and lock["fallbacks"] == 0
     for (a, b), e, r in zip(held, lock["energies"], lock["residuals"]):
@@ -225,11 +226,10 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
         a = scan.solve(held, warm=warm, lockstep=16, **kw)
         b = scan.solve(held, warm=warm, lockstep=0, **kw)
         np.testing.assert_allclose(a["energies"], b["energies"], rtol=1e-10, atol=0)
+        # every window gets the products of its own one-by-one solve (FP32
+        # lanes from its own switch on), so the counts agree
         assert a["warm_starts"] == b["warm_starts"]
-        if "matvec_precision" not in kw:
-            # FP64 products either way (with FP32 products, a lockstep window
-            # that has switched may still get FP64 lanes while others have not)
-            assert a["iterations"] == b["iterations"]
+        assert a["iterations"] == b["iterations"] and a["fp32_iterations"] == b["fp32_iterations"]
 
 
 def test_continuation_targets(qa):
