@@ -18,6 +18,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <functional>
 #include <memory>
@@ -166,7 +167,8 @@ struct WindowBuffers {
     if (pin_n >= n) return;
     if (pin) QMBED_HIP(hipHostFree(pin));
     pin = nullptr;
-    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin), n * sizeof(double), hipHostMallocDefault));
+    // coherent: a batched read-back kernel writes it directly
+    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin), n * sizeof(double), hipHostMallocCoherent));
     pin_n = n;
   }
 };
@@ -213,6 +215,49 @@ struct PooledBuffers {
   PooledBuffers& operator=(const PooledBuffers&) = delete;
 };
 
+// Read-backs of lockstep windows in one launch: a window registers the copy
+// of its slot area, flush() issues every registered copy with one kernel
+// (into the windows' coherent pinned areas) and records one event. A window
+// waits on the event of the flush that carried its copy; the events form a
+// ring, and an event recorded again since is later in stream order, so
+// waiting on it is still correct.
+class PostBatch {
+ public:
+  explicit PostBatch(hipStream_t s) : s_(s) {
+    for (auto& e : ev_) QMBED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  ~PostBatch() {
+    for (auto e : ev_)
+      if (e) (void)hipEventDestroy(e);
+  }
+  PostBatch(const PostBatch&) = delete;
+  PostBatch& operator=(const PostBatch&) = delete;
+  // the generation of the flush that will carry this copy
+  long add(const double* src, double* dst, size_t count) {
+    jobs_.push_back({src, dst, int(count)});
+    return gen_;
+  }
+  void flush() {
+    if (jobs_.empty()) return;
+    for (size_t i = 0; i < jobs_.size(); i += 16)
+      lk::copy_multi(jobs_.data() + i, int(std::min<size_t>(16, jobs_.size() - i)), s_);
+    QMBED_HIP(hipEventRecord(ev_[size_t(gen_ % kRing)], s_));
+    jobs_.clear();
+    ++gen_;
+  }
+  void wait(long g) {
+    if (g >= gen_) flush();
+    QMBED_HIP(hipEventSynchronize(ev_[size_t(g % kRing)]));
+  }
+
+ private:
+  static constexpr long kRing = 64;
+  hipStream_t s_;
+  std::array<hipEvent_t, kRing> ev_{};
+  std::vector<lk::CopyJob> jobs_;
+  long gen_ = 0;
+};
+
 // One Lanczos window on the device. A step is six launches and no host
 // round trip: H·v_j, the partials of α_j = <v_j, H·v_j>, the overlaps of
 // w₀ = H·v_j − α_j v_j − β_{j−1} v_{j−1} with V (and ‖w₀‖²), their
@@ -229,7 +274,9 @@ struct PooledBuffers {
 // truncates the window to the steps before it.
 class LanczosWindow {
  public:
-  LanczosWindow(const Operator& op, bool complex, int m, bool fp32, hipStream_t s, double eta = DGKS)
+  // clear_flags = false: the caller clears flag_area() before the first step
+  LanczosWindow(const Operator& op, bool complex, int m, bool fp32, hipStream_t s, double eta = DGKS,
+                bool clear_flags = true)
       : op_(op), c_(complex), n_(op.dim()), m_(m), fp32_(fp32), eta_(eta), s_(s), buf_(acquire_buffers()) {
     const size_t esz = c_ ? 2 : 1;
     grow(buf_->V, esz * n_ * size_t(m_));
@@ -246,7 +293,7 @@ class LanczosWindow {
     h_ = buf_->slots.data() + head;
     scratch_ = h_ + 2 * (m_ + 1);
     flags_ = reinterpret_cast<int*>(buf_->slots.data() + 2 * m_);
-    QMBED_HIP(hipMemsetAsync(flags_, 0, flag_doubles_ * sizeof(double), s_));
+    if (clear_flags) QMBED_HIP(hipMemsetAsync(flags_, 0, flag_doubles_ * sizeof(double), s_));
     buf_->ensure_pin(head);
   }
   ~LanczosWindow() {
@@ -260,6 +307,9 @@ class LanczosWindow {
   double* V() const { return buf_->V.data(); }
   double* w() const { return buf_->w.data(); }
   double* slots() const { return buf_->slots.data(); }
+  double* flag_area() const { return reinterpret_cast<double*>(flags_); }
+  size_t flag_doubles() const { return flag_doubles_; }
+  lk::Workspace& workspace() { return buf_->ws; }
 
   // Install the start vector (device, FP64, already normalized) as column 0.
   // External matvec mode (windows solved in lockstep): the caller launches
@@ -282,6 +332,10 @@ class LanczosWindow {
 
   void start_from(const double* d_v0) {
     QMBED_HIP(hipMemcpyAsync(col(0), d_v0, esz() * n_ * 8, hipMemcpyDeviceToDevice, s_));
+    start_prepared();
+  }
+  // the caller has written the start into col(0) (and cleared the flags)
+  void start_prepared() {
     if (external_ && (il_.x64 || il_.x32)) lk::scatter_lane(col(0), n_, il_, s_);
     use32_ = false;
     alpha_.clear();
@@ -364,12 +418,20 @@ class LanczosWindow {
     ++launched_;
   }
 
-  // Queue the read-back of the coefficients of the launched steps.
+  // Queue the read-back of the coefficients of the launched steps (with a
+  // post batch: registered there, copied at its next flush, which comes
+  // before this window launches again or waits for it).
   void post() {
-    QMBED_HIP(hipMemcpyAsync(buf_->pin, slots(), (2 * size_t(m_) + flag_doubles_) * 8, hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipEventRecord(buf_->ev, s_));
+    const size_t count = 2 * size_t(m_) + flag_doubles_;
+    if (pb_) {
+      post_gen_ = pb_->add(slots(), buf_->pin, count);
+    } else {
+      QMBED_HIP(hipMemcpyAsync(buf_->pin, slots(), count * 8, hipMemcpyDeviceToHost, s_));
+      QMBED_HIP(hipEventRecord(buf_->ev, s_));
+    }
     posted_ = launched_;
   }
+  void set_post_batch(PostBatch* pb) { pb_ = pb; }
   int posted() const { return posted_; }
 
   // Wait for the posted read-back and parse it. A pending second pass (halt
@@ -383,7 +445,8 @@ class LanczosWindow {
     bool rewound = false;
   };
   Sync collect() {
-    QMBED_HIP(hipEventSynchronize(buf_->ev));
+    if (pb_) pb_->wait(post_gen_);
+    else QMBED_HIP(hipEventSynchronize(buf_->ev));
     const int L = posted_;
     const double* hb = buf_->pin;
     const int* hf = reinterpret_cast<const int*>(buf_->pin + 2 * m_);
@@ -555,6 +618,8 @@ class LanczosWindow {
   double last_beta_ = 0;
   int launched_ = 0;
   int posted_ = 0;
+  PostBatch* pb_ = nullptr;
+  long post_gen_ = 0;
   int size_ = 0;
   long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0, matvecs32_ = 0;
   bool broke_ = false;
@@ -593,9 +658,12 @@ class WindowRun {
             const double* start_norm2 = nullptr, const lk::Interleave* il = nullptr,
             std::function<void(const double*, double*)> apply64 = {})
       : op_(op), c_(c), o_(o), m_(m), ctl_(ctl), accept_tol_(accept_tol), out_(out), s_(s),
-        start_norm2_(start_norm2), win_(op, c, m, fp32, s, ctl.dgks_eta) {
+        start_norm2_(start_norm2), win_(op, c, m, fp32, s, ctl.dgks_eta, d_start != nullptr) {
     if (il) win_.set_external(*il, std::move(apply64));
-    win_.start_from(d_start);
+    // d_start == nullptr: the caller writes the start into the window's
+    // first column and clears its flags (lockstep windows, in one launch)
+    if (d_start) win_.start_from(d_start);
+    else win_.start_prepared();
     k_ = o.eigenpairs;
     next_check_ = std::max(k_ + 1, ctl.check_every);
     // Read-backs are posted at convergence checks, at the window's end, and
@@ -826,7 +894,8 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
                                       o.max_iterations);
   if (m <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
-  // lanes of up to two launches of 8 windows each
+  // lanes of all 16 windows
+  const int max_width = op.scan_batch_max_width();
   if (lanes64.size() < 16 * n) lanes64.resize(16 * n);
   if (lanes32.size() < 16 * n) lanes32.resize(16 * n);
   std::vector<WindowResult> res(static_cast<size_t>(nb));
@@ -840,9 +909,22 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       double* ys[1] = {y};
       op.apply_scan_batch(x, false, 1, am, dg, ys, nullptr, s);
     };
-    runs.emplace_back(new WindowRun(op, false, o, int(m), pt.start, fp32, ctl, o.tolerance, res[size_t(b)], s, nullptr,
+    runs.emplace_back(new WindowRun(op, false, o, int(m), nullptr, fp32, ctl, o.tolerance, res[size_t(b)], s, nullptr,
                                     &no_lanes, apply64));
   }
+  // the starts, normalized into the windows' first columns, and the windows'
+  // flag areas cleared, for all windows at once
+  {
+    std::vector<lk::StartJob> sj;
+    for (int b = 0; b < nb; ++b) {
+      auto& w = runs[size_t(b)]->window();
+      sj.push_back({pts[size_t(b)].start, w.col(0), w.flag_area(), int(w.flag_doubles())});
+    }
+    lk::start_multi(sj.data(), nb, n, runs[0]->window().workspace(), s);
+  }
+  // the windows' read-backs of a tick go in one copy launch
+  PostBatch pb(s);
+  for (auto& r : runs) r->window().set_post_batch(&pb);
   std::vector<char> failed(static_cast<size_t>(nb), 0);
   // every tick of a window launches a step or collects a read-back, so a
   // window ends within a few ticks per step of its window size
@@ -851,7 +933,9 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
     if (tick > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
     // the windows stepping now, by the precision of their products (each
     // window's own FP32 switch, as in a single solve), packed into
-    // consecutive lanes, 8 per launch
+    // consecutive lanes: launches of up to 8 lanes, or one of 16 for 13 or
+    // more FP32 windows (a 16-lane FP32 launch costs 0.85 of two 8-lane ones;
+    // in FP64 it costs more than two, at half the occupancy)
     bool active = false;
     for (int prec = 0; prec < 2; ++prec) {
       const bool use32 = prec == 1;
@@ -875,9 +959,10 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
         w.count_external(use32);
         ++k;
       }
-      for (int g0 = 0; g0 < k; g0 += 8) {
-        const int kg = std::min(8, k - g0);
-        const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : 8;
+      const int widest = use32 && k >= 13 ? max_width : 8;
+      for (int g0 = 0; g0 < k; g0 += widest) {
+        const int kg = std::min(widest, k - g0);
+        const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : kg <= 8 ? 8 : 16;
         double* l64 = lanes64.data() + size_t(g0) * n;
         float* l32 = lanes32.data() + size_t(g0) * n;
         lk::interleave(src + g0, n, width, use32 ? nullptr : l64, use32 ? l32 : nullptr, s);
@@ -918,6 +1003,7 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
         failed[size_t(b)] = 1;  // the caller solves this point on its own
       }
     }
+    pb.flush();
   }
   QMBED_HIP(hipStreamSynchronize(s));
   std::vector<LockstepResult> out(static_cast<size_t>(nb));

@@ -1146,7 +1146,7 @@ __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const SX* _
 
 // Scan points of one real family in lockstep: per point an amplitude table
 // (gather form, real), a per-row diagonal, an output vector and a gate.
-constexpr int kMaxScanBatch = 8;
+constexpr int kMaxScanBatch = 16;
 struct ScanBatch {
   const double* amp[kMaxScanBatch];  // per lane: a table of n_amp entries
   const double* diag[kMaxScanBatch];
@@ -2164,8 +2164,12 @@ bool Operator::scan_batch_supported(hipStream_t s) const {
     if (!h) return false;
   if (t.diag_parts != int(components_.size()) + 1) return false;
   if (t.stream.size() == 0 && !build_stream(const_cast<Operator&>(*this), s)) return false;
-  // LDS of the widest launch: the lane tables and the two small tables
-  return t.compact && sizeof(double) * (size_t(kMaxScanBatch) * size_t(t.n_amp) + 81) <= 48 * 1024;
+  // LDS of an 8-lane launch: the lane tables and the two small tables
+  return t.compact && sizeof(double) * (8 * size_t(t.n_amp) + 81) <= 48 * 1024;
+}
+
+int Operator::scan_batch_max_width() const {
+  return sizeof(double) * (16 * size_t(tables_->n_amp) + 81) <= 48 * 1024 ? 16 : 8;
 }
 
 int Operator::scan_batch_amp_count() const { return tables_->n_amp; }
@@ -2209,16 +2213,23 @@ void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, do
                        d_diags + b * 2 * n);
     QMBED_HIP(hipGetLastError());
   }
-  QMBED_HIP(hipMemcpyAsync(d_amps, amps.data(), amps.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  QMBED_HIP(hipStreamSynchronize(s));  // amps dies here
+  // staged through the tables' pinned area (free once its last copy has
+  // run), so the copy is asynchronous
+  if (t.staged) QMBED_HIP(hipEventSynchronize(t.staged));
+  t.stage.ensure(amps.size());
+  std::copy(amps.begin(), amps.end(), t.stage.p);
+  QMBED_HIP(hipMemcpyAsync(d_amps, t.stage.p, amps.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  if (!t.staged) QMBED_HIP(hipEventCreateWithFlags(&t.staged, hipEventDisableTiming));
+  QMBED_HIP(hipEventRecord(t.staged, s));
 }
 
 void Operator::apply_scan_batch(const void* x, bool fp32, int width, const double* const* amps,
                                 const double* const* diags, double* const* y, const int* const* gates,
                                 hipStream_t s) const {
   auto& t = *tables_;
-  if (!(width == 1 || width == 2 || width == 4 || width == 8))
-    throw internal_state("batched scan width must be 1, 2, 4 or 8");
+  if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16))
+    throw internal_state("batched scan width must be 1, 2, 4, 8 or 16");
+  if (width > scan_batch_max_width()) throw internal_state("batched scan width beyond the LDS of its tables");
   if (!t.compact || t.stream.size() == 0) throw internal_state("batched scan action without a compact stream");
   HvArgs a = t.args(*basis_);
   a.stream = t.stream.data();
@@ -2248,14 +2259,16 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
       case 1: QMBED_SCAN_LAUNCH(float, 1) break;
       case 2: QMBED_SCAN_LAUNCH(float, 2) break;
       case 4: QMBED_SCAN_LAUNCH(float, 4) break;
-      default: QMBED_SCAN_LAUNCH(float, 8) break;
+      case 8: QMBED_SCAN_LAUNCH(float, 8) break;
+      default: QMBED_SCAN_LAUNCH(float, 16) break;
     }
   } else {
     switch (width) {
       case 1: QMBED_SCAN_LAUNCH(double, 1) break;
       case 2: QMBED_SCAN_LAUNCH(double, 2) break;
       case 4: QMBED_SCAN_LAUNCH(double, 4) break;
-      default: QMBED_SCAN_LAUNCH(double, 8) break;
+      case 8: QMBED_SCAN_LAUNCH(double, 8) break;
+      default: QMBED_SCAN_LAUNCH(double, 16) break;
     }
   }
 #undef QMBED_SCAN_LAUNCH

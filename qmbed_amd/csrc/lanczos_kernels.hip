@@ -635,8 +635,12 @@ void normalize(bool c, const void* w, const double* nrm2_slot, size_t n, void* o
 
 void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, void* Y, hipStream_t s) {
   const size_t esz = c ? 16 : 8;
-  for (int q0 = 0; q0 < k; q0 += 4) {
-    int kk = std::min(4, k - q0);
+  // chunks of up to 16 columns: V is read once per chunk, and every column's
+  // sum runs in the same order whatever the chunk
+  for (int q0 = 0; q0 < k;) {
+    const int left = k - q0;
+    int kk = left >= 16 ? 16 : left >= 8 ? 8 : std::min(4, left);
+    if (size_t(m) * kk * esz > 64 * 1024) kk = std::min(4, left);
     size_t sm = size_t(m) * kk * esz;
     if (sm > 64 * 1024) throw unsupported_backend("Ritz assembly window exceeds the LDS coefficient budget");
     QMBED_DISPATCH(c, {
@@ -646,10 +650,67 @@ void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, voi
         case 1: hipLaunchKernelGGL((k_gemm_vc<E, 1>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
         case 2: hipLaunchKernelGGL((k_gemm_vc<E, 2>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
         case 3: hipLaunchKernelGGL((k_gemm_vc<E, 3>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
-        default: hipLaunchKernelGGL((k_gemm_vc<E, 4>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
+        case 4: hipLaunchKernelGGL((k_gemm_vc<E, 4>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
+        case 8: hipLaunchKernelGGL((k_gemm_vc<E, 8>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
+        default: hipLaunchKernelGGL((k_gemm_vc<E, 16>), dim3(nblocks(n)), dim3(kT), sm, s, cE<E>(V), n, m, C, Yq); break;
       }
     });
+    q0 += kk;
   }
+}
+
+// Starts of lockstep windows: the squared norms of nv vectors (the per-block
+// partials and their sums of nrm2sq_to_slot, for every vector at once), then
+// out_q = x_q / ‖x_q‖ (as normalize) with the window's flag area cleared.
+struct NormJobs {
+  const double* x[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_nrm2_multi(NormJobs J, size_t n, double* partial) {
+  __shared__ double sh[kT / 64];
+  const double* x = J.x[blockIdx.y];
+  double s = 0;
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) s += x[i] * x[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kT) void k_sum_multi(const double* partial, int nb, double* slots) {
+  __shared__ double sh[kT / 64];
+  const double t = sum_partials(partial + size_t(blockIdx.x) * nb, nb, sh);
+  if (threadIdx.x == 0) slots[blockIdx.x] = t;
+}
+struct StartJobs {
+  const double* x[kMaxJobs];
+  double* out[kMaxJobs];
+  double* zero[kMaxJobs];
+  int nzero[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_start_multi(StartJobs J, const double* nrm2, size_t n) {
+  const int q = blockIdx.y;
+  const double inv = 1.0 / sqrt(nrm2[q]);
+  const double* x = J.x[q];
+  double* out = J.out[q];
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) out[i] = x[i] * inv;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < J.nzero[q]; i += kT) J.zero[q][i] = 0.0;
+}
+
+void start_multi(const StartJob* jobs, int nj, size_t n, Workspace& ws, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep starts of 1 to 16 windows");
+  const int nb = nblocks(n);
+  if (ws.partial.size() < size_t(nj) * nb || ws.slots.size() < size_t(nj)) throw internal_state("start workspace too small");
+  NormJobs N{};
+  StartJobs S{};
+  for (int q = 0; q < nj; ++q) {
+    N.x[q] = jobs[q].x;
+    S.x[q] = jobs[q].x;
+    S.out[q] = jobs[q].out;
+    S.zero[q] = jobs[q].zero;
+    S.nzero[q] = jobs[q].nzero;
+  }
+  hipLaunchKernelGGL(k_nrm2_multi, dim3(nb, nj), dim3(kT), 0, s, N, n, ws.partial.data());
+  hipLaunchKernelGGL(k_sum_multi, dim3(nj), dim3(kT), 0, s, ws.partial.data(), nb, ws.slots.data());
+  hipLaunchKernelGGL(k_start_multi, dim3(nblocks(n), nj), dim3(kT), 0, s, S, ws.slots.data(), n);
+  QMBED_HIP(hipGetLastError());
 }
 
 void gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C, DeviceArray<double>& partial,
@@ -849,6 +910,26 @@ void lanczos_finish_multi(const FinishJob* jobs, int nj, size_t n, double eta, h
   QMBED_HIP(hipGetLastError());
 }
 
+struct CopyJobs {
+  CopyJob j[kMaxJobs];
+};
+// block q copies job q
+__global__ __launch_bounds__(kT) void k_copy_multi(CopyJobs J) {
+  const CopyJob jb = J.j[blockIdx.x];
+  for (int i = threadIdx.x; i < jb.count; i += kT) jb.dst[i] = jb.src[i];
+}
+
+void copy_multi(const CopyJob* jobs, int nj, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("batched copy of 1 to 16 arrays");
+  CopyJobs J{};
+  for (int q = 0; q < nj; ++q) {
+    if (jobs[q].count < 0 || !jobs[q].src || !jobs[q].dst) throw internal_state("batched copy job without its arrays");
+    J.j[q] = jobs[q];
+  }
+  hipLaunchKernelGGL(k_copy_multi, dim3(nj), dim3(kT), 0, s, J);
+  QMBED_HIP(hipGetLastError());
+}
+
 void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, double eta, double* beta, int* halt,
                     int* hist, void* out, float* out32, Workspace& ws, hipStream_t s, bool second_pass,
                     const Interleave* il) {
@@ -866,13 +947,13 @@ __global__ __launch_bounds__(kT) void k_scatter_lane(const double* __restrict__ 
   }
 }
 
-struct Lanes8 {
-  const double* src[8];
+struct Lanes16 {
+  const double* src[16];
 };
 // one row per thread: the W values of a row are gathered from the W
 // columns (coalesced reads) and stored side by side as whole vectors
 template <typename T, int W>
-__global__ __launch_bounds__(kT) void k_interleave(Lanes8 l, size_t n, T* __restrict__ x) {
+__global__ __launch_bounds__(kT) void k_interleave(Lanes16 l, size_t n, T* __restrict__ x) {
   struct alignas(sizeof(T) * W) Row {
     T v[W];
   };
@@ -886,10 +967,10 @@ __global__ __launch_bounds__(kT) void k_interleave(Lanes8 l, size_t n, T* __rest
 }
 
 void interleave(const double* const* src, size_t n, int width, double* x64, float* x32, hipStream_t s) {
-  if (!(width == 1 || width == 2 || width == 4 || width == 8))
-    throw internal_state("interleave width must be 1, 2, 4 or 8");
+  if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16))
+    throw internal_state("interleave width must be 1, 2, 4, 8 or 16");
   if (!x64 == !x32) throw internal_state("interleave writes exactly one of the FP64 and FP32 lanes");
-  Lanes8 l{};
+  Lanes16 l{};
   for (int b = 0; b < width; ++b) l.src[b] = src[b];
   const dim3 g(nblocks(n)), t(kT);
 #define QMBED_IL(T, W, X) hipLaunchKernelGGL((k_interleave<T, W>), g, t, 0, s, l, n, X)
@@ -897,12 +978,14 @@ void interleave(const double* const* src, size_t n, int width, double* x64, floa
     if (width == 1) QMBED_IL(double, 1, x64);
     else if (width == 2) QMBED_IL(double, 2, x64);
     else if (width == 4) QMBED_IL(double, 4, x64);
-    else QMBED_IL(double, 8, x64);
+    else if (width == 8) QMBED_IL(double, 8, x64);
+    else QMBED_IL(double, 16, x64);
   } else {
     if (width == 1) QMBED_IL(float, 1, x32);
     else if (width == 2) QMBED_IL(float, 2, x32);
     else if (width == 4) QMBED_IL(float, 4, x32);
-    else QMBED_IL(float, 8, x32);
+    else if (width == 8) QMBED_IL(float, 8, x32);
+    else QMBED_IL(float, 16, x32);
   }
 #undef QMBED_IL
   QMBED_HIP(hipGetLastError());

@@ -96,6 +96,24 @@ struct FinishJob {
   double* out;
 };
 void lanczos_finish_multi(const FinishJob* jobs, int nj, size_t n, double eta, hipStream_t s);
+// Starts of up to 16 lockstep windows in three launches: out = x / ‖x‖
+// (each bit for bit as nrm2sq_to_slot and normalize give it), and
+// zero[0..nzero) cleared (the window's flag area).
+struct StartJob {
+  const double* x;
+  double* out;
+  double* zero;
+  int nzero;
+};
+void start_multi(const StartJob* jobs, int nj, size_t n, Workspace& ws, hipStream_t s);
+// Up to 16 small copies in one launch (the read-backs of lockstep windows
+// into host-visible pinned memory): dst[i] = src[i] for i < count.
+struct CopyJob {
+  const double* src;
+  double* dst;
+  int count;
+};
+void copy_multi(const CopyJob* jobs, int nj, hipStream_t s);
 // One lane of interleaved copies of a real vector: x64[i·stride + lane] (FP64)
 // and x32[i·stride + lane] (FP32), either optional. Windows solved in
 // lockstep keep their next vectors side by side this way, so that one
@@ -115,7 +133,7 @@ void lanczos_finish(bool c, const void* w, size_t n, const void* d_h, int j, dou
 // the interleaved copies of a real vector (one lane)
 void scatter_lane(const double* x, size_t n, const Interleave& il, hipStream_t s);
 // x[i·width + b] = src[b][i] for b < width (src[b] == nullptr: zero), as
-// FP64 (x64) or FP32 (x32); width ≤ 8
+// FP64 (x64) or FP32 (x32); width 1, 2, 4, 8 or 16
 void interleave(const double* const* src, size_t n, int width, double* x64, float* x32, hipStream_t s);
 
 }  // namespace lk

@@ -280,9 +280,12 @@ class Operator {
   // doubles at d_amps + b·count) and the per-row diagonal (re[N], im[N] at
   // d_diags + b·2N) of the family at coefs[b] (real coefficients)
   int scan_batch_amp_count() const;
+  // the widest apply_scan_batch launch the lane tables fit (8 or 16)
+  int scan_batch_max_width() const;
   void scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, double* d_amps, double* d_diags,
                          hipStream_t s) const;
-  // y_b = H(c_b) x_b for the lanes b < width (1, 2, 4 or 8) of an input
+  // y_b = H(c_b) x_b for the lanes b < width (1, 2, 4, 8 or 16, at most
+  // scan_batch_max_width) of an input
   // interleaved as x[row·width + b] (FP32 or FP64); y[b] == nullptr skips a
   // lane, and a nonzero *gates[b] (optional) voids it like the single-vector
   // gate. amps[b] / diags[b]: the lane's point tables from scan_batch_tables.
@@ -466,18 +469,18 @@ struct Eigensystem {
 };
 
 // Lockstep Lanczos windows for several evaluations of one real family (scan
-// points): one window per point, all stepping together, with one
-// apply_scan_batch launch per step for every window's H·v (the windows'
-// next vectors interleaved, so the gathers of all points share cache lines).
+// points): one window per point, all stepping together, with
+// apply_scan_batch launches for every window's H·v (the windows' next
+// vectors interleaved, so the gathers of all points share cache lines).
 // Each window runs the single-window algorithm (same checks, FP32 schedule,
 // DGKS passes) at the default window size. Each step packs the next vectors
-// of the windows still stepping into consecutive lanes (launch width 1, 2, 4
-// or 8), so converged windows cost nothing. lanes64 / lanes32: the
-// interleaved inputs (grown as needed, kept by the caller between calls). A
-// point whose window fails comes back with ok == false, for the caller to
-// solve on its own.
+// of the windows still stepping into consecutive lanes, grouped by each
+// window's product precision (launch width 1, 2, 4, 8 or 16), so converged
+// windows cost nothing. lanes64 / lanes32: the interleaved inputs (grown as
+// needed, kept by the caller between calls). A point whose window fails
+// comes back with ok == false, for the caller to solve on its own.
 struct LockstepPoint {
-  const double* start;  // normalized FP64 start vector (device)
+  const double* start;  // FP64 start vector (device), normalized here
   const double* amps;   // the point's amplitude table (device)
   const double* diag;   // the point's per-row diagonal (device)
 };

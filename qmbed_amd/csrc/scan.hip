@@ -609,8 +609,6 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
   oo.eigenpairs = 1;
   EigshControls ctl1 = ctl;
   ctl1.keep_vectors = false;  // energies only
-  lk::Workspace ws;
-  DeviceArray<double> slot(2);
   for (int i0 = 0; i0 < n; i0 += W) {
     const int nb = std::min(W, n - i0);
     auto t0 = std::chrono::steady_clock::now();
@@ -661,13 +659,10 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
         QMBED_HIP(hipMemcpyAsync(lk_y_.data(), Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s_));
         lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
       }
-      std::vector<LockstepPoint> pts;
-      for (int l = 0; l < nl; ++l) {
-        double* v = lk_starts_.data() + size_t(l) * N;
-        lk::nrm2sq_to_slot(false, v, N, slot.data(), ws, s_);
-        lk::normalize(false, v, slot.data(), N, v, nullptr, nullptr, s_);
-        pts.push_back({v, lk_amps_.data() + size_t(l) * count, lk_diags_.data() + size_t(l) * 2 * N});
-      }
+      std::vector<LockstepPoint> pts;  // starts normalized by eigsh_lockstep
+      for (int l = 0; l < nl; ++l)
+        pts.push_back({lk_starts_.data() + size_t(l) * N, lk_amps_.data() + size_t(l) * count,
+                       lk_diags_.data() + size_t(l) * 2 * N});
       const auto res = eigsh_lockstep(op_, oo, ctl1, pts, lk_lanes64_, lk_lanes32_, s_);
       QMBED_HIP(hipStreamSynchronize(s_));  // Y dies here
       const double dt = seconds_since(t0);
