@@ -155,13 +155,28 @@ struct WindowBuffers {
   double* pin = nullptr;  // pinned read-back area
   size_t pin_n = 0;
   hipEvent_t ev = nullptr;
+  double* pin2 = nullptr;  // pinned area of a convergence test (coefficients up, results down)
+  size_t pin2_n = 0;
+  hipEvent_t ev2 = nullptr;
   WindowBuffers() {
     QMBED_HIP(hipGetDevice(&device));
     QMBED_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    QMBED_HIP(hipEventCreateWithFlags(&ev2, hipEventDisableTiming));
   }
   ~WindowBuffers() {
     if (pin) (void)hipHostFree(pin);
     if (ev) (void)hipEventDestroy(ev);
+    if (pin2) (void)hipHostFree(pin2);
+    if (ev2) (void)hipEventDestroy(ev2);
+  }
+  // no copy through pin2 may be in flight
+  void ensure_pin2(size_t n) {
+    if (pin2_n >= n) return;
+    if (pin2) QMBED_HIP(hipHostFree(pin2));
+    pin2 = nullptr;
+    pin2_n = 0;
+    QMBED_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin2), n * sizeof(double), hipHostMallocDefault));
+    pin2_n = n;
   }
   void ensure_pin(size_t n) {
     if (pin_n >= n) return;
@@ -539,29 +554,58 @@ class LanczosWindow {
   // and residuals.
   const double* ritz_vectors() const { return buf_->ritz.data(); }
   std::vector<Candidate> ritz(int k, SpectrumTarget target, double shift) {
+    ritz_post(k, target, shift);
+    return ritz_collect();
+  }
+  // ritz() in two halves: ritz_post launches the Ritz vectors and their FP64
+  // tests, the results copied into pinned memory (the coefficients go up
+  // through the same area); ritz_collect waits for them. Between the two the
+  // window must not step (its w() holds the products of the test).
+  void ritz_post(int k, SpectrumTarget target, double shift) {
     DeviceArray<double>& ritz_buf = buf_->ritz;
     const int sz = size_;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
     std::vector<double> b(beta_.begin(), beta_.begin() + std::max(0, sz - 1));
     std::vector<std::vector<double>> vecs;
     const auto vals = selected_ritz_values(a, b, target, k, shift, &vecs);
-    const size_t e = esz();
-    std::vector<double> C(e * size_t(sz) * vals.size(), 0.0);
-    for (size_t q = 0; q < vals.size(); ++q)
+    const size_t e = esz(), nv = vals.size(), nc = e * size_t(sz) * nv;
+    buf_->ensure_pin2(nc + 3 * nv);  // the previous test has been collected
+    double* C = buf_->pin2;
+    std::fill(C, C + nc, 0.0);
+    for (size_t q = 0; q < nv; ++q)
       for (int jj = 0; jj < sz; ++jj) C[e * (q * sz + jj)] = vecs[q][size_t(jj)];
-    grow(buf_->coef, C.size());  // rayleigh() below synchronizes before C dies
-    QMBED_HIP(hipMemcpyAsync(buf_->coef.data(), C.data(), C.size() * 8, hipMemcpyHostToDevice, s_));
-    grow(ritz_buf, e * n_ * vals.size());
-    lk::gemm_vc(c_, V(), n_, sz, buf_->coef.data(), int(vals.size()), ritz_buf.data(), s_);
-    auto rr = rayleigh(ritz_buf.data(), int(vals.size()));
+    grow(buf_->coef, nc);
+    QMBED_HIP(hipMemcpyAsync(buf_->coef.data(), C, nc * 8, hipMemcpyHostToDevice, s_));
+    grow(ritz_buf, e * n_ * nv);
+    lk::gemm_vc(c_, V(), n_, sz, buf_->coef.data(), int(nv), ritz_buf.data(), s_);
+    rayleigh_launch(ritz_buf.data(), int(nv));
+    QMBED_HIP(hipMemcpyAsync(C + nc, scratch_, 3 * nv * sizeof(double), hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipEventRecord(buf_->ev2, s_));
+    ritz_nc_ = nc;
+    ritz_nv_ = nv;
+  }
+  std::vector<Candidate> ritz_collect() {
+    QMBED_HIP(hipEventSynchronize(buf_->ev2));
+    const double* h = buf_->pin2 + ritz_nc_;
     std::vector<Candidate> out;
-    for (size_t q = 0; q < vals.size(); ++q) out.push_back({rr[q].first, rr[q].second, int(q)});
+    for (size_t q = 0; q < ritz_nv_; ++q) out.push_back({h[3 * q + 1], h[3 * q + 2], int(q)});
     return out;
   }
 
   // Normalize the nv columns of Y in place; FP64 Rayleigh quotients and
   // residual norms, read back with one synchronization.
   std::vector<std::pair<double, double>> rayleigh(double* Y, int nv) {
+    rayleigh_launch(Y, nv);
+    std::vector<double> host(3 * size_t(nv));
+    QMBED_HIP(hipMemcpyAsync(host.data(), scratch_, host.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    QMBED_HIP(hipStreamSynchronize(s_));
+    std::vector<std::pair<double, double>> out;
+    for (int q = 0; q < nv; ++q) out.push_back({host[3 * size_t(q) + 1], host[3 * size_t(q) + 2]});
+    return out;
+  }
+  // the launches of rayleigh(): scratch_[3q + 1] = Rayleigh quotient,
+  // scratch_[3q + 2] = residual norm of column q
+  void rayleigh_launch(double* Y, int nv) {
     if (3 * size_t(nv) > 3 * size_t(m_ + 1) + 64) throw internal_state("Rayleigh batch exceeds the slot area");
     const size_t e = esz();
     for (int q = 0; q < nv; ++q) {
@@ -576,12 +620,6 @@ class LanczosWindow {
       lk::redot_to_slot(c_, y, w(), n_, sl + 1, buf_->ws, s_);
       lk::residual_to_slot(c_, w(), y, sl + 1, n_, sl + 2, buf_->ws, s_);
     }
-    std::vector<double> host(3 * size_t(nv));
-    QMBED_HIP(hipMemcpyAsync(host.data(), scratch_, host.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipStreamSynchronize(s_));
-    std::vector<std::pair<double, double>> out;
-    for (int q = 0; q < nv; ++q) out.push_back({host[3 * size_t(q) + 1], host[3 * size_t(q) + 2]});
-    return out;
   }
 
   long matvecs64() const { return matvecs64_; }
@@ -620,6 +658,7 @@ class LanczosWindow {
   int posted_ = 0;
   PostBatch* pb_ = nullptr;
   long post_gen_ = 0;
+  size_t ritz_nc_ = 0, ritz_nv_ = 0;
   int size_ = 0;
   long passes_ = 0, seconds_ = 0, matvecs_ = 0, matvecs64_ = 0, matvecs32_ = 0;
   bool broke_ = false;
@@ -689,12 +728,23 @@ class WindowRun {
     QMBED_HIP(hipStreamSynchronize(s_));
   }
   // this tick launches a step
-  bool stepping() const { return !done_ && win_.launched() < m_; }
+  bool stepping() const { return !done_ && !verifying_ && win_.launched() < m_; }
+  // Convergence tests without a host wait (lockstep windows): a tick that
+  // would test launches the test and returns; the window's next tick, when
+  // the device has long run it (other windows' steps came after it), reads
+  // the results and decides. The window launches nothing in between, so its
+  // device work and results are those of the waiting form.
+  void set_async_tests(bool on) { async_tests_ = on; }
   LanczosWindow& window() { return win_; }
 
   // stepped: the caller has launched this tick's step (lockstep phases)
   void tick(bool stepped = false) {
     if (done_) return;
+    if (verifying_) {
+      verifying_ = false;
+      decide(win_.ritz_collect(), verify_at_end_);
+      return;
+    }
     if (stepped) win_.advance();
     else if (win_.launched() < m_) win_.step();
     const int launched = win_.launched();
@@ -747,7 +797,19 @@ class WindowRun {
       out_.iterations = win_.matvecs();
       throw non_convergence(sz, INFINITY);
     }
-    auto cands = win_.ritz(k_, o_.target, o_.shift);
+    if (async_tests_) {
+      win_.ritz_post(k_, o_.target, o_.shift);
+      verifying_ = true;
+      verify_at_end_ = at_end;
+      return;
+    }
+    decide(win_.ritz(k_, o_.target, o_.shift), at_end);
+  }
+
+ private:
+  // the FP64 test of the candidates: accept, continue, or fail
+  void decide(std::vector<Candidate> cands, bool at_end) {
+    const int sz = win_.size();
     std::stable_sort(cands.begin(), cands.end(), [&](const Candidate& a, const Candidate& b) {
       return target_before(o_.target == SpectrumTarget::BothEnds ? SpectrumTarget::SmallestAlgebraic : o_.target,
                            a.value, b.value, o_.shift);
@@ -805,6 +867,7 @@ class WindowRun {
   LanczosWindow win_;
   int k_ = 1, next_check_ = 0, lag_ = 2, interval_ = 1, next_sync_ = 1;
   bool pending_ = false, pending_conv_ = false, done_ = false;
+  bool async_tests_ = false, verifying_ = false, verify_at_end_ = false;
 };
 
 void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
@@ -924,7 +987,10 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   }
   // the windows' read-backs of a tick go in one copy launch
   PostBatch pb(s);
-  for (auto& r : runs) r->window().set_post_batch(&pb);
+  for (auto& r : runs) {
+    r->window().set_post_batch(&pb);
+    r->set_async_tests(true);
+  }
   std::vector<char> failed(static_cast<size_t>(nb), 0);
   // every tick of a window launches a step or collects a read-back, so a
   // window ends within a few ticks per step of its window size
