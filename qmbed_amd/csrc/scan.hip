@@ -609,23 +609,45 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
   oo.eigenpairs = 1;
   EigshControls ctl1 = ctl;
   ctl1.keep_vectors = false;  // energies only
+  // the continuation coefficients of a batch, on host threads
+  auto continuations = [&](int i0) {
+    std::vector<std::future<std::vector<double>>> fy;
+    for (int b = 0; b < std::min(W, n - i0); ++b) {
+      const double* p = points + size_t(i0 + b) * P_;
+      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+    }
+    return fy;
+  };
+  std::vector<std::future<std::vector<double>>> next_y;
+  if (start == Start::SnapshotRR) next_y = continuations(0);
   for (int i0 = 0; i0 < n; i0 += W) {
     const int nb = std::min(W, n - i0);
     auto t0 = std::chrono::steady_clock::now();
-    // the start coefficients: the continuation (host threads) or the network
-    // (one forward pass for the batch); the plain start needs none
+    // the start coefficients: the continuation (host threads, computed while
+    // the previous batch ran on the device; the next batch's start here) or
+    // the network (one forward pass for the batch); the plain start needs none
     std::vector<std::vector<double>> ys(static_cast<size_t>(nb));
-    if (start == Start::SnapshotRR) {
-      std::vector<std::future<std::vector<double>>> fy;
-      for (int b = 0; b < nb; ++b) {
-        const double* p = points + size_t(i0 + b) * P_;
-        fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
-      }
-      for (int b = 0; b < nb; ++b) ys[size_t(b)] = fy[size_t(b)].get();
-    } else if (start == Start::Network) {
+    if (start == Start::Network) {  // before the tables: predict waits for the stream
       const auto y = predict(points + size_t(i0) * P_, nb);
       for (int b = 0; b < nb; ++b)
         ys[size_t(b)].assign(y.begin() + size_t(b) * ns_, y.begin() + size_t(b + 1) * ns_);
+    }
+    // the tables of every point of the batch (indexed by point), set up
+    // while the continuation threads run; the batch tables and starts live
+    // in grow-only members: no device allocation (and no synchronizing free)
+    // per batch
+    auto grow = [](DeviceArray<double>& d, size_t want) {
+      if (d.size() < want) d.resize(want);
+    };
+    std::vector<std::vector<cplx>> coefs;
+    for (int b = 0; b < nb; ++b) coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
+    grow(lk_amps_, size_t(nb) * size_t(count));
+    grow(lk_diags_, size_t(nb) * 2 * N);
+    op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
+    if (start == Start::SnapshotRR) {
+      for (int b = 0; b < nb; ++b) ys[size_t(b)] = next_y[size_t(b)].get();
+      next_y.clear();
+      if (i0 + W < n) next_y = continuations(i0 + W);
     }
     std::vector<int> lanes;  // points with a usable start
     for (int b = 0; b < nb; ++b) {
@@ -636,35 +658,30 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
     std::vector<char> solved(static_cast<size_t>(nb), 0);
     if (lanes.size() >= 2) {
       const int nl = int(lanes.size());
-      std::vector<std::vector<cplx>> coefs;
-      std::vector<double> Y(start == Start::Plain ? 0 : size_t(ns_) * size_t(nl));
-      for (int l = 0; l < nl; ++l) {
-        const int b = lanes[size_t(l)];
-        coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
-        if (start != Start::Plain) std::copy(ys[size_t(b)].begin(), ys[size_t(b)].end(), Y.begin() + size_t(l) * ns_);
-      }
-      // the batch tables and starts live in grow-only members: no device
-      // allocation (and no synchronizing free) per batch
-      auto grow = [](DeviceArray<double>& d, size_t want) {
-        if (d.size() < want) d.resize(want);
-      };
-      grow(lk_amps_, size_t(nl) * size_t(count));
-      grow(lk_diags_, size_t(nl) * 2 * N);
       grow(lk_starts_, size_t(nl) * N);
-      grow(lk_y_, Y.size());
-      op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
       if (start == Start::Plain) {
         for (int l = 0; l < nl; ++l) deterministic_start_device(lk_starts_.data() + size_t(l) * N, N, oo.seed, s_, false);
       } else {
-        QMBED_HIP(hipMemcpyAsync(lk_y_.data(), Y.data(), Y.size() * 8, hipMemcpyHostToDevice, s_));
+        // the coefficients go up through pinned memory (free: the previous
+        // batch ended with a synchronization)
+        const size_t ny = size_t(ns_) * size_t(nl);
+        cpin_.ensure(ny);
+        for (int l = 0; l < nl; ++l) {
+          const auto& y = ys[size_t(lanes[size_t(l)])];
+          std::copy(y.begin(), y.end(), cpin_.p + size_t(l) * ns_);
+        }
+        grow(lk_y_, ny);
+        QMBED_HIP(hipMemcpyAsync(lk_y_.data(), cpin_.p, ny * 8, hipMemcpyHostToDevice, s_));
         lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
       }
       std::vector<LockstepPoint> pts;  // starts normalized by eigsh_lockstep
-      for (int l = 0; l < nl; ++l)
-        pts.push_back({lk_starts_.data() + size_t(l) * N, lk_amps_.data() + size_t(l) * count,
-                       lk_diags_.data() + size_t(l) * 2 * N});
+      for (int l = 0; l < nl; ++l) {
+        const int b = lanes[size_t(l)];
+        pts.push_back({lk_starts_.data() + size_t(l) * N, lk_amps_.data() + size_t(b) * count,
+                       lk_diags_.data() + size_t(b) * 2 * N});
+      }
       const auto res = eigsh_lockstep(op_, oo, ctl1, pts, lk_lanes64_, lk_lanes32_, s_);
-      QMBED_HIP(hipStreamSynchronize(s_));  // Y dies here
+      QMBED_HIP(hipStreamSynchronize(s_));  // cpin_ is free again
       const double dt = seconds_since(t0);
       for (int l = 0; l < nl; ++l) {
         if (!res[size_t(l)].ok) continue;
