@@ -396,22 +396,15 @@ class LanczosWindow {
   int* halt() const { return flags_ + m_; }
 
   // The phases of step j = launched() for windows stepping in lockstep (the
-  // caller launched H·v_j into w(), and launches the α, reduction and
-  // finishing phases of all windows together): the same launches as step().
+  // caller launched H·v_j into w(), and launches each phase of all windows
+  // together): the same work as step().
   lk::ThreeTerm three_term() const {
     const int j = launched_;
     return lk::ThreeTerm{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr, slots() + j};
   }
   lk::AlphaJob alpha_job() { return {col(launched_), w(), &buf_->ws, halt()}; }
-  void overlaps_phase() {
-    const lk::ThreeTerm three = three_term();
-    lk::lanczos_overlaps(c_, V(), n_, launched_ + 1, w(), buf_->ws, s_, true, halt(), &three);
-  }
   lk::HredJob hred_job() { return {&buf_->ws, launched_ + 1, h_, halt()}; }
-  void subtract_phase() {
-    const lk::ThreeTerm three = three_term();
-    lk::lanczos_subtract(c_, V(), n_, launched_ + 1, w(), h_, buf_->ws, s_, true, halt(), &three, false);
-  }
+  lk::PassJob pass_job() { return {V(), launched_ + 1, w(), &buf_->ws, h_, halt(), three_term()}; }
   lk::FinishJob finish_job() {
     const int j = launched_;
     return {w(), &buf_->ws, h_, j, slots() + m_ + j, halt(), flags_ + j, j + 1 < m_ ? col(j + 1) : nullptr};
@@ -1037,9 +1030,9 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       }
     }
     if (!active) break;
-    // the rest of the step: α of every stepping window in one launch, the
-    // overlaps per window, their reductions in one launch, the subtractions
-    // per window, the finishing kernels in one launch
+    // the rest of the step, each phase in one launch for every stepping
+    // window: α, the overlaps (one launch per column tile in use), their
+    // reductions, the subtractions, the finishing kernels
     std::vector<int> stepping;
     for (int b = 0; b < nb; ++b) {
       auto& r = *runs[size_t(b)];
@@ -1051,10 +1044,12 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       std::vector<lk::FinishJob> fj;
       for (int b : stepping) aj.push_back(runs[size_t(b)]->window().alpha_job());
       lk::lanczos_alpha_multi(aj.data(), int(aj.size()), n, s);
-      for (int b : stepping) runs[size_t(b)]->window().overlaps_phase();
+      std::vector<lk::PassJob> pj;
+      for (int b : stepping) pj.push_back(runs[size_t(b)]->window().pass_job());
+      lk::lanczos_overlaps_multi(pj.data(), int(pj.size()), n, s);
       for (int b : stepping) hj.push_back(runs[size_t(b)]->window().hred_job());
       lk::lanczos_hred_multi(hj.data(), int(hj.size()), n, s);
-      for (int b : stepping) runs[size_t(b)]->window().subtract_phase();
+      lk::lanczos_subtract_multi(pj.data(), int(pj.size()), n, s);
       for (int b : stepping) fj.push_back(runs[size_t(b)]->window().finish_job());
       lk::lanczos_finish_multi(fj.data(), int(fj.size()), n, ctl.dgks_eta, s);
     }

@@ -168,17 +168,19 @@ __device__ __forceinline__ E three_apply(const Three<E>& t, E x, size_t i, doubl
 // is unconditional (row and column indices are clamped, out-of-range rows
 // carry w₀ = 0), so the CT·RPT loads of a thread are in flight together.
 // Output: partial[x·ld + col_off + c].
-template <typename E, int CT, int RPT = kRPT>
-__global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, int m, int mm, const E* __restrict__ w,
-                                             E* __restrict__ partial, int ld, int col_off, const int* gate,
-                                             Three<E> t) {
+// (bx, by, nbx: the block's row block, column group, and the number of row
+// blocks, so that one launch can hold the blocks of several windows)
+template <typename E, int CT, int RPT>
+__device__ __forceinline__ void dots_body(const E* __restrict__ V, size_t n, int m, int mm, const E* __restrict__ w,
+                                          E* __restrict__ partial, int ld, int col_off, const int* gate,
+                                          const Three<E>& t, unsigned bx, unsigned by, unsigned nbx) {
   if (gate && *gate) return;
   __shared__ E sh[CT][kT / 64];
   __shared__ double shd[kT / 64];
   __shared__ double bc;
   double alpha = 0, beta = 0;
   if (t.vj) three_coefs(t, shd, &bc, alpha, beta);
-  const int c0 = blockIdx.y * CT;
+  const int c0 = by * CT;
   const int nc = min(CT, mm - c0);
   const size_t tiles = (n + size_t(kT * RPT) - 1) / size_t(kT * RPT);
   E acc[CT];
@@ -190,7 +192,7 @@ __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, 
   const int cj = m - 1 - c0, cjm1 = m - 2 - c0;
   const bool tile_j = t.vj && t.vj == V + size_t(m - 1) * n && cj >= 0 && cj < CT;
   const bool tile_jm1 = t.vjm1 && m >= 2 && t.vjm1 == V + size_t(m - 2) * n && cjm1 >= 0 && cjm1 < CT;
-  for (size_t tt = blockIdx.x; tt < tiles; tt += gridDim.x) {
+  for (size_t tt = bx; tt < tiles; tt += nbx) {
     const size_t r0 = tt * (kT * RPT) + threadIdx.x;
     size_t ic[RPT];
 #pragma unroll
@@ -243,8 +245,14 @@ __global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, 
   if (threadIdx.x < nc) {
     E s = zero<E>();
     for (int w2 = 0; w2 < kT / 64; ++w2) s = add(s, sh[threadIdx.x][w2]);
-    partial[size_t(blockIdx.x) * ld + col_off + c0 + threadIdx.x] = s;
+    partial[size_t(bx) * ld + col_off + c0 + threadIdx.x] = s;
   }
+}
+template <typename E, int CT, int RPT = kRPT>
+__global__ __launch_bounds__(kT) void k_dots(const E* __restrict__ V, size_t n, int m, int mm, const E* __restrict__ w,
+                                             E* __restrict__ partial, int ld, int col_off, const int* gate,
+                                             Three<E> t) {
+  dots_body<E, CT, RPT>(V, n, m, mm, w, partial, ld, col_off, gate, t, blockIdx.x, blockIdx.y, gridDim.x);
 }
 
 // h[c] = Σ_b partial[b·ld + c] over the rb row blocks of the overlap kernel,
@@ -280,9 +288,9 @@ __global__ __launch_bounds__(kT) void k_hred(const E* __restrict__ partial, int 
 // block 0 also stores α_j. With partial2 (last chunk) the per-block ‖w‖²
 // partials of the result follow.
 template <typename E>
-__global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, int mc, const E* __restrict__ hc,
-                                            E* __restrict__ w, double* __restrict__ partial2, const int* gate,
-                                            Three<E> t) {
+__device__ __forceinline__ void sub_body(const E* __restrict__ V, size_t n, int mc, const E* __restrict__ hc,
+                                         E* __restrict__ w, double* __restrict__ partial2, const int* gate,
+                                         const Three<E>& t, unsigned bx, unsigned nbx) {
   if (gate && *gate) return;
   __shared__ E sh_h[kMaxCols];
   __shared__ double sh[kT / 64];
@@ -290,7 +298,7 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
   double alpha = 0, beta = 0;
   if (t.vj) {
     three_coefs(t, sh, &bc, alpha, beta);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *t.alpha = alpha;
+    if (bx == 0 && threadIdx.x == 0) *t.alpha = alpha;
   }
   for (int c = threadIdx.x; c < mc; c += kT) sh_h[c] = hc[c];
   __syncthreads();
@@ -300,7 +308,7 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
   const bool last_jm1 = last_j && t.vjm1 && mc >= 2 && t.vjm1 == V + size_t(mc - 2) * n;
   const int cend = mc - (last_j ? (last_jm1 ? 2 : 1) : 0);
   double s = 0;
-  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) {
+  for (size_t i = bx * size_t(kT) + threadIdx.x; i < n; i += size_t(nbx) * kT) {
     E acc = zero<E>();
     int c = 0;
     for (; c + 4 <= cend; c += 4) {
@@ -330,7 +338,13 @@ __global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, i
   }
   if (!partial2) return;
   s = block_sum(s, sh);
-  if (threadIdx.x == 0) partial2[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial2[bx] = s;
+}
+template <typename E>
+__global__ __launch_bounds__(kT) void k_sub(const E* __restrict__ V, size_t n, int mc, const E* __restrict__ hc,
+                                            E* __restrict__ w, double* __restrict__ partial2, const int* gate,
+                                            Three<E> t) {
+  sub_body<E>(V, n, mc, hc, w, partial2, gate, t, blockIdx.x, gridDim.x);
 }
 
 // End of a Lanczos step. Every block forms ‖w‖² = Σ partial2 (fixed order).
@@ -859,6 +873,95 @@ void lanczos_subtract(bool c, const void* V, size_t n, int m, void* w, void* d_h
                          mE<E>(w), last ? ws.partial2.data() : nullptr, gate, c0 == 0 ? t : Three<E>());
     }
   });
+}
+
+// The overlap and subtraction passes of several lockstep windows (real, one
+// column chunk each, first passes) in one launch each: blockIdx.z = window,
+// and each window's blocks compute exactly what its own launch would.
+struct DotsJobs {
+  const double* V[kMaxJobs];
+  const double* w[kMaxJobs];
+  double* partial[kMaxJobs];
+  const int* gate[kMaxJobs];
+  Three<double> t[kMaxJobs];
+  int m[kMaxJobs], rb[kMaxJobs], groups[kMaxJobs], ld[kMaxJobs];
+};
+template <int CT>
+__global__ __launch_bounds__(kT) void k_dots_multi(DotsJobs J, size_t n) {
+  const int q = blockIdx.z;
+  if (int(blockIdx.x) >= J.rb[q] || int(blockIdx.y) >= J.groups[q]) return;
+  dots_body<double, CT, kRPT>(J.V[q], n, J.m[q], J.m[q] + 1, J.w[q], J.partial[q], J.ld[q], 0, J.gate[q], J.t[q],
+                              blockIdx.x, blockIdx.y, unsigned(J.rb[q]));
+}
+struct SubJobs {
+  const double* V[kMaxJobs];
+  const double* h[kMaxJobs];
+  double* w[kMaxJobs];
+  double* partial2[kMaxJobs];
+  const int* gate[kMaxJobs];
+  Three<double> t[kMaxJobs];
+  int m[kMaxJobs];
+};
+__global__ __launch_bounds__(kT) void k_sub_multi(SubJobs J, size_t n) {
+  const int q = blockIdx.y;
+  sub_body<double>(J.V[q], n, J.m[q], J.h[q], J.w[q], J.partial2[q], J.gate[q], J.t[q], blockIdx.x, gridDim.x);
+}
+
+void lanczos_overlaps_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep phase launch of 1 to 16 windows");
+  // one launch per column tile in use (the windows of a batch mostly share one)
+  for (int ct : {4, 8, 12, 16}) {
+    DotsJobs J{};
+    int k = 0, rbmax = 0, gmax = 0;
+    for (int q = 0; q < nj; ++q) {
+      const PassJob& jb = jobs[q];
+      if (jb.m < 1 || jb.m > kMaxCols) throw internal_state("lockstep overlaps of a window beyond one column chunk");
+      if (dots_ct<double>(jb.m + 1) != ct) continue;
+      int rb, ld;
+      pass_layout<double>(n, jb.m, true, rb, ld);
+      const size_t need = size_t(rb) * size_t(ld);
+      if (jb.ws->partial.size() < need) jb.ws->partial.resize(need * 2);
+      J.V[k] = jb.V;
+      J.w[k] = jb.w;
+      J.partial[k] = jb.ws->partial.data();
+      J.gate[k] = jb.gate;
+      J.t[k] = three_of<double>(&jb.three, *jb.ws, n);
+      J.m[k] = jb.m;
+      J.rb[k] = rb;
+      J.groups[k] = chunk_groups<double>(jb.m + 1);
+      J.ld[k] = ld;
+      rbmax = std::max(rbmax, rb);
+      gmax = std::max(gmax, J.groups[k]);
+      ++k;
+    }
+    if (k == 0) continue;
+    const dim3 grid{unsigned(rbmax), unsigned(gmax), unsigned(k)};
+    switch (ct) {
+      case 4: hipLaunchKernelGGL(k_dots_multi<4>, grid, dim3(kT), 0, s, J, n); break;
+      case 8: hipLaunchKernelGGL(k_dots_multi<8>, grid, dim3(kT), 0, s, J, n); break;
+      case 12: hipLaunchKernelGGL(k_dots_multi<12>, grid, dim3(kT), 0, s, J, n); break;
+      default: hipLaunchKernelGGL(k_dots_multi<16>, grid, dim3(kT), 0, s, J, n); break;
+    }
+    QMBED_HIP(hipGetLastError());
+  }
+}
+
+void lanczos_subtract_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s) {
+  if (nj < 1 || nj > kMaxJobs) throw internal_state("lockstep phase launch of 1 to 16 windows");
+  SubJobs J{};
+  for (int q = 0; q < nj; ++q) {
+    const PassJob& jb = jobs[q];
+    if (jb.m < 1 || jb.m > kMaxCols) throw internal_state("lockstep subtraction of a window beyond one column chunk");
+    J.V[q] = jb.V;
+    J.h[q] = jb.h;
+    J.w[q] = jb.w;
+    J.partial2[q] = jb.ws->partial2.data();
+    J.gate[q] = jb.gate;
+    J.t[q] = three_of<double>(&jb.three, *jb.ws, n);
+    J.m[q] = jb.m;
+  }
+  hipLaunchKernelGGL(k_sub_multi, dim3(unsigned(sub_blocks(n)), unsigned(nj)), dim3(kT), 0, s, J, n);
+  QMBED_HIP(hipGetLastError());
 }
 
 void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s) {

@@ -114,6 +114,22 @@ struct CopyJob {
   int count;
 };
 void copy_multi(const CopyJob* jobs, int nj, hipStream_t s);
+// The overlap pass (lanczos_overlaps with ‖w₀‖²) and the subtraction
+// (lanczos_subtract, reduce = false) of the first pass of a step, for up to
+// 16 lockstep windows in one launch each (the overlaps: one per column tile
+// in use); every window's blocks compute exactly what its own launches would.
+// m = j + 1 columns, at most one column chunk.
+struct PassJob {
+  const double* V;
+  int m;
+  double* w;
+  Workspace* ws;
+  const double* h;  // the reduced overlaps (subtraction)
+  const int* gate;
+  ThreeTerm three;
+};
+void lanczos_overlaps_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s);
+void lanczos_subtract_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s);
 // One lane of interleaved copies of a real vector: x64[i·stride + lane] (FP64)
 // and x32[i·stride + lane] (FP32), either optional. Windows solved in
 // lockstep keep their next vectors side by side this way, so that one
