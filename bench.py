@@ -14,7 +14,8 @@ it, and the 2-64-64-32 FP32 network trained on the GPU by the library's own
 kernels.
 
 One step = 16 held-out grid points (one lockstep batch: a Lanczos window per
-point, one shared H·v launch per step for every 8 windows still running),
+point, shared H·v launches per step over the windows still running, up to 16
+FP32 or 8 FP64 vectors per launch),
 each solved to residual
 ≤ 1e-8·max(|E|,1) with the FP32-matvec schedule from the snapshot start: the
 lowest eigenvector of the family projected on the 32-dimensional snapshot
@@ -458,8 +459,9 @@ def main():
                                "space (the 32 leading principal components of the ground states of 256 plain-Lanczos "
                                "training points): v0 = lowest eigenvector of the family projected on that space (the 2-64-64-32 FP32 MLP start is reported in "
                                "extras.scan.network_start_fp32) + FP32-matvec schedule; ground state to residual "
-                               "<= 1e-8*max(|E|,1); 16 points per step, solved in one lockstep batch (one shared H.v launch per "
-                               "Lanczos step for every 8 points still running, over their interleaved vectors); baseline "
+                               "<= 1e-8*max(|E|,1); 16 points per step, solved in one lockstep batch (shared H.v launches per "
+                               "Lanczos step over the interleaved vectors of the points still running, up to 16 FP32 "
+                               "or 8 FP64 vectors per launch; the other phases one launch for all points); baseline "
                                "plain FP64 one eigsh per point",
                    "global_batch": BATCH},
         "roofline": {"bound": "hbm", "achieved": hvmix["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -168,10 +168,13 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * Answers energies, residuals, iterations (total, FP32-matvec and FP64),
  * per-point seconds and fallback count. The points are solved (any start)
  * in lockstep batches of up to "lockstep" (solver key, 0-16, default 16; 0
- * or 1: one by one): one Lanczos window per point, one shared H·v launch per
- * step and 8 windows over the interleaved vectors of the windows still stepping (each
- * point with its own couplings). Each window runs the single-window
- * algorithm; a point whose window fails is solved on its own. Needs a real
+ * or 1: one by one): one Lanczos window per point; per step, shared H·v
+ * launches over the interleaved vectors of the windows still stepping (each
+ * point with its own couplings; the windows grouped by their own product
+ * precision, up to 16 FP32 or 8 FP64 vectors per launch), and every other
+ * phase of the step in one launch for all of them. Each window runs the
+ * single-window algorithm, so results equal one-by-one solves bit for bit;
+ * a point whose window fails is solved on its own. Needs a real
  * family with Hermitian parts, no momentum phases and the compact compiled
  * stream; otherwise the points are solved one by one. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);

@@ -993,7 +993,7 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
     // the windows stepping now, by the precision of their products (each
     // window's own FP32 switch, as in a single solve), packed into
     // consecutive lanes: launches of up to 8 lanes, or one of 16 for 13 or
-    // more FP32 windows (a 16-lane FP32 launch costs 0.85 of two 8-lane ones;
+    // more FP32 windows (a 16-lane FP32 launch costs 0.88 of two 8-lane ones;
     // in FP64 it costs more than two, at half the occupancy)
     bool active = false;
     for (int prec = 0; prec < 2; ++prec) {
