@@ -728,6 +728,9 @@ class WindowRun {
   // the results and decides. The window launches nothing in between, so its
   // device work and results are those of the waiting form.
   void set_async_tests(bool on) { async_tests_ = on; }
+  // the accepted Ritz vector (one wanted pair) is copied to d (device, e·N),
+  // in stream order, instead of into the result
+  void set_vector_destination(double* d) { vec_dst_ = d; }
   LanczosWindow& window() { return win_; }
 
   // stepped: the caller has launched this tick's step (lockstep phases)
@@ -832,7 +835,13 @@ class WindowRun {
       Error e = non_convergence(sz, fail);
       throw e;
     }
-    if (ctl_.keep_vectors) copy_vectors(cands);
+    if (vec_dst_ && cands.size() == 1) {
+      const size_t e = c_ ? 2 : 1, n = op_.dim();
+      QMBED_HIP(hipMemcpyAsync(vec_dst_, win_.ritz_vectors() + e * n * cands[0].column, e * n * 8,
+                               hipMemcpyDeviceToDevice, s_));
+    } else if (ctl_.keep_vectors) {
+      copy_vectors(cands);
+    }
     for (auto& cd : cands) {
       out_.es.eigenvalues.push_back(cd.value);
       out_.es.residuals.push_back(cd.residual);
@@ -861,6 +870,7 @@ class WindowRun {
   int k_ = 1, next_check_ = 0, lag_ = 2, interval_ = 1, next_sync_ = 1;
   bool pending_ = false, pending_conv_ = false, done_ = false;
   bool async_tests_ = false, verifying_ = false, verify_at_end_ = false;
+  double* vec_dst_ = nullptr;
 };
 
 void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
@@ -980,9 +990,11 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   }
   // the windows' read-backs of a tick go in one copy launch
   PostBatch pb(s);
-  for (auto& r : runs) {
+  for (int b = 0; b < nb; ++b) {
+    auto& r = runs[size_t(b)];
     r->window().set_post_batch(&pb);
     r->set_async_tests(true);
+    if (pts[size_t(b)].vector) r->set_vector_destination(pts[size_t(b)].vector);
   }
   std::vector<char> failed(static_cast<size_t>(nb), 0);
   // every tick of a window launches a step or collects a read-back, so a

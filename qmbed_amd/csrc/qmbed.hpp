@@ -483,6 +483,7 @@ struct LockstepPoint {
   const double* start;  // FP64 start vector (device), normalized here
   const double* amps;   // the point's amplitude table (device)
   const double* diag;   // the point's per-row diagonal (device)
+  double* vector = nullptr;  // optional: receives the accepted Ritz vector (device, N)
 };
 struct LockstepResult {
   Eigensystem es;

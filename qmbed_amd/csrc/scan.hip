@@ -89,6 +89,10 @@ __global__ __launch_bounds__(kT) void k_scale_copy(const double* __restrict__ x,
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) y[i] = f * x[i];
 }
 
+__global__ __launch_bounds__(kT) void k_scale(double* __restrict__ x, size_t n, double f) {
+  for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) x[i] = f * x[i];
+}
+
 __global__ __launch_bounds__(kT) void k_fill(double* __restrict__ x, size_t n, double v) {
   for (size_t i = blockIdx.x * size_t(kT) + threadIdx.x; i < n; i += size_t(gridDim.x) * kT) x[i] = v;
 }
@@ -428,14 +432,12 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   const int n_plain = opt_.provisional ? std::min(n, opt_.snapshots) : n;
   bool provisional = false;
   DeviceArray<double> v0;
-  for (int i = 0; i < n; ++i) {
-    if (i == n_plain && n_plain < n) {
-      std::vector<double> Lp;
-      std::vector<int> sp;
-      build_space(psi.data(), n_plain, Lp, sp);
-      provisional = ns_ > 0;
-      if (provisional) v0.resize(N);
-    }
+  rep.energies.assign(size_t(n), 0.0);
+  rep.iterations.assign(size_t(n), 0);
+  EigshOptions oo = o;
+  oo.eigenpairs = 1;
+  // one training solve on its own
+  auto solve_one = [&](int i) {
     const double* point = points + size_t(i) * P_;
     const double* start = nullptr;
     if (provisional) {
@@ -445,22 +447,101 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
       lk::gemm_vc(false, S_.data(), N, ns_, dy.data(), 1, v0.data(), s_);
       QMBED_HIP(hipStreamSynchronize(s_));  // y and dy die here
       start = v0.data();
-      ++rep.provisional_starts;
     }
     op_.evaluate_coefficients(coefficients_of(point), s_);
     if (!op_.hermitian()) throw non_hermitian();
     DeviceArray<double> vecs;
     bool cv = false;
-    EigshOptions oo = o;
-    oo.eigenpairs = 1;
     Eigensystem es = eigsh_on_device(op_, oo, start, false, ctl, vecs, cv, s_);
     if (cv) throw unsupported_backend("the learned warm start is built for real-valued families");
     const double sign = gauge_sign(vecs.data());
     hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(N)), dim3(kT), 0, s_, vecs.data(), N, sign,
                        psi.data() + N * size_t(i));
     QMBED_HIP(hipStreamSynchronize(s_));  // vecs dies here
-    rep.energies.push_back(es.eigenvalues[0]);
-    rep.iterations.push_back(es.iterations);
+    rep.energies[size_t(i)] = es.eigenvalues[0];
+    rep.iterations[size_t(i)] = es.iterations;
+  };
+  // up to `lockstep` training solves at a time in lockstep windows (the
+  // same windows, products and starts as one by one, so the same states);
+  // each accepted Ritz vector goes straight into its column of psi
+  auto solve_batch = [&](int i0, int nb, std::vector<char>& done) {
+    const int count = op_.scan_batch_amp_count();
+    std::vector<std::future<std::vector<double>>> fy;
+    if (provisional)
+      for (int b = 0; b < nb; ++b) {
+        const double* p = points + size_t(i0 + b) * P_;
+        fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+      }
+    std::vector<std::vector<cplx>> coefs;
+    for (int b = 0; b < nb; ++b) coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
+    if (lk_amps_.size() < size_t(nb) * size_t(count)) lk_amps_.resize(size_t(nb) * size_t(count));
+    if (lk_diags_.size() < size_t(nb) * 2 * N) lk_diags_.resize(size_t(nb) * 2 * N);
+    op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
+    std::vector<std::vector<double>> ys(static_cast<size_t>(nb));
+    for (size_t b = 0; b < fy.size(); ++b) ys[b] = fy[b].get();
+    std::vector<int> lanes;
+    for (int b = 0; b < nb; ++b) {
+      double cn = 0;
+      for (double v : ys[size_t(b)]) cn += v * v;
+      if (!provisional || (std::isfinite(cn) && cn > 1.0e-6)) lanes.push_back(b);
+    }
+    if (lanes.size() < 2) return;
+    const int nl = int(lanes.size());
+    if (lk_starts_.size() < size_t(nl) * N) lk_starts_.resize(size_t(nl) * N);
+    if (!provisional) {
+      for (int l = 0; l < nl; ++l) deterministic_start_device(lk_starts_.data() + size_t(l) * N, N, oo.seed, s_, false);
+    } else {
+      const size_t ny = size_t(ns_) * size_t(nl);
+      cpin_.ensure(ny);
+      for (int l = 0; l < nl; ++l) {
+        const auto& y = ys[size_t(lanes[size_t(l)])];
+        std::copy(y.begin(), y.end(), cpin_.p + size_t(l) * ns_);
+      }
+      if (lk_y_.size() < ny) lk_y_.resize(ny);
+      QMBED_HIP(hipMemcpyAsync(lk_y_.data(), cpin_.p, ny * 8, hipMemcpyHostToDevice, s_));
+      lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
+    }
+    std::vector<LockstepPoint> pts;
+    for (int l = 0; l < nl; ++l) {
+      const int b = lanes[size_t(l)];
+      pts.push_back({lk_starts_.data() + size_t(l) * N, lk_amps_.data() + size_t(b) * count,
+                     lk_diags_.data() + size_t(b) * 2 * N, psi.data() + N * size_t(i0 + b)});
+    }
+    EigshControls c1 = ctl;
+    c1.keep_vectors = false;  // the vectors go to psi
+    const auto res = eigsh_lockstep(op_, oo, c1, pts, lk_lanes64_, lk_lanes32_, s_);
+    QMBED_HIP(hipStreamSynchronize(s_));  // cpin_ is free again
+    for (int l = 0; l < nl; ++l) {
+      if (!res[size_t(l)].ok) continue;
+      const int i = i0 + lanes[size_t(l)];
+      double* col = psi.data() + N * size_t(i);
+      if (gauge_sign(col) < 0) hipLaunchKernelGGL(k_scale, dim3(grid_for(N)), dim3(kT), 0, s_, col, N, -1.0);
+      rep.energies[size_t(i)] = res[size_t(l)].es.eigenvalues[0];
+      rep.iterations[size_t(i)] = res[size_t(l)].es.iterations;
+      done[size_t(lanes[size_t(l)])] = 1;
+    }
+    QMBED_HIP(hipStreamSynchronize(s_));
+  };
+  // (small sectors take the dense route, one by one)
+  const bool batched = ctl.lockstep > 1 && N > 256 && op_.scan_batch_supported(s_);
+  const int W = std::max(1, std::min(ctl.lockstep, 16));
+  for (int i0 = 0; i0 < n;) {
+    if (i0 == n_plain && n_plain < n) {
+      std::vector<double> Lp;
+      std::vector<int> sp;
+      build_space(psi.data(), n_plain, Lp, sp);
+      provisional = ns_ > 0;
+      if (provisional) v0.resize(N);
+    }
+    // a batch never crosses the switch to provisional starts
+    const int lim = i0 < n_plain ? n_plain : n;
+    const int nb = batched ? std::min(W, lim - i0) : 1;
+    std::vector<char> done(static_cast<size_t>(nb), 0);
+    if (nb >= 2) solve_batch(i0, nb, done);
+    for (int b = 0; b < nb; ++b)
+      if (!done[size_t(b)]) solve_one(i0 + b);
+    if (provisional) rep.provisional_starts += nb;
+    i0 += nb;
   }
   rep.solve_seconds = seconds_since(t0);
   // 2. the snapshot space of all training states
@@ -580,7 +661,8 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
     return out;
   }
-  if (ctl.lockstep > 1 && op_.scan_batch_supported(s_)) return solve_lockstep(points, n, o, ctl, start);
+  if (ctl.lockstep > 1 && op_.dim() > 256 && op_.scan_batch_supported(s_))
+    return solve_lockstep(points, n, o, ctl, start);
   if (start != Start::SnapshotRR) {
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
     return out;
