@@ -156,8 +156,10 @@ typedef struct qmbed_hip_scan qmbed_hip_scan;
 qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* options_json, char** error);
 void qmbed_hip_scan_free(qmbed_hip_scan* scan);
 /* Solve the n training points (row-major n × components) with the plain
- * start and solver_json (as qmbed_hip_eigsh_json; eigenpairs is taken as 1),
- * select the snapshot ground states and train the FP32 network on the GPU.
+ * start and solver_json (as qmbed_hip_eigsh_json; eigenpairs is taken as 1;
+ * in lockstep batches of up to "lockstep" points as qmbed_hip_scan_solve,
+ * results equal to one-by-one solves), select the snapshot ground states and
+ * train the FP32 network on the GPU.
  * Answers a JSON report (energies, iterations, snapshot indices, timings,
  * first/last training loss). */
 char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json);
