@@ -236,14 +236,35 @@ struct PooledBuffers {
 // waits on the event of the flush that carried its copy; the events form a
 // ring, and an event recorded again since is later in stream order, so
 // waiting on it is still correct.
+// The rings' events are kept per device between batches (creating 64
+// events per batch cost a measurable share of a batch's host time).
+static std::mutex g_ring_mu;
+static std::vector<std::pair<int, hipEvent_t>> g_ring_events;
+
 class PostBatch {
  public:
   explicit PostBatch(hipStream_t s) : s_(s) {
-    for (auto& e : ev_) QMBED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    QMBED_HIP(hipGetDevice(&device_));
+    {
+      std::lock_guard<std::mutex> lk(g_ring_mu);
+      size_t k = 0;
+      for (size_t i = 0; i < g_ring_events.size() && k < ev_.size();) {
+        if (g_ring_events[i].first == device_) {
+          ev_[k++] = g_ring_events[i].second;
+          g_ring_events.erase(g_ring_events.begin() + long(i));
+        } else {
+          ++i;
+        }
+      }
+    }
+    for (auto& e : ev_)
+      if (!e) QMBED_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
+  // the caller has synchronized the stream (no recorded event is pending)
   ~PostBatch() {
+    std::lock_guard<std::mutex> lk(g_ring_mu);
     for (auto e : ev_)
-      if (e) (void)hipEventDestroy(e);
+      if (e) g_ring_events.push_back({device_, e});
   }
   PostBatch(const PostBatch&) = delete;
   PostBatch& operator=(const PostBatch&) = delete;
@@ -268,6 +289,7 @@ class PostBatch {
  private:
   static constexpr long kRing = 64;
   hipStream_t s_;
+  int device_ = 0;
   std::array<hipEvent_t, kRing> ev_{};
   std::vector<lk::CopyJob> jobs_;
   long gen_ = 0;
