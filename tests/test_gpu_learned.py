@@ -232,6 +232,35 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
         assert a["iterations"] == b["iterations"] and a["fp32_iterations"] == b["fp32_iterations"]
 
 
+@pytest.mark.parametrize("provisional", [True, False])
+def test_lockstep_training_matches_one_by_one(qa, provisional):
+    # 20 training points in lockstep batches (with provisional starts: the
+    # first 6 from the plain start in one batch, then 14 from the provisional
+    # snapshot space; without: batches of 16 and 4) against the same training
+    # one by one: the same energies, iterations, snapshot space and network,
+    # bit for bit, so the scans of both solve alike
+    m = _ising_scan_model(qa)
+    hx = np.linspace(0.2, 2.0, 5)
+    hz = np.linspace(0.0, 0.5, 4)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    opts = dict(snapshots=6, epochs=50, seed=0, space="principal", provisional=provisional)
+    a, b = qa.Scan(m, **opts), qa.Scan(m, **opts)
+    ra = a.train(grid, tolerance=1e-10)
+    rb = b.train(grid, tolerance=1e-10, lockstep=0)
+    print("[lockstep training]", provisional, ra["iterations"], flush=True)
+    assert ra["iterations"] == rb["iterations"]
+    np.testing.assert_array_equal(ra["energies"], rb["energies"])
+    assert ra["provisional_starts"] == rb["provisional_starts"] == (14 if provisional else 0)
+    assert ra["snapshot_indices"] == rb["snapshot_indices"]
+    assert ra["loss_first"] == rb["loss_first"] and ra["loss_last"] == rb["loss_last"]
+    held = grid[:5] + np.array([0.05, 0.02])
+    np.testing.assert_array_equal(a.predict(held), b.predict(held))
+    sa = a.solve(held, warm="rayleigh_ritz", matvec_precision="fp32")
+    sb = b.solve(held, warm="rayleigh_ritz", matvec_precision="fp32")
+    np.testing.assert_array_equal(sa["energies"], sb["energies"])
+    assert sa["iterations"] == sb["iterations"]
+
+
 def test_continuation_targets(qa):
     m = _ising_scan_model(qa)
     hx = np.linspace(0.2, 2.0, 6)
