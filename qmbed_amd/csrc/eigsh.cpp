@@ -576,7 +576,15 @@ class LanczosWindow {
   // tests, the results copied into pinned memory (the coefficients go up
   // through the same area); ritz_collect waits for them. Between the two the
   // window must not step (its w() holds the products of the test).
-  void ritz_post(int k, SpectrumTarget target, double shift) {
+  void ritz_post(int k, SpectrumTarget target, double shift) { ritz_test_own(ritz_prepare(k, target, shift)); }
+  // the tests of the nv prepared Ritz vectors with the window's own products
+  void ritz_test_own(int nv) {
+    rayleigh_launch(buf_->ritz.data(), nv);
+    ritz_copy_results();
+  }
+  // the Ritz vectors of a test (ritz_post without their Rayleigh quotients);
+  // returns their count
+  int ritz_prepare(int k, SpectrumTarget target, double shift) {
     DeviceArray<double>& ritz_buf = buf_->ritz;
     const int sz = size_;
     std::vector<double> a(alpha_.begin(), alpha_.begin() + sz);
@@ -593,11 +601,29 @@ class LanczosWindow {
     QMBED_HIP(hipMemcpyAsync(buf_->coef.data(), C, nc * 8, hipMemcpyHostToDevice, s_));
     grow(ritz_buf, e * n_ * nv);
     lk::gemm_vc(c_, V(), n_, sz, buf_->coef.data(), int(nv), ritz_buf.data(), s_);
-    rayleigh_launch(ritz_buf.data(), int(nv));
-    QMBED_HIP(hipMemcpyAsync(C + nc, scratch_, 3 * nv * sizeof(double), hipMemcpyDeviceToHost, s_));
-    QMBED_HIP(hipEventRecord(buf_->ev2, s_));
     ritz_nc_ = nc;
     ritz_nv_ = nv;
+    return int(nv);
+  }
+  // the test's results into the pinned area, for ritz_collect
+  void ritz_copy_results() {
+    QMBED_HIP(hipMemcpyAsync(buf_->pin2 + ritz_nc_, scratch_, 3 * ritz_nv_ * sizeof(double), hipMemcpyDeviceToHost,
+                             s_));
+    QMBED_HIP(hipEventRecord(buf_->ev2, s_));
+  }
+  // A test of one Ritz vector whose product the caller launches (lockstep
+  // windows: the tests of a tick share one H·v launch): verify_begin
+  // normalizes the vector (returned), the caller writes H·y into w(), and
+  // verify_end forms the Rayleigh quotient and residual and posts them.
+  double* verify_begin() {
+    double* y = buf_->ritz.data();
+    rayleigh_pre(y, scratch_);
+    return y;
+  }
+  void verify_end() {
+    ++matvecs64_;
+    rayleigh_post(buf_->ritz.data(), scratch_);
+    ritz_copy_results();
   }
   std::vector<Candidate> ritz_collect() {
     QMBED_HIP(hipEventSynchronize(buf_->ev2));
@@ -626,15 +652,21 @@ class LanczosWindow {
     for (int q = 0; q < nv; ++q) {
       double* y = Y + e * n_ * size_t(q);
       double* sl = scratch_ + 3 * q;
-      lk::nrm2sq_to_slot(c_, y, n_, sl, buf_->ws, s_);
-      lk::normalize(c_, y, sl, n_, y, nullptr, nullptr, s_);
+      rayleigh_pre(y, sl);
       if (c_) op_.apply(reinterpret_cast<const cplx*>(y), reinterpret_cast<cplx*>(w()), s_);
       else if (apply64_) apply64_(y, w());
       else op_.apply(y, w(), s_);
       ++matvecs64_;
-      lk::redot_to_slot(c_, y, w(), n_, sl + 1, buf_->ws, s_);
-      lk::residual_to_slot(c_, w(), y, sl + 1, n_, sl + 2, buf_->ws, s_);
+      rayleigh_post(y, sl);
     }
+  }
+  void rayleigh_pre(double* y, double* sl) {
+    lk::nrm2sq_to_slot(c_, y, n_, sl, buf_->ws, s_);
+    lk::normalize(c_, y, sl, n_, y, nullptr, nullptr, s_);
+  }
+  void rayleigh_post(const double* y, double* sl) {
+    lk::redot_to_slot(c_, y, w(), n_, sl + 1, buf_->ws, s_);
+    lk::residual_to_slot(c_, w(), y, sl + 1, n_, sl + 2, buf_->ws, s_);
   }
 
   long matvecs64() const { return matvecs64_; }
@@ -705,6 +737,55 @@ struct WindowResult {
 // start_norm2: a pinned host copy of the start's ‖·‖², queued before the
 // window on the same stream and checked at its first read-back (nullptr:
 // already checked).
+// The convergence tests of the lockstep windows testing in one tick: their
+// Ritz vectors' products in shared FP64 launches (as the steps' products,
+// each lane computing what its own launch would), then each window's
+// Rayleigh quotient and residual.
+class VerifyBatch {
+ public:
+  VerifyBatch(const Operator& op, const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64, hipStream_t s)
+      : op_(op), pts_(pts), lanes64_(lanes64), s_(s) {}
+  void add(int lane, LanczosWindow* w, const double* y) { items_.push_back({lane, w, y}); }
+  void flush() {
+    const size_t n = op_.dim();
+    for (size_t g0 = 0; g0 < items_.size(); g0 += 8) {
+      const int kg = int(std::min<size_t>(8, items_.size() - g0));
+      const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : 8;
+      const double* src[16] = {};
+      const double* amps[16] = {};
+      const double* diags[16] = {};
+      double* ys[16] = {};
+      for (int q = 0; q < kg; ++q) {
+        const Item& it = items_[g0 + size_t(q)];
+        src[q] = it.y;
+        amps[q] = pts_[size_t(it.lane)].amps;
+        diags[q] = pts_[size_t(it.lane)].diag;
+        ys[q] = it.w->w();
+      }
+      const double* x = src[0];  // one lane: the vector itself
+      if (width > 1) {
+        lk::interleave(src, n, width, lanes64_.data(), nullptr, s_);
+        x = lanes64_.data();
+      }
+      op_.apply_scan_batch(x, false, width, amps, diags, ys, nullptr, s_);
+    }
+    for (auto& it : items_) it.w->verify_end();
+    items_.clear();
+  }
+
+ private:
+  struct Item {
+    int lane;
+    LanczosWindow* w;
+    const double* y;
+  };
+  const Operator& op_;
+  const std::vector<LockstepPoint>& pts_;
+  DeviceArray<double>& lanes64_;
+  hipStream_t s_;
+  std::vector<Item> items_;
+};
+
 class WindowRun {
  public:
   WindowRun(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
@@ -753,6 +834,11 @@ class WindowRun {
   // the accepted Ritz vector (one wanted pair) is copied to d (device, e·N),
   // in stream order, instead of into the result
   void set_vector_destination(double* d) { vec_dst_ = d; }
+  // tests of one real Ritz vector share the batch's product launches
+  void set_verify_batch(VerifyBatch* vb, int lane) {
+    vb_ = vb;
+    vb_lane_ = lane;
+  }
   LanczosWindow& window() { return win_; }
 
   // stepped: the caller has launched this tick's step (lockstep phases)
@@ -816,7 +902,9 @@ class WindowRun {
       throw non_convergence(sz, INFINITY);
     }
     if (async_tests_) {
-      win_.ritz_post(k_, o_.target, o_.shift);
+      const int nv = win_.ritz_prepare(k_, o_.target, o_.shift);
+      if (vb_ && nv == 1 && !c_) vb_->add(vb_lane_, &win_, win_.verify_begin());
+      else win_.ritz_test_own(nv);
       verifying_ = true;
       verify_at_end_ = at_end;
       return;
@@ -893,6 +981,8 @@ class WindowRun {
   bool pending_ = false, pending_conv_ = false, done_ = false;
   bool async_tests_ = false, verifying_ = false, verify_at_end_ = false;
   double* vec_dst_ = nullptr;
+  VerifyBatch* vb_ = nullptr;
+  int vb_lane_ = 0;
 };
 
 void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const double* d_start, bool fp32,
@@ -1012,10 +1102,12 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
   }
   // the windows' read-backs of a tick go in one copy launch
   PostBatch pb(s);
+  VerifyBatch vb(op, pts, lanes64, s);
   for (int b = 0; b < nb; ++b) {
     auto& r = runs[size_t(b)];
     r->window().set_post_batch(&pb);
     r->set_async_tests(true);
+    r->set_verify_batch(&vb, b);
     if (pts[size_t(b)].vector) r->set_vector_destination(pts[size_t(b)].vector);
   }
   std::vector<char> failed(static_cast<size_t>(nb), 0);
@@ -1098,6 +1190,7 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
         failed[size_t(b)] = 1;  // the caller solves this point on its own
       }
     }
+    vb.flush();
     pb.flush();
   }
   QMBED_HIP(hipStreamSynchronize(s));
