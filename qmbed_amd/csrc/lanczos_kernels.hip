@@ -1075,7 +1075,8 @@ void interleave(const double* const* src, size_t n, int width, double* x64, floa
   if (!x64 == !x32) throw internal_state("interleave writes exactly one of the FP64 and FP32 lanes");
   Lanes16 l{};
   for (int b = 0; b < width; ++b) l.src[b] = src[b];
-  const dim3 g(nblocks(n)), t(kT);
+  // one row per thread (the column loads of more waves in flight)
+  const dim3 g(unsigned(std::max<size_t>(1, (n + kT - 1) / kT))), t(kT);
 #define QMBED_IL(T, W, X) hipLaunchKernelGGL((k_interleave<T, W>), g, t, 0, s, l, n, X)
   if (x64) {
     if (width == 1) QMBED_IL(double, 1, x64);
