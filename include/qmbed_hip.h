@@ -177,8 +177,9 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * phase of the step in one launch for all of them. Each window runs the
  * single-window algorithm, so results equal one-by-one solves bit for bit;
  * a point whose window fails is solved on its own. Needs a real
- * family with Hermitian parts, no momentum phases and the compact compiled
- * stream; otherwise the points are solved one by one. */
+ * family with Hermitian parts, no momentum phases, the compact compiled
+ * stream and more than 256 rows; otherwise the points are solved one by
+ * one. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);
 /* Network outputs (n × snapshots, row-major) of n points; returns the
  * snapshot count, or -1 on failure. */
