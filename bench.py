@@ -59,6 +59,9 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB device copy (summary and commands in profiles/hv_stream_ising_L26.md)
 HV_MIXED_TRAFFIC_BYTES = 275021548
 HV_FP32_TRAFFIC_BYTES = 267536152
+# HBM bytes per launch of the 16-lane FP32 batched scan H·v (the roofline
+# kernel), from the same counter runs; None until measured
+SCAN_W16_TRAFFIC_BYTES = None
 L_SCAN = 26
 BATCH = 16
 # the headline start: the lowest eigenvector of the family projected on the
@@ -66,6 +69,11 @@ BATCH = 16
 # True) is measured beside it. The space: the SNAPSHOTS leading principal
 # components of the training ground states.
 WARM = "rayleigh_ritz"
+OTHER = True
+TARGETS = "projections"
+EPOCHS = 4000
+HEADLINE_TEXT = ("start v0 = S y, y the lowest eigenvector of the family projected on a 32-dimensional snapshot "
+                 "space S (the leading principal components of 256 plain-Lanczos training ground states)")
 SPACE = "principal"
 N_TRAIN = 256
 SNAPSHOTS = 32
@@ -125,22 +133,32 @@ def hv_roofline(model, dtype, vec_bytes, iters=50):
 
 def cpu_baseline(n_target):
     """The CPU oracle (single-threaded restatement of the reference algorithms)
-    on a bounded sample of the scan: one ground-state eigsh (256-vector DGKS
-    window, as the reference runs it) of the same Hamiltonian at L = 22
-    (N = 96,909) at hx = 1, hz = 0.25, sector build excluded (a scan builds
-    it once); solves/s scaled to the L = 26 sector linearly in N (both the
-    COO matvec and the reorthogonalization are linear in N)."""
+    on a bounded sample of the scan: one ground-state eigsh each (the
+    reference's non-restarted 256-vector DGKS window) of the same Hamiltonian
+    at hx = 1, hz = 0.25 in the k = 0, p = +1 sectors of L = 20 and L = 22,
+    sector build and assembly timed apart. Both the stored matvec and the
+    reorthogonalization are linear in N at a fixed window, so the seconds per
+    solve per row measured at the two sizes are compared, and the larger
+    sample's rate is scaled to the L = 26 sector by N."""
     try:
         from oracle import oracle_ffi
         oracle_ffi.build()
-        m = oracle_ffi.Model(22, ising_terms(22, 1.0, 0.25), momentum=0, parity=1, norm="pauli_cartesian")
-        t0 = time.perf_counter()
-        r = m.eigsh(1)
-        dt = time.perf_counter() - t0
-        return {"value": (1.0 / dt) * (m.dim / n_target), "unit": "solves/s", "cores": 1, "kind": "port",
-                "sample": f"oracle eigsh(1) of the same Ising chain at L=22, k=0, p=+1 (N={m.dim}), hx=1, hz=0.25: "
-                          f"{dt:.1f} s (E0={r['eigenvalues'][0]:.10f}, window {r.get('iterations')}), build "
-                          f"excluded; scaled to N={n_target} linearly in N"}
+        rows = []
+        for L in (20, 22):
+            t0 = time.perf_counter()
+            m = oracle_ffi.Model(L, ising_terms(L, 1.0, 0.25), momentum=0, parity=1, norm="pauli_cartesian")
+            build = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            r = m.eigsh(1)
+            dt = time.perf_counter() - t0
+            rows.append((L, m.dim, build, dt, r.get("iterations"), r["eigenvalues"][0]))
+        (l1, n1, b1, t1, i1, _), (l2, n2, b2, t2, i2, e2) = rows
+        per_row = (t1 / n1, t2 / n2)
+        return {"value": (1.0 / t2) * (n2 / n_target), "unit": "solves/s", "cores": 1, "kind": "port",
+                "sample": f"oracle eigsh(1), window {i2}, Ising L={l2} k=0 p=+1 (N={n2}) at hx=1 hz=0.25: "
+                          f"{t2:.2f} s (build {b2:.1f} s apart, E0={e2:.10f}); L={l1} (N={n1}): {t1:.2f} s; "
+                          f"s/solve/row {per_row[0]:.3e} vs {per_row[1]:.3e} (ratio {per_row[1] / per_row[0]:.2f}); "
+                          f"rate scaled to N={n_target} linearly in N"}
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "solves/s", "cores": 1, "kind": "port", "sample": f"unavailable: {e}"}
 
@@ -148,8 +166,10 @@ def cpu_baseline(n_target):
 def hv_profile(qa):
     """Kernels for counter runs: the FP32 and FP64 H·v of the headline sector
     at hx = 1, hz = 0.25 (20 timed launches each after the autotuner's own),
-    and 5 device-to-device copies of 256 MiB, whose counter readings
-    calibrate the bytes the H·v counters report."""
+    the 16-lane and 8-lane FP32 batched scan H·v at held-out grid points (2
+    untimed and 20 timed launches each), and 5 device-to-device copies of
+    256 MiB, whose counter readings calibrate the bytes the H·v counters
+    report."""
     static, comps = ising_family(L_SCAN)
     model = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
                      components=comps)
@@ -157,6 +177,12 @@ def hv_profile(qa):
     out = {"hv_fp32": hv_roofline(model, np.float32, 4, iters=20),
            "hv_fp64": hv_roofline(model, np.float64, 8, iters=20),
            "hv_mixed": hv_roofline(model, "mixed", 4, iters=20)}
+    grid, perm = scan_grid()
+    pts = grid[perm[N_TRAIN:N_TRAIN + 16]]
+    nnz = model.nnz()
+    for w in (16, 8):
+        ms = model.time_scan_batch(pts[:w], fp32=True, iters=20)
+        out[f"scan_fp32_w{w}"] = {"ms": ms, "bytes_per_launch": scan_bytes_per_launch(model.dim, nnz, w, True)}
     L = qa.library()
     nbytes = 256 << 20
     a, b = qa.DeviceBuffer(nbytes), qa.DeviceBuffer(nbytes)
@@ -252,6 +278,34 @@ def mean(xs):
     return float(np.mean(xs)) if len(xs) else None
 
 
+def scan_bytes_per_launch(dim, nnz, width, fp32=True):
+    """Algorithmic bytes of one batched scan H·v launch over `width` lanes:
+    per row the representative's 4-byte record and `width` FP64 outputs, per
+    stored entry one 4-byte stream word and the `width` inputs of its target
+    row (one shared gather), N·(4 + 8W) + nnz·(4 + sW)."""
+    sx = 4 if fp32 else 8
+    return dim * (4 + 8 * width) + nnz * (4 + sx * width)
+
+
+def mode_summary(rs, ref_energies=None, seconds=None, points=None):
+    """Compact per-mode figures of a list of scan_solve results."""
+    def cat(key):
+        return [x for r in rs for x in r[key]]
+    it = cat("iterations")
+    out = {"mean_iterations": mean(it), "max_iterations": int(max(it)) if it else None,
+           "max_residual": float(np.max(cat("residuals")))}
+    if "fp32_iterations" in rs[0]:
+        out["mean_fp32_iterations"] = mean(cat("fp32_iterations"))
+    if "fallbacks" in rs[0]:
+        out["fallbacks"] = int(sum(r["fallbacks"] for r in rs))
+    if seconds is not None and points:
+        out["solves_per_s"] = points / seconds
+    if ref_energies is not None:
+        e = np.array(cat("energies"))
+        out["max_energy_rel_diff_vs_plain_fp64"] = float(np.max(np.abs(e - ref_energies) / np.abs(ref_energies)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,16 +313,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extended", action="store_true")
+    ap.add_argument("--details", default=None,
+                    help="write the per-point iterations, times and energy differences of every mode to this JSON file")
     ap.add_argument("--hv-profile", action="store_true",
-                    help="only the headline sector's FP32 and FP64 H·v (20 launches each) and 5 device copies of "
-                         "256 MiB, for hardware-counter runs")
+                    help="only the headline sector's FP32 and FP64 H·v (20 launches each), 20 launches of the "
+                         "16-lane and 8-lane batched scan H·v and 5 device copies of 256 MiB, for hardware-counter "
+                         "runs")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the 72-sector XXZ L=24 ground-state batch and the 64-state evolution (one timed run "
                          "each after a warm-up), for kernel traces")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: there is no multi-GPU path (sectors, scan points and right-hand sides "
-              "are independent units that shard without communication). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: every GPU box here has one, and there is no multi-GPU path (sectors, scan "
+              "points and right-hand sides are independent units that shard without communication). Run with "
+              "--gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import qmbed_amd as qa
@@ -299,7 +357,8 @@ def main():
     train = grid[perm[:N_TRAIN]]
     n_bench = BATCH * (args.warmup + args.steps)
     bench_pts = grid[perm[N_TRAIN:N_TRAIN + n_bench]]
-    scan = qa.Scan(model, snapshots=SNAPSHOTS, epochs=4000, learning_rate=2e-3, seed=0, space=SPACE)
+    scan = qa.Scan(model, snapshots=SNAPSHOTS, epochs=EPOCHS, learning_rate=2e-3, seed=0, space=SPACE,
+                   targets=TARGETS)
     t0 = time.perf_counter()
     rep = scan.train(train, tolerance=1e-10)
     train_s = time.perf_counter() - t0
@@ -309,104 +368,99 @@ def main():
     # the reference solves a scan; the lockstep batching is measured beside it
     plain_kw = {"tolerance": TOL, "lockstep": 0}
     batches = [bench_pts[i * BATCH:(i + 1) * BATCH] for i in range(args.warmup + args.steps)]
+    timed = batches[args.warmup:]
+    modes = {  # name: (start, solver keys); the first is the headline
+        "headline": (WARM, warm_kw),
+        "plain_fp64": (False, plain_kw),
+        "other_start_fp32": (OTHER, warm_kw),
+        "headline_start_fp64": (WARM, {"tolerance": TOL}),
+        "headline_one_by_one": (WARM, dict(warm_kw, lockstep=0)),
+        "plain_fp64_lockstep": (False, {"tolerance": TOL}),
+    }
     # warm-up: every mode measured below, on the warm-up batches
     for b in batches[:args.warmup]:
-        scan.solve(b, warm=WARM, **warm_kw)
-        scan.solve(b, warm=False, **plain_kw)
-        scan.solve(b, warm=WARM, tolerance=TOL)
-        scan.solve(b, warm=True, **warm_kw)
-        scan.solve(b, warm=WARM, lockstep=0, **warm_kw)
-        scan.solve(b, warm=False, tolerance=TOL)
+        for start, kw in modes.values():
+            scan.solve(b, warm=start, **kw)
 
-    # ---- timed: learned snapshot start + FP32 schedule ----
+    # ---- timed: the headline start + FP32 schedule; the launch timer
+    # brackets every batched scan H·v launch with HIP events on its stream ----
     qa.library().qmbed_hip_synchronize()
+    qa.launch_timer(True)
     t0 = time.perf_counter()
-    warm = [scan.solve(b, warm=WARM, **warm_kw) for b in batches[args.warmup:]]
+    warm = [scan.solve(b, warm=WARM, **warm_kw) for b in timed]
     qa.library().qmbed_hip_synchronize()
     dt = time.perf_counter() - t0
+    qa.launch_timer(False)
+    launches = qa.launch_timer_report()
     value = args.steps * BATCH / dt
     ms_per_step = dt / args.steps * 1e3
+    npts = args.steps * BATCH
 
-    # ---- the same points: plain FP64 start, the same start in FP64, and
-    # the network's start with the FP32 schedule ----
-    t0 = time.perf_counter()
-    plain = [scan.solve(b, warm=False, **plain_kw) for b in batches[args.warmup:]]
-    plain_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    warm64 = [scan.solve(b, warm=WARM, tolerance=TOL) for b in batches[args.warmup:]]
-    warm64_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    net = [scan.solve(b, warm=True, **warm_kw) for b in batches[args.warmup:]]
-    net_s = time.perf_counter() - t0
-    # the same start without lockstep, and the plain start with it
-    t0 = time.perf_counter()
-    warm1 = [scan.solve(b, warm=WARM, lockstep=0, **warm_kw) for b in batches[args.warmup:]]
-    warm1_s = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    plain_ls = [scan.solve(b, warm=False, tolerance=TOL) for b in batches[args.warmup:]]
-    plain_ls_s = time.perf_counter() - t0
-
-    def cat(rs, key):
-        return [x for r in rs for x in r[key]]
-
-    ew, ep = np.array(cat(warm, "energies")), np.array(cat(plain, "energies"))
-    rel = np.abs(ew - ep) / np.abs(ep)
-    plain_rate = args.steps * BATCH / plain_s
+    # ---- the same points in the other modes ----
+    results = {"headline": (warm, dt)}
+    for name, (start, kw) in modes.items():
+        if name == "headline":
+            continue
+        t0 = time.perf_counter()
+        rs = [scan.solve(b, warm=start, **kw) for b in timed]
+        results[name] = (rs, time.perf_counter() - t0)
+    ep = np.array([x for r in results["plain_fp64"][0] for x in r["energies"]])
+    summary = {name: mode_summary(rs, ep, sec, npts) for name, (rs, sec) in results.items()}
+    plain_rate = summary["plain_fp64"]["solves_per_s"]
     n_grid = len(grid)
     setup_s = build_s + train_s
     amort_warm = n_grid / (setup_s + n_grid / value)
     amort_plain = n_grid / (build_s + n_grid / plain_rate)
 
     model.set_parameters(hx=1.0, hz=0.25)
+    nnz = model.nnz()
+    hvmix = hv_roofline(model, "mixed", 4)
     hv32 = hv_roofline(model, np.float32, 4)
     hv64 = hv_roofline(model, np.float64, 8)
-    hvmix = hv_roofline(model, "mixed", 4)
+
+    # roofline: the 16-lane FP32 batched scan H·v, the headline's dominant kernel
+    key16 = [k for k in launches if k.startswith("hv_scan_batch_kernel<float, 16,")]
+    rk = key16[0] if key16 else max(launches, key=lambda k: launches[k]["ms_total"]) if launches else None
+    if rk is not None:
+        width = int(rk.split(",")[1])
+        fp32k = rk.startswith("hv_scan_batch_kernel<float")
+        algo16 = scan_bytes_per_launch(model.dim, nnz, width, fp32k)
+        ach = algo16 / (launches[rk]["ms_mean"] * 1e-3) / 1e9
+    else:
+        algo16 = ach = None
+    kernel_share = {k: {"launches": v["launches"], "ms_mean": round(v["ms_mean"], 5),
+                        "ms_total": round(v["ms_total"], 3)} for k, v in launches.items()}
+    # the same kernel on its own (20 launches at 16 fixed points of the scan grid)
+    alone = {}
+    for w in (16, 8):
+        ms = model.time_scan_batch(bench_pts[:w], fp32=True, iters=20)
+        b = scan_bytes_per_launch(model.dim, nnz, w, True)
+        alone[f"fp32_w{w}"] = {"ms": round(ms, 5), "bytes_per_launch": b,
+                               "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    def hv_short(h):
+        return {"ms": round(h["ms"], 5), "bytes_per_launch": h["bytes_per_launch"], "frac": round(h["frac"], 4),
+                "mode": h["mode"]}
 
     extras = {
         "scan": {
-            "sector_dim": model.dim, "grid_points": n_grid, "training_points": N_TRAIN, "snapshots": rep["snapshots"],
-            "snapshot_space": SPACE, "provisional_training_starts": rep["provisional_starts"],
-            "build_s": build_s, "training_s": train_s, "training_solve_s": rep["solve_seconds"],
-            "gram_s": rep["gram_seconds"], "network_train_s": rep["train_seconds"],
-            "network_loss_first": rep["loss_first"], "network_loss_last": rep["loss_last"],
-            "training_mean_iterations": mean(rep["iterations"]),
-            "timed_points": args.steps * BATCH,
-            "warm_fp32": {"solves_per_s": value, "mean_iterations": mean(cat(warm, "iterations")),
-                          "mean_fp32_iterations": mean(cat(warm, "fp32_iterations")),
-                          "fallbacks": sum(r["fallbacks"] for r in warm),
-                          "max_residual": float(np.max(cat(warm, "residuals"))),
-                          "iterations": cat(warm, "iterations"), "fp32_iterations": cat(warm, "fp32_iterations"),
-                          "fp64_iterations": cat(warm, "fp64_iterations"),
-                          "ms": [round(1e3 * x, 2) for x in cat(warm, "seconds")]},
-            "plain_fp64": {"solves_per_s": plain_rate, "mean_iterations": mean(cat(plain, "iterations")),
-                           "max_residual": float(np.max(cat(plain, "residuals"))),
-                           "iterations": cat(plain, "iterations"),
-                           "ms": [round(1e3 * x, 2) for x in cat(plain, "seconds")]},
-            "warm_fp64": {"solves_per_s": args.steps * BATCH / warm64_s,
-                          "mean_iterations": mean(cat(warm64, "iterations")), "iterations": cat(warm64, "iterations"),
-                          "ms": [round(1e3 * x, 2) for x in cat(warm64, "seconds")]},
-            "network_start_fp32": {
-                "solves_per_s": args.steps * BATCH / net_s, "mean_iterations": mean(cat(net, "iterations")),
-                "iterations": cat(net, "iterations"), "fallbacks": sum(r["fallbacks"] for r in net),
-                "speedup_vs_plain": (args.steps * BATCH / net_s) / plain_rate,
-                "iteration_ratio_plain_over_network": mean(cat(plain, "iterations")) / mean(cat(net, "iterations")),
-                "max_energy_rel_diff_vs_plain_fp64": float(np.max(np.abs(np.array(cat(net, "energies")) - ep)
-                                                                  / np.abs(ep)))},
-            "warm_fp32_one_by_one": {"solves_per_s": args.steps * BATCH / warm1_s,
-                                     "mean_iterations": mean(cat(warm1, "iterations"))},
-            "plain_fp64_lockstep": {"solves_per_s": args.steps * BATCH / plain_ls_s,
-                                    "mean_iterations": mean(cat(plain_ls, "iterations")),
-                                    "speedup_of_warm_fp32": value / (args.steps * BATCH / plain_ls_s)},
-            "speedup_vs_plain": value / plain_rate,
-            "iteration_ratio_plain_over_warm": mean(cat(plain, "iterations")) / mean(cat(warm, "iterations")),
-            "energy_rel_diff_vs_plain_fp64": {"max": float(rel.max()), "per_point": rel.tolist()},
-            "amortized_full_scan_solves_per_s": {"warm_fp32_incl_setup": amort_warm,
-                                                  "plain_fp64_incl_build": amort_plain,
-                                                  "speedup": amort_warm / amort_plain},
+            "dim": model.dim, "nnz": nnz, "grid_points": n_grid, "training_points": N_TRAIN,
+            "snapshots": rep["snapshots"], "network_targets": TARGETS, "timed_points": npts,
+            "setup_s": {"build": round(build_s, 3), "training": round(train_s, 3),
+                        "training_solves": round(rep["solve_seconds"], 3), "network": round(rep["train_seconds"], 3)},
+            "network_loss": [rep["loss_first"], rep["loss_last"]],
+            "modes": summary,
+            "headline_vs_plain_fp64": {"throughput": value / plain_rate,
+                                       "iterations": summary["plain_fp64"]["mean_iterations"]
+                                       / summary["headline"]["mean_iterations"]},
+            "amortized_full_scan": {"headline_incl_setup": amort_warm, "plain_fp64_incl_build": amort_plain,
+                                    "speedup": amort_warm / amort_plain},
         },
-        "hv_c5_fp32": dict(hv32, traffic=HV_FP32_TRAFFIC_BYTES if hv32["mode"] == "stream" else None),
-        "hv_c5_fp64": hv64,
-        "hv_c5_fp32_in_fp64_out": hvmix,
+        "scan_hv_launches": kernel_share,
+        "scan_hv_alone": alone,
+        "hv_c5_fp32_in_fp64_out": hv_short(hvmix),
+        "hv_c5_fp32": hv_short(hv32),
+        "hv_c5_fp64": hv_short(hv64),
     }
 
     if not args.no_extended:
@@ -418,27 +472,35 @@ def main():
         t0 = time.perf_counter()
         rs = [xxz.eigsh(6, tolerance=1e-10) for _ in range(5)]
         dtx = time.perf_counter() - t0
-        extras["xxz_L24_eigsh6"] = {"setup_s": xs, "solves_per_s": 5 / dtx, "iterations": rs[-1]["iterations"],
-                                    "eigenvalues": rs[-1]["eigenvalues"], "residuals": rs[-1]["residuals"]}
-        extras["hv_xxz_L24_fp64"] = hv_roofline(xxz, np.float64, 8)
+        extras["xxz_L24_eigsh6"] = {"setup_s": round(xs, 3), "solves_per_s": 5 / dtx,
+                                    "iterations": rs[-1]["iterations"], "E0": rs[-1]["eigenvalues"][0],
+                                    "max_residual": max(rs[-1]["residuals"])}
+        extras["hv_xxz_L24_fp64"] = hv_short(hv_roofline(xxz, np.float64, 8))
         xxz.close()
         # J1-J2 L=28 (N=1,432,860): FP32/FP64 H·v and ground state
         t0 = time.perf_counter()
         big = qa.Model({"kind": "spin", "sites": 28, "up": 14, "momentum": 0}, j1j2_terms(28))
-        extras["j1j2_L28_setup_s"] = time.perf_counter() - t0
-        extras["hv_j1j2_L28_fp32"] = hv_roofline(big, np.float32, 4, iters=20)
-        extras["hv_j1j2_L28_fp64"] = hv_roofline(big, np.float64, 8, iters=20)
+        j = {"setup_s": round(time.perf_counter() - t0, 3),
+             "hv_fp32": hv_short(hv_roofline(big, np.float32, 4, iters=20)),
+             "hv_fp64": hv_short(hv_roofline(big, np.float64, 8, iters=20))}
         for prec in ("fp64", "fp32"):
             t0 = time.perf_counter()
             gs = big.eigsh(1, tolerance=1e-10, matvec_precision=prec)
-            extras[f"j1j2_L28_gs_{prec}"] = {"s": time.perf_counter() - t0, "E0": gs["eigenvalues"][0],
-                                              "residual": gs["residuals"][0], "iterations": gs["iterations"],
-                                              "fp32_iterations": gs["fp32_iterations"]}
-        e64 = extras["j1j2_L28_gs_fp64"]["E0"]
-        extras["j1j2_L28_gs_fp32"]["rel_err_vs_fp64"] = abs(extras["j1j2_L28_gs_fp32"]["E0"] - e64) / abs(e64)
+            j[f"gs_{prec}"] = {"s": round(time.perf_counter() - t0, 3), "E0": gs["eigenvalues"][0],
+                               "residual": gs["residuals"][0], "iterations": gs["iterations"],
+                               "fp32_iterations": gs["fp32_iterations"]}
+        j["gs_fp32"]["rel_err_vs_fp64"] = abs(j["gs_fp32"]["E0"] - j["gs_fp64"]["E0"]) / abs(j["gs_fp64"]["E0"])
+        extras["j1j2_L28"] = j
         big.close()
         extras.update(sector_batch(qa))
         extras.update(expm_batch(qa))
+
+    if args.details:
+        with open(args.details, "w") as f:
+            json.dump({name: {"iterations": [x for r in rs for x in r["iterations"]],
+                              "energies": [x for r in rs for x in r["energies"]],
+                              "ms": [round(1e3 * x, 3) for r in rs for x in r["seconds"]]}
+                       for name, (rs, _) in results.items()}, f)
 
     scan.close()
     line = {
@@ -452,23 +514,19 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 matvec, f64 Lanczos/reorth/polish",
         "data": "synthetic (Hamiltonian couplings of the scan grid; seeded training and benchmark points)",
-        "config": {"workload": f"Ising chain -ZZ - hx X - hz Z (Pauli), L=26 periodic, k=0, p=+1 (N={model.dim}), "
-                               "64x64 grid hx in [0.2,2], hz in [0,0.5]; warm start from a 32-dimensional snapshot "
-                               "space (the 32 leading principal components of the ground states of 256 plain-Lanczos "
-                               "training points): v0 = lowest eigenvector of the family projected on that space (the 2-64-64-32 FP32 MLP start is reported in "
-                               "extras.scan.network_start_fp32) + FP32-matvec schedule; ground state to residual "
-                               "<= 1e-8*max(|E|,1); 16 points per step, solved in one lockstep batch (shared H.v launches per "
-                               "Lanczos step over the interleaved vectors of the points still running, up to 16 FP32 "
-                               "or 8 FP64 vectors per launch; the other phases one launch for all points); baseline "
-                               "plain FP64 one eigsh per point",
+        "config": {"workload": f"Ising chain -ZZ - hx X - hz Z (Pauli), L=26 periodic, k=0, p=+1 (N={model.dim}, "
+                               f"nnz={nnz}), 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT}; FP32-matvec "
+                               "schedule; ground state to residual <= 1e-8*max(|E|,1); one step = 16 held-out "
+                               "points solved in one lockstep batch; baseline plain FP64, one eigsh per point",
                    "global_batch": BATCH},
-        "roofline": {"bound": "hbm", "achieved": hvmix["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hvmix["frac"], "traffic": HV_MIXED_TRAFFIC_BYTES if hvmix["mode"] == "stream" else None,
-                     "kernel": f"H·v of the FP32 schedule (FP32 input gathered, FP64 accumulation and output), "
-                               f"L=26 Ising sector ({hvmix['mode']} mode); algorithmic bytes N·(4+8) + nnz·(4+4) "
-                               f"= {hvmix['bytes_per_launch']} per launch; pure FP32 H·v in extras.hv_c5_fp32"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS if ach else None,
+                     "traffic": SCAN_W16_TRAFFIC_BYTES if rk and rk.startswith("hv_scan_batch_kernel<float, 16,")
+                     else None,
+                     "kernel": f"{rk}: mean of {launches[rk]['launches'] if rk else 0} launches in the timed region "
+                               f"(HIP events on its stream); algorithmic bytes N(4+8W)+nnz(4+4W) = {algo16}"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(model.dim),
         "extras": extras,
     }

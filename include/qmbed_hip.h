@@ -32,19 +32,61 @@ char* qmbed_run_json(const char* request);
  * release_expm_action. Other operations answer an error. */
 char* qmbed_command_json(const char* request);
 
+/* ABI-stable complex scalar of the coefficient callbacks
+ * (QmbedComplex64, bindings/capi/src/lib.rs:452-458). */
+typedef struct QmbedComplex64 {
+  double real;
+  double imaginary;
+} QmbedComplex64;
+
+/* QmbedDriveCallback (bindings/capi/src/lib.rs:460-469): fill `count`
+ * ordered component coefficients for one physical time; nonzero aborts. */
+typedef int32_t (*QmbedDriveCallback)(void* context, double time, QmbedComplex64* coefficients, size_t count);
+
 /* Replaces qmbed_evolve_model_with_drive_json
- * (bindings/capi/src/lib.rs:6153-6192); exported so existing loaders resolve
- * it. Driven evolution is outside this engine: always answers an error. */
-typedef int (*qmbed_drive_callback)(void* context, double time, double* values, size_t count);
-char* qmbed_evolve_model_with_drive_json(const char* request, qmbed_drive_callback callback, void* context);
+ * (bindings/capi/src/lib.rs:6153-6179); exported so existing loaders resolve
+ * it. Driven evolution is outside this engine: a null request answers
+ * "request pointer is null", a null callback "drive callback is null",
+ * anything else an unsupported-backend error. */
+char* qmbed_evolve_model_with_drive_json(const char* request, QmbedDriveCallback callback, void* context);
+
+/* QuSpin-compatible user-basis callback layouts
+ * (bindings/capi/src/lib.rs:471-521). */
+typedef struct QmbedUserOpResult32 {
+  QmbedComplex64 matrix_element;
+  uint32_t state;
+} QmbedUserOpResult32;
+typedef struct QmbedUserOpResult64 {
+  QmbedComplex64 matrix_element;
+  uint64_t state;
+} QmbedUserOpResult64;
+typedef int32_t (*QmbedUserOp32Callback)(QmbedUserOpResult32* result, int8_t op, int32_t site, int32_t sites,
+                                         const uint32_t* arguments);
+typedef uint32_t (*QmbedUserNextState32Callback)(uint32_t state, uint32_t counter, uint32_t sites,
+                                                 const uint32_t* arguments);
+typedef uint32_t (*QmbedUserPreCheck32Callback)(uint32_t state, uint32_t sites, const uint32_t* arguments);
+typedef uint32_t (*QmbedUserMap32Callback)(uint32_t state, int32_t sites, int8_t* sign, const uint32_t* arguments);
+typedef int32_t (*QmbedUserOp64Callback)(QmbedUserOpResult64* result, int8_t op, int32_t site, int32_t sites,
+                                         const uint64_t* arguments);
+typedef uint64_t (*QmbedUserNextState64Callback)(uint64_t state, uint64_t counter, uint64_t sites,
+                                                 const uint64_t* arguments);
+typedef uint64_t (*QmbedUserPreCheck64Callback)(uint64_t state, uint64_t sites, const uint64_t* arguments);
+typedef uint64_t (*QmbedUserMap64Callback)(uint64_t state, int32_t sites, int8_t* sign, const uint64_t* arguments);
 
 /* Replace qmbed_register_user_basis_{32,64}_json
- * (bindings/capi/src/lib.rs:6194-6288); exported for loader compatibility,
- * always answer an error (user bases are outside this engine). */
-typedef int (*qmbed_user_basis_callback_32)(void* context, const uint32_t* input, uint32_t* output, size_t count);
-typedef int (*qmbed_user_basis_callback_64)(void* context, const uint64_t* input, uint64_t* output, size_t count);
-char* qmbed_register_user_basis_32_json(const char* request, qmbed_user_basis_callback_32 callback, void* context);
-char* qmbed_register_user_basis_64_json(const char* request, qmbed_user_basis_callback_64 callback, void* context);
+ * (bindings/capi/src/lib.rs:6181-6288), with the reference's six
+ * parameters; exported for loader compatibility. User bases are outside
+ * this engine: after the reference's own argument checks (null request,
+ * null operator callback, null map array with map_count > 0) they answer an
+ * unsupported-backend error. */
+char* qmbed_register_user_basis_32_json(const char* request, QmbedUserOp32Callback op,
+                                        QmbedUserNextState32Callback next_state,
+                                        QmbedUserPreCheck32Callback pre_check,
+                                        const QmbedUserMap32Callback* map_callbacks, size_t map_count);
+char* qmbed_register_user_basis_64_json(const char* request, QmbedUserOp64Callback op,
+                                        QmbedUserNextState64Callback next_state,
+                                        QmbedUserPreCheck64Callback pre_check,
+                                        const QmbedUserMap64Callback* map_callbacks, size_t map_count);
 
 /* Replaces qmbed_string_free (bindings/capi/src/lib.rs:6290-6297). NULL is
  * allowed. */
@@ -89,6 +131,23 @@ int qmbed_hip_apply_host(const qmbed_hip_model* model, int dtype, const void* x,
 /* average milliseconds of `iters` back-to-back H·v launches (HIP events on
  * the model stream) */
 double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void* x, void* y, int iters);
+/* Average milliseconds of `iters` launches of the batched scan H·v (the
+ * kernel of lockstep scan solves) over `width` lanes (1, 2, 4, 8 or 16)
+ * of FP32 (fp32 != 0) or FP64 interleaved inputs, each lane its own point
+ * of the family (points: width rows of one real coefficient per
+ * component), timed with HIP events on the model stream after two
+ * untimed launches. -2 when the model does not support batched scan
+ * launches of that width, -1 on failure. No counterpart in the reference. */
+double qmbed_hip_time_scan_batch(qmbed_hip_model* model, int width, int fp32, const double* points, int iters);
+/* Launch timer (no counterpart in the reference): on != 0 clears the
+ * records and brackets every later batched scan H·v launch with two HIP
+ * events on its stream; 0 stops recording. The report answers
+ * {"status":"ok","result":{"<kernel instance>":{"launches","ms_total",
+ * "ms_mean"},...}} over the launches recorded since the last switch-on
+ * (it waits for them). */
+int qmbed_hip_launch_timer(int on);
+char* qmbed_hip_launch_timer_report(void);
+
 /* eigsh with a SolverRequest JSON; response as qmbed_run_json's result plus
  * fp32_iterations / fp64_iterations / reorthogonalization counters. Accepts
  * the extra solver keys "matvec_precision" ("fp64"|"fp32") and

@@ -94,6 +94,12 @@ def library():
     L.qmbed_hip_gemm_atb.restype = c_int
     L.qmbed_hip_gemm_atb.argtypes = [c_void_p, ctypes.c_size_t, c_int, c_void_p, c_int, c_void_p]
     L.qmbed_hip_device_count.restype = c_int
+    L.qmbed_hip_time_scan_batch.restype = ctypes.c_double
+    L.qmbed_hip_time_scan_batch.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
+    L.qmbed_hip_launch_timer.restype = c_int
+    L.qmbed_hip_launch_timer.argtypes = [c_int]
+    L.qmbed_hip_launch_timer_report.restype = c_void_p
+    L.qmbed_hip_launch_timer_report.argtypes = []
     _lib = L
     return L
 
@@ -268,6 +274,18 @@ class Model:
             raise QmbedError("timed apply failed")
         return ms
 
+    def time_scan_batch(self, points, fp32=True, iters=20):
+        """Average ms per launch of the batched scan H·v over len(points)
+        lanes (1, 2, 4, 8 or 16), lane b evaluated at points[b] (one real
+        coefficient per component), FP32 or FP64 interleaved inputs."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        ms = library().qmbed_hip_time_scan_batch(self._h, pts.shape[0], 1 if fp32 else 0, pts.ctypes.data, iters)
+        if ms == -2:
+            raise QmbedError("batched scan launches of this width are not supported by the model")
+        if ms < 0:
+            raise QmbedError("timed batched scan launch failed")
+        return ms
+
     def apply_mixed(self, x):
         """FP32 input, FP64 accumulation and output (compiled-stream mode):
         float32 → float64, or complex64 → complex128."""
@@ -354,6 +372,18 @@ class Model:
         if parameters is not None:
             req["parameters"] = {n: [complex(v).real, complex(v).imag] for n, v in parameters.items()}
         return _call_model(self._h, req)
+
+
+def launch_timer(on):
+    """Switch the library's launch timer on (clearing its records) or off."""
+    if library().qmbed_hip_launch_timer(1 if on else 0) != 0:
+        raise QmbedError("launch timer switch failed")
+
+
+def launch_timer_report():
+    """{kernel instance: {"launches", "ms_total", "ms_mean"}} of the launches
+    recorded since the timer was switched on."""
+    return _envelope(library().qmbed_hip_launch_timer_report())
 
 
 def eigsh_batch(models, k, target="smallest_algebraic", **solver):

@@ -8,9 +8,13 @@
 // the caller until qmbed_string_free.
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
+#include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <optional>
 #include <set>
 #include <sstream>
 
@@ -48,15 +52,28 @@ const Value& need(const Value& obj, const char* key, const std::string& ctx) {
   return *v;
 }
 
+// integers within ±2^53 (exact in a double, and far inside long's range)
 long as_long(const Value& v, const std::string& what) {
-  if (v.kind != Value::Number || std::floor(v.num) != v.num) bad(what + ": expected an integer");
+  if (v.kind != Value::Number || !std::isfinite(v.num) || std::floor(v.num) != v.num)
+    bad(what + ": expected an integer");
+  if (std::fabs(v.num) > 9007199254740992.0) bad(what + ": integer out of range");
   return long(v.num);
+}
+
+int as_int(const Value& v, const std::string& what) {
+  const long x = as_long(v, what);
+  if (x < long(INT32_MIN) || x > long(INT32_MAX)) bad(what + ": integer out of range");
+  return int(x);
 }
 
 uint64_t as_u64(const Value& v, const std::string& what) {
   if (v.kind != Value::Number) bad(what + ": expected an integer");
-  if (v.raw.find_first_of(".eE-") != std::string::npos) bad(what + ": expected a non-negative integer");
-  return std::strtoull(v.raw.c_str(), nullptr, 10);
+  if (v.raw.empty() || v.raw.find_first_not_of("0123456789") != std::string::npos)
+    bad(what + ": expected a non-negative integer");
+  errno = 0;
+  const unsigned long long u = std::strtoull(v.raw.c_str(), nullptr, 10);
+  if (errno == ERANGE) bad(what + ": integer out of range");
+  return uint64_t(u);
 }
 
 double as_double(const Value& v, const std::string& what) {
@@ -91,22 +108,22 @@ SpinBasisSpec parse_basis(const Value& b) {
   if (kind->raw != "spin")
     throw unsupported_backend("the MI355X engine implements spin bases; basis kind `" + kind->raw + "` is not built");
   SpinBasisSpec s;
-  s.sites = int(as_long(need(b, "sites", "basis"), "sites"));
+  s.sites = as_int(need(b, "sites", "basis"), "sites");
   if (s.sites < 0) bad("sites must be non-negative");
-  if (auto* v = b.get("spin_twice"); v && !v->is_null()) s.spin_twice = int(as_long(*v, "spin_twice"));
+  if (auto* v = b.get("spin_twice"); v && !v->is_null()) s.spin_twice = as_int(*v, "spin_twice");
   const Value* up = b.get("up");
   const Value* ups = b.get("up_sectors");
   if (up && !up->is_null() && ups && !ups->is_null()) bad("spin basis accepts either up or up_sectors, not both");
-  if (up && !up->is_null()) s.up = int(as_long(*up, "up"));
+  if (up && !up->is_null()) s.up = as_int(*up, "up");
   if (ups && !ups->is_null()) {
-    for (auto& e : ups->array()) s.up_sectors.push_back(int(as_long(e, "up_sectors")));
+    for (auto& e : ups->array()) s.up_sectors.push_back(as_int(e, "up_sectors"));
     // canonical_particle_sectors: sorted unique, within capacity
     std::sort(s.up_sectors.begin(), s.up_sectors.end());
     s.up_sectors.erase(std::unique(s.up_sectors.begin(), s.up_sectors.end()), s.up_sectors.end());
     if (s.up_sectors.empty()) throw invalid_sector("spin particle sectors must not be empty");
   }
-  if (auto* v = b.get("momentum"); v && !v->is_null()) s.momentum = int(as_long(*v, "momentum"));
-  if (auto* v = b.get("parity"); v && !v->is_null()) s.parity = int(as_long(*v, "parity"));
+  if (auto* v = b.get("momentum"); v && !v->is_null()) s.momentum = as_int(*v, "momentum");
+  if (auto* v = b.get("parity"); v && !v->is_null()) s.parity = as_int(*v, "parity");
   const Value* norm = b.get("normalization");
   if (norm && !norm->is_null()) {
     if (norm->kind != Value::String) bad("normalization must be a string");
@@ -152,7 +169,7 @@ std::vector<OperatorSpec> parse_terms(const Value& t) {
     for (auto& cpl : need(term, "couplings", "term").array()) {
       Coupling c;
       c.coefficient = as_complex(need(cpl, "coefficient", "coupling"), "coefficient");
-      for (auto& s : need(cpl, "sites", "coupling").array()) c.sites.push_back(int(as_long(s, "sites")));
+      for (auto& s : need(cpl, "sites", "coupling").array()) c.sites.push_back(as_int(s, "sites"));
       spec.couplings.push_back(c);
     }
     out.push_back(spec);
@@ -193,7 +210,7 @@ struct SolverReq {
 // SolverRequest (capi :2064-2150): defaults tol 1e-10, iterations 1000, seed 0
 SolverReq parse_solver(const Value& s) {
   SolverReq r;
-  r.opts.eigenpairs = int(as_long(need(s, "eigenpairs", "solver"), "eigenpairs"));
+  r.opts.eigenpairs = as_int(need(s, "eigenpairs", "solver"), "eigenpairs");
   const Value& tg = need(s, "target", "solver");
   const Value* kind = tg.get("kind");
   if (!kind || kind->kind != Value::String) bad("invalid binding request: missing field `kind`");
@@ -659,12 +676,60 @@ std::string run_command(const std::string& req) {
   throw unsupported_backend("operation `" + op + "` is outside the MI355X spin eigsh/evolve/expm path");
 }
 
+// The first invalid UTF-8 sequence of a NUL-terminated request, described
+// as the reference's binding reports it (the display of Rust's Utf8Error):
+// "invalid utf-8 sequence of {n} bytes from index {i}" for a bad sequence,
+// "incomplete utf-8 byte sequence from index {i}" for one cut short by the
+// end. Well-formed UTF-8 (RFC 3629: no overlong forms, no surrogates, at
+// most U+10FFFF) gives nothing.
+std::optional<std::string> utf8_error(const char* text) {
+  const auto* v = reinterpret_cast<const unsigned char*>(text);
+  const size_t len = std::strlen(text);
+  auto cont = [](unsigned char c) { return (c & 0xC0u) == 0x80u; };
+  size_t i = 0;
+  while (i < len) {
+    const unsigned char c = v[i];
+    if (c < 0x80) {
+      ++i;
+      continue;
+    }
+    const size_t start = i;
+    const int width = c >= 0xC2 && c <= 0xDF ? 2 : c >= 0xE0 && c <= 0xEF ? 3 : c >= 0xF0 && c <= 0xF4 ? 4 : 0;
+    auto invalid = [&](int n) {
+      return "invalid utf-8 sequence of " + std::to_string(n) + " bytes from index " + std::to_string(start);
+    };
+    auto incomplete = [&]() { return "incomplete utf-8 byte sequence from index " + std::to_string(start); };
+    if (width == 0) return invalid(1);
+    if (i + 1 >= len) return incomplete();
+    const unsigned char c1 = v[i + 1];
+    bool ok1;
+    if (width == 2) {
+      ok1 = cont(c1);
+    } else if (width == 3) {
+      ok1 = (c == 0xE0 && c1 >= 0xA0 && c1 <= 0xBF) || (c >= 0xE1 && c <= 0xEC && cont(c1)) ||
+            (c == 0xED && c1 >= 0x80 && c1 <= 0x9F) || (c >= 0xEE && cont(c1));
+    } else {
+      ok1 = (c == 0xF0 && c1 >= 0x90 && c1 <= 0xBF) || (c >= 0xF1 && c <= 0xF3 && cont(c1)) ||
+            (c == 0xF4 && c1 >= 0x80 && c1 <= 0x8F);
+    }
+    if (!ok1) return invalid(1);
+    for (int k = 2; k < width; ++k) {
+      if (i + size_t(k) >= len) return incomplete();
+      if (!cont(v[i + size_t(k)])) return invalid(k);
+    }
+    i += size_t(width);
+  }
+  return std::nullopt;
+}
+
 template <typename F>
 char* guarded(const char* request, F&& f) {
   std::string resp;
   try {
     if (!request) {
       resp = "{\"status\":\"error\",\"error\":\"request pointer is null\"}";
+    } else if (auto bad_utf8 = utf8_error(request)) {
+      resp = error_json("request is not UTF-8: " + *bad_utf8);
     } else {
       resp = f(std::string(request));
     }
@@ -676,6 +741,19 @@ char* guarded(const char* request, F&& f) {
     resp = error_json("internal state error: native binding failure");
   }
   return dup_c(resp);
+}
+
+// the argument checks of the reference's registration entry points
+// (bindings/capi/src/lib.rs:6207-6216, :6253-6262), then the answer that user
+// bases are not built here
+template <typename Op, typename Map>
+char* user_basis_unsupported(const char* request, Op op, const Map* maps, size_t map_count) {
+  if (request && !op) return dup_c("{\"status\":\"error\",\"error\":\"user operator callback is null\"}");
+  if (request && map_count > 0 && !maps)
+    return dup_c("{\"status\":\"error\",\"error\":\"user symmetry callback array is null\"}");
+  return guarded(request, [](const std::string&) -> std::string {
+    throw unsupported_backend("user-defined bases are outside the MI355X spin-sector path");
+  });
 }
 
 }  // namespace
@@ -717,28 +795,30 @@ char* qmbed_run_json(const char* request) { return guarded(request, run_solve); 
 
 char* qmbed_command_json(const char* request) { return guarded(request, run_command); }
 
-char* qmbed_evolve_model_with_drive_json(const char* request, qmbed_drive_callback callback, void* context) {
-  (void)callback;
+char* qmbed_evolve_model_with_drive_json(const char* request, QmbedDriveCallback callback, void* context) {
   (void)context;
+  if (request && !callback) return dup_c("{\"status\":\"error\",\"error\":\"drive callback is null\"}");
   return guarded(request, [](const std::string&) -> std::string {
     throw unsupported_backend("driven (time-dependent) evolution is outside the MI355X static-Hamiltonian path");
   });
 }
 
-char* qmbed_register_user_basis_32_json(const char* request, qmbed_user_basis_callback_32 callback, void* context) {
-  (void)callback;
-  (void)context;
-  return guarded(request, [](const std::string&) -> std::string {
-    throw unsupported_backend("user-defined bases are outside the MI355X spin-sector path");
-  });
+char* qmbed_register_user_basis_32_json(const char* request, QmbedUserOp32Callback op,
+                                        QmbedUserNextState32Callback next_state,
+                                        QmbedUserPreCheck32Callback pre_check,
+                                        const QmbedUserMap32Callback* map_callbacks, size_t map_count) {
+  (void)next_state;
+  (void)pre_check;
+  return user_basis_unsupported(request, op, map_callbacks, map_count);
 }
 
-char* qmbed_register_user_basis_64_json(const char* request, qmbed_user_basis_callback_64 callback, void* context) {
-  (void)callback;
-  (void)context;
-  return guarded(request, [](const std::string&) -> std::string {
-    throw unsupported_backend("user-defined bases are outside the MI355X spin-sector path");
-  });
+char* qmbed_register_user_basis_64_json(const char* request, QmbedUserOp64Callback op,
+                                        QmbedUserNextState64Callback next_state,
+                                        QmbedUserPreCheck64Callback pre_check,
+                                        const QmbedUserMap64Callback* map_callbacks, size_t map_count) {
+  (void)next_state;
+  (void)pre_check;
+  return user_basis_unsupported(request, op, map_callbacks, map_count);
 }
 
 void qmbed_string_free(char* s) { std::free(s); }
@@ -896,6 +976,75 @@ double qmbed_hip_time_apply(const qmbed_hip_model* m, int dtype, const void* x, 
   }
 }
 
+double qmbed_hip_time_scan_batch(qmbed_hip_model* m, int width, int fp32, const double* points, int iters) {
+  try {
+    auto* tm = reinterpret_cast<TypedModel*>(m);
+    std::lock_guard<std::mutex> g(tm->m->mu);
+    Operator& op = tm->m->materialized();
+    hipStream_t s = tm->m->stream;
+    if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16)) return -2.0;
+    if (width > op.scan_batch_max_width() || !op.scan_batch_supported(s) || !points || iters < 1) return -2.0;
+    const size_t n = op.dim();
+    const int count = op.scan_batch_amp_count();
+    const size_t P = tm->m->components.size();
+    std::vector<std::vector<cplx>> coefs(static_cast<size_t>(width));
+    for (int b = 0; b < width; ++b)
+      for (size_t p = 0; p < P; ++p) coefs[size_t(b)].push_back(cplx(points[size_t(b) * P + p], 0.0));
+    DeviceArray<double> amps(size_t(width) * size_t(count)), diags(size_t(width) * 2 * n), ys(size_t(width) * n);
+    const size_t xbytes = (fp32 ? 4 : 8) * size_t(width) * n;
+    DeviceArray<uint8_t> x(xbytes);
+    op.scan_batch_tables(coefs, amps.data(), diags.data(), s);
+    // every lane holds the deterministic start vector
+    DeviceArray<double> v(n);
+    deterministic_start_device(v.data(), n, 0, s, false);
+    std::vector<const double*> src(static_cast<size_t>(width), v.data());
+    lk::interleave(src.data(), n, width, fp32 ? nullptr : reinterpret_cast<double*>(x.data()),
+                   fp32 ? reinterpret_cast<float*>(x.data()) : nullptr, s);
+    const double* am[16] = {};
+    const double* dg[16] = {};
+    double* yp[16] = {};
+    for (int b = 0; b < width; ++b) {
+      am[b] = amps.data() + size_t(b) * count;
+      dg[b] = diags.data() + size_t(b) * 2 * n;
+      yp[b] = ys.data() + size_t(b) * n;
+    }
+    for (int i = 0; i < 2; ++i) op.apply_scan_batch(x.data(), fp32 != 0, width, am, dg, yp, nullptr, s);
+    hipEvent_t a, b;
+    QMBED_HIP(hipEventCreate(&a));
+    QMBED_HIP(hipEventCreate(&b));
+    QMBED_HIP(hipEventRecord(a, s));
+    for (int i = 0; i < iters; ++i) op.apply_scan_batch(x.data(), fp32 != 0, width, am, dg, yp, nullptr, s);
+    QMBED_HIP(hipEventRecord(b, s));
+    QMBED_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    QMBED_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return double(ms) / double(iters);
+  } catch (...) {
+    return -1.0;
+  }
+}
+
+int qmbed_hip_launch_timer(int on) {
+  try {
+    launch_timer_switch(on != 0);
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+char* qmbed_hip_launch_timer_report(void) {
+  try {
+    return dup_c("{\"status\":\"ok\",\"result\":" + launch_timer_report() + "}");
+  } catch (Error& e) {
+    return dup_c(error_json(e.message));
+  } catch (std::exception& e) {
+    return dup_c(error_json(std::string("internal state error: ") + e.what()));
+  }
+}
+
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
   return guarded(solver_json, [&](const std::string& req) {
     auto* tm = reinterpret_cast<const TypedModel*>(m);
@@ -999,14 +1148,14 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
     if (options_json) {
       json::Value v = parse_or_throw(options_json);
       if (v.kind != json::Value::Obj) bad("scan options must be an object");
-      if (auto* x = v.get("snapshots"); x && !x->is_null()) o.snapshots = int(as_long(*x, "snapshots"));
+      if (auto* x = v.get("snapshots"); x && !x->is_null()) o.snapshots = as_int(*x, "snapshots");
       if (auto* x = v.get("hidden"); x && !x->is_null()) {
         auto& a = x->array();
         if (a.size() != 2) bad("hidden must hold two layer widths");
-        o.hidden1 = int(as_long(a[0], "hidden"));
-        o.hidden2 = int(as_long(a[1], "hidden"));
+        o.hidden1 = as_int(a[0], "hidden");
+        o.hidden2 = as_int(a[1], "hidden");
       }
-      if (auto* x = v.get("epochs"); x && !x->is_null()) o.epochs = int(as_long(*x, "epochs"));
+      if (auto* x = v.get("epochs"); x && !x->is_null()) o.epochs = as_int(*x, "epochs");
       if (auto* x = v.get("learning_rate"); x && !x->is_null()) o.learning_rate = float(as_double(*x, "learning_rate"));
       o.final_learning_rate = o.learning_rate;  // constant unless given
       if (auto* x = v.get("final_learning_rate"); x && !x->is_null())
@@ -1026,7 +1175,7 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
         else bad("space must be \"greedy\" or \"principal\"");
       }
       if (auto* x = v.get("samples"); x && !x->is_null()) {
-        o.samples = int(as_long(*x, "samples"));
+        o.samples = as_int(*x, "samples");
         if (o.samples < 0 || o.samples > 65536) bad("samples must lie in [0, 65536]");
       }
     }

@@ -1,11 +1,14 @@
 // Error constructors (display strings follow QMBED src/error.rs:39-73) and
 // small host utilities shared by the library.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <sstream>
 
-#include "qmbed.hpp"
+#include "internal.hpp"
 
 namespace qmbed {
 
@@ -95,6 +98,105 @@ int device_count() {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) return 0;
   return n;
+}
+
+// ------------------------------------------------------------ launch timer --
+namespace {
+
+struct LaunchTimer {
+  std::atomic<bool> on{false};
+  std::mutex mu;
+  std::vector<hipEvent_t> pool;  // events free for reuse
+  struct Rec {
+    std::string kernel;
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t take() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    QMBED_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void recycle() {
+    for (auto& r : recs) {
+      if (r.a) pool.push_back(r.a);
+      if (r.b) pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+};
+
+LaunchTimer& timer() {
+  static LaunchTimer* t = new LaunchTimer;  // lives until the process ends
+  return *t;
+}
+
+}  // namespace
+
+int launch_timer_begin(const char* kernel, hipStream_t s) {
+  auto& t = timer();
+  if (!t.on.load(std::memory_order_relaxed)) return -1;
+  std::lock_guard<std::mutex> g(t.mu);
+  LaunchTimer::Rec r;
+  r.kernel = kernel;
+  r.a = t.take();
+  QMBED_HIP(hipEventRecord(r.a, s));
+  t.recs.push_back(r);
+  return int(t.recs.size()) - 1;
+}
+
+void launch_timer_end(int record, hipStream_t s) {
+  if (record < 0) return;
+  auto& t = timer();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (size_t(record) >= t.recs.size()) return;
+  auto& r = t.recs[size_t(record)];
+  r.b = t.take();
+  QMBED_HIP(hipEventRecord(r.b, s));
+}
+
+void launch_timer_switch(bool on) {
+  auto& t = timer();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (on) {
+    for (auto& r : t.recs) {  // pending records of an earlier run complete first
+      if (r.b) QMBED_HIP(hipEventSynchronize(r.b));
+    }
+    t.recycle();
+  }
+  t.on.store(on, std::memory_order_relaxed);
+}
+
+std::string launch_timer_report() {
+  auto& t = timer();
+  std::lock_guard<std::mutex> g(t.mu);
+  std::map<std::string, std::pair<long, double>> sum;
+  for (auto& r : t.recs) {
+    if (!r.b) continue;
+    QMBED_HIP(hipEventSynchronize(r.b));
+    float ms = 0;
+    QMBED_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto& e = sum[r.kernel];
+    e.first += 1;
+    e.second += double(ms);
+  }
+  std::ostringstream os;
+  os.precision(17);
+  os << '{';
+  bool first = true;
+  for (auto& kv : sum) {
+    if (!first) os << ',';
+    first = false;
+    os << '"' << kv.first << "\":{\"launches\":" << kv.second.first << ",\"ms_total\":" << kv.second.second
+       << ",\"ms_mean\":" << kv.second.second / double(kv.second.first) << '}';
+  }
+  os << '}';
+  return os.str();
 }
 
 }  // namespace qmbed

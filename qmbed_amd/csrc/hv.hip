@@ -17,6 +17,7 @@
 // product, which is what the flip-group tables hold.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <set>
@@ -2247,6 +2248,10 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
   const dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t sm = sizeof(double) * (size_t(width) * size_t(a.n_amp) + 81);
   const int sym = sym_code(*basis_);
+  char kname[64];
+  std::snprintf(kname, sizeof kname, "hv_scan_batch_kernel<%s, %d, %d>", fp32 ? "float" : "double", width,
+                sym < 0 || sym > 3 ? 3 : sym);
+  const int timed = launch_timer_begin(kname, s);
 #define QMBED_SCAN_LAUNCH(SX, W)                                                                                   \
   switch (sym) {                                                                                                  \
     case 0: hipLaunchKernelGGL((hv_scan_batch_kernel<SX, W, 0>), grid, block, sm, s, a, sb, static_cast<const SX*>(x)); break; \
@@ -2273,6 +2278,7 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
   }
 #undef QMBED_SCAN_LAUNCH
   QMBED_HIP(hipGetLastError());
+  launch_timer_end(timed, s);
 }
 
 bool prepare_interleaved_action(const Operator& op, hipStream_t s) {

@@ -39,6 +39,19 @@ class SectorBatchAction {
   DeviceArray<uint8_t> sec_;  // per-sector kernel arguments
 };
 
+// ---- launch timer (common.cpp) ----
+// While switched on, the instrumented launches (the batched scan H·v) are
+// bracketed by two HIP events on their own stream; the report resolves the
+// pairs and sums their elapsed times per kernel instance. Off, a launch pays
+// one relaxed atomic load.
+// Returns a record index, or -1 when the timer is off.
+int launch_timer_begin(const char* kernel, hipStream_t s);
+void launch_timer_end(int record, hipStream_t s);
+void launch_timer_switch(bool on);
+// {"kernel": {"launches": n, "ms_total": t, "ms_mean": t / n}, ...};
+// synchronizes the recorded events
+std::string launch_timer_report();
+
 // ---- FP32 MLP of the learned warm start (mlp.hip) ----
 int mlp_parameter_count(int in, int h1, int h2, int out);
 // Y [n][out] = MLP(X [n][in]) on the device

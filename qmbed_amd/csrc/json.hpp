@@ -63,12 +63,24 @@ class Parser {
   void ws() {
     while (p_ < s_.size() && (s_[p_] == ' ' || s_[p_] == '\n' || s_[p_] == '\r' || s_[p_] == '\t')) ++p_;
   }
+  // nesting limit of arrays and objects (serde_json's default recursion
+  // limit): deeper requests are rejected instead of exhausting the stack
+  static constexpr int kMaxDepth = 128;
+  int depth_ = 0;
+  struct Nest {
+    Parser& p;
+    explicit Nest(Parser& q) : p(q) {
+      if (++p.depth_ > kMaxDepth) p.fail("recursion limit exceeded");
+    }
+    ~Nest() { --p.depth_; }
+  };
   Value value() {
     ws();
     if (p_ >= s_.size()) fail("EOF while parsing a value");
     char c = s_[p_];
     Value v;
     if (c == '{') {
+      Nest nest(*this);
       ++p_;
       v.kind = Value::Obj;
       v.obj = std::make_shared<Object>();
@@ -98,6 +110,7 @@ class Parser {
       }
     }
     if (c == '[') {
+      Nest nest(*this);
       ++p_;
       v.kind = Value::Arr;
       v.arr = std::make_shared<Array>();
@@ -153,6 +166,36 @@ class Parser {
     if (!end || *end) fail("invalid number");
     return v;
   }
+  unsigned hex4() {
+    if (p_ + 4 > s_.size()) fail("EOF while parsing a string");
+    unsigned cp = 0;
+    for (int k = 0; k < 4; ++k) {
+      const char h = s_[p_ + size_t(k)];
+      const int d = h >= '0' && h <= '9' ? h - '0' : h >= 'a' && h <= 'f' ? h - 'a' + 10
+                                                   : h >= 'A' && h <= 'F' ? h - 'A' + 10 : -1;
+      if (d < 0) fail("invalid escape");
+      cp = cp * 16 + unsigned(d);
+    }
+    p_ += 4;
+    return cp;
+  }
+  static void utf8(std::string& out, unsigned cp) {
+    if (cp < 0x80) {
+      out += char(cp);
+    } else if (cp < 0x800) {
+      out += char(0xC0 | (cp >> 6));
+      out += char(0x80 | (cp & 0x3F));
+    } else if (cp < 0x10000) {
+      out += char(0xE0 | (cp >> 12));
+      out += char(0x80 | ((cp >> 6) & 0x3F));
+      out += char(0x80 | (cp & 0x3F));
+    } else {
+      out += char(0xF0 | (cp >> 18));
+      out += char(0x80 | ((cp >> 12) & 0x3F));
+      out += char(0x80 | ((cp >> 6) & 0x3F));
+      out += char(0x80 | (cp & 0x3F));
+    }
+  }
   std::string str() {
     ++p_;  // opening quote
     std::string out;
@@ -171,23 +214,23 @@ class Parser {
           case 'r': out += '\r'; break;
           case 't': out += '\t'; break;
           case 'u': {
-            if (p_ + 4 > s_.size()) fail("bad unicode escape");
-            unsigned cp = std::stoul(s_.substr(p_, 4), nullptr, 16);
-            p_ += 4;
-            if (cp < 0x80) out += char(cp);
-            else if (cp < 0x800) {
-              out += char(0xC0 | (cp >> 6));
-              out += char(0x80 | (cp & 0x3F));
-            } else {
-              out += char(0xE0 | (cp >> 12));
-              out += char(0x80 | ((cp >> 6) & 0x3F));
-              out += char(0x80 | (cp & 0x3F));
+            unsigned cp = hex4();
+            if (cp >= 0xDC00 && cp <= 0xDFFF) fail("lone trailing surrogate in hex escape");
+            if (cp >= 0xD800 && cp <= 0xDBFF) {  // a leading surrogate needs its trailing half
+              if (p_ + 2 > s_.size() || s_[p_] != '\\' || s_[p_ + 1] != 'u')
+                fail("lone leading surrogate in hex escape");
+              p_ += 2;
+              const unsigned lo = hex4();
+              if (lo < 0xDC00 || lo > 0xDFFF) fail("lone leading surrogate in hex escape");
+              cp = 0x10000 + ((cp - 0xD800) << 10) + (lo - 0xDC00);
             }
+            utf8(out, cp);
             break;
           }
           default: fail("bad escape");
         }
       } else {
+        if (static_cast<unsigned char>(c) < 0x20) fail("control character (\\u0000-\\u001F) found while parsing a string");
         out += c;
       }
     }

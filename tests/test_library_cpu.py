@@ -97,3 +97,85 @@ def test_no_cpu_fallback_without_library(tmp_path, monkeypatch):
     finally:
         monkeypatch.delenv("QMBED_HIP_LIBRARY")
         importlib.reload(qmbed_amd)
+
+
+def _raw(fn, *args):
+    import json
+
+    import qmbed_amd
+
+    lib = qmbed_amd.library()
+    f = getattr(lib, fn)
+    f.restype = ctypes.c_void_p
+    return json.loads(qmbed_amd._take(f(*args)))
+
+
+def test_user_basis_entry_points_take_the_reference_arguments(qa):
+    # six parameters: request, operator, next_state, pre_check, map_callbacks, map_count
+    # (bindings/capi/src/lib.rs:6194-6251); the argument checks come first (:6207-6216)
+    op32 = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int8, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_void_p)(lambda *a: 0)
+    op64 = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int8, ctypes.c_int32, ctypes.c_int32,
+                            ctypes.c_void_p)(lambda *a: 0)
+    req = b'{"sites": 4, "states_per_site": 2, "allowed_ops": "xyz"}'
+    for fn, op in (("qmbed_register_user_basis_32_json", op32), ("qmbed_register_user_basis_64_json", op64)):
+        lib = qa.library()
+        getattr(lib, fn).argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_size_t]
+        assert _raw(fn, None, op, None, None, None, 0) == {"status": "error", "error": "request pointer is null"}
+        assert _raw(fn, req, None, None, None, None, 0) == {"status": "error",
+                                                            "error": "user operator callback is null"}
+        assert _raw(fn, req, op, None, None, None, 2) == {"status": "error",
+                                                          "error": "user symmetry callback array is null"}
+        r = _raw(fn, req, op, None, None, None, 0)
+        assert r["status"] == "error" and r["error"].startswith("unsupported backend")
+
+
+def test_drive_entry_point_checks_its_callback(qa):
+    # QmbedDriveCallback(context, time, QmbedComplex64* coefficients, count) (lib.rs:464-469, :6153-6179)
+    lib = qa.library()
+    lib.qmbed_evolve_model_with_drive_json.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]
+    cb = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                          ctypes.c_size_t)(lambda *a: 0)
+    req = b'{"handle": "1"}'
+    assert _raw("qmbed_evolve_model_with_drive_json", None, cb, None)["error"] == "request pointer is null"
+    assert _raw("qmbed_evolve_model_with_drive_json", req, None, None)["error"] == "drive callback is null"
+    r = _raw("qmbed_evolve_model_with_drive_json", req, cb, None)
+    assert r["status"] == "error" and r["error"].startswith("unsupported backend")
+
+
+def test_non_utf8_requests(qa):
+    # CStr::to_str failures, displayed as Rust's Utf8Error (lib.rs:6096-6103)
+    lib = qa.library()
+    for fn in ("qmbed_run_json", "qmbed_command_json"):
+        getattr(lib, fn).argtypes = [ctypes.c_char_p]
+        r = _raw(fn, b'{"a\xff": 1}')
+        assert r == {"status": "error", "error": "request is not UTF-8: invalid utf-8 sequence of 1 bytes from index 3"}
+        r = _raw(fn, b'{"\xe2\x82')
+        assert r["error"] == "request is not UTF-8: incomplete utf-8 byte sequence from index 2"
+        r = _raw(fn, b'{"\xe2\x28\xa1"}')  # bad second byte of a 3-byte sequence
+        assert r["error"] == "request is not UTF-8: invalid utf-8 sequence of 1 bytes from index 2"
+        r = _raw(fn, b'{"\xe2\x82\x28"}')  # bad third byte
+        assert r["error"] == "request is not UTF-8: invalid utf-8 sequence of 2 bytes from index 2"
+        r = _raw(fn, b'{"\xed\xa0\x80"}')  # an encoded surrogate
+        assert r["error"] == "request is not UTF-8: invalid utf-8 sequence of 1 bytes from index 2"
+        r = _raw(fn, '{"é€😀": 1}'.encode())  # valid multi-byte text parses on to the schema checks
+        assert not r["error"].startswith("request is not UTF-8")
+
+
+def test_deep_nesting_and_bad_numbers_are_errors(qa):
+    deep = "[" * 5000 + "]" * 5000
+    r = qa.raw_call("qmbed_run_json", deep)
+    assert r["status"] == "error" and "recursion limit exceeded" in r["error"]
+    ok_depth = '{"basis": ' + "[" * 100 + "]" * 100 + "}"
+    r = qa.raw_call("qmbed_run_json", ok_depth)
+    assert "recursion limit" not in r["error"]
+    r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis", "basis": {"kind": "spin", "sites": 1e300}})
+    assert r["error"] == "invalid options: sites: integer out of range"
+    r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis",
+                                           "basis": {"kind": "spin", "sites": 4, "up": 4294967298}})
+    assert r["error"] == "invalid options: up: integer out of range"
+    r = qa.raw_call("qmbed_run_json", '{"basis": "\\ud800"}')
+    assert r["status"] == "error" and "surrogate" in r["error"]
+    r = qa.raw_call("qmbed_run_json", '{"basis": "\\ud83d\\ude00"}')  # a surrogate pair is one code point
+    assert "surrogate" not in r["error"]
