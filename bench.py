@@ -8,35 +8,34 @@ momentum 0, parity +1 (N = 1,296,858), scanned on the 64×64 grid
 hx ∈ [0.2, 2.0], hz ∈ [0, 0.5]. The operator is one compiled family
 (couplings re-evaluated per grid point on the same sector tables). Setup,
 timed and reported separately: 256 seeded training points solved by plain
-Lanczos (tolerance 1e-10), the 32-dimensional snapshot space of their ground
-states (the 32 leading principal components) with the family's projection on
-it, and the 2-64-64-32 FP32 network trained on the GPU by the library's own
-kernels.
+Lanczos (tolerance 1e-10), the 32-dimensional snapshot space S of their
+ground states (the 32 leading principal components) with the family's
+projection on it, and the 2-64-64-32 FP32 network trained on the GPU by the
+library's own kernels.
 
 One step = 16 held-out grid points (one lockstep batch: a Lanczos window per
 point, shared H·v launches per step over the windows still running, up to 16
-FP32 or 8 FP64 vectors per launch),
-each solved to residual
-≤ 1e-8·max(|E|,1) with the FP32-matvec schedule from the snapshot start: the
-lowest eigenvector of the family projected on the 32-dimensional snapshot
-space.
-`value` is ground-state solves per second over the timed steps. Beside it
-(extras): the plain FP64 solve (deterministic start, one eigsh per point) of the
-same points, the
-same start in FP64, the network's start (S·MLP(c)) with the FP32 schedule,
-mean Lanczos iterations of each, per-point energy differences, the amortized
-throughput of the whole
-4,096-point scan including setup, the H·v bandwidth at this sector, and the
-FP64 6-eigenpair solve of the XXZ L=24 sector, the FP32/FP64 H·v of the
-J1-J2 L=28 sector, the 72 XXZ L=24 momentum/magnetization sectors solved in
-one lockstep batch beside one-by-one solves, and exp(-iHt)ψ of 64 seeded
-states at t ∈ {0.1, 1, 10} on the XXZ L=24 up=12 k=0 sector.
+FP32 or 8 FP64 vectors per launch), each solved to residual
+≤ 1e-8·max(|E|,1) with the FP32-matvec schedule from the learned start
+S·(y(c) + MLP(c)): y(c) is the lowest eigenvector of the family projected on
+S, and the network maps the couplings c to the step from y(c) to the
+normalized projection of the ground state on S (`--start snapshot` makes the
+plain S·y(c) start the headline instead). `value` is ground-state solves per
+second over the timed steps. Beside it (extras, one compact JSON line;
+`--details FILE` writes the per-point figures): the same points solved by
+plain FP64 eigsh one point at a time (the baseline), from the other start,
+from the headline start in FP64 and one point at a time, and plain FP64 in
+lockstep; the amortized throughput of the whole 4,096-point scan including
+setup; H·v bandwidth at this sector; the FP64 6-eigenpair solve of the XXZ
+L=24 sector, the J1-J2 L=28 sector's H·v and ground state, the 72 XXZ L=24
+momentum/magnetization sectors in one lockstep batch against one by one, and
+exp(-iHt)ψ of 64 seeded states at t ∈ {0.1, 1, 10}.
 
-The roofline object is the H·v of the FP32 schedule at the headline sector
-(FP32 input, FP64 accumulation and output): algorithmic bytes
-N·(4+8) + nnz·(4+4) per launch over the kernel time measured with HIP events
-on the stream the kernel runs on. The pure FP32 H·v (N·(4+4) + nnz·(4+4))
-is reported beside it.
+The roofline object is the 16-lane FP32 batched scan H·v, the kernel the
+timed steps spend most of their time in: the mean duration of its launches
+inside the timed region, from HIP events the library's launch timer records
+on the kernel's own stream, against its algorithmic bytes
+N·(4 + 8W) + nnz·(4 + 4W) per launch (W = 16 lanes sharing one index stream).
 """
 
 import argparse
@@ -59,21 +58,26 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB device copy (summary and commands in profiles/hv_stream_ising_L26.md)
 HV_MIXED_TRAFFIC_BYTES = 275021548
 HV_FP32_TRAFFIC_BYTES = 267536152
-# HBM bytes per launch of the 16-lane FP32 batched scan H·v (the roofline
-# kernel), from the same counter runs; None until measured
-SCAN_W16_TRAFFIC_BYTES = None
+# HBM bytes per launch of the 16-lane FP32 batched scan H·v at 16 held-out
+# grid points (the roofline kernel), from the same counter runs and
+# calibration (profiles/scan_batch_hv_ising_L26.md)
+SCAN_W16_TRAFFIC_BYTES = 2459658887
 L_SCAN = 26
 BATCH = 16
 # the headline start: the lowest eigenvector of the family projected on the
 # snapshot space (scan solve mode "rayleigh_ritz"); the network's start (mode
 # True) is measured beside it. The space: the SNAPSHOTS leading principal
 # components of the training ground states.
-WARM = "rayleigh_ritz"
-OTHER = True
-TARGETS = "projections"
+STARTS = {"network": True, "snapshot": "rayleigh_ritz"}
+HEADLINE_TEXT = {
+    "network": "learned start v0 = S (y(c) + MLP(c)): S a 32-dimensional snapshot space (the leading principal "
+               "components of 256 plain-Lanczos training ground states), y(c) the lowest eigenvector of the family "
+               "projected on S, MLP the 2-64-64-32 FP32 network trained on the GPU to map (hx, hz) to the step from "
+               "y(c) to the normalized projection of the ground state on S",
+    "snapshot": "snapshot start v0 = S y(c): S a 32-dimensional snapshot space (the leading principal components of "
+                "256 plain-Lanczos training ground states), y(c) the lowest eigenvector of the family projected on S",
+}
 EPOCHS = 4000
-HEADLINE_TEXT = ("start v0 = S y, y the lowest eigenvector of the family projected on a 32-dimensional snapshot "
-                 "space S (the leading principal components of 256 plain-Lanczos training ground states)")
 SPACE = "principal"
 N_TRAIN = 256
 SNAPSHOTS = 32
@@ -287,22 +291,26 @@ def scan_bytes_per_launch(dim, nnz, width, fp32=True):
     return dim * (4 + 8 * width) + nnz * (4 + sx * width)
 
 
+def sig(x, d=4):
+    """x rounded to d significant digits (JSON size)."""
+    if x is None or x == 0 or not np.isfinite(x):
+        return x
+    return float(f"{x:.{d}g}")
+
+
 def mode_summary(rs, ref_energies=None, seconds=None, points=None):
-    """Compact per-mode figures of a list of scan_solve results."""
+    """Compact per-mode figures of a list of scan_solve results: solves/s,
+    mean and max Lanczos iterations, mean FP32-matvec iterations, the largest
+    relative energy difference to the plain FP64 solve."""
     def cat(key):
         return [x for r in rs for x in r[key]]
     it = cat("iterations")
-    out = {"mean_iterations": mean(it), "max_iterations": int(max(it)) if it else None,
-           "max_residual": float(np.max(cat("residuals")))}
-    if "fp32_iterations" in rs[0]:
-        out["mean_fp32_iterations"] = mean(cat("fp32_iterations"))
-    if "fallbacks" in rs[0]:
-        out["fallbacks"] = int(sum(r["fallbacks"] for r in rs))
-    if seconds is not None and points:
-        out["solves_per_s"] = points / seconds
+    out = {"sps": sig(points / seconds) if seconds else None, "it": sig(mean(it)), "it_max": int(max(it))}
+    if any(cat("fp32_iterations")):
+        out["it32"] = sig(mean(cat("fp32_iterations")))
     if ref_energies is not None:
         e = np.array(cat("energies"))
-        out["max_energy_rel_diff_vs_plain_fp64"] = float(np.max(np.abs(e - ref_energies) / np.abs(ref_energies)))
+        out["dE"] = sig(float(np.max(np.abs(e - ref_energies) / np.abs(ref_energies))), 2)
     return out
 
 
@@ -313,6 +321,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extended", action="store_true")
+    ap.add_argument("--start", choices=sorted(STARTS), default="network",
+                    help="the headline start (the other is measured beside it)")
+    ap.add_argument("--targets", choices=("correction", "projections", "continuation"), default="correction",
+                    help="what the network learns")
     ap.add_argument("--details", default=None,
                     help="write the per-point iterations, times and energy differences of every mode to this JSON file")
     ap.add_argument("--hv-profile", action="store_true",
@@ -357,8 +369,10 @@ def main():
     train = grid[perm[:N_TRAIN]]
     n_bench = BATCH * (args.warmup + args.steps)
     bench_pts = grid[perm[N_TRAIN:N_TRAIN + n_bench]]
+    WARM = STARTS[args.start]
+    OTHER = STARTS["snapshot" if args.start == "network" else "network"]
     scan = qa.Scan(model, snapshots=SNAPSHOTS, epochs=EPOCHS, learning_rate=2e-3, seed=0, space=SPACE,
-                   targets=TARGETS)
+                   targets=args.targets)
     t0 = time.perf_counter()
     rep = scan.train(train, tolerance=1e-10)
     train_s = time.perf_counter() - t0
@@ -372,7 +386,7 @@ def main():
     modes = {  # name: (start, solver keys); the first is the headline
         "headline": (WARM, warm_kw),
         "plain_fp64": (False, plain_kw),
-        "other_start_fp32": (OTHER, warm_kw),
+        ("snapshot" if args.start == "network" else "network") + "_start_fp32": (OTHER, warm_kw),
         "headline_start_fp64": (WARM, {"tolerance": TOL}),
         "headline_one_by_one": (WARM, dict(warm_kw, lockstep=0)),
         "plain_fp64_lockstep": (False, {"tolerance": TOL}),
@@ -406,7 +420,9 @@ def main():
         results[name] = (rs, time.perf_counter() - t0)
     ep = np.array([x for r in results["plain_fp64"][0] for x in r["energies"]])
     summary = {name: mode_summary(rs, ep, sec, npts) for name, (rs, sec) in results.items()}
-    plain_rate = summary["plain_fp64"]["solves_per_s"]
+    plain_rate = npts / results["plain_fp64"][1]
+    max_res = float(np.max([x for r in warm for x in r["residuals"]]))
+    fallbacks = int(sum(r["fallbacks"] for r in warm))
     n_grid = len(grid)
     setup_s = build_s + train_s
     amort_warm = n_grid / (setup_s + n_grid / value)
@@ -428,72 +444,66 @@ def main():
         ach = algo16 / (launches[rk]["ms_mean"] * 1e-3) / 1e9
     else:
         algo16 = ach = None
-    kernel_share = {k: {"launches": v["launches"], "ms_mean": round(v["ms_mean"], 5),
-                        "ms_total": round(v["ms_total"], 3)} for k, v in launches.items()}
+    top = sorted(launches, key=lambda k: -launches[k]["ms_total"])[:4]
+    kernel_share = {k.replace("hv_scan_batch_kernel", "scan"): [launches[k]["launches"], sig(launches[k]["ms_mean"])]
+                    for k in top}
     # the same kernel on its own (20 launches at 16 fixed points of the scan grid)
     alone = {}
     for w in (16, 8):
         ms = model.time_scan_batch(bench_pts[:w], fp32=True, iters=20)
         b = scan_bytes_per_launch(model.dim, nnz, w, True)
-        alone[f"fp32_w{w}"] = {"ms": round(ms, 5), "bytes_per_launch": b,
-                               "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        alone[f"w{w}"] = [sig(ms), sig(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)]
 
     def hv_short(h):
-        return {"ms": round(h["ms"], 5), "bytes_per_launch": h["bytes_per_launch"], "frac": round(h["frac"], 4),
-                "mode": h["mode"]}
+        return [sig(h["ms"]), sig(h["frac"], 3)]
 
     extras = {
         "scan": {
-            "dim": model.dim, "nnz": nnz, "grid_points": n_grid, "training_points": N_TRAIN,
-            "snapshots": rep["snapshots"], "network_targets": TARGETS, "timed_points": npts,
-            "setup_s": {"build": round(build_s, 3), "training": round(train_s, 3),
-                        "training_solves": round(rep["solve_seconds"], 3), "network": round(rep["train_seconds"], 3)},
-            "network_loss": [rep["loss_first"], rep["loss_last"]],
+            "nnz": nnz, "targets": args.targets, "timed_points": npts, "max_residual": sig(max_res, 3),
+            "fallbacks": fallbacks,
+            "setup_s": [sig(build_s, 3), sig(train_s, 3), sig(rep["solve_seconds"], 3), sig(rep["train_seconds"], 3)],
+            "loss": [sig(rep["loss_first"], 3), sig(rep["loss_last"], 3)],
             "modes": summary,
-            "headline_vs_plain_fp64": {"throughput": value / plain_rate,
-                                       "iterations": summary["plain_fp64"]["mean_iterations"]
-                                       / summary["headline"]["mean_iterations"]},
-            "amortized_full_scan": {"headline_incl_setup": amort_warm, "plain_fp64_incl_build": amort_plain,
-                                    "speedup": amort_warm / amort_plain},
+            "vs_plain_fp64": [sig(value / plain_rate), sig(summary["plain_fp64"]["it"] / summary["headline"]["it"])],
+            "full_scan_incl_setup": [sig(amort_warm), sig(amort_plain), sig(amort_warm / amort_plain)],
         },
         "scan_hv_launches": kernel_share,
         "scan_hv_alone": alone,
-        "hv_c5_fp32_in_fp64_out": hv_short(hvmix),
-        "hv_c5_fp32": hv_short(hv32),
-        "hv_c5_fp64": hv_short(hv64),
+        "hv_c5": {"mixed": hv_short(hvmix), "fp32": hv_short(hv32), "fp64": hv_short(hv64)},
     }
 
     if not args.no_extended:
         # XXZ L=24 (N=112,720): 6 lowest eigenpairs, FP64, tolerance 1e-10
-        t0 = time.perf_counter()
         xxz = qa.Model({"kind": "spin", "sites": 24, "up": 12, "momentum": 0}, xxz_terms(24, 1.5))
-        xs = time.perf_counter() - t0
         xxz.eigsh(6, tolerance=1e-10)
         t0 = time.perf_counter()
         rs = [xxz.eigsh(6, tolerance=1e-10) for _ in range(5)]
         dtx = time.perf_counter() - t0
-        extras["xxz_L24_eigsh6"] = {"setup_s": round(xs, 3), "solves_per_s": 5 / dtx,
-                                    "iterations": rs[-1]["iterations"], "E0": rs[-1]["eigenvalues"][0],
-                                    "max_residual": max(rs[-1]["residuals"])}
-        extras["hv_xxz_L24_fp64"] = hv_short(hv_roofline(xxz, np.float64, 8))
+        extras["xxz_L24_eigsh6"] = {"sps": sig(5 / dtx), "it": rs[-1]["iterations"],
+                                    "hv_fp64": hv_short(hv_roofline(xxz, np.float64, 8))}
         xxz.close()
         # J1-J2 L=28 (N=1,432,860): FP32/FP64 H·v and ground state
-        t0 = time.perf_counter()
         big = qa.Model({"kind": "spin", "sites": 28, "up": 14, "momentum": 0}, j1j2_terms(28))
-        j = {"setup_s": round(time.perf_counter() - t0, 3),
-             "hv_fp32": hv_short(hv_roofline(big, np.float32, 4, iters=20)),
+        j = {"hv_fp32": hv_short(hv_roofline(big, np.float32, 4, iters=20)),
              "hv_fp64": hv_short(hv_roofline(big, np.float64, 8, iters=20))}
+        e = {}
         for prec in ("fp64", "fp32"):
             t0 = time.perf_counter()
             gs = big.eigsh(1, tolerance=1e-10, matvec_precision=prec)
-            j[f"gs_{prec}"] = {"s": round(time.perf_counter() - t0, 3), "E0": gs["eigenvalues"][0],
-                               "residual": gs["residuals"][0], "iterations": gs["iterations"],
-                               "fp32_iterations": gs["fp32_iterations"]}
-        j["gs_fp32"]["rel_err_vs_fp64"] = abs(j["gs_fp32"]["E0"] - j["gs_fp64"]["E0"]) / abs(j["gs_fp64"]["E0"])
+            e[prec] = gs["eigenvalues"][0]
+            j[f"gs_{prec}"] = [sig(time.perf_counter() - t0, 3), gs["iterations"], gs["fp32_iterations"]]
+        j["E0"] = e["fp64"]
+        j["fp32_rel_err"] = abs(e["fp32"] - e["fp64"]) / abs(e["fp64"])
         extras["j1j2_L28"] = j
         big.close()
-        extras.update(sector_batch(qa))
-        extras.update(expm_batch(qa))
+        sb = sector_batch(qa)["c4_sectors72"]
+        extras["c4_sectors72"] = {f"k{k}": [sig(sb[f"eigenpairs_{k}"]["batch_s"], 3),
+                                            sig(sb[f"eigenpairs_{k}"]["one_by_one_s"], 3),
+                                            sig(sb[f"eigenpairs_{k}"]["max_rel_diff_vs_one_by_one"], 2)]
+                                  for k in (1, 6)}
+        eb = expm_batch(qa)["c4_expm64"]
+        extras["c4_expm64"] = [sig(eb["seconds"], 3), eb["matrix_vector_products"],
+                               sig(eb["max_norm_deviation"], 2)]
 
     if args.details:
         with open(args.details, "w") as f:
@@ -517,7 +527,7 @@ def main():
         "dtype": "f32 matvec, f64 Lanczos/reorth/polish",
         "data": "synthetic (Hamiltonian couplings of the scan grid; seeded training and benchmark points)",
         "config": {"workload": f"Ising chain -ZZ - hx X - hz Z (Pauli), L=26 periodic, k=0, p=+1 (N={model.dim}, "
-                               f"nnz={nnz}), 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT}; FP32-matvec "
+                               f"nnz={nnz}), 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT[args.start]}; FP32-matvec "
                                "schedule; ground state to residual <= 1e-8*max(|E|,1); one step = 16 held-out "
                                "points solved in one lockstep batch; baseline plain FP64, one eigsh per point",
                    "global_batch": BATCH},
@@ -525,8 +535,8 @@ def main():
                      "frac": ach / HBM_PEAK_GBS if ach else None,
                      "traffic": SCAN_W16_TRAFFIC_BYTES if rk and rk.startswith("hv_scan_batch_kernel<float, 16,")
                      else None,
-                     "kernel": f"{rk}: mean of {launches[rk]['launches'] if rk else 0} launches in the timed region "
-                               f"(HIP events on its stream); algorithmic bytes N(4+8W)+nnz(4+4W) = {algo16}"},
+                     "kernel": f"{rk}, mean of {launches[rk]['launches'] if rk else 0} launches in the timed region "
+                               f"(HIP events on its stream); N(4+8W)+nnz(4+4W) = {algo16} B"},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(model.dim),
         "extras": extras,
     }

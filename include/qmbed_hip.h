@@ -206,7 +206,9 @@ char* qmbed_hip_model_set_parameters(qmbed_hip_model* model, const char* paramet
  * the coefficients of the training ground states over the snapshots;
  * "continuation": it learns the lowest eigenvector of the family projected
  * on the snapshots at the training points and `samples` seeded points of
- * their bounding box. provisional: training points after the first
+ * their bounding box; "correction": it learns the step from that lowest
+ * eigenvector y(c) to the training ground state's normalized projection on
+ * the snapshots, and its start is S·(y(c) + MLP(c)). provisional: training points after the first
  * `snapshots` start from the projection on the snapshot space of those
  * already solved. space "greedy": `snapshots` training ground states chosen
  * by pivoted Cholesky of their Gram matrix; "principal": the `snapshots`
@@ -223,7 +225,8 @@ void qmbed_hip_scan_free(qmbed_hip_scan* scan);
  * first/last training loss). */
 char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json);
 /* Ground states of n points. warm = 0: the plain deterministic start;
- * 1: the network's start S·MLP(c); 2: the snapshot Rayleigh-Ritz start
+ * 1: the network's start S·MLP(c) (S·(y(c) + MLP(c)) under correction
+ * targets); 2: the snapshot Rayleigh-Ritz start
  * S·y, y the lowest eigenvector of Sᵀ H(c) S (eigenvector continuation over
  * the learned snapshots; the projected parts are formed once at training).
  * Answers energies, residuals, iterations (total, FP32-matvec and FP64),
@@ -240,8 +243,9 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * stream and more than 256 rows; otherwise the points are solved one by
  * one. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);
-/* Network outputs (n × snapshots, row-major) of n points; returns the
- * snapshot count, or -1 on failure. */
+/* Network outputs (n × snapshots, row-major) of n points, in coefficient
+ * units (under correction targets: the learned correction, without y(c));
+ * returns the snapshot count, or -1 on failure. */
 int qmbed_hip_scan_predict(qmbed_hip_scan* scan, const double* points, int n, double* coefficients);
 
 /* The FP32 network kernels themselves (host buffers; for checks against a

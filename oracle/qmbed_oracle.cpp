@@ -13,9 +13,7 @@
 // Each routine names the reference file:line it restates (paths relative to
 // the QMBED source tree).
 //
-// Not restated here ("parity unpinned" for these inputs): the thick-restart
-// route taken when an explicit krylov_dimension < 96 is requested
-// (src/solve/mod.rs:1644-2167) and shift-invert targets.
+// Not restated here: shift-invert targets.
 
 #include <algorithm>
 #include <cmath>
@@ -769,6 +767,232 @@ static Eigensystem dense_eigsh(const Model& m, const EigshOptions& o) {
   return es;
 }
 
+// ------------------------------------------------------- thick restart ----
+
+// orthonormalize_{real,complex}_with_projection: src/solve/mod.rs:1519-1586
+// (complex :1588-1641). Modified Gram-Schmidt against the locked vectors,
+// then the basis (recording the basis overlaps of the first pass), a DGKS
+// second pass when the norm fell to 1/sqrt(2) of its start, and a drop when
+// the remainder is not finite or at most 1e-14.
+static bool orthonormalize(const std::vector<std::vector<cplx>>& locked, const std::vector<std::vector<cplx>>& basis,
+                           std::vector<cplx>& v, Eigensystem& es, std::vector<cplx>* projection) {
+  const double DGKS = 0.7071067811865476;
+  const size_t n = v.size();
+  const double before = cnorm(v);
+  auto pass = [&](bool record) {
+    for (auto& l : locked) {
+      const cplx ov = cinner(l, v);
+      for (size_t i = 0; i < n; ++i) v[i] -= ov * l[i];
+    }
+    for (auto& b : basis) {
+      const cplx ov = cinner(b, v);
+      if (record && projection) projection->push_back(ov);
+      for (size_t i = 0; i < n; ++i) v[i] -= ov * b[i];
+    }
+  };
+  if (projection) projection->clear();
+  pass(true);
+  double nr = cnorm(v);
+  const bool second = before > EPS && nr <= DGKS * before;
+  if (second) {
+    pass(false);
+    nr = cnorm(v);
+  }
+  es.passes += 1 + (second ? 1 : 0);
+  es.second += second ? 1 : 0;
+  if (!std::isfinite(nr) || nr <= 1e-14) return false;
+  for (auto& z : v) z /= nr;
+  return true;
+}
+
+// thick_restarted_eigsh_{real,complex}: src/solve/mod.rs:1651-1913 (complex
+// :1914-2167), window thick_restart_dimension :1644-1649. The real route is
+// taken for real operators and real initial vectors (:2169-2196); here both
+// run in complex arithmetic, the real one on vectors with zero imaginary
+// parts and Ritz vectors made real.
+static Eigensystem thick_restart(const Model& m, const EigshOptions& o, const std::vector<std::vector<cplx>>& init) {
+  const long n = m.dim();
+  const long rd = std::min(o.krylov > 0 ? o.krylov : std::max<long>(4L * o.k + 24, 32), n);
+  if (rd <= o.k) throw OracleError("invalid options: the effective Krylov dimension must exceed eigenpairs");
+  bool real = m.real;
+  for (auto& v : init)
+    for (auto& z : v)
+      if (std::abs(z.imag()) > 1e-14) real = false;
+  struct Kept {
+    std::vector<cplx> vec, act;
+    bool has_act;
+  };
+  struct Cand {
+    double value;
+    std::vector<cplx> vec;
+    double residual;
+    std::vector<cplx> act;
+  };
+  std::vector<Kept> retained;
+  for (auto& v : init) {
+    Kept k{v, {}, false};
+    if (real)
+      for (auto& z : k.vec) z = z.real();
+    retained.push_back(std::move(k));
+  }
+  std::vector<Cand> locked;
+  Eigensystem es;
+  long iterations = 0;
+  double last_residual = INFINITY;
+  u64 cycle = 0;
+  auto locked_vectors = [&]() {
+    std::vector<std::vector<cplx>> lv;
+    for (auto& c : locked) lv.push_back(c.vec);
+    return lv;
+  };
+  while (iterations < o.max_iter && long(locked.size()) < o.k) {
+    const auto lv = locked_vectors();
+    std::vector<std::vector<cplx>> basis, applied;
+    bool reuse = !retained.empty();
+    for (auto& r : retained) reuse = reuse && r.has_act;
+    for (auto& r : retained) {
+      if (reuse) {
+        const double nr = cnorm(r.vec);
+        if (std::isfinite(nr) && nr > 1e-14) {
+          for (auto& z : r.vec) z /= nr;
+          for (auto& z : r.act) z /= nr;
+          basis.push_back(r.vec);
+          applied.push_back(r.act);
+        }
+      } else {
+        std::vector<cplx> v = r.vec;
+        if (orthonormalize(lv, basis, v, es, nullptr)) basis.push_back(v);
+      }
+      if (long(basis.size()) >= rd / 2) break;
+    }
+    retained.clear();
+    if (basis.empty()) {
+      auto d = deterministic_start(n, o.seed + cycle * 0x9e3779b9ULL);
+      std::vector<cplx> v(n);
+      for (long i = 0; i < n; ++i) v[i] = d[i];
+      if (orthonormalize(lv, basis, v, es, nullptr)) basis.push_back(v);
+    }
+    if (basis.empty()) break;
+    const size_t known = applied.size();
+    std::vector<std::vector<cplx>> columns;
+    std::vector<cplx> proj;
+    if (reuse) {
+      for (size_t c = 0; c < known; ++c) {
+        if (long(basis.size()) >= rd) break;
+        std::vector<cplx> v = applied[c];
+        const bool keep = orthonormalize(lv, basis, v, es, &proj);
+        columns.emplace_back(proj.begin(), proj.begin() + long(c) + 1);
+        if (keep) basis.push_back(v);
+      }
+    }
+    size_t cursor = known;
+    std::vector<cplx> out(n);
+    while (cursor < basis.size() && iterations < o.max_iter && long(applied.size()) < rd) {
+      apply(m, basis[cursor].data(), out.data());
+      ++iterations;
+      applied.push_back(out);
+      std::vector<cplx> v = out;
+      const bool keep = orthonormalize(lv, basis, v, es, &proj);
+      columns.emplace_back(proj.begin(), proj.begin() + long(cursor) + 1);
+      if (long(basis.size()) < rd && keep) basis.push_back(v);
+      ++cursor;
+    }
+    basis.resize(applied.size());
+    if (basis.empty()) break;
+    const int size = int(basis.size());
+    std::vector<cplx> P(size_t(size) * size, cplx(0, 0));
+    for (int c = 0; c < int(columns.size()) && c < size; ++c)
+      for (int r = 0; r <= c && r < int(columns[size_t(c)].size()); ++r) {
+        const cplx v = real ? cplx(columns[size_t(c)][size_t(r)].real(), 0) : columns[size_t(c)][size_t(r)];
+        P[size_t(r) * size + c] = v;
+        P[size_t(c) * size + r] = std::conj(v);
+      }
+    std::vector<double> vals;
+    std::vector<cplx> U;
+    hermitian_eig(size, P, vals, U);
+    const long active = o.k - long(locked.size());
+    const long keep_limit = std::max<long>(std::min<long>(active, rd / 2), 1);
+    const auto idx = select_indices(vals, o.target, int(std::min<long>(keep_limit, size)));
+    std::vector<Cand> cands;
+    for (int q : idx) {
+      cplx ph(1, 0);
+      if (real) {  // the eigenvector of a real symmetric matrix, made real
+        double best = 0;
+        for (int j = 0; j < size; ++j)
+          if (std::abs(U[size_t(j) * size + q]) > best) {
+            best = std::abs(U[size_t(j) * size + q]);
+            ph = U[size_t(j) * size + q] / best;
+          }
+      }
+      std::vector<cplx> vec(n, cplx(0, 0)), act(n, cplx(0, 0));
+      for (int j = 0; j < size; ++j) {
+        cplx c = U[size_t(j) * size + q] / ph;
+        if (real) c = c.real();
+        for (long i = 0; i < n; ++i) {
+          vec[i] += c * basis[size_t(j)][i];
+          act[i] += c * applied[size_t(j)][i];
+        }
+      }
+      const double rn = cnorm(vec);
+      if (!std::isfinite(rn) || rn <= EPS) throw NonConvergence{iterations, rn};
+      for (auto& z : vec) z /= rn;
+      for (auto& z : act) z /= rn;
+      const double value = cinner(vec, act).real();
+      double r2 = 0;
+      for (long i = 0; i < n; ++i) r2 += std::norm(act[i] - value * vec[i]);
+      cands.push_back({value, vec, std::sqrt(r2), act});
+    }
+    // sort_real_candidates (stable): src/solve/mod.rs:1484-1508
+    std::stable_sort(cands.begin(), cands.end(), [&](const Cand& a, const Cand& b) {
+      return target_less(o.target == BE ? SA : o.target, a.value, b.value);
+    });
+    last_residual = 0;
+    for (long r = 0; r < active && r < long(cands.size()); ++r) last_residual = std::max(last_residual, cands[size_t(r)].residual);
+    std::vector<Kept> next;
+    for (size_t r = 0; r < cands.size(); ++r) {
+      auto& c = cands[r];
+      const bool conv = c.residual <= o.tol * std::max(std::abs(c.value), 1.0);
+      if (long(r) < active && conv && long(locked.size()) < o.k) {
+        std::vector<cplx> v = c.vec;
+        if (orthonormalize(locked_vectors(), {}, v, es, nullptr)) locked.push_back({c.value, v, c.residual, {}});
+      } else if (long(next.size()) < keep_limit) {
+        next.push_back({c.vec, c.act, true});
+      }
+    }
+    retained = std::move(next);
+    ++cycle;
+  }
+  if (long(locked.size()) < o.k) throw NonConvergence{iterations, last_residual};
+  std::stable_sort(locked.begin(), locked.end(), [&](const Cand& a, const Cand& b) {
+    return target_less(o.target == BE ? SA : o.target, a.value, b.value);
+  });
+  locked.resize(size_t(o.k));
+  for (auto& c : locked) {
+    es.values.push_back(c.value);
+    es.vectors.push_back(c.vec);
+    es.residuals.push_back(c.residual);
+  }
+  es.iterations = iterations;
+  return es;
+}
+
+// the thick restart and its dense fallback for small full windows:
+// src/solve/mod.rs:2306-2327
+static Eigensystem thick_restart_route(const Model& m, const EigshOptions& o,
+                                       const std::vector<std::vector<cplx>>& init) {
+  const long n = m.dim();
+  try {
+    return thick_restart(m, o, init);
+  } catch (NonConvergence& nc) {
+    if (n <= 2048 && o.max_iter >= n && (o.krylov <= 0 || o.krylov == n)) {
+      auto r = dense_eigsh(m, o);
+      r.iterations += nc.iterations;
+      return r;
+    }
+    throw;
+  }
+}
+
 // eigsh + expanding_lanczos_eigsh dispatch: src/solve/mod.rs:2242-2350, 364-372, 425-448
 static Eigensystem eigsh(const Model& m, EigshOptions o, const std::vector<cplx>* initial) {
   long n = m.dim();
@@ -806,7 +1030,58 @@ static Eigensystem eigsh(const Model& m, EigshOptions o, const std::vector<cplx>
     }
   }
   if (o.krylov >= 96) return lanczos_eigsh(m, o, initial);
-  throw OracleError("unsupported backend: the oracle does not restate the thick-restart route");
+  std::vector<std::vector<cplx>> sub;
+  if (initial) sub.push_back(*initial);
+  return thick_restart_route(m, o, sub);
+}
+
+// eigsh_with_initial_subspace: src/solve/mod.rs:2385-2410 (validation done by
+// the caller); one or no vector runs the single-start dispatch above
+static Eigensystem eigsh_subspace(const Model& m, EigshOptions o, const std::vector<std::vector<cplx>>& sub) {
+  long n = m.dim();
+  if (sub.size() <= 1) return eigsh(m, o, sub.empty() ? nullptr : &sub[0]);
+  if (o.k <= 0 || o.k >= n)
+    throw OracleError("invalid options: eigenpairs must be positive and smaller than the operator dimension");
+  if (!std::isfinite(o.tol) || o.tol <= 0 || o.max_iter <= 0)
+    throw OracleError("invalid options: tolerance and max_iterations must be positive");
+  if (o.krylov > 0 && (o.krylov <= o.k || o.krylov > n))
+    throw OracleError("invalid options: krylov_dimension must exceed eigenpairs and not exceed dimension");
+  if (n <= 128 || (n <= 256 && o.krylov <= 0)) return dense_eigsh(m, o);
+  // expanding_lanczos_eigsh with a genuine multi-vector start: the
+  // unrestarted window takes only one start, so every multi-vector request
+  // goes to the thick restart (src/solve/mod.rs:2249-2304)
+  return thick_restart_route(m, o, sub);
+}
+
+// balanced_subspace_start: src/solve/mod.rs:708-727
+static std::vector<cplx> balanced_subspace_start(const std::vector<std::vector<cplx>>& sub) {
+  if (sub.empty()) throw OracleError("invalid options: initial subspace must not be empty");
+  const size_t n = sub[0].size();
+  std::vector<cplx> out(n, cplx(0, 0));
+  const double count = double(sub.size());
+  for (size_t j = 0; j < sub.size(); ++j) {
+    const cplx c = std::polar(1.0 / std::sqrt(count), 2.0 * M_PI * double(j) / count);
+    for (size_t i = 0; i < n; ++i) out[i] += c * sub[j][i];
+  }
+  double nr = cnorm(out);
+  if (!std::isfinite(nr) || nr <= EPS) {
+    out = sub[0];
+    nr = cnorm(out);
+    if (!std::isfinite(nr) || nr <= EPS) throw NonConvergence{0, nr};
+  }
+  for (auto& z : out) z /= nr;
+  return out;
+}
+
+// eigsh_with_workspace: src/solve/mod.rs:2419-2444 (the workspace holds the
+// previous solve's eigenvectors; its dimension matches the operator's)
+static Eigensystem eigsh_workspace(const Model& m, EigshOptions o, const std::vector<std::vector<cplx>>& ws) {
+  if (ws.empty()) return eigsh(m, o, nullptr);
+  if ((o.krylov <= 0 || o.krylov >= 96) && ws.size() > 1) {
+    auto start = balanced_subspace_start(ws);
+    return eigsh(m, o, &start);
+  }
+  return eigsh_subspace(m, o, ws);
 }
 
 // --------------------------------------------------------- evolution ------
@@ -1234,6 +1509,61 @@ int oracle_eigsh(void* h, int k, int target, long krylov, double tol, long max_i
     std::ostringstream os;
     os << "solver did not converge after " << nc.iterations << " iterations; residual=" << nc.residual;
     g_err = os.str();
+    return -2;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// eigsh from an initial subspace of nsub complex interleaved vectors
+// (eigsh_with_initial_subspace, mode 0) or a workspace holding them
+// (eigsh_with_workspace, mode 1); outputs as oracle_eigsh
+int oracle_eigsh_subspace(void* h, int k, int target, long krylov, double tol, long max_iter, unsigned long long seed,
+                          int nsub, const double* subspace, int mode, double* values, double* residuals,
+                          double* vectors, long* info) {
+  auto* m = static_cast<Model*>(h);
+  try {
+    EigshOptions o{k, target, krylov, tol, max_iter, seed};
+    std::vector<std::vector<cplx>> sub;
+    for (int j = 0; j < nsub; ++j) {
+      const cplx* v = reinterpret_cast<const cplx*>(subspace) + size_t(j) * m->dim();
+      sub.emplace_back(v, v + m->dim());
+    }
+    Eigensystem es = mode == 1 ? eigsh_workspace(*m, o, sub) : eigsh_subspace(*m, o, sub);
+    for (int i = 0; i < k; ++i) {
+      values[i] = es.values[i];
+      residuals[i] = es.residuals[i];
+      if (vectors) std::copy(es.vectors[i].begin(), es.vectors[i].end(), reinterpret_cast<cplx*>(vectors) + size_t(i) * m->dim());
+    }
+    info[0] = es.iterations;
+    info[1] = es.passes;
+    info[2] = es.second;
+    return 0;
+  } catch (NonConvergence& nc) {
+    std::ostringstream os;
+    os << "solver did not converge after " << nc.iterations << " iterations; residual=" << nc.residual;
+    g_err = os.str();
+    return -2;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// balanced_subspace_start of nsub complex interleaved vectors of length n
+int oracle_balanced_start(int nsub, long n, const double* subspace, double* out) {
+  try {
+    std::vector<std::vector<cplx>> sub;
+    for (int j = 0; j < nsub; ++j) {
+      const cplx* v = reinterpret_cast<const cplx*>(subspace) + size_t(j) * n;
+      sub.emplace_back(v, v + n);
+    }
+    auto r = balanced_subspace_start(sub);
+    std::copy(r.begin(), r.end(), reinterpret_cast<cplx*>(out));
+    return 0;
+  } catch (NonConvergence&) {
+    g_err = "initial subspace combines to zero";
     return -2;
   } catch (std::exception& e) {
     g_err = e.what();

@@ -1165,7 +1165,8 @@ qmbed_hip_scan* qmbed_hip_scan_create(qmbed_hip_model* model, const char* option
         if (x->kind != json::Value::String) bad("targets must be a string");
         if (x->raw == "projections") o.targets = ScanOptions::Targets::Projections;
         else if (x->raw == "continuation") o.targets = ScanOptions::Targets::Continuation;
-        else bad("targets must be \"projections\" or \"continuation\"");
+        else if (x->raw == "correction") o.targets = ScanOptions::Targets::Correction;
+        else bad("targets must be \"projections\", \"continuation\" or \"correction\"");
       }
       if (auto* x = v.get("provisional"); x && x->kind == json::Value::Bool) o.provisional = x->b;
       if (auto* x = v.get("space"); x && !x->is_null()) {

@@ -573,16 +573,38 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   }
   const int nt = int(pts.size() / size_t(P_));
   std::vector<float> X(size_t(nt) * P_), T(size_t(nt) * ns_);
+  std::vector<double> Td(size_t(nt) * ns_);
   for (int i = 0; i < nt; ++i) {
     for (int p = 0; p < P_; ++p)
       X[size_t(i) * P_ + p] = float((pts[size_t(i) * P_ + p] - center_[size_t(p)]) / scale_[size_t(p)]);
+    double* t = Td.data() + size_t(i) * ns_;
     if (opt_.targets == ScanOptions::Targets::Continuation) {
       const auto y = continuation(pts.data() + size_t(i) * P_);
-      for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(y[size_t(k)]);
+      for (int k = 0; k < ns_; ++k) t[k] = y[size_t(k)];
     } else {
-      for (int k = 0; k < ns_; ++k) T[size_t(i) * ns_ + k] = float(Lm[size_t(i) + size_t(n) * size_t(k)]);
+      for (int k = 0; k < ns_; ++k) t[k] = Lm[size_t(i) + size_t(n) * size_t(k)];
+      if (opt_.targets == ScanOptions::Targets::Correction) {
+        // the normalized projection of ψ_i (gauge as the continuation's:
+        // both have a positive overlap with the all-ones vector) minus the
+        // continuation coefficients at the same point
+        double nn = 0;
+        for (int k = 0; k < ns_; ++k) nn += t[k] * t[k];
+        const double inv = nn > 0 ? 1.0 / std::sqrt(nn) : 0.0;
+        const auto y = continuation(pts.data() + size_t(i) * P_);
+        for (int k = 0; k < ns_; ++k) t[k] = t[k] * inv - y[size_t(k)];
+      }
     }
   }
+  // correction targets are small: the network learns them at unit
+  // root-mean-square, and its outputs are scaled back
+  target_scale_ = 1.0;
+  if (opt_.targets == ScanOptions::Targets::Correction) {
+    double ss = 0;
+    for (double v : Td) ss += v * v;
+    const double rms = Td.empty() ? 0.0 : std::sqrt(ss / double(Td.size()));
+    if (rms > 0 && std::isfinite(rms)) target_scale_ = rms;
+  }
+  for (size_t e = 0; e < Td.size(); ++e) T[e] = float(Td[e] / target_scale_);
   const int np = mlp_parameter_count(P_, opt_.hidden1, opt_.hidden2, ns_);
   std::vector<float> init(static_cast<size_t>(np), 0.0f);
   {
@@ -650,7 +672,29 @@ std::vector<double> WarmStartScan::predict(const double* points, int n) {
   std::vector<float> Y(size_t(n) * ns_);
   QMBED_HIP(hipMemcpyAsync(Y.data(), dY.data(), Y.size() * 4, hipMemcpyDeviceToHost, s_));
   QMBED_HIP(hipStreamSynchronize(s_));
-  return std::vector<double>(Y.begin(), Y.end());
+  std::vector<double> out(Y.size());
+  for (size_t e = 0; e < Y.size(); ++e) out[e] = double(Y[e]) * target_scale_;
+  return out;
+}
+
+std::vector<std::vector<double>> WarmStartScan::network_coefficients(const double* points, int n) {
+  const auto y = predict(points, n);
+  std::vector<std::vector<double>> out(static_cast<size_t>(std::max(n, 0)));
+  const bool corr = opt_.targets == ScanOptions::Targets::Correction;
+  std::vector<std::future<std::vector<double>>> fy;
+  if (corr)
+    for (int b = 0; b < n; ++b) {
+      const double* p = points + size_t(b) * P_;
+      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
+    }
+  for (int b = 0; b < n; ++b) {
+    out[size_t(b)].assign(y.begin() + size_t(b) * ns_, y.begin() + size_t(b + 1) * ns_);
+    if (corr) {
+      const auto c = fy[size_t(b)].get();
+      for (int k = 0; k < ns_; ++k) out[size_t(b)][size_t(k)] += c[size_t(k)];
+    }
+  }
+  return out;
 }
 
 std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* points, int n, const EigshOptions& o,
@@ -709,11 +753,8 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
     // the previous batch ran on the device; the next batch's start here) or
     // the network (one forward pass for the batch); the plain start needs none
     std::vector<std::vector<double>> ys(static_cast<size_t>(nb));
-    if (start == Start::Network) {  // before the tables: predict waits for the stream
-      const auto y = predict(points + size_t(i0) * P_, nb);
-      for (int b = 0; b < nb; ++b)
-        ys[size_t(b)].assign(y.begin() + size_t(b) * ns_, y.begin() + size_t(b + 1) * ns_);
-    }
+    if (start == Start::Network)  // before the tables: predict waits for the stream
+      ys = network_coefficients(points + size_t(i0) * P_, nb);
     // the tables of every point of the batch (indexed by point), set up
     // while the continuation threads run; the batch tables and starts live
     // in grow-only members: no device allocation (and no synchronizing free)
@@ -804,8 +845,10 @@ WarmStartScan::PointResult WarmStartScan::solve(const double* point, const Eigsh
     if (v0_.size() != N) v0_.resize(N);
     if (dc_.size() != size_t(ns_)) dc_.resize(static_cast<size_t>(ns_));
     std::vector<double> c(static_cast<size_t>(ns_));
-    if (start == Start::SnapshotRR) {
-      c = y ? *y : continuation(point);
+    const bool host_coefficients =
+        start == Start::SnapshotRR || opt_.targets == ScanOptions::Targets::Correction;
+    if (host_coefficients) {
+      c = start == Start::SnapshotRR ? (y ? *y : continuation(point)) : network_coefficients(point, 1)[0];
       if (c.size() != size_t(ns_)) throw internal_state("continuation coefficients of the wrong length");
       // staged through pinned memory (the previous point's copy completed
       // before its solve returned)

@@ -19,7 +19,12 @@ struct ScanOptions {
   // family projected on the snapshots, Sᵀ H(c) S (eigenvector continuation),
   // at the training points and at `samples` seeded points of their bounding
   // box, so the network learns that map over the whole domain.
-  enum class Targets { Projections, Continuation };
+  // Correction: the step from the continuation coefficients to the training
+  // ground state's own normalized projection on the snapshots,
+  // Sᵀψ/‖Sᵀψ‖ − y(c), scaled to unit root-mean-square; the network's start
+  // is then S·(y(c) + MLP(c)), the continuation start moved toward the best
+  // start the snapshot space holds (the projection of the ground state).
+  enum class Targets { Projections, Continuation, Correction };
   Targets targets = Targets::Projections;
   int samples = 512;
   // training points after the first `snapshots` start from the projection
@@ -117,6 +122,10 @@ class WarmStartScan {
   DeviceArray<float> lk_lanes32_;
   DeviceArray<float> params_;  // network parameters
   std::vector<double> center_, scale_;
+  double target_scale_ = 1.0;  // network outputs × target_scale_ = coefficients (or corrections)
+  // the coefficients of the network's start at n points (host): the
+  // network's outputs, plus the continuation under correction targets
+  std::vector<std::vector<double>> network_coefficients(const double* points, int n);
 };
 
 }  // namespace qmbed

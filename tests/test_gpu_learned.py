@@ -121,8 +121,49 @@ def test_learned_warm_start_scan(qa, oracle):
             assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
             assert r["residuals"][i] <= 1e-10 * abs(e0)
     ri = np.mean(rr["iterations"])
-    print(f"[learned] snapshot Rayleigh-Ritz start: {ri:.1f} iterations", flush=True)
-    assert ri < wi
+    print(f"[learned] snapshot Rayleigh-Ritz start: {ri:.1f} iterations (network on projection targets {wi:.1f})",
+          flush=True)
+    assert ri < pi
+
+
+@pytest.mark.parametrize("lockstep", [0, 16])
+def test_learned_correction_start(qa, oracle, lockstep):
+    # the network learns the step from the snapshot Rayleigh-Ritz
+    # coefficients y(c) to the normalized projection of the ground state on
+    # the snapshots; its start S·(y(c) + MLP(c)) must need no more Lanczos
+    # iterations than S·y(c), and fewer than the plain start, at the same
+    # energies (oracle, dense)
+    hx = np.linspace(0.2, 2.0, 8)
+    hz = np.linspace(0.0, 0.5, 8)
+    grid = np.array([(a, b) for a in hx for b in hz])
+    perm = np.random.default_rng(0).permutation(len(grid))
+    train, held = grid[perm[:32]], grid[perm[32:48]]
+    m = _ising_scan_model(qa)
+    scan = qa.Scan(m, snapshots=8, epochs=3000, learning_rate=3e-3, seed=0, targets="correction",
+                   space="principal")
+    rep = scan.train(train)
+    assert rep["loss_last"] < 0.2 * rep["loss_first"]
+    kw = {"lockstep": lockstep}
+    net = scan.solve(held, warm=True, **kw)
+    rr = scan.solve(held, warm="rayleigh_ritz", **kw)
+    plain = scan.solve(held, warm=False, **kw)
+    net32 = scan.solve(held, warm=True, matvec_precision="fp32", **kw)
+    assert net["warm_starts"] == len(held) and net["fallbacks"] == 0
+    for i, (a, b) in enumerate(held):
+        e0 = np.linalg.eigvalsh(oracle.Model(L, cases.ising(L, a, b), momentum=0, parity=1,
+                                             norm="pauli_cartesian").dense())[0]
+        for r in (net, rr, plain, net32):
+            assert abs(r["energies"][i] - e0) <= 1e-10 * abs(e0)
+            assert r["residuals"][i] <= 1e-10 * max(abs(e0), 1.0)
+    ni, ri, pi = (float(np.mean(r["iterations"])) for r in (net, rr, plain))
+    print(f"[learned correction] mean iterations: network {ni:.2f}, snapshot {ri:.2f}, plain {pi:.2f}, "
+          f"network+fp32 {np.mean(net32['iterations']):.2f}", flush=True)
+    assert ni <= ri
+    assert ni < pi
+    # the network's outputs are corrections: small against the unit-norm
+    # coefficient vectors they move
+    y = scan.predict(held)
+    assert np.all(np.linalg.norm(y, axis=1) < 0.5)
 
 
 @pytest.mark.parametrize("n,p,q", [(1, 1, 1), (1000, 3, 5), (70000, 32, 32), (100001, 33, 65), (5000, 64, 1),
