@@ -157,6 +157,32 @@ char* qmbed_hip_launch_timer_report(void);
  * below) and "reorthogonalization_eta" (DGKS threshold, (0, 1]). */
 char* qmbed_hip_eigsh_json(const qmbed_hip_model* model, const char* solver_json);
 
+/* The solver JSON of qmbed_hip_eigsh_json also accepts "initial_subspace":
+ * [[[re, im], ...], ...] (several start vectors; not with initial_vector),
+ * solved as eigsh_with_initial_subspace (src/solve/mod.rs:2385-2410): one
+ * vector takes the single-start route, several the thick restart, which
+ * keeps them as a subspace. Explicit krylov_dimension below 96 takes the
+ * thick restart too (src/solve/mod.rs:1644-2196, 2299-2327). */
+
+/* EigshWorkspace (src/solve/mod.rs:477-526): the converged eigenvectors of
+ * the previous solve of a related operator, host-resident. */
+typedef struct qmbed_hip_workspace qmbed_hip_workspace;
+qmbed_hip_workspace* qmbed_hip_workspace_create(void);
+void qmbed_hip_workspace_free(qmbed_hip_workspace* workspace);
+long qmbed_hip_workspace_dimension(const qmbed_hip_workspace* workspace);
+int qmbed_hip_workspace_vectors(const qmbed_hip_workspace* workspace);
+/* set_initial_subspace: `count` complex vectors of `dimension` entries
+ * (interleaved re, im), each validated as an eigsh initial vector. */
+char* qmbed_hip_workspace_set(qmbed_hip_workspace* workspace, long dimension, int count, const double* vectors);
+/* eigsh_with_workspace (src/solve/mod.rs:2419-2444): with a workspace of
+ * the model's dimension holding more than one vector, the large-window
+ * route (no krylov_dimension, or one of at least 96) starts from
+ * balanced_subspace_start (:708-727), every other case from the whole
+ * subspace; the workspace then holds this solve's eigenvectors. Response as
+ * qmbed_hip_eigsh_json. */
+char* qmbed_hip_eigsh_workspace_json(const qmbed_hip_model* model, qmbed_hip_workspace* workspace,
+                                     const char* solver_json);
+
 /* Solve `count` models (typically the symmetry sectors of one Hamiltonian)
  * with one SolverRequest in a lockstep batch: every Lanczos launch covers all
  * unconverged sectors. Response: {"kind":"eigensystems","results":[one

@@ -65,6 +65,9 @@ def lib():
         L.oracle_expm.argtypes = [c_void_p, c_double, c_double, c_int, c_double, c_long, c_int, c_void_p,
                                   c_void_p, c_void_p]
         L.oracle_hermitian_eigvals.argtypes = [c_int, c_void_p, c_void_p]
+        L.oracle_eigsh_subspace.argtypes = [c_void_p, c_int, c_int, c_long, c_double, c_long, ctypes.c_ulonglong,
+                                            c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+        L.oracle_balanced_start.argtypes = [c_int, c_long, c_void_p, c_void_p]
         _lib = L
     return _lib
 
@@ -163,6 +166,24 @@ class Model:
         return {"eigenvalues": vals, "residuals": res, "eigenvectors": vecs, "iterations": int(info[0]),
                 "reorthogonalization_passes": int(info[1]), "conditional_second_passes": int(info[2])}
 
+    def eigsh_subspace(self, k, subspace, target="smallest_algebraic", krylov_dimension=None, tolerance=1e-10,
+                       max_iterations=1000, seed=0, workspace=False, vectors=False):
+        """eigsh_with_initial_subspace (workspace=False) or eigsh_with_workspace
+        on a workspace holding `subspace` (workspace=True)."""
+        sub = np.ascontiguousarray(np.atleast_2d(subspace), dtype=np.complex128) if len(subspace) else \
+            np.zeros((0, self.dim), dtype=np.complex128)
+        vals = np.zeros(k)
+        res = np.zeros(k)
+        vecs = np.zeros((k, self.dim), dtype=np.complex128) if vectors else None
+        info = np.zeros(3, dtype=np.int64)
+        rc = lib().oracle_eigsh_subspace(self._h, k, TARGETS[target], krylov_dimension or 0, tolerance,
+                                         max_iterations, seed, sub.shape[0], _ptr(sub), 1 if workspace else 0,
+                                         _ptr(vals), _ptr(res), _ptr(vecs), _ptr(info))
+        if rc != 0:
+            raise OracleError(lib().oracle_last_error().decode())
+        return {"eigenvalues": vals, "residuals": res, "eigenvectors": vecs, "iterations": int(info[0]),
+                "reorthogonalization_passes": int(info[1]), "conditional_second_passes": int(info[2])}
+
     def evolve(self, psi, times, krylov_dimension=30, tolerance=1e-10, max_substeps=10000):
         psi = np.ascontiguousarray(psi, dtype=np.complex128)
         times = np.ascontiguousarray(times, dtype=np.float64)
@@ -192,6 +213,15 @@ class Model:
 def deterministic_start(n, seed=0):
     out = np.zeros(n)
     lib().oracle_deterministic_start(n, seed, _ptr(out))
+    return out
+
+
+def balanced_start(subspace):
+    sub = np.ascontiguousarray(np.atleast_2d(subspace), dtype=np.complex128)
+    out = np.zeros(sub.shape[1], dtype=np.complex128)
+    rc = lib().oracle_balanced_start(sub.shape[0], sub.shape[1], _ptr(sub), _ptr(out))
+    if rc != 0:
+        raise OracleError(lib().oracle_last_error().decode())
     return out
 
 

@@ -94,6 +94,17 @@ def library():
     L.qmbed_hip_gemm_atb.restype = c_int
     L.qmbed_hip_gemm_atb.argtypes = [c_void_p, ctypes.c_size_t, c_int, c_void_p, c_int, c_void_p]
     L.qmbed_hip_device_count.restype = c_int
+    L.qmbed_hip_workspace_create.restype = c_void_p
+    L.qmbed_hip_workspace_create.argtypes = []
+    L.qmbed_hip_workspace_free.argtypes = [c_void_p]
+    L.qmbed_hip_workspace_dimension.restype = c_long
+    L.qmbed_hip_workspace_dimension.argtypes = [c_void_p]
+    L.qmbed_hip_workspace_vectors.restype = c_int
+    L.qmbed_hip_workspace_vectors.argtypes = [c_void_p]
+    L.qmbed_hip_workspace_set.restype = c_void_p
+    L.qmbed_hip_workspace_set.argtypes = [c_void_p, c_long, c_int, c_void_p]
+    L.qmbed_hip_eigsh_workspace_json.restype = c_void_p
+    L.qmbed_hip_eigsh_workspace_json.argtypes = [c_void_p, c_void_p, c_char_p]
     L.qmbed_hip_time_scan_batch.restype = ctypes.c_double
     L.qmbed_hip_time_scan_batch.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
     L.qmbed_hip_launch_timer.restype = c_int
@@ -367,6 +378,15 @@ class Model:
         self.is_real = bool(response["result"]["real"])
         return [complex(a, b) for a, b in response["result"]["values"]]
 
+    def eigsh_workspace(self, workspace, k, target="smallest_algebraic", parameters=None, **solver):
+        """eigsh_with_workspace: started from `workspace` (a Workspace), which
+        then holds this solve's eigenvectors."""
+        req = {"eigenpairs": k, "target": {"kind": target}, **solver}
+        if parameters is not None:
+            req["parameters"] = {n: [complex(v).real, complex(v).imag] for n, v in parameters.items()}
+        return _envelope(library().qmbed_hip_eigsh_workspace_json(self._h, workspace.handle,
+                                                                  json.dumps(req).encode()))
+
     def eigsh(self, k, target="smallest_algebraic", parameters=None, **solver):
         req = {"eigenpairs": k, "target": {"kind": target}, **solver}
         if parameters is not None:
@@ -384,6 +404,41 @@ def launch_timer_report():
     """{kernel instance: {"launches", "ms_total", "ms_mean"}} of the launches
     recorded since the timer was switched on."""
     return _envelope(library().qmbed_hip_launch_timer_report())
+
+
+class Workspace:
+    """EigshWorkspace (QMBED src/solve/mod.rs:477-526): the eigenvectors of
+    the previous workspace solve, reused by the next (Model.eigsh_workspace)."""
+
+    def __init__(self):
+        self._h = library().qmbed_hip_workspace_create()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def dimension(self):
+        return int(library().qmbed_hip_workspace_dimension(self._h))
+
+    def __len__(self):
+        return int(library().qmbed_hip_workspace_vectors(self._h))
+
+    def set(self, dimension, vectors):
+        """set_initial_subspace: complex vectors (count × dimension)."""
+        v = np.ascontiguousarray(np.array(vectors, dtype=np.complex128).reshape(-1, dimension))
+        return _envelope(library().qmbed_hip_workspace_set(self._h, int(dimension), v.shape[0], v.ctypes.data))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().qmbed_hip_workspace_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def eigsh_batch(models, k, target="smallest_algebraic", **solver):

@@ -204,6 +204,10 @@ struct SolverReq {
   bool eigenvectors = false;
   std::vector<cplx> initial;
   bool has_initial = false;
+  // extension: a whole initial subspace (eigsh_with_initial_subspace,
+  // src/solve/mod.rs:2385-2410)
+  std::vector<std::vector<cplx>> subspace;
+  bool has_subspace = false;
   EigshControls ctl;
 };
 
@@ -234,6 +238,12 @@ SolverReq parse_solver(const Value& s) {
   if (iv && !iv->is_null()) {
     r.initial = as_cvec(*iv, "initial_vector");
     r.has_initial = true;
+  }
+  if (auto* v = s.get("initial_subspace"); v && !v->is_null()) {
+    if (r.has_initial) bad("initial_vector and initial_subspace cannot be combined");
+    if (v->kind != Value::Arr) bad("initial_subspace must be an array of vectors");
+    for (auto& e : *v->arr) r.subspace.push_back(as_cvec(e, "initial_subspace"));
+    r.has_subspace = true;
   }
   // engine controls (extensions; absent in reference requests)
   if (auto* v = s.get("matvec_precision"); v && v->kind == Value::String) {
@@ -442,6 +452,7 @@ Eigensystem solve_model(Model& m, const SolverReq& r, const Value* parameters = 
   const Operator& op = m.bind(parameters);
   if (r.has_initial && r.initial.size() != op.dim())
     throw dimension_mismatch("eigsh initial vector does not match the operator");
+  if (r.has_subspace) return eigsh_subspace(op, r.opts, r.subspace, r.ctl, r.eigenvectors, m.stream);
   return eigsh(op, r.opts, r.has_initial ? r.initial.data() : nullptr, r.ctl, r.eigenvectors, m.stream);
 }
 
@@ -459,7 +470,7 @@ std::string counters_body(size_t dim, const Eigensystem& es, bool vectors) {
 // its current evaluation; the batch runs on the first model's stream.
 std::string solve_models(const std::vector<std::shared_ptr<Model>>& models, const SolverReq& r) {
   if (models.empty()) bad("the batch names no models");
-  if (r.has_initial) bad("the sector batch starts every sector from the deterministic start vector");
+  if (r.has_initial || r.has_subspace) bad("the sector batch starts every sector from the deterministic start vector");
   std::vector<Model*> order;
   for (auto& m : models) order.push_back(m.get());
   std::sort(order.begin(), order.end());
@@ -1056,6 +1067,49 @@ char* qmbed_hip_eigsh_json(const qmbed_hip_model* m, const char* solver_json) {
     }
     SolverReq r = parse_solver(v);
     Eigensystem es = solve_model(*tm->m, r, v.get("parameters"));
+    return ok(counters_body(tm->m->basis->len(), es, r.eigenvectors));
+  });
+}
+
+qmbed_hip_workspace* qmbed_hip_workspace_create(void) {
+  return reinterpret_cast<qmbed_hip_workspace*>(new EigshWorkspace());
+}
+
+void qmbed_hip_workspace_free(qmbed_hip_workspace* w) { delete reinterpret_cast<EigshWorkspace*>(w); }
+
+long qmbed_hip_workspace_dimension(const qmbed_hip_workspace* w) {
+  return long(reinterpret_cast<const EigshWorkspace*>(w)->dimension);
+}
+
+int qmbed_hip_workspace_vectors(const qmbed_hip_workspace* w) {
+  return int(reinterpret_cast<const EigshWorkspace*>(w)->subspace.size());
+}
+
+char* qmbed_hip_workspace_set(qmbed_hip_workspace* w, long dimension, int count, const double* vectors) {
+  return guarded("{}", [&](const std::string&) {
+    if (dimension < 0 || count < 0 || (count > 0 && !vectors)) bad("workspace vectors must be given");
+    std::vector<std::vector<cplx>> vs(static_cast<size_t>(count));
+    for (int j = 0; j < count; ++j) {
+      const double* v = vectors + size_t(j) * 2 * size_t(dimension);
+      vs[size_t(j)].resize(size_t(dimension));
+      for (long i = 0; i < dimension; ++i) vs[size_t(j)][size_t(i)] = cplx(v[2 * i], v[2 * i + 1]);
+    }
+    reinterpret_cast<EigshWorkspace*>(w)->set(size_t(dimension), std::move(vs));
+    return ok("{\"kind\":\"workspace\",\"dimension\":" + std::to_string(dimension) +
+              ",\"vectors\":" + std::to_string(count) + "}");
+  });
+}
+
+char* qmbed_hip_eigsh_workspace_json(const qmbed_hip_model* m, qmbed_hip_workspace* w, const char* solver_json) {
+  return guarded(solver_json, [&](const std::string& req) {
+    auto* tm = reinterpret_cast<const TypedModel*>(m);
+    auto& ws = *reinterpret_cast<EigshWorkspace*>(w);
+    json::Value v = parse_or_throw(req);
+    SolverReq r = parse_solver(v);
+    if (r.has_initial || r.has_subspace) bad("a workspace solve takes its start from the workspace");
+    std::lock_guard<std::mutex> g(tm->m->mu);
+    const Operator& op = tm->m->bind(v.get("parameters"));
+    Eigensystem es = eigsh_workspace(op, r.opts, ws, r.ctl, tm->m->stream);
     return ok(counters_body(tm->m->basis->len(), es, r.eigenvectors));
   });
 }

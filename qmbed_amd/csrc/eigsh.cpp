@@ -1058,6 +1058,12 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
 
 }  // namespace
 
+// the dense route, for the thick restart's fallback (thick_restart.cpp)
+Eigensystem dense_eigsh_route(const Operator& op, const EigshOptions& o, DeviceArray<double>& vec_out,
+                              bool& complex_out, hipStream_t s) {
+  return dense_eigsh(op, o, vec_out, complex_out, s);
+}
+
 std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& o, const EigshControls& ctl,
                                            const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
                                            DeviceArray<float>& lanes32, hipStream_t s) {
@@ -1225,6 +1231,13 @@ void balanced_start(const DeviceArray<double>& vecs, int k, size_t n, bool c, De
   QMBED_HIP(hipStreamSynchronize(s));
 }
 
+// the smallest explicit window that runs the unrestarted tridiagonal
+// Lanczos (TRIDIAGONAL_LANCZOS_WINDOW, src/solve/mod.rs)
+constexpr long kTridiagonalWindow = 96;
+
+Eigensystem thick_restart_eigsh(const Operator& op, const EigshOptions& o, const std::vector<const double*>& init,
+                                bool c, DeviceArray<double>& vec_out, bool& complex_vectors, hipStream_t s);
+
 Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* d_initial, bool initial_complex,
                        const EigshControls& ctl, DeviceArray<double>& vec_out, bool& complex_vectors, hipStream_t s) {
   const size_t n = op.dim();
@@ -1234,6 +1247,13 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
   const bool dense = n <= 128 || (n <= 256 && !o.krylov_dimension);
   if (dense) return dense_eigsh(op, o, vec_out, complex_vectors, s);
   const bool c = !op.is_real() || initial_complex;
+  // explicit windows below the tridiagonal cut-off run the thick restart
+  // (src/solve/mod.rs:2299-2327)
+  if (o.krylov_dimension && *o.krylov_dimension < kTridiagonalWindow) {
+    std::vector<const double*> init;
+    if (d_initial) init.push_back(d_initial);
+    return thick_restart_eigsh(op, o, init, c, vec_out, complex_vectors, s);
+  }
   complex_vectors = c;
   const size_t e = c ? 2 : 1;
   // start vector (pooled scratch: no allocation per solve)
@@ -1288,11 +1308,11 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
           balanced_start(w2.vectors, o.eigenpairs, n, c, polish_start, s);
           d_start = polish_start.data();
           use_fp32 = false;
-          if (!fixed) {
-            const long remaining = budget > spent ? budget - spent : 0;
-            sub = std::min(sub, remaining);
-            if (sub <= o.eigenpairs) throw non_convergence(spent, err.residual);
-          }
+          // the FP64 repeat stays within the iteration budget, for a fixed
+          // window as for an expanding one
+          const long remaining = budget > spent ? budget - spent : 0;
+          sub = std::min(sub, remaining);
+          if (sub <= o.eigenpairs) throw non_convergence(spent, err.residual);
           continue;
         }
         if (fixed) throw;
@@ -1359,6 +1379,117 @@ Eigensystem eigsh(const Operator& op, const EigshOptions& o, const cplx* initial
       es.eigenvectors.push_back(std::move(v));
     }
   }
+  return es;
+}
+
+void validate_eigsh_initial(const std::vector<cplx>& v, size_t dimension) {
+  if (v.size() != dimension) throw dimension_mismatch("eigsh initial vector does not match the operator");
+  double nn = 0;
+  for (const cplx& z : v) {
+    if (!std::isfinite(z.real()) || !std::isfinite(z.imag()))
+      throw invalid_options("eigsh initial vector must contain only finite values");
+    nn += std::norm(z);
+  }
+  if (!std::isfinite(std::sqrt(nn)) || std::sqrt(nn) <= F64_EPS)
+    throw invalid_options("eigsh initial vector must have nonzero finite norm");
+}
+
+std::vector<cplx> balanced_subspace_start(const std::vector<std::vector<cplx>>& sub) {
+  if (sub.empty()) throw invalid_options("initial subspace must not be empty");
+  const size_t n = sub[0].size();
+  const double count = double(sub.size());
+  std::vector<cplx> out(n, cplx(0, 0));
+  for (size_t j = 0; j < sub.size(); ++j) {
+    const cplx coef = std::polar(1.0 / std::sqrt(count), 2.0 * M_PI * double(j) / count);
+    for (size_t i = 0; i < n; ++i) out[i] += coef * sub[j][i];
+  }
+  auto normalize = [](std::vector<cplx>& v) {
+    double nn = 0;
+    for (const cplx& z : v) nn += std::norm(z);
+    const double nr = std::sqrt(nn);
+    if (!std::isfinite(nr) || nr <= F64_EPS) return false;
+    for (cplx& z : v) z /= nr;
+    return true;
+  };
+  if (!normalize(out)) {
+    out = sub[0];
+    if (!normalize(out)) throw non_convergence(0, 0.0);
+  }
+  return out;
+}
+
+Eigensystem eigsh_subspace(const Operator& op, const EigshOptions& o, const std::vector<std::vector<cplx>>& sub,
+                           const EigshControls& ctl, bool want_vectors, hipStream_t s) {
+  const size_t n = op.dim();
+  o.validate(n);
+  if (sub.empty()) return eigsh(op, o, nullptr, ctl, want_vectors, s);
+  for (const auto& v : sub) validate_eigsh_initial(v, n);
+  if (n <= 128 || (n <= 256 && !o.krylov_dimension)) return eigsh(op, o, nullptr, ctl, want_vectors, s);
+  if (sub.size() == 1) return eigsh(op, o, sub[0].data(), ctl, want_vectors, s);
+  if (o.target == SpectrumTarget::Shift)
+    throw unsupported_backend("shift-invert targets are not implemented on the MI355X Krylov path");
+  // several vectors: the thick restart keeps them as a subspace
+  bool c = !op.is_real();
+  for (const auto& v : sub)
+    for (const cplx& z : v)
+      if (std::fabs(z.imag()) > 1.0e-14) c = true;
+  const size_t e = c ? 2 : 1;
+  DeviceArray<double> d(e * n * sub.size());
+  std::vector<double> h(e * n);
+  std::vector<const double*> init;
+  for (size_t j = 0; j < sub.size(); ++j) {
+    for (size_t i = 0; i < n; ++i) {
+      if (c) {
+        h[2 * i] = sub[j][i].real();
+        h[2 * i + 1] = sub[j][i].imag();
+      } else {
+        h[i] = sub[j][i].real();
+      }
+    }
+    QMBED_HIP(hipMemcpy(d.data() + j * e * n, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    init.push_back(d.data() + j * e * n);
+  }
+  DeviceArray<double> vecs;
+  bool cv = false;
+  Eigensystem es = thick_restart_eigsh(op, o, init, c, vecs, cv, s);
+  if (want_vectors) {
+    const size_t ev = cv ? 2 : 1;
+    std::vector<double> hv(ev * n * es.eigenvalues.size());
+    QMBED_HIP(hipMemcpy(hv.data(), vecs.data(), hv.size() * 8, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < es.eigenvalues.size(); ++q) {
+      std::vector<cplx> v(n);
+      for (size_t i = 0; i < n; ++i)
+        v[i] = cv ? cplx(hv[ev * (q * n + i)], hv[ev * (q * n + i) + 1]) : cplx(hv[q * n + i], 0);
+      es.eigenvectors.push_back(std::move(v));
+    }
+  }
+  return es;
+}
+
+void EigshWorkspace::set(size_t dim, std::vector<std::vector<cplx>> vectors) {
+  for (const auto& v : vectors) validate_eigsh_initial(v, dim);
+  dimension = dim;
+  subspace = std::move(vectors);
+}
+
+Eigensystem eigsh_workspace(const Operator& op, const EigshOptions& o, EigshWorkspace& ws, const EigshControls& ctl,
+                            hipStream_t s) {
+  const size_t n = op.dim();
+  Eigensystem es;
+  if (ws.dimension == n && !ws.subspace.empty()) {
+    if ((!o.krylov_dimension || *o.krylov_dimension >= kTridiagonalWindow) && ws.subspace.size() > 1) {
+      const auto start = balanced_subspace_start(ws.subspace);
+      o.validate(n);
+      validate_eigsh_initial(start, n);
+      es = eigsh(op, o, start.data(), ctl, true, s);
+    } else {
+      es = eigsh_subspace(op, o, ws.subspace, ctl, true, s);
+    }
+  } else {
+    es = eigsh(op, o, nullptr, ctl, true, s);
+  }
+  ws.dimension = n;
+  ws.subspace = es.eigenvectors;
   return es;
 }
 

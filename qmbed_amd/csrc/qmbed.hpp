@@ -498,6 +498,38 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
 Eigensystem eigsh(const Operator& op, const EigshOptions& opts, const cplx* initial = nullptr,
                   const EigshControls& ctl = EigshControls(), bool want_vectors = false, hipStream_t s = nullptr);
 
+// eigsh_with_initial_subspace: QMBED src/solve/mod.rs:2385-2410. Every
+// vector is validated (length, finite entries, nonzero norm); no vector runs
+// eigsh, one runs eigsh from it, several run the thick restart with the whole
+// subspace kept (small problems take the dense route either way).
+Eigensystem eigsh_subspace(const Operator& op, const EigshOptions& opts, const std::vector<std::vector<cplx>>& sub,
+                           const EigshControls& ctl, bool want_vectors, hipStream_t s);
+
+// EigshWorkspace: QMBED src/solve/mod.rs:477-526, the converged invariant
+// subspace of the previous solve of a related operator (host vectors).
+struct EigshWorkspace {
+  size_t dimension = 0;
+  std::vector<std::vector<cplx>> subspace;
+  void clear() {
+    dimension = 0;
+    subspace.clear();
+  }
+  // set_initial_subspace: every vector validated against `dim`
+  void set(size_t dim, std::vector<std::vector<cplx>> vectors);
+};
+// eigsh_with_workspace: QMBED src/solve/mod.rs:2419-2444. A workspace of the
+// operator's dimension with more than one vector starts the large-window
+// route (no krylov_dimension, or at least 96) from balanced_subspace_start
+// (:708-727) and every other case from the whole subspace; the workspace
+// then holds this solve's eigenvectors.
+Eigensystem eigsh_workspace(const Operator& op, const EigshOptions& opts, EigshWorkspace& ws,
+                            const EigshControls& ctl, hipStream_t s);
+// balanced_subspace_start: Σ_j e^{2πij/J} v_j / √J normalized (v_0 when the
+// sum vanishes)
+std::vector<cplx> balanced_subspace_start(const std::vector<std::vector<cplx>>& sub);
+// validate_eigsh_initial: QMBED src/solve/mod.rs:671-692
+void validate_eigsh_initial(const std::vector<cplx>& v, size_t dimension);
+
 // Many sectors at once (sector_batch.hip): the Krylov-route Hermitian
 // sectors run one fully reorthogonalized Lanczos window each, in lockstep, so
 // every launch of a step covers all of them; sectors that do not converge
@@ -597,6 +629,12 @@ class ExpmActionPlan {
 // when want_vectors.
 void tridiagonal_eigen(const std::vector<double>& alpha, const std::vector<double>& beta, std::vector<double>& values,
                        std::vector<double>* vecs);
+
+// Host dense Hermitian eigen-decomposition (Householder tridiagonalization,
+// implicit QL, back-transformation) of a row-major n×n matrix: values
+// ascending, vectors row-major with column j the unit eigenvector of
+// values[j].
+void hermitian_eigen(int n, std::vector<cplx> a, std::vector<double>& values, std::vector<cplx>& vectors);
 
 // Index selection per spectral target (QMBED src/solve/mod.rs:1002-1036).
 std::vector<int> select_indices(const std::vector<double>& values, SpectrumTarget target, int count, double shift = 0.0);

@@ -227,4 +227,100 @@ std::vector<double> tridiagonal_eigvec(const std::vector<double>& alpha, const s
   return x;
 }
 
+// Dense Hermitian eigen-decomposition for the projected matrices of the
+// thick-restarted Lanczos (orders up to a few hundred): Householder
+// reduction to tridiagonal form, a diagonal phase that makes the
+// off-diagonal real and non-negative, the implicit QL iteration above, and
+// back-transformation. a: row-major n×n (only its Hermitian part is used);
+// values ascending; vectors row-major n×n with column j the unit eigenvector
+// of values[j].
+void hermitian_eigen(int n, std::vector<cplx> a, std::vector<double>& values, std::vector<cplx>& vectors) {
+  auto at = [&](int i, int j) -> cplx& { return a[size_t(i) * n + j]; };
+  for (int i = 0; i < n; ++i)  // symmetrize: the Hermitian part of a
+    for (int j = i; j < n; ++j) {
+      const cplx h = 0.5 * (at(i, j) + std::conj(at(j, i)));
+      at(i, j) = h;
+      at(j, i) = std::conj(h);
+    }
+  std::vector<cplx> q(size_t(n) * n, cplx(0, 0));
+  for (int i = 0; i < n; ++i) q[size_t(i) * n + i] = 1.0;
+  std::vector<cplx> u(n), p(n), qu(n);
+  for (int k = 0; k + 2 < n; ++k) {
+    // reflector H = I - 2 u u^H (u on rows k+1..n-1) with H x = alpha e_{k+1}
+    const int m0 = k + 1;
+    double sigma2 = 0;
+    for (int i = m0; i < n; ++i) sigma2 += std::norm(at(i, k));
+    const double sigma = std::sqrt(sigma2);
+    double below = 0;
+    for (int i = m0 + 1; i < n; ++i) below += std::norm(at(i, k));
+    if (below == 0.0) continue;  // already tridiagonal in this column
+    const cplx x0 = at(m0, k);
+    const double ax0 = std::abs(x0);
+    const cplx ph = ax0 > 0 ? x0 / ax0 : cplx(1, 0);
+    const cplx alpha = -ph * sigma;
+    for (int i = m0; i < n; ++i) u[i] = at(i, k);
+    u[m0] -= alpha;
+    double un = 0;
+    for (int i = m0; i < n; ++i) un += std::norm(u[i]);
+    un = std::sqrt(un);
+    for (int i = m0; i < n; ++i) u[i] /= un;
+    // trailing block: A <- A - 2 u q^H - 2 q u^H, p = A u, q = p - (u^H p) u
+    for (int i = m0; i < n; ++i) {
+      cplx acc(0, 0);
+      for (int j = m0; j < n; ++j) acc += at(i, j) * u[j];
+      p[i] = acc;
+    }
+    cplx mu(0, 0);
+    for (int i = m0; i < n; ++i) mu += std::conj(u[i]) * p[i];
+    const double mur = mu.real();
+    for (int i = m0; i < n; ++i) p[i] -= mur * u[i];
+    for (int i = m0; i < n; ++i)
+      for (int j = m0; j < n; ++j) at(i, j) -= 2.0 * (u[i] * std::conj(p[j]) + p[i] * std::conj(u[j]));
+    at(m0, k) = alpha;
+    at(k, m0) = std::conj(alpha);
+    for (int i = m0 + 1; i < n; ++i) {
+      at(i, k) = 0;
+      at(k, i) = 0;
+    }
+    // Q <- Q H on columns m0..n-1
+    for (int r = 0; r < n; ++r) {
+      cplx acc(0, 0);
+      for (int j = m0; j < n; ++j) acc += q[size_t(r) * n + j] * u[j];
+      qu[r] = acc;
+    }
+    for (int r = 0; r < n; ++r)
+      for (int j = m0; j < n; ++j) q[size_t(r) * n + j] -= 2.0 * qu[r] * std::conj(u[j]);
+  }
+  // T = Q^H A Q: real diagonal, complex subdiagonal t_i = T[i+1][i]; with
+  // D_0 = 1, D_{i+1} = D_i t_i / |t_i|, D^H T D is real symmetric with
+  // off-diagonal |t_i|
+  std::vector<double> d(n), e(std::max(n - 1, 0));
+  std::vector<cplx> ph(n, cplx(1, 0));
+  for (int i = 0; i < n; ++i) d[i] = at(i, i).real();
+  for (int i = 0; i + 1 < n; ++i) {
+    const cplx t = at(i + 1, i);
+    const double at_ = std::abs(t);
+    e[i] = at_;
+    ph[i + 1] = at_ > 0 ? ph[i] * (t / at_) : ph[i];
+  }
+  std::vector<double> z(size_t(n) * n, 0.0);
+  for (int i = 0; i < n; ++i) z[size_t(i) * n + i] = 1.0;
+  tql(n, d, e, z.data());
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d[x] < d[y]; });
+  values.resize(n);
+  vectors.assign(size_t(n) * n, cplx(0, 0));
+  for (int jj = 0; jj < n; ++jj) {
+    const int j = order[jj];
+    values[jj] = d[j];
+    // eigenvector Q D z_j
+    for (int r = 0; r < n; ++r) {
+      cplx acc(0, 0);
+      for (int i = 0; i < n; ++i) acc += q[size_t(r) * n + i] * ph[i] * z[size_t(i) * n + j];
+      vectors[size_t(r) * n + jj] = acc;
+    }
+  }
+}
+
 }  // namespace qmbed

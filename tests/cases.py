@@ -69,3 +69,22 @@ def seeded_state(n, seed):
     rng = np.random.default_rng(seed)
     g = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     return g / np.linalg.norm(g)
+
+
+def diagonal_chain(L, scale=1.0, offset=None):
+    """H = Σ_i scale·2^i z_i + offset (angular-momentum z = ±1/2, no
+    symmetry): on the 2^L computational states, state s has energy
+    scale·(s − (2^L − 1)/2) + offset, so the sector matrix is the diagonal
+    operator of the reference's solver tests (value index·scale with
+    offset = scale·(2^L − 1)/2, the default)."""
+    if offset is None:
+        offset = scale * (2 ** L - 1) / 2
+    return [("z", [(scale * 2.0 ** i, [i]) for i in range(L)]), ("I", [(offset, [0])])]
+
+
+def isolated_low_chain(L):
+    """Diagonal H = z_0 + 2 z_1 + Σ_{i≥2} (200 + 10 i) z_i + offset: the
+    values 0, 1, 2, 3 and every other value at least 100 above them, the
+    shape of the reference's isolated-low-values solver test."""
+    w = [1.0, 2.0] + [200.0 + 10.0 * i for i in range(2, L)]
+    return [("z", [(c, [i]) for i, c in enumerate(w)]), ("I", [(sum(w) / 2, [0])])]

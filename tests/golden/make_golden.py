@@ -61,9 +61,15 @@ def fingerprint(model, nforms=4, complex_=False):
 CASES = {
     "c1": dict(L=16, spec=lambda: cases.heisenberg(16), up=8, k=0, p=None, norm="angular", eig=4),
     "c2": dict(L=24, spec=lambda: cases.xxz(24, 1.5), up=12, k=0, p=None, norm="angular", eig=6),
-    "c2k5": dict(L=24, spec=lambda: cases.xxz(24, 1.0), up=11, k=5, p=None, norm="angular", eig=0),
+    # two of the 72 XXZ L=24 sectors of the batch: up=11 k=5 (complex) and up=12 k=0
+    "c2k5": dict(L=24, spec=lambda: cases.xxz(24, 1.0), up=11, k=5, p=None, norm="angular", eig=6),
+    "c4k0": dict(L=24, spec=lambda: cases.xxz(24, 1.0), up=12, k=0, p=None, norm="angular", eig=6),
     "c3": dict(L=28, spec=lambda: cases.j1j2(28), up=14, k=0, p=None, norm="angular", eig=0),
-    "c5": dict(L=26, spec=lambda: cases.ising(26, 0.7, 0.2), up=None, k=0, p=1, norm="pauli_cartesian", eig=0),
+    # three points of the L=26 Ising scan family: an interior point, the
+    # headline's H·v point and one next to the critical line (hx = 1, hz -> 0)
+    "c5": dict(L=26, spec=lambda: cases.ising(26, 0.7, 0.2), up=None, k=0, p=1, norm="pauli_cartesian", eig=1),
+    "c5a": dict(L=26, spec=lambda: cases.ising(26, 1.0, 0.25), up=None, k=0, p=1, norm="pauli_cartesian", eig=1),
+    "c5b": dict(L=26, spec=lambda: cases.ising(26, 1.0, 0.02), up=None, k=0, p=1, norm="pauli_cartesian", eig=1),
 }
 
 
@@ -75,9 +81,12 @@ def make(name):
            "normalization": c["norm"], "terms": c["spec"](), "built_s": time.time() - t0}
     out.update(fingerprint(m, complex_=not m.is_real))
     if c["eig"]:
+        t1 = time.time()
         r = m.eigsh(c["eig"])
         out["eigenvalues"] = [float(v) for v in r["eigenvalues"]]
+        out["residuals"] = [float(v) for v in r["residuals"]]
         out["eigsh_iterations"] = r["iterations"]
+        out["eigsh_s"] = time.time() - t1
     with open(os.path.join(HERE, f"{name}.json"), "w") as f:
         json.dump(out, f, indent=1, default=lambda z: [z.real, z.imag] if isinstance(z, complex) else z)
     print(name, "done", time.time() - t0, "s", flush=True)

@@ -15,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ["c1", "c2", "c2k5", "c3", "c5"]
+CASES = ["c1", "c2", "c2k5", "c4k0", "c3", "c5", "c5a", "c5b"]
 
 
 def load(name):
@@ -72,7 +72,7 @@ def test_fingerprint(qa, monkeypatch, name, mode):
     m.close()
 
 
-@pytest.mark.parametrize("name", ["c1", "c2"])
+@pytest.mark.parametrize("name", ["c1", "c2", "c2k5", "c4k0", "c5", "c5a", "c5b"])
 def test_eigenvalues(qa, name):
     d = load(name)
     m = qa.Model(basis_of(d), terms_of(d))
@@ -92,3 +92,23 @@ def test_majumdar_ghosh_ground_state(qa):
         r = m.eigsh(1, matvec_precision=prec)
         assert abs(r["eigenvalues"][0] + 3 * 28 / 8) <= 1e-10 * 10.5
         assert r["residuals"][0] <= 1e-10 * 10.5
+
+
+def test_batch_sectors_against_oracle_eigenvalues(qa):
+    # two of the 72 XXZ L=24 sectors of the batch, pinned by the oracle's own
+    # eigsh (tests/golden/c2k5.json: up=11 k=5, complex; c4k0.json: up=12
+    # k=0): the six lowest eigenvalues from the full 72-sector lockstep batch
+    L = 24
+    keys = [(up, k) for up in (11, 12, 13) for k in range(L)]
+    spec = terms_of(load("c4k0"))
+    models = [qa.Model({"kind": "spin", "sites": L, "up": up, "momentum": k}, spec) for up, k in keys]
+    out = qa.eigsh_batch(models, 6)
+    for name, key in (("c2k5", (11, 5)), ("c4k0", (12, 0))):
+        d = load(name)
+        assert terms_of(d) == spec
+        r = out["results"][keys.index(key)]
+        np.testing.assert_allclose(r["eigenvalues"], d["eigenvalues"], rtol=1e-10, atol=0, err_msg=name)
+        for lam, res in zip(r["eigenvalues"], r["residuals"]):
+            assert res <= 1e-10 * max(abs(lam), 1.0)
+    for m in models:
+        m.close()

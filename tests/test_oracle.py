@@ -347,3 +347,123 @@ def test_oracle_nilpotent_taylor(oracle):
     m = oracle.Model(1, [("+", [(1.0, [0])])], check_hermiticity=False, check_particles=False)
     out, info = m.expm(1.0, [np.array([1.0, 0.0])])
     np.testing.assert_allclose(out[0], [1.0, 1.0], atol=1e-15)
+
+
+# ------------------------------------------------- thick restart / subspace --
+# thick_restarted_eigsh (src/solve/mod.rs:1644-2196), eigsh_with_initial_subspace
+# (:2385-2410), eigsh_with_workspace (:2419-2444), balanced_subspace_start
+# (:708-727): pinned by dense LAPACK spectra, the reference's own control-flow
+# anchors (tests/solver_extensions.rs:100-154, tests/visible_contract.rs:1041-1059)
+# and the closed form of the balanced start.
+
+TR_CASES = [
+    # (L, spec, up, momentum, k, krylov, target)
+    (16, cases.heisenberg(16), 8, 0, 1, 12, "smallest_algebraic"),
+    (16, cases.heisenberg(16), 8, 0, 4, 24, "smallest_algebraic"),
+    (16, cases.xxz(16, 1.5), 8, 1, 3, 20, "smallest_algebraic"),  # complex sector
+    (16, cases.j1j2(16), 8, 0, 2, 40, "largest_algebraic"),
+    (14, cases.heisenberg(14), 7, 2, 2, 16, "largest_magnitude"),
+    (11, cases.dm_chain(11), None, 3, 2, 30, "both_ends"),
+]
+
+
+@pytest.mark.parametrize("case", range(len(TR_CASES)))
+def test_oracle_thick_restart_matches_dense(oracle, case):
+    L, spec, up, k_mom, k, kd, target = TR_CASES[case]
+    m = oracle.Model(L, spec, up=up, momentum=k_mom)
+    dense = np.linalg.eigvalsh(m.dense())
+    r = m.eigsh(k, target=target, krylov_dimension=kd, tolerance=1e-10, vectors=True)
+    if target == "smallest_algebraic":
+        ref = dense[:k]
+    elif target == "largest_algebraic":
+        ref = dense[::-1][:k]
+    elif target == "largest_magnitude":
+        ref = dense[np.argsort(-np.abs(dense), kind="stable")][:k]
+    else:
+        lo, hi = k // 2, k - k // 2
+        ref = np.sort(np.concatenate([dense[:lo], dense[len(dense) - hi:]]))
+    np.testing.assert_allclose(r["eigenvalues"], ref, rtol=1e-10, atol=1e-12)
+    H = m.dense()
+    for v, lam, res in zip(r["eigenvectors"], r["eigenvalues"], r["residuals"]):
+        assert abs(np.linalg.norm(v) - 1) < 1e-12
+        true_res = np.linalg.norm(H @ v - lam * v)
+        assert true_res <= 1e-10 * max(abs(lam), 1) * 1.01 + 1e-14
+        assert abs(true_res - res) <= 1e-9 * max(abs(lam), 1)
+    assert r["iterations"] > kd  # the window restarted
+    assert r["iterations"] <= 1000
+
+
+def test_oracle_thick_restart_diagonal_kats(oracle):
+    # tests/solver_extensions.rs:100-123 on a 512-dimensional diagonal with
+    # values i/511 (the reference's has 300 values i/299): a first solve
+    # restarts, a workspace solve from its converged subspace stays within one
+    # window, with the same eigenvalues
+    m = oracle.Model(9, cases.diagonal_chain(9, 1.0 / 511))
+    np.testing.assert_allclose(np.sort(np.diag(m.dense()).real), np.arange(512) / 511, atol=1e-15)
+    first = m.eigsh_subspace(3, [], krylov_dimension=16, tolerance=1e-10, seed=19, workspace=True, vectors=True)
+    assert first["iterations"] > 16
+    np.testing.assert_allclose(first["eigenvalues"], [0, 1 / 511, 2 / 511], atol=1e-10)
+    second = m.eigsh_subspace(3, first["eigenvectors"], krylov_dimension=16, tolerance=1e-10, seed=19,
+                              workspace=True)
+    assert second["iterations"] <= 16
+    np.testing.assert_allclose(second["eigenvalues"], first["eigenvalues"], atol=1e-14)
+    assert np.all(second["residuals"] <= 1e-10)
+    # a small window that cannot converge answers NonConvergence
+    # (tests/solver_extensions.rs:73-87: krylov 2, max_iterations 2)
+    with pytest.raises(oracle.OracleError, match="did not converge"):
+        m.eigsh(1, krylov_dimension=2, tolerance=1e-12, max_iterations=2, seed=7)
+
+
+def test_oracle_full_window_iteration_count(oracle):
+    # tests/visible_contract.rs:1041-1059: diagonal 0..255, three lowest,
+    # window 160 (>= 96: one unrestarted window), 160 iterations
+    m = oracle.Model(8, cases.diagonal_chain(8))
+    r = m.eigsh(3, krylov_dimension=160, tolerance=1e-8, max_iterations=192, seed=17)
+    np.testing.assert_allclose(r["eigenvalues"], [0, 1, 2], atol=1e-7)
+    assert np.all(r["residuals"] < 1e-7)
+    assert r["iterations"] == 160
+
+
+def test_oracle_large_workspace_window_keeps_tridiagonal_path(oracle):
+    # tests/solver_extensions.rs:126-154 shape: three isolated low values, a
+    # window of 100: both solves take the unrestarted route (the second from
+    # the balanced start of the first's three eigenvectors)
+    L = 9
+    m = oracle.Model(L, cases.isolated_low_chain(L))
+    vals = np.diag(m.dense()).real
+    np.testing.assert_allclose(np.sort(vals)[:5], [0, 1, 2, 3, 220], atol=1e-12)  # four low values, the rest far above
+    first = m.eigsh_subspace(3, [], krylov_dimension=100, tolerance=1e-10, max_iterations=100, seed=23,
+                             workspace=True, vectors=True)
+    second = m.eigsh_subspace(3, first["eigenvectors"], krylov_dimension=100, tolerance=1e-10, max_iterations=100,
+                              seed=23, workspace=True)
+    np.testing.assert_allclose(second["eigenvalues"], np.sort(vals)[:3], atol=1e-10)
+    np.testing.assert_allclose(first["eigenvalues"], second["eigenvalues"], atol=1e-12)
+
+
+def test_oracle_balanced_start_closed_form(oracle):
+    rng = np.random.default_rng(5)
+    for J in (1, 2, 3, 5):
+        sub = rng.standard_normal((J, 40)) + 1j * rng.standard_normal((J, 40))
+        ref = sum(np.exp(2j * np.pi * j / J) / np.sqrt(J) * sub[j] for j in range(J))
+        ref /= np.linalg.norm(ref)
+        np.testing.assert_allclose(oracle.balanced_start(sub), ref, atol=1e-14)
+    # the phases cancel: the sum of a unit vector and itself times e^{iπ}
+    # is below f64::EPSILON, and the first vector is used
+    v = rng.standard_normal(40)
+    v /= np.linalg.norm(v)
+    out = oracle.balanced_start(np.array([v, v]))
+    np.testing.assert_allclose(out, v / np.linalg.norm(v), atol=1e-14)
+
+
+def test_oracle_initial_subspace_route(oracle):
+    # several start vectors take the thick restart and keep them: starting
+    # from exact eigenvectors, the first window converges
+    L = 16
+    m = oracle.Model(L, cases.heisenberg(L), up=8, momentum=0)
+    w, V = np.linalg.eigh(m.dense())
+    r = m.eigsh_subspace(2, V[:, :3].T, tolerance=1e-10)
+    np.testing.assert_allclose(r["eigenvalues"], w[:2], rtol=1e-12)
+    assert r["iterations"] <= 2 * (4 * 2 + 24)
+    # one vector is eigsh_with_initial (the expanding window)
+    r1 = m.eigsh_subspace(2, V[:, :1].T, tolerance=1e-10)
+    np.testing.assert_allclose(r1["eigenvalues"], w[:2], rtol=1e-10)
