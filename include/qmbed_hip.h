@@ -260,14 +260,16 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * in lockstep batches of up to "lockstep" (solver key, 0-16, default 16; 0
  * or 1: one by one): one Lanczos window per point; per step, shared H·v
  * launches over the interleaved vectors of the windows still stepping (each
- * point with its own couplings; the windows grouped by their own product
- * precision, up to 16 FP32 or 8 FP64 vectors per launch), and every other
- * phase of the step in one launch for all of them. Each window runs the
- * single-window algorithm, so results equal one-by-one solves bit for bit;
- * a point whose window fails is solved on its own. Needs a real
- * family with Hermitian parts, no momentum phases, the compact compiled
- * stream and more than 256 rows; otherwise the points are solved one by
- * one. */
+ * point with its own couplings and each window at its own product
+ * precision: up to 16 FP32 and 8 FP64 vectors in one launch), and every
+ * other phase of the step in one launch for all of them. Each window runs
+ * the single-window algorithm, so results equal one-by-one solves bit for
+ * bit; a point whose window fails is solved on its own. The solver key
+ * "refill" (bool, default false) hands a finished window's slot to the next
+ * point at once instead of running the points in batches that start and
+ * end together. Needs a real family with Hermitian parts, no momentum
+ * phases, the compact compiled stream, more than 256 rows and no explicit
+ * window below 96; otherwise the points are solved one by one. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);
 /* Network outputs (n × snapshots, row-major) of n points, in coefficient
  * units (under correction targets: the learned correction, without y(c));

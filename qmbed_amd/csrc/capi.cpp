@@ -252,6 +252,7 @@ SolverReq parse_solver(const Value& s) {
     else bad("matvec_precision must be \"fp64\" or \"fp32\"");
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
+  if (auto* v = s.get("refill"); v && !v->is_null()) r.ctl.lockstep_refill = as_bool(v, false);
   if (auto* v = s.get("batch_rows"); v && !v->is_null()) {
     const long c = as_long(*v, "batch_rows");
     if (c < 0) bad("batch_rows must not be negative");

@@ -371,6 +371,21 @@ class LanczosWindow {
     QMBED_HIP(hipMemcpyAsync(col(0), d_v0, esz() * n_ * 8, hipMemcpyDeviceToDevice, s_));
     start_prepared();
   }
+  // Make the window ready for a new start (lockstep slots refilled with the
+  // next point): every per-solve counter and host record cleared; the
+  // buffers stay. The window must have finished (no read-back, test or
+  // step of the old start pending on the host side); its device work ahead
+  // in the stream completes before anything of the new start.
+  void reset_for_new_start() {
+    launched_ = posted_ = size_ = 0;
+    post_gen_ = 0;
+    ritz_nc_ = ritz_nv_ = 0;
+    passes_ = seconds_ = matvecs_ = matvecs64_ = matvecs32_ = 0;
+    last_beta_ = 0;
+    broke_ = false;
+    alpha_.clear();
+    beta_.clear();
+  }
   // the caller has written the start into col(0) (and cleared the flags)
   void start_prepared() {
     if (external_ && (il_.x64 || il_.x32)) lk::scatter_lane(col(0), n_, il_, s_);
@@ -792,7 +807,7 @@ class WindowRun {
             const EigshControls& ctl, double accept_tol, WindowResult& out, hipStream_t s,
             const double* start_norm2 = nullptr, const lk::Interleave* il = nullptr,
             std::function<void(const double*, double*)> apply64 = {})
-      : op_(op), c_(c), o_(o), m_(m), ctl_(ctl), accept_tol_(accept_tol), out_(out), s_(s),
+      : op_(op), c_(c), o_(o), m_(m), ctl_(ctl), accept_tol_(accept_tol), out_ptr_(&out), s_(s),
         start_norm2_(start_norm2), win_(op, c, m, fp32, s, ctl.dgks_eta, d_start != nullptr) {
     if (il) win_.set_external(*il, std::move(apply64));
     // d_start == nullptr: the caller writes the start into the window's
@@ -812,14 +827,28 @@ class WindowRun {
     next_sync_ = interval_;
   }
   bool done() const { return done_; }
+  // A finished run takes a new start (the caller has written it into the
+  // window's first column and cleared its flags); results go to `out` from
+  // here on. Same checks, schedule and arithmetic as a fresh run.
+  void restart(WindowResult& out) {
+    out_ptr_ = &out;
+    win_.reset_for_new_start();
+    win_.start_prepared();
+    next_check_ = std::max(k_ + 1, ctl_.check_every);
+    interval_ = std::max(1, ctl_.check_every);
+    next_sync_ = interval_;
+    pending_ = pending_conv_ = done_ = false;
+    verifying_ = verify_at_end_ = false;
+    start_norm2_ = nullptr;
+  }
   // the Ritz vectors of the window's test, in candidate order, into the
   // result (the window's own buffer is pooled)
   void copy_vectors(const std::vector<Candidate>& cands) {
     const size_t e = c_ ? 2 : 1;
     const size_t n = op_.dim();
-    out_.vectors.resize(e * n * cands.size());
+    out().vectors.resize(e * n * cands.size());
     for (size_t q = 0; q < cands.size(); ++q)
-      QMBED_HIP(hipMemcpyAsync(out_.vectors.data() + e * n * q, win_.ritz_vectors() + e * n * cands[q].column,
+      QMBED_HIP(hipMemcpyAsync(out().vectors.data() + e * n * q, win_.ritz_vectors() + e * n * cands[q].column,
                                e * n * 8, hipMemcpyDeviceToDevice, s_));
     QMBED_HIP(hipStreamSynchronize(s_));
   }
@@ -868,8 +897,8 @@ class WindowRun {
       const double nn = *start_norm2_;
       start_norm2_ = nullptr;
       if (!std::isfinite(nn) || std::sqrt(nn) <= F64_EPS) {
-        out_.bad_start = true;
-        out_.start_norm = std::sqrt(nn);
+        out().bad_start = true;
+        out().start_norm = std::sqrt(nn);
         done_ = true;
         return;
       }
@@ -898,7 +927,7 @@ class WindowRun {
       }
     }
     if (sz <= k_) {
-      out_.iterations = win_.matvecs();
+      out().iterations = win_.matvecs();
       throw non_convergence(sz, INFINITY);
     }
     if (async_tests_) {
@@ -930,17 +959,17 @@ class WindowRun {
       // them, so the window ends here and the caller restarts in FP64 from
       // these Ritz vectors (accurate to about the FP32 floor).
       if (win_.matvecs32() > 0) {
-        out_.iterations = win_.matvecs();
-        out_.fp32_matvecs = win_.matvecs32();
+        out().iterations = win_.matvecs();
+        out().fp32_matvecs = win_.matvecs32();
         copy_vectors(cands);
         throw non_convergence(sz, fail);
       }
       next_check_ = sz + ctl_.check_every;
       return;
     }
-    out_.iterations = win_.matvecs();
+    out().iterations = win_.matvecs();
     if (fail > 0) {
-      out_.fp32_matvecs = win_.matvecs32();
+      out().fp32_matvecs = win_.matvecs32();
       copy_vectors(cands);
       Error e = non_convergence(sz, fail);
       throw e;
@@ -953,16 +982,16 @@ class WindowRun {
       copy_vectors(cands);
     }
     for (auto& cd : cands) {
-      out_.es.eigenvalues.push_back(cd.value);
-      out_.es.residuals.push_back(cd.residual);
+      out().es.eigenvalues.push_back(cd.value);
+      out().es.residuals.push_back(cd.residual);
     }
-    out_.es.iterations = sz;
-    out_.es.reorthogonalization_passes = win_.passes();
-    out_.es.conditional_second_passes = win_.seconds();
-    out_.es.converged = true;
-    out_.es.fp32_iterations = win_.matvecs32();
-    out_.es.fp64_iterations = win_.matvecs() - win_.matvecs32() + win_.matvecs64();
-    out_.ok = true;
+    out().es.iterations = sz;
+    out().es.reorthogonalization_passes = win_.passes();
+    out().es.conditional_second_passes = win_.seconds();
+    out().es.converged = true;
+    out().es.fp32_iterations = win_.matvecs32();
+    out().es.fp64_iterations = win_.matvecs() - win_.matvecs32() + win_.matvecs64();
+    out().ok = true;
     done_ = true;
   }
 
@@ -973,10 +1002,11 @@ class WindowRun {
   int m_;
   const EigshControls& ctl_;
   double accept_tol_;
-  WindowResult& out_;
+  WindowResult* out_ptr_;
   hipStream_t s_;
   const double* start_norm2_;
   LanczosWindow win_;
+  WindowResult& out() { return *out_ptr_; }
   int k_ = 1, next_check_ = 0, lag_ = 2, interval_ = 1, next_sync_ = 1;
   bool pending_ = false, pending_conv_ = false, done_ = false;
   bool async_tests_ = false, verifying_ = false, verify_at_end_ = false;
@@ -1064,13 +1094,12 @@ Eigensystem dense_eigsh_route(const Operator& op, const EigshOptions& o, DeviceA
   return dense_eigsh(op, o, vec_out, complex_out, s);
 }
 
-std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& o, const EigshControls& ctl,
-                                           const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
-                                           DeviceArray<float>& lanes32, hipStream_t s) {
-  const int nb = int(pts.size());
+void eigsh_lockstep_stream(const Operator& op, const EigshOptions& o, const EigshControls& ctl, int slots,
+                           const LockstepFeed& feed, DeviceArray<double>& lanes64, DeviceArray<float>& lanes32,
+                           hipStream_t s) {
   const size_t n = op.dim();
   o.validate(n);
-  if (nb < 1 || nb > 16) throw internal_state("a lockstep batch holds 1 to 16 windows");
+  if (slots < 1 || slots > 16) throw internal_state("a lockstep batch holds 1 to 16 windows");
   if (n <= 256 || o.target == SpectrumTarget::Shift) throw internal_state("lockstep windows need the Krylov route");
   const long m = o.krylov_dimension
                      ? std::min<long>(std::min<long>(*o.krylov_dimension, o.max_iterations), long(n))
@@ -1078,87 +1107,131 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
                                       o.max_iterations);
   if (m <= o.eigenpairs) throw invalid_options("the effective Krylov dimension must exceed eigenpairs");
   const bool fp32 = ctl.precision == MatvecPrecision::FP32;
-  // lanes of all 16 windows
   const int max_width = op.scan_batch_max_width();
   if (lanes64.size() < 16 * n) lanes64.resize(16 * n);
   if (lanes32.size() < 16 * n) lanes32.resize(16 * n);
-  std::vector<WindowResult> res(static_cast<size_t>(nb));
-  std::vector<std::unique_ptr<WindowRun>> runs;
+  // per slot: the point it holds (index, tables, start), its window and result
+  std::vector<LockstepPoint> pts(static_cast<size_t>(slots));
+  std::vector<int> point(static_cast<size_t>(slots), -1);
+  std::vector<long> began(static_cast<size_t>(slots), 0);
+  std::vector<WindowResult> res(static_cast<size_t>(slots));
+  std::vector<std::unique_ptr<WindowRun>> runs(static_cast<size_t>(slots));
+  std::vector<char> failed(static_cast<size_t>(slots), 0);
   const lk::Interleave no_lanes;
-  for (int b = 0; b < nb; ++b) {
-    const LockstepPoint pt = pts[size_t(b)];
-    auto apply64 = [&op, pt, s](const double* x, double* y) {
-      const double* am[1] = {pt.amps};
-      const double* dg[1] = {pt.diag};
-      double* ys[1] = {y};
-      op.apply_scan_batch(x, false, 1, am, dg, ys, nullptr, s);
-    };
-    runs.emplace_back(new WindowRun(op, false, o, int(m), nullptr, fp32, ctl, o.tolerance, res[size_t(b)], s, nullptr,
-                                    &no_lanes, apply64));
-  }
-  // the starts, normalized into the windows' first columns, and the windows'
-  // flag areas cleared, for all windows at once
-  {
+  PostBatch pb(s);
+  VerifyBatch vb(op, pts, lanes64, s);
+  // the starts of the slots (re)filled in one tick, normalized into their
+  // windows' first columns and their flag areas cleared in three launches
+  auto start_slots = [&](const std::vector<int>& fill) {
     std::vector<lk::StartJob> sj;
-    for (int b = 0; b < nb; ++b) {
+    for (int b : fill) {
       auto& w = runs[size_t(b)]->window();
       sj.push_back({pts[size_t(b)].start, w.col(0), w.flag_area(), int(w.flag_doubles())});
     }
-    lk::start_multi(sj.data(), nb, n, runs[0]->window().workspace(), s);
-  }
-  // the windows' read-backs of a tick go in one copy launch
-  PostBatch pb(s);
-  VerifyBatch vb(op, pts, lanes64, s);
-  for (int b = 0; b < nb; ++b) {
+    if (!sj.empty()) lk::start_multi(sj.data(), int(sj.size()), n, runs[size_t(fill[0])]->window().workspace(), s);
+  };
+  auto wire = [&](int b) {
     auto& r = runs[size_t(b)];
     r->window().set_post_batch(&pb);
     r->set_async_tests(true);
     r->set_verify_batch(&vb, b);
-    if (pts[size_t(b)].vector) r->set_vector_destination(pts[size_t(b)].vector);
+    r->set_vector_destination(pts[size_t(b)].vector);
+  };
+  {
+    std::vector<int> fill;
+    for (int b = 0; b < slots; ++b) {
+      point[size_t(b)] = feed.next(b, pts[size_t(b)]);
+      if (point[size_t(b)] < 0) break;
+      const LockstepPoint* pt = &pts[size_t(b)];
+      auto apply64 = [&op, pt, s](const double* x, double* y) {
+        const double* am[1] = {pt->amps};
+        const double* dg[1] = {pt->diag};
+        double* ys[1] = {y};
+        op.apply_scan_batch(x, false, 1, am, dg, ys, nullptr, s);
+      };
+      runs[size_t(b)].reset(new WindowRun(op, false, o, int(m), nullptr, fp32, ctl, o.tolerance, res[size_t(b)], s,
+                                          nullptr, &no_lanes, apply64));
+      wire(b);
+      fill.push_back(b);
+    }
+    start_slots(fill);
   }
-  std::vector<char> failed(static_cast<size_t>(nb), 0);
   // every tick of a window launches a step or collects a read-back, so a
   // window ends within a few ticks per step of its window size
   const long max_ticks = 8 * (m + 8);
   for (long tick = 0;; ++tick) {
-    if (tick > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
     // the windows stepping now, by the precision of their products (each
-    // window's own FP32 switch, as in a single solve), packed into
-    // consecutive lanes: launches of up to 8 lanes, or one of 16 for 13 or
-    // more FP32 windows (a 16-lane FP32 launch costs 0.88 of two 8-lane ones;
-    // in FP64 it costs more than two, at half the occupancy)
+    // window's own FP32 switch, as in a single solve): their next vectors
+    // interleaved per precision, and one launch over the index stream for
+    // the FP32 lanes (up to 16) and up to 8 FP64 lanes together (an FP64
+    // launch wider than 8 runs at half the occupancy; further FP64 lanes get
+    // a launch of their own)
     bool active = false;
-    for (int prec = 0; prec < 2; ++prec) {
-      const bool use32 = prec == 1;
+    struct Lane {
+      const double* src;
+      const double* amps;
+      const double* diag;
+      double* y;
+      const int* gate;
+    };
+    std::vector<Lane> l32, l64;
+    for (int b = 0; b < slots; ++b) {
+      if (point[size_t(b)] < 0) continue;
+      auto& r = *runs[size_t(b)];
+      if (failed[size_t(b)] || r.done()) continue;
+      active = true;
+      if (tick - began[size_t(b)] > max_ticks) throw internal_state("lockstep Lanczos windows did not finish");
+      if (!r.stepping()) continue;
+      auto& w = r.window();
+      const bool use32 = w.fp32_active();
+      (use32 ? l32 : l64).push_back({w.col(w.launched()), pts[size_t(b)].amps, pts[size_t(b)].diag, w.w(), w.gate()});
+      w.count_external(use32);
+    }
+    auto pow2 = [](int k) { return k <= 0 ? 0 : k <= 1 ? 1 : k <= 2 ? 2 : k <= 4 ? 4 : k <= 8 ? 8 : 16; };
+    auto launch = [&](const std::vector<Lane>& a32, size_t o32, int k32, const std::vector<Lane>& a64, size_t o64,
+                      int k64, size_t lane64_base) {
+      const int w32 = pow2(k32), w64 = pow2(k64);
       const double* src[16] = {};
-      const double* amps[16] = {};
-      const double* diags[16] = {};
-      double* ys[16] = {};
-      const int* gates[16] = {};
-      int k = 0;
-      for (int b = 0; b < nb; ++b) {
-        auto& r = *runs[size_t(b)];
-        if (failed[size_t(b)] || r.done()) continue;
-        active = true;
-        if (!r.stepping() || r.window().fp32_active() != use32) continue;
-        auto& w = r.window();
-        src[k] = w.col(w.launched());
-        amps[k] = pts[size_t(b)].amps;
-        diags[k] = pts[size_t(b)].diag;
-        ys[k] = w.w();
-        gates[k] = w.gate();
-        w.count_external(use32);
-        ++k;
+      const double* amps[24] = {};
+      const double* diags[24] = {};
+      double* ys[24] = {};
+      const int* gates[24] = {};
+      for (int q = 0; q < k32; ++q) {
+        const Lane& L = a32[o32 + size_t(q)];
+        src[q] = L.src;
+        amps[q] = L.amps;
+        diags[q] = L.diag;
+        ys[q] = L.y;
+        gates[q] = L.gate;
       }
-      const int widest = use32 && k >= 13 ? max_width : 8;
-      for (int g0 = 0; g0 < k; g0 += widest) {
-        const int kg = std::min(widest, k - g0);
-        const int width = kg <= 1 ? 1 : kg <= 2 ? 2 : kg <= 4 ? 4 : kg <= 8 ? 8 : 16;
-        double* l64 = lanes64.data() + size_t(g0) * n;
-        float* l32 = lanes32.data() + size_t(g0) * n;
-        lk::interleave(src + g0, n, width, use32 ? nullptr : l64, use32 ? l32 : nullptr, s);
-        op.apply_scan_batch(use32 ? static_cast<const void*>(l32) : static_cast<const void*>(l64), use32, width,
-                            amps + g0, diags + g0, ys + g0, gates + g0, s);
+      if (w32) lk::interleave(src, n, w32, nullptr, lanes32.data(), s);
+      const double* src64[16] = {};
+      for (int q = 0; q < k64; ++q) {
+        const Lane& L = a64[o64 + size_t(q)];
+        src64[q] = L.src;
+        amps[w32 + q] = L.amps;
+        diags[w32 + q] = L.diag;
+        ys[w32 + q] = L.y;
+        gates[w32 + q] = L.gate;
+      }
+      double* x64 = lanes64.data() + lane64_base * n;
+      if (w64) lk::interleave(src64, n, w64, x64, nullptr, s);
+      op.apply_scan_mixed(lanes32.data(), w32, x64, w64, amps, diags, ys, gates, s);
+    };
+    {
+      // chunks of at most max_width FP32 and 8 FP64 lanes, together when
+      // their lane tables fit the LDS
+      const size_t na = size_t(op.scan_batch_amp_count());
+      auto fits = [&](int w) { return sizeof(double) * (size_t(w) * na + 81) <= 64 * 1024; };
+      int o32 = 0, o64 = 0, k32 = int(l32.size()), k64 = int(l64.size());
+      size_t base64 = 0;
+      while (o32 < k32 || o64 < k64) {
+        int c32 = std::min(k32 - o32, max_width), c64 = std::min(k64 - o64, 8);
+        if (c32 > 0 && c64 > 0 && !fits(pow2(c32) + pow2(c64))) c64 = 0;
+        launch(l32, size_t(o32), c32, l64, size_t(o64), c64, base64);
+        o32 += c32;
+        o64 += c64;
+        if (c64 > 0) base64 = (base64 + 8) % 16;
       }
     }
     if (!active) break;
@@ -1166,7 +1239,8 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
     // window: α, the overlaps (one launch per column tile in use), their
     // reductions, the subtractions, the finishing kernels
     std::vector<int> stepping;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < slots; ++b) {
+      if (point[size_t(b)] < 0) continue;
       auto& r = *runs[size_t(b)];
       if (!failed[size_t(b)] && !r.done() && r.stepping()) stepping.push_back(b);
     }
@@ -1185,9 +1259,10 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
       for (int b : stepping) fj.push_back(runs[size_t(b)]->window().finish_job());
       lk::lanczos_finish_multi(fj.data(), int(fj.size()), n, ctl.dgks_eta, s);
     }
-    std::vector<char> stepped(static_cast<size_t>(nb), 0);
+    std::vector<char> stepped(static_cast<size_t>(slots), 0);
     for (int b : stepping) stepped[size_t(b)] = 1;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < slots; ++b) {
+      if (point[size_t(b)] < 0) continue;
       auto& r = *runs[size_t(b)];
       if (failed[size_t(b)] || r.done()) continue;
       try {
@@ -1198,13 +1273,44 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
     }
     vb.flush();
     pb.flush();
+    // finished windows hand their results over and take the next point
+    std::vector<int> fill;
+    for (int b = 0; b < slots; ++b) {
+      if (point[size_t(b)] < 0) continue;
+      auto& r = *runs[size_t(b)];
+      if (!failed[size_t(b)] && !r.done()) continue;
+      LockstepResult out;
+      out.ok = !failed[size_t(b)] && res[size_t(b)].ok;
+      if (out.ok) out.es = res[size_t(b)].es;
+      feed.done(point[size_t(b)], std::move(out));
+      point[size_t(b)] = feed.next(b, pts[size_t(b)]);
+      if (point[size_t(b)] < 0) continue;
+      res[size_t(b)] = WindowResult();
+      failed[size_t(b)] = 0;
+      began[size_t(b)] = tick + 1;
+      r.restart(res[size_t(b)]);
+      r.set_vector_destination(pts[size_t(b)].vector);
+      fill.push_back(b);
+    }
+    start_slots(fill);
   }
   QMBED_HIP(hipStreamSynchronize(s));
+}
+
+std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& o, const EigshControls& ctl,
+                                           const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
+                                           DeviceArray<float>& lanes32, hipStream_t s) {
+  const int nb = int(pts.size());
   std::vector<LockstepResult> out(static_cast<size_t>(nb));
-  for (int b = 0; b < nb; ++b) {
-    out[size_t(b)].ok = !failed[size_t(b)] && res[size_t(b)].ok;
-    if (out[size_t(b)].ok) out[size_t(b)].es = res[size_t(b)].es;
-  }
+  int next = 0;
+  LockstepFeed feed;
+  feed.next = [&](int, LockstepPoint& pt) {
+    if (next >= nb) return -1;
+    pt = pts[size_t(next)];
+    return next++;
+  };
+  feed.done = [&](int i, LockstepResult&& r) { out[size_t(i)] = std::move(r); };
+  eigsh_lockstep_stream(op, o, ctl, nb, feed, lanes64, lanes32, s);
   return out;
 }
 

@@ -1146,8 +1146,10 @@ __global__ void __launch_bounds__(kBlock) hv_stream_kernel(HvArgs a, const SX* _
 }
 
 // Scan points of one real family in lockstep: per point an amplitude table
-// (gather form, real), a per-row diagonal, an output vector and a gate.
-constexpr int kMaxScanBatch = 16;
+// (gather form, real), a per-row diagonal, an output vector and a gate; up
+// to 16 lanes of one precision (hv_scan_batch_kernel) or 16 FP32 and 8 FP64
+// lanes together (hv_scan_mixed_kernel).
+constexpr int kMaxScanBatch = 24;
 struct ScanBatch {
   const double* amp[kMaxScanBatch];  // per lane: a table of n_amp entries
   const double* diag[kMaxScanBatch];
@@ -1222,6 +1224,92 @@ __global__ void __launch_bounds__(kBlock) hv_scan_batch_kernel(HvArgs a, ScanBat
       if (SYM) scale *= sq_r * s_crsq[field(w, sc, a.c_cb)];
 #pragma unroll
       for (int b = 0; b < B; ++b) acc[b] += s_amp[b * a.n_amp + off] * scale * double(xv[e][b]);
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int b = 0; b < B; ++b)
+    if (on[b]) sb.y[b][row] = acc[b];
+}
+
+// The same action with lanes of both precisions in one launch: lanes
+// b < W32 read the FP32 interleaved input x32[row·W32 + b], lanes
+// W32 <= b < W32 + W64 the FP64 input x64[row·W64 + (b − W32)]. Windows of a
+// continuous lockstep batch sit in different stages of the FP32 schedule (a
+// window that has just taken its slot runs FP64 products first), and one
+// pass over the index stream serves all of them. Every lane computes what
+// hv_scan_batch_kernel computes for it (same terms, same order).
+template <int W32, int W64, int SYM>
+__global__ void __launch_bounds__(kBlock) hv_scan_mixed_kernel(HvArgs a, ScanBatch sb, const float* __restrict__ x32,
+                                                               const double* __restrict__ x64) {
+  constexpr int B = W32 + W64;
+  extern __shared__ double smem_d[];
+  double* s_amp = smem_d;                // B tables of n_amp
+  double* s_sq = s_amp + B * a.n_amp;    // 65
+  double* s_crsq = s_sq + 65;            // 16
+  bool on[B];
+  bool any = false;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    on[b] = sb.y[b] != nullptr && !(sb.gate[b] && *sb.gate[b]);
+    any = any || on[b];
+  }
+  if (!any) return;
+  for (int i = threadIdx.x; i < B * a.n_amp; i += kBlock) {
+    const int b = i / a.n_amp;
+    s_amp[i] = on[b] ? sb.amp[b][i - b * a.n_amp] : 0.0;
+  }
+  for (int i = threadIdx.x; i < 65; i += kBlock) s_sq[i] = sqrt(double(i));
+  for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t slice = row / kSlice, lane = row % kSlice;
+  const uint32_t n_slices = (a.n + kSlice - 1) / kSlice;
+  if (slice >= n_slices) return;  // whole wavefronts leave together
+  const bool live = row < a.n;
+  const uint32_t base = a.slice_ptr[slice];
+  const uint32_t width = (a.slice_ptr[slice + 1] - base) / kSlice;
+  double sq_r = 1;
+  if (SYM && live) sq_r = s_sq[a.orbits[row]];
+  const uint32_t own = live ? row : 0u;
+  double acc[B];
+#pragma unroll
+  for (int b = 0; b < W32; ++b) acc[b] = on[b] ? sb.diag[b][own] * double(x32[size_t(own) * W32 + b]) : 0.0;
+#pragma unroll
+  for (int b = 0; b < W64; ++b)
+    acc[W32 + b] = on[W32 + b] ? sb.diag[W32 + b][own] * x64[size_t(own) * W64 + b] : 0.0;
+  constexpr int E = B > 12 ? 2 : 4;
+  const uint32_t* words = a.stream + base + lane;
+  const int so = a.c_b, sc = so + a.c_ob, ss = sc + a.c_cb;
+  const uint32_t imask = (1u << a.c_b) - 1u;
+  for (uint32_t k0 = 0; k0 < width; k0 += E) {
+    uint32_t ent[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t w = words[size_t(kSlice) * min(k0 + e, width - 1)];
+      ent[e] = k0 + e < width ? w : 0xFFFFFFFFu;
+    }
+    float xf[E][W32 > 0 ? W32 : 1];
+    double xd[E][W64 > 0 ? W64 : 1];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const size_t idx = ent[e] != 0xFFFFFFFFu ? size_t(ent[e] & imask) : 0;
+#pragma unroll
+      for (int b = 0; b < W32; ++b) xf[e][b] = x32[idx * W32 + b];
+#pragma unroll
+      for (int b = 0; b < W64; ++b) xd[e][b] = x64[idx * W64 + b];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const uint32_t w = ent[e];
+      if (w == 0xFFFFFFFFu) continue;
+      const int off = a.c_off_base + int(field(w, so, a.c_ob));
+      double scale = field(w, ss, a.c_sb) ? -1.0 : 1.0;
+      if (SYM) scale *= sq_r * s_crsq[field(w, sc, a.c_cb)];
+#pragma unroll
+      for (int b = 0; b < W32; ++b) acc[b] += s_amp[b * a.n_amp + off] * scale * double(xf[e][b]);
+#pragma unroll
+      for (int b = 0; b < W64; ++b) acc[W32 + b] += s_amp[(W32 + b) * a.n_amp + off] * scale * xd[e][b];
     }
   }
   if (!live) return;
@@ -2175,6 +2263,50 @@ int Operator::scan_batch_max_width() const {
 
 int Operator::scan_batch_amp_count() const { return tables_->n_amp; }
 
+std::vector<double> Operator::scan_amp_table(const std::vector<cplx>& c) const {
+  auto& t = *tables_;
+  const int na = t.n_amp;
+  if (c.size() != components_.size())
+    throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for " +
+                             std::to_string(components_.size()) + " components");
+  for (const auto& v : c)
+    if (!std::isfinite(v.real()) || v.imag() != 0.0) throw invalid_options("batched scan coefficients must be finite and real");
+  // the gather table in upload order (diagonal groups, then off-diagonal)
+  std::vector<double> amps(static_cast<size_t>(na));
+  size_t k = 0;
+  for (auto* v : {&diag_, &off_})
+    for (auto& g : *v)
+      for (size_t pat = 0; pat < g.parts[0].size(); ++pat) {
+        cplx a = g.parts[0][pat];
+        for (size_t p = 0; p < c.size(); ++p) a += c[p] * g.parts[p + 1][pat];
+        amps[k++] = a.real();
+      }
+  if (k != size_t(na)) throw internal_state("batched scan amplitude table of the wrong length");
+  return amps;
+}
+
+void Operator::scan_point_diagonal(const std::vector<cplx>& c, double* d_diag, hipStream_t s) const {
+  auto& t = *tables_;
+  const size_t n = dim();
+  if (c.size() != components_.size())
+    throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for " +
+                             std::to_string(components_.size()) + " components");
+  DiagCoefs kc;
+  kc.parts = t.diag_parts;
+  kc.real_parts = 1;
+  kc.complex_out = 0;
+  kc.re[0] = 1.0;
+  kc.im[0] = 0.0;
+  for (size_t p = 0; p < c.size(); ++p) {
+    kc.re[p + 1] = c[p].real();
+    kc.im[p + 1] = 0.0;
+  }
+  const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(diag_combine_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, t.part_diag.data(), n, kc,
+                     d_diag);
+  QMBED_HIP(hipGetLastError());
+}
+
 void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, double* d_amps, double* d_diags,
                                  hipStream_t s) const {
   auto& t = *tables_;
@@ -2182,37 +2314,9 @@ void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, do
   const int na = t.n_amp;
   std::vector<double> amps(coefs.size() * size_t(na));
   for (size_t b = 0; b < coefs.size(); ++b) {
-    const auto& c = coefs[b];
-    if (c.size() != components_.size())
-      throw dimension_mismatch("received " + std::to_string(c.size()) + " operator coefficients for " +
-                               std::to_string(components_.size()) + " components");
-    for (const auto& v : c)
-      if (!std::isfinite(v.real()) || v.imag() != 0.0)
-        throw invalid_options("batched scan coefficients must be finite and real");
-    // the gather table in upload order (diagonal groups, then off-diagonal)
-    size_t k = 0;
-    for (auto* v : {&diag_, &off_})
-      for (auto& g : *v)
-        for (size_t pat = 0; pat < g.parts[0].size(); ++pat) {
-          cplx a = g.parts[0][pat];
-          for (size_t p = 0; p < c.size(); ++p) a += c[p] * g.parts[p + 1][pat];
-          amps[b * size_t(na) + k++] = a.real();
-        }
-    if (k != size_t(na)) throw internal_state("batched scan amplitude table of the wrong length");
-    DiagCoefs kc;
-    kc.parts = t.diag_parts;
-    kc.real_parts = 1;
-    kc.complex_out = 0;
-    kc.re[0] = 1.0;
-    kc.im[0] = 0.0;
-    for (size_t p = 0; p < c.size(); ++p) {
-      kc.re[p + 1] = c[p].real();
-      kc.im[p + 1] = 0.0;
-    }
-    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(diag_combine_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, t.part_diag.data(), n, kc,
-                       d_diags + b * 2 * n);
-    QMBED_HIP(hipGetLastError());
+    const auto a = scan_amp_table(coefs[b]);
+    std::copy(a.begin(), a.end(), amps.begin() + long(b * size_t(na)));
+    scan_point_diagonal(coefs[b], d_diags + b * 2 * n, s);
   }
   // staged through the tables' pinned area (free once its last copy has
   // run), so the copy is asynchronous
@@ -2228,7 +2332,7 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
                                 const double* const* diags, double* const* y, const int* const* gates,
                                 hipStream_t s) const {
   auto& t = *tables_;
-  if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16))
+  if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16) || width > kMaxScanBatch)
     throw internal_state("batched scan width must be 1, 2, 4, 8 or 16");
   if (width > scan_batch_max_width()) throw internal_state("batched scan width beyond the LDS of its tables");
   if (!t.compact || t.stream.size() == 0) throw internal_state("batched scan action without a compact stream");
@@ -2277,6 +2381,66 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
     }
   }
 #undef QMBED_SCAN_LAUNCH
+  QMBED_HIP(hipGetLastError());
+  launch_timer_end(timed, s);
+}
+
+void Operator::apply_scan_mixed(const float* x32, int w32, const double* x64, int w64, const double* const* amps,
+                                const double* const* diags, double* const* y, const int* const* gates,
+                                hipStream_t s) const {
+  auto& t = *tables_;
+  auto pow2_or_zero = [](int w, int cap) { return w == 0 || ((w & (w - 1)) == 0 && w <= cap); };
+  if (!pow2_or_zero(w32, 16) || !pow2_or_zero(w64, 8) || w32 + w64 == 0)
+    throw internal_state("mixed scan widths must be 0, 1, 2, 4, 8 or 16 (FP32) and 0, 1, 2, 4 or 8 (FP64)");
+  if (w64 == 0) return apply_scan_batch(x32, true, w32, amps, diags, y, gates, s);
+  if (w32 == 0) return apply_scan_batch(x64, false, w64, amps, diags, y, gates, s);
+  const int B = w32 + w64;
+  if (B > kMaxScanBatch) throw internal_state("mixed scan launch of more lanes than a batch holds");
+  if (sizeof(double) * (size_t(B) * size_t(t.n_amp) + 81) > 64 * 1024)
+    throw internal_state("mixed scan width beyond the LDS of its tables");
+  if (!t.compact || t.stream.size() == 0) throw internal_state("batched scan action without a compact stream");
+  HvArgs a = t.args(*basis_);
+  a.stream = t.stream.data();
+  a.slice_ptr = t.slice_ptr.data();
+  a.img = nullptr;
+  if (a.n == 0) return;
+  ScanBatch sb{};
+  for (int b = 0; b < B; ++b) {
+    sb.amp[b] = amps[b];
+    sb.diag[b] = diags[b];
+    sb.y[b] = y[b];
+    sb.gate[b] = gates ? gates[b] : nullptr;
+  }
+  const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
+  const dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
+  const size_t sm = sizeof(double) * (size_t(B) * size_t(t.n_amp) + 81);
+  const int sym = sym_code(*basis_);
+  char kname[64];
+  std::snprintf(kname, sizeof kname, "hv_scan_mixed_kernel<%d, %d, %d>", w32, w64, sym < 0 || sym > 3 ? 3 : sym);
+  const int timed = launch_timer_begin(kname, s);
+#define QMBED_MIXED_SYM(W32, W64)                                                                                    \
+  switch (sym) {                                                                                                      \
+    case 0: hipLaunchKernelGGL((hv_scan_mixed_kernel<W32, W64, 0>), grid, block, sm, s, a, sb, x32, x64); break;    \
+    case 1: hipLaunchKernelGGL((hv_scan_mixed_kernel<W32, W64, 1>), grid, block, sm, s, a, sb, x32, x64); break;    \
+    case 2: hipLaunchKernelGGL((hv_scan_mixed_kernel<W32, W64, 2>), grid, block, sm, s, a, sb, x32, x64); break;    \
+    default: hipLaunchKernelGGL((hv_scan_mixed_kernel<W32, W64, 3>), grid, block, sm, s, a, sb, x32, x64); break;   \
+  }
+#define QMBED_MIXED_W64(W32)                 \
+  switch (w64) {                             \
+    case 1: QMBED_MIXED_SYM(W32, 1) break;   \
+    case 2: QMBED_MIXED_SYM(W32, 2) break;   \
+    case 4: QMBED_MIXED_SYM(W32, 4) break;   \
+    default: QMBED_MIXED_SYM(W32, 8) break;  \
+  }
+  switch (w32) {
+    case 1: QMBED_MIXED_W64(1) break;
+    case 2: QMBED_MIXED_W64(2) break;
+    case 4: QMBED_MIXED_W64(4) break;
+    case 8: QMBED_MIXED_W64(8) break;
+    default: QMBED_MIXED_W64(16) break;
+  }
+#undef QMBED_MIXED_W64
+#undef QMBED_MIXED_SYM
   QMBED_HIP(hipGetLastError());
   launch_timer_end(timed, s);
 }

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <complex>
+#include <functional>
 #include <cstdint>
 #include <memory>
 #include <optional>
@@ -284,6 +285,11 @@ class Operator {
   int scan_batch_max_width() const;
   void scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, double* d_amps, double* d_diags,
                          hipStream_t s) const;
+  // the two halves of scan_batch_tables for one point: its gather-form
+  // amplitude table (host, scan_batch_amp_count entries) and its per-row
+  // diagonal written to d_diag (device, re[N] then im[N])
+  std::vector<double> scan_amp_table(const std::vector<cplx>& c) const;
+  void scan_point_diagonal(const std::vector<cplx>& c, double* d_diag, hipStream_t s) const;
   // y_b = H(c_b) x_b for the lanes b < width (1, 2, 4, 8 or 16, at most
   // scan_batch_max_width) of an input
   // interleaved as x[row·width + b] (FP32 or FP64); y[b] == nullptr skips a
@@ -291,6 +297,12 @@ class Operator {
   // gate. amps[b] / diags[b]: the lane's point tables from scan_batch_tables.
   void apply_scan_batch(const void* x, bool fp32, int width, const double* const* amps, const double* const* diags,
                         double* const* y, const int* const* gates, hipStream_t s) const;
+  // Both precisions in one launch: lanes b < w32 read the FP32 interleaved
+  // x32 (stride w32), lanes w32 <= b < w32 + w64 the FP64 interleaved x64
+  // (stride w64); w32 in {0, 1, 2, 4, 8, 16}, w64 in {0, 1, 2, 4, 8}. Each
+  // lane's result is the one apply_scan_batch gives it.
+  void apply_scan_mixed(const float* x32, int w32, const double* x64, int w64, const double* const* amps,
+                        const double* const* diags, double* const* y, const int* const* gates, hipStream_t s) const;
 
   // per-row Σ|H_ij| over j != i merged per column, plus |H_ii - shift|;
   // returns (trace, max row sum) — ‖A-μI‖₁ for Hermitian A
@@ -451,6 +463,9 @@ struct EigshControls {
   // scans from the snapshot start: points solved in lockstep per batch (one
   // shared H·v launch per step and 8 windows; 0 or 1 solves them one by one)
   int lockstep = 16;
+  // lockstep scans: a window that finishes hands its slot to the next point
+  // at once (false: the points go in batches that start and end together)
+  bool lockstep_refill = false;
   // return the eigenvectors (false: only eigenvalues and residuals, with no
   // device allocation per solve)
   bool keep_vectors = true;
@@ -492,6 +507,22 @@ struct LockstepResult {
 std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOptions& opts, const EigshControls& ctl,
                                            const std::vector<LockstepPoint>& pts, DeviceArray<double>& lanes64,
                                            DeviceArray<float>& lanes32, hipStream_t s);
+// Continuous lockstep: `slots` windows (1 to 16) step together, and a slot
+// whose window has finished is refilled at once with the next point, so the
+// launches stay wide until the points run out. feed.next(slot, pt) fills pt
+// with the next point's device tables and start (written in stream order
+// into the slot's own areas, which the slot's previous point no longer
+// reads) and returns its index, or -1 when none is left; feed.done(index,
+// result) hands over each point's result (ok == false: the caller solves
+// that point on its own). Every window runs the single-window algorithm, so
+// each point's result is that of its one-by-one solve.
+struct LockstepFeed {
+  std::function<int(int slot, LockstepPoint& pt)> next;
+  std::function<void(int index, LockstepResult&& result)> done;
+};
+void eigsh_lockstep_stream(const Operator& op, const EigshOptions& opts, const EigshControls& ctl, int slots,
+                           const LockstepFeed& feed, DeviceArray<double>& lanes64, DeviceArray<float>& lanes32,
+                           hipStream_t s);
 
 // eigsh: QMBED src/solve/mod.rs:2335-2452 (dispatch), 1116-1456 (Lanczos).
 // `initial` is a host vector of length dim (nullptr: deterministic start).

@@ -193,6 +193,21 @@ WarmStartScan::WarmStartScan(Operator& family, hipStream_t s, const ScanOptions&
   (void)mlp_parameter_count(P_, o.hidden1, o.hidden2, o.snapshots);
 }
 
+WarmStartScan::~WarmStartScan() = default;
+
+// the continuation coefficients of n points, on up to 8 host threads
+std::vector<std::vector<double>> WarmStartScan::continuations_of(const double* points, int n) const {
+  std::vector<std::vector<double>> ys(static_cast<size_t>(std::max(n, 0)));
+  const int nt = std::min(8, std::max(n, 1));
+  std::vector<std::future<void>> fs;
+  for (int t = 0; t < nt; ++t)
+    fs.push_back(std::async(std::launch::async, [&, t] {
+      for (int i = t; i < n; i += nt) ys[size_t(i)] = continuation(points + size_t(i) * P_);
+    }));
+  for (auto& f : fs) f.get();
+  return ys;
+}
+
 std::vector<cplx> WarmStartScan::coefficients_of(const double* point) const {
   std::vector<cplx> c(static_cast<size_t>(P_));
   for (int p = 0; p < P_; ++p) {
@@ -523,7 +538,10 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
     QMBED_HIP(hipStreamSynchronize(s_));
   };
   // (small sectors take the dense route, one by one)
-  const bool batched = ctl.lockstep > 1 && N > 256 && op_.scan_batch_supported(s_);
+  // lockstep windows run the unrestarted route (explicit windows below 96
+  // thick-restart, one point at a time)
+  const bool batched = ctl.lockstep > 1 && N > 256 && (!o.krylov_dimension || *o.krylov_dimension >= 96) &&
+                       op_.scan_batch_supported(s_);
   const int W = std::max(1, std::min(ctl.lockstep, 16));
   for (int i0 = 0; i0 < n;) {
     if (i0 == n_plain && n_plain < n) {
@@ -681,18 +699,12 @@ std::vector<std::vector<double>> WarmStartScan::network_coefficients(const doubl
   const auto y = predict(points, n);
   std::vector<std::vector<double>> out(static_cast<size_t>(std::max(n, 0)));
   const bool corr = opt_.targets == ScanOptions::Targets::Correction;
-  std::vector<std::future<std::vector<double>>> fy;
-  if (corr)
-    for (int b = 0; b < n; ++b) {
-      const double* p = points + size_t(b) * P_;
-      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
-    }
+  std::vector<std::vector<double>> c;
+  if (corr) c = continuations_of(points, n);
   for (int b = 0; b < n; ++b) {
     out[size_t(b)].assign(y.begin() + size_t(b) * ns_, y.begin() + size_t(b + 1) * ns_);
-    if (corr) {
-      const auto c = fy[size_t(b)].get();
-      for (int k = 0; k < ns_; ++k) out[size_t(b)][size_t(k)] += c[size_t(k)];
-    }
+    if (corr)
+      for (int k = 0; k < ns_; ++k) out[size_t(b)][size_t(k)] += c[size_t(b)][size_t(k)];
   }
   return out;
 }
@@ -705,7 +717,8 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
     return out;
   }
-  if (ctl.lockstep > 1 && op_.dim() > 256 && op_.scan_batch_supported(s_))
+  if (ctl.lockstep > 1 && op_.dim() > 256 && (!o.krylov_dimension || *o.krylov_dimension >= 96) &&
+      op_.scan_batch_supported(s_))
     return solve_lockstep(points, n, o, ctl, start);
   if (start != Start::SnapshotRR) {
     for (int i = 0; i < n; ++i) out.push_back(solve(points + size_t(i) * P_, o, ctl, start));
@@ -721,10 +734,16 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
   return out;
 }
 
-// Snapshot starts of up to ctl.lockstep points at a time, solved in lockstep
-// (eigsh_lockstep): the points' amplitude tables and diagonals side by side,
-// their starts S·y_b from one product, one shared H·v launch per step. A
-// point whose start vanishes or whose window fails is solved on its own.
+// The points in lockstep (eigsh_lockstep_stream): up to ctl.lockstep
+// windows step together with shared H·v launches. In batches (the default)
+// the points go in groups of ctl.lockstep that start together (their starts
+// S·y formed by one product per group, which reads S once) and a group ends
+// with its slowest window; with ctl.lockstep_refill a window that finishes
+// hands its slot to the next point at once (its start formed on its own).
+// The points' amplitude tables and start coefficients are made up front
+// (one upload each); a slot's diagonal is written when the slot takes its
+// point. A point whose start vanishes or whose window fails is solved on its
+// own.
 std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const double* points, int n, const EigshOptions& o,
                                                                       const EigshControls& ctl, Start start) {
   std::vector<PointResult> out(static_cast<size_t>(n));
@@ -735,92 +754,105 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
   oo.eigenpairs = 1;
   EigshControls ctl1 = ctl;
   ctl1.keep_vectors = false;  // energies only
-  // the continuation coefficients of a batch, on host threads
-  auto continuations = [&](int i0) {
-    std::vector<std::future<std::vector<double>>> fy;
-    for (int b = 0; b < std::min(W, n - i0); ++b) {
-      const double* p = points + size_t(i0 + b) * P_;
-      fy.push_back(std::async(std::launch::async, [this, p] { return continuation(p); }));
-    }
-    return fy;
+  auto t0 = std::chrono::steady_clock::now();
+  auto grow = [](DeviceArray<double>& d, size_t want) {
+    if (d.size() < want) d.resize(want);
   };
-  std::vector<std::future<std::vector<double>>> next_y;
-  if (start == Start::SnapshotRR) next_y = continuations(0);
-  for (int i0 = 0; i0 < n; i0 += W) {
-    const int nb = std::min(W, n - i0);
-    auto t0 = std::chrono::steady_clock::now();
-    // the start coefficients: the continuation (host threads, computed while
-    // the previous batch ran on the device; the next batch's start here) or
-    // the network (one forward pass for the batch); the plain start needs none
-    std::vector<std::vector<double>> ys(static_cast<size_t>(nb));
-    if (start == Start::Network)  // before the tables: predict waits for the stream
-      ys = network_coefficients(points + size_t(i0) * P_, nb);
-    // the tables of every point of the batch (indexed by point), set up
-    // while the continuation threads run; the batch tables and starts live
-    // in grow-only members: no device allocation (and no synchronizing free)
-    // per batch
-    auto grow = [](DeviceArray<double>& d, size_t want) {
-      if (d.size() < want) d.resize(want);
-    };
-    std::vector<std::vector<cplx>> coefs;
-    for (int b = 0; b < nb; ++b) coefs.push_back(coefficients_of(points + size_t(i0 + b) * P_));
-    grow(lk_amps_, size_t(nb) * size_t(count));
-    grow(lk_diags_, size_t(nb) * 2 * N);
-    op_.scan_batch_tables(coefs, lk_amps_.data(), lk_diags_.data(), s_);
-    if (start == Start::SnapshotRR) {
-      for (int b = 0; b < nb; ++b) ys[size_t(b)] = next_y[size_t(b)].get();
-      next_y.clear();
-      if (i0 + W < n) next_y = continuations(i0 + W);
-    }
-    std::vector<int> lanes;  // points with a usable start
-    for (int b = 0; b < nb; ++b) {
-      double cn = 0;
-      for (double v : ys[size_t(b)]) cn += v * v;
-      if (start == Start::Plain || (std::isfinite(cn) && cn > 1.0e-6)) lanes.push_back(b);
-    }
-    std::vector<char> solved(static_cast<size_t>(nb), 0);
-    if (lanes.size() >= 2) {
-      const int nl = int(lanes.size());
-      grow(lk_starts_, size_t(nl) * N);
-      if (start == Start::Plain) {
-        for (int l = 0; l < nl; ++l) deterministic_start_device(lk_starts_.data() + size_t(l) * N, N, oo.seed, s_, false);
-      } else {
-        // the coefficients go up through pinned memory (free: the previous
-        // batch ended with a synchronization)
-        const size_t ny = size_t(ns_) * size_t(nl);
-        cpin_.ensure(ny);
-        for (int l = 0; l < nl; ++l) {
-          const auto& y = ys[size_t(lanes[size_t(l)])];
-          std::copy(y.begin(), y.end(), cpin_.p + size_t(l) * ns_);
-        }
-        grow(lk_y_, ny);
-        QMBED_HIP(hipMemcpyAsync(lk_y_.data(), cpin_.p, ny * 8, hipMemcpyHostToDevice, s_));
-        lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data(), nl, lk_starts_.data(), s_);
-      }
-      std::vector<LockstepPoint> pts;  // starts normalized by eigsh_lockstep
-      for (int l = 0; l < nl; ++l) {
-        const int b = lanes[size_t(l)];
-        pts.push_back({lk_starts_.data() + size_t(l) * N, lk_amps_.data() + size_t(b) * count,
-                       lk_diags_.data() + size_t(b) * 2 * N});
-      }
-      const auto res = eigsh_lockstep(op_, oo, ctl1, pts, lk_lanes64_, lk_lanes32_, s_);
-      QMBED_HIP(hipStreamSynchronize(s_));  // cpin_ is free again
-      const double dt = seconds_since(t0);
-      for (int l = 0; l < nl; ++l) {
-        if (!res[size_t(l)].ok) continue;
-        const int b = lanes[size_t(l)];
-        PointResult& r = out[size_t(i0 + b)];
-        r.es = res[size_t(l)].es;
-        r.warm = start != Start::Plain;
-        r.seconds = dt / double(nl);
-        solved[size_t(b)] = 1;
-      }
-    }
-    for (int b = 0; b < nb; ++b)
-      if (!solved[size_t(b)])
-        out[size_t(i0 + b)] = solve(points + size_t(i0 + b) * P_, o, ctl, start,
-                                    start == Start::SnapshotRR ? &ys[size_t(b)] : nullptr);
+  // the amplitude tables of all points, in one upload
+  std::vector<std::vector<cplx>> coefs(static_cast<size_t>(n));
+  amp_pin_.ensure(size_t(n) * size_t(count));
+  for (int i = 0; i < n; ++i) {
+    coefs[size_t(i)] = coefficients_of(points + size_t(i) * P_);
+    const auto a = op_.scan_amp_table(coefs[size_t(i)]);
+    std::copy(a.begin(), a.end(), amp_pin_.p + size_t(i) * count);
   }
+  grow(lk_amps_, size_t(n) * size_t(count));
+  QMBED_HIP(hipMemcpyAsync(lk_amps_.data(), amp_pin_.p, size_t(n) * size_t(count) * 8, hipMemcpyHostToDevice, s_));
+  // the start coefficients of all points: the continuation (host threads)
+  // or the network's (one forward pass); the plain start needs none
+  std::vector<std::vector<double>> ys;
+  if (start == Start::SnapshotRR) ys = continuations_of(points, n);
+  else if (start == Start::Network) ys = network_coefficients(points, n);
+  std::vector<int> queue;  // points with a usable start, in order
+  for (int i = 0; i < n; ++i) {
+    double cn = 0;
+    if (start != Start::Plain)
+      for (double v : ys[size_t(i)]) cn += v * v;
+    if (start == Start::Plain || (std::isfinite(cn) && cn > 1.0e-6)) queue.push_back(i);
+  }
+  // the usable points' coefficients in queue order, in one upload
+  const size_t nq = queue.size();
+  if (start != Start::Plain && nq > 0) {
+    cpin_.ensure(nq * ns_);
+    for (size_t q = 0; q < nq; ++q) std::copy(ys[size_t(queue[q])].begin(), ys[size_t(queue[q])].end(), cpin_.p + q * ns_);
+    grow(lk_y_, nq * ns_);
+    QMBED_HIP(hipMemcpyAsync(lk_y_.data(), cpin_.p, nq * ns_ * 8, hipMemcpyHostToDevice, s_));
+  }
+  std::vector<char> solved(static_cast<size_t>(n), 0);
+  int solved_n = 0;
+  LockstepFeed feed;
+  feed.done = [&](int i, LockstepResult&& r) {
+    if (!r.ok) return;
+    PointResult& pr = out[size_t(i)];
+    pr.es = std::move(r.es);
+    pr.warm = start != Start::Plain;
+    solved[size_t(i)] = 1;
+    ++solved_n;
+  };
+  if (std::min<size_t>(size_t(W), nq) >= 2) {
+    const int slots = int(std::min<size_t>(size_t(W), nq));
+    grow(lk_diags_, size_t(slots) * 2 * N);
+    grow(lk_starts_, size_t(slots) * N);
+    size_t cursor = 0;
+    if (ctl.lockstep_refill) {
+      feed.next = [&](int slot, LockstepPoint& pt) {
+        if (cursor >= nq) return -1;
+        const size_t q = cursor++;
+        const int i = queue[q];
+        double* diag = lk_diags_.data() + size_t(slot) * 2 * N;
+        double* st = lk_starts_.data() + size_t(slot) * N;
+        op_.scan_point_diagonal(coefs[size_t(i)], diag, s_);
+        if (start == Start::Plain) deterministic_start_device(st, N, oo.seed, s_, false);
+        else lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data() + q * ns_, 1, st, s_);
+        pt = LockstepPoint{st, lk_amps_.data() + size_t(i) * count, diag};
+        return i;
+      };
+      eigsh_lockstep_stream(op_, oo, ctl1, slots, feed, lk_lanes64_, lk_lanes32_, s_);
+    } else {
+      while (cursor < nq) {
+        const size_t q0 = cursor;
+        const int nb = int(std::min<size_t>(size_t(slots), nq - q0));
+        for (int b = 0; b < nb; ++b)
+          op_.scan_point_diagonal(coefs[size_t(queue[q0 + size_t(b)])], lk_diags_.data() + size_t(b) * 2 * N, s_);
+        if (start == Start::Plain) {
+          for (int b = 0; b < nb; ++b)
+            deterministic_start_device(lk_starts_.data() + size_t(b) * N, N, oo.seed, s_, false);
+        } else {
+          lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data() + q0 * ns_, nb, lk_starts_.data(), s_);
+        }
+        int given = 0;
+        feed.next = [&](int slot, LockstepPoint& pt) {
+          if (given >= nb) return -1;
+          const int b = given++;
+          (void)slot;
+          const int i = queue[q0 + size_t(b)];
+          pt = LockstepPoint{lk_starts_.data() + size_t(b) * N, lk_amps_.data() + size_t(i) * count,
+                             lk_diags_.data() + size_t(b) * 2 * N};
+          return i;
+        };
+        eigsh_lockstep_stream(op_, oo, ctl1, nb, feed, lk_lanes64_, lk_lanes32_, s_);
+        cursor += size_t(nb);
+      }
+    }
+  }
+  QMBED_HIP(hipStreamSynchronize(s_));  // the pinned staging areas are free again
+  const double dt = seconds_since(t0);
+  for (int i = 0; i < n; ++i)
+    if (solved[size_t(i)]) out[size_t(i)].seconds = dt / double(std::max(solved_n, 1));
+  for (int i = 0; i < n; ++i)
+    if (!solved[size_t(i)])
+      out[size_t(i)] = solve(points + size_t(i) * P_, o, ctl, start,
+                             start == Start::SnapshotRR ? &ys[size_t(i)] : nullptr);
   return out;
 }
 

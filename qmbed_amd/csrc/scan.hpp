@@ -1,6 +1,8 @@
 // Learned Lanczos warm start for parameter scans (scan.hip).
 #pragma once
 
+#include <future>
+
 #include "qmbed.hpp"
 
 namespace qmbed {
@@ -44,6 +46,9 @@ class WarmStartScan {
   // `family` must stay alive and is re-evaluated by every call; s: the
   // stream all work runs on
   WarmStartScan(Operator& family, hipStream_t s, const ScanOptions& o);
+  ~WarmStartScan();
+  WarmStartScan(const WarmStartScan&) = delete;
+  WarmStartScan& operator=(const WarmStartScan&) = delete;
 
   struct TrainReport {
     std::vector<double> energies;
@@ -126,6 +131,8 @@ class WarmStartScan {
   // the coefficients of the network's start at n points (host): the
   // network's outputs, plus the continuation under correction targets
   std::vector<std::vector<double>> network_coefficients(const double* points, int n);
+  std::vector<std::vector<double>> continuations_of(const double* points, int n) const;
+  PinnedArray<double> amp_pin_;  // staging of the points' amplitude tables
 };
 
 }  // namespace qmbed

@@ -144,11 +144,16 @@ def test_batch_errors(qa):
         qa.eigsh_batch([a, a], 1)
     with pytest.raises(qa.QmbedError, match="FP64"):
         qa.eigsh_batch([a], 1, matvec_precision="fp32")
-    # a window too small to converge: the same error as the single solver
+    # a small explicit window thick-restarts, in the batch (its single-sector
+    # finish) as in the single solver
+    rb = qa.eigsh_batch([a], 1, krylov_dimension=4)["results"][0]
+    rs = a.eigsh(1, krylov_dimension=4)
+    assert rb["eigenvalues"] == rs["eigenvalues"] and rb["iterations"] == rs["iterations"] > 4
+    # a budget too small to converge: the same error as the single solver
     with pytest.raises(qa.QmbedError, match="did not converge") as e1:
-        qa.eigsh_batch([a], 1, krylov_dimension=4)
+        qa.eigsh_batch([a], 1, krylov_dimension=4, max_iterations=8)
     with pytest.raises(qa.QmbedError, match="did not converge") as e2:
-        a.eigsh(1, krylov_dimension=4)
+        a.eigsh(1, krylov_dimension=4, max_iterations=8)
     assert str(e1.value) == str(e2.value)
 
 
