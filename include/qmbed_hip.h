@@ -294,6 +294,16 @@ int qmbed_hip_mlp_train(int in, int h1, int h2, int out, float* params, const fl
  * C is p × q, column-major, on the device; the call synchronizes. Return 0
  * on success, -1 on failure. */
 int qmbed_hip_gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C);
+/* qmbed_hip_gemm_atb without device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-block partial products live in a static device array of 4 MiB
+ * (524,288 doubles) per device, held until the process ends: every p, q <= 64
+ * fits, and larger shapes up to that size; beyond it the call answers -2
+ * and launches nothing. Same kernels and result as qmbed_hip_gemm_atb; the
+ * call synchronizes the device. Calls that overlap on one device race on
+ * the array. */
+int qmbed_hip_gemm_atb_noalloc(const double* A, size_t n, int p, const double* B, int q, double* C);
 
 void* qmbed_hip_malloc(size_t bytes);
 void qmbed_hip_free(void* ptr);

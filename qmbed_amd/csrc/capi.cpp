@@ -1402,6 +1402,16 @@ int qmbed_hip_gemm_atb(const double* A, size_t n, int p, const double* B, int q,
     return -1;
   }
 }
+int qmbed_hip_gemm_atb_noalloc(const double* A, size_t n, int p, const double* B, int q, double* C) {
+  try {
+    if (!A || !B || !C || p < 0 || q < 0) return -1;
+    if (!lk::gemm_atb_scratch(A, n, p, B, q, C, nullptr)) return -2;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+  } catch (...) {
+    return -1;
+  }
+}
+
 int qmbed_hip_device_count(void) { return device_count(); }
 
 }  // extern "C"

@@ -9,6 +9,8 @@
 // in-kernel fence between blocks is needed (an agent-scope release writes back
 // the whole L2 of an XCD, which costs more than the redundant adds from L2).
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "device_common.hpp"
 #include "internal.hpp"
@@ -727,18 +729,69 @@ void start_multi(const StartJob* jobs, int nj, size_t n, Workspace& ws, hipStrea
   QMBED_HIP(hipGetLastError());
 }
 
+// the launch shape of gemm_atb: tiles of the output and row blocks, about
+// 512 blocks in all, at least one row chunk each; partial doubles needed
+static void atb_shape(size_t n, int p, int q, int& tiles_j, int& tiles, int& bx, size_t& need) {
+  const int tiles_i = (p + kAtbTile - 1) / kAtbTile;
+  tiles_j = (q + kAtbTile - 1) / kAtbTile;
+  tiles = tiles_i * tiles_j;
+  const size_t chunks = std::max<size_t>(1, (n + kAtbRows - 1) / kAtbRows);
+  bx = int(std::min<size_t>(chunks, size_t(std::max(1, 512 / tiles))));
+  need = size_t(tiles) * size_t(bx) * kAtbTile * kAtbTile;
+}
+
+// every block writes its whole partial tile, so `partial` needs no clearing
+static void atb_launch(const double* A, size_t n, int p, const double* B, int q, double* C, double* partial,
+                       int tiles_j, int tiles, int bx, hipStream_t s) {
+  hipLaunchKernelGGL(k_atb, dim3(bx, tiles), dim3(kT), 0, s, A, n, p, B, q, tiles_j, partial);
+  hipLaunchKernelGGL(k_atb_reduce, dim3((p * q + kT - 1) / kT), dim3(kT), 0, s, partial, bx, p, q, tiles_j, C);
+  QMBED_HIP(hipGetLastError());
+}
+
 void gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C, DeviceArray<double>& partial,
               hipStream_t s) {
   if (p <= 0 || q <= 0) return;
-  const int tiles_i = (p + kAtbTile - 1) / kAtbTile, tiles_j = (q + kAtbTile - 1) / kAtbTile;
-  const size_t chunks = std::max<size_t>(1, (n + kAtbRows - 1) / kAtbRows);
-  // about 512 blocks in all, at least one chunk each
-  const int bx = int(std::min<size_t>(chunks, size_t(std::max(1, 512 / (tiles_i * tiles_j)))));
-  const size_t need = size_t(tiles_i) * tiles_j * bx * kAtbTile * kAtbTile;
+  int tiles_j, tiles, bx;
+  size_t need;
+  atb_shape(n, p, q, tiles_j, tiles, bx, need);
   if (partial.size() < need) partial.resize(need);
-  hipLaunchKernelGGL(k_atb, dim3(bx, tiles_i * tiles_j), dim3(kT), 0, s, A, n, p, B, q, tiles_j, partial.data());
-  hipLaunchKernelGGL(k_atb_reduce, dim3((p * q + kT - 1) / kT), dim3(kT), 0, s, partial.data(), bx, p, q, tiles_j, C);
-  QMBED_HIP(hipGetLastError());
+  atb_launch(A, n, p, B, q, C, partial.data(), tiles_j, tiles, bx, s);
+}
+
+// The allocation-free variant's scratch: the partial products of
+// gemm_atb_scratch, one copy per device (held until the process ends). Host
+// code uses the device address hipGetSymbolAddress gives for it on each
+// device, never the array's name.
+constexpr size_t kAtbScratchDoubles = size_t(1) << 19;  // 4 MiB
+static __device__ __align__(256) double atb_scratch[kAtbScratchDoubles];
+
+size_t atb_scratch_capacity() { return kAtbScratchDoubles; }
+
+// the current device's address of atb_scratch (looked up once per device)
+static double* atb_scratch_ptr() {
+  static std::mutex mu;
+  static std::vector<double*> ptrs;
+  int dev = 0;
+  QMBED_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  if (size_t(dev) >= ptrs.size()) ptrs.resize(size_t(dev) + 1, nullptr);
+  if (!ptrs[size_t(dev)]) {
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(atb_scratch));
+    if (e != hipSuccess || !p) hip_check(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(atb_scratch)");
+    ptrs[size_t(dev)] = p;
+  }
+  return ptrs[size_t(dev)];
+}
+
+bool gemm_atb_scratch(const double* A, size_t n, int p, const double* B, int q, double* C, hipStream_t s) {
+  if (p <= 0 || q <= 0) return true;
+  int tiles_j, tiles, bx;
+  size_t need;
+  atb_shape(n, p, q, tiles_j, tiles, bx, need);
+  if (need > kAtbScratchDoubles) return false;
+  atb_launch(A, n, p, B, q, C, atb_scratch_ptr(), tiles_j, tiles, bx, s);
+  return true;
 }
 
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s) {

@@ -30,6 +30,12 @@ void gemm_vc(bool c, const void* V, size_t n, int m, const void* d_C, int k, voi
 // summed in a fixed order; `partial` is grown as needed.
 void gemm_atb(const double* A, size_t n, int p, const double* B, int q, double* C, DeviceArray<double>& partial,
               hipStream_t s);
+// The same product with its partials in a static device array of
+// atb_scratch_capacity() doubles per device (no allocation). False, with
+// nothing launched, when the shape needs more partials than that. Calls
+// that overlap on one device race on the array.
+bool gemm_atb_scratch(const double* A, size_t n, int p, const double* B, int q, double* C, hipStream_t s);
+size_t atb_scratch_capacity();
 void to_f32(bool c, const void* x, size_t n, float* y, hipStream_t s);
 void from_f32(bool c, const float* y, size_t n, void* x, hipStream_t s, const int* gate = nullptr);
 // slot = |y - lam x|
