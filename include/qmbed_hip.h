@@ -264,10 +264,12 @@ char* qmbed_hip_scan_train(qmbed_hip_scan* scan, const double* points, int n, co
  * precision: up to 16 FP32 and 8 FP64 vectors in one launch), and every
  * other phase of the step in one launch for all of them. Each window runs
  * the single-window algorithm, so results equal one-by-one solves bit for
- * bit; a point whose window fails is solved on its own. The solver key
- * "refill" (bool, default false) hands a finished window's slot to the next
- * point at once instead of running the points in batches that start and
- * end together. Needs a real family with Hermitian parts, no momentum
+ * bit; a point whose window fails is solved on its own. By default the
+ * points run in batches that start and end together; the solver key
+ * "refill_group" (0-16) lets free slots take the next points once that many
+ * are free (the starts of one refill formed together), and "refill" (bool)
+ * is refill_group 1: a finished window's slot goes to the next point at
+ * once. Needs a real family with Hermitian parts, no momentum
  * phases, the compact compiled stream, more than 256 rows and no explicit
  * window below 96; otherwise the points are solved one by one. */
 char* qmbed_hip_scan_solve(qmbed_hip_scan* scan, const double* points, int n, const char* solver_json, int warm);

@@ -252,7 +252,12 @@ SolverReq parse_solver(const Value& s) {
     else bad("matvec_precision must be \"fp64\" or \"fp32\"");
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
-  if (auto* v = s.get("refill"); v && !v->is_null()) r.ctl.lockstep_refill = as_bool(v, false);
+  if (auto* v = s.get("refill"); v && !v->is_null()) r.ctl.lockstep_refill = as_bool(v, false) ? 1 : 0;
+  if (auto* v = s.get("refill_group"); v && !v->is_null()) {
+    const long g = as_long(*v, "refill_group");
+    if (g < 0 || g > 16) bad("refill_group must lie in [0, 16]");
+    r.ctl.lockstep_refill = int(g);
+  }
   if (auto* v = s.get("batch_rows"); v && !v->is_null()) {
     const long c = as_long(*v, "batch_rows");
     if (c < 0) bad("batch_rows must not be negative");

@@ -109,6 +109,7 @@ struct LaunchTimer {
   std::vector<hipEvent_t> pool;  // events free for reuse
   struct Rec {
     std::string kernel;
+    double bytes = 0;
     hipEvent_t a = nullptr, b = nullptr;
   };
   std::vector<Rec> recs;
@@ -138,12 +139,13 @@ LaunchTimer& timer() {
 
 }  // namespace
 
-int launch_timer_begin(const char* kernel, hipStream_t s) {
+int launch_timer_begin(const char* kernel, hipStream_t s, double bytes) {
   auto& t = timer();
   if (!t.on.load(std::memory_order_relaxed)) return -1;
   std::lock_guard<std::mutex> g(t.mu);
   LaunchTimer::Rec r;
   r.kernel = kernel;
+  r.bytes = bytes;
   r.a = t.take();
   QMBED_HIP(hipEventRecord(r.a, s));
   t.recs.push_back(r);
@@ -175,15 +177,20 @@ void launch_timer_switch(bool on) {
 std::string launch_timer_report() {
   auto& t = timer();
   std::lock_guard<std::mutex> g(t.mu);
-  std::map<std::string, std::pair<long, double>> sum;
+  struct Sum {
+    long launches = 0;
+    double ms = 0, bytes = 0;
+  };
+  std::map<std::string, Sum> sum;
   for (auto& r : t.recs) {
     if (!r.b) continue;
     QMBED_HIP(hipEventSynchronize(r.b));
     float ms = 0;
     QMBED_HIP(hipEventElapsedTime(&ms, r.a, r.b));
     auto& e = sum[r.kernel];
-    e.first += 1;
-    e.second += double(ms);
+    e.launches += 1;
+    e.ms += double(ms);
+    e.bytes += r.bytes;
   }
   std::ostringstream os;
   os.precision(17);
@@ -192,8 +199,10 @@ std::string launch_timer_report() {
   for (auto& kv : sum) {
     if (!first) os << ',';
     first = false;
-    os << '"' << kv.first << "\":{\"launches\":" << kv.second.first << ",\"ms_total\":" << kv.second.second
-       << ",\"ms_mean\":" << kv.second.second / double(kv.second.first) << '}';
+    const Sum& e = kv.second;
+    os << '"' << kv.first << "\":{\"launches\":" << e.launches << ",\"ms_total\":" << e.ms
+       << ",\"ms_mean\":" << e.ms / double(e.launches) << ",\"bytes_total\":" << e.bytes
+       << ",\"GBs\":" << (e.ms > 0 ? e.bytes / (e.ms * 1.0e6) : 0.0) << '}';
   }
   os << '}';
   return os.str();

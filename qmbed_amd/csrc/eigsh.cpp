@@ -1117,6 +1117,7 @@ void eigsh_lockstep_stream(const Operator& op, const EigshOptions& o, const Eigs
   std::vector<WindowResult> res(static_cast<size_t>(slots));
   std::vector<std::unique_ptr<WindowRun>> runs(static_cast<size_t>(slots));
   std::vector<char> failed(static_cast<size_t>(slots), 0);
+  std::vector<char> free_slot(static_cast<size_t>(slots), 0);  // finished, waiting for a refill
   const lk::Interleave no_lanes;
   PostBatch pb(s);
   VerifyBatch vb(op, pts, lanes64, s);
@@ -1139,6 +1140,7 @@ void eigsh_lockstep_stream(const Operator& op, const EigshOptions& o, const Eigs
   };
   {
     std::vector<int> fill;
+    if (feed.prepare) feed.prepare(slots);
     for (int b = 0; b < slots; ++b) {
       point[size_t(b)] = feed.next(b, pts[size_t(b)]);
       if (point[size_t(b)] < 0) break;
@@ -1273,26 +1275,47 @@ void eigsh_lockstep_stream(const Operator& op, const EigshOptions& o, const Eigs
     }
     vb.flush();
     pb.flush();
-    // finished windows hand their results over and take the next point
-    std::vector<int> fill;
+    // finished windows hand their results over; free slots take the next
+    // points once enough of them are free
+    int running = 0;
     for (int b = 0; b < slots; ++b) {
-      if (point[size_t(b)] < 0) continue;
+      if (point[size_t(b)] < 0 || free_slot[size_t(b)]) continue;
       auto& r = *runs[size_t(b)];
-      if (!failed[size_t(b)] && !r.done()) continue;
+      if (!failed[size_t(b)] && !r.done()) {
+        ++running;
+        continue;
+      }
       LockstepResult out;
       out.ok = !failed[size_t(b)] && res[size_t(b)].ok;
       if (out.ok) out.es = res[size_t(b)].es;
       feed.done(point[size_t(b)], std::move(out));
-      point[size_t(b)] = feed.next(b, pts[size_t(b)]);
-      if (point[size_t(b)] < 0) continue;
-      res[size_t(b)] = WindowResult();
-      failed[size_t(b)] = 0;
-      began[size_t(b)] = tick + 1;
-      r.restart(res[size_t(b)]);
-      r.set_vector_destination(pts[size_t(b)].vector);
-      fill.push_back(b);
+      free_slot[size_t(b)] = 1;
     }
-    start_slots(fill);
+    int nfree = 0;
+    for (int b = 0; b < slots; ++b) nfree += (point[size_t(b)] >= 0 && free_slot[size_t(b)]) ? 1 : 0;
+    if (nfree > 0 && (nfree >= feed.group || running == 0)) {
+      if (feed.prepare) feed.prepare(nfree);
+      std::vector<int> fill;
+      for (int b = 0; b < slots; ++b) {
+        if (point[size_t(b)] < 0 || !free_slot[size_t(b)]) continue;
+        point[size_t(b)] = feed.next(b, pts[size_t(b)]);
+        if (point[size_t(b)] < 0) continue;
+        auto& r = *runs[size_t(b)];
+        free_slot[size_t(b)] = 0;
+        res[size_t(b)] = WindowResult();
+        failed[size_t(b)] = 0;
+        began[size_t(b)] = tick + 1;
+        r.restart(res[size_t(b)]);
+        r.set_vector_destination(pts[size_t(b)].vector);
+        fill.push_back(b);
+      }
+      start_slots(fill);
+    }
+    if (running == 0) {
+      bool any = false;
+      for (int b = 0; b < slots; ++b) any = any || (point[size_t(b)] >= 0 && !free_slot[size_t(b)]);
+      if (!any) break;
+    }
   }
   QMBED_HIP(hipStreamSynchronize(s));
 }

@@ -40,16 +40,18 @@ class SectorBatchAction {
 };
 
 // ---- launch timer (common.cpp) ----
-// While switched on, the instrumented launches (the batched scan H·v) are
+// While switched on, the instrumented launches (the batched scan H·v and
+// the lockstep windows' overlap and subtraction passes) are
 // bracketed by two HIP events on their own stream; the report resolves the
 // pairs and sums their elapsed times per kernel instance. Off, a launch pays
 // one relaxed atomic load.
-// Returns a record index, or -1 when the timer is off.
-int launch_timer_begin(const char* kernel, hipStream_t s);
+// Returns a record index, or -1 when the timer is off. `bytes`: the
+// launch's algorithmic bytes (0: not stated), summed per kernel instance.
+int launch_timer_begin(const char* kernel, hipStream_t s, double bytes = 0.0);
 void launch_timer_end(int record, hipStream_t s);
 void launch_timer_switch(bool on);
-// {"kernel": {"launches": n, "ms_total": t, "ms_mean": t / n}, ...};
-// synchronizes the recorded events
+// {"kernel": {"launches": n, "ms_total": t, "ms_mean": t / n, "bytes_total":
+// b, "GBs": b / t}, ...}; synchronizes the recorded events
 std::string launch_timer_report();
 
 // ---- FP32 MLP of the learned warm start (mlp.hip) ----

@@ -9,6 +9,7 @@
 // in-kernel fence between blocks is needed (an agent-scope release writes back
 // the whole L2 of an XCD, which costs more than the redundant adds from L2).
 #include <algorithm>
+#include <cstdio>
 #include <mutex>
 #include <vector>
 
@@ -989,6 +990,12 @@ void lanczos_overlaps_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s
     }
     if (k == 0) continue;
     const dim3 grid{unsigned(rbmax), unsigned(gmax), unsigned(k)};
+    // algorithmic bytes: every window's columns and w once
+    double bytes = 0;
+    for (int q = 0; q < k; ++q) bytes += double(J.m[q] + 1) * 8.0 * double(n);
+    char kname[32];
+    std::snprintf(kname, sizeof kname, "lk::k_dots_multi<%d>", ct);
+    const int timed = launch_timer_begin(kname, s, bytes);
     switch (ct) {
       case 4: hipLaunchKernelGGL(k_dots_multi<4>, grid, dim3(kT), 0, s, J, n); break;
       case 8: hipLaunchKernelGGL(k_dots_multi<8>, grid, dim3(kT), 0, s, J, n); break;
@@ -996,6 +1003,7 @@ void lanczos_overlaps_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s
       default: hipLaunchKernelGGL(k_dots_multi<16>, grid, dim3(kT), 0, s, J, n); break;
     }
     QMBED_HIP(hipGetLastError());
+    launch_timer_end(timed, s);
   }
 }
 
@@ -1013,8 +1021,13 @@ void lanczos_subtract_multi(const PassJob* jobs, int nj, size_t n, hipStream_t s
     J.t[q] = three_of<double>(&jb.three, *jb.ws, n);
     J.m[q] = jb.m;
   }
+  // algorithmic bytes: every window's columns read once, w read and written
+  double bytes = 0;
+  for (int q = 0; q < nj; ++q) bytes += double(J.m[q] + 2) * 8.0 * double(n);
+  const int timed = launch_timer_begin("lk::k_sub_multi", s, bytes);
   hipLaunchKernelGGL(k_sub_multi, dim3(unsigned(sub_blocks(n)), unsigned(nj)), dim3(kT), 0, s, J, n);
   QMBED_HIP(hipGetLastError());
+  launch_timer_end(timed, s);
 }
 
 void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s) {

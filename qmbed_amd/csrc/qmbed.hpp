@@ -463,9 +463,10 @@ struct EigshControls {
   // scans from the snapshot start: points solved in lockstep per batch (one
   // shared H·v launch per step and 8 windows; 0 or 1 solves them one by one)
   int lockstep = 16;
-  // lockstep scans: a window that finishes hands its slot to the next point
-  // at once (false: the points go in batches that start and end together)
-  bool lockstep_refill = false;
+  // lockstep scans: free slots take the next points once this many are free
+  // (or none is running); 0 or lockstep and above: batches that start and
+  // end together
+  int lockstep_refill = 0;
   // return the eigenvectors (false: only eigenvalues and residuals, with no
   // device allocation per solve)
   bool keep_vectors = true;
@@ -519,6 +520,13 @@ std::vector<LockstepResult> eigsh_lockstep(const Operator& op, const EigshOption
 struct LockstepFeed {
   std::function<int(int slot, LockstepPoint& pt)> next;
   std::function<void(int index, LockstepResult&& result)> done;
+  // optional: called before the next() calls of one refill with the number
+  // of free slots, so the caller can prepare that many points at once
+  std::function<void(int free)> prepare;
+  // free slots are refilled once at least `group` of them are free, or when
+  // no window is left running (group >= slots: batches that start and end
+  // together)
+  int group = 1;
 };
 void eigsh_lockstep_stream(const Operator& op, const EigshOptions& opts, const EigshControls& ctl, int slots,
                            const LockstepFeed& feed, DeviceArray<double>& lanes64, DeviceArray<float>& lanes32,

@@ -735,11 +735,10 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_many(const double* 
 }
 
 // The points in lockstep (eigsh_lockstep_stream): up to ctl.lockstep
-// windows step together with shared H·v launches. In batches (the default)
-// the points go in groups of ctl.lockstep that start together (their starts
-// S·y formed by one product per group, which reads S once) and a group ends
-// with its slowest window; with ctl.lockstep_refill a window that finishes
-// hands its slot to the next point at once (its start formed on its own).
+// windows step together with shared H·v launches. Free slots take the next
+// points once ctl.lockstep_refill of them are free (the default, 0: when
+// all are, i.e. batches that start and end together); the starts S·y of one
+// refill are formed by one product, which reads S once.
 // The points' amplitude tables and start coefficients are made up front
 // (one upload each); a slot's diagonal is written when the slot takes its
 // point. A point whose start vanishes or whose window fails is solved on its
@@ -803,47 +802,30 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
     const int slots = int(std::min<size_t>(size_t(W), nq));
     grow(lk_diags_, size_t(slots) * 2 * N);
     grow(lk_starts_, size_t(slots) * N);
-    size_t cursor = 0;
-    if (ctl.lockstep_refill) {
-      feed.next = [&](int slot, LockstepPoint& pt) {
-        if (cursor >= nq) return -1;
-        const size_t q = cursor++;
-        const int i = queue[q];
-        double* diag = lk_diags_.data() + size_t(slot) * 2 * N;
-        double* st = lk_starts_.data() + size_t(slot) * N;
-        op_.scan_point_diagonal(coefs[size_t(i)], diag, s_);
-        if (start == Start::Plain) deterministic_start_device(st, N, oo.seed, s_, false);
-        else lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data() + q * ns_, 1, st, s_);
-        pt = LockstepPoint{st, lk_amps_.data() + size_t(i) * count, diag};
-        return i;
-      };
-      eigsh_lockstep_stream(op_, oo, ctl1, slots, feed, lk_lanes64_, lk_lanes32_, s_);
-    } else {
-      while (cursor < nq) {
-        const size_t q0 = cursor;
-        const int nb = int(std::min<size_t>(size_t(slots), nq - q0));
-        for (int b = 0; b < nb; ++b)
-          op_.scan_point_diagonal(coefs[size_t(queue[q0 + size_t(b)])], lk_diags_.data() + size_t(b) * 2 * N, s_);
-        if (start == Start::Plain) {
-          for (int b = 0; b < nb; ++b)
-            deterministic_start_device(lk_starts_.data() + size_t(b) * N, N, oo.seed, s_, false);
-        } else {
-          lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data() + q0 * ns_, nb, lk_starts_.data(), s_);
-        }
-        int given = 0;
-        feed.next = [&](int slot, LockstepPoint& pt) {
-          if (given >= nb) return -1;
-          const int b = given++;
-          (void)slot;
-          const int i = queue[q0 + size_t(b)];
-          pt = LockstepPoint{lk_starts_.data() + size_t(b) * N, lk_amps_.data() + size_t(i) * count,
-                             lk_diags_.data() + size_t(b) * 2 * N};
-          return i;
-        };
-        eigsh_lockstep_stream(op_, oo, ctl1, nb, feed, lk_lanes64_, lk_lanes32_, s_);
-        cursor += size_t(nb);
+    size_t cursor = 0, q0 = 0;
+    feed.group = ctl.lockstep_refill <= 0 || ctl.lockstep_refill >= slots ? slots : ctl.lockstep_refill;
+    // the starts of one refill (up to `slots` points), formed together: S is
+    // read once per refill; a slot's diagonal goes into the slot's own area
+    feed.prepare = [&](int nfree) {
+      q0 = cursor;
+      const int cnt = int(std::min<size_t>(size_t(nfree), nq - cursor));
+      if (cnt <= 0) return;
+      if (start == Start::Plain) {
+        for (int b = 0; b < cnt; ++b) deterministic_start_device(lk_starts_.data() + size_t(b) * N, N, oo.seed, s_, false);
+      } else {
+        lk::gemm_vc(false, S_.data(), N, ns_, lk_y_.data() + q0 * ns_, cnt, lk_starts_.data(), s_);
       }
-    }
+    };
+    feed.next = [&](int slot, LockstepPoint& pt) {
+      if (cursor >= nq) return -1;
+      const size_t q = cursor++;
+      const int i = queue[q];
+      double* diag = lk_diags_.data() + size_t(slot) * 2 * N;
+      op_.scan_point_diagonal(coefs[size_t(i)], diag, s_);
+      pt = LockstepPoint{lk_starts_.data() + (q - q0) * N, lk_amps_.data() + size_t(i) * count, diag};
+      return i;
+    };
+    eigsh_lockstep_stream(op_, oo, ctl1, slots, feed, lk_lanes64_, lk_lanes32_, s_);
   }
   QMBED_HIP(hipStreamSynchronize(s_));  // the pinned staging areas are free again
   const double dt = seconds_since(t0);

@@ -264,8 +264,8 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
     # refilled slots: a finished window hands its slot to the next point
     # (4 slots for 11 points: every slot is refilled at least once, and the
     # windows' FP32 and FP64 lanes share launches); the same per-point solves
-    for ls in (4, 8):
-        refill = scan.solve(held, warm="rayleigh_ritz", lockstep=ls, refill=True, **kw)
+    for ls, rk in ((4, dict(refill=True)), (8, dict(refill=True)), (8, dict(refill_group=3))):
+        refill = scan.solve(held, warm="rayleigh_ritz", lockstep=ls, **rk, **kw)
         assert refill["iterations"] == one["iterations"] and refill["fp32_iterations"] == one["fp32_iterations"]
         np.testing.assert_allclose(refill["energies"], one["energies"], rtol=1e-10, atol=0)
     with pytest.raises(qa.QmbedError, match="lockstep must lie"):
