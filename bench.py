@@ -202,6 +202,38 @@ def hv_profile(qa):
     print(json.dumps(out))
 
 
+HV_MODE_CASES = {
+    "c2_xxz_L24": ({"kind": "spin", "sites": 24, "up": 12, "momentum": 0}, lambda: xxz_terms(24, 1.5)),
+    "c3_j1j2_L28": ({"kind": "spin", "sites": 28, "up": 14, "momentum": 0}, lambda: j1j2_terms(28)),
+    "c5_ising_L26": ({"kind": "spin", "sites": 26, "pauli": True, "momentum": 0, "parity": 1},
+                     lambda: ising_terms(26, 1.0, 0.25)),
+}
+
+
+def hv_modes(qa, iters=20):
+    """The FP64 H·v of the XXZ L=24 up=12 k=0, J1-J2 L=28 up=14 k=0 and Ising
+    L=26 k=0 p=+1 sectors in each mode on its own: the on-the-fly modes
+    (orbit walk + sorted-state search; image table) and the compiled stream,
+    each with the autotuner's lanes per row for that mode. Per mode: [ms,
+    fraction of HBM peak for N(4+8) + nnz(4+8) bytes, lanes per row]."""
+    out = {}
+    for name, (basis, terms) in HV_MODE_CASES.items():
+        row = {}
+        for mode in ("walk", "image", "stream"):
+            os.environ["QMBED_HV_MODE"] = mode
+            try:
+                m = qa.Model(basis, terms())
+                h = hv_roofline(m, np.float64, 8, iters)
+                row[mode] = [sig(h["ms"]), sig(h["frac"], 3), h["lanes_per_row"]]
+                m.close()
+            except qa.QmbedError as e:
+                row[mode] = f"unavailable: {e}"[:80]
+            finally:
+                del os.environ["QMBED_HV_MODE"]
+        out[name] = row
+    return out
+
+
 def sector_batch(qa):
     """XXZ Δ=1, L=24, up ∈ {11,12,13} × k ∈ 0..23: the 72 sectors' ground
     states (and 6 lowest pairs) in one lockstep batch, beside the same solves
@@ -333,6 +365,9 @@ def main():
                     help="only the headline sector's FP32 and FP64 H·v (20 launches each), 20 launches of the "
                          "16-lane and 8-lane batched scan H·v and 5 device copies of 256 MiB, for hardware-counter "
                          "runs")
+    ap.add_argument("--hv-modes", action="store_true",
+                    help="only the FP64 H·v of the XXZ L=24, J1-J2 L=28 and Ising L=26 sectors in the walk, image "
+                         "and stream modes (20 timed launches each)")
     ap.add_argument("--batch-profile", action="store_true",
                     help="only the 72-sector XXZ L=24 ground-state batch and the 64-state evolution (one timed run "
                          "each after a warm-up), for kernel traces")
@@ -347,6 +382,9 @@ def main():
 
     if args.hv_profile:
         hv_profile(qa)
+        return
+    if args.hv_modes:
+        print(json.dumps(hv_modes(qa)))
         return
     if args.batch_profile:
         L = 24
@@ -506,6 +544,7 @@ def main():
         j["fp32_rel_err"] = abs(e["fp32"] - e["fp64"]) / abs(e["fp64"])
         extras["j1j2_L28"] = j
         big.close()
+        extras["hv_modes_fp64"] = hv_modes(qa)
         sb = sector_batch(qa)["c4_sectors72"]
         extras["c4_sectors72"] = {f"k{k}": [sig(sb[f"eigenpairs_{k}"]["batch_s"], 3),
                                             sig(sb[f"eigenpairs_{k}"]["one_by_one_s"], 3),

@@ -273,6 +273,23 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   hipLaunchKernelGGL(splitter_kernel, dim3(unsigned((nheads + 255) / 256)), dim3(256), 0, stream, b->states_.data(),
                      padded, 4, int(nheads), b->heads_.data());
   QMBED_HIP(hipGetLastError());
+  // the 4-ary pyramid under a splitter block of 2^shift states: levels
+  // k = 1 .. (shift − 1) / 2 (the top level's 4 entries span the block)
+  b->pyramid_top_ = shift >= 1 ? (shift - 1) / 2 : 0;
+  if (b->pyramid_top_ >= SpinBasis1D::kMaxLevels) throw internal_state("sorted-state pyramid too deep");
+  b->level_off_.assign(size_t(SpinBasis1D::kMaxLevels), 0);
+  size_t tot = 0;
+  for (int k = 1; k <= b->pyramid_top_; ++k) {
+    b->level_off_[size_t(k)] = tot;
+    tot += ((padded >> (2 * k)) + 8 + 3) / 4 * 4;
+  }
+  b->pyramid_.resize(std::max<size_t>(tot, 4));
+  for (int k = 1; k <= b->pyramid_top_; ++k) {
+    const size_t cnt_k = ((padded >> (2 * k)) + 8 + 3) / 4 * 4;
+    hipLaunchKernelGGL(splitter_kernel, dim3(unsigned((cnt_k + 255) / 256)), dim3(256), 0, stream,
+                       b->states_.data(), total, 2 * k, int(cnt_k), b->pyramid_.data() + b->level_off_[size_t(k)]);
+    QMBED_HIP(hipGetLastError());
+  }
   QMBED_HIP(hipStreamSynchronize(stream));
   return b;
 }

@@ -48,6 +48,10 @@ struct HvArgs {
   int sh;
   const uint32_t* heads;
   uint32_t nheads;
+  // the 4-ary pyramid under each splitter block: lvl[k][i] = states[4^k i],
+  // lvl[0] = states, levels 0..lvl_top
+  const uint32_t* lvl[SpinBasis1D::kMaxLevels];
+  int lvl_top;
   // lanes per row (power of two) and the mask of flip groups a lane owns
   int lpr;
   uint32_t lane_groups;
@@ -63,6 +67,7 @@ struct HvArgs {
   const uint32_t* meta;  // 4 words per group: flip, packed sites, nzmask, amp offset
   const double* amp;     // real: 1 double per entry; complex: 2 doubles per entry
   int n_diag, n_off, n_amp;
+  int max_sites;         // the most sites of any group (at most 5)
   const double* chars;   // 2*L doubles (re, im) of χ(a) = exp(-2πi k a / L)
   // batched columns
   int nvec;
@@ -91,6 +96,14 @@ __device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t ma
   return uint32_t(y >> (L - j)) & mask;
 }
 __device__ __forceinline__ uint32_t reflect_l(uint32_t s, int L) { return __brev(s) >> (32 - L); }
+
+// pattern_of with a uniform trip count (the operator's largest group): the
+// loop is scalar-controlled, absent sites masked off at the end
+__device__ __forceinline__ int pattern_upto(uint32_t r, uint32_t packed, int nmax) {
+  int pat = 0;
+  for (int i = 0; i < nmax; ++i) pat |= int(__builtin_amdgcn_ubfe(r, (packed >> (3 + 5 * i)) & 31u, 1)) << i;
+  return pat & ((1 << (packed & 7u)) - 1);
+}
 
 __device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
   const int ns = packed & 7;
@@ -148,6 +161,69 @@ __device__ __forceinline__ int find_state(uint32_t key, const uint32_t* spl, int
   return found ? int(16 * line + uint32_t(c) - 1) : -1;
 }
 
+// find_state for up to B keys at once (in[b] false: no key, idx[b] = -1):
+// each level's loads are issued for all keys before any is used, so the B
+// dependent chains overlap instead of running one after another.
+template <int B>
+__device__ __forceinline__ void find_states(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* spl,
+                                            const HvArgs& a, int (&idx)[B]) {
+  int pos[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) pos[b] = 0;
+  // splitter counts are powers of two up to 8192: the steps unrolled with
+  // a uniform test each
+#pragma unroll
+  for (int lg = 12; lg >= 0; --lg) {
+    const int step = 1 << lg;
+    if (step < a.nspl) {
+#pragma unroll
+      for (int b = 0; b < B; ++b) pos[b] += spl[pos[b] + step] <= key[b] ? step : 0;
+    }
+  }
+  // Below the splitter, the 4-ary pyramid: at level k one aligned 16-byte
+  // node of 4 entries states[4^k i]; the count of entries <= key names the
+  // child. Every entry of a node before the block start is <= key (it
+  // precedes the splitter), every entry past the block is > key (it follows
+  // the next splitter), so the count is right for the nodes that straddle
+  // the block's ends. One line access per level instead of the four of a
+  // 64-byte line, the L1's rate being one line per clock for scattered loads.
+  // (every node read stays in its level: base is the start of a block, or
+  // of a child node, of a real state whatever the key, so the loads need no
+  // predicate; a key below its node only clears its bit in `ok`)
+  uint32_t base[B];
+  uint32_t ok = 0;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    base[b] = uint32_t(pos[b]) << a.sh;
+    ok |= in[b] ? 1u << b : 0u;
+  }
+  for (int k = a.lvl_top; k > 0; --k) {
+    const uint32_t* lv = a.lvl[k];
+    uint4 nd[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) nd[b] = __ldg(reinterpret_cast<const uint4*>(lv + ((base[b] >> (2 * k)) & ~3u)));
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const uint32_t g = (base[b] >> (2 * k)) & ~3u;
+      const uint32_t c = uint32_t(nd[b].x <= key[b]) + uint32_t(nd[b].y <= key[b]) + uint32_t(nd[b].z <= key[b]) +
+                         uint32_t(nd[b].w <= key[b]);
+      if (c == 0) ok &= ~(1u << b);
+      base[b] = (g + (c > 0 ? c - 1 : 0u)) << (2 * k);
+    }
+  }
+  // level 0: the 4 states of the final node; the key is there or absent
+  uint4 nd[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) nd[b] = __ldg(reinterpret_cast<const uint4*>(a.lvl[0] + (base[b] & ~3u)));
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const uint32_t g = base[b] & ~3u;
+    const uint32_t k = key[b];
+    const int at = nd[b].x == k ? 0 : nd[b].y == k ? 1 : nd[b].z == k ? 2 : nd[b].w == k ? 3 : -1;
+    idx[b] = ((ok >> b) & 1u) && at >= 0 ? int(g) + at : -1;
+  }
+}
+
 // Colex (Gosper-order) rank of a fixed-weight state: Σ C(position, ordinal)
 // over its set bits (QMBED src/basis/mod.rs:640-659).
 __device__ __forceinline__ uint32_t rank_colex(uint32_t s, const uint32_t* binom) {
@@ -179,12 +255,15 @@ __device__ __forceinline__ int move_delta(uint32_t s, int p, int q, const uint32
   return d;
 }
 
-// Orbit walk of a parent state t.
+// Orbit walk of a parent state t, one rotation at a time (64-bit shift and
+// mask per rotation); `walk` below computes the same and falls back to this
+// form only for momentum phases when the rotation count does not fit below
+// the state's bits (L >= 28).
 // SYM: 1 translation, 2 reflection only, 3 translation + reflection.
 // On return: rep, orbit size, and the phase as (ci: character index, sign)
 // such that phase = sign * χ(ci).
 template <int SYM, bool SHIFT>
-__device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci, int& sign) {
+__device__ __forceinline__ void walk_scan(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci, int& sign) {
   const int L = a.L;
   sign = 1;
   ci = 0;
@@ -252,6 +331,77 @@ __device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep,
     rep = mn;
     orbit = (mnp == mn) ? d : 2 * d;
     ci = (L - jmin) % L;
+  }
+}
+
+// The orbit walk with one instruction per rotation: with the doubled state
+// y = t·2^L + t left-aligned in 64 bits (hi:lo), v_alignbit(hi, lo, 32 − j)
+// is a 32-bit word whose top L bits are rotl(t, j) and whose low 32 − L bits
+// follow from the doubled string. The minimum over the words has rotl's
+// minimum in its top bits whatever the low bits hold; with momentum phases
+// the low bits carry j instead, so the minimum also names the first rotation
+// that reaches it (the strict-less scan's choice). Same results as
+// walk_scan for every state (checked exhaustively for L <= 20 and on
+// samples and all periodic states up to L = 31).
+template <int SYM, bool SHIFT>
+__device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci, int& sign) {
+  const int L = a.L;
+  if (SYM == 2) {
+    walk_scan<2, false>(t, a, rep, orbit, ci, sign);
+    return;
+  }
+  const int lowb = 32 - L;  // L <= 31
+  if (SHIFT && L > (1 << lowb)) {
+    walk_scan<SYM, SHIFT>(t, a, rep, orbit, ci, sign);
+    return;
+  }
+  sign = 1;
+  const uint32_t low = (1u << lowb) - 1u;
+  uint32_t hi, lo;
+  auto min_word = [&](uint32_t s) {
+    const uint64_t y = (uint64_t(s) | (uint64_t(s) << L)) << (64 - 2 * L);
+    hi = uint32_t(y >> 32);
+    lo = uint32_t(y);
+    uint32_t mn = SHIFT ? (hi & ~low) : hi;
+    auto word = [&](int j) {
+      const uint32_t w = __builtin_amdgcn_alignbit(hi, lo, uint32_t(32 - j));
+      return SHIFT ? (w & ~low) | uint32_t(j) : w;
+    };
+    // two rotations per step (one three-input minimum)
+    int j = 1;
+    for (; j + 1 < L; j += 2) mn = min(mn, min(word(j), word(j + 1)));
+    if (j < L) mn = min(mn, word(j));
+    return mn;
+  };
+  const uint32_t mn = min_word(t);
+  // the orbit's period: the smallest divisor dv of L with rotl(t, dv) = t
+  // (descending, the last match wins)
+  int d = L;
+#pragma unroll 1
+  for (int i = a.n_div - 1; i >= 0; --i) {
+    const int dv = a.divisors[i];
+    if ((__builtin_amdgcn_alignbit(hi, lo, uint32_t(32 - dv)) >> lowb) == t) d = dv;
+  }
+  const uint32_t rep_o = mn >> lowb;
+  const int j_o = SHIFT ? int(mn & low) : 0;
+  if (SYM == 1) {
+    rep = rep_o;
+    orbit = d;
+    ci = (L - j_o) % L;
+    return;
+  }
+  const uint32_t mnp = min_word(reflect_l(t, L));
+  const uint32_t rep_p = mnp >> lowb;
+  const int j_p = SHIFT ? int(mnp & low) : 0;
+  if (rep_p < rep_o) {
+    rep = rep_p;
+    orbit = 2 * d;
+    ci = (L - j_p) % L;
+    sign = a.parity;
+  } else {
+    rep = rep_o;
+    orbit = (rep_p == rep_o) ? d : 2 * d;
+    ci = (L - j_o) % L;
   }
 }
 
@@ -446,81 +596,98 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
       }
     }
   } else {
+  const int nmax = a.max_sites;
   for (int w0 = 0; w0 < a.n_off; w0 += 32) {
     uint32_t act = 0;
     const int gend = min(32, a.n_off - w0);
+    // g is uniform here: the group words go to scalar registers
     for (int gi = 0; gi < gend; ++gi) {
       const int g = a.n_diag + w0 + gi;
-      const int pat = pattern_of(r, s_meta[4 * g + 1]);
-      act |= ((s_meta[4 * g + 2] >> pat) & 1u) << gi;
+      const uint32_t packed = __builtin_amdgcn_readfirstlane(s_meta[4 * g + 1]);
+      const uint32_t nz = __builtin_amdgcn_readfirstlane(s_meta[4 * g + 2]);
+      const int pat = pattern_upto(r, packed, nmax);
+      act |= ((nz >> pat) & 1u) << gi;
     }
     act &= a.lane_groups << sub;
+    // Batched as in the image mode: the walks of up to B targets (vector
+    // ALU), then their searches level by level, then the B input entries,
+    // then the B products in target order (the order of the one-at-a-time
+    // loop, so the sums are the same).
+    constexpr int B = NV > 1 ? 4 : 8;
     while (act) {
-      const int gi = __ffs(act) - 1;
-      act &= act - 1;
-      const int g = a.n_diag + w0 + gi;
-      const uint32_t t = r ^ s_meta[4 * g];
-      const int off = int(s_meta[4 * g + 3]) + pattern_of(r, s_meta[4 * g + 1]);
-      int idx, ci = 0, sign = 1;
-      S rsq_t = 1;
-      if (IMG) {
-        uint32_t rt = t;
-        if (a.up >= 0) {
-          const uint32_t hp = s_hop[g];
-          if (hp >> 16) {
-            const int p = hp & 255, q = (hp >> 8) & 255;
-            rt = ((r >> p) & 1u) ? rank_r + uint32_t(move_delta(r, p, q, s_bin))
-                                 : rank_r - uint32_t(move_delta(t, p, q, s_bin));
-          } else {
-            if (__popc(t) != a.up) continue;  // leaves the fixed-magnetization sector
-            rt = rank_colex(t, s_bin);
-          }
+      uint32_t key[B];
+      bool in[B];
+      int offs[B], cis[B], sgn[B], orb[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        in[b] = false;
+        key[b] = 0;
+        offs[b] = 0;
+        cis[b] = 0;
+        sgn[b] = 1;
+        orb[b] = 1;
+        if (act) {
+          const int gi = __ffs(act) - 1;
+          act &= act - 1;
+          const int g = a.n_diag + w0 + gi;
+          const uint32_t t = r ^ s_meta[4 * g];
+          offs[b] = int(s_meta[4 * g + 3]) + pattern_upto(r, s_meta[4 * g + 1], nmax);
+          uint32_t rep = t;
+          if (SYM) walk<SYM, SHIFT>(t, a, rep, orb[b], cis[b], sgn[b]);
+          key[b] = rep;
+          in[b] = true;
         }
-        const uint32_t e = __ldg(a.img + rt);
-        if (e == 0xFFFFFFFFu) continue;
-        idx = int(e & ((1u << a.img_bits) - 1u));
-        const int code = int((e >> a.img_bits) & 15u);
-        sign = ((e >> (a.img_bits + 4)) & 1u) ? -1 : 1;
-        ci = int((e >> (a.img_bits + 5)) & 31u);
-        if (SYM) rsq_t = S(s_crsq[code]);
-      } else {
-        uint32_t rep = t;
-        int orbit = 1;
-        if (SYM) walk<SYM, SHIFT>(t, a, rep, orbit, ci, sign);
-        idx = find_state(rep, s_spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
-        if (idx < 0) continue;
-        if (SYM) rsq_t = S(s_rsq[orbit]);
       }
-      Cx<S> c;
-      if (CMODE == 2) {
-        c.re = S(s_amp[2 * off]);
-        c.im = S(s_amp[2 * off + 1]);
-      } else {
-        c.re = S(s_amp[off]);
-        c.im = 0;
-      }
-      S scale = S(sign);
-      if (SYM) scale *= sq_r * rsq_t;
-      c.re *= scale;
-      c.im *= scale;
-      if ((SYM == 1 || SYM == 3) && SHIFT) {
-        const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
-        if (CMODE == 2) {
-          const S re = c.re * pr - c.im * pi;
-          c.im = c.re * pi + c.im * pr;
-          c.re = re;
-        } else {
-          c.re *= pr;  // real characters (k = L/2): pi == 0
+      int idx[B];
+      find_states<B>(key, in, s_spl, a, idx);
+      S xr[B][NV], xi[B][NV];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+        const int at = idx[b] >= 0 ? idx[b] : int(row);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          if (CMODE == 0) {
+            xr[b][v] = x[v * a.ld + at];
+            xi[b][v] = 0;
+          } else {
+            xr[b][v] = x[2 * (v * a.ld + at)];
+            xi[b][v] = x[2 * (v * a.ld + at) + 1];
+          }
         }
       }
 #pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        if (CMODE == 0) {
-          acc[v].re += c.re * x[v * a.ld + idx];
+      for (int b = 0; b < B; ++b) {
+        if (idx[b] < 0) continue;
+        Cx<S> c;
+        if (CMODE == 2) {
+          c.re = S(s_amp[2 * offs[b]]);
+          c.im = S(s_amp[2 * offs[b] + 1]);
         } else {
-          const S xr = x[2 * (v * a.ld + idx)], xi = x[2 * (v * a.ld + idx) + 1];
-          acc[v].re += c.re * xr - c.im * xi;
-          acc[v].im += c.re * xi + c.im * xr;
+          c.re = S(s_amp[offs[b]]);
+          c.im = 0;
+        }
+        S scale = S(sgn[b]);
+        if (SYM) scale *= sq_r * S(s_rsq[orb[b]]);
+        c.re *= scale;
+        c.im *= scale;
+        if ((SYM == 1 || SYM == 3) && SHIFT) {
+          const S pr = S(s_chr[2 * cis[b]]), pi = S(s_chr[2 * cis[b] + 1]);
+          if (CMODE == 2) {
+            const S re = c.re * pr - c.im * pi;
+            c.im = c.re * pi + c.im * pr;
+            c.re = re;
+          } else {
+            c.re *= pr;  // real characters (k = L/2): pi == 0
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          if (CMODE == 0) {
+            acc[v].re += c.re * xr[b][v];
+          } else {
+            acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
+            acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+          }
         }
       }
     }
@@ -1454,6 +1621,7 @@ struct Operator::DeviceTables {
   DeviceArray<double> famp;
   DeviceArray<double> chars;
   int n_diag = 0, n_off = 0, n_amp = 0;
+  int max_sites = 0;
   bool complex_coef = false;  // amplitudes or characters complex
   bool hermitian = true;      // sector matrix Hermitian: gather form valid
   bool family = false;        // parameterized: amplitudes are re-evaluated
@@ -1501,6 +1669,8 @@ struct Operator::DeviceTables {
     a.sh = b.split_shift();
     a.heads = b.heads();
     a.nheads = uint32_t(b.heads_count());
+    a.lvl_top = b.pyramid_top();
+    for (int k = 0; k <= a.lvl_top; ++k) a.lvl[k] = b.level(k);
     a.lpr = lpr;
     uint32_t mask = 0;
     for (int i = 0; i < 32; i += lpr) mask |= 1u << i;
@@ -1515,6 +1685,7 @@ struct Operator::DeviceTables {
     a.amp = amp.data();
     a.n_diag = n_diag;
     a.n_off = n_off;
+    a.max_sites = max_sites;
     a.n_amp = n_amp;
     a.chars = chars.data();
     a.nvec = 1;
@@ -1657,6 +1828,7 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
     for (auto& g : *v) {
       uint32_t packed = uint32_t(g.nsites);
       for (int i = 0; i < g.nsites; ++i) packed |= uint32_t(g.sites[i]) << (3 + 5 * i);
+      t->max_sites = std::max(t->max_sites, int(g.nsites));
       // nonzero patterns: of the table, or of any part of a family (the
       // structure must not depend on the coefficients)
       uint32_t nz = 0;

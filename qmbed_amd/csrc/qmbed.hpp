@@ -182,6 +182,12 @@ class SpinBasis1D {
   size_t heads_count() const { return heads_.size(); }
   int split_count() const { return split_count_; }   // padded (power of two)
   int split_shift() const { return split_shift_; }
+  // Below a splitter, a 4-ary pyramid: level k holds states[4^k i] (level 0
+  // is the state table), padded with 0xFFFFFFFF to whole 16-byte nodes and
+  // beyond the last node a lookup can read; levels 1..pyramid_top() exist.
+  static constexpr int kMaxLevels = 11;
+  int pyramid_top() const { return pyramid_top_; }
+  const uint32_t* level(int k) const { return k == 0 ? states_.data() : pyramid_.data() + level_off_[size_t(k)]; }
 
  private:
   SpinBasisSpec spec_;
@@ -191,6 +197,9 @@ class SpinBasis1D {
   DeviceArray<uint8_t> orbits_;
   DeviceArray<uint32_t> splitters_;
   DeviceArray<uint32_t> heads_;
+  DeviceArray<uint32_t> pyramid_;
+  std::vector<size_t> level_off_;
+  int pyramid_top_ = 0;
   int split_count_ = 0;
   int split_shift_ = 0;
 };
