@@ -439,7 +439,7 @@ class LanczosWindow {
     const int j = launched_;
     return lk::ThreeTerm{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr, slots() + j};
   }
-  lk::AlphaJob alpha_job() { return {col(launched_), w(), &buf_->ws, halt()}; }
+  lk::AlphaJob alpha_job() { return {col(launched_), w(), &buf_->ws, halt(), slots() + launched_}; }
   lk::HredJob hred_job() { return {&buf_->ws, launched_ + 1, h_, halt()}; }
   lk::PassJob pass_job() { return {V(), launched_ + 1, w(), &buf_->ws, h_, halt(), three_term()}; }
   lk::FinishJob finish_job() {
@@ -456,7 +456,7 @@ class LanczosWindow {
     if (!external_) matvec(j, gate);
     const lk::ThreeTerm three{col(j), j > 0 ? col(j - 1) : nullptr, j > 0 ? slots() + m_ + (j - 1) : nullptr,
                               slots() + j};
-    lk::lanczos_alpha(c_, col(j), w(), n_, buf_->ws, s_, gate);
+    lk::lanczos_alpha(c_, col(j), w(), n_, buf_->ws, s_, gate, slots() + j);
     lk::lanczos_overlaps(c_, V(), n_, j + 1, w(), buf_->ws, s_, true, gate, &three);
     lk::lanczos_subtract(c_, V(), n_, j + 1, w(), h_, buf_->ws, s_, true, gate, &three);
     finish(j, false);

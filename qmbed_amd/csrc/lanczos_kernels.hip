@@ -133,26 +133,34 @@ __global__ __launch_bounds__(kT) void k_alpha(const E* __restrict__ vj, const E*
   alpha_body<E>(vj, w, n, partial, gate);
 }
 
+// α_j = Σ of the k_alpha partials (fixed order), once, into the step's α
+// slot, where the overlap and subtraction blocks read it (one block; the
+// blocks of those passes each summed the partials before)
+__device__ __forceinline__ void alpha_fin_body(const double* partial, int nb, double* alpha, const int* gate) {
+  if (gate && *gate) return;
+  __shared__ double sh[kT / 64];
+  const double a = sum_partials(partial, nb, sh);
+  if (threadIdx.x == 0) *alpha = a;
+}
+__global__ __launch_bounds__(kT) void k_alpha_fin(const double* partial, int nb, double* alpha, const int* gate) {
+  alpha_fin_body(partial, nb, alpha, gate);
+}
+
 // The three-term part of a first pass (QMBED src/solve/mod.rs:1040-1048):
-// w₀ = w − α_j v_j − β_{j−1} v_{j−1}, α_j = Σ pa (the k_alpha partials, added
-// by every block in the same order). vj == nullptr: no three-term part.
+// w₀ = w − α_j v_j − β_{j−1} v_{j−1}, α_j from its slot (k_alpha_fin).
+// vj == nullptr: no three-term part.
 template <typename E>
 struct Three {
   const E* vj = nullptr;
   const E* vjm1 = nullptr;           // null at j = 0
   const double* beta_prev = nullptr;  // β_{j−1}
-  const double* pa = nullptr;
-  int nba = 0;
-  double* alpha = nullptr;  // α_j slot (written by block 0 of the subtract)
+  const double* alpha = nullptr;      // α_j slot
 };
 
 // α_j and β_{j−1} into (alpha, beta) in every thread of the block
 template <typename E>
-__device__ __forceinline__ void three_coefs(const Three<E>& t, double* sh, double* bc, double& alpha, double& beta) {
-  const double a = sum_partials(t.pa, t.nba, sh);
-  if (threadIdx.x == 0) *bc = a;
-  __syncthreads();
-  alpha = *bc;
+__device__ __forceinline__ void three_coefs(const Three<E>& t, double& alpha, double& beta) {
+  alpha = *t.alpha;
   beta = t.vjm1 ? *t.beta_prev : 0.0;
 }
 
@@ -179,10 +187,8 @@ __device__ __forceinline__ void dots_body(const E* __restrict__ V, size_t n, int
                                           const Three<E>& t, unsigned bx, unsigned by, unsigned nbx) {
   if (gate && *gate) return;
   __shared__ E sh[CT][kT / 64];
-  __shared__ double shd[kT / 64];
-  __shared__ double bc;
   double alpha = 0, beta = 0;
-  if (t.vj) three_coefs(t, shd, &bc, alpha, beta);
+  if (t.vj) three_coefs(t, alpha, beta);
   const int c0 = by * CT;
   const int nc = min(CT, mm - c0);
   const size_t tiles = (n + size_t(kT * RPT) - 1) / size_t(kT * RPT);
@@ -297,12 +303,8 @@ __device__ __forceinline__ void sub_body(const E* __restrict__ V, size_t n, int 
   if (gate && *gate) return;
   __shared__ E sh_h[kMaxCols];
   __shared__ double sh[kT / 64];
-  __shared__ double bc;
   double alpha = 0, beta = 0;
-  if (t.vj) {
-    three_coefs(t, sh, &bc, alpha, beta);
-    if (bx == 0 && threadIdx.x == 0) *t.alpha = alpha;
-  }
+  if (t.vj) three_coefs(t, alpha, beta);
   for (int c = threadIdx.x; c < mc; c += kT) sh_h[c] = hc[c];
   __syncthreads();
   // v_j and v_{j−1} are normally the last two columns of this chunk: their
@@ -418,10 +420,15 @@ struct AlphaJobs {
   const double* w[kMaxJobs];
   double* partial[kMaxJobs];
   const int* gate[kMaxJobs];
+  double* alpha[kMaxJobs];
 };
 __global__ __launch_bounds__(kT) void k_alpha_multi(AlphaJobs J, size_t n) {
   const int q = blockIdx.y;
   alpha_body<double>(J.vj[q], J.w[q], n, J.partial[q], J.gate[q]);
+}
+__global__ __launch_bounds__(kT) void k_alpha_fin_multi(AlphaJobs J, int nb) {
+  const int q = blockIdx.x;
+  alpha_fin_body(J.partial[q], nb, J.alpha[q], J.gate[q]);
 }
 struct HredJobs {
   const double* partial[kMaxJobs];
@@ -855,15 +862,17 @@ static Three<E> three_of(const ThreeTerm* tt, Workspace& ws, size_t n) {
   t.vj = cE<E>(tt->vj);
   t.vjm1 = cE<E>(tt->vjm1);
   t.beta_prev = tt->beta_prev;
-  t.pa = ws.partial3.data();
-  t.nba = nblocks(n);
   t.alpha = tt->alpha;
+  (void)ws;
+  (void)n;
   return t;
 }
 
-void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate) {
+void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate,
+                   double* alpha) {
   QMBED_DISPATCH(c, hipLaunchKernelGGL(k_alpha<E>, dim3(nblocks(n)), dim3(kT), 0, s, cE<E>(vj), cE<E>(w), n,
                                        ws.partial3.data(), gate));
+  hipLaunchKernelGGL(k_alpha_fin, dim3(1), dim3(kT), 0, s, ws.partial3.data(), nblocks(n), alpha, gate);
 }
 
 void lanczos_overlaps(bool c, const void* V, size_t n, int m, const void* w, Workspace& ws, hipStream_t s,
@@ -1038,8 +1047,11 @@ void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s) 
     J.w[q] = jobs[q].w;
     J.partial[q] = jobs[q].ws->partial3.data();
     J.gate[q] = jobs[q].gate;
+    J.alpha[q] = jobs[q].alpha;
   }
   hipLaunchKernelGGL(k_alpha_multi, dim3(nblocks(n), nj), dim3(kT), 0, s, J, n);
+  QMBED_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_alpha_fin_multi, dim3(nj), dim3(kT), 0, s, J, nblocks(n));
   QMBED_HIP(hipGetLastError());
 }
 

@@ -54,15 +54,17 @@ void residual_to_slot(bool c, const void* y, const void* x, const double* lam_sl
 // no ‖w‖², ungated) and lanczos_finish(second_pass), which clears *halt, and
 // relaunches the voided steps.
 
-// per-block partials of α_j = Re <v_j, w>
-void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate);
+// α_j = Re <v_j, w> into the α slot (per-block partials, then one fixed-order
+// sum)
+void lanczos_alpha(bool c, const void* vj, const void* w, size_t n, Workspace& ws, hipStream_t s, const int* gate,
+                   double* alpha);
 
 // the three-term part of a first pass
 struct ThreeTerm {
   const void* vj;
   const void* vjm1;         // null at j = 0
   const double* beta_prev;  // β_{j−1} slot (unused at j = 0)
-  double* alpha;            // α_j slot, written by lanczos_subtract
+  double* alpha;            // α_j slot, written by lanczos_alpha
 };
 
 // per-row-block overlaps of w₀ with V[:, 0..m) (and ‖w₀‖² when with_ww)
@@ -82,6 +84,7 @@ struct AlphaJob {
   const double* w;
   Workspace* ws;
   const int* gate;
+  double* alpha;  // the step's α slot
 };
 void lanczos_alpha_multi(const AlphaJob* jobs, int nj, size_t n, hipStream_t s);
 struct HredJob {
