@@ -45,6 +45,10 @@ def lib():
         L.oracle_model_new.restype = c_void_p
         L.oracle_model_new.argtypes = [c_int, c_int, P(c_int), c_int, c_int, c_int, c_int, c_int,
                                        ctypes.c_char_p, c_int, c_int]
+        L.oracle_model_new_general.restype = c_void_p
+        L.oracle_model_new_general.argtypes = [c_int, c_int, P(c_int), c_int, c_int, c_int, c_int, c_int,
+                                               ctypes.c_char_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                               c_int]
         L.oracle_model_free.argtypes = [c_void_p]
         L.oracle_dim.restype = c_long
         L.oracle_dim.argtypes = [c_void_p]
@@ -94,14 +98,35 @@ class Model:
     """Oracle sector + assembled operator (row-sorted triplets)."""
 
     def __init__(self, sites, terms=(), up=None, up_sectors=None, momentum=None, parity=None,
-                 norm="angular", check_hermiticity=True, check_particles=True):
+                 norm="angular", check_hermiticity=True, check_particles=True, symmetries=None, reverse=False):
+        """symmetries: the general sector's generators, as the C ABI's
+        requests ({"destinations": [...], "local_permutations": [[a, b], ...]
+        or None, "sector": q}); reverse: maximum representatives and
+        descending rows."""
         L = lib()
         secs = list(up_sectors or [])
         arr = (ctypes.c_int * max(len(secs), 1))(*secs) if secs else None
-        h = L.oracle_model_new(sites, -1 if up is None else up, arr, len(secs),
-                               0 if momentum is None else 1, 0 if momentum is None else momentum,
-                               0 if parity is None else parity, NORMS[norm],
-                               terms_text(terms).encode(), int(check_hermiticity), int(check_particles))
+        args = (sites, -1 if up is None else up, arr, len(secs),
+                0 if momentum is None else 1, 0 if momentum is None else momentum,
+                0 if parity is None else parity, NORMS[norm],
+                terms_text(terms).encode(), int(check_hermiticity), int(check_particles))
+        if symmetries or reverse:
+            syms = list(symmetries or [])
+            dest = np.zeros((max(len(syms), 1), sites), dtype=np.int32)
+            local = np.zeros((max(len(syms), 1), sites, 2), dtype=np.int32)
+            local[:, :, 1] = 1
+            sectors = np.zeros(max(len(syms), 1), dtype=np.int32)
+            for g, q in enumerate(syms):
+                d = list(q["destinations"])
+                if len(d) != sites:
+                    raise OracleError(f"invalid options: symmetry map has {len(d)} sites, expected {sites}")
+                dest[g] = d
+                if q.get("local_permutations") is not None:
+                    local[g] = np.asarray(q["local_permutations"], dtype=np.int32)
+                sectors[g] = int(q["sector"])
+            h = L.oracle_model_new_general(*args, len(syms), _ptr(dest), _ptr(local), _ptr(sectors), int(reverse))
+        else:
+            h = L.oracle_model_new(*args)
         if not h:
             raise OracleError(L.oracle_last_error().decode())
         self._h = h

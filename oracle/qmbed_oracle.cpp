@@ -13,14 +13,17 @@
 // Each routine names the reference file:line it restates (paths relative to
 // the QMBED source tree).
 //
-// Not restated here: shift-invert targets.
+// Not restated here: shift-invert targets, matrix-valued (2-D irrep)
+// symmetry sectors.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <sstream>
@@ -93,6 +96,16 @@ struct Image {
 
 enum Norm { ANGULAR = 0, PAULI = 1, PAULI_CARTESIAN = 2 };
 
+// One generator of a general symmetry sector: destinations[source] is the
+// target site, local[source] the permutation of the site's two digits, and
+// sector the character index (LatticeSymmetryMap, src/basis/mod.rs:2967-3060)
+struct GeneralMap {
+  std::vector<int> dest;
+  std::vector<std::array<int, 2>> local;
+  int sector = 0;
+  int period = 1;
+};
+
 struct Basis {
   int L = 0;
   int up = -1;                  // -1: no fixed up count
@@ -101,11 +114,21 @@ struct Basis {
   int k = 0;
   int parity = 0;  // 0: none
   int norm = ANGULAR;
+  // general symmetry sector (GeneralBasis + SymmetryReducer) and the
+  // descending-row ("reverse") convention
+  std::vector<GeneralMap> gens;
+  bool reverse = false;
   std::vector<u64> states;
   std::vector<int> orbit;
   std::unordered_map<u64, Image> lookup;
-  bool symmetric() const { return has_k || parity != 0; }
+  std::unordered_map<u64, long> rows;  // row of each state when not ascending
+  bool general() const { return !gens.empty(); }
+  bool symmetric() const { return has_k || parity != 0 || general(); }
   long index(u64 s) const {
+    if (reverse || general()) {
+      auto it = rows.find(s);
+      return it == rows.end() ? -1 : it->second;
+    }
     auto it = std::lower_bound(states.begin(), states.end(), s);
     if (it == states.end() || *it != s) return -1;
     return long(it - states.begin());
@@ -193,6 +216,129 @@ static void symmetry_sector(Basis& b, std::vector<u64> parents) {
   }
 }
 
+// LatticeSymmetryMap::mapped_state for spin one-half (distinguishable
+// sites, so the map phase is 1): src/basis/mod.rs:3100-3132
+static u64 apply_map(const GeneralMap& g, u64 s) {
+  u64 out = 0;
+  for (size_t src = 0; src < g.dest.size(); ++src) {
+    const int digit = int((s >> src) & 1);
+    out |= u64(g.local[src][size_t(digit)]) << g.dest[src];
+  }
+  return out;
+}
+
+// LatticeSymmetryMap::new checks and combined_permutation_period:
+// src/basis/mod.rs:2978-3060, 3221-3268
+static void validate_map(GeneralMap& g, int L) {
+  if (int(g.dest.size()) != L) {
+    std::ostringstream os;
+    os << "invalid options: symmetry map has " << g.dest.size() << " sites, expected " << L;
+    throw OracleError(os.str());
+  }
+  if (g.dest.empty()) throw OracleError("invalid Hilbert-space sector: a lattice symmetry requires sites and local states");
+  std::vector<int> seen(size_t(L), 0);
+  for (int d : g.dest) {
+    if (d < 0 || d >= L || seen[size_t(d)]++)
+      throw OracleError("incompatible symmetry request: symmetry site destinations must be a bijection");
+  }
+  for (auto& p : g.local)
+    if (!((p[0] == 0 && p[1] == 1) || (p[0] == 1 && p[1] == 0)))
+      throw OracleError("incompatible symmetry request: every local-state map must be a permutation");
+  // period: lcm of the cycle lengths of (site, digit) -> (dest, local digit)
+  const int elements = 2 * L;
+  std::vector<char> visited(size_t(elements), 0);
+  long period = 1;
+  for (int seed = 0; seed < elements; ++seed) {
+    if (visited[size_t(seed)]) continue;
+    int cur = seed;
+    long cycle = 0;
+    while (!visited[size_t(cur)]) {
+      visited[size_t(cur)] = 1;
+      ++cycle;
+      const int src = cur / 2, digit = cur % 2;
+      cur = g.dest[size_t(src)] * 2 + g.local[size_t(src)][size_t(digit)];
+    }
+    long a = period, bb = cycle;
+    while (bb) {
+      long t = a % bb;
+      a = bb;
+      bb = t;
+    }
+    period = period / a * cycle;
+  }
+  g.period = int(period);
+}
+
+// GeneralBasis::from_reducer_impl with orbit seeds (no parent-membership
+// requirement) and trace_symmetry_orbit: src/basis/mod.rs:3464-3541,
+// 4163-4300. Representatives are the minimum or (reverse) maximum orbit
+// state; rows follow the parent's (ascending) order of the representatives.
+static void general_sector(Basis& b, const std::vector<u64>& parents) {
+  const double TAU = 6.283185307179586;
+  std::vector<cplx> steps;
+  for (auto& g : b.gens) {
+    const long nsec = ((long(g.sector) % g.period) + g.period) % g.period;
+    steps.push_back(std::polar(1.0, -TAU * double(nsec) / double(g.period)));
+  }
+  std::unordered_set<u64> visited;
+  std::vector<std::pair<u64, int>> reps;
+  for (u64 seed : parents) {
+    if (visited.count(seed)) continue;
+    // closure of every generator at its period (phases are 1 here)
+    for (auto& g : b.gens) {
+      u64 c = seed;
+      for (int i = 0; i < g.period; ++i) c = apply_map(g, c);
+      if (c != seed) throw OracleError("incompatible symmetry request: a symmetry map does not close at its declared period");
+    }
+    std::map<u64, cplx> coeff;  // ordered: deterministic iteration
+    coeff[seed] = cplx(1, 0);
+    std::deque<u64> queue{seed};
+    bool compatible = true;
+    while (!queue.empty()) {
+      const u64 src = queue.front();
+      queue.pop_front();
+      const cplx cs = coeff[src];
+      for (size_t gi = 0; gi < b.gens.size(); ++gi) {
+        const u64 tgt = apply_map(b.gens[gi], src);
+        const cplx ct = cs * steps[gi];
+        auto it = coeff.find(tgt);
+        if (it != coeff.end()) {
+          if (std::abs(it->second - ct) > 1.0e-10) compatible = false;
+        } else {
+          coeff[tgt] = ct;
+          queue.push_back(tgt);
+        }
+      }
+    }
+    for (auto& kv : coeff) visited.insert(kv.first);
+    if (!compatible) continue;
+    const u64 rep = b.reverse ? coeff.rbegin()->first : coeff.begin()->first;
+    const cplx rc = coeff[rep];
+    const cplx gauge = rc / std::abs(rc);
+    double nrm = 0;
+    for (auto& kv : coeff) nrm += std::norm(kv.second);
+    nrm = std::sqrt(nrm);
+    const int osize = int(coeff.size());
+    const double expected = 1.0 / std::sqrt(double(osize));
+    for (auto& kv : coeff) {
+      const cplx normalized = kv.second / (gauge * nrm);
+      if (std::abs(std::abs(normalized) - expected) > 1.0e-10)
+        throw OracleError(
+            "incompatible symmetry request: symmetry maps do not define a one-dimensional orbit sector");
+      b.lookup[kv.first] = Image{rep, normalized / expected, osize};
+    }
+    reps.push_back({rep, osize});
+  }
+  // parent rows are ascending states for every spin parent, and an
+  // orbit-seed representative outside the parent sorts by its value: both
+  // are the ascending order of the states
+  std::sort(reps.begin(), reps.end());
+  for (auto& r : reps) {
+    b.states.push_back(r.first);
+    b.orbit.push_back(r.second);
+  }
+}
+
 // SpinBasisBuilder::build: src/basis/mod.rs:1312-1412 (spin one-half only)
 static void build_basis(Basis& b) {
   if (b.L <= 0) throw OracleError("invalid Hilbert-space sector: symmetry sectors require at least one site");
@@ -206,7 +352,22 @@ static void build_basis(Basis& b) {
   } else {
     parents = fixed_weight_states(b.L, b.up);
   }
-  symmetry_sector(b, std::move(parents));
+  if (b.general()) {
+    // build_spin_basis (bindings/capi/src/lib.rs:2849-2920)
+    if (b.has_k || b.parity) throw OracleError("invalid options: built-in and general spin symmetries cannot be mixed");
+    for (auto& g : b.gens) validate_map(g, b.L);
+    general_sector(b, parents);
+  } else {
+    symmetry_sector(b, std::move(parents));
+  }
+  // ordered_basis: the reverse convention lists the rows in descending order
+  // (bindings/capi/src/lib.rs:3412-3414)
+  if (b.reverse) {
+    std::reverse(b.states.begin(), b.states.end());
+    std::reverse(b.orbit.begin(), b.orbit.end());
+  }
+  if (b.reverse || b.general())
+    for (size_t i = 0; i < b.states.size(); ++i) b.rows[b.states[i]] = long(i);
 }
 
 // ------------------------------------------------------------- operator ----
@@ -1407,6 +1568,58 @@ void* oracle_model_new(int sites, int up, const int* up_sectors, int n_up_sector
     m->basis.k = k;
     m->basis.parity = parity;
     m->basis.norm = norm;
+    build_basis(m->basis);
+    std::istringstream in(terms_text ? terms_text : "");
+    std::string line;
+    while (std::getline(in, line)) {
+      std::istringstream ls(line);
+      std::string sym;
+      double re, im;
+      if (!(ls >> sym >> re >> im)) continue;
+      std::vector<int> s;
+      int x;
+      while (ls >> x) s.push_back(x);
+      if (m->terms.empty() || m->terms.back().symbols != sym) m->terms.push_back(Term{sym, {}});
+      m->terms.back().couplings.push_back(Coupling{cplx(re, im), s});
+    }
+    if (!m->terms.empty()) assemble(*m, check_herm != 0, check_particles != 0);
+    return m;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+// The general symmetry sector: nsym generators, dest [nsym][sites], local
+// [nsym][sites][2], sectors [nsym]; reverse: maximum representatives and
+// descending rows (with nsym = 0: the rows of the plain or built-in sector
+// reversed).
+void* oracle_model_new_general(int sites, int up, const int* up_sectors, int n_up_sectors, int has_k, int k,
+                               int parity, int norm, const char* terms_text, int check_herm, int check_particles,
+                               int nsym, const int* dest, const int* local, const int* sectors, int reverse) {
+  try {
+    auto* m = new Model();
+    m->basis.L = sites;
+    m->basis.up = up;
+    for (int i = 0; i < n_up_sectors; ++i) m->basis.up_sectors.push_back(up_sectors[i]);
+    std::sort(m->basis.up_sectors.begin(), m->basis.up_sectors.end());
+    m->basis.up_sectors.erase(std::unique(m->basis.up_sectors.begin(), m->basis.up_sectors.end()),
+                              m->basis.up_sectors.end());
+    m->basis.has_k = has_k != 0;
+    m->basis.k = k;
+    m->basis.parity = parity;
+    m->basis.norm = norm;
+    m->basis.reverse = reverse != 0;
+    for (int g = 0; g < nsym; ++g) {
+      GeneralMap gm;
+      for (int i = 0; i < sites; ++i) {
+        gm.dest.push_back(dest[size_t(g) * size_t(sites) + size_t(i)]);
+        gm.local.push_back({local[(size_t(g) * size_t(sites) + size_t(i)) * 2],
+                            local[(size_t(g) * size_t(sites) + size_t(i)) * 2 + 1]});
+      }
+      gm.sector = sectors[g];
+      m->basis.gens.push_back(gm);
+    }
     build_basis(m->basis);
     std::istringstream in(terms_text ? terms_text : "");
     std::string line;

@@ -467,3 +467,148 @@ def test_oracle_initial_subspace_route(oracle):
     # one vector is eigsh_with_initial (the expanding window)
     r1 = m.eigsh_subspace(2, V[:, :1].T, tolerance=1e-10)
     np.testing.assert_allclose(r1["eigenvalues"], w[:2], rtol=1e-10)
+
+
+# ------------------------------------------- general symmetry sectors ------
+# GeneralBasis + SymmetryReducer (QMBED src/basis/mod.rs:3330-3650,
+# 4096-4507) as the C ABI builds it (bindings/capi/src/lib.rs:2849-2920,
+# 3103-3129, 3412-3422).
+
+def translation(L, step=1):
+    return [(i + step) % L for i in range(L)]
+
+
+def reflection(L):
+    return [L - 1 - i for i in range(L)]
+
+
+def general(dest, sector, flip=False):
+    q = {"destinations": list(dest), "sector": sector}
+    if flip:
+        q["local_permutations"] = [[1, 0]] * len(dest)
+    return q
+
+
+def perm_matrix(L, dest, flip=False):
+    """U|s> = |g s> on all 2^L states (bit i = site i)."""
+    dim = 2 ** L
+    U = np.zeros((dim, dim))
+    for s in range(dim):
+        t = 0
+        for i in range(L):
+            b = (s >> i) & 1
+            if flip:
+                b ^= 1
+            t |= b << dest[i]
+        U[t, s] = 1.0
+    return U
+
+
+def test_general_translation_matches_builtin_sector(oracle):
+    # the reference's own check (bindings/capi/src/lib.rs:8073-8100): the
+    # runtime translation map [1, 2, 3, 4, 5, 0] at sector 1 describes the
+    # built-in momentum-1 basis of L=6 up=3
+    spec = cases.heisenberg(6)
+    g = oracle.Model(6, spec, up=3, symmetries=[general(translation(6), 1)])
+    b = oracle.Model(6, spec, up=3, momentum=1)
+    assert list(g.states()[0]) == [7, 11, 13] == list(b.states()[0])
+    # and for every momentum of L=8, with equal reduced matrices
+    spec = cases.dm_chain(8)
+    for k in range(8):
+        g = oracle.Model(8, spec, up=4, symmetries=[general(translation(8), k)])
+        b = oracle.Model(8, spec, up=4, momentum=k)
+        np.testing.assert_array_equal(g.states()[0], b.states()[0])
+        np.testing.assert_array_equal(g.states()[1], b.states()[1])
+        np.testing.assert_allclose(g.dense(), b.dense(), atol=1e-13)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("L,up,zflip", [(8, 4, True), (7, 3, False), (8, None, True)])
+def test_general_sectors_union_is_full_spectrum(oracle, L, up, zflip, reverse):
+    # translation, reflection and (at half filling / the full space) spin
+    # inversion: every sector combination together spans the parent space
+    # (translation and reflection commute only at k = 0 and k = L/2, where a
+    # reflection character splits the momentum sector; the other momenta
+    # stay whole)
+    spec = cases.j1j2(L)
+    full = np.linalg.eigvalsh(oracle.Model(L, spec, up=up).dense())
+    parts = []
+    for k in range(L):
+        for p in ((0, 1) if (2 * k) % L == 0 else (None,)):
+            for z in ((0, 1) if zflip else (None,)):
+                gens = [general(translation(L), k)]
+                if p is not None:
+                    gens.append(general(reflection(L), p))
+                if z is not None:
+                    gens.append(general(list(range(L)), z, flip=True))
+                m = oracle.Model(L, spec, up=up, symmetries=gens, reverse=reverse)
+                if m.dim:
+                    d = m.dense()
+                    assert np.allclose(d, d.conj().T, atol=1e-13)
+                    parts.append(np.linalg.eigvalsh(d))
+    np.testing.assert_allclose(np.sort(np.concatenate(parts)), full, atol=1e-11)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("config", ["t3_dm", "t4_p_z_j1j2"])
+def test_general_symmetric_states_carry_the_sector_characters(oracle, reverse, config):
+    # |r~> = N^-1/2 Σ_t phase(t)|t> from the oracle's images: orthonormal,
+    # U_g |r~> = exp(2πi q/p) |r~> for every generator g of period p and
+    # sector q, and <r~_i|H|r~_j> is the reduced matrix
+    L, up = 8, 4
+    if config == "t3_dm":  # complex characters and couplings
+        spec = cases.dm_chain(L)
+        gens = [(translation(L), 3, False, 8)]
+    else:
+        spec = cases.j1j2(L)
+        gens = [(translation(L), 4, False, 8), (reflection(L), 1, False, 2), (list(range(L)), 1, True, 2)]
+    m = oracle.Model(L, spec, up=up, symmetries=[general(d, q, f) for d, q, f, _ in gens], reverse=reverse)
+    states, orbits = m.states()
+    assert m.dim > 0
+    dim = 2 ** L
+    B = np.zeros((dim, m.dim), dtype=complex)
+    for t in range(dim):
+        if bin(t).count("1") != up:
+            continue
+        img = m.image(t)
+        if img is None:
+            continue
+        rep, phase, orb = img
+        i = list(states).index(rep)
+        assert orb == orbits[i]
+        B[t, i] = phase / np.sqrt(orb)
+    np.testing.assert_allclose(B.conj().T @ B, np.eye(m.dim), atol=1e-13)
+    for d, q, f, p in gens:
+        U = perm_matrix(L, d, f)
+        np.testing.assert_allclose(U @ B, np.exp(2j * np.pi * q / p) * B, atol=1e-13)
+    H = full_hamiltonian(L, spec)
+    np.testing.assert_allclose(m.dense(), B.conj().T @ H @ B, atol=1e-13)
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_general_representatives_and_row_order(oracle, reverse):
+    L, up = 10, 5
+    gens = [general(translation(L), 0), general(reflection(L), 0), general(list(range(L)), 0, flip=True)]
+    m = oracle.Model(L, cases.heisenberg(L), up=up, symmetries=gens, reverse=reverse)
+    states = [int(s) for s in m.states()[0]]
+    mask = (1 << L) - 1
+    for r in states:
+        orbit = {rotate(r, k, L) for k in range(L)}
+        orbit |= {reflect(t, L) for t in orbit}
+        orbit |= {t ^ mask for t in orbit}
+        assert r == (max(orbit) if reverse else min(orbit))
+    assert states == sorted(states, reverse=reverse)
+    # without symmetries the reverse convention only reverses the rows
+    plain = oracle.Model(L, cases.heisenberg(L), up=up, reverse=True)
+    asc = oracle.Model(L, cases.heisenberg(L), up=up)
+    np.testing.assert_array_equal(plain.states()[0], asc.states()[0][::-1])
+
+
+def test_general_sector_errors(oracle):
+    spec = cases.heisenberg(6)
+    with pytest.raises(oracle.OracleError, match="expected 6"):
+        oracle.Model(6, spec, up=3, symmetries=[general([1, 0], 0)])
+    with pytest.raises(oracle.OracleError, match="bijection"):
+        oracle.Model(6, spec, up=3, symmetries=[general([1, 1, 2, 3, 4, 5], 0)])
+    with pytest.raises(oracle.OracleError, match="cannot be mixed"):
+        oracle.Model(6, spec, up=3, momentum=1, symmetries=[general(translation(6), 1)])
