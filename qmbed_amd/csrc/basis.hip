@@ -13,8 +13,28 @@
 // without floating-point phases. The orbit size is the number of distinct
 // states carrying a nonzero coefficient: d, or 2d when the reflection maps the
 // translation orbit to a different one.
+//
+// General symmetry sectors (GeneralBasis::from_orbit_seeds_with_ordering +
+// SymmetryReducer, src/basis/mod.rs:3464-3541, 4163-4300, built by the C
+// ABI's build_spin_basis, bindings/capi/src/lib.rs:2849-2920): the group G
+// generated by the maps is closed on the host, every element being
+// s -> P(s) XOR f (a site permutation and a digit flip mask), with its
+// character χ(g) as an integer index modulo the lcm of the generator
+// periods. The reference's breadth-first orbit trace accepts an orbit iff a
+// phase function c(g s) = χ(g) c(s) exists on it; that holds for every orbit
+// or for none when two words for one element carry different phases (then
+// the sector is empty), and otherwise iff χ(h) = 1 for every h fixing the
+// seed. Each parent state s is the seed of its orbit iff no element maps it
+// to a smaller parent state; the representative is the orbit's minimum (or
+// maximum under the reverse convention) and the orbit size |G| / |Stab(s)|.
+// The representatives are then sorted (the parent's ascending rows).
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
+#include <map>
 #include <numeric>
 
 #include "qmbed.hpp"
@@ -151,6 +171,99 @@ __global__ void __launch_bounds__(kThreads) enumerate_kernel(EnumParams p, uint3
   }
 }
 
+// g(s) for a general group element: the bit of site i moves to dest[i], then
+// the flipped digits toggle
+__device__ __forceinline__ uint32_t group_apply(uint32_t s, const uint8_t* __restrict__ dest, uint32_t flip, int L) {
+  uint32_t out = 0;
+  for (int i = 0; i < L; ++i) out |= ((s >> i) & 1u) << dest[i];
+  return out ^ flip;
+}
+
+struct GroupParams {
+  const uint8_t* dest;   // [n][32]
+  const uint32_t* flip;  // [n]
+  const uint8_t* phase;  // [n]: character index of χ(g⁻¹) (0 iff χ(g) = 1)
+  int n;
+  int max_rep;
+};
+
+__device__ __forceinline__ bool in_parent(uint32_t t, const EnumParams& p) {
+  return p.up >= 0 ? __popc(t) == p.up : ((p.popmask >> __popc(t)) & 1ull) != 0;
+}
+
+// s seeds a compatible orbit: returns its orbit size and representative
+// (0 when s is not the seed or the orbit does not survive the projection)
+__device__ int general_rep_orbit(uint32_t s, const EnumParams& p, const GroupParams& gp, uint32_t& rep) {
+  rep = s;
+  int stab = 0;
+  bool ok = true;
+  for (int g = 0; g < gp.n; ++g) {
+    const uint32_t t = group_apply(s, gp.dest + 32 * g, gp.flip[g], p.L);
+    if (t == s) {
+      ++stab;
+      if (gp.phase[g] != 0) ok = false;
+    } else if (t < s && in_parent(t, p)) {
+      ok = false;  // a smaller parent state seeds this orbit
+    }
+    rep = gp.max_rep ? max(rep, t) : min(rep, t);
+  }
+  return ok ? gp.n / stab : 0;
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(kThreads) general_enumerate_kernel(EnumParams p, GroupParams gp,
+                                                                     uint32_t* block_counts,
+                                                                     const uint64_t* block_offsets, uint32_t* states,
+                                                                     uint8_t* orbits) {
+  __shared__ uint32_t scan[kThreads];
+  const uint64_t tid = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const uint64_t first = tid * kChunk;
+  uint32_t count = 0;
+  uint32_t s = 0;
+  int todo = 0;
+  if (first < p.n_parents) {
+    todo = int(min<uint64_t>(kChunk, p.n_parents - first));
+    s = (p.up >= 0) ? unrank_colex(first, p.L, p.up, p.binom) : uint32_t(first);
+    uint32_t t = s, rep;
+    for (int i = 0; i < todo; ++i) {
+      if (in_parent(t, p) && general_rep_orbit(t, p, gp, rep) > 0) ++count;
+      if (i + 1 < todo) t = (p.up >= 0) ? gosper_next(t) : t + 1;
+    }
+  }
+  scan[threadIdx.x] = count;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    uint32_t v = threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+    __syncthreads();
+    scan[threadIdx.x] += v;
+    __syncthreads();
+  }
+  if (!WRITE) {
+    if (threadIdx.x == kThreads - 1) block_counts[blockIdx.x] = scan[kThreads - 1];
+    return;
+  }
+  uint64_t out = block_offsets[blockIdx.x] + scan[threadIdx.x] - count;
+  uint32_t t = s, rep;
+  for (int i = 0; i < todo; ++i) {
+    if (in_parent(t, p)) {
+      const int o = general_rep_orbit(t, p, gp, rep);
+      if (o > 0) {
+        states[out] = rep;
+        orbits[out] = uint8_t(o);
+        ++out;
+      }
+    }
+    if (i + 1 < todo) t = (p.up >= 0) ? gosper_next(t) : t + 1;
+  }
+}
+
+// rows in descending order: out[i] = in[n - 1 - i]
+template <typename T>
+__global__ void reverse_kernel(const T* __restrict__ in, uint64_t n, T* __restrict__ out) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[n - 1 - i];
+}
+
 __global__ void splitter_kernel(const uint32_t* states, uint64_t n, int shift, int count, uint32_t* spl) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
@@ -165,6 +278,127 @@ uint64_t binom_host(int n, int k) {
   return r;
 }
 
+// The symmetry group of a general sector, closed on the host.
+struct GroupElement {
+  std::array<uint8_t, 32> dest{};
+  uint32_t flip = 0;
+  long chi = 0;  // χ(g) = exp(-2πi chi / M)
+};
+
+struct Group {
+  std::vector<GroupElement> elems;
+  long M = 1;
+  bool consistent = true;  // one phase per element
+};
+
+constexpr int kMaxGroup = 255;  // orbit sizes are stored in 8 bits
+constexpr long kMaxCharOrder = 1L << 20;
+constexpr int kMaxChars = 64;   // character-table capacity of the H·v kernels
+
+long gcd_l(long a, long b) {
+  while (b) {
+    const long t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// runtime_symmetry_sector + LatticeSymmetryMap::new for spin one-half
+// (bindings/capi/src/lib.rs:3103-3127; src/basis/mod.rs:2978-3060,
+// 3221-3268): map checks, generator periods, group closure
+Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
+  struct Gen {
+    std::array<uint8_t, 32> dest{};
+    uint32_t flip = 0;  // destination-side flip mask
+    long period = 1;
+    long sector = 0;
+  };
+  std::vector<Gen> gens;
+  for (const auto& m : maps) {
+    if (long(m.destinations.size()) != long(L))
+      throw invalid_options("symmetry map has " + std::to_string(m.destinations.size()) + " sites, expected " +
+                            std::to_string(L));
+    if (m.destinations.empty()) throw invalid_sector("a lattice symmetry requires sites and local states");
+    std::vector<char> seen(size_t(L), 0);
+    for (long d : m.destinations)
+      if (d < 0 || d >= L || seen[size_t(d)]++)
+        throw incompatible_symmetry("symmetry site destinations must be a bijection");
+    std::vector<int> flipped(size_t(L), 0);
+    if (!m.local_permutations.empty()) {
+      if (long(m.local_permutations.size()) != long(L))
+        throw incompatible_symmetry("every local-state map must be a permutation");
+      for (int i = 0; i < L; ++i) {
+        const auto& q = m.local_permutations[size_t(i)];
+        const bool ident = q.size() == 2 && q[0] == 0 && q[1] == 1;
+        const bool swap = q.size() == 2 && q[0] == 1 && q[1] == 0;
+        if (!ident && !swap) throw incompatible_symmetry("every local-state map must be a permutation");
+        flipped[size_t(i)] = swap ? 1 : 0;
+      }
+    }
+    Gen g;
+    for (int i = 0; i < L; ++i) {
+      g.dest[size_t(i)] = uint8_t(m.destinations[size_t(i)]);
+      if (flipped[size_t(i)]) g.flip |= 1u << m.destinations[size_t(i)];
+    }
+    // combined_permutation_period: lcm of the cycles of (site, digit)
+    std::vector<char> visited(size_t(2 * L), 0);
+    long period = 1;
+    for (int seed = 0; seed < 2 * L; ++seed) {
+      if (visited[size_t(seed)]) continue;
+      int cur = seed;
+      long cycle = 0;
+      while (!visited[size_t(cur)]) {
+        visited[size_t(cur)] = 1;
+        ++cycle;
+        const int src = cur / 2, digit = cur % 2;
+        cur = int(m.destinations[size_t(src)]) * 2 + (digit ^ flipped[size_t(src)]);
+      }
+      period = period / gcd_l(period, cycle) * cycle;
+      if (period > kMaxCharOrder)
+        throw unsupported_backend("symmetry map periods beyond 2^20 are not built on the MI355X path");
+    }
+    g.period = period;
+    g.sector = ((long(m.sector) % period) + period) % period;
+    gens.push_back(g);
+  }
+  Group G;
+  for (auto& g : gens) {
+    G.M = G.M / gcd_l(G.M, g.period) * g.period;
+    if (G.M > kMaxCharOrder)
+      throw unsupported_backend("symmetry character orders beyond 2^20 are not built on the MI355X path");
+  }
+  // breadth-first closure from the identity: h = gen ∘ e, i.e.
+  // h(s) = P_gen(P_e(s) ^ f_e) ^ f_gen
+  std::map<std::pair<std::array<uint8_t, 32>, uint32_t>, size_t> index;
+  GroupElement id;
+  for (int i = 0; i < L; ++i) id.dest[size_t(i)] = uint8_t(i);
+  G.elems.push_back(id);
+  index[{id.dest, id.flip}] = 0;
+  for (size_t head = 0; head < G.elems.size(); ++head) {
+    for (const auto& g : gens) {
+      const GroupElement e = G.elems[head];
+      GroupElement h;
+      for (int i = 0; i < L; ++i) h.dest[size_t(i)] = g.dest[e.dest[size_t(i)]];
+      uint32_t pf = 0;
+      for (int i = 0; i < L; ++i)
+        if ((e.flip >> i) & 1u) pf |= 1u << g.dest[size_t(i)];
+      h.flip = pf ^ g.flip;
+      h.chi = (e.chi + g.sector * (G.M / g.period)) % G.M;
+      auto it = index.find({h.dest, h.flip});
+      if (it != index.end()) {
+        if (G.elems[it->second].chi != h.chi) G.consistent = false;
+        continue;
+      }
+      if (G.elems.size() >= size_t(kMaxGroup))
+        throw unsupported_backend("general symmetry groups of more than 255 elements are not built on the MI355X path");
+      index[{h.dest, h.flip}] = G.elems.size();
+      G.elems.push_back(h);
+    }
+  }
+  return G;
+}
+
 }  // namespace
 
 std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hipStream_t stream) {
@@ -172,6 +406,8 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   if (spec.spin_twice == 0) throw invalid_sector("spin_twice must be positive");
   if (spec.normalization != Normalization::AngularMomentum && spec.spin_twice != 1)
     throw invalid_options("the Pauli convention is defined only for spin one-half");
+  if (!spec.symmetries.empty() && (spec.momentum || spec.parity))
+    throw invalid_options("built-in and general spin symmetries cannot be mixed");
   if (spec.spin_twice != 1)
     throw unsupported_backend("the MI355X spin engine implements spin one-half sectors");
   const int L = spec.sites;
@@ -191,6 +427,9 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   if (spec.parity && kmom > 0 && 2 * kmom != L)
     throw incompatible_symmetry("parity can share a one-dimensional sector with momentum only at k=0 or k=pi");
   if (L == 0) throw unsupported_backend("empty lattices are not supported by the MI355X spin engine");
+  // the general sector's maps are checked and closed before any device work
+  Group G;
+  if (!spec.symmetries.empty()) G = close_group(spec.symmetries, L);
 
   auto b = std::shared_ptr<SpinBasis1D>(new SpinBasis1D());
   b->spec_ = spec;
@@ -230,12 +469,71 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   QMBED_HIP(hipMemcpyAsync(d_binom.data(), binom.data(), binom.size() * 8, hipMemcpyHostToDevice, stream));
   p.binom = d_binom.data();
 
-  const uint64_t threads = (p.n_parents + kChunk - 1) / kChunk;
+  // general symmetry sector: the group, its character table and the
+  // device copies of its elements
+  GroupParams gp{};
+  bool group_empty = false;
+  if (!spec.symmetries.empty()) {
+    group_empty = !G.consistent;
+    const long M = G.M;
+    // distinct values of χ(g⁻¹) = exp(-2πi (M - chi) / M), 1 first
+    std::vector<long> vals;
+    for (auto& e : G.elems) vals.push_back((M - e.chi) % M);
+    std::sort(vals.begin(), vals.end());
+    vals.erase(std::unique(vals.begin(), vals.end()), vals.end());
+    if (vals.size() > size_t(kMaxChars))
+      throw unsupported_backend("general symmetry sectors with more than 64 character values are not built on the "
+                                "MI355X path");
+    const double tau = 6.283185307179586;
+    b->chars_.clear();
+    b->real_chars_ = true;
+    for (long c : vals) {
+      cplx v;
+      if (c == 0) v = cplx(1, 0);
+      else if (2 * c == M) v = cplx(-1, 0);
+      else if (4 * c == M) v = cplx(0, -1);
+      else if (4 * c == 3 * M) v = cplx(0, 1);
+      else v = std::polar(1.0, -tau * double(c) / double(M));
+      if (v.imag() != 0.0) b->real_chars_ = false;
+      b->chars_.push_back(v);
+    }
+    const size_t n = G.elems.size();
+    std::vector<uint8_t> dest(32 * n, 0), phase(n, 0);
+    std::vector<uint32_t> flip(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+      std::copy(G.elems[i].dest.begin(), G.elems[i].dest.end(), dest.begin() + long(32 * i));
+      flip[i] = G.elems[i].flip;
+      const long c = (M - G.elems[i].chi) % M;
+      phase[i] = uint8_t(std::lower_bound(vals.begin(), vals.end(), c) - vals.begin());
+    }
+    b->gdest_.resize(dest.size());
+    b->gflip_.resize(n);
+    b->gphase_.resize(n);
+    QMBED_HIP(hipMemcpyAsync(b->gdest_.data(), dest.data(), dest.size(), hipMemcpyHostToDevice, stream));
+    QMBED_HIP(hipMemcpyAsync(b->gflip_.data(), flip.data(), n * 4, hipMemcpyHostToDevice, stream));
+    QMBED_HIP(hipMemcpyAsync(b->gphase_.data(), phase.data(), n, hipMemcpyHostToDevice, stream));
+    g.n_group = int(n);
+    g.n_chars = int(vals.size());
+    g.max_rep = spec.reverse;
+    gp.dest = b->gdest_.data();
+    gp.flip = b->gflip_.data();
+    gp.phase = b->gphase_.data();
+    gp.n = int(n);
+    gp.max_rep = spec.reverse ? 1 : 0;
+  }
+  g.reversed = spec.reverse;
+
+  const uint64_t threads = group_empty ? 0 : (p.n_parents + kChunk - 1) / kChunk;
   const uint64_t blocks = std::max<uint64_t>(1, (threads + kThreads - 1) / kThreads);
   DeviceArray<uint32_t> d_counts(blocks);
   DeviceArray<uint64_t> d_offsets(blocks);
-  hipLaunchKernelGGL(enumerate_kernel<false>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, d_counts.data(),
-                     nullptr, nullptr, nullptr);
+  if (group_empty) p.n_parents = 0;  // every orbit trace conflicts: an empty sector
+  if (g.n_group)
+    hipLaunchKernelGGL(general_enumerate_kernel<false>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, gp,
+                       d_counts.data(), nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL(enumerate_kernel<false>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p,
+                       d_counts.data(), nullptr, nullptr, nullptr);
   QMBED_HIP(hipGetLastError());
   std::vector<uint32_t> counts(blocks);
   QMBED_HIP(hipMemcpyAsync(counts.data(), d_counts.data(), blocks * 4, hipMemcpyDeviceToHost, stream));
@@ -252,9 +550,42 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   QMBED_HIP(hipMemsetAsync(b->states_.data(), 0xFF, padded * 4, stream));
   b->orbits_.resize(std::max<uint64_t>(total, 1));
   QMBED_HIP(hipMemcpyAsync(d_offsets.data(), offsets.data(), blocks * 8, hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, nullptr,
-                     d_offsets.data(), b->states_.data(), b->orbits_.data());
-  QMBED_HIP(hipGetLastError());
+  if (g.n_group) {
+    hipLaunchKernelGGL(general_enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, gp,
+                       nullptr, d_offsets.data(), b->states_.data(), b->orbits_.data());
+    QMBED_HIP(hipGetLastError());
+    // the representatives in ascending order (seeds are ascending, their
+    // orbit maxima, or minima outside the parent, need not be)
+    if (total > 1) {
+      if (total >= (uint64_t(1) << 31)) throw unsupported_backend("general symmetry sector too large to sort");
+      DeviceArray<uint32_t> keys(total);
+      DeviceArray<uint8_t> vals(total);
+      QMBED_HIP(hipMemcpyAsync(keys.data(), b->states_.data(), total * 4, hipMemcpyDeviceToDevice, stream));
+      QMBED_HIP(hipMemcpyAsync(vals.data(), b->orbits_.data(), total, hipMemcpyDeviceToDevice, stream));
+      size_t tmp_bytes = 0;
+      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.data(), b->states_.data(), vals.data(),
+                                                   b->orbits_.data(), int(total), 0, L, stream));
+      DeviceArray<uint8_t> tmp(std::max<size_t>(tmp_bytes, 1));
+      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, keys.data(), b->states_.data(),
+                                                   vals.data(), b->orbits_.data(), int(total), 0, L, stream));
+      QMBED_HIP(hipStreamSynchronize(stream));
+    }
+  } else {
+    hipLaunchKernelGGL(enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, nullptr,
+                       d_offsets.data(), b->states_.data(), b->orbits_.data());
+    QMBED_HIP(hipGetLastError());
+  }
+  if (g.reversed && total > 0) {
+    b->rstates_.resize(padded);
+    QMBED_HIP(hipMemsetAsync(b->rstates_.data(), 0xFF, padded * 4, stream));
+    b->rorbits_.resize(total);
+    const unsigned rb = unsigned((total + 255) / 256);
+    hipLaunchKernelGGL(reverse_kernel<uint32_t>, dim3(rb), dim3(256), 0, stream, b->states_.data(), total,
+                       b->rstates_.data());
+    hipLaunchKernelGGL(reverse_kernel<uint8_t>, dim3(rb), dim3(256), 0, stream, b->orbits_.data(), total,
+                       b->rorbits_.data());
+    QMBED_HIP(hipGetLastError());
+  }
 
   // search splitters for the LDS top of the sorted-state lookup
   int shift = 0;
@@ -303,6 +634,18 @@ std::vector<uint32_t> SpinBasis1D::states_host() const {
 std::vector<uint8_t> SpinBasis1D::orbits_host() const {
   std::vector<uint8_t> h(n_);
   if (n_) QMBED_HIP(hipMemcpy(h.data(), orbits_.data(), n_, hipMemcpyDeviceToHost));
+  return h;
+}
+
+std::vector<uint32_t> SpinBasis1D::row_states_host() const {
+  std::vector<uint32_t> h = states_host();
+  if (reversed()) std::reverse(h.begin(), h.end());
+  return h;
+}
+
+std::vector<uint8_t> SpinBasis1D::row_orbits_host() const {
+  std::vector<uint8_t> h = orbits_host();
+  if (reversed()) std::reverse(h.begin(), h.end());
   return h;
 }
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -134,15 +135,35 @@ SpinBasisSpec parse_basis(const Value& b) {
   } else {
     s.normalization = as_bool(b.get("pauli"), false) ? Normalization::PauliCartesian : Normalization::AngularMomentum;
   }
+  // SymmetryRequest {destinations, local_permutations?, sector} (capi
+  // :906-911); checked against the lattice when the sector is built
   const Value* sym = b.get("symmetries");
-  if (sym && !sym->is_null() && !sym->array().empty()) {
-    if (s.momentum || s.parity) bad("built-in and general spin symmetries cannot be mixed");
-    throw unsupported_backend("general `symmetries` sectors are not built on the MI355X path");
+  if (sym && !sym->is_null()) {
+    for (auto& q : sym->array()) {
+      if (q.kind != Value::Obj) bad("invalid binding request: symmetries must hold objects");
+      SymmetryMapSpec m;
+      for (auto& d : need(q, "destinations", "symmetry").array()) {
+        const uint64_t u = as_u64(d, "destinations");
+        m.destinations.push_back(u > uint64_t(LONG_MAX) ? LONG_MAX : long(u));
+      }
+      if (auto* lp = q.get("local_permutations"); lp && !lp->is_null()) {
+        for (auto& perm : lp->array()) {
+          std::vector<long> digits;
+          for (auto& d : perm.array()) {
+            const uint64_t u = as_u64(d, "local_permutations");
+            digits.push_back(u > uint64_t(LONG_MAX) ? LONG_MAX : long(u));
+          }
+          m.local_permutations.push_back(std::move(digits));
+        }
+      }
+      m.sector = as_int(need(q, "sector", "symmetry"), "sector");
+      s.symmetries.push_back(std::move(m));
+    }
   }
+  if (!s.symmetries.empty() && (s.momentum || s.parity)) bad("built-in and general spin symmetries cannot be mixed");
   if (auto* v = b.get("matrix_symmetry"); v && !v->is_null())
     bad("matrix-valued symmetry sectors materialize as projected operator models");
-  if (as_bool(b.get("reverse"), false))
-    throw unsupported_backend("reverse representative ordering is not built on the MI355X path");
+  s.reverse = as_bool(b.get("reverse"), false);
   return s;
 }
 
@@ -534,7 +555,7 @@ std::string run_command(const std::string& req) {
   if (op == "describe_basis") {
     SpinBasisSpec spec = parse_basis(need(v, "basis", "describe_basis"));
     auto b = SpinBasis1D::build(spec, nullptr);
-    auto states = b->states_host();
+    auto states = b->row_states_host();
     std::string o = "{\"kind\":\"basis\",\"dimension\":" + std::to_string(b->len()) + ",\"states\":[";
     for (size_t i = 0; i < states.size(); ++i) {
       if (i) o += ',';
@@ -881,10 +902,10 @@ int qmbed_hip_model_is_real(const qmbed_hip_model* m) {
 int qmbed_hip_model_states(const qmbed_hip_model* m, uint32_t* states, uint8_t* orbits) {
   try {
     auto& b = *reinterpret_cast<const TypedModel*>(m)->m->basis;
-    auto s = b.states_host();
+    auto s = b.row_states_host();
     std::memcpy(states, s.data(), s.size() * 4);
     if (orbits) {
-      auto o = b.orbits_host();
+      auto o = b.row_orbits_host();
       std::memcpy(orbits, o.data(), o.size());
     }
     return 0;

@@ -32,10 +32,13 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxGroups = 128;
 constexpr int kMaxAmps = 1024;
+constexpr int kMaxChars = 64;  // character-table entries staged in LDS
 
 struct HvArgs {
-  const uint32_t* states;
-  const uint8_t* orbits;
+  const uint32_t* states;   // ascending representatives (the search table)
+  const uint32_t* rstates;  // representatives in row order
+  const uint8_t* orbits;    // orbit sizes in row order
+  int rev;                  // rows descending: row = n - 1 - ascending index
   uint32_t n;
   int L;
   uint32_t mask;
@@ -68,7 +71,15 @@ struct HvArgs {
   const double* amp;     // real: 1 double per entry; complex: 2 doubles per entry
   int n_diag, n_off, n_amp;
   int max_sites;         // the most sites of any group (at most 5)
-  const double* chars;   // 2*L doubles (re, im) of χ(a) = exp(-2πi k a / L)
+  const double* chars;   // 2*nchar doubles (re, im): χ(a) = exp(-2πi k a / L), or the
+                         // general sector's character values
+  int nchar;
+  // general symmetry group (ngroup elements; 0: built-in symmetries)
+  const uint8_t* gdest;   // [ngroup][32] site destinations
+  const uint32_t* gflip;  // destination-side flip masks
+  const uint8_t* gphase;  // character index of χ(g⁻¹)
+  int ngroup;
+  int maxrep;             // representatives are orbit maxima
   // batched columns
   int nvec;
   size_t ld;
@@ -405,6 +416,28 @@ __device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep,
   }
 }
 
+// General symmetry group: rep = min (or max) of g(t) over the elements, the
+// orbit size |G| / |Stab(t)| and the phase χ(h⁻¹) of the first element h
+// with h(t) = rep (t = h⁻¹ rep). Elements whose images tie differ by a
+// stabilizer element, whose character is 1 on every orbit of the sector.
+__device__ __forceinline__ void general_walk(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci) {
+  rep = t;
+  int best = 0, stab = 0;
+  for (int g = 0; g < a.ngroup; ++g) {
+    const uint8_t* d = a.gdest + 32 * g;
+    uint32_t u = 0;
+    for (int i = 0; i < a.L; ++i) u |= ((t >> i) & 1u) << d[i];
+    u ^= a.gflip[g];
+    stab += u == t ? 1 : 0;
+    if (a.maxrep ? u > rep : u < rep) {
+      rep = u;
+      best = g;
+    }
+  }
+  orbit = a.ngroup / max(stab, 1);
+  ci = a.gphase[best];
+}
+
 template <typename S>
 struct Cx {
   S re, im;
@@ -418,8 +451,8 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   if (a.gate && *a.gate) return;
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
-  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
-  double* s_sq = s_chr + 2 * 32;                           // 65
+  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*nchar
+  double* s_sq = s_chr + 2 * kMaxChars;                    // 65
   double* s_rsq = s_sq + 65;                               // 65
   double* s_crsq = s_rsq + 65;                             // 16 (image mode)
   uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_crsq + 16);
@@ -440,7 +473,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
     for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
   }
   if (SYM == 1 || SYM == 3)
-    for (int i = threadIdx.x; i < 2 * a.L; i += kBlock) s_chr[i] = a.chars[i];
+    for (int i = threadIdx.x; i < 2 * a.nchar; i += kBlock) s_chr[i] = a.chars[i];
   for (int i = threadIdx.x; i < 65; i += kBlock) {
     s_sq[i] = sqrt(double(i));
     s_rsq[i] = i ? 1.0 / sqrt(double(i)) : 0.0;
@@ -451,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   const uint32_t row = gid / uint32_t(a.lpr);
   const int sub = int(gid % uint32_t(a.lpr));
   if (row >= a.n) return;  // all lanes of a row leave together
-  const uint32_t r = a.states[row];
+  const uint32_t r = a.rstates[row];
   S sq_r = 1;
   if (SYM) sq_r = S(s_sq[a.orbits[row]]);
   uint32_t rank_r = 0;
@@ -740,8 +773,11 @@ __device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_ne
   } else if (sym == 3) {
     if (shift_needed) walk<3, true>(t, a, rep, orbit, ci, sign);
     else walk<3, false>(t, a, rep, orbit, ci, sign);
+  } else if (sym == 4) {
+    general_walk(t, a, rep, orbit, ci);
   }
   idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
+  if (idx >= 0 && a.rev) idx = int(a.n) - 1 - idx;
   return idx >= 0;
 }
 
@@ -752,7 +788,7 @@ __device__ __forceinline__ void column_value(const HvArgs& a, const double* famp
   const double scale = double(sign) * (sym ? sqrt(nsrc / double(orbit)) : 1.0);
   cre *= scale;
   cim *= scale;
-  if (sym == 1 || sym == 3) {
+  if (sym == 1 || sym == 3 || sym == 4) {
     const double pr = a.chars[2 * ci], pi = -a.chars[2 * ci + 1];
     re = cre * pr - cim * pi;
     im = cre * pi + cim * pr;
@@ -765,7 +801,7 @@ __device__ __forceinline__ void column_value(const HvArgs& a, const double* famp
 template <int CAP>
 __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
                                const uint32_t* spl, uint32_t col, ColList<CAP>& out) {
-  const uint32_t r = a.states[col];
+  const uint32_t r = a.rstates[col];
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   for (int g = 0; g < a.n_diag; ++g) {
     const int off = int(meta[4 * g + 3]) + pattern_of(r, meta[4 * g + 1]);
@@ -805,7 +841,7 @@ __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const doub
 // merged entry H[row, col] (all contributions of column col landing on row)
 __device__ void entry_at(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
                          const uint32_t* spl, uint32_t col, uint32_t row, double& vre, double& vim) {
-  const uint32_t r = a.states[col];
+  const uint32_t r = a.rstates[col];
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   vre = 0;
   vim = 0;
@@ -910,7 +946,7 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double*
                                                          const double2* __restrict__ x, double* y) {
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
-  const uint32_t r = a.states[col];
+  const uint32_t r = a.rstates[col];
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   const double2 xc = x[col];
   double dre = 0, dim_ = 0;
@@ -1031,7 +1067,7 @@ __global__ void __launch_bounds__(kBlock) stream_count_kernel(HvArgs a, uint32_t
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   int cnt = 0;
   if (row < a.n) {
-    ActiveGroups ag(a.meta, a.n_diag, a.n_off, a.states[row]);
+    ActiveGroups ag(a.meta, a.n_diag, a.n_off, a.rstates[row]);
     for (int w = 0; w < a.n_off; w += 32) cnt += __popc(ag.window(w));
   }
   for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o, 64));
@@ -1048,7 +1084,7 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
   const uint32_t width = (slice_ptr[slice + 1] - base) / kSlice;
   uint32_t k = 0;
   if (row < a.n) {
-    const uint32_t r = a.states[row];
+    const uint32_t r = a.rstates[row];
     ActiveGroups ag(a.meta, a.n_diag, a.n_off, r);
     for (int g = ag.next(); g >= 0; g = ag.next()) {
       uint32_t e = 0xFFFFFFFFu;
@@ -1093,8 +1129,8 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
   auto at = [&](int v, uint32_t i) -> size_t { return IL ? size_t(i) * a.ld + v : size_t(v) * a.ld + i; };
   extern __shared__ double smem_d[];
   double* s_amp = smem_d;                                  // n_amp or 2*n_amp
-  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*L
-  double* s_sq = s_chr + 2 * 32;                           // 65
+  double* s_chr = s_amp + (CMODE == 2 ? 2 : 1) * a.n_amp;  // 2*nchar
+  double* s_sq = s_chr + 2 * kMaxChars;                    // 65
   double* s_crsq = s_sq + 65;                              // 16
   uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_crsq + 16);
   const int G = a.n_diag + a.n_off;
@@ -1103,7 +1139,7 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
   for (int i = threadIdx.x; i < 4 * G; i += kBlock) s_meta[i] = a.meta[i];
   for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
   if (SYM == 1 || SYM == 3)
-    for (int i = threadIdx.x; i < 2 * a.L; i += kBlock) s_chr[i] = a.chars[i];
+    for (int i = threadIdx.x; i < 2 * a.nchar; i += kBlock) s_chr[i] = a.chars[i];
   for (int i = threadIdx.x; i < 65; i += kBlock) s_sq[i] = sqrt(double(i));
   __syncthreads();
 
@@ -1116,9 +1152,12 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
   const uint32_t width = (a.slice_ptr[slice + 1] - base) / kSlice;
   // the state itself is needed only to walk the groups
   const bool walk = !(a.compact && a.diag);
-  const uint32_t r = (live && walk) ? a.states[row] : 0u;
+  const uint32_t r = (live && walk) ? a.rstates[row] : 0u;
   S sq_r = 1;
-  if (SYM && live) sq_r = S(s_sq[a.orbits[row]]);
+  if (SYM && live) {
+    const int o = a.orbits[row];
+    sq_r = S(o <= 64 ? s_sq[o] : sqrt(double(o)));  // general groups reach 255
+  }
 
   Cx<S> acc[NV];
   {
@@ -1540,7 +1579,7 @@ __global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp,
   __syncthreads();
   const int G = a.n_diag;
   for (size_t row = size_t(blockIdx.x) * kBlock + threadIdx.x; row < a.n; row += size_t(gridDim.x) * kBlock) {
-    const uint32_t r = a.states[row];
+    const uint32_t r = a.rstates[row];
     double re0 = 0, re1 = 0, im0 = 0, im1 = 0;
     int g = 0;
     for (; g + 2 <= G; g += 2) {
@@ -1601,12 +1640,26 @@ __global__ void __launch_bounds__(kBlock) diag_combine_kernel(const double* __re
   }
 }
 
+// The symmetry kind of the orbit walks (target_of): 0 none, 1 translation,
+// 2 reflection, 3 both, 4 a general group
 int sym_code(const SpinBasis1D& b) {
   const SectorGeometry& g = b.geometry();
+  if (g.n_group) return 4;
   if (g.kmom >= 0 && g.parity) return 3;
   if (g.kmom >= 0) return 1;
   if (g.parity) return 2;
   return 0;
+}
+
+// The symmetry kind of the stream kernels, which read orbit codes, signs
+// and character indices from the words: a general group has no separate
+// sign, so its words decode like a translation sector's
+int kernel_sym(const SpinBasis1D& b) { return b.general() ? 1 : sym_code(b); }
+
+// the target words carry a character index
+bool phased(const SpinBasis1D& b) {
+  const SectorGeometry& g = b.geometry();
+  return g.n_group ? g.n_chars > 1 : g.kmom > 0;
 }
 
 }  // namespace
@@ -1657,7 +1710,15 @@ struct Operator::DeviceTables {
     HvArgs a{};
     const SectorGeometry& g = b.geometry();
     a.states = b.states();
-    a.orbits = b.orbits();
+    a.rstates = b.row_states();
+    a.orbits = b.row_orbits();
+    a.rev = b.reversed() ? 1 : 0;
+    a.ngroup = g.n_group;
+    a.gdest = b.group_dest();
+    a.gflip = b.group_flip();
+    a.gphase = b.group_phase();
+    a.maxrep = g.max_rep ? 1 : 0;
+    a.nchar = g.n_group ? g.n_chars : (g.kmom >= 0 ? g.L : 1);
     a.n = uint32_t(b.len());
     a.L = g.L;
     a.mask = g.mask;
@@ -1711,7 +1772,7 @@ static bool sector_matrix_hermitian(const Operator& op, hipStream_t s) {
   DeviceArray<int> flags(2);
   QMBED_HIP(hipMemsetAsync(flags.data(), 0, 2 * sizeof(int), s));
   hipLaunchKernelGGL(hermiticity_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
-                     op.tables_->famp.data(), sym_code(b), b.geometry().kmom > 0 ? 1 : 0, flags.data());
+                     op.tables_->famp.data(), sym_code(b), phased(b) ? 1 : 0, flags.data());
   QMBED_HIP(hipGetLastError());
   int h[2] = {0, 0};
   QMBED_HIP(hipMemcpyAsync(h, flags.data(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1846,8 +1907,14 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   }
   if (n_amp > kMaxAmps) throw unsupported_backend("the operator's amplitude tables exceed the H·v kernel's LDS budget");
   t->n_amp = n_amp;
-  std::vector<double> chars(64, 0.0);
-  if (geo.kmom >= 0) {
+  std::vector<double> chars(2 * kMaxChars, 0.0);
+  if (geo.n_group) {
+    const auto& cv = basis_->characters();
+    for (size_t i = 0; i < cv.size() && i < size_t(kMaxChars); ++i) {
+      chars[2 * i] = cv[i].real();
+      chars[2 * i + 1] = cv[i].imag();
+    }
+  } else if (geo.kmom >= 0) {
     const double tau = 6.283185307179586;
     for (int s = 0; s < geo.L; ++s) {
       double angle = -tau * double(geo.kmom * s) / double(geo.L);
@@ -1865,9 +1932,9 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
     chars[0] = 1.0;
   }
   t->meta.resize(std::max<size_t>(meta.size(), 4));
-  t->chars.resize(64);
+  t->chars.resize(chars.size());
   if (!meta.empty()) QMBED_HIP(hipMemcpy(t->meta.data(), meta.data(), meta.size() * 4, hipMemcpyHostToDevice));
-  QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), 64 * 8, hipMemcpyHostToDevice));
+  QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), chars.size() * 8, hipMemcpyHostToDevice));
   op->tables_ = t;
   if (!family) {
     upload_amplitudes(*op, op->diag_, op->off_, nullptr);
@@ -1993,7 +2060,7 @@ namespace {
 
 size_t hv_smem(const HvArgs& a, int cmode, bool img) {
   const size_t G = size_t(a.n_diag + a.n_off);
-  return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 130 + 16) +
+  return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 2 * kMaxChars + 130 + 16) +
          sizeof(uint32_t) * (4 * G + G + (img ? 1024 : size_t(a.nspl)));
 }
 
@@ -2031,7 +2098,8 @@ void launch_hv_stream(const HvArgs& a, int sym, bool shift, const SX* x, S* y, h
   const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
   dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t G = size_t(a.n_diag + a.n_off);
-  const size_t sm = sizeof(double) * ((CMODE == 2 ? 2 : 1) * size_t(a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G;
+  const size_t sm =
+      sizeof(double) * ((CMODE == 2 ? 2 : 1) * size_t(a.n_amp) + 2 * kMaxChars + 65 + 16) + sizeof(uint32_t) * 4 * G;
 #define QMBED_HVS_CASE(SYMV, SH) \
   hipLaunchKernelGGL((hv_stream_kernel<S, CMODE, SYMV, SH, NV, SX, IL>), grid, block, sm, s, a, x, y);
   switch (sym) {
@@ -2059,6 +2127,8 @@ void launch_hv_stream(const HvArgs& a, int sym, bool shift, const SX* x, S* y, h
 template <typename S, int CMODE, int NV>
 void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
   if (a.n == 0) return;
+  if (!a.stream && (a.ngroup || a.rev))
+    throw internal_state("general and reversed sectors are applied through the compiled stream only");
   if (a.stream) launch_hv_stream<S, CMODE, NV>(a, sym, shift, x, y, s);
   else if (a.img) launch_hv_mode<S, CMODE, NV, true>(a, sym, shift, x, y, s);
   else launch_hv_mode<S, CMODE, NV, false>(a, sym, shift, x, y, s);
@@ -2086,10 +2156,19 @@ static bool prepare_codes(Operator& op, std::vector<uint8_t>& code) {
     }
   if (!b.symmetric()) sizes = {1};
   if (g.kmom < 0 && g.parity) sizes = {1, 2};
+  if (b.general()) {  // the orbit sizes the sector holds
+    sizes.clear();
+    std::vector<uint8_t> o = b.orbits_host();
+    std::vector<char> seen(256, 0);
+    for (uint8_t v : o) seen[v] = 1;
+    for (int v = 1; v < 256; ++v)
+      if (seen[size_t(v)]) sizes.push_back(v);
+    if (sizes.empty()) sizes = {1};
+  }
   std::sort(sizes.begin(), sizes.end());
   sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
   if (sizes.size() > 16) return false;
-  code.assign(64, 0);
+  code.assign(256, 0);
   std::vector<double> crsq(16, 0.0);
   for (size_t i = 0; i < sizes.size(); ++i) {
     code[size_t(sizes[i])] = uint8_t(i);
@@ -2131,11 +2210,14 @@ static bool build_stream(Operator& op, hipStream_t s) {
     t.c_ob = bits_for(size_t(std::max(t.n_amp - base, 1)));
     t.c_cb = bits_for(size_t(t.n_codes));
     t.c_sb = g.parity ? 1 : 0;
-    const int chb = g.kmom > 0 ? bits_for(size_t(g.L)) : 0;
+    const int chb = phased(b) ? bits_for(size_t(g.n_group ? g.n_chars : g.L)) : 0;
     // QMBED_HV_STREAM_FORMAT=groups keeps the group-walk words everywhere
     const char* fmt = std::getenv("QMBED_HV_STREAM_FORMAT");
     const bool force_groups = fmt && std::string(fmt) == "groups";
     t.compact = !force_groups && t.c_b + t.c_ob + t.c_cb + t.c_sb + chb <= 32;
+    // a general group's words are compact only (its character indices may
+    // exceed the group-walk word's 5 bits)
+    if (b.general() && !t.compact) return false;
   }
   const uint32_t n_slices = uint32_t((n + kSlice - 1) / kSlice);
   HvArgs a = t.args(b);
@@ -2160,11 +2242,11 @@ static bool build_stream(Operator& op, hipStream_t s) {
   t.slice_ptr.resize(ptr.size());
   QMBED_HIP(hipMemcpy(t.slice_ptr.data(), ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
   t.stream.resize(std::max<uint64_t>(total, 1));
-  DeviceArray<uint8_t> dcode(64);
-  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 64, hipMemcpyHostToDevice));
+  DeviceArray<uint8_t> dcode(256);
+  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 256, hipMemcpyHostToDevice));
   a = t.args(b);
-  hipLaunchKernelGGL(stream_fill_kernel, dim3(grid), dim3(kBlock), 0, s, a, sym_code(b),
-                     b.geometry().kmom > 0 ? 1 : 0, n_slices, dcode.data(), t.slice_ptr.data(), t.stream.data());
+  hipLaunchKernelGGL(stream_fill_kernel, dim3(grid), dim3(kBlock), 0, s, a, sym_code(b), phased(b) ? 1 : 0, n_slices,
+                     dcode.data(), t.slice_ptr.data(), t.stream.data());
   QMBED_HIP(hipGetLastError());
   QMBED_HIP(hipStreamSynchronize(s));
   return true;
@@ -2205,6 +2287,9 @@ static bool build_image_table(Operator& op, hipStream_t s) {
       }
       hop.push_back(h);
     }
+  // the group walk of a general sector and the row reversal are served by
+  // the compiled stream only
+  if (b.general() || b.reversed()) return false;
   std::vector<uint8_t> code;
   if (!prepare_codes(op, code)) return false;
   const bool fixed_up = spec.up.has_value() && spec.up_sectors.empty();
@@ -2225,15 +2310,15 @@ static bool build_image_table(Operator& op, hipStream_t s) {
   if (parents * 4 > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
   t.binom.resize(1024);
   t.hop.resize(std::max<size_t>(hop.size(), 1));
-  DeviceArray<uint8_t> dcode(64);
+  DeviceArray<uint8_t> dcode(256);
   QMBED_HIP(hipMemcpy(t.binom.data(), binom.data(), 4096, hipMemcpyHostToDevice));
   if (!hop.empty()) QMBED_HIP(hipMemcpy(t.hop.data(), hop.data(), hop.size() * 4, hipMemcpyHostToDevice));
-  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 64, hipMemcpyHostToDevice));
+  QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 256, hipMemcpyHostToDevice));
   t.img.resize(parents);
   HvArgs a = t.args(b);
   const uint64_t threads = (parents + kImgChunk - 1) / kImgChunk;
   hipLaunchKernelGGL(image_kernel, dim3(unsigned((threads + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, a, sym_code(b),
-                     g.kmom > 0 ? 1 : 0, popmask, uint32_t(parents), dcode.data(), t.img.data());
+                     phased(b) ? 1 : 0, popmask, uint32_t(parents), dcode.data(), t.img.data());
   QMBED_HIP(hipGetLastError());
   QMBED_HIP(hipStreamSynchronize(s));
   return true;
@@ -2260,16 +2345,24 @@ static void tune_lanes(Operator& op) {
   // candidates
   const char* mode_env = std::getenv("QMBED_HV_MODE");
   const std::string mode = mode_env ? mode_env : "auto";
-  const bool want_walk = mode == "auto" || mode == "walk";
+  // general groups and reversed rows run on the compiled stream only
+  const bool stream_only = b.general() || b.reversed();
+  const bool want_walk = (mode == "auto" || mode == "walk") && !stream_only;
   const bool have_img = (mode == "auto" || mode == "image") && build_image_table(op, s);
-  const bool have_stream = (mode == "auto" || mode == "stream") && build_stream(op, s);
+  const bool have_stream = (mode == "auto" || mode == "stream" || stream_only) && build_stream(op, s);
+  if (stream_only && !have_stream) {
+    (void)hipStreamDestroy(s);
+    throw unsupported_backend(b.general()
+                                  ? "the general symmetry sector does not fit the compiled index stream"
+                                  : "the reversed sector does not fit the compiled index stream");
+  }
   DeviceArray<double> x(2 * n), y(2 * n);
   QMBED_HIP(hipMemsetAsync(x.data(), 0, 2 * n * 8, s));
   hipEvent_t e0, e1;
   QMBED_HIP(hipEventCreate(&e0));
   QMBED_HIP(hipEventCreate(&e1));
-  const int sym = sym_code(b);
-  const bool sh = b.geometry().kmom > 0;
+  const int sym = kernel_sym(b);
+  const bool sh = phased(b);
   float best = 1e30f, dflt = 1e30f;
   int best_lpr = 1, best_mode = 0;  // 0 walk, 1 image, 2 stream
   int dflt_mode = -1;
@@ -2338,7 +2431,7 @@ static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_
   if (a.n == 0) return;
   QMBED_HIP(hipMemsetAsync(y, 0, size_t(a.n) * sizeof(cplx), s));
   hipLaunchKernelGGL(scatter_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
-                     op.tables_->famp.data(), sym_code(b), b.geometry().kmom > 0 ? 1 : 0,
+                     op.tables_->famp.data(), sym_code(b), phased(b) ? 1 : 0,
                      reinterpret_cast<const double2*>(x), reinterpret_cast<double*>(y));
   QMBED_HIP(hipGetLastError());
 }
@@ -2347,14 +2440,14 @@ void Operator::apply(const double* x, double* y, hipStream_t s, const int* gate)
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
-  launch_hv<double, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
+  launch_hv<double, 0, 1>(a, kernel_sym(*basis_), phased(*basis_), x, y, s);
 }
 
 void Operator::apply(const float* x, float* y, hipStream_t s, const int* gate) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
-  launch_hv<float, 0, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, x, y, s);
+  launch_hv<float, 0, 1>(a, kernel_sym(*basis_), phased(*basis_), x, y, s);
 }
 
 void Operator::apply(const cplx* x, cplx* y, hipStream_t s, const int* gate) const {
@@ -2366,8 +2459,8 @@ void Operator::apply(const cplx* x, cplx* y, hipStream_t s, const int* gate) con
   a.gate = gate;
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
-  if (real_) launch_hv<double, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
-  else launch_hv<double, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xd, yd, s);
+  if (real_) launch_hv<double, 1, 1>(a, kernel_sym(*basis_), phased(*basis_), xd, yd, s);
+  else launch_hv<double, 2, 1>(a, kernel_sym(*basis_), phased(*basis_), xd, yd, s);
 }
 
 void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s, const int* gate) const {
@@ -2377,16 +2470,16 @@ void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipSt
   a.gate = gate;
   const float* xf = reinterpret_cast<const float*>(x);
   float* yf = reinterpret_cast<float*>(y);
-  if (real_) launch_hv<float, 1, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
-  else launch_hv<float, 2, 1>(a, sym_code(*basis_), basis_->geometry().kmom > 0, xf, yf, s);
+  if (real_) launch_hv<float, 1, 1>(a, kernel_sym(*basis_), phased(*basis_), xf, yf, s);
+  else launch_hv<float, 2, 1>(a, kernel_sym(*basis_), phased(*basis_), xf, yf, s);
 }
 
 bool Operator::apply_mixed(const float* x, double* y, bool complex_vectors, hipStream_t s, const int* gate) const {
   if (!tables_->use_stream || !tables_->hermitian) return false;
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
-  const int sym = sym_code(*basis_);
-  const bool sh = basis_->geometry().kmom > 0;
+  const int sym = kernel_sym(*basis_);
+  const bool sh = phased(*basis_);
   if (a.n == 0) return true;
   if (!complex_vectors) {
     if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
@@ -2407,8 +2500,8 @@ bool Operator::apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nv
   a.img = nullptr;
   a.ld = ld;
   if (a.n == 0) return true;
-  const int sym = sym_code(*basis_);
-  const bool sh = basis_->geometry().kmom > 0;
+  const int sym = kernel_sym(*basis_);
+  const bool sh = phased(*basis_);
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
   int done = 0;
@@ -2437,7 +2530,8 @@ bool Operator::apply_batch_interleaved(const cplx* x, cplx* y, size_t ld, int nv
 
 bool Operator::scan_batch_supported(hipStream_t s) const {
   auto& t = *tables_;
-  if (!parameterized() || dim() == 0 || !parts_real_ || !basis_->real_characters() || basis_->geometry().kmom > 0)
+  if (!parameterized() || dim() == 0 || !parts_real_ || !basis_->real_characters() || phased(*basis_) ||
+      basis_->general())
     return false;
   for (bool h : part_hermitian_)
     if (!h) return false;
@@ -2541,7 +2635,7 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
   const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
   const dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t sm = sizeof(double) * (size_t(width) * size_t(a.n_amp) + 81);
-  const int sym = sym_code(*basis_);
+  const int sym = kernel_sym(*basis_);
   char kname[64];
   std::snprintf(kname, sizeof kname, "hv_scan_batch_kernel<%s, %d, %d>", fp32 ? "float" : "double", width,
                 sym < 0 || sym > 3 ? 3 : sym);
@@ -2604,7 +2698,7 @@ void Operator::apply_scan_mixed(const float* x32, int w32, const double* x64, in
   const size_t n_slices = (size_t(a.n) + kSlice - 1) / kSlice;
   const dim3 grid(unsigned((n_slices * kSlice + kBlock - 1) / kBlock)), block(kBlock);
   const size_t sm = sizeof(double) * (size_t(B) * size_t(t.n_amp) + 81);
-  const int sym = sym_code(*basis_);
+  const int sym = kernel_sym(*basis_);
   char kname[64];
   std::snprintf(kname, sizeof kname, "hv_scan_mixed_kernel<%d, %d, %d>", w32, w64, sym < 0 || sym > 3 ? 3 : sym);
   const int timed = launch_timer_begin(kname, s);
@@ -2653,8 +2747,8 @@ void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) cons
   HvArgs a = tables_->args(*basis_);
   const double* xd = reinterpret_cast<const double*>(x);
   double* yd = reinterpret_cast<double*>(y);
-  const int sym = sym_code(*basis_);
-  const bool sh = basis_->geometry().kmom > 0;
+  const int sym = kernel_sym(*basis_);
+  const bool sh = phased(*basis_);
   int done = 0;
   while (done < nvec) {
     int left = nvec - done;
@@ -2685,7 +2779,7 @@ static void run_columns(const Operator& op, cplx* dense, double* colsum, double*
   QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
   dim3 grid((a.n + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(column_entries_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
-                     b.geometry().kmom > 0 ? 1 : 0, dense, colsum, dre, dim_, overflow.data(), shift, count);
+                     phased(b) ? 1 : 0, dense, colsum, dre, dim_, overflow.data(), shift, count);
   QMBED_HIP(hipGetLastError());
   int of = 0;
   QMBED_HIP(hipMemcpyAsync(&of, overflow.data(), sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2717,7 +2811,7 @@ SectorBatchAction::SectorBatchAction(const std::vector<Operator*>& ops, const st
     Operator& op = *ops[i];
     const SpinBasis1D& b = op.basis();
     if (!op.hermitian()) throw non_hermitian();
-    const int sym = sym_code(b);
+    const int sym = kernel_sym(b);
     if (i == 0) sym_ = sym;
     if (sym != sym_) throw unsupported_backend("sector batch: every sector must use the same symmetry kind");
     auto& t = *op.tables_;
@@ -2734,10 +2828,11 @@ SectorBatchAction::SectorBatchAction(const std::vector<Operator*>& ops, const st
     bs.a.gate = nullptr;
     bs.offset = offsets[i];
     bs.cmode = op.is_real() ? 1 : 2;
-    bs.shift = b.geometry().kmom > 0 ? 1 : 0;
+    bs.shift = phased(b) ? 1 : 0;
     host[i] = bs;
     const size_t G = size_t(bs.a.n_diag + bs.a.n_off);
-    smem_ = std::max(smem_, sizeof(double) * (2 * size_t(bs.a.n_amp) + 64 + 65 + 16) + sizeof(uint32_t) * 4 * G);
+    smem_ = std::max(smem_,
+                     sizeof(double) * (2 * size_t(bs.a.n_amp) + 2 * kMaxChars + 65 + 16) + sizeof(uint32_t) * 4 * G);
   }
   sec_.resize(std::max<size_t>(1, host.size() * sizeof(BatchSector)));
   if (!host.empty())

@@ -134,6 +134,17 @@ struct PinnedArray {
 // ------------------------------------------------------------------ basis --
 enum class Normalization { AngularMomentum = 0, Pauli = 1, PauliCartesian = 2 };
 
+// One generator of a general symmetry sector (the C ABI's SymmetryRequest,
+// QMBED bindings/capi/src/lib.rs:906-911; LatticeSymmetryMap,
+// src/basis/mod.rs:2967-3060): site `source` moves to destinations[source]
+// and its digit d becomes local_permutations[source][d] (empty: identity on
+// every site); `sector` is the character index q of e^{-2πi q / period}.
+struct SymmetryMapSpec {
+  std::vector<long> destinations;
+  std::vector<std::vector<long>> local_permutations;
+  int sector = 0;
+};
+
 struct SpinBasisSpec {
   int sites = 0;
   int spin_twice = 1;
@@ -142,6 +153,13 @@ struct SpinBasisSpec {
   std::optional<int> momentum;
   std::optional<int> parity;
   Normalization normalization = Normalization::AngularMomentum;
+  // general symmetry sector (GeneralBasis + SymmetryReducer); cannot be
+  // combined with momentum or parity
+  std::vector<SymmetryMapSpec> symmetries;
+  // the QuSpin row convention (bindings/capi/src/lib.rs:3412-3422): rows in
+  // descending order of the representatives, and (general sectors) the
+  // maximum state of each orbit as its representative
+  bool reverse = false;
 };
 
 // Symmetry-sector data consumed by the kernels (all integers; characters are
@@ -153,6 +171,15 @@ struct SectorGeometry {
   int parity = 0;    // 0 none, +1/-1
   int n_div = 0;     // proper divisors of L in ascending order (period search)
   int divisors[16] = {0};
+  // general symmetry sector: the group generated by the maps, as n_group
+  // elements s -> P(s) XOR flip (element 0 is the identity); n_chars
+  // distinct character values; representatives are orbit maxima when
+  // max_rep (minima otherwise)
+  int n_group = 0;
+  int n_chars = 1;
+  bool max_rep = false;
+  // rows in descending order of the representatives
+  bool reversed = false;
 };
 
 // SpinBasis1D: QMBED src/basis/mod.rs:1018-1035, 1312-1412.
@@ -165,13 +192,32 @@ class SpinBasis1D {
   size_t len() const { return n_; }
   const SpinBasisSpec& spec() const { return spec_; }
   const SectorGeometry& geometry() const { return geo_; }
-  bool symmetric() const { return geo_.kmom >= 0 || geo_.parity != 0; }
+  bool symmetric() const { return geo_.kmom >= 0 || geo_.parity != 0 || geo_.n_group > 0; }
+  bool general() const { return geo_.n_group > 0; }
+  bool reversed() const { return geo_.reversed; }
   // character phases are real when the momentum is 0 or L/2
-  bool real_characters() const { return geo_.kmom < 0 || geo_.kmom == 0 || 2 * geo_.kmom == geo_.L; }
+  bool real_characters() const {
+    if (general()) return real_chars_;
+    return geo_.kmom < 0 || geo_.kmom == 0 || 2 * geo_.kmom == geo_.L;
+  }
   const uint32_t* states() const { return states_.data(); }
   const uint8_t* orbits() const { return orbits_.data(); }
+  // the representatives and orbit sizes in row order (the ascending tables
+  // above, or their reversal when reversed())
+  const uint32_t* row_states() const { return reversed() ? rstates_.data() : states_.data(); }
+  const uint8_t* row_orbits() const { return reversed() ? rorbits_.data() : orbits_.data(); }
+  // general sectors: per group element the site destinations (32 bytes),
+  // the destination-side flip mask and the character-table index of
+  // χ(g⁻¹); the distinct character values (index 0 holds 1)
+  const uint8_t* group_dest() const { return gdest_.data(); }
+  const uint32_t* group_flip() const { return gflip_.data(); }
+  const uint8_t* group_phase() const { return gphase_.data(); }
+  const std::vector<cplx>& characters() const { return chars_; }
   std::vector<uint32_t> states_host() const;
   std::vector<uint8_t> orbits_host() const;
+  // row order (descending when reversed())
+  std::vector<uint32_t> row_states_host() const;
+  std::vector<uint8_t> row_orbits_host() const;
 
   // Top of the sorted-state search tree kept in LDS by the H·v kernels:
   // splitters[i] = states[i << split_shift], padded to a power of two.
@@ -198,6 +244,13 @@ class SpinBasis1D {
   DeviceArray<uint32_t> splitters_;
   DeviceArray<uint32_t> heads_;
   DeviceArray<uint32_t> pyramid_;
+  DeviceArray<uint32_t> rstates_;
+  DeviceArray<uint8_t> rorbits_;
+  DeviceArray<uint8_t> gdest_;
+  DeviceArray<uint32_t> gflip_;
+  DeviceArray<uint8_t> gphase_;
+  std::vector<cplx> chars_;
+  bool real_chars_ = true;
   std::vector<size_t> level_off_;
   int pyramid_top_ = 0;
   int split_count_ = 0;

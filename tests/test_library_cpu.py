@@ -179,3 +179,31 @@ def test_deep_nesting_and_bad_numbers_are_errors(qa):
     assert r["status"] == "error" and "surrogate" in r["error"]
     r = qa.raw_call("qmbed_run_json", '{"basis": "\\ud83d\\ude00"}')  # a surrogate pair is one code point
     assert "surrogate" not in r["error"]
+
+
+def test_general_symmetry_requests_checked_before_device_work(qa):
+    # the lattice-map checks of runtime_symmetry_sector and
+    # LatticeSymmetryMap::new (bindings/capi/src/lib.rs:3103-3127,
+    # src/basis/mod.rs:2978-3060) and the request-level checks of
+    # build_spin_basis (:2849-2880), with the reference's texts
+    def err(**extra):
+        basis = {"kind": "spin", "sites": 6, "up": 3, **extra}
+        r = qa.raw_call("qmbed_command_json", {"operation": "describe_basis", "basis": basis})
+        assert r["status"] == "error"
+        return r["error"]
+
+    t6 = [1, 2, 3, 4, 5, 0]
+    assert err(symmetries=[{"destinations": [1, 0], "sector": 0}]) == \
+        "invalid options: symmetry map has 2 sites, expected 6"
+    assert err(symmetries=[{"destinations": [1, 1, 2, 3, 4, 5], "sector": 0}]) == \
+        "incompatible symmetry request: symmetry site destinations must be a bijection"
+    assert err(symmetries=[{"destinations": [0, 1, 2, 3, 4, 9], "sector": 0}]) == \
+        "incompatible symmetry request: symmetry site destinations must be a bijection"
+    assert err(symmetries=[{"destinations": t6, "local_permutations": [[0, 0]] * 6, "sector": 0}]) == \
+        "incompatible symmetry request: every local-state map must be a permutation"
+    assert err(symmetries=[{"destinations": t6, "local_permutations": [[1, 0]] * 5, "sector": 0}]) == \
+        "incompatible symmetry request: every local-state map must be a permutation"
+    assert err(symmetries=[{"destinations": t6, "sector": 1}], momentum=1) == \
+        "invalid options: built-in and general spin symmetries cannot be mixed"
+    assert err(matrix_symmetry={"dimension": 2, "selected_row": 0, "generators": []}) == \
+        "invalid options: matrix-valued symmetry sectors materialize as projected operator models"
