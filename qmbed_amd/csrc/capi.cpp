@@ -474,9 +474,11 @@ std::string eigensystem_body(size_t dim, const Eigensystem& es, bool vectors, bo
   return o;
 }
 
-Eigensystem solve_model(Model& m, const SolverReq& r, const Value* parameters = nullptr) {
+Eigensystem solve_model(Model& m, const SolverReq& r, const Value* parameters = nullptr,
+                        std::optional<MatrixFormat> format = std::nullopt) {
   std::lock_guard<std::mutex> g(m.mu);
-  const Operator& op = m.bind(parameters);
+  Operator& op = m.bind(parameters);
+  if (format) op.use_format(*format, m.stream);
   if (r.has_initial && r.initial.size() != op.dim())
     throw dimension_mismatch("eigsh initial vector does not match the operator");
   if (r.has_subspace) return eigsh_subspace(op, r.opts, r.subspace, r.ctl, r.eigenvectors, m.stream);
@@ -523,6 +525,32 @@ std::string solve_models(const std::vector<std::shared_ptr<Model>>& models, cons
   return o;
 }
 
+// y_c = A x_c (conj_io: conj(A conj(x_c))) for host columns, through the
+// model's current action
+std::vector<std::vector<cplx>> apply_columns(const Operator& op, const std::vector<std::vector<cplx>>& cols,
+                                             bool conj_io, hipStream_t s) {
+  const size_t n = op.dim();
+  std::vector<std::vector<cplx>> out;
+  if (cols.empty()) return out;
+  DeviceArray<cplx> x(std::max<size_t>(n, 1)), y(std::max<size_t>(n, 1));
+  for (const auto& c : cols) {
+    std::vector<cplx> in = c;
+    if (conj_io)
+      for (auto& e : in) e = std::conj(e);
+    std::vector<cplx> res(n);
+    if (n) {
+      QMBED_HIP(hipMemcpyAsync(x.data(), in.data(), n * sizeof(cplx), hipMemcpyHostToDevice, s));
+      op.apply(x.data(), y.data(), s);
+      QMBED_HIP(hipMemcpyAsync(res.data(), y.data(), n * sizeof(cplx), hipMemcpyDeviceToHost, s));
+      QMBED_HIP(hipStreamSynchronize(s));
+    }
+    if (conj_io)
+      for (auto& e : res) e = std::conj(e);
+    out.push_back(std::move(res));
+  }
+  return out;
+}
+
 std::string run_solve(const std::string& req) {
   Value v;
   try {
@@ -533,9 +561,9 @@ std::string run_solve(const std::string& req) {
   if (v.kind != Value::Obj) bad("invalid binding request: expected an object");
   auto m = make_model(need(v, "basis", "invalid binding request"), need(v, "terms", "invalid binding request"),
                       nullptr);
-  (void)parse_format(v.get("format"));
+  const MatrixFormat format = parse_format(v.get("format"));
   SolverReq r = parse_solver(need(v, "solver", "invalid binding request"));
-  Eigensystem es = solve_model(*m, r);
+  Eigensystem es = solve_model(*m, r, nullptr, format);
   return "{\"status\":\"ok\",\"result\":" + eigensystem_body(m->basis->len(), es, r.eigenvectors, false) + "}";
 }
 
@@ -582,10 +610,113 @@ std::string run_command(const std::string& req) {
     const Value& h = need(v, "handle", "eigsh_model");
     if (h.kind != Value::String) bad("handle must be a string");
     auto m = registered_model(h.raw);
-    (void)parse_format(v.get("format"));
+    const MatrixFormat format = parse_format(v.get("format"));
     SolverReq r = parse_solver(need(v, "solver", "eigsh_model"));
-    Eigensystem es = solve_model(*m, r, v.get("parameters"));
+    Eigensystem es = solve_model(*m, r, v.get("parameters"), format);
     return ok(eigensystem_body(m->basis->len(), es, r.eigenvectors, true));
+  }
+  if (op == "describe_model") {
+    // registered_describe_model (capi :5990-5997): the sector's rows
+    const Value& h = need(v, "handle", "describe_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    auto states = m->basis->row_states_host();
+    std::string o = "{\"kind\":\"basis\",\"dimension\":" + std::to_string(m->basis->len()) + ",\"states\":[";
+    for (size_t i = 0; i < states.size(); ++i) {
+      if (i) o += ',';
+      o += '"' + std::to_string(states[i]) + '"';
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "materialize_model") {
+    // command_operator (capi :3913-3940): the stored entries in row-major
+    // order for the requested format (Operator::triplets)
+    const Value& h = need(v, "handle", "materialize_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    const MatrixFormat format = parse_format(v.get("format"));
+    std::vector<Operator::Triplet> trip;
+    size_t n = 0;
+    {
+      std::lock_guard<std::mutex> g(m->mu);
+      Operator& o = m->bind(v.get("parameters"));
+      n = o.dim();
+      trip = o.triplets(m->stream);
+    }
+    if (format == MatrixFormat::Dia) {
+      std::set<long> diagonals;
+      for (auto& t : trip) diagonals.insert(long(t.row) - long(t.col));
+      if (diagonals.size() > std::max<size_t>(2 * n, 1))
+        throw unsupported_backend("the requested operator is not usefully diagonal-banded");
+    }
+    static const char* names[] = {"dense", "csc", "csr", "dia", "matrix_free"};
+    std::string o = "{\"kind\":\"operator\",\"shape\":[" + std::to_string(n) + "," + std::to_string(n) +
+                    "],\"format\":\"" + names[int(format)] + "\",\"entries\":[";
+    for (size_t i = 0; i < trip.size(); ++i) {
+      if (i) o += ',';
+      o += "{\"row\":" + std::to_string(trip[i].row) + ",\"column\":" + std::to_string(trip[i].col) + ",\"value\":[";
+      json::write_number(o, trip[i].value.real());
+      o += ',';
+      json::write_number(o, trip[i].value.imag());
+      o += "]}";
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "apply_model") {
+    // command_apply_model (capi :4351-4360): A·x for each vector, with the
+    // requested action (normal, transpose, conjugate, adjoint)
+    const Value& h = need(v, "handle", "apply_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    std::string action = "normal";
+    if (auto* a = v.get("action"); a && !a->is_null()) {
+      if (a->kind != Value::String) bad("action must be a string");
+      action = a->raw;
+      if (action != "normal" && action != "transpose" && action != "conjugate" && action != "adjoint")
+        bad("invalid binding command: unknown variant `" + action + "`");
+    }
+    std::vector<std::vector<cplx>> cols;
+    for (auto& c : need(v, "vectors", "apply_model").array()) cols.push_back(as_cvec(c, "vectors"));
+    std::vector<std::vector<cplx>> out;
+    size_t n = 0;
+    {
+      std::lock_guard<std::mutex> g(m->mu);
+      Operator& o = m->bind(v.get("parameters"));
+      n = o.dim();
+      for (auto& c : cols)
+        if (c.size() != n) throw dimension_mismatch("operator input has the wrong length");
+      // a Hermitian matrix is its own adjoint, and its transpose is its
+      // conjugate: A^T x = conj(A conj(x))
+      const bool herm = o.hermitian();
+      const bool conj_io = action == "conjugate" || (herm && action == "transpose");
+      if (!herm && (action == "transpose" || action == "adjoint"))
+        throw unsupported_backend("transposed actions of a non-Hermitian sector matrix are not built on the MI355X path");
+      out = apply_columns(o, cols, conj_io, m->stream);
+    }
+    std::string o = "{\"kind\":\"vectors\",\"dimension\":" + std::to_string(n) + ",\"vectors\":[";
+    for (size_t c = 0; c < out.size(); ++c) {
+      if (c) o += ',';
+      write_vec(o, out[c]);
+    }
+    o += "]}";
+    return ok(o);
+  }
+  if (op == "eigh_model") {
+    // command_eigh (capi :5290-5299): the whole spectrum of the dense matrix
+    const Value& h = need(v, "handle", "eigh_model");
+    if (h.kind != Value::String) bad("handle must be a string");
+    auto m = registered_model(h.raw);
+    const bool vectors = as_bool(v.get("eigenvectors"), false);
+    Eigensystem es;
+    {
+      std::lock_guard<std::mutex> g(m->mu);
+      Operator& o = m->bind(v.get("parameters"));
+      if (!o.hermitian()) throw non_hermitian();
+      es = eigh_all(o, vectors, m->stream);
+    }
+    return ok(eigensystem_body(m->basis->len(), es, vectors, true));
   }
   if (op == "eigsh_models") {
     // extension: several registered models (sectors) solved in one batch

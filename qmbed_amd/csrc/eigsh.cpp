@@ -1024,7 +1024,7 @@ void run_window(const Operator& op, bool c, const EigshOptions& o, int m, const 
 
 // Dense route for small problems: GPU materialization + rocSOLVER heevd.
 Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<double>& vec_out, bool& complex_out,
-                        hipStream_t s) {
+                        hipStream_t s, bool refine = true) {
   const size_t n = op.dim();
   DeviceArray<double> A(2 * n * n);
   op.materialize_dense(reinterpret_cast<cplx*>(A.data()), s);
@@ -1048,7 +1048,7 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
                                       E.data(), info.data());
   QMBED_HIP(hipStreamSynchronize(s));
   rocblas_destroy_handle(h);
-  if (st != rocblas_status_success) throw internal_state("rocsolver_zheevd failed");
+  if (st != rocblas_status_success) throw internal_state("rocsolver_zheev failed");
   int hinfo = 0;
   QMBED_HIP(hipMemcpy(&hinfo, info.data(), sizeof(int), hipMemcpyDeviceToHost));
   if (hinfo != 0) throw non_convergence(0, INFINITY);
@@ -1077,8 +1077,12 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
     double* y = vec_out.data() + e * n * q;
     QMBED_HIP(hipMemcpyAsync(y, col.data(), e * n * 8, hipMemcpyHostToDevice, s));
     auto r = rq.rayleigh(y, 1)[0];
-    es.eigenvalues.push_back(r.first);
-    es.residuals.push_back(r.second);
+    // eigh keeps the decomposition's eigenvalue and reports ‖Hv − λv‖ for it
+    // (src/solve/mod.rs:586-605); eigsh refines it by the Rayleigh quotient
+    const double lam = refine ? r.first : vals[size_t(sel[q])];
+    es.eigenvalues.push_back(lam);
+    es.residuals.push_back(refine ? r.second : std::sqrt(std::max(0.0, r.second * r.second +
+                                                                           (r.first - lam) * (r.first - lam))));
   }
   es.iterations = 1;
   es.converged = true;
@@ -1087,6 +1091,36 @@ Eigensystem dense_eigsh(const Operator& op, const EigshOptions& o, DeviceArray<d
 
 
 }  // namespace
+
+// eigh: the complete spectrum of the dense sector matrix (QMBED
+// src/solve/mod.rs:586-620, the model's eigh at src/interop.rs:589-599): every
+// eigenpair ascending, the residual ‖Hv − λv‖ of each, iterations 1 (0 for an
+// empty operator), eigenvectors only when asked for.
+Eigensystem eigh_all(const Operator& op, bool want_vectors, hipStream_t s) {
+  Eigensystem es;
+  const size_t n = op.dim();
+  es.iterations = n ? 1 : 0;
+  es.converged = true;
+  if (n == 0) return es;
+  if (n > 32768) throw unsupported_backend("dense eigh beyond 32768 rows is not built on the MI355X path");
+  EigshOptions o;
+  o.eigenpairs = int(n);
+  o.target = SpectrumTarget::SmallestAlgebraic;
+  DeviceArray<double> vecs;
+  bool c = false;
+  es = dense_eigsh(op, o, vecs, c, s, /*refine=*/false);
+  es.iterations = 1;
+  if (want_vectors) {
+    const size_t e = c ? 2 : 1;
+    std::vector<double> h(e * n * n);
+    QMBED_HIP(hipMemcpy(h.data(), vecs.data(), h.size() * 8, hipMemcpyDeviceToHost));
+    es.eigenvectors.assign(n, std::vector<cplx>(n));
+    for (size_t q = 0; q < n; ++q)
+      for (size_t i = 0; i < n; ++i)
+        es.eigenvectors[q][i] = c ? cplx(h[e * n * q + 2 * i], h[e * n * q + 2 * i + 1]) : cplx(h[n * q + i], 0.0);
+  }
+  return es;
+}
 
 // the dense route, for the thick restart's fallback (thick_restart.cpp)
 Eigensystem dense_eigsh_route(const Operator& op, const EigshOptions& o, DeviceArray<double>& vec_out,
@@ -1364,6 +1398,10 @@ void balanced_start(const DeviceArray<double>& vecs, int k, size_t n, bool c, De
 // Lanczos (TRIDIAGONAL_LANCZOS_WINDOW, src/solve/mod.rs)
 constexpr long kTridiagonalWindow = 96;
 
+Eigensystem shift_invert_eigsh(const Operator& op, const EigshOptions& o, const double* d_initial,
+                               bool initial_complex, DeviceArray<double>& vec_out, bool& complex_vectors,
+                               hipStream_t s);
+
 Eigensystem thick_restart_eigsh(const Operator& op, const EigshOptions& o, const std::vector<const double*>& init,
                                 bool c, DeviceArray<double>& vec_out, bool& complex_vectors, hipStream_t s);
 
@@ -1371,10 +1409,11 @@ Eigensystem eigsh_impl(const Operator& op, const EigshOptions& o, const double* 
                        const EigshControls& ctl, DeviceArray<double>& vec_out, bool& complex_vectors, hipStream_t s) {
   const size_t n = op.dim();
   o.validate(n);
-  if (o.target == SpectrumTarget::Shift)
-    throw unsupported_backend("shift-invert targets are not implemented on the MI355X Krylov path");
   const bool dense = n <= 128 || (n <= 256 && !o.krylov_dimension);
   if (dense) return dense_eigsh(op, o, vec_out, complex_vectors, s);
+  // the Shift target: Lanczos on (A − σI)⁻¹ (shift_invert.hip)
+  if (o.target == SpectrumTarget::Shift)
+    return shift_invert_eigsh(op, o, d_initial, initial_complex, vec_out, complex_vectors, s);
   const bool c = !op.is_real() || initial_complex;
   // explicit windows below the tridiagonal cut-off run the thick restart
   // (src/solve/mod.rs:2299-2327)
@@ -1556,7 +1595,8 @@ Eigensystem eigsh_subspace(const Operator& op, const EigshOptions& o, const std:
   if (n <= 128 || (n <= 256 && !o.krylov_dimension)) return eigsh(op, o, nullptr, ctl, want_vectors, s);
   if (sub.size() == 1) return eigsh(op, o, sub[0].data(), ctl, want_vectors, s);
   if (o.target == SpectrumTarget::Shift)
-    throw unsupported_backend("shift-invert targets are not implemented on the MI355X Krylov path");
+    throw unsupported_backend("shift-invert targets from several start vectors (the thick restart) are not built on "
+                              "the MI355X path");
   // several vectors: the thick restart keeps them as a subspace
   bool c = !op.is_real();
   for (const auto& v : sub)

@@ -20,8 +20,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
 #include <set>
 #include <string>
+
+#include <hipcub/hipcub.hpp>
 
 #include "internal.hpp"
 
@@ -913,6 +916,144 @@ __global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const 
   }
 }
 
+// Stored sector matrix (the reference's assembled formats,
+// src/operator/mod.rs:2091-2216): column `col`'s stored entries (merged per
+// row, |v| > eps, the diagonal included) sorted by row at ptr[col]: the CSC
+// arrays. The count pass is column_entries_kernel's count mode, which keeps
+// exactly the same entries.
+__global__ void __launch_bounds__(kBlock) column_fill_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
+                                                             const uint64_t* __restrict__ ptr, uint32_t* rows,
+                                                             double2* vals, int* overflow) {
+  const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
+  if (col >= a.n) return;
+  ColList<kMaxColEntries> C;
+  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  if (C.overflow) {
+    *overflow = 1;
+    return;
+  }
+  // keep the stored off-diagonal entries, then order them by row
+  int m = 0;
+  for (int k = 0; k < C.cnt; ++k) {
+    if (hypot(C.vre[k], C.vim[k]) <= kEps) continue;
+    C.rows[m] = C.rows[k];
+    C.vre[m] = C.vre[k];
+    C.vim[m] = C.vim[k];
+    ++m;
+  }
+  for (int k = 1; k < m; ++k) {
+    const int r = C.rows[k];
+    const double vr = C.vre[k], vi = C.vim[k];
+    int j = k - 1;
+    while (j >= 0 && C.rows[j] > r) {
+      C.rows[j + 1] = C.rows[j];
+      C.vre[j + 1] = C.vre[j];
+      C.vim[j + 1] = C.vim[j];
+      --j;
+    }
+    C.rows[j + 1] = r;
+    C.vre[j + 1] = vr;
+    C.vim[j + 1] = vi;
+  }
+  const bool diag = hypot(C.dre, C.dim) > kEps;
+  uint64_t out = ptr[col];
+  bool placed = !diag;
+  for (int k = 0; k < m; ++k) {
+    if (!placed && uint32_t(C.rows[k]) > col) {
+      rows[out] = col;
+      vals[out++] = make_double2(C.dre, C.dim);
+      placed = true;
+    }
+    rows[out] = uint32_t(C.rows[k]);
+    vals[out++] = make_double2(C.vre[k], C.vim[k]);
+  }
+  if (!placed) {
+    rows[out] = col;
+    vals[out] = make_double2(C.dre, C.dim);
+  }
+}
+
+// the column of every CSC entry
+__global__ void csc_columns_kernel(const uint64_t* __restrict__ ptr, uint32_t n, uint32_t* cols) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (uint64_t k = ptr[c]; k < ptr[c + 1]; ++k) cols[k] = c;
+}
+
+// CSR values from CSC entries: Hermitian matrices read column r as the
+// conjugate of row r (perm == nullptr: same positions); otherwise entry k of
+// the CSR is CSC entry perm[k] (stable order by row, then column)
+__global__ void csr_values_kernel(const double2* __restrict__ vals, const uint32_t* __restrict__ perm,
+                                  const uint32_t* __restrict__ ccols, uint64_t nnz, int complex_out, double* out,
+                                  uint32_t* cols) {
+  const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint64_t src = perm ? perm[k] : k;
+  const double2 v = vals[src];
+  if (perm) cols[k] = ccols[src];
+  if (complex_out) {
+    out[2 * k] = v.x;
+    out[2 * k + 1] = perm ? v.y : -v.y;
+  } else {
+    out[k] = v.x;
+  }
+}
+
+// row counts of sorted row keys
+__global__ void row_count_kernel(const uint32_t* __restrict__ rows, uint64_t nnz, uint32_t* counts) {
+  const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k < nnz) atomicAdd(counts + rows[k], 1u);
+}
+
+// y = A x from the stored CSR: 8 lanes per row, each summing every 8th
+// entry of the row, then a fixed butterfly over the 8 lanes (deterministic).
+// CMODE 0: real values and vectors; 1: real values, complex vectors; 2:
+// complex values and vectors. SX: input type, S: accumulation and output.
+constexpr int kCsrLanes = 8;
+
+template <typename S, int CMODE, typename SX>
+__global__ void __launch_bounds__(kBlock) csr_kernel(const uint64_t* __restrict__ ptr,
+                                                     const uint32_t* __restrict__ col,
+                                                     const double* __restrict__ val, uint32_t n,
+                                                     const SX* __restrict__ x, S* __restrict__ y,
+                                                     const int* gate) {
+  if (gate && *gate) return;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t row = gid / kCsrLanes;
+  const int sub = int(gid % kCsrLanes);
+  S re = 0, im = 0;
+  if (row < n) {
+    const uint64_t end = ptr[row + 1];
+    for (uint64_t k = ptr[row] + uint64_t(sub); k < end; k += kCsrLanes) {
+      const uint32_t c = col[k];
+      if (CMODE == 0) {
+        re += S(val[k]) * S(x[c]);
+      } else if (CMODE == 1) {
+        const S v = S(val[k]);
+        re += v * S(x[2 * size_t(c)]);
+        im += v * S(x[2 * size_t(c) + 1]);
+      } else {
+        const S vr = S(val[2 * k]), vi = S(val[2 * k + 1]);
+        const S xr = S(x[2 * size_t(c)]), xi = S(x[2 * size_t(c) + 1]);
+        re += vr * xr - vi * xi;
+        im += vr * xi + vi * xr;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = kCsrLanes / 2; o > 0; o >>= 1) {
+    re += __shfl_xor(re, o, kCsrLanes);
+    if (CMODE != 0) im += __shfl_xor(im, o, kCsrLanes);
+  }
+  if (row >= n || sub != 0) return;
+  if (CMODE == 0) {
+    y[row] = re;
+  } else {
+    y[2 * size_t(row)] = re;
+    y[2 * size_t(row) + 1] = im;
+  }
+}
+
 // Hermiticity of the sector matrix, entry by entry as the reference checks
 // it: every stored entry (j, r, v) needs |v - conj(H[r, j])| <= 1e-12
 // (src/operator/mod.rs:1240-1242, 1313-1398, 2212-2214). Lane r lists column
@@ -1700,6 +1841,17 @@ struct Operator::DeviceTables {
   // im[n])), from which each evaluation combines the diagonal
   DeviceArray<double> part_diag;
   int diag_parts = 0;
+  // stored sector matrix (a stored format was requested): CSR rows with
+  // the reference's stored entries; built on demand for the current
+  // evaluation (version counts the coefficient evaluations)
+  bool use_csr = false;
+  DeviceArray<uint64_t> csr_ptr;
+  DeviceArray<uint32_t> csr_col;
+  DeviceArray<double> csr_val;  // one double per entry (real) or (re, im)
+  bool csr_complex = false;
+  size_t csr_nnz = 0;
+  long csr_version = -1;
+  long version = 0;
   // pinned staging of the amplitude uploads and the event after their copies
   PinnedArray<double> stage;
   hipEvent_t staged = nullptr;
@@ -2040,6 +2192,7 @@ void Operator::evaluate_coefficients(const std::vector<cplx>& c, hipStream_t s) 
   }
   coefficients_ = c;
   auto& t = *tables_;
+  ++t.version;
   real_ = parts_real_ && real_c && basis_->real_characters();
   t.complex_coef = !real_;
   upload_amplitudes(*this, diag_, off_, s, &c, parts_real_);
@@ -2436,8 +2589,46 @@ static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_
   QMBED_HIP(hipGetLastError());
 }
 
+// The stored-CSR action when the operator uses a stored format; returns
+// false (nothing launched) otherwise. cm: 0 real vectors, 1 complex
+// (interleaved) vectors; f32: FP32 input (and FP32 output unless mixed).
+static bool csr_apply(const Operator* op, int cm, bool f32, const void* x, void* y, hipStream_t s, const int* gate,
+                      bool mixed = false) {
+  auto& t = *op->tables_;
+  if (!t.use_csr) return false;
+  const_cast<Operator*>(op)->refresh_stored(s);
+  const uint32_t n = uint32_t(op->dim());
+  if (n == 0) return true;
+  const dim3 grid(unsigned((size_t(n) * kCsrLanes + kBlock - 1) / kBlock)), block(kBlock);
+  const int mode = cm == 0 ? 0 : (t.csr_complex ? 2 : 1);
+  if (cm == 0 && t.csr_complex) throw unsupported_backend("real action requires an operator with real-valued storage");
+  const uint64_t* p = t.csr_ptr.data();
+  const uint32_t* c = t.csr_col.data();
+  const double* v = t.csr_val.data();
+#define QMBED_CSR(S, M, SX)                                                                                  \
+  hipLaunchKernelGGL((csr_kernel<S, M, SX>), grid, block, 0, s, p, c, v, n, static_cast<const SX*>(x), \
+                     static_cast<S*>(y), gate)
+  if (!f32) {
+    if (mode == 0) QMBED_CSR(double, 0, double);
+    else if (mode == 1) QMBED_CSR(double, 1, double);
+    else QMBED_CSR(double, 2, double);
+  } else if (mixed) {
+    if (mode == 0) QMBED_CSR(double, 0, float);
+    else if (mode == 1) QMBED_CSR(double, 1, float);
+    else QMBED_CSR(double, 2, float);
+  } else {
+    if (mode == 0) QMBED_CSR(float, 0, float);
+    else if (mode == 1) QMBED_CSR(float, 1, float);
+    else QMBED_CSR(float, 2, float);
+  }
+#undef QMBED_CSR
+  QMBED_HIP(hipGetLastError());
+  return true;
+}
+
 void Operator::apply(const double* x, double* y, hipStream_t s, const int* gate) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+  if (csr_apply(this, 0, false, x, y, s, gate)) return;
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
   launch_hv<double, 0, 1>(a, kernel_sym(*basis_), phased(*basis_), x, y, s);
@@ -2445,12 +2636,14 @@ void Operator::apply(const double* x, double* y, hipStream_t s, const int* gate)
 
 void Operator::apply(const float* x, float* y, hipStream_t s, const int* gate) const {
   if (!real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+  if (csr_apply(this, 0, true, x, y, s, gate)) return;
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
   launch_hv<float, 0, 1>(a, kernel_sym(*basis_), phased(*basis_), x, y, s);
 }
 
 void Operator::apply(const cplx* x, cplx* y, hipStream_t s, const int* gate) const {
+  if (csr_apply(this, 1, false, x, y, s, gate)) return;
   if (!tables_->hermitian) {
     scatter_apply(*this, x, y, s);
     return;
@@ -2464,6 +2657,7 @@ void Operator::apply(const cplx* x, cplx* y, hipStream_t s, const int* gate) con
 }
 
 void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipStream_t s, const int* gate) const {
+  if (csr_apply(this, 1, true, x, y, s, gate)) return;
   if (!tables_->hermitian)
     throw unsupported_backend("single-precision action of a non-Hermitian sector matrix is not provided");
   HvArgs a = tables_->args(*basis_);
@@ -2475,6 +2669,10 @@ void Operator::apply(const std::complex<float>* x, std::complex<float>* y, hipSt
 }
 
 bool Operator::apply_mixed(const float* x, double* y, bool complex_vectors, hipStream_t s, const int* gate) const {
+  if (tables_->use_csr) {
+    if (!complex_vectors && !real_) throw unsupported_backend("real action requires an operator with real-valued storage");
+    return csr_apply(this, complex_vectors ? 1 : 0, true, x, y, s, gate, true);
+  }
   if (!tables_->use_stream || !tables_->hermitian) return false;
   HvArgs a = tables_->args(*basis_);
   a.gate = gate;
@@ -2740,6 +2938,10 @@ bool prepare_interleaved_action(const Operator& op, hipStream_t s) {
 }
 
 void Operator::apply_batch(const cplx* x, cplx* y, int nvec, hipStream_t s) const {
+  if (tables_->use_csr) {
+    for (int v = 0; v < nvec; ++v) apply(x + size_t(v) * dim(), y + size_t(v) * dim(), s);
+    return;
+  }
   if (!tables_->hermitian) {
     for (int v = 0; v < nvec; ++v) scatter_apply(*this, x + size_t(v) * dim(), y + size_t(v) * dim(), s);
     return;
@@ -2797,6 +2999,153 @@ size_t Operator::stored_entries(hipStream_t s) const {
   size_t tot = 0;
   for (uint32_t c : h) tot += c;
   return tot;
+}
+
+// Assemble the stored sector matrix as CSR rows (OperatorBuilder::build
+// with a stored format, src/operator/mod.rs:2091-2216): count and fill the
+// CSC columns, then read them as rows, conjugated, when the matrix is
+// Hermitian, or order the entries by row (a stable radix sort keeps the
+// column order within a row) when it is not.
+void Operator::refresh_stored(hipStream_t s) {
+  auto& t = *tables_;
+  if (t.csr_version == t.version) return;
+  const SpinBasis1D& b = *basis_;
+  const size_t n = b.len();
+  t.csr_ptr.resize(n + 1);
+  if (n == 0) {
+    const uint64_t z = 0;
+    QMBED_HIP(hipMemcpy(t.csr_ptr.data(), &z, 8, hipMemcpyHostToDevice));
+    t.csr_nnz = 0;
+    t.csr_version = t.version;
+    return;
+  }
+  HvArgs a = t.args(b);
+  DeviceArray<uint32_t> cnt(n);
+  DeviceArray<int> overflow(1);
+  QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
+  const dim3 grid(unsigned((n + kBlock - 1) / kBlock));
+  hipLaunchKernelGGL(column_entries_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b),
+                     phased(b) ? 1 : 0, nullptr, nullptr, nullptr, nullptr, overflow.data(), cplx(0, 0), cnt.data());
+  QMBED_HIP(hipGetLastError());
+  std::vector<uint32_t> h(n);
+  QMBED_HIP(hipMemcpyAsync(h.data(), cnt.data(), n * 4, hipMemcpyDeviceToHost, s));
+  int of = 0;
+  QMBED_HIP(hipMemcpyAsync(&of, overflow.data(), sizeof(int), hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  if (of) throw unsupported_backend("a column has more distinct entries than the column-entry kernel holds");
+  std::vector<uint64_t> ptr(n + 1, 0);
+  for (size_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + h[i];
+  const uint64_t nnz = ptr[n];
+  if (nnz >= (uint64_t(1) << 32)) throw unsupported_backend("the stored matrix exceeds 2^32 entries");
+  DeviceArray<uint64_t> cptr(n + 1);
+  QMBED_HIP(hipMemcpyAsync(cptr.data(), ptr.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  DeviceArray<uint32_t> rows(std::max<uint64_t>(nnz, 1));
+  DeviceArray<double2> vals(std::max<uint64_t>(nnz, 1));
+  hipLaunchKernelGGL(column_fill_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b), phased(b) ? 1 : 0,
+                     cptr.data(), rows.data(), vals.data(), overflow.data());
+  QMBED_HIP(hipGetLastError());
+  t.csr_complex = !real_;
+  t.csr_val.resize(std::max<uint64_t>(nnz, 1) * (t.csr_complex ? 2 : 1));
+  const unsigned eg = unsigned((nnz + 255) / 256);
+  if (t.hermitian) {
+    QMBED_HIP(hipMemcpyAsync(t.csr_ptr.data(), cptr.data(), (n + 1) * 8, hipMemcpyDeviceToDevice, s));
+    t.csr_col = std::move(rows);
+    if (nnz)
+      hipLaunchKernelGGL(csr_values_kernel, dim3(eg), dim3(256), 0, s, vals.data(), nullptr, nullptr, nnz,
+                         t.csr_complex ? 1 : 0, t.csr_val.data(), nullptr);
+  } else {
+    DeviceArray<uint32_t> ccols(std::max<uint64_t>(nnz, 1)), perm_in(std::max<uint64_t>(nnz, 1)),
+        perm(std::max<uint64_t>(nnz, 1)), keys(std::max<uint64_t>(nnz, 1));
+    hipLaunchKernelGGL(csc_columns_kernel, grid, dim3(kBlock), 0, s, cptr.data(), uint32_t(n), ccols.data());
+    std::vector<uint32_t> iota(nnz);
+    std::iota(iota.begin(), iota.end(), 0u);
+    QMBED_HIP(hipMemcpyAsync(perm_in.data(), iota.data(), nnz * 4, hipMemcpyHostToDevice, s));
+    int bits = 1;
+    while ((size_t(1) << bits) < n) ++bits;
+    size_t tmp_bytes = 0;
+    QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rows.data(), keys.data(), perm_in.data(),
+                                                 perm.data(), int(nnz), 0, bits, s));
+    DeviceArray<uint8_t> tmp(std::max<size_t>(tmp_bytes, 1));
+    QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, rows.data(), keys.data(), perm_in.data(),
+                                                 perm.data(), int(nnz), 0, bits, s));
+    t.csr_col.resize(std::max<uint64_t>(nnz, 1));
+    if (nnz)
+      hipLaunchKernelGGL(csr_values_kernel, dim3(eg), dim3(256), 0, s, vals.data(), perm.data(), ccols.data(), nnz,
+                         t.csr_complex ? 1 : 0, t.csr_val.data(), t.csr_col.data());
+    DeviceArray<uint32_t> rc(n);
+    QMBED_HIP(hipMemsetAsync(rc.data(), 0, n * 4, s));
+    if (nnz) hipLaunchKernelGGL(row_count_kernel, dim3(eg), dim3(256), 0, s, keys.data(), nnz, rc.data());
+    QMBED_HIP(hipMemcpyAsync(h.data(), rc.data(), n * 4, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + h[i];
+    QMBED_HIP(hipMemcpyAsync(t.csr_ptr.data(), ptr.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  }
+  QMBED_HIP(hipGetLastError());
+  QMBED_HIP(hipStreamSynchronize(s));
+  t.csr_nnz = nnz;
+  t.csr_version = t.version;
+}
+
+void Operator::use_format(MatrixFormat f, hipStream_t s) {
+  format_ = f;
+  auto& t = *tables_;
+  t.use_csr = f != MatrixFormat::MatrixFree;
+  if (!t.use_csr) {
+    t.csr_ptr.release();
+    t.csr_col.release();
+    t.csr_val.release();
+    t.csr_version = -1;
+    return;
+  }
+  refresh_stored(s);
+  if (f == MatrixFormat::Dia) {
+    // the reference refuses a banded layout with more distinct diagonals
+    // than 2·min(rows, columns) (src/operator/mod.rs:2174-2186)
+    const size_t n = dim();
+    std::vector<uint64_t> ptr(n + 1);
+    std::vector<uint32_t> col(t.csr_nnz);
+    QMBED_HIP(hipMemcpy(ptr.data(), t.csr_ptr.data(), (n + 1) * 8, hipMemcpyDeviceToHost));
+    if (t.csr_nnz) QMBED_HIP(hipMemcpy(col.data(), t.csr_col.data(), t.csr_nnz * 4, hipMemcpyDeviceToHost));
+    std::vector<char> seen(2 * n + 1, 0);
+    size_t distinct = 0;
+    for (size_t r = 0; r < n; ++r)
+      for (uint64_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+        const size_t d = size_t(long(r) - long(col[k]) + long(n));
+        if (!seen[d]) {
+          seen[d] = 1;
+          ++distinct;
+        }
+      }
+    if (distinct > std::max<size_t>(2 * n, 1)) {
+      use_format(MatrixFormat::MatrixFree, s);
+      throw unsupported_backend("the requested operator is not usefully diagonal-banded");
+    }
+  }
+}
+
+std::vector<Operator::Triplet> Operator::triplets(hipStream_t s) const {
+  auto* self = const_cast<Operator*>(this);
+  const bool had = tables_->use_csr;
+  tables_->use_csr = true;
+  self->refresh_stored(s);
+  const size_t n = dim();
+  auto& t = *tables_;
+  std::vector<uint64_t> ptr(n + 1, 0);
+  std::vector<uint32_t> col(t.csr_nnz);
+  std::vector<double> val(t.csr_nnz * (t.csr_complex ? 2 : 1));
+  QMBED_HIP(hipMemcpy(ptr.data(), t.csr_ptr.data(), (n + 1) * 8, hipMemcpyDeviceToHost));
+  if (t.csr_nnz) {
+    QMBED_HIP(hipMemcpy(col.data(), t.csr_col.data(), col.size() * 4, hipMemcpyDeviceToHost));
+    QMBED_HIP(hipMemcpy(val.data(), t.csr_val.data(), val.size() * 8, hipMemcpyDeviceToHost));
+  }
+  tables_->use_csr = had;
+  std::vector<Triplet> out;
+  out.reserve(t.csr_nnz);
+  for (size_t r = 0; r < n; ++r)
+    for (uint64_t k = ptr[r]; k < ptr[r + 1]; ++k)
+      out.push_back(Triplet{uint32_t(r), col[k],
+                            t.csr_complex ? cplx(val[2 * k], val[2 * k + 1]) : cplx(val[k], 0.0)});
+  return out;
 }
 
 static_assert(SectorBatchAction::kRows == kBlock, "batched blocks are H·v blocks");

@@ -381,6 +381,25 @@ class Operator {
   // dense materialization (row-major, complex) on the device
   void materialize_dense(cplx* dense, hipStream_t s) const;
 
+  // Storage format (OperatorBuilder::build(format), QMBED
+  // src/operator/mod.rs:2091-2216): a stored format (csr, csc, dense, dia)
+  // assembles the sector matrix on the device with the reference's stored
+  // entries (merged per (row, column), |v| > f64::EPSILON) as CSR rows, and
+  // the single-vector actions then run as sparse products over them;
+  // matrix_free keeps the on-the-fly kernels. Dia answers unsupported
+  // backend when the matrix is not diagonal-banded. A family re-assembles
+  // the stored matrix after each evaluation.
+  void use_format(MatrixFormat f, hipStream_t s);
+  // the stored entries in row-major order (Operator::triplets,
+  // src/operator/mod.rs:937-1003)
+  struct Triplet {
+    uint32_t row, col;
+    cplx value;
+  };
+  std::vector<Triplet> triplets(hipStream_t s) const;
+  // (re)assemble the stored matrix for the current evaluation
+  void refresh_stored(hipStream_t s);
+
   // Operator families (QMBED src/operator/mod.rs:2710-2830, 3140-3230).
   bool parameterized() const { return !components_.empty(); }
   const std::vector<ComponentSpec>& components() const { return components_; }
@@ -598,6 +617,9 @@ void eigsh_lockstep_stream(const Operator& op, const EigshOptions& opts, const E
 // `initial` is a host vector of length dim (nullptr: deterministic start).
 Eigensystem eigsh(const Operator& op, const EigshOptions& opts, const cplx* initial = nullptr,
                   const EigshControls& ctl = EigshControls(), bool want_vectors = false, hipStream_t s = nullptr);
+
+// eigh: the complete dense spectrum of a sector (QMBED src/solve/mod.rs:586-620)
+Eigensystem eigh_all(const Operator& op, bool want_vectors, hipStream_t s);
 
 // eigsh_with_initial_subspace: QMBED src/solve/mod.rs:2385-2410. Every
 // vector is validated (length, finite entries, nonzero norm); no vector runs
