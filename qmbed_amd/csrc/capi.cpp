@@ -374,12 +374,41 @@ std::vector<ComponentSpec> parse_components(const Value* v) {
   return out;
 }
 
+// site_permutation (EdModel::with_site_permutation, src/interop.rs:682-708,
+// 1328-1345): every entry below the permutation's length, no repeats
+std::vector<int> parse_site_permutation(const Value& p) {
+  std::vector<uint64_t> raw;
+  for (auto& e : p.array()) raw.push_back(as_u64(e, "site_permutation"));
+  const uint64_t len = raw.size();
+  for (uint64_t site : raw)
+    if (site >= len) throw invalid_site(long(std::min<uint64_t>(site, uint64_t(LONG_MAX))), long(len));
+  std::set<uint64_t> uniq(raw.begin(), raw.end());
+  if (uniq.size() != raw.size()) bad("site permutation must be bijective");
+  return std::vector<int>(raw.begin(), raw.end());
+}
+
+// every coupling's site s becomes permutation[s] (OperatorSpec::
+// with_site_permutation, src/operator/mod.rs:269-309)
+void permute_sites(std::vector<OperatorSpec>& terms, const std::vector<int>& perm) {
+  for (auto& t : terms)
+    for (auto& c : t.couplings)
+      for (int& site : c.sites) {
+        if (site < 0 || size_t(site) >= perm.size()) throw invalid_site(site, long(perm.size()));
+        site = perm[size_t(site)];
+      }
+}
+
 std::shared_ptr<Model> make_model(const Value& basis, const Value& terms, const Value* checks,
-                                  const Value* components = nullptr) {
+                                  const Value* components = nullptr, const Value* site_permutation = nullptr) {
   auto m = std::make_shared<Model>();
   m->spec = parse_basis(basis);
   m->terms = parse_terms(terms);
   m->components = parse_components(components);
+  if (site_permutation && !site_permutation->is_null()) {
+    const std::vector<int> perm = parse_site_permutation(*site_permutation);
+    permute_sites(m->terms, perm);
+    for (auto& c : m->components) permute_sites(c.terms, perm);
+  }
   m->checks = parse_checks(checks);
   QMBED_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
   m->basis = SpinBasis1D::build(m->spec, m->stream);
@@ -389,6 +418,49 @@ std::shared_ptr<Model> make_model(const Value& basis, const Value& terms, const 
 std::mutex g_reg_mu;
 std::map<uint64_t, std::shared_ptr<Model>> g_models;
 std::atomic<uint64_t> g_next_model{1};
+
+std::string register_model_json(const std::shared_ptr<Model>& m) {
+  const uint64_t id = g_next_model.fetch_add(1);
+  const size_t dim = m->basis->len();
+  {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    g_models[id] = m;
+  }
+  return "{\"kind\":\"model\",\"handle\":\"" + std::to_string(id) + "\",\"dimension\":" + std::to_string(dim) + "}";
+}
+
+// deferred basis plans (create_basis_plan): the request parts, parsed again
+// when materialized
+struct BasisPlan {
+  Value basis;
+  Value site_permutation;
+  Value checks;
+};
+std::map<uint64_t, std::shared_ptr<BasisPlan>> g_plans;
+std::atomic<uint64_t> g_next_plan{1};
+
+// parse_basis_plan_handle (capi :1706-1718)
+uint64_t parse_plan_handle(const std::string& h) {
+  char* end = nullptr;
+  errno = 0;
+  unsigned long long v = h.empty() ? 0 : std::strtoull(h.c_str(), &end, 10);
+  if (h.empty() || !end || *end || errno || h[0] == '-' || h[0] == '+') {
+    std::string q;
+    json::write_string(q, h);
+    bad("basis-plan handle " + q + " is not a positive integer");
+  }
+  if (v == 0) bad("basis-plan handle must be positive");
+  return v;
+}
+
+std::shared_ptr<BasisPlan> registered_plan(const Value& h) {
+  if (h.kind != Value::String) bad("plan_handle must be a string");
+  const uint64_t id = parse_plan_handle(h.raw);
+  std::lock_guard<std::mutex> g(g_reg_mu);
+  auto it = g_plans.find(id);
+  if (it == g_plans.end()) bad("basis-plan handle " + std::to_string(id) + " is not registered");
+  return it->second;
+}
 
 struct ExpmEntry {
   std::shared_ptr<Model> model;
@@ -593,19 +665,47 @@ std::string run_command(const std::string& req) {
     return ok(o);
   }
   if (op == "create_model") {
-    if (auto* p = v.get("site_permutation"); p && !p->is_null())
-      throw unsupported_backend("site permutations are not built on the MI355X path");
     auto m = make_model(need(v, "basis", "create_model"), need(v, "terms", "create_model"), v.get("checks"),
-                        v.get("components"));
-    uint64_t id = g_next_model.fetch_add(1);
-    size_t dim = m->basis->len();
+                        v.get("components"), v.get("site_permutation"));
+    return ok(register_model_json(m));
+  }
+  if (op == "create_basis_plan") {
+    // create_basis_plan (capi :3753-3768): the request kept for a later
+    // materialization; the basis is checked now
+    auto plan = std::make_shared<BasisPlan>();
+    plan->basis = need(v, "basis", "create_basis_plan");
+    if (auto* p = v.get("site_permutation"); p && !p->is_null()) plan->site_permutation = *p;
+    if (auto* c = v.get("checks"); c && !c->is_null()) plan->checks = *c;
+    (void)parse_basis(plan->basis);
+    (void)parse_checks(v.get("checks"));
+    if (auto* p = v.get("site_permutation"); p && !p->is_null()) (void)parse_site_permutation(*p);
+    const uint64_t id = g_next_plan.fetch_add(1);
     {
       std::lock_guard<std::mutex> g(g_reg_mu);
-      g_models[id] = m;
+      g_plans[id] = plan;
     }
-    return ok("{\"kind\":\"model\",\"handle\":\"" + std::to_string(id) + "\",\"dimension\":" + std::to_string(dim) +
-              "}");
+    return ok("{\"kind\":\"basis_plan\",\"handle\":\"" + std::to_string(id) + "\"}");
   }
+  if (op == "materialize_basis_plan") {
+    // materialize_basis_plan (capi :3770-3781): a model with no terms on the
+    // plan's basis, checks and site permutation
+    auto plan = registered_plan(need(v, "plan_handle", "materialize_basis_plan"));
+    json::Value empty;
+    empty.kind = json::Value::Arr;
+    empty.arr = std::make_shared<std::vector<json::Value>>();
+    auto m = make_model(plan->basis, empty, plan->checks.kind == json::Value::Null ? nullptr : &plan->checks, nullptr,
+                        plan->site_permutation.kind == json::Value::Null ? nullptr : &plan->site_permutation);
+    return ok(register_model_json(m));
+  }
+  if (op == "release_basis_plan") {
+    const Value& h = need(v, "plan_handle", "release_basis_plan");
+    if (h.kind != Value::String) bad("plan_handle must be a string");
+    const uint64_t id = parse_plan_handle(h.raw);
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    if (!g_plans.erase(id)) bad("basis-plan handle " + std::to_string(id) + " is not registered");
+    return ok("{\"kind\":\"released\",\"handle\":\"" + std::to_string(id) + "\"}");
+  }
+
   if (op == "eigsh_model") {
     const Value& h = need(v, "handle", "eigsh_model");
     if (h.kind != Value::String) bad("handle must be a string");

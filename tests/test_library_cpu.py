@@ -207,3 +207,23 @@ def test_general_symmetry_requests_checked_before_device_work(qa):
         "invalid options: built-in and general spin symmetries cannot be mixed"
     assert err(matrix_symmetry={"dimension": 2, "selected_row": 0, "generators": []}) == \
         "invalid options: matrix-valued symmetry sectors materialize as projected operator models"
+
+
+def test_site_permutation_and_basis_plan_requests_checked(qa):
+    # validate_site_permutation (src/interop.rs:1328-1345) and the basis-plan
+    # handles (bindings/capi/src/lib.rs:1706-1718, 1799-1823)
+    def err(req):
+        r = qa.raw_call("qmbed_command_json", req)
+        assert r["status"] == "error"
+        return r["error"]
+
+    base = {"operation": "create_model", "basis": {"kind": "spin", "sites": 3},
+            "terms": [{"product": {"local": ["z", "z"]}, "couplings": [{"coefficient": [1.0, 0.0], "sites": [0, 1]}]}]}
+    assert err({**base, "site_permutation": [0, 3, 1]}) == "site 3 is outside a lattice with 3 sites"
+    assert err({**base, "site_permutation": [0, 1, 1]}) == "invalid options: site permutation must be bijective"
+    assert err({"operation": "materialize_basis_plan", "plan_handle": "0"}) == \
+        "invalid options: basis-plan handle must be positive"
+    assert err({"operation": "materialize_basis_plan", "plan_handle": "x"}) == \
+        'invalid options: basis-plan handle "x" is not a positive integer'
+    assert err({"operation": "release_basis_plan", "plan_handle": "41"}) == \
+        "invalid options: basis-plan handle 41 is not registered"
