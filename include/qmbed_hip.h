@@ -115,6 +115,9 @@ long qmbed_hip_model_dimension(const qmbed_hip_model* model);
 int qmbed_hip_model_is_real(const qmbed_hip_model* model);
 /* host copies of the sorted representatives and their orbit sizes */
 int qmbed_hip_model_states(const qmbed_hip_model* model, uint32_t* states, uint8_t* orbits);
+// the representatives as 64-bit words in row order (sectors of any width;
+// qmbed_hip_model_states answers -2 for the wide sectors of L >= 32)
+int qmbed_hip_model_states64(const qmbed_hip_model* model, uint64_t* states, uint8_t* orbits);
 /* entries the reference's assembly would store for this operator (merged
  * per (row, column), |v| > machine epsilon, diagonal included); -1 on error */
 long qmbed_hip_model_nnz(const qmbed_hip_model* model);

@@ -58,6 +58,7 @@ def library():
     L.qmbed_hip_model_dimension.argtypes = [c_void_p]
     L.qmbed_hip_model_is_real.argtypes = [c_void_p]
     L.qmbed_hip_model_states.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.qmbed_hip_model_states64.argtypes = [c_void_p, c_void_p, c_void_p]
     L.qmbed_hip_model_nnz.restype = c_long
     L.qmbed_hip_model_nnz.argtypes = [c_void_p]
     L.qmbed_hip_model_hv_config.argtypes = [c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
@@ -251,9 +252,15 @@ class Model:
         return self._h
 
     def states(self):
+        """(representatives, orbit sizes) in row order: uint32 words, or
+        uint64 words for the wide sectors of 32 or more sites."""
         s = np.zeros(self.dim, dtype=np.uint32)
         o = np.zeros(self.dim, dtype=np.uint8)
-        if library().qmbed_hip_model_states(self._h, s.ctypes.data, o.ctypes.data) != 0:
+        rc = library().qmbed_hip_model_states(self._h, s.ctypes.data, o.ctypes.data)
+        if rc == -2:
+            s = np.zeros(self.dim, dtype=np.uint64)
+            rc = library().qmbed_hip_model_states64(self._h, s.ctypes.data, o.ctypes.data)
+        if rc != 0:
             raise QmbedError("state copy failed")
         return s, o
 

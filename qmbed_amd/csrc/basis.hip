@@ -49,70 +49,91 @@ constexpr int kMaxSplit = 8192;  // LDS splitter capacity (32 KiB)
 
 struct EnumParams {
   int L;
-  uint32_t mask;
+  uint64_t mask;
   int up;                // -1: enumerate 0..2^L-1
   uint64_t popmask;      // allowed popcounts when enumerating 0..2^L-1 (bit i: popcount i)
   int kmom;              // -1: no translation symmetry
   int parity;            // 0 none
   uint64_t n_parents;
-  const uint64_t* binom; // binom[n*33+k]
+  const uint64_t* binom; // binom[n*65+k], n, k <= 64
 };
 
-__device__ __forceinline__ uint32_t rotl_l(uint64_t y, int L, int j, uint32_t mask) {
-  // y = s | s << L ; rotate s left by j inside L bits
-  return uint32_t(y >> (L - j)) & mask;
+// States are 32-bit words for L <= 31 and 64-bit words (wide sectors) for
+// 32 <= L <= 63; these helpers are written once for both widths.
+template <typename ST>
+__device__ __forceinline__ int popc_st(ST s) {
+  if constexpr (sizeof(ST) == 4) return __popc(s);
+  else return __popcll(s);
 }
 
-__device__ __forceinline__ uint32_t reflect_l(uint32_t s, int L) { return __brev(s) >> (32 - L); }
+// s rotated left by j (0 <= j < L) inside L bits; a narrow state rotates
+// inside its doubled 64-bit word y = s | s << L
+template <typename ST>
+__device__ __forceinline__ ST rot_st(ST s, int j, int L, uint64_t mask) {
+  if constexpr (sizeof(ST) == 4) {
+    const uint64_t y = uint64_t(s) | (uint64_t(s) << L);
+    return uint32_t(y >> (L - j)) & uint32_t(mask);
+  } else {
+    return j ? (((s << j) | (s >> (L - j))) & mask) : s;
+  }
+}
 
-__device__ uint32_t unrank_colex(uint64_t rank, int L, int up, const uint64_t* binom) {
-  uint32_t s = 0;
+template <typename ST>
+__device__ __forceinline__ ST reflect_st(ST s, int L) {
+  if constexpr (sizeof(ST) == 4) return __brev(s) >> (32 - L);
+  else return __brevll(s) >> (64 - L);
+}
+
+template <typename ST>
+__device__ ST unrank_colex(uint64_t rank, int L, int up, const uint64_t* binom) {
+  ST s = 0;
   int hi = L - 1;
   for (int k = up; k >= 1; --k) {
     int lo = k - 1;
     int h = hi;
     while (lo < h) {
       int mid = (lo + h + 1) >> 1;
-      if (binom[mid * 33 + k] <= rank) lo = mid;
+      if (binom[mid * 65 + k] <= rank) lo = mid;
       else h = mid - 1;
     }
-    s |= 1u << lo;
-    rank -= binom[lo * 33 + k];
+    s |= ST(1) << lo;
+    rank -= binom[lo * 65 + k];
     hi = lo - 1;
   }
   return s;
 }
 
-__device__ __forceinline__ uint32_t gosper_next(uint32_t s) {
-  uint32_t low = s & (0u - s);
-  uint32_t ripple = s + low;
-  return (((ripple ^ s) >> 2) / low) | ripple;
+template <typename ST>
+__device__ __forceinline__ ST gosper_next(ST s) {
+  ST low = s & (ST(0) - s);
+  ST ripple = s + low;
+  if constexpr (sizeof(ST) == 4) return (((ripple ^ s) >> 2) / low) | ripple;
+  else return (((ripple ^ s) >> 2) >> __ffsll(static_cast<long long>(low)) - 1) | ripple;  // low is a power of two
 }
 
 // Returns the orbit size (>0) if s is a kept representative, 0 otherwise.
-__device__ int rep_orbit(uint32_t s, const EnumParams& p) {
+template <typename ST>
+__device__ int rep_orbit(ST s, const EnumParams& p) {
   const int L = p.L;
   if (p.kmom < 0 && p.parity == 0) return 1;
   if (p.kmom < 0) {  // reflection only: orbit {s, P s}
-    uint32_t ps = reflect_l(s, L);
+    ST ps = reflect_st(s, L);
     if (ps < s) return 0;
     if (ps == s) return p.parity == 1 ? 1 : 0;
     return 2;
   }
-  uint64_t y = uint64_t(s) | (uint64_t(s) << L);
   int d = L;
   for (int j = 1; j < L; ++j) {
-    uint32_t r = rotl_l(y, L, j, p.mask);
+    ST r = rot_st(s, j, L, p.mask);
     if (r < s) return 0;
     if (r == s && d == L) d = j;
   }
   if ((p.kmom * d) % L != 0) return 0;
   if (p.parity == 0) return d;
-  uint32_t ps = reflect_l(s, L);
-  uint64_t yp = uint64_t(ps) | (uint64_t(ps) << L);
+  ST ps = reflect_st(s, L);
   int jm = -1;
   for (int j = 0; j < L; ++j) {
-    uint32_t r = rotl_l(yp, L, j, p.mask);
+    ST r = rot_st(ps, j, L, p.mask);
     if (r < s) return 0;
     if (r == s && jm < 0) jm = j;
   }
@@ -122,22 +143,22 @@ __device__ int rep_orbit(uint32_t s, const EnumParams& p) {
   return (p.parity * chi == 1) ? d : 0;
 }
 
-template <bool WRITE>
+template <typename ST, bool WRITE>
 __global__ void __launch_bounds__(kThreads) enumerate_kernel(EnumParams p, uint32_t* block_counts,
-                                                             const uint64_t* block_offsets, uint32_t* states,
+                                                             const uint64_t* block_offsets, ST* states,
                                                              uint8_t* orbits) {
   __shared__ uint32_t scan[kThreads];
   const uint64_t tid = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
   const uint64_t first = tid * kChunk;
   uint32_t count = 0;
-  uint32_t s = 0;
+  ST s = 0;
   int todo = 0;
   if (first < p.n_parents) {
     todo = int(min<uint64_t>(kChunk, p.n_parents - first));
-    s = (p.up >= 0) ? unrank_colex(first, p.L, p.up, p.binom) : uint32_t(first);
-    uint32_t t = s;
+    s = (p.up >= 0) ? unrank_colex<ST>(first, p.L, p.up, p.binom) : ST(first);
+    ST t = s;
     for (int i = 0; i < todo; ++i) {
-      bool ok = (p.up >= 0) || ((p.popmask >> __popc(t)) & 1ull);
+      bool ok = (p.up >= 0) || ((p.popmask >> popc_st(t)) & 1ull);
       if (ok && rep_orbit(t, p) > 0) ++count;
       if (i + 1 < todo) t = (p.up >= 0) ? gosper_next(t) : t + 1;
     }
@@ -156,9 +177,9 @@ __global__ void __launch_bounds__(kThreads) enumerate_kernel(EnumParams p, uint3
     return;
   }
   uint64_t out = block_offsets[blockIdx.x] + scan[threadIdx.x] - count;
-  uint32_t t = s;
+  ST t = s;
   for (int i = 0; i < todo; ++i) {
-    bool ok = (p.up >= 0) || ((p.popmask >> __popc(t)) & 1ull);
+    bool ok = (p.up >= 0) || ((p.popmask >> popc_st(t)) & 1ull);
     if (ok) {
       int o = rep_orbit(t, p);
       if (o > 0) {
@@ -173,32 +194,35 @@ __global__ void __launch_bounds__(kThreads) enumerate_kernel(EnumParams p, uint3
 
 // g(s) for a general group element: the bit of site i moves to dest[i], then
 // the flipped digits toggle
-__device__ __forceinline__ uint32_t group_apply(uint32_t s, const uint8_t* __restrict__ dest, uint32_t flip, int L) {
-  uint32_t out = 0;
-  for (int i = 0; i < L; ++i) out |= ((s >> i) & 1u) << dest[i];
-  return out ^ flip;
+template <typename ST>
+__device__ __forceinline__ ST group_apply(ST s, const uint8_t* __restrict__ dest, uint64_t flip, int L) {
+  ST out = 0;
+  for (int i = 0; i < L; ++i) out |= ((s >> i) & ST(1)) << dest[i];
+  return out ^ ST(flip);
 }
 
 struct GroupParams {
-  const uint8_t* dest;   // [n][32]
-  const uint32_t* flip;  // [n]
+  const uint8_t* dest;   // [n][64]
+  const uint64_t* flip;  // [n]
   const uint8_t* phase;  // [n]: character index of χ(g⁻¹) (0 iff χ(g) = 1)
   int n;
   int max_rep;
 };
 
-__device__ __forceinline__ bool in_parent(uint32_t t, const EnumParams& p) {
-  return p.up >= 0 ? __popc(t) == p.up : ((p.popmask >> __popc(t)) & 1ull) != 0;
+template <typename ST>
+__device__ __forceinline__ bool in_parent(ST t, const EnumParams& p) {
+  return p.up >= 0 ? popc_st(t) == p.up : ((p.popmask >> popc_st(t)) & 1ull) != 0;
 }
 
 // s seeds a compatible orbit: returns its orbit size and representative
 // (0 when s is not the seed or the orbit does not survive the projection)
-__device__ int general_rep_orbit(uint32_t s, const EnumParams& p, const GroupParams& gp, uint32_t& rep) {
+template <typename ST>
+__device__ int general_rep_orbit(ST s, const EnumParams& p, const GroupParams& gp, ST& rep) {
   rep = s;
   int stab = 0;
   bool ok = true;
   for (int g = 0; g < gp.n; ++g) {
-    const uint32_t t = group_apply(s, gp.dest + 32 * g, gp.flip[g], p.L);
+    const ST t = group_apply(s, gp.dest + 64 * g, gp.flip[g], p.L);
     if (t == s) {
       ++stab;
       if (gp.phase[g] != 0) ok = false;
@@ -210,21 +234,21 @@ __device__ int general_rep_orbit(uint32_t s, const EnumParams& p, const GroupPar
   return ok ? gp.n / stab : 0;
 }
 
-template <bool WRITE>
+template <typename ST, bool WRITE>
 __global__ void __launch_bounds__(kThreads) general_enumerate_kernel(EnumParams p, GroupParams gp,
                                                                      uint32_t* block_counts,
-                                                                     const uint64_t* block_offsets, uint32_t* states,
+                                                                     const uint64_t* block_offsets, ST* states,
                                                                      uint8_t* orbits) {
   __shared__ uint32_t scan[kThreads];
   const uint64_t tid = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
   const uint64_t first = tid * kChunk;
   uint32_t count = 0;
-  uint32_t s = 0;
+  ST s = 0;
   int todo = 0;
   if (first < p.n_parents) {
     todo = int(min<uint64_t>(kChunk, p.n_parents - first));
-    s = (p.up >= 0) ? unrank_colex(first, p.L, p.up, p.binom) : uint32_t(first);
-    uint32_t t = s, rep;
+    s = (p.up >= 0) ? unrank_colex<ST>(first, p.L, p.up, p.binom) : ST(first);
+    ST t = s, rep;
     for (int i = 0; i < todo; ++i) {
       if (in_parent(t, p) && general_rep_orbit(t, p, gp, rep) > 0) ++count;
       if (i + 1 < todo) t = (p.up >= 0) ? gosper_next(t) : t + 1;
@@ -243,7 +267,7 @@ __global__ void __launch_bounds__(kThreads) general_enumerate_kernel(EnumParams 
     return;
   }
   uint64_t out = block_offsets[blockIdx.x] + scan[threadIdx.x] - count;
-  uint32_t t = s, rep;
+  ST t = s, rep;
   for (int i = 0; i < todo; ++i) {
     if (in_parent(t, p)) {
       const int o = general_rep_orbit(t, p, gp, rep);
@@ -273,15 +297,15 @@ __global__ void splitter_kernel(const uint32_t* states, uint64_t n, int shift, i
 
 uint64_t binom_host(int n, int k) {
   if (k < 0 || k > n) return 0;
-  uint64_t r = 1;
-  for (int i = 1; i <= k; ++i) r = r * uint64_t(n - k + i) / uint64_t(i);
-  return r;
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; ++i) r = r * unsigned(n - k + i) / unsigned(i);
+  return uint64_t(r);
 }
 
 // The symmetry group of a general sector, closed on the host.
 struct GroupElement {
-  std::array<uint8_t, 32> dest{};
-  uint32_t flip = 0;
+  std::array<uint8_t, 64> dest{};
+  uint64_t flip = 0;
   long chi = 0;  // χ(g) = exp(-2πi chi / M)
 };
 
@@ -309,8 +333,8 @@ long gcd_l(long a, long b) {
 // 3221-3268): map checks, generator periods, group closure
 Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
   struct Gen {
-    std::array<uint8_t, 32> dest{};
-    uint32_t flip = 0;  // destination-side flip mask
+    std::array<uint8_t, 64> dest{};
+    uint64_t flip = 0;  // destination-side flip mask
     long period = 1;
     long sector = 0;
   };
@@ -339,7 +363,7 @@ Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
     Gen g;
     for (int i = 0; i < L; ++i) {
       g.dest[size_t(i)] = uint8_t(m.destinations[size_t(i)]);
-      if (flipped[size_t(i)]) g.flip |= 1u << m.destinations[size_t(i)];
+      if (flipped[size_t(i)]) g.flip |= uint64_t(1) << m.destinations[size_t(i)];
     }
     // combined_permutation_period: lcm of the cycles of (site, digit)
     std::vector<char> visited(size_t(2 * L), 0);
@@ -370,7 +394,7 @@ Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
   }
   // breadth-first closure from the identity: h = gen ∘ e, i.e.
   // h(s) = P_gen(P_e(s) ^ f_e) ^ f_gen
-  std::map<std::pair<std::array<uint8_t, 32>, uint32_t>, size_t> index;
+  std::map<std::pair<std::array<uint8_t, 64>, uint64_t>, size_t> index;
   GroupElement id;
   for (int i = 0; i < L; ++i) id.dest[size_t(i)] = uint8_t(i);
   G.elems.push_back(id);
@@ -380,9 +404,9 @@ Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
       const GroupElement e = G.elems[head];
       GroupElement h;
       for (int i = 0; i < L; ++i) h.dest[size_t(i)] = g.dest[e.dest[size_t(i)]];
-      uint32_t pf = 0;
+      uint64_t pf = 0;
       for (int i = 0; i < L; ++i)
-        if ((e.flip >> i) & 1u) pf |= 1u << g.dest[size_t(i)];
+        if ((e.flip >> i) & 1u) pf |= uint64_t(1) << g.dest[size_t(i)];
       h.flip = pf ^ g.flip;
       h.chi = (e.chi + g.sector * (G.M / g.period)) % G.M;
       auto it = index.find({h.dest, h.flip});
@@ -399,6 +423,77 @@ Group close_group(const std::vector<SymmetryMapSpec>& maps, int L) {
   return G;
 }
 
+// Enumerate the sector's representatives into `states` (ascending, padded
+// with all-ones words to whole 64-byte lines) and orbit sizes, and their
+// reversal when rows descend; returns the dimension.
+template <typename ST>
+uint64_t enumerate_sector(EnumParams p, const GroupParams& gp, bool general, bool reversed, DeviceArray<ST>& states,
+                          DeviceArray<ST>& rstates, DeviceArray<uint8_t>& orbits, DeviceArray<uint8_t>& rorbits,
+                          hipStream_t stream) {
+  const uint64_t threads = (p.n_parents + kChunk - 1) / kChunk;
+  const uint64_t blocks = std::max<uint64_t>(1, (threads + kThreads - 1) / kThreads);
+  if (blocks >= (uint64_t(1) << 31)) throw unsupported_backend("the parent sector is too large to enumerate");
+  DeviceArray<uint32_t> d_counts(blocks);
+  DeviceArray<uint64_t> d_offsets(blocks);
+  if (general)
+    hipLaunchKernelGGL((general_enumerate_kernel<ST, false>), dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p,
+                       gp, d_counts.data(), nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((enumerate_kernel<ST, false>), dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p,
+                       d_counts.data(), nullptr, nullptr, nullptr);
+  QMBED_HIP(hipGetLastError());
+  std::vector<uint32_t> counts(blocks);
+  QMBED_HIP(hipMemcpyAsync(counts.data(), d_counts.data(), blocks * 4, hipMemcpyDeviceToHost, stream));
+  QMBED_HIP(hipStreamSynchronize(stream));
+  std::vector<uint64_t> offsets(blocks);
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < blocks; ++i) {
+    offsets[i] = total;
+    total += counts[i];
+  }
+  const uint64_t line = 64 / sizeof(ST);
+  const uint64_t padded = std::max<uint64_t>(line, (total + line - 1) / line * line);
+  states.resize(padded);
+  QMBED_HIP(hipMemsetAsync(states.data(), 0xFF, padded * sizeof(ST), stream));
+  orbits.resize(std::max<uint64_t>(total, 1));
+  QMBED_HIP(hipMemcpyAsync(d_offsets.data(), offsets.data(), blocks * 8, hipMemcpyHostToDevice, stream));
+  if (general) {
+    hipLaunchKernelGGL((general_enumerate_kernel<ST, true>), dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, gp,
+                       nullptr, d_offsets.data(), states.data(), orbits.data());
+    QMBED_HIP(hipGetLastError());
+    // the representatives in ascending order (seeds are ascending, their
+    // orbit maxima, or minima outside the parent, need not be)
+    if (total > 1) {
+      if (total >= (uint64_t(1) << 31)) throw unsupported_backend("general symmetry sector too large to sort");
+      DeviceArray<ST> keys(total);
+      DeviceArray<uint8_t> vals(total);
+      QMBED_HIP(hipMemcpyAsync(keys.data(), states.data(), total * sizeof(ST), hipMemcpyDeviceToDevice, stream));
+      QMBED_HIP(hipMemcpyAsync(vals.data(), orbits.data(), total, hipMemcpyDeviceToDevice, stream));
+      size_t tmp_bytes = 0;
+      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.data(), states.data(), vals.data(),
+                                                   orbits.data(), int(total), 0, p.L, stream));
+      DeviceArray<uint8_t> tmp(std::max<size_t>(tmp_bytes, 1));
+      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, keys.data(), states.data(), vals.data(),
+                                                   orbits.data(), int(total), 0, p.L, stream));
+      QMBED_HIP(hipStreamSynchronize(stream));
+    }
+  } else {
+    hipLaunchKernelGGL((enumerate_kernel<ST, true>), dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, nullptr,
+                       d_offsets.data(), states.data(), orbits.data());
+    QMBED_HIP(hipGetLastError());
+  }
+  if (reversed && total > 0) {
+    rstates.resize(padded);
+    QMBED_HIP(hipMemsetAsync(rstates.data(), 0xFF, padded * sizeof(ST), stream));
+    rorbits.resize(total);
+    const unsigned rb = unsigned((total + 255) / 256);
+    hipLaunchKernelGGL(reverse_kernel<ST>, dim3(rb), dim3(256), 0, stream, states.data(), total, rstates.data());
+    hipLaunchKernelGGL(reverse_kernel<uint8_t>, dim3(rb), dim3(256), 0, stream, orbits.data(), total, rorbits.data());
+    QMBED_HIP(hipGetLastError());
+  }
+  return total;
+}
+
 }  // namespace
 
 std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hipStream_t stream) {
@@ -411,7 +506,8 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   if (spec.spin_twice != 1)
     throw unsupported_backend("the MI355X spin engine implements spin one-half sectors");
   const int L = spec.sites;
-  if (L > 31) throw unsupported_backend("the MI355X spin engine packs states in 32 bits (at most 31 sites)");
+  if (L > 63) throw unsupported_backend("the MI355X spin engine packs states in 64 bits (at most 63 sites)");
+  const bool wide = L > 31;  // 64-bit states
   if (spec.up && *spec.up > L) throw invalid_sector("particle count exceeds site count");
   for (int u : spec.up_sectors)
     if (u < 0 || u > L) throw invalid_sector("particle sector exceeds the spin capacity");
@@ -435,13 +531,12 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   b->spec_ = spec;
   SectorGeometry& g = b->geo_;
   g.L = L;
-  g.mask = (L == 32) ? 0xFFFFFFFFu : ((1u << L) - 1u);
+  g.mask = (uint64_t(1) << L) - 1u;
+  g.wide = wide;
   g.kmom = kmom;
   g.parity = spec.parity ? *spec.parity : 0;
   for (int d = 1; d < L; ++d)
     if (L % d == 0 && g.n_div < 16) g.divisors[g.n_div++] = d;
-  if ((kmom >= 0 || g.parity) && L == 32)
-    throw unsupported_backend("symmetry-reduced sectors support at most 31 sites");
 
   EnumParams p{};
   p.L = L;
@@ -452,7 +547,7 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
     p.up = *spec.up;
     p.n_parents = binom_host(L, p.up);
   } else {
-    if (L > 31) throw unsupported_backend("unconstrained sectors support at most 31 sites");
+    if (L > 36) throw unsupported_backend("sectors without a fixed up count support at most 36 sites");
     p.up = -1;
     p.n_parents = uint64_t(1) << L;
     p.popmask = 0;
@@ -462,9 +557,9 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
       for (int u : spec.up_sectors) p.popmask |= 1ull << u;
     }
   }
-  std::vector<uint64_t> binom(33 * 33);
-  for (int n = 0; n <= 32; ++n)
-    for (int k = 0; k <= 32; ++k) binom[n * 33 + k] = binom_host(n, k);
+  std::vector<uint64_t> binom(65 * 65);
+  for (int n = 0; n <= 64; ++n)
+    for (int k = 0; k <= 64; ++k) binom[n * 65 + k] = binom_host(n, k);
   DeviceArray<uint64_t> d_binom(binom.size());
   QMBED_HIP(hipMemcpyAsync(d_binom.data(), binom.data(), binom.size() * 8, hipMemcpyHostToDevice, stream));
   p.binom = d_binom.data();
@@ -498,10 +593,10 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
       b->chars_.push_back(v);
     }
     const size_t n = G.elems.size();
-    std::vector<uint8_t> dest(32 * n, 0), phase(n, 0);
-    std::vector<uint32_t> flip(n, 0);
+    std::vector<uint8_t> dest(64 * n, 0), phase(n, 0);
+    std::vector<uint64_t> flip(n, 0);
     for (size_t i = 0; i < n; ++i) {
-      std::copy(G.elems[i].dest.begin(), G.elems[i].dest.end(), dest.begin() + long(32 * i));
+      std::copy(G.elems[i].dest.begin(), G.elems[i].dest.end(), dest.begin() + long(64 * i));
       flip[i] = G.elems[i].flip;
       const long c = (M - G.elems[i].chi) % M;
       phase[i] = uint8_t(std::lower_bound(vals.begin(), vals.end(), c) - vals.begin());
@@ -510,7 +605,7 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
     b->gflip_.resize(n);
     b->gphase_.resize(n);
     QMBED_HIP(hipMemcpyAsync(b->gdest_.data(), dest.data(), dest.size(), hipMemcpyHostToDevice, stream));
-    QMBED_HIP(hipMemcpyAsync(b->gflip_.data(), flip.data(), n * 4, hipMemcpyHostToDevice, stream));
+    QMBED_HIP(hipMemcpyAsync(b->gflip_.data(), flip.data(), n * 8, hipMemcpyHostToDevice, stream));
     QMBED_HIP(hipMemcpyAsync(b->gphase_.data(), phase.data(), n, hipMemcpyHostToDevice, stream));
     g.n_group = int(n);
     g.n_chars = int(vals.size());
@@ -523,69 +618,23 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
   }
   g.reversed = spec.reverse;
 
-  const uint64_t threads = group_empty ? 0 : (p.n_parents + kChunk - 1) / kChunk;
-  const uint64_t blocks = std::max<uint64_t>(1, (threads + kThreads - 1) / kThreads);
-  DeviceArray<uint32_t> d_counts(blocks);
-  DeviceArray<uint64_t> d_offsets(blocks);
   if (group_empty) p.n_parents = 0;  // every orbit trace conflicts: an empty sector
-  if (g.n_group)
-    hipLaunchKernelGGL(general_enumerate_kernel<false>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, gp,
-                       d_counts.data(), nullptr, nullptr, nullptr);
-  else
-    hipLaunchKernelGGL(enumerate_kernel<false>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p,
-                       d_counts.data(), nullptr, nullptr, nullptr);
-  QMBED_HIP(hipGetLastError());
-  std::vector<uint32_t> counts(blocks);
-  QMBED_HIP(hipMemcpyAsync(counts.data(), d_counts.data(), blocks * 4, hipMemcpyDeviceToHost, stream));
-  QMBED_HIP(hipStreamSynchronize(stream));
-  std::vector<uint64_t> offsets(blocks);
   uint64_t total = 0;
-  for (uint64_t i = 0; i < blocks; ++i) {
-    offsets[i] = total;
-    total += counts[i];
+  if (wide) {
+    total = enumerate_sector<uint64_t>(p, gp, g.n_group > 0, g.reversed, b->states64_, b->rstates64_, b->orbits_,
+                                       b->rorbits_, stream);
+  } else {
+    total = enumerate_sector<uint32_t>(p, gp, g.n_group > 0, g.reversed, b->states_, b->rstates_, b->orbits_,
+                                       b->rorbits_, stream);
   }
   b->n_ = total;
-  const uint64_t padded = std::max<uint64_t>(16, (total + 15) / 16 * 16);
-  b->states_.resize(padded);
-  QMBED_HIP(hipMemsetAsync(b->states_.data(), 0xFF, padded * 4, stream));
-  b->orbits_.resize(std::max<uint64_t>(total, 1));
-  QMBED_HIP(hipMemcpyAsync(d_offsets.data(), offsets.data(), blocks * 8, hipMemcpyHostToDevice, stream));
-  if (g.n_group) {
-    hipLaunchKernelGGL(general_enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, gp,
-                       nullptr, d_offsets.data(), b->states_.data(), b->orbits_.data());
-    QMBED_HIP(hipGetLastError());
-    // the representatives in ascending order (seeds are ascending, their
-    // orbit maxima, or minima outside the parent, need not be)
-    if (total > 1) {
-      if (total >= (uint64_t(1) << 31)) throw unsupported_backend("general symmetry sector too large to sort");
-      DeviceArray<uint32_t> keys(total);
-      DeviceArray<uint8_t> vals(total);
-      QMBED_HIP(hipMemcpyAsync(keys.data(), b->states_.data(), total * 4, hipMemcpyDeviceToDevice, stream));
-      QMBED_HIP(hipMemcpyAsync(vals.data(), b->orbits_.data(), total, hipMemcpyDeviceToDevice, stream));
-      size_t tmp_bytes = 0;
-      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.data(), b->states_.data(), vals.data(),
-                                                   b->orbits_.data(), int(total), 0, L, stream));
-      DeviceArray<uint8_t> tmp(std::max<size_t>(tmp_bytes, 1));
-      QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, keys.data(), b->states_.data(),
-                                                   vals.data(), b->orbits_.data(), int(total), 0, L, stream));
-      QMBED_HIP(hipStreamSynchronize(stream));
-    }
-  } else {
-    hipLaunchKernelGGL(enumerate_kernel<true>, dim3(unsigned(blocks)), dim3(kThreads), 0, stream, p, nullptr,
-                       d_offsets.data(), b->states_.data(), b->orbits_.data());
-    QMBED_HIP(hipGetLastError());
+  if (wide) {
+    // wide sectors are searched by a plain binary search (build-time
+    // kernels only: their products run on the compiled stream)
+    QMBED_HIP(hipStreamSynchronize(stream));
+    return b;
   }
-  if (g.reversed && total > 0) {
-    b->rstates_.resize(padded);
-    QMBED_HIP(hipMemsetAsync(b->rstates_.data(), 0xFF, padded * 4, stream));
-    b->rorbits_.resize(total);
-    const unsigned rb = unsigned((total + 255) / 256);
-    hipLaunchKernelGGL(reverse_kernel<uint32_t>, dim3(rb), dim3(256), 0, stream, b->states_.data(), total,
-                       b->rstates_.data());
-    hipLaunchKernelGGL(reverse_kernel<uint8_t>, dim3(rb), dim3(256), 0, stream, b->orbits_.data(), total,
-                       b->rorbits_.data());
-    QMBED_HIP(hipGetLastError());
-  }
+  const uint64_t padded = b->states_.size();
 
   // search splitters for the LDS top of the sorted-state lookup
   int shift = 0;
@@ -626,8 +675,21 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
 }
 
 std::vector<uint32_t> SpinBasis1D::states_host() const {
+  if (wide()) throw unsupported_backend("the sector's states are 64-bit words");
   std::vector<uint32_t> h(n_);
   if (n_) QMBED_HIP(hipMemcpy(h.data(), states_.data(), n_ * 4, hipMemcpyDeviceToHost));
+  return h;
+}
+
+std::vector<uint64_t> SpinBasis1D::row_states_host64() const {
+  std::vector<uint64_t> h(n_);
+  if (wide()) {
+    if (n_) QMBED_HIP(hipMemcpy(h.data(), states64_.data(), n_ * 8, hipMemcpyDeviceToHost));
+  } else {
+    const auto narrow = states_host();
+    std::copy(narrow.begin(), narrow.end(), h.begin());
+  }
+  if (reversed()) std::reverse(h.begin(), h.end());
   return h;
 }
 

@@ -655,7 +655,7 @@ std::string run_command(const std::string& req) {
   if (op == "describe_basis") {
     SpinBasisSpec spec = parse_basis(need(v, "basis", "describe_basis"));
     auto b = SpinBasis1D::build(spec, nullptr);
-    auto states = b->row_states_host();
+    auto states = b->row_states_host64();
     std::string o = "{\"kind\":\"basis\",\"dimension\":" + std::to_string(b->len()) + ",\"states\":[";
     for (size_t i = 0; i < states.size(); ++i) {
       if (i) o += ',';
@@ -720,7 +720,7 @@ std::string run_command(const std::string& req) {
     const Value& h = need(v, "handle", "describe_model");
     if (h.kind != Value::String) bad("handle must be a string");
     auto m = registered_model(h.raw);
-    auto states = m->basis->row_states_host();
+    auto states = m->basis->row_states_host64();
     std::string o = "{\"kind\":\"basis\",\"dimension\":" + std::to_string(m->basis->len()) + ",\"states\":[";
     for (size_t i = 0; i < states.size(); ++i) {
       if (i) o += ',';
@@ -1133,8 +1133,24 @@ int qmbed_hip_model_is_real(const qmbed_hip_model* m) {
 int qmbed_hip_model_states(const qmbed_hip_model* m, uint32_t* states, uint8_t* orbits) {
   try {
     auto& b = *reinterpret_cast<const TypedModel*>(m)->m->basis;
+    if (b.wide()) return -2;
     auto s = b.row_states_host();
     std::memcpy(states, s.data(), s.size() * 4);
+    if (orbits) {
+      auto o = b.row_orbits_host();
+      std::memcpy(orbits, o.data(), o.size());
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
+int qmbed_hip_model_states64(const qmbed_hip_model* m, uint64_t* states, uint8_t* orbits) {
+  try {
+    auto& b = *reinterpret_cast<const TypedModel*>(m)->m->basis;
+    auto s = b.row_states_host64();
+    std::memcpy(states, s.data(), s.size() * 8);
     if (orbits) {
       auto o = b.row_orbits_host();
       std::memcpy(orbits, o.data(), o.size());

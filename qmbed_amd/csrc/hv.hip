@@ -33,7 +33,7 @@ namespace qmbed {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxGroups = 128;
+constexpr int kMaxGroups = 256;  // a J1-J2 chain of 63 sites holds 252
 constexpr int kMaxAmps = 1024;
 constexpr int kMaxChars = 64;  // character-table entries staged in LDS
 
@@ -42,6 +42,13 @@ struct HvArgs {
   const uint32_t* rstates;  // representatives in row order
   const uint8_t* orbits;    // orbit sizes in row order
   int rev;                  // rows descending: row = n - 1 - ascending index
+  // wide sectors (L >= 32): 64-bit representatives (ascending, and in row
+  // order), the 64-bit lattice mask and per group [flip, sites (6 bits
+  // each above a 3-bit count), nonzero-pattern mask, amplitude offset]
+  const uint64_t* states64;
+  const uint64_t* rstates64;
+  uint64_t mask64;
+  const uint64_t* meta64;
   uint32_t n;
   int L;
   uint32_t mask;
@@ -79,7 +86,7 @@ struct HvArgs {
   int nchar;
   // general symmetry group (ngroup elements; 0: built-in symmetries)
   const uint8_t* gdest;   // [ngroup][32] site destinations
-  const uint32_t* gflip;  // destination-side flip masks
+  const uint64_t* gflip;  // destination-side flip masks
   const uint8_t* gphase;  // character index of χ(g⁻¹)
   int ngroup;
   int maxrep;             // representatives are orbit maxima
@@ -97,6 +104,11 @@ struct HvArgs {
   // c_ob + c_cb) | χ-index << (c_b + c_ob + c_cb + c_sb); no group walk
   int compact;
   int c_b, c_ob, c_cb, c_sb, c_off_base;
+  // split compact words (the fields do not fit 32 bits): the word holds
+  // idx | entry << c_b, and one byte per word (same layout as the stream)
+  // holds the fields above, orbit code | sign << c_cb | χ-index << (c_cb +
+  // c_sb); null otherwise
+  const uint8_t* aux;
   // per-row diagonal (re[n], then im[n]) of the current coefficients, or null
   const double* diag;
 };
@@ -423,14 +435,15 @@ __device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep,
 // orbit size |G| / |Stab(t)| and the phase χ(h⁻¹) of the first element h
 // with h(t) = rep (t = h⁻¹ rep). Elements whose images tie differ by a
 // stabilizer element, whose character is 1 on every orbit of the sector.
-__device__ __forceinline__ void general_walk(uint32_t t, const HvArgs& a, uint32_t& rep, int& orbit, int& ci) {
+template <typename ST>
+__device__ __forceinline__ void general_walk(ST t, const HvArgs& a, ST& rep, int& orbit, int& ci) {
   rep = t;
   int best = 0, stab = 0;
   for (int g = 0; g < a.ngroup; ++g) {
-    const uint8_t* d = a.gdest + 32 * g;
-    uint32_t u = 0;
-    for (int i = 0; i < a.L; ++i) u |= ((t >> i) & 1u) << d[i];
-    u ^= a.gflip[g];
+    const uint8_t* d = a.gdest + 64 * g;
+    ST u = 0;
+    for (int i = 0; i < a.L; ++i) u |= ((t >> i) & ST(1)) << d[i];
+    u ^= ST(a.gflip[g]);
     stab += u == t ? 1 : 0;
     if (a.maxrep ? u > rep : u < rep) {
       rep = u;
@@ -762,8 +775,130 @@ struct ColList {
   bool overflow = false;
 };
 
-__device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_needed, uint32_t t, const uint32_t* spl,
+// Wide sectors (64-bit states): the translation/reflection orbit walk one
+// rotation at a time (rep, orbit size, phase = sign·χ(ci) as in walk_scan),
+// and a plain binary search of the ascending representatives; these serve
+// the one-time build kernels only (products run on the compiled stream).
+__device__ __forceinline__ uint64_t rot64(uint64_t s, int j, int L, uint64_t mask) {
+  return j ? (((s << j) | (s >> (L - j))) & mask) : s;
+}
+
+__device__ void walk64(uint64_t t, const HvArgs& a, int sym, uint64_t& rep, int& orbit, int& ci, int& sign) {
+  const int L = a.L;
+  sign = 1;
+  ci = 0;
+  const uint64_t ps = __brevll(t) >> (64 - L);
+  if (sym == 2) {
+    if (ps < t) {
+      rep = ps;
+      orbit = 2;
+      sign = a.parity;
+    } else {
+      rep = t;
+      orbit = (ps == t) ? 1 : 2;
+    }
+    return;
+  }
+  uint64_t mn = t;
+  int jmin = 0, d = L;
+  for (int j = 1; j < L; ++j) {
+    const uint64_t r = rot64(t, j, L, a.mask64);
+    if (r < mn) {
+      mn = r;
+      jmin = j;
+    }
+    if (r == t && d == L) d = j;
+  }
+  if (sym == 1) {
+    rep = mn;
+    orbit = d;
+    ci = (L - jmin) % L;
+    return;
+  }
+  uint64_t mnp = ps;
+  int jminp = 0;
+  for (int j = 1; j < L; ++j) {
+    const uint64_t r = rot64(ps, j, L, a.mask64);
+    if (r < mnp) {
+      mnp = r;
+      jminp = j;
+    }
+  }
+  if (mnp < mn) {
+    rep = mnp;
+    orbit = 2 * d;
+    ci = (L - jminp) % L;
+    sign = a.parity;
+  } else {
+    rep = mn;
+    orbit = (mnp == mn) ? d : 2 * d;
+    ci = (L - jmin) % L;
+  }
+}
+
+__device__ __forceinline__ int find_state64(uint64_t key, const uint64_t* __restrict__ states, uint32_t n) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (states[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && states[lo] == key) ? int(lo) : -1;
+}
+
+// the accessors of a row's state and of a group's flip mask, site pattern,
+// nonzero-pattern mask and amplitude offset, for either state width
+template <typename ST>
+__device__ __forceinline__ ST row_state(const HvArgs& a, uint32_t row) {
+  if constexpr (sizeof(ST) == 4) return a.rstates[row];
+  else return a.rstates64[row];
+}
+
+__device__ __forceinline__ int pattern_of64(uint64_t r, uint64_t packed) {
+  const int ns = int(packed & 7u);
+  int pat = 0;
+  for (int i = 0; i < ns; ++i) {
+    const int site = int((packed >> (3 + 6 * i)) & 63u);
+    pat |= int((r >> site) & 1u) << i;
+  }
+  return pat;
+}
+
+template <typename ST>
+__device__ __forceinline__ ST g_flip(const HvArgs& a, const uint32_t* meta, int g) {
+  if constexpr (sizeof(ST) == 4) return meta[4 * g];
+  else return a.meta64[4 * g];
+}
+template <typename ST>
+__device__ __forceinline__ int g_pattern(const HvArgs& a, const uint32_t* meta, ST r, int g) {
+  if constexpr (sizeof(ST) == 4) return pattern_of(r, meta[4 * g + 1]);
+  else return pattern_of64(r, a.meta64[4 * g + 1]);
+}
+template <typename ST>
+__device__ __forceinline__ uint32_t g_nz(const HvArgs& a, const uint32_t* meta, int g) {
+  if constexpr (sizeof(ST) == 4) return meta[4 * g + 2];
+  else return uint32_t(a.meta64[4 * g + 2]);
+}
+template <typename ST>
+__device__ __forceinline__ int g_off(const HvArgs& a, const uint32_t* meta, int g) {
+  if constexpr (sizeof(ST) == 4) return int(meta[4 * g + 3]);
+  else return int(a.meta64[4 * g + 3]);
+}
+
+template <typename ST>
+__device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_needed, ST t, const uint32_t* spl,
                                           int& idx, int& orbit, int& ci, int& sign) {
+  if constexpr (sizeof(ST) == 8) {
+    uint64_t rep = t;
+    orbit = 1;
+    ci = 0;
+    sign = 1;
+    if (sym == 4) general_walk(t, a, rep, orbit, ci);
+    else if (sym) walk64(t, a, sym, rep, orbit, ci, sign);
+    idx = find_state64(rep, a.states64, a.n);
+    if (idx >= 0 && a.rev) idx = int(a.n) - 1 - idx;
+    return idx >= 0;
+  } else {
   uint32_t rep = t;
   orbit = 1;
   ci = 0;
@@ -782,6 +917,7 @@ __device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_ne
   idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
   if (idx >= 0 && a.rev) idx = int(a.n) - 1 - idx;
   return idx >= 0;
+  }
 }
 
 // contribution value c·a·sqrt(N_src/N_t)·conj(sign·χ(ci))
@@ -801,24 +937,24 @@ __device__ __forceinline__ void column_value(const HvArgs& a, const double* famp
   }
 }
 
-template <int CAP>
+template <typename ST, int CAP>
 __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
                                const uint32_t* spl, uint32_t col, ColList<CAP>& out) {
-  const uint32_t r = a.rstates[col];
+  const ST r = row_state<ST>(a, col);
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   for (int g = 0; g < a.n_diag; ++g) {
-    const int off = int(meta[4 * g + 3]) + pattern_of(r, meta[4 * g + 1]);
+    const int off = g_off<ST>(a, meta, g) + g_pattern<ST>(a, meta, r, g);
     out.dre += famp[2 * off];
     out.dim += famp[2 * off + 1];
   }
   const int G = a.n_diag + a.n_off;
   for (int g = a.n_diag; g < G; ++g) {
-    const int pat = pattern_of(r, meta[4 * g + 1]);
-    if (!((meta[4 * g + 2] >> pat) & 1u)) continue;
+    const int pat = g_pattern<ST>(a, meta, r, g);
+    if (!((g_nz<ST>(a, meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of(a, sym, shift_needed, r ^ meta[4 * g], spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), spl, idx, orbit, ci, sign)) continue;
     double re, im;
-    column_value(a, famp, int(meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    column_value(a, famp, g_off<ST>(a, meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
     if (uint32_t(idx) == col) {
       out.dre += re;
       out.dim += im;
@@ -842,28 +978,29 @@ __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const doub
 }
 
 // merged entry H[row, col] (all contributions of column col landing on row)
+template <typename ST>
 __device__ void entry_at(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
                          const uint32_t* spl, uint32_t col, uint32_t row, double& vre, double& vim) {
-  const uint32_t r = a.rstates[col];
+  const ST r = row_state<ST>(a, col);
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   vre = 0;
   vim = 0;
   if (row == col) {
     for (int g = 0; g < a.n_diag; ++g) {
-      const int off = int(meta[4 * g + 3]) + pattern_of(r, meta[4 * g + 1]);
+      const int off = g_off<ST>(a, meta, g) + g_pattern<ST>(a, meta, r, g);
       vre += famp[2 * off];
       vim += famp[2 * off + 1];
     }
   }
   const int G = a.n_diag + a.n_off;
   for (int g = a.n_diag; g < G; ++g) {
-    const int pat = pattern_of(r, meta[4 * g + 1]);
-    if (!((meta[4 * g + 2] >> pat) & 1u)) continue;
+    const int pat = g_pattern<ST>(a, meta, r, g);
+    if (!((g_nz<ST>(a, meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of(a, sym, shift_needed, r ^ meta[4 * g], spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), spl, idx, orbit, ci, sign)) continue;
     if (uint32_t(idx) != row) continue;
     double re, im;
-    column_value(a, famp, int(meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    column_value(a, famp, g_off<ST>(a, meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
     vre += re;
     vim += im;
   }
@@ -873,6 +1010,7 @@ constexpr int kMaxColEntries = 128;
 constexpr double kEps = 2.220446049250313e-16;
 
 // dense column write / entry counts / diagonal / absolute column sums
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
                                                                 cplx* dense, double* colsum, double* diag_re,
                                                                 double* diag_im, int* overflow, cplx shift,
@@ -880,7 +1018,7 @@ __global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const 
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
   if (C.overflow) {
     *overflow = 1;
     return;
@@ -921,13 +1059,14 @@ __global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const 
 // row, |v| > eps, the diagonal included) sorted by row at ptr[col]: the CSC
 // arrays. The count pass is column_entries_kernel's count mode, which keeps
 // exactly the same entries.
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) column_fill_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
                                                              const uint64_t* __restrict__ ptr, uint32_t* rows,
                                                              double2* vals, int* overflow) {
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
   if (C.overflow) {
     *overflow = 1;
     return;
@@ -1005,6 +1144,25 @@ __global__ void row_count_kernel(const uint32_t* __restrict__ rows, uint64_t nnz
   if (k < nnz) atomicAdd(counts + rows[k], 1u);
 }
 
+// Hermiticity from the stored entries: entry k of H in row-major order is
+// (keys[k], ccols[perm[k]], vals[perm[k]]); CSC entry k read as H^H is
+// (ccols[k], rows[k], conj(vals[k])) in the same order. Equal positions
+// and |v - conj(w)| <= 1e-12 everywhere make H Hermitian (flags[0] stays
+// 0); a position mismatch (an entry the other side lacks) sets flags[1].
+__global__ void herm_compare_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                                    const uint32_t* __restrict__ ccols, const uint32_t* __restrict__ rows,
+                                    const double2* __restrict__ vals, uint64_t nnz, int* flags) {
+  const uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t src = perm[k];
+  if (keys[k] != ccols[k] || ccols[src] != rows[k]) {
+    flags[1] = 1;
+    return;
+  }
+  const double2 v = vals[src], w = vals[k];
+  if (hypot(v.x - w.x, v.y + w.y) > 1.0e-12) flags[0] = 1;
+}
+
 // y = A x from the stored CSR: 8 lanes per row, each summing every 8th
 // entry of the row, then a fixed butterfly over the 8 lanes (deterministic).
 // CMODE 0: real values and vectors; 1: real values, complex vectors; 2:
@@ -1058,12 +1216,13 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(const uint64_t* __restrict_
 // it: every stored entry (j, r, v) needs |v - conj(H[r, j])| <= 1e-12
 // (src/operator/mod.rs:1240-1242, 1313-1398, 2212-2214). Lane r lists column
 // r and re-evaluates column j for each of its entries.
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
                                                              int* flags) {
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
   if (C.overflow) {
     flags[1] = 1;
     return;
@@ -1074,7 +1233,7 @@ __global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const dou
   for (int k = 0; k < C.cnt && !bad; ++k) {
     if (hypot(C.vre[k], C.vim[k]) <= kEps) continue;  // not stored
     double wr, wi;
-    entry_at(a, a.meta, famp, sym, shift_needed, a.spl, uint32_t(C.rows[k]), col, wr, wi);
+    entry_at<ST>(a, a.meta, famp, sym, shift_needed, a.spl, uint32_t(C.rows[k]), col, wr, wi);
     if (hypot(wr, wi) <= kEps) wr = wi = 0.0;
     if (hypot(C.vre[k] - wr, C.vim[k] + wi) > tol) bad = true;
   }
@@ -1083,16 +1242,17 @@ __global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const dou
 
 // y = H x by columns with atomic accumulation, for sector matrices that are
 // not Hermitian (assembly checks disabled); y must be zeroed first.
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double* famp, int sym, int shift_needed,
                                                          const double2* __restrict__ x, double* y) {
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
-  const uint32_t r = a.rstates[col];
+  const ST r = row_state<ST>(a, col);
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   const double2 xc = x[col];
   double dre = 0, dim_ = 0;
   for (int g = 0; g < a.n_diag; ++g) {
-    const int off = int(a.meta[4 * g + 3]) + pattern_of(r, a.meta[4 * g + 1]);
+    const int off = g_off<ST>(a, a.meta, g) + g_pattern<ST>(a, a.meta, r, g);
     dre += famp[2 * off];
     dim_ += famp[2 * off + 1];
   }
@@ -1100,12 +1260,12 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double*
   atomicAdd(y + 2 * col + 1, dre * xc.y + dim_ * xc.x);
   const int G = a.n_diag + a.n_off;
   for (int g = a.n_diag; g < G; ++g) {
-    const int pat = pattern_of(r, a.meta[4 * g + 1]);
-    if (!((a.meta[4 * g + 2] >> pat) & 1u)) continue;
+    const int pat = g_pattern<ST>(a, a.meta, r, g);
+    if (!((g_nz<ST>(a, a.meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), a.spl, idx, orbit, ci, sign)) continue;
     double re, im;
-    column_value(a, famp, int(a.meta[4 * g + 3]) + pat, sym, ns, orbit, ci, sign, re, im);
+    column_value(a, famp, g_off<ST>(a, a.meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
     atomicAdd(y + 2 * idx, re * xc.x - im * xc.y);
     atomicAdd(y + 2 * idx + 1, re * xc.y + im * xc.x);
   }
@@ -1144,7 +1304,7 @@ __global__ void __launch_bounds__(kBlock) image_kernel(HvArgs a, int sym, int sh
     uint32_t e = 0xFFFFFFFFu;
     if (a.up >= 0 || ((popmask >> __popc(s)) & 1ull)) {
       int idx, orbit, ci, sign;
-      if (target_of(a, sym, shift_needed, s, a.spl, idx, orbit, ci, sign))
+      if (target_of<uint32_t>(a, sym, shift_needed, s, a.spl, idx, orbit, ci, sign))
         e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) |
             (uint32_t(sign < 0) << (a.img_bits + 4)) | (uint32_t(ci) << (a.img_bits + 5));
     }
@@ -1172,21 +1332,29 @@ __global__ void __launch_bounds__(kBlock) image_kernel(HvArgs a, int sym, int sh
 constexpr int kSlice = 64;
 
 // the active off-diagonal groups of representative r in group order
-struct ActiveGroups {
+// (ST: the state width; wide states read the 64-bit group words meta64)
+template <typename ST>
+struct ActiveGroupsT {
   const uint32_t* meta;
+  const uint64_t* meta64;
   int n_diag, n_off;
-  uint32_t r;
+  ST r;
   int w0;
   uint32_t act;
-  __device__ ActiveGroups(const uint32_t* m, int nd, int no, uint32_t rr)
-      : meta(m), n_diag(nd), n_off(no), r(rr), w0(-32), act(0) {}
+  __device__ ActiveGroupsT(const uint32_t* m, int nd, int no, ST rr, const uint64_t* m64 = nullptr)
+      : meta(m), meta64(m64), n_diag(nd), n_off(no), r(rr), w0(-32), act(0) {}
   __device__ uint32_t window(int w) const {
     uint32_t out = 0;
     const int gend = min(32, n_off - w);
     for (int gi = 0; gi < gend; ++gi) {
       const int g = n_diag + w + gi;
-      const int pat = pattern_of(r, meta[4 * g + 1]);
-      out |= ((meta[4 * g + 2] >> pat) & 1u) << gi;
+      if constexpr (sizeof(ST) == 4) {
+        const int pat = pattern_of(r, meta[4 * g + 1]);
+        out |= ((meta[4 * g + 2] >> pat) & 1u) << gi;
+      } else {
+        const int pat = pattern_of64(r, meta64[4 * g + 1]);
+        out |= uint32_t((meta64[4 * g + 2] >> pat) & 1u) << gi;
+      }
     }
     return out;
   }
@@ -1202,22 +1370,25 @@ struct ActiveGroups {
     return n_diag + w0 + gi;
   }
 };
+using ActiveGroups = ActiveGroupsT<uint32_t>;
 
 // width_s = the largest active-group count among the slice's rows
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) stream_count_kernel(HvArgs a, uint32_t n_slices, uint32_t* width) {
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   int cnt = 0;
   if (row < a.n) {
-    ActiveGroups ag(a.meta, a.n_diag, a.n_off, a.rstates[row]);
+    ActiveGroupsT<ST> ag(a.meta, a.n_diag, a.n_off, row_state<ST>(a, row), a.meta64);
     for (int w = 0; w < a.n_off; w += 32) cnt += __popc(ag.window(w));
   }
   for (int o = 32; o > 0; o >>= 1) cnt = max(cnt, __shfl_xor(cnt, o, 64));
   if ((threadIdx.x & 63) == 0 && row / kSlice < n_slices) width[row / kSlice] = uint32_t(cnt);
 }
 
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, int shift_needed, uint32_t n_slices,
                                                              const uint8_t* orbit_code, const uint32_t* slice_ptr,
-                                                             uint32_t* out) {
+                                                             uint32_t* out, uint8_t* aux_out) {
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t slice = row / kSlice, lane = row % kSlice;
   if (slice >= n_slices) return;
@@ -1225,22 +1396,27 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
   const uint32_t width = (slice_ptr[slice + 1] - base) / kSlice;
   uint32_t k = 0;
   if (row < a.n) {
-    const uint32_t r = a.rstates[row];
-    ActiveGroups ag(a.meta, a.n_diag, a.n_off, r);
+    const ST r = row_state<ST>(a, row);
+    ActiveGroupsT<ST> ag(a.meta, a.n_diag, a.n_off, r, a.meta64);
     for (int g = ag.next(); g >= 0; g = ag.next()) {
       uint32_t e = 0xFFFFFFFFu;
       int idx, orbit, ci, sign;
-      if (target_of(a, sym, shift_needed, r ^ a.meta[4 * g], a.spl, idx, orbit, ci, sign)) {
+      if (target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), a.spl, idx, orbit, ci, sign)) {
         if (a.compact) {
-          const uint32_t off = a.meta[4 * g + 3] + uint32_t(pattern_of(r, a.meta[4 * g + 1])) - uint32_t(a.c_off_base);
-          int sh = a.c_b;
-          e = uint32_t(idx) | (off << sh);
-          sh += a.c_ob;
-          if (a.c_cb) e |= uint32_t(orbit_code[orbit]) << sh;
-          sh += a.c_cb;
-          if (a.c_sb) e |= uint32_t(sign < 0) << sh;
-          sh += a.c_sb;
-          if (shift_needed) e |= uint32_t(ci) << sh;
+          const uint32_t off = uint32_t(g_off<ST>(a, a.meta, g)) + uint32_t(g_pattern<ST>(a, a.meta, r, g)) -
+                               uint32_t(a.c_off_base);
+          // the fields above the entry, as the split layout's byte
+          uint32_t hi = 0;
+          int hs = 0;
+          if (a.c_cb) hi |= uint32_t(orbit_code[orbit]) << hs;
+          hs += a.c_cb;
+          if (a.c_sb) hi |= uint32_t(sign < 0) << hs;
+          hs += a.c_sb;
+          if (shift_needed) hi |= uint32_t(ci) << hs;
+          const int sc = a.c_b + a.c_ob;
+          e = uint32_t(idx) | (off << a.c_b);
+          if (aux_out) aux_out[base + kSlice * k + lane] = uint8_t(hi);
+          else if (sc < 32) e |= hi << sc;
         } else {
           e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) | (uint32_t(sign < 0) << (a.img_bits + 4)) |
               (uint32_t(ci) << (a.img_bits + 5));
@@ -1250,7 +1426,10 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
       ++k;
     }
   }
-  for (; k < width; ++k) out[base + kSlice * k + lane] = 0xFFFFFFFFu;
+  for (; k < width; ++k) {
+    out[base + kSlice * k + lane] = 0xFFFFFFFFu;
+    if (aux_out) aux_out[base + kSlice * k + lane] = 0;
+  }
 }
 
 // H·v from the compiled stream: one lane per row; the words of B ordinals
@@ -1338,14 +1517,19 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
   if (a.compact) {
     // every entry names its amplitude, orbit code, sign and character: no
     // group walk; loads are unconditional (clamped ordinal, then masked)
-    const int so = a.c_b, sc = so + a.c_ob, ss = sc + a.c_cb, sh = ss + a.c_sb;
+    // the fields above the amplitude entry: the word's top bits, or the
+    // split layout's byte
+    const int so = a.c_b, sc = so + a.c_ob;
     const uint32_t imask = (1u << a.c_b) - 1u;
+    const uint8_t* auxw = a.aux ? a.aux + base + lane : nullptr;
     for (uint32_t k0 = 0; k0 < width; k0 += B) {
-      uint32_t ent[B];
+      uint32_t ent[B], hi[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
-        const uint32_t e = words[size_t(kSlice) * min(k0 + b, width - 1)];
+        const size_t pos = size_t(kSlice) * min(k0 + b, width - 1);
+        const uint32_t e = words[pos];
         ent[b] = k0 + b < width ? e : 0xFFFFFFFFu;
+        hi[b] = auxw ? uint32_t(auxw[pos]) : (sc < 32 ? e >> sc : 0u);
       }
       S xr[B][NV], xi[B][NV];
 #pragma unroll
@@ -1367,8 +1551,8 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
         const uint32_t e = ent[b];
         if (e == 0xFFFFFFFFu) continue;
         const int off = a.c_off_base + int(field(e, so, a.c_ob));
-        S scale = field(e, ss, a.c_sb) ? S(-1) : S(1);
-        if (SYM) scale *= sq_r * S(s_crsq[field(e, sc, a.c_cb)]);
+        S scale = field(hi[b], a.c_cb, a.c_sb) ? S(-1) : S(1);
+        if (SYM) scale *= sq_r * S(s_crsq[field(hi[b], 0, a.c_cb)]);
         Cx<S> c;
         if (CMODE == 2) {
           c.re = S(s_amp[2 * off]) * scale;
@@ -1378,7 +1562,7 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
           c.im = 0;
         }
         if ((SYM == 1 || SYM == 3) && SHIFT) {
-          const int ci = int(e >> sh);
+          const int ci = int(hi[b] >> (a.c_cb + a.c_sb));
           const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
           if (CMODE == 2) {
             const S re = c.re * pr - c.im * pi;
@@ -1707,6 +1891,7 @@ __device__ __forceinline__ int pattern_fixed(uint32_t r, uint32_t packed) {
   return pat & ((1 << (packed & 7u)) - 1);
 }
 
+template <typename ST>
 __global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp, double* out) {
   // The amplitude table and the diagonal groups' (sites, offset) words in
   // LDS. Every row reads n_diag entries at row-dependent offsets; the group
@@ -1716,11 +1901,31 @@ __global__ void __launch_bounds__(kBlock) diag_kernel(HvArgs a, int complex_amp,
   __shared__ uint2 s_grp[kMaxGroups];
   const int namp = (complex_amp ? 2 : 1) * a.n_amp;
   for (int i = threadIdx.x; i < namp; i += kBlock) s_amp[i] = a.amp[i];
-  for (int g = threadIdx.x; g < a.n_diag; g += kBlock) s_grp[g] = make_uint2(a.meta[4 * g + 1], a.meta[4 * g + 3]);
+  if constexpr (sizeof(ST) == 4)
+    for (int g = threadIdx.x; g < a.n_diag; g += kBlock) s_grp[g] = make_uint2(a.meta[4 * g + 1], a.meta[4 * g + 3]);
   __syncthreads();
   const int G = a.n_diag;
+  if constexpr (sizeof(ST) == 8) {
+    // wide states: the 64-bit group words, one group at a time
+    for (size_t row = size_t(blockIdx.x) * kBlock + threadIdx.x; row < a.n; row += size_t(gridDim.x) * kBlock) {
+      const uint64_t r = a.rstates64[row];
+      double re = 0, im = 0;
+      for (int g = 0; g < G; ++g) {
+        const int o = int(a.meta64[4 * g + 3]) + pattern_of64(r, a.meta64[4 * g + 1]);
+        if (complex_amp) {
+          re += s_amp[2 * o];
+          im += s_amp[2 * o + 1];
+        } else {
+          re += s_amp[o];
+        }
+      }
+      out[row] = re;
+      out[a.n + row] = im;
+    }
+    return;
+  }
   for (size_t row = size_t(blockIdx.x) * kBlock + threadIdx.x; row < a.n; row += size_t(gridDim.x) * kBlock) {
-    const uint32_t r = a.rstates[row];
+    const uint32_t r = uint32_t(row_state<ST>(a, uint32_t(row)));
     double re0 = 0, re1 = 0, im0 = 0, im1 = 0;
     int g = 0;
     for (; g + 2 <= G; g += 2) {
@@ -1781,6 +1986,14 @@ __global__ void __launch_bounds__(kBlock) diag_combine_kernel(const double* __re
   }
 }
 
+// Launch the 32-bit or 64-bit state instantiation of a build kernel (wide
+// sectors hold 64-bit states).
+#define QMBED_WIDTH_LAUNCH(basis, K, ...)                                \
+  do {                                                                  \
+    if ((basis).wide()) hipLaunchKernelGGL(K<uint64_t>, __VA_ARGS__);   \
+    else hipLaunchKernelGGL(K<uint32_t>, __VA_ARGS__);                  \
+  } while (0)
+
 // The symmetry kind of the orbit walks (target_of): 0 none, 1 translation,
 // 2 reflection, 3 both, 4 a general group
 int sym_code(const SpinBasis1D& b) {
@@ -1811,6 +2024,7 @@ bool phased(const SpinBasis1D& b) {
 // kernels.
 struct Operator::DeviceTables {
   DeviceArray<uint32_t> meta;
+  DeviceArray<uint64_t> meta64;  // wide sectors: [flip, sites, nonzero mask, offset] per group
   DeviceArray<double> amp;
   DeviceArray<double> famp;
   DeviceArray<double> chars;
@@ -1831,8 +2045,10 @@ struct Operator::DeviceTables {
   // the stream is kept for the interleaved multi-column action even when
   // the single-vector autotuner prefers another mode
   bool use_stream_interleaved = false;
-  // compact stream layout (field widths; see HvArgs)
+  // compact stream layout (field widths; see HvArgs), and the byte per word
+  // of the split layout
   bool compact = false;
+  DeviceArray<uint8_t> aux;
   int c_b = 0, c_ob = 0, c_cb = 0, c_sb = 0, c_off_base = 0;
   int n_codes = 1;
   // per-row diagonal of the current coefficients (re[n], im[n])
@@ -1873,7 +2089,11 @@ struct Operator::DeviceTables {
     a.nchar = g.n_group ? g.n_chars : (g.kmom >= 0 ? g.L : 1);
     a.n = uint32_t(b.len());
     a.L = g.L;
-    a.mask = g.mask;
+    a.mask = uint32_t(g.mask);
+    a.mask64 = g.mask;
+    a.states64 = b.states64();
+    a.rstates64 = b.row_states64();
+    a.meta64 = meta64.data();
     a.parity = g.parity;
     a.n_div = g.n_div;
     for (int i = 0; i < 16; ++i) a.divisors[i] = g.divisors[i];
@@ -1907,6 +2127,7 @@ struct Operator::DeviceTables {
     a.stream = use_stream ? stream.data() : nullptr;
     a.slice_ptr = slice_ptr.data();
     a.compact = compact ? 1 : 0;
+    a.aux = aux.size() ? aux.data() : nullptr;
     a.c_b = c_b;
     a.c_ob = c_ob;
     a.c_cb = c_cb;
@@ -1917,13 +2138,75 @@ struct Operator::DeviceTables {
   }
 };
 
+// Hermiticity from the stored entries (two column passes, a radix sort and
+// an entry-wise comparison); returns -1 when it cannot decide (an entry
+// whose transposed partner is not stored, or too little device memory),
+// else 1 (Hermitian) or 0.
+static int stored_hermitian(const Operator& op, hipStream_t s) {
+  const SpinBasis1D& b = op.basis();
+  HvArgs a = op.tables_->args(b);
+  const size_t n = a.n;
+  DeviceArray<uint32_t> cnt(n);
+  DeviceArray<int> flags(2);
+  QMBED_HIP(hipMemsetAsync(flags.data(), 0, 2 * sizeof(int), s));
+  const dim3 grid(unsigned((n + kBlock - 1) / kBlock));
+  QMBED_WIDTH_LAUNCH(b, column_entries_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
+                     phased(b) ? 1 : 0, nullptr, nullptr, nullptr, nullptr, flags.data() + 1, cplx(0, 0), cnt.data());
+  QMBED_HIP(hipGetLastError());
+  std::vector<uint32_t> h(n);
+  int of = 0;
+  QMBED_HIP(hipMemcpyAsync(h.data(), cnt.data(), n * 4, hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipMemcpyAsync(&of, flags.data() + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  if (of) return -1;
+  std::vector<uint64_t> ptr(n + 1, 0);
+  for (size_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + h[i];
+  const uint64_t nnz = ptr[n];
+  if (nnz == 0) return 1;
+  if (nnz >= (uint64_t(1) << 31)) return -1;
+  size_t free_b = 0, total_b = 0;
+  QMBED_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (nnz * 48 > free_b / 2) return -1;
+  DeviceArray<uint64_t> cptr(n + 1);
+  QMBED_HIP(hipMemcpyAsync(cptr.data(), ptr.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+  DeviceArray<uint32_t> rows(nnz), ccols(nnz), keys(nnz), perm_in(nnz), perm(nnz);
+  DeviceArray<double2> vals(nnz);
+  QMBED_WIDTH_LAUNCH(b, column_fill_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
+                     phased(b) ? 1 : 0, cptr.data(), rows.data(), vals.data(), flags.data() + 1);
+  hipLaunchKernelGGL(csc_columns_kernel, grid, dim3(kBlock), 0, s, cptr.data(), uint32_t(n), ccols.data());
+  std::vector<uint32_t> iota(nnz);
+  std::iota(iota.begin(), iota.end(), 0u);
+  QMBED_HIP(hipMemcpyAsync(perm_in.data(), iota.data(), nnz * 4, hipMemcpyHostToDevice, s));
+  int bits = 1;
+  while ((size_t(1) << bits) < n) ++bits;
+  size_t tmp_bytes = 0;
+  QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, rows.data(), keys.data(), perm_in.data(),
+                                               perm.data(), int(nnz), 0, bits, s));
+  DeviceArray<uint8_t> tmp(std::max<size_t>(tmp_bytes, 1));
+  QMBED_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.data(), tmp_bytes, rows.data(), keys.data(), perm_in.data(),
+                                               perm.data(), int(nnz), 0, bits, s));
+  QMBED_HIP(hipMemsetAsync(flags.data(), 0, 2 * sizeof(int), s));
+  hipLaunchKernelGGL(herm_compare_kernel, dim3(unsigned((nnz + 255) / 256)), dim3(256), 0, s, keys.data(),
+                     perm.data(), ccols.data(), rows.data(), vals.data(), nnz, flags.data());
+  QMBED_HIP(hipGetLastError());
+  int hf[2] = {0, 0};
+  QMBED_HIP(hipMemcpyAsync(hf, flags.data(), 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  QMBED_HIP(hipStreamSynchronize(s));
+  if (hf[1]) return -1;
+  return hf[0] ? 0 : 1;
+}
+
 static bool sector_matrix_hermitian(const Operator& op, hipStream_t s) {
   const SpinBasis1D& b = op.basis();
   HvArgs a = op.tables_->args(b);
   if (a.n == 0) return true;
+  // the entry-wise check from the stored entries when it can decide; the
+  // per-entry re-evaluation below otherwise (the reference's rule either way)
+  const int fast = stored_hermitian(op, s);
+  if (fast >= 0) return fast == 1;
   DeviceArray<int> flags(2);
   QMBED_HIP(hipMemsetAsync(flags.data(), 0, 2 * sizeof(int), s));
-  hipLaunchKernelGGL(hermiticity_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
+  QMBED_WIDTH_LAUNCH(b, hermiticity_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
                      op.tables_->famp.data(), sym_code(b), phased(b) ? 1 : 0, flags.data());
   QMBED_HIP(hipGetLastError());
   int h[2] = {0, 0};
@@ -1989,7 +2272,7 @@ static void upload_amplitudes(Operator& op, const std::vector<FlipGroup>& diag, 
     if (t.diag.size() != 2 * n) t.diag.resize(2 * n);
     HvArgs a = t.args(b);
     const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
-    hipLaunchKernelGGL(diag_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, a, t.complex_coef ? 1 : 0,
+    QMBED_WIDTH_LAUNCH(b, diag_kernel, dim3(unsigned(blocks)), dim3(kBlock), 0, s, a, t.complex_coef ? 1 : 0,
                        t.diag.data());
     QMBED_HIP(hipGetLastError());
   }
@@ -2036,11 +2319,16 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
     throw unsupported_backend("the operator has more distinct local actions than the H·v kernel tables hold");
   t->complex_coef = !op->real_;
   std::vector<uint32_t> meta;
+  std::vector<uint64_t> meta64;
   int n_amp = 0;
   for (auto* v : {&op->diag_, &op->off_}) {
     for (auto& g : *v) {
       uint32_t packed = uint32_t(g.nsites);
-      for (int i = 0; i < g.nsites; ++i) packed |= uint32_t(g.sites[i]) << (3 + 5 * i);
+      uint64_t packed64 = uint64_t(g.nsites);
+      for (int i = 0; i < g.nsites; ++i) {
+        packed |= uint32_t(g.sites[i] & 31) << (3 + 5 * i);
+        packed64 |= uint64_t(g.sites[i]) << (3 + 6 * i);
+      }
       t->max_sites = std::max(t->max_sites, int(g.nsites));
       // nonzero patterns: of the table, or of any part of a family (the
       // structure must not depend on the coefficients)
@@ -2050,10 +2338,14 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
         for (auto& part : g.parts) on = on || part[p] != cplx(0, 0);
         if (on) nz |= 1u << p;
       }
-      meta.push_back(g.flip);
+      meta.push_back(uint32_t(g.flip));
       meta.push_back(packed);
       meta.push_back(nz);
       meta.push_back(uint32_t(n_amp));
+      meta64.push_back(g.flip);
+      meta64.push_back(packed64);
+      meta64.push_back(nz);
+      meta64.push_back(uint64_t(n_amp));
       n_amp += int(g.amp.size());
     }
   }
@@ -2086,6 +2378,11 @@ std::shared_ptr<Operator> OperatorBuilder::build(MatrixFormat format) const {
   t->meta.resize(std::max<size_t>(meta.size(), 4));
   t->chars.resize(chars.size());
   if (!meta.empty()) QMBED_HIP(hipMemcpy(t->meta.data(), meta.data(), meta.size() * 4, hipMemcpyHostToDevice));
+  if (basis_->wide()) {
+    t->meta64.resize(std::max<size_t>(meta64.size(), 4));
+    if (!meta64.empty())
+      QMBED_HIP(hipMemcpy(t->meta64.data(), meta64.data(), meta64.size() * 8, hipMemcpyHostToDevice));
+  }
   QMBED_HIP(hipMemcpy(t->chars.data(), chars.data(), chars.size() * 8, hipMemcpyHostToDevice));
   op->tables_ = t;
   if (!family) {
@@ -2280,8 +2577,8 @@ void launch_hv_stream(const HvArgs& a, int sym, bool shift, const SX* x, S* y, h
 template <typename S, int CMODE, int NV>
 void launch_hv(const HvArgs& a, int sym, bool shift, const S* x, S* y, hipStream_t s) {
   if (a.n == 0) return;
-  if (!a.stream && (a.ngroup || a.rev))
-    throw internal_state("general and reversed sectors are applied through the compiled stream only");
+  if (!a.stream && (a.ngroup || a.rev || a.states64))
+    throw internal_state("general, reversed and wide sectors are applied through the compiled stream only");
   if (a.stream) launch_hv_stream<S, CMODE, NV>(a, sym, shift, x, y, s);
   else if (a.img) launch_hv_mode<S, CMODE, NV, true>(a, sym, shift, x, y, s);
   else launch_hv_mode<S, CMODE, NV, false>(a, sym, shift, x, y, s);
@@ -2329,7 +2626,6 @@ static bool prepare_codes(Operator& op, std::vector<uint8_t>& code) {
   }
   int bits = 1;
   while ((size_t(1) << bits) < b.len()) ++bits;
-  if (bits + 10 > 32) return false;
   t.code_rsq.resize(16);
   QMBED_HIP(hipMemcpy(t.code_rsq.data(), crsq.data(), 16 * 8, hipMemcpyHostToDevice));
   t.img_bits = bits;
@@ -2348,6 +2644,7 @@ static int bits_for(size_t values) {
 static bool build_stream(Operator& op, hipStream_t s) {
   const SpinBasis1D& b = op.basis();
   auto& t = *op.tables_;
+  bool split = false;
   std::vector<uint8_t> code;
   if (!prepare_codes(op, code)) return false;
   const size_t n = b.len();
@@ -2368,15 +2665,22 @@ static bool build_stream(Operator& op, hipStream_t s) {
     const char* fmt = std::getenv("QMBED_HV_STREAM_FORMAT");
     const bool force_groups = fmt && std::string(fmt) == "groups";
     t.compact = !force_groups && t.c_b + t.c_ob + t.c_cb + t.c_sb + chb <= 32;
+    // the split layout when the fields above the entry fit one byte
+    split = !force_groups && !t.compact && t.c_b + t.c_ob <= 32 && t.c_cb + t.c_sb + chb <= 8;
+    if (split) t.compact = true;
     // a general group's words are compact only (its character indices may
-    // exceed the group-walk word's 5 bits)
+    // exceed the group-walk word's 5 bits), and so are a wide sector's (its
+    // products read no states) when its per-row diagonal exists
     if (b.general() && !t.compact) return false;
+    if (b.wide() && !(t.compact && t.diag.size() >= 2 * n)) return false;
+    // the group-walk word: index, 4 code bits, sign, 5 χ-index bits
+    if (!t.compact && t.img_bits + 10 > 32) return false;
   }
   const uint32_t n_slices = uint32_t((n + kSlice - 1) / kSlice);
   HvArgs a = t.args(b);
   DeviceArray<uint32_t> width(n_slices);
   const unsigned grid = unsigned((size_t(n_slices) * kSlice + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(stream_count_kernel, dim3(grid), dim3(kBlock), 0, s, a, n_slices, width.data());
+  QMBED_WIDTH_LAUNCH(b, stream_count_kernel, dim3(grid), dim3(kBlock), 0, s, a, n_slices, width.data());
   QMBED_HIP(hipGetLastError());
   std::vector<uint32_t> w(n_slices);
   QMBED_HIP(hipMemcpyAsync(w.data(), width.data(), size_t(n_slices) * 4, hipMemcpyDeviceToHost, s));
@@ -2391,15 +2695,17 @@ static bool build_stream(Operator& op, hipStream_t s) {
   ptr[n_slices] = uint32_t(total);
   size_t free_b = 0, total_b = 0;
   QMBED_HIP(hipMemGetInfo(&free_b, &total_b));
-  if (total * 4 > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
+  if (total * (split ? 5 : 4) > std::min<uint64_t>(uint64_t(16) << 30, free_b / 4)) return false;
   t.slice_ptr.resize(ptr.size());
   QMBED_HIP(hipMemcpy(t.slice_ptr.data(), ptr.data(), ptr.size() * 4, hipMemcpyHostToDevice));
   t.stream.resize(std::max<uint64_t>(total, 1));
+  if (split) t.aux.resize(std::max<uint64_t>(total, 1));
+  else t.aux.release();
   DeviceArray<uint8_t> dcode(256);
   QMBED_HIP(hipMemcpy(dcode.data(), code.data(), 256, hipMemcpyHostToDevice));
   a = t.args(b);
-  hipLaunchKernelGGL(stream_fill_kernel, dim3(grid), dim3(kBlock), 0, s, a, sym_code(b), phased(b) ? 1 : 0, n_slices,
-                     dcode.data(), t.slice_ptr.data(), t.stream.data());
+  QMBED_WIDTH_LAUNCH(b, stream_fill_kernel, dim3(grid), dim3(kBlock), 0, s, a, sym_code(b), phased(b) ? 1 : 0, n_slices,
+                     dcode.data(), t.slice_ptr.data(), t.stream.data(), split ? t.aux.data() : nullptr);
   QMBED_HIP(hipGetLastError());
   QMBED_HIP(hipStreamSynchronize(s));
   return true;
@@ -2423,8 +2729,8 @@ static bool build_image_table(Operator& op, hipStream_t s) {
   for (auto* v : {&op.diagonal_groups(), &op.offdiagonal_groups()})
     for (auto& grp : *v) {
       uint32_t h = 0;
-      if (grp.flip && __builtin_popcount(grp.flip) == 2) {
-        const int p = __builtin_ctz(grp.flip), q = 31 - __builtin_clz(grp.flip);
+      if (grp.flip && __builtin_popcountll(grp.flip) == 2) {
+        const int p = __builtin_ctzll(grp.flip), q = 63 - __builtin_clzll(grp.flip);
         // a two-site hop: amplitudes only where exactly one of p, q is up
         bool hopping = true;
         for (size_t pat = 0; pat < grp.amp.size(); ++pat) {
@@ -2442,9 +2748,10 @@ static bool build_image_table(Operator& op, hipStream_t s) {
     }
   // the group walk of a general sector and the row reversal are served by
   // the compiled stream only
-  if (b.general() || b.reversed()) return false;
+  if (b.general() || b.reversed() || b.wide()) return false;
   std::vector<uint8_t> code;
   if (!prepare_codes(op, code)) return false;
+  if (t.img_bits + 10 > 32) return false;  // index, code, sign and χ-index in one word
   const bool fixed_up = spec.up.has_value() && spec.up_sectors.empty();
   uint64_t parents = 0, popmask = ~0ull;
   if (fixed_up) {
@@ -2498,16 +2805,17 @@ static void tune_lanes(Operator& op) {
   // candidates
   const char* mode_env = std::getenv("QMBED_HV_MODE");
   const std::string mode = mode_env ? mode_env : "auto";
-  // general groups and reversed rows run on the compiled stream only
-  const bool stream_only = b.general() || b.reversed();
+  // general groups, reversed rows and 64-bit states run on the compiled
+  // stream only
+  const bool stream_only = b.general() || b.reversed() || b.wide();
   const bool want_walk = (mode == "auto" || mode == "walk") && !stream_only;
   const bool have_img = (mode == "auto" || mode == "image") && build_image_table(op, s);
   const bool have_stream = (mode == "auto" || mode == "stream" || stream_only) && build_stream(op, s);
   if (stream_only && !have_stream) {
     (void)hipStreamDestroy(s);
-    throw unsupported_backend(b.general()
-                                  ? "the general symmetry sector does not fit the compiled index stream"
-                                  : "the reversed sector does not fit the compiled index stream");
+    throw unsupported_backend(b.wide()      ? "the wide sector does not fit the compiled index stream"
+                              : b.general() ? "the general symmetry sector does not fit the compiled index stream"
+                                            : "the reversed sector does not fit the compiled index stream");
   }
   DeviceArray<double> x(2 * n), y(2 * n);
   QMBED_HIP(hipMemsetAsync(x.data(), 0, 2 * n * 8, s));
@@ -2565,6 +2873,7 @@ static void tune_lanes(Operator& op) {
   if (best_mode != 2) {
     op.tables_->stream.release();
     op.tables_->slice_ptr.release();
+    op.tables_->aux.release();
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -2583,7 +2892,7 @@ static void scatter_apply(const Operator& op, const cplx* x, cplx* y, hipStream_
   HvArgs a = op.tables_->args(b);
   if (a.n == 0) return;
   QMBED_HIP(hipMemsetAsync(y, 0, size_t(a.n) * sizeof(cplx), s));
-  hipLaunchKernelGGL(scatter_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
+  QMBED_WIDTH_LAUNCH(b, scatter_kernel, dim3((a.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a,
                      op.tables_->famp.data(), sym_code(b), phased(b) ? 1 : 0,
                      reinterpret_cast<const double2*>(x), reinterpret_cast<double*>(y));
   QMBED_HIP(hipGetLastError());
@@ -2736,7 +3045,7 @@ bool Operator::scan_batch_supported(hipStream_t s) const {
   if (t.diag_parts != int(components_.size()) + 1) return false;
   if (t.stream.size() == 0 && !build_stream(const_cast<Operator&>(*this), s)) return false;
   // LDS of an 8-lane launch: the lane tables and the two small tables
-  return t.compact && sizeof(double) * (8 * size_t(t.n_amp) + 81) <= 48 * 1024;
+  return t.compact && t.aux.size() == 0 && sizeof(double) * (8 * size_t(t.n_amp) + 81) <= 48 * 1024;
 }
 
 int Operator::scan_batch_max_width() const {
@@ -2980,7 +3289,7 @@ static void run_columns(const Operator& op, cplx* dense, double* colsum, double*
   DeviceArray<int> overflow(1);
   QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
   dim3 grid((a.n + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL(column_entries_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
+  QMBED_WIDTH_LAUNCH(b, column_entries_kernel, grid, dim3(kBlock), 0, s, a, op.tables_->famp.data(), sym_code(b),
                      phased(b) ? 1 : 0, dense, colsum, dre, dim_, overflow.data(), shift, count);
   QMBED_HIP(hipGetLastError());
   int of = 0;
@@ -3024,7 +3333,7 @@ void Operator::refresh_stored(hipStream_t s) {
   DeviceArray<int> overflow(1);
   QMBED_HIP(hipMemsetAsync(overflow.data(), 0, sizeof(int), s));
   const dim3 grid(unsigned((n + kBlock - 1) / kBlock));
-  hipLaunchKernelGGL(column_entries_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b),
+  QMBED_WIDTH_LAUNCH(b, column_entries_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b),
                      phased(b) ? 1 : 0, nullptr, nullptr, nullptr, nullptr, overflow.data(), cplx(0, 0), cnt.data());
   QMBED_HIP(hipGetLastError());
   std::vector<uint32_t> h(n);
@@ -3041,7 +3350,7 @@ void Operator::refresh_stored(hipStream_t s) {
   QMBED_HIP(hipMemcpyAsync(cptr.data(), ptr.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
   DeviceArray<uint32_t> rows(std::max<uint64_t>(nnz, 1));
   DeviceArray<double2> vals(std::max<uint64_t>(nnz, 1));
-  hipLaunchKernelGGL(column_fill_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b), phased(b) ? 1 : 0,
+  QMBED_WIDTH_LAUNCH(b, column_fill_kernel, grid, dim3(kBlock), 0, s, a, t.famp.data(), sym_code(b), phased(b) ? 1 : 0,
                      cptr.data(), rows.data(), vals.data(), overflow.data());
   QMBED_HIP(hipGetLastError());
   t.csr_complex = !real_;

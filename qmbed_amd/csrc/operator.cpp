@@ -248,9 +248,9 @@ static void build_groups(const std::vector<OperatorSpec>& terms, const std::vect
       uint32_t flip_local = 0;
       for (size_t i = 0; i < adj.size(); ++i)
         if (adj[i] == '+' || adj[i] == '-' || adj[i] == 'x' || adj[i] == 'y') flip_local ^= 1u << slot[i];
-      uint32_t flip = 0;
+      uint64_t flip = 0;
       for (size_t u = 0; u < uniq.size(); ++u)
-        if ((flip_local >> u) & 1) flip |= 1u << uniq[u];
+        if ((flip_local >> u) & 1) flip |= uint64_t(1) << uniq[u];
       // amplitudes per local pattern
       const int np = 1 << uniq.size();
       std::vector<cplx> local_amp(np, cplx(0, 0));

@@ -166,7 +166,8 @@ struct SpinBasisSpec {
 // tabulated on the host).
 struct SectorGeometry {
   int L = 0;
-  uint32_t mask = 0;
+  uint64_t mask = 0;
+  bool wide = false;  // 64-bit states (32 <= L <= 63)
   int kmom = -1;     // canonical momentum, -1 when no translation symmetry
   int parity = 0;    // 0 none, +1/-1
   int n_div = 0;     // proper divisors of L in ascending order (period search)
@@ -201,22 +202,29 @@ class SpinBasis1D {
     return geo_.kmom < 0 || geo_.kmom == 0 || 2 * geo_.kmom == geo_.L;
   }
   const uint32_t* states() const { return states_.data(); }
+  // wide sectors (L >= 32): the representatives as 64-bit words, ascending
+  // and in row order; states() and the search tables are empty then
+  bool wide() const { return geo_.wide; }
+  const uint64_t* states64() const { return states64_.data(); }
+  const uint64_t* row_states64() const { return reversed() ? rstates64_.data() : states64_.data(); }
   const uint8_t* orbits() const { return orbits_.data(); }
   // the representatives and orbit sizes in row order (the ascending tables
   // above, or their reversal when reversed())
   const uint32_t* row_states() const { return reversed() ? rstates_.data() : states_.data(); }
   const uint8_t* row_orbits() const { return reversed() ? rorbits_.data() : orbits_.data(); }
-  // general sectors: per group element the site destinations (32 bytes),
+  // general sectors: per group element the site destinations (64 bytes),
   // the destination-side flip mask and the character-table index of
   // χ(g⁻¹); the distinct character values (index 0 holds 1)
   const uint8_t* group_dest() const { return gdest_.data(); }
-  const uint32_t* group_flip() const { return gflip_.data(); }
+  const uint64_t* group_flip() const { return gflip_.data(); }
   const uint8_t* group_phase() const { return gphase_.data(); }
   const std::vector<cplx>& characters() const { return chars_; }
   std::vector<uint32_t> states_host() const;
   std::vector<uint8_t> orbits_host() const;
   // row order (descending when reversed())
   std::vector<uint32_t> row_states_host() const;
+  // row order as 64-bit words, for sectors of either width
+  std::vector<uint64_t> row_states_host64() const;
   std::vector<uint8_t> row_orbits_host() const;
 
   // Top of the sorted-state search tree kept in LDS by the H·v kernels:
@@ -246,8 +254,9 @@ class SpinBasis1D {
   DeviceArray<uint32_t> pyramid_;
   DeviceArray<uint32_t> rstates_;
   DeviceArray<uint8_t> rorbits_;
+  DeviceArray<uint64_t> states64_, rstates64_;
   DeviceArray<uint8_t> gdest_;
-  DeviceArray<uint32_t> gflip_;
+  DeviceArray<uint64_t> gflip_;
   DeviceArray<uint8_t> gphase_;
   std::vector<cplx> chars_;
   bool real_chars_ = true;
@@ -293,7 +302,7 @@ enum class MatrixFormat { Dense, Csc, Csr, Dia, MatrixFree };
 // bits. The gather-form amplitude for row state r is amp[pattern(r)], where
 // pattern packs the bits of r at `sites` (bit i <- site sites[i]).
 struct FlipGroup {
-  uint32_t flip = 0;
+  uint64_t flip = 0;
   int nsites = 0;
   int sites[5] = {0, 0, 0, 0, 0};
   std::vector<cplx> amp;  // 1 << nsites entries
