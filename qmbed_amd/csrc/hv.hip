@@ -23,6 +23,7 @@
 #include <numeric>
 #include <set>
 #include <string>
+#include <type_traits>
 
 #include <hipcub/hipcub.hpp>
 
@@ -1516,73 +1517,89 @@ __device__ __forceinline__ void stream_rows(const HvArgs& a, const SX* __restric
   const uint32_t* words = a.stream + base + lane;
   if (a.compact) {
     // every entry names its amplitude, orbit code, sign and character: no
-    // group walk; loads are unconditional (clamped ordinal, then masked)
-    // the fields above the amplitude entry: the word's top bits, or the
-    // split layout's byte
-    const int so = a.c_b, sc = so + a.c_ob;
+    // group walk; loads are unconditional (clamped ordinal, then masked).
+    // SPLIT: the fields above the amplitude entry come from the split
+    // layout's byte per word instead of the word's top bits (two loop
+    // bodies, so the 32-bit layout keeps its registers)
+    const int so = a.c_b, sc = so + a.c_ob, ss = sc + a.c_cb, sh = ss + a.c_sb;
     const uint32_t imask = (1u << a.c_b) - 1u;
-    const uint8_t* auxw = a.aux ? a.aux + base + lane : nullptr;
-    for (uint32_t k0 = 0; k0 < width; k0 += B) {
-      uint32_t ent[B], hi[B];
+    auto run = [&](auto split_tag) {
+      constexpr bool SPLIT = decltype(split_tag)::value;
+      const uint8_t* auxw = SPLIT ? a.aux + base + lane : nullptr;
+      for (uint32_t k0 = 0; k0 < width; k0 += B) {
+        uint32_t ent[B];
+        uint32_t hi[SPLIT ? B : 1];
 #pragma unroll
-      for (int b = 0; b < B; ++b) {
-        const size_t pos = size_t(kSlice) * min(k0 + b, width - 1);
-        const uint32_t e = words[pos];
-        ent[b] = k0 + b < width ? e : 0xFFFFFFFFu;
-        hi[b] = auxw ? uint32_t(auxw[pos]) : (sc < 32 ? e >> sc : 0u);
-      }
-      S xr[B][NV], xi[B][NV];
+        for (int b = 0; b < B; ++b) {
+          const size_t pos = size_t(kSlice) * min(k0 + b, width - 1);
+          const uint32_t e = words[pos];
+          ent[b] = k0 + b < width ? e : 0xFFFFFFFFu;
+          if constexpr (SPLIT) hi[b] = uint32_t(auxw[pos]);
+        }
+        S xr[B][NV], xi[B][NV];
 #pragma unroll
-      for (int b = 0; b < B; ++b) {
-        const int idx = ent[b] != 0xFFFFFFFFu ? int(ent[b] & imask) : 0;
+        for (int b = 0; b < B; ++b) {
+          const int idx = ent[b] != 0xFFFFFFFFu ? int(ent[b] & imask) : 0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          if (CMODE == 0) {
-            xr[b][v] = S(x[at(v, idx)]);
-            xi[b][v] = 0;
-          } else {
-            xr[b][v] = S(x[2 * at(v, idx)]);
-            xi[b][v] = S(x[2 * at(v, idx) + 1]);
+          for (int v = 0; v < NV; ++v) {
+            if (CMODE == 0) {
+              xr[b][v] = S(x[at(v, idx)]);
+              xi[b][v] = 0;
+            } else {
+              xr[b][v] = S(x[2 * at(v, idx)]);
+              xi[b][v] = S(x[2 * at(v, idx) + 1]);
+            }
           }
         }
-      }
 #pragma unroll
-      for (int b = 0; b < B; ++b) {
-        const uint32_t e = ent[b];
-        if (e == 0xFFFFFFFFu) continue;
-        const int off = a.c_off_base + int(field(e, so, a.c_ob));
-        S scale = field(hi[b], a.c_cb, a.c_sb) ? S(-1) : S(1);
-        if (SYM) scale *= sq_r * S(s_crsq[field(hi[b], 0, a.c_cb)]);
-        Cx<S> c;
-        if (CMODE == 2) {
-          c.re = S(s_amp[2 * off]) * scale;
-          c.im = S(s_amp[2 * off + 1]) * scale;
-        } else {
-          c.re = S(s_amp[off]) * scale;
-          c.im = 0;
-        }
-        if ((SYM == 1 || SYM == 3) && SHIFT) {
-          const int ci = int(hi[b] >> (a.c_cb + a.c_sb));
-          const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+        for (int b = 0; b < B; ++b) {
+          const uint32_t e = ent[b];
+          if (e == 0xFFFFFFFFu) continue;
+          const int off = a.c_off_base + int(field(e, so, a.c_ob));
+          uint32_t code, neg, ci = 0;
+          if constexpr (SPLIT) {
+            code = field(hi[b], 0, a.c_cb);
+            neg = field(hi[b], a.c_cb, a.c_sb);
+            if ((SYM == 1 || SYM == 3) && SHIFT) ci = hi[b] >> (a.c_cb + a.c_sb);
+          } else {
+            code = field(e, sc, a.c_cb);
+            neg = field(e, ss, a.c_sb);
+            if ((SYM == 1 || SYM == 3) && SHIFT) ci = e >> sh;
+          }
+          S scale = neg ? S(-1) : S(1);
+          if (SYM) scale *= sq_r * S(s_crsq[code]);
+          Cx<S> c;
           if (CMODE == 2) {
-            const S re = c.re * pr - c.im * pi;
-            c.im = c.re * pi + c.im * pr;
-            c.re = re;
+            c.re = S(s_amp[2 * off]) * scale;
+            c.im = S(s_amp[2 * off + 1]) * scale;
           } else {
-            c.re *= pr;  // real characters (k = L/2): pi == 0
+            c.re = S(s_amp[off]) * scale;
+            c.im = 0;
           }
-        }
+          if ((SYM == 1 || SYM == 3) && SHIFT) {
+            const S pr = S(s_chr[2 * ci]), pi = S(s_chr[2 * ci + 1]);
+            if (CMODE == 2) {
+              const S re = c.re * pr - c.im * pi;
+              c.im = c.re * pi + c.im * pr;
+              c.re = re;
+            } else {
+              c.re *= pr;  // real characters (k = L/2): pi == 0
+            }
+          }
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          if (CMODE == 0) {
-            acc[v].re += c.re * xr[b][v];
-          } else {
-            acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
-            acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+          for (int v = 0; v < NV; ++v) {
+            if (CMODE == 0) {
+              acc[v].re += c.re * xr[b][v];
+            } else {
+              acc[v].re += c.re * xr[b][v] - c.im * xi[b][v];
+              acc[v].im += c.re * xi[b][v] + c.im * xr[b][v];
+            }
           }
         }
       }
-    }
+    };
+    if (a.aux) run(std::true_type{});
+    else run(std::false_type{});
     if (!live) return;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
