@@ -45,7 +45,6 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kChunk = 64;       // parents per thread
-constexpr int kMaxSplit = 8192;  // LDS splitter capacity (32 KiB)
 
 struct EnumParams {
   int L;
@@ -288,11 +287,70 @@ __global__ void reverse_kernel(const T* __restrict__ in, uint64_t n, T* __restri
   if (i < n) out[i] = in[n - 1 - i];
 }
 
-__global__ void splitter_kernel(const uint32_t* states, uint64_t n, int shift, int count, uint32_t* spl) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  uint64_t pos = uint64_t(i) << shift;
-  spl[i] = pos < n ? states[pos] : 0xFFFFFFFFu;
+// First index in [lo, hi) of the sorted states whose value is >= v (hi
+// when none is).
+__device__ uint32_t lower_bound_state(const uint32_t* __restrict__ states, uint32_t lo, uint32_t hi, uint64_t v) {
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (uint64_t(states[mid]) < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// State directory, pass 1: start[c] = first state of cell c (states whose
+// value >> cb is c), for c = 0 .. cells (start[cells] = n).
+__global__ void dir_start_kernel(const uint32_t* __restrict__ states, uint32_t n, int cb, uint32_t cells,
+                                 uint32_t* __restrict__ start) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > cells) return;
+  start[c] = lower_bound_state(states, 0, n, uint64_t(c) << cb);
+}
+
+// Pass 2: per cell the largest shift s <= cb at which no sub-bucket
+// (states sharing value >> s) holds more than k states. The longest run of
+// equal value >> s only shrinks as s falls, so s is found by bisection
+// over passes through the cell's states; s = 0 always qualifies (the
+// states are distinct).
+__global__ void dir_shift_kernel(const uint32_t* __restrict__ states, const uint32_t* __restrict__ start, int cb,
+                                 uint32_t cells, uint32_t k, uint32_t* __restrict__ shift) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cells) return;
+  const uint32_t lo = start[c], hi = start[c + 1];
+  auto fits = [&](int s) {
+    uint32_t run = 0, prev = 0;
+    for (uint32_t i = lo; i < hi; ++i) {
+      const uint32_t v = states[i] >> s;
+      run = (i > lo && v == prev) ? run + 1 : 1;
+      if (run > k) return false;
+      prev = v;
+    }
+    return true;
+  };
+  int good = 0, bad = cb + 1;  // fits(good), and every s >= bad fails
+  while (bad - good > 1) {
+    const int mid = (good + bad) / 2;
+    if (fits(mid)) good = mid;
+    else bad = mid;
+  }
+  shift[c] = uint32_t(good);
+}
+
+// Pass 3: entry j of the directory = the first state of its sub-bucket.
+// The cell of j comes from a bisection over the cells' first entries.
+__global__ void dir_fill_kernel(const uint32_t* __restrict__ states, const uint32_t* __restrict__ start,
+                                const uint32_t* __restrict__ shift, const uint64_t* __restrict__ base, int cb,
+                                uint32_t cells, uint64_t total, uint32_t* __restrict__ dir) {
+  const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (j >= total) return;
+  uint32_t c = 0, hi = cells;  // largest c with base[c] <= j
+  while (hi - c > 1) {
+    const uint32_t mid = (c + hi) / 2;
+    if (base[mid] <= j) c = mid;
+    else hi = mid;
+  }
+  const uint64_t v = (uint64_t(c) << cb) + ((j - base[c]) << shift[c]);
+  dir[j] = lower_bound_state(states, start[c], start[c + 1], v);
 }
 
 uint64_t binom_host(int n, int k) {
@@ -451,8 +509,11 @@ uint64_t enumerate_sector(EnumParams p, const GroupParams& gp, bool general, boo
     offsets[i] = total;
     total += counts[i];
   }
+  // whole 64-byte lines, and at least one all-ones line past the last state
+  // (the directory lookup reads 12 states from a 16-byte boundary at or
+  // below the key's sub-bucket)
   const uint64_t line = 64 / sizeof(ST);
-  const uint64_t padded = std::max<uint64_t>(line, (total + line - 1) / line * line);
+  const uint64_t padded = (total + line + line - 1) / line * line;
   states.resize(padded);
   QMBED_HIP(hipMemsetAsync(states.data(), 0xFF, padded * sizeof(ST), stream));
   orbits.resize(std::max<uint64_t>(total, 1));
@@ -634,44 +695,49 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
     QMBED_HIP(hipStreamSynchronize(stream));
     return b;
   }
-  const uint64_t padded = b->states_.size();
-
-  // search splitters for the LDS top of the sorted-state lookup
-  int shift = 0;
-  while (((total + (uint64_t(1) << shift) - 1) >> shift) > uint64_t(kMaxSplit)) ++shift;
-  uint64_t used = std::max<uint64_t>(1, (total + (uint64_t(1) << shift) - 1) >> shift);
-  int cnt = 1;
-  while (uint64_t(cnt) < used) cnt <<= 1;
-  b->split_shift_ = shift;
-  b->split_count_ = cnt;
-  b->splitters_.resize(cnt);
-  hipLaunchKernelGGL(splitter_kernel, dim3((cnt + 255) / 256), dim3(256), 0, stream, b->states_.data(), total, shift,
-                     cnt, b->splitters_.data());
-  QMBED_HIP(hipGetLastError());
-  const uint64_t nheads = std::max<uint64_t>(16, (padded / 16 + 15) / 16 * 16);
-  b->heads_.resize(nheads);
-  hipLaunchKernelGGL(splitter_kernel, dim3(unsigned((nheads + 255) / 256)), dim3(256), 0, stream, b->states_.data(),
-                     padded, 4, int(nheads), b->heads_.data());
-  QMBED_HIP(hipGetLastError());
-  // the 4-ary pyramid under a splitter block of 2^shift states: levels
-  // k = 1 .. (shift − 1) / 2 (the top level's 4 entries span the block)
-  b->pyramid_top_ = shift >= 1 ? (shift - 1) / 2 : 0;
-  if (b->pyramid_top_ >= SpinBasis1D::kMaxLevels) throw internal_state("sorted-state pyramid too deep");
-  b->level_off_.assign(size_t(SpinBasis1D::kMaxLevels), 0);
-  size_t tot = 0;
-  for (int k = 1; k <= b->pyramid_top_; ++k) {
-    b->level_off_[size_t(k)] = tot;
-    tot += ((padded >> (2 * k)) + 8 + 3) / 4 * 4;
-  }
-  b->pyramid_.resize(std::max<size_t>(tot, 4));
-  for (int k = 1; k <= b->pyramid_top_; ++k) {
-    const size_t cnt_k = ((padded >> (2 * k)) + 8 + 3) / 4 * 4;
-    hipLaunchKernelGGL(splitter_kernel, dim3(unsigned((cnt_k + 255) / 256)), dim3(256), 0, stream,
-                       b->states_.data(), total, 2 * k, int(cnt_k), b->pyramid_.data() + b->level_off_[size_t(k)]);
-    QMBED_HIP(hipGetLastError());
-  }
+  build_directory(*b, stream);
   QMBED_HIP(hipStreamSynchronize(stream));
   return b;
+}
+
+// The state directory of a 32-bit sector (see SpinBasis1D::directory).
+void SpinBasis1D::build_directory(SpinBasis1D& b, hipStream_t stream) {
+  const uint32_t n = uint32_t(b.n_);
+  const int L = b.geo_.L;
+  const int bits = std::min(L, kDirCellBits);
+  const int cb = L - bits;
+  const uint32_t cells = 1u << bits;
+  b.dir_shift_ = cb;
+  DeviceArray<uint32_t> start(cells + 1), shift(cells);
+  hipLaunchKernelGGL(dir_start_kernel, dim3((cells + 1 + 255) / 256), dim3(256), 0, stream, b.states_.data(), n, cb,
+                     cells, start.data());
+  QMBED_HIP(hipGetLastError());
+  hipLaunchKernelGGL(dir_shift_kernel, dim3((cells + 63) / 64), dim3(64), 0, stream, b.states_.data(), start.data(),
+                     cb, cells, uint32_t(kDirBucket), shift.data());
+  QMBED_HIP(hipGetLastError());
+  std::vector<uint32_t> sh(cells);
+  QMBED_HIP(hipMemcpyAsync(sh.data(), shift.data(), size_t(cells) * 4, hipMemcpyDeviceToHost, stream));
+  QMBED_HIP(hipStreamSynchronize(stream));
+  std::vector<uint64_t> base(cells);
+  std::vector<uint32_t> desc(2 * size_t(cells));
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < cells; ++c) {
+    base[c] = total;
+    if (total >= (uint64_t(1) << 32)) throw unsupported_backend("sector too large for the state directory");
+    desc[2 * size_t(c)] = uint32_t(total);
+    desc[2 * size_t(c) + 1] = sh[c];
+    total += uint64_t(1) << (cb - int(sh[c]));
+  }
+  if (total >= (uint64_t(1) << 32)) throw unsupported_backend("sector too large for the state directory");
+  DeviceArray<uint64_t> dbase(cells);
+  QMBED_HIP(hipMemcpyAsync(dbase.data(), base.data(), size_t(cells) * 8, hipMemcpyHostToDevice, stream));
+  b.dir_cells_.resize(2 * size_t(cells));
+  QMBED_HIP(hipMemcpyAsync(b.dir_cells_.data(), desc.data(), desc.size() * 4, hipMemcpyHostToDevice, stream));
+  b.dir_.resize(std::max<uint64_t>(total, 1));
+  hipLaunchKernelGGL(dir_fill_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, b.states_.data(),
+                     start.data(), shift.data(), dbase.data(), cb, cells, total, b.dir_.data());
+  QMBED_HIP(hipGetLastError());
+  QMBED_HIP(hipStreamSynchronize(stream));
 }
 
 std::vector<uint32_t> SpinBasis1D::states_host() const {

@@ -6,9 +6,9 @@
 //       + Σ_{flip groups g active on r} a_g(pattern(r)) · sqrt(N_r/N_t) · χ(t) · x[idx(rep(t))]
 // where t = r XOR flip_g. rep(t) and the group element mapping rep(t) to t are
 // found by walking t's translation (and reflection) orbit in registers; the
-// characters χ are tabulated in LDS; idx(rep) comes from a binary search of
-// the sorted representative table whose top levels (one splitter per
-// 2^split_shift states) are staged in LDS.
+// characters χ are tabulated in LDS; idx(rep) comes from the state
+// directory: a cell table in LDS, one sub-bucket start (L2) and one window
+// of the sorted representative table.
 //
 // This evaluates the same matrix as the reference's column-streaming
 // assembly H[row(rep_t), col(r)] += c·a·sqrt(N_r/N_t)·conj(phase_t)
@@ -56,16 +56,13 @@ struct HvArgs {
   int parity;
   int n_div;
   int divisors[16];
-  // search
-  const uint32_t* spl;
-  int nspl;   // power of two
-  int sh;
-  const uint32_t* heads;
-  uint32_t nheads;
-  // the 4-ary pyramid under each splitter block: lvl[k][i] = states[4^k i],
-  // lvl[0] = states, levels 0..lvl_top
-  const uint32_t* lvl[SpinBasis1D::kMaxLevels];
-  int lvl_top;
+  // the state directory: per cell (first entry, shift) pairs (staged in
+  // LDS by the walk-mode blocks), the sub-buckets' first state indices, the
+  // cell shift cb (cell = value >> cb) and the cell count
+  const uint32_t* dcells;
+  const uint32_t* dir;
+  int dcb;
+  int ncells;
   // lanes per row (power of two) and the mask of flip groups a lane owns
   int lpr;
   uint32_t lane_groups;
@@ -142,113 +139,56 @@ __device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
   return pat;
 }
 
-// Count of the 16 sorted entries of one aligned 64-byte line that are
-// <= key (and whether key occurs), read as four 16-byte loads.
-__device__ __forceinline__ int line_count(const uint32_t* __restrict__ line, uint32_t key, bool& found) {
-  const uint4* p = reinterpret_cast<const uint4*>(line);
-  const uint4 a = __ldg(p), b = __ldg(p + 1), c = __ldg(p + 2), d = __ldg(p + 3);
-  int cnt = int(a.x <= key) + int(a.y <= key) + int(a.z <= key) + int(a.w <= key) + int(b.x <= key) +
-            int(b.y <= key) + int(b.z <= key) + int(b.w <= key) + int(c.x <= key) + int(c.y <= key) +
-            int(c.z <= key) + int(c.w <= key) + int(d.x <= key) + int(d.y <= key) + int(d.z <= key) +
-            int(d.w <= key);
-  found = (a.x == key) | (a.y == key) | (a.z == key) | (a.w == key) | (b.x == key) | (b.y == key) | (b.z == key) |
-          (b.w == key) | (c.x == key) | (c.y == key) | (c.z == key) | (c.w == key) | (d.x == key) | (d.y == key) |
-          (d.z == key) | (d.w == key);
-  return cnt;
-}
-
-// Index of `key` in the sorted state table, or -1. Three levels:
-//  1. LDS splitters (one per 2^sh states): binary search, no global traffic;
-//  2. the aligned 64-byte line of line heads covering the bucket (heads[i] =
-//     states[16 i], a small table that stays in L2);
-//  3. the aligned 64-byte line of states holding the answer.
-// Levels 2 and 3 are single line reads counted without dependent loads.
-__device__ __forceinline__ int find_state(uint32_t key, const uint32_t* spl, int nspl, int sh,
-                                          const uint32_t* __restrict__ states, const uint32_t* __restrict__ heads,
-                                          uint32_t nheads) {
-  int pos = 0;
-  for (int step = nspl >> 1; step > 0; step >>= 1)
-    if (spl[pos + step] <= key) pos += step;
-  uint32_t line;
-  bool found;
-  if (sh <= 4) {
-    line = (uint32_t(pos) << sh) >> 4;
-  } else if (sh <= 8) {
-    const uint32_t hl = (uint32_t(pos) << (sh - 4)) & ~15u;
-    const int c = line_count(heads + hl, key, found);
-    if (c == 0) return -1;
-    line = hl + uint32_t(c) - 1;
-  } else {
-    uint32_t h0 = uint32_t(pos) << (sh - 4);
-    for (uint32_t step = (1u << (sh - 4)) >> 1; step > 0; step >>= 1)
-      if (h0 + step < nheads && __ldg(heads + h0 + step) <= key) h0 += step;
-    line = h0;
-  }
-  const int c = line_count(states + 16 * size_t(line), key, found);
-  return found ? int(16 * line + uint32_t(c) - 1) : -1;
-}
-
-// find_state for up to B keys at once (in[b] false: no key, idx[b] = -1):
-// each level's loads are issued for all keys before any is used, so the B
-// dependent chains overlap instead of running one after another.
+// Index of each of up to B keys in the sorted state table, or -1 (in[b]
+// false: no key). The key's cell descriptor (LDS) names its sub-bucket
+// j = first + (key mod 2^cb) >> shift; dir[j] is the sub-bucket's first
+// state; the 12 states from that index rounded down to a 16-byte boundary
+// hold the whole sub-bucket (at most 8 states, at most 3 places in), so the
+// key is present exactly when one of them equals it. Each stage's loads are
+// issued for all B keys before any is used.
 template <int B>
-__device__ __forceinline__ void find_states(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* spl,
-                                            const HvArgs& a, int (&idx)[B]) {
-  int pos[B];
-#pragma unroll
-  for (int b = 0; b < B; ++b) pos[b] = 0;
-  // splitter counts are powers of two up to 8192: the steps unrolled with
-  // a uniform test each
-#pragma unroll
-  for (int lg = 12; lg >= 0; --lg) {
-    const int step = 1 << lg;
-    if (step < a.nspl) {
-#pragma unroll
-      for (int b = 0; b < B; ++b) pos[b] += spl[pos[b] + step] <= key[b] ? step : 0;
-    }
-  }
-  // Below the splitter, the 4-ary pyramid: at level k one aligned 16-byte
-  // node of 4 entries states[4^k i]; the count of entries <= key names the
-  // child. Every entry of a node before the block start is <= key (it
-  // precedes the splitter), every entry past the block is > key (it follows
-  // the next splitter), so the count is right for the nodes that straddle
-  // the block's ends. One line access per level instead of the four of a
-  // 64-byte line, the L1's rate being one line per clock for scattered loads.
-  // (every node read stays in its level: base is the start of a block, or
-  // of a child node, of a real state whatever the key, so the loads need no
-  // predicate; a key below its node only clears its bit in `ok`)
-  uint32_t base[B];
-  uint32_t ok = 0;
+__device__ __forceinline__ void dir_find(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* cells,
+                                         const HvArgs& a, int (&idx)[B]) {
+  static_assert(SpinBasis1D::kDirWindow == 12 && SpinBasis1D::kDirBucket + 3 <= SpinBasis1D::kDirWindow,
+                "the window is three 16-byte loads covering a sub-bucket");
+  const uint32_t lmask = (1u << a.dcb) - 1u;
+  uint32_t lo[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) {
-    base[b] = uint32_t(pos[b]) << a.sh;
-    ok |= in[b] ? 1u << b : 0u;
+    const uint2 d = reinterpret_cast<const uint2*>(cells)[key[b] >> a.dcb];
+    const uint32_t j = d.x + ((key[b] & lmask) >> d.y);
+    lo[b] = in[b] ? __ldg(a.dir + j) & ~3u : 0u;
   }
-  for (int k = a.lvl_top; k > 0; --k) {
-    const uint32_t* lv = a.lvl[k];
-    uint4 nd[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) nd[b] = __ldg(reinterpret_cast<const uint4*>(lv + ((base[b] >> (2 * k)) & ~3u)));
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      const uint32_t g = (base[b] >> (2 * k)) & ~3u;
-      const uint32_t c = uint32_t(nd[b].x <= key[b]) + uint32_t(nd[b].y <= key[b]) + uint32_t(nd[b].z <= key[b]) +
-                         uint32_t(nd[b].w <= key[b]);
-      if (c == 0) ok &= ~(1u << b);
-      base[b] = (g + (c > 0 ? c - 1 : 0u)) << (2 * k);
-    }
-  }
-  // level 0: the 4 states of the final node; the key is there or absent
-  uint4 nd[B];
-#pragma unroll
-  for (int b = 0; b < B; ++b) nd[b] = __ldg(reinterpret_cast<const uint4*>(a.lvl[0] + (base[b] & ~3u)));
+  uint4 w[B][3];
 #pragma unroll
   for (int b = 0; b < B; ++b) {
-    const uint32_t g = base[b] & ~3u;
+    const uint4* p = reinterpret_cast<const uint4*>(a.states + lo[b]);
+    w[b][0] = __ldg(p);
+    w[b][1] = __ldg(p + 1);
+    w[b][2] = __ldg(p + 2);
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
     const uint32_t k = key[b];
-    const int at = nd[b].x == k ? 0 : nd[b].y == k ? 1 : nd[b].z == k ? 2 : nd[b].w == k ? 3 : -1;
-    idx[b] = ((ok >> b) & 1u) && at >= 0 ? int(g) + at : -1;
+    int pos = -1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      pos = w[b][q].x == k ? 4 * q : pos;
+      pos = w[b][q].y == k ? 4 * q + 1 : pos;
+      pos = w[b][q].z == k ? 4 * q + 2 : pos;
+      pos = w[b][q].w == k ? 4 * q + 3 : pos;
+    }
+    idx[b] = in[b] && pos >= 0 ? int(lo[b]) + pos : -1;
   }
+}
+
+// dir_find for one key, with the cell table in global memory
+__device__ __forceinline__ int dir_find1(uint32_t key, const HvArgs& a) {
+  const uint32_t k[1] = {key};
+  const bool in[1] = {true};
+  int idx[1];
+  dir_find<1>(k, in, a.dcells, a, idx);
+  return idx[0];
 }
 
 // Colex (Gosper-order) rank of a fixed-weight state: Σ C(position, ordinal)
@@ -475,7 +415,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
   uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_crsq + 16);
   uint32_t* s_hop = s_meta + 4 * (a.n_diag + a.n_off);     // G (image mode)
   uint32_t* s_bin = s_hop + (a.n_diag + a.n_off);          // 1024 (image mode)
-  uint32_t* s_spl = IMG ? s_bin + 1024 : s_hop;            // nspl (walk mode)
+  uint32_t* s_cells = IMG ? s_bin + 1024 : s_hop;          // 2*ncells (walk mode; 8-byte aligned)
 
   const int G = a.n_diag + a.n_off;
   const int namp = (CMODE == 2 ? 2 : 1) * a.n_amp;
@@ -487,7 +427,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
       for (int i = threadIdx.x; i < 1024; i += kBlock) s_bin[i] = a.binom[i];
     for (int i = threadIdx.x; i < 16; i += kBlock) s_crsq[i] = a.code_rsq[i];
   } else {
-    for (int i = threadIdx.x; i < a.nspl; i += kBlock) s_spl[i] = a.spl[i];
+    for (int i = threadIdx.x; i < 2 * a.ncells; i += kBlock) s_cells[i] = a.dcells[i];
   }
   if (SYM == 1 || SYM == 3)
     for (int i = threadIdx.x; i < 2 * a.nchar; i += kBlock) s_chr[i] = a.chars[i];
@@ -689,7 +629,7 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
         }
       }
       int idx[B];
-      find_states<B>(key, in, s_spl, a, idx);
+      dir_find<B>(key, in, s_cells, a, idx);
       S xr[B][NV], xi[B][NV];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
@@ -887,8 +827,8 @@ __device__ __forceinline__ int g_off(const HvArgs& a, const uint32_t* meta, int 
 }
 
 template <typename ST>
-__device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_needed, ST t, const uint32_t* spl,
-                                          int& idx, int& orbit, int& ci, int& sign) {
+__device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_needed, ST t, int& idx, int& orbit,
+                                          int& ci, int& sign) {
   if constexpr (sizeof(ST) == 8) {
     uint64_t rep = t;
     orbit = 1;
@@ -915,7 +855,7 @@ __device__ __forceinline__ bool target_of(const HvArgs& a, int sym, int shift_ne
   } else if (sym == 4) {
     general_walk(t, a, rep, orbit, ci);
   }
-  idx = find_state(rep, spl, a.nspl, a.sh, a.states, a.heads, a.nheads);
+  idx = dir_find1(rep, a);
   if (idx >= 0 && a.rev) idx = int(a.n) - 1 - idx;
   return idx >= 0;
   }
@@ -940,7 +880,7 @@ __device__ __forceinline__ void column_value(const HvArgs& a, const double* famp
 
 template <typename ST, int CAP>
 __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
-                               const uint32_t* spl, uint32_t col, ColList<CAP>& out) {
+                               uint32_t col, ColList<CAP>& out) {
   const ST r = row_state<ST>(a, col);
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   for (int g = 0; g < a.n_diag; ++g) {
@@ -953,7 +893,7 @@ __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const doub
     const int pat = g_pattern<ST>(a, meta, r, g);
     if (!((g_nz<ST>(a, meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), idx, orbit, ci, sign)) continue;
     double re, im;
     column_value(a, famp, g_off<ST>(a, meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
     if (uint32_t(idx) == col) {
@@ -981,7 +921,7 @@ __device__ void collect_column(const HvArgs& a, const uint32_t* meta, const doub
 // merged entry H[row, col] (all contributions of column col landing on row)
 template <typename ST>
 __device__ void entry_at(const HvArgs& a, const uint32_t* meta, const double* famp, int sym, int shift_needed,
-                         const uint32_t* spl, uint32_t col, uint32_t row, double& vre, double& vim) {
+                         uint32_t col, uint32_t row, double& vre, double& vim) {
   const ST r = row_state<ST>(a, col);
   const double ns = sym ? double(a.orbits[col]) : 1.0;
   vre = 0;
@@ -998,7 +938,7 @@ __device__ void entry_at(const HvArgs& a, const uint32_t* meta, const double* fa
     const int pat = g_pattern<ST>(a, meta, r, g);
     if (!((g_nz<ST>(a, meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, meta, g)), idx, orbit, ci, sign)) continue;
     if (uint32_t(idx) != row) continue;
     double re, im;
     column_value(a, famp, g_off<ST>(a, meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
@@ -1019,7 +959,7 @@ __global__ void __launch_bounds__(kBlock) column_entries_kernel(HvArgs a, const 
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, col, C);
   if (C.overflow) {
     *overflow = 1;
     return;
@@ -1067,7 +1007,7 @@ __global__ void __launch_bounds__(kBlock) column_fill_kernel(HvArgs a, const dou
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, col, C);
   if (C.overflow) {
     *overflow = 1;
     return;
@@ -1223,7 +1163,7 @@ __global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const dou
   const uint32_t col = blockIdx.x * kBlock + threadIdx.x;
   if (col >= a.n) return;
   ColList<kMaxColEntries> C;
-  collect_column<ST>(a, a.meta, famp, sym, shift_needed, a.spl, col, C);
+  collect_column<ST>(a, a.meta, famp, sym, shift_needed, col, C);
   if (C.overflow) {
     flags[1] = 1;
     return;
@@ -1234,7 +1174,7 @@ __global__ void __launch_bounds__(kBlock) hermiticity_kernel(HvArgs a, const dou
   for (int k = 0; k < C.cnt && !bad; ++k) {
     if (hypot(C.vre[k], C.vim[k]) <= kEps) continue;  // not stored
     double wr, wi;
-    entry_at<ST>(a, a.meta, famp, sym, shift_needed, a.spl, uint32_t(C.rows[k]), col, wr, wi);
+    entry_at<ST>(a, a.meta, famp, sym, shift_needed, uint32_t(C.rows[k]), col, wr, wi);
     if (hypot(wr, wi) <= kEps) wr = wi = 0.0;
     if (hypot(C.vre[k] - wr, C.vim[k] + wi) > tol) bad = true;
   }
@@ -1264,7 +1204,7 @@ __global__ void __launch_bounds__(kBlock) scatter_kernel(HvArgs a, const double*
     const int pat = g_pattern<ST>(a, a.meta, r, g);
     if (!((g_nz<ST>(a, a.meta, g) >> pat) & 1u)) continue;
     int idx, orbit, ci, sign;
-    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), a.spl, idx, orbit, ci, sign)) continue;
+    if (!target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), idx, orbit, ci, sign)) continue;
     double re, im;
     column_value(a, famp, g_off<ST>(a, a.meta, g) + pat, sym, ns, orbit, ci, sign, re, im);
     atomicAdd(y + 2 * idx, re * xc.x - im * xc.y);
@@ -1305,7 +1245,7 @@ __global__ void __launch_bounds__(kBlock) image_kernel(HvArgs a, int sym, int sh
     uint32_t e = 0xFFFFFFFFu;
     if (a.up >= 0 || ((popmask >> __popc(s)) & 1ull)) {
       int idx, orbit, ci, sign;
-      if (target_of<uint32_t>(a, sym, shift_needed, s, a.spl, idx, orbit, ci, sign))
+      if (target_of<uint32_t>(a, sym, shift_needed, s, idx, orbit, ci, sign))
         e = uint32_t(idx) | (uint32_t(orbit_code[orbit]) << a.img_bits) |
             (uint32_t(sign < 0) << (a.img_bits + 4)) | (uint32_t(ci) << (a.img_bits + 5));
     }
@@ -1402,7 +1342,7 @@ __global__ void __launch_bounds__(kBlock) stream_fill_kernel(HvArgs a, int sym, 
     for (int g = ag.next(); g >= 0; g = ag.next()) {
       uint32_t e = 0xFFFFFFFFu;
       int idx, orbit, ci, sign;
-      if (target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), a.spl, idx, orbit, ci, sign)) {
+      if (target_of<ST>(a, sym, shift_needed, ST(r ^ g_flip<ST>(a, a.meta, g)), idx, orbit, ci, sign)) {
         if (a.compact) {
           const uint32_t off = uint32_t(g_off<ST>(a, a.meta, g)) + uint32_t(g_pattern<ST>(a, a.meta, r, g)) -
                                uint32_t(a.c_off_base);
@@ -2114,13 +2054,10 @@ struct Operator::DeviceTables {
     a.parity = g.parity;
     a.n_div = g.n_div;
     for (int i = 0; i < 16; ++i) a.divisors[i] = g.divisors[i];
-    a.spl = b.splitters();
-    a.nspl = b.split_count();
-    a.sh = b.split_shift();
-    a.heads = b.heads();
-    a.nheads = uint32_t(b.heads_count());
-    a.lvl_top = b.pyramid_top();
-    for (int k = 0; k <= a.lvl_top; ++k) a.lvl[k] = b.level(k);
+    a.dcells = b.directory_cells();
+    a.dir = b.directory();
+    a.dcb = b.directory_shift();
+    a.ncells = b.wide() ? 0 : (1 << (g.L - a.dcb));
     a.lpr = lpr;
     uint32_t mask = 0;
     for (int i = 0; i < 32; i += lpr) mask |= 1u << i;
@@ -2528,7 +2465,7 @@ namespace {
 size_t hv_smem(const HvArgs& a, int cmode, bool img) {
   const size_t G = size_t(a.n_diag + a.n_off);
   return sizeof(double) * ((cmode == 2 ? 2 : 1) * size_t(a.n_amp) + 2 * kMaxChars + 130 + 16) +
-         sizeof(uint32_t) * (4 * G + G + (img ? 1024 : size_t(a.nspl)));
+         sizeof(uint32_t) * (4 * G + G + (img ? 1024 : 2 * size_t(a.ncells)));
 }
 
 template <typename S, int CMODE, int NV, bool IMG>

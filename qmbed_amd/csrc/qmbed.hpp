@@ -227,21 +227,20 @@ class SpinBasis1D {
   std::vector<uint64_t> row_states_host64() const;
   std::vector<uint8_t> row_orbits_host() const;
 
-  // Top of the sorted-state search tree kept in LDS by the H·v kernels:
-  // splitters[i] = states[i << split_shift], padded to a power of two.
-  const uint32_t* splitters() const { return splitters_.data(); }
-  // line heads: heads[i] = states[16 i]; states and heads are padded with
-  // 0xFFFFFFFF to whole 64-byte lines so a lookup reads aligned lines
-  const uint32_t* heads() const { return heads_.data(); }
-  size_t heads_count() const { return heads_.size(); }
-  int split_count() const { return split_count_; }   // padded (power of two)
-  int split_shift() const { return split_shift_; }
-  // Below a splitter, a 4-ary pyramid: level k holds states[4^k i] (level 0
-  // is the state table), padded with 0xFFFFFFFF to whole 16-byte nodes and
-  // beyond the last node a lookup can read; levels 1..pyramid_top() exist.
-  static constexpr int kMaxLevels = 11;
-  int pyramid_top() const { return pyramid_top_; }
-  const uint32_t* level(int k) const { return k == 0 ? states_.data() : pyramid_.data() + level_off_[size_t(k)]; }
+  // State directory of a 32-bit sector (the H·v kernels' lookup): the top
+  // min(L, kDirCellBits) bits of a value name its cell; cell c splits its
+  // value range into 2^(cb − s_c) sub-buckets of 2^s_c values, s_c the
+  // largest shift with at most kDirBucket states in every sub-bucket.
+  // cells() holds per cell (first directory entry, s_c); directory()[j]
+  // the index of the first state of sub-bucket j. A lookup reads the
+  // kDirWindow states from that index rounded down to a 16-byte boundary,
+  // which cover the sub-bucket (the state table ends in an all-ones line).
+  static constexpr int kDirCellBits = 11;
+  static constexpr int kDirBucket = 8;
+  static constexpr int kDirWindow = 12;
+  const uint32_t* directory_cells() const { return dir_cells_.data(); }
+  const uint32_t* directory() const { return dir_.data(); }
+  int directory_shift() const { return dir_shift_; }  // cb = L − cell bits
 
  private:
   SpinBasisSpec spec_;
@@ -249,9 +248,6 @@ class SpinBasis1D {
   size_t n_ = 0;
   DeviceArray<uint32_t> states_;
   DeviceArray<uint8_t> orbits_;
-  DeviceArray<uint32_t> splitters_;
-  DeviceArray<uint32_t> heads_;
-  DeviceArray<uint32_t> pyramid_;
   DeviceArray<uint32_t> rstates_;
   DeviceArray<uint8_t> rorbits_;
   DeviceArray<uint64_t> states64_, rstates64_;
@@ -260,10 +256,9 @@ class SpinBasis1D {
   DeviceArray<uint8_t> gphase_;
   std::vector<cplx> chars_;
   bool real_chars_ = true;
-  std::vector<size_t> level_off_;
-  int pyramid_top_ = 0;
-  int split_count_ = 0;
-  int split_shift_ = 0;
+  DeviceArray<uint32_t> dir_cells_, dir_;
+  int dir_shift_ = 0;
+  static void build_directory(SpinBasis1D& b, hipStream_t stream);
 };
 
 // --------------------------------------------------------------- operator --
