@@ -142,6 +142,13 @@ double qmbed_hip_time_apply(const qmbed_hip_model* model, int dtype, const void*
  * untimed launches. -2 when the model does not support batched scan
  * launches of that width, -1 on failure. No counterpart in the reference. */
 double qmbed_hip_time_scan_batch(qmbed_hip_model* model, int width, int fp32, const double* points, int iters);
+/* y_b = H(c_b) x_b through the same batched scan H·v for `width` lanes (1, 2,
+ * 4, 8 or 16): points as above, x and y `width` host vectors of the
+ * dimension (column b at offset b·n), the inputs rounded to FP32 when
+ * fp32 != 0. 0 on success, -2 when the model does not support batched scan
+ * launches of that width, -1 on failure. No counterpart in the reference. */
+int qmbed_hip_apply_scan_batch(qmbed_hip_model* model, int width, int fp32, const double* points, const double* x,
+                               double* y);
 /* Launch timer (no counterpart in the reference): on != 0 clears the
  * records and brackets every later batched scan H·v launch with two HIP
  * events on its stream; 0 stops recording. The report answers

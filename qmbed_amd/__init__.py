@@ -108,6 +108,8 @@ def library():
     L.qmbed_hip_eigsh_workspace_json.argtypes = [c_void_p, c_void_p, c_char_p]
     L.qmbed_hip_time_scan_batch.restype = ctypes.c_double
     L.qmbed_hip_time_scan_batch.argtypes = [c_void_p, c_int, c_int, c_void_p, c_int]
+    L.qmbed_hip_apply_scan_batch.restype = c_int
+    L.qmbed_hip_apply_scan_batch.argtypes = [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]
     L.qmbed_hip_launch_timer.restype = c_int
     L.qmbed_hip_launch_timer.argtypes = [c_int]
     L.qmbed_hip_launch_timer_report.restype = c_void_p
@@ -291,6 +293,23 @@ class Model:
         if ms < 0:
             raise QmbedError("timed apply failed")
         return ms
+
+    def apply_scan_batch(self, points, x, fp32=True):
+        """y[b] = H(points[b]) x[b] through the batched scan H·v (the kernel
+        of lockstep scan solves) for len(points) lanes (1, 2, 4, 8 or 16);
+        x real, shape (lanes, dim), rounded to FP32 when fp32."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        xs = np.ascontiguousarray(x, dtype=np.float64)
+        if xs.shape != (pts.shape[0], self.dim):
+            raise QmbedError("x must hold one vector of the dimension per point")
+        y = np.zeros_like(xs)
+        rc = library().qmbed_hip_apply_scan_batch(self._h, pts.shape[0], 1 if fp32 else 0, pts.ctypes.data,
+                                                  xs.ctypes.data, y.ctypes.data)
+        if rc == -2:
+            raise QmbedError("batched scan launches of this width are not supported by the model")
+        if rc < 0:
+            raise QmbedError("batched scan launch failed")
+        return y
 
     def time_scan_batch(self, points, fp32=True, iters=20):
         """Average ms per launch of the batched scan H·v over len(points)

@@ -336,11 +336,12 @@ __global__ void dir_shift_kernel(const uint32_t* __restrict__ states, const uint
   shift[c] = uint32_t(good);
 }
 
-// Pass 3: entry j of the directory = the first state of its sub-bucket.
-// The cell of j comes from a bisection over the cells' first entries.
+// Pass 3: slot j (4 words) of the directory: the index of the first state
+// of sub-bucket j and its states (at most 3, ascending, then all ones). The
+// cell of j comes from a bisection over the cells' first slots.
 __global__ void dir_fill_kernel(const uint32_t* __restrict__ states, const uint32_t* __restrict__ start,
                                 const uint32_t* __restrict__ shift, const uint64_t* __restrict__ base, int cb,
-                                uint32_t cells, uint64_t total, uint32_t* __restrict__ dir) {
+                                uint32_t cells, uint64_t total, uint32_t* __restrict__ slots) {
   const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (j >= total) return;
   uint32_t c = 0, hi = cells;  // largest c with base[c] <= j
@@ -349,8 +350,14 @@ __global__ void dir_fill_kernel(const uint32_t* __restrict__ states, const uint3
     if (base[mid] <= j) c = mid;
     else hi = mid;
   }
-  const uint64_t v = (uint64_t(c) << cb) + ((j - base[c]) << shift[c]);
-  dir[j] = lower_bound_state(states, start[c], start[c + 1], v);
+  const uint64_t local = j - base[c];
+  const uint64_t v = (uint64_t(c) << cb) + (local << shift[c]);
+  const uint64_t vend = (uint64_t(c) << cb) + ((local + 1) << shift[c]);
+  const uint32_t first = lower_bound_state(states, start[c], start[c + 1], v);
+  const uint32_t last = lower_bound_state(states, first, start[c + 1], vend);
+  uint32_t* out = slots + 4 * j;
+  out[0] = first;
+  for (uint32_t i = 0; i < 3; ++i) out[1 + i] = first + i < last ? states[first + i] : 0xFFFFFFFFu;
 }
 
 uint64_t binom_host(int n, int k) {
@@ -509,11 +516,8 @@ uint64_t enumerate_sector(EnumParams p, const GroupParams& gp, bool general, boo
     offsets[i] = total;
     total += counts[i];
   }
-  // whole 64-byte lines, and at least one all-ones line past the last state
-  // (the directory lookup reads 12 states from a 16-byte boundary at or
-  // below the key's sub-bucket)
   const uint64_t line = 64 / sizeof(ST);
-  const uint64_t padded = (total + line + line - 1) / line * line;
+  const uint64_t padded = std::max<uint64_t>(line, (total + line - 1) / line * line);
   states.resize(padded);
   QMBED_HIP(hipMemsetAsync(states.data(), 0xFF, padded * sizeof(ST), stream));
   orbits.resize(std::max<uint64_t>(total, 1));
@@ -733,7 +737,7 @@ void SpinBasis1D::build_directory(SpinBasis1D& b, hipStream_t stream) {
   QMBED_HIP(hipMemcpyAsync(dbase.data(), base.data(), size_t(cells) * 8, hipMemcpyHostToDevice, stream));
   b.dir_cells_.resize(2 * size_t(cells));
   QMBED_HIP(hipMemcpyAsync(b.dir_cells_.data(), desc.data(), desc.size() * 4, hipMemcpyHostToDevice, stream));
-  b.dir_.resize(std::max<uint64_t>(total, 1));
+  b.dir_.resize(4 * std::max<uint64_t>(total, 1));
   hipLaunchKernelGGL(dir_fill_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, b.states_.data(),
                      start.data(), shift.data(), dbase.data(), cb, cells, total, b.dir_.data());
   QMBED_HIP(hipGetLastError());

@@ -1311,6 +1311,49 @@ double qmbed_hip_time_scan_batch(qmbed_hip_model* m, int width, int fp32, const 
   }
 }
 
+int qmbed_hip_apply_scan_batch(qmbed_hip_model* m, int width, int fp32, const double* points, const double* x,
+                               double* y) {
+  try {
+    auto* tm = reinterpret_cast<TypedModel*>(m);
+    std::lock_guard<std::mutex> g(tm->m->mu);
+    Operator& op = tm->m->materialized();
+    hipStream_t s = tm->m->stream;
+    if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16)) return -2;
+    if (width > op.scan_batch_max_width() || !op.scan_batch_supported(s)) return -2;
+    if (!points || !x || !y) return -1;
+    const size_t n = op.dim();
+    if (n == 0) return 0;
+    const int count = op.scan_batch_amp_count();
+    const size_t P = tm->m->components.size();
+    std::vector<std::vector<cplx>> coefs(static_cast<size_t>(width));
+    for (int b = 0; b < width; ++b)
+      for (size_t p = 0; p < P; ++p) coefs[size_t(b)].push_back(cplx(points[size_t(b) * P + p], 0.0));
+    DeviceArray<double> amps(size_t(width) * size_t(count)), diags(size_t(width) * 2 * n), ys(size_t(width) * n);
+    DeviceArray<double> cols(size_t(width) * n);
+    DeviceArray<uint8_t> xi((fp32 ? 4 : 8) * size_t(width) * n);
+    op.scan_batch_tables(coefs, amps.data(), diags.data(), s);
+    QMBED_HIP(hipMemcpyAsync(cols.data(), x, size_t(width) * n * 8, hipMemcpyHostToDevice, s));
+    std::vector<const double*> src(static_cast<size_t>(width));
+    for (int b = 0; b < width; ++b) src[size_t(b)] = cols.data() + size_t(b) * n;
+    lk::interleave(src.data(), n, width, fp32 ? nullptr : reinterpret_cast<double*>(xi.data()),
+                   fp32 ? reinterpret_cast<float*>(xi.data()) : nullptr, s);
+    const double* am[16] = {};
+    const double* dg[16] = {};
+    double* yp[16] = {};
+    for (int b = 0; b < width; ++b) {
+      am[b] = amps.data() + size_t(b) * count;
+      dg[b] = diags.data() + size_t(b) * 2 * n;
+      yp[b] = ys.data() + size_t(b) * n;
+    }
+    op.apply_scan_batch(xi.data(), fp32 != 0, width, am, dg, yp, nullptr, s);
+    QMBED_HIP(hipMemcpyAsync(y, ys.data(), size_t(width) * n * 8, hipMemcpyDeviceToHost, s));
+    QMBED_HIP(hipStreamSynchronize(s));
+    return 0;
+  } catch (...) {
+    return -1;
+  }
+}
+
 int qmbed_hip_launch_timer(int on) {
   try {
     launch_timer_switch(on != 0);

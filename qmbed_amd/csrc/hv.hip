@@ -56,9 +56,9 @@ struct HvArgs {
   int parity;
   int n_div;
   int divisors[16];
-  // the state directory: per cell (first entry, shift) pairs (staged in
-  // LDS by the walk-mode blocks), the sub-buckets' first state indices, the
-  // cell shift cb (cell = value >> cb) and the cell count
+  // the state directory: per cell (first slot, shift) pairs (staged in LDS
+  // by the walk-mode blocks), the 16-word slots of the sub-buckets, the cell
+  // shift cb (cell = value >> cb) and the cell count
   const uint32_t* dcells;
   const uint32_t* dir;
   int dcb;
@@ -139,46 +139,29 @@ __device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
   return pat;
 }
 
-// Index of each of up to B keys in the sorted state table, or -1 (in[b]
-// false: no key). The key's cell descriptor (LDS) names its sub-bucket
-// j = first + (key mod 2^cb) >> shift; dir[j] is the sub-bucket's first
-// state; the 12 states from that index rounded down to a 16-byte boundary
-// hold the whole sub-bucket (at most 8 states, at most 3 places in), so the
-// key is present exactly when one of them equals it. Each stage's loads are
-// issued for all B keys before any is used.
+// Index of each of up to B keys in the sorted state table (-1: absent, or
+// in[b] false). The key's cell descriptor (LDS) names its sub-bucket
+// j = first + (key mod 2^cb) >> shift, whose 16-byte slot holds the index
+// of its first state and its states (at most 3, padded with all ones); the
+// key is present exactly when one of them equals it. The slots of all B
+// keys are loaded before any is used, unconditionally: every key below 2^L
+// names a slot, and in[b] enters only the result.
 template <int B>
 __device__ __forceinline__ void dir_find(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* cells,
                                          const HvArgs& a, int (&idx)[B]) {
-  static_assert(SpinBasis1D::kDirWindow == 12 && SpinBasis1D::kDirBucket + 3 <= SpinBasis1D::kDirWindow,
-                "the window is three 16-byte loads covering a sub-bucket");
+  static_assert(SpinBasis1D::kDirBucket == 3, "a slot holds 3 states");
   const uint32_t lmask = (1u << a.dcb) - 1u;
-  uint32_t lo[B];
+  uint4 w[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const uint2 d = reinterpret_cast<const uint2*>(cells)[key[b] >> a.dcb];
-    const uint32_t j = d.x + ((key[b] & lmask) >> d.y);
-    lo[b] = in[b] ? __ldg(a.dir + j) & ~3u : 0u;
-  }
-  uint4 w[B][3];
-#pragma unroll
-  for (int b = 0; b < B; ++b) {
-    const uint4* p = reinterpret_cast<const uint4*>(a.states + lo[b]);
-    w[b][0] = __ldg(p);
-    w[b][1] = __ldg(p + 1);
-    w[b][2] = __ldg(p + 2);
+    w[b] = __ldg(reinterpret_cast<const uint4*>(a.dir) + (d.x + ((key[b] & lmask) >> d.y)));
   }
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const uint32_t k = key[b];
-    int pos = -1;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      pos = w[b][q].x == k ? 4 * q : pos;
-      pos = w[b][q].y == k ? 4 * q + 1 : pos;
-      pos = w[b][q].z == k ? 4 * q + 2 : pos;
-      pos = w[b][q].w == k ? 4 * q + 3 : pos;
-    }
-    idx[b] = in[b] && pos >= 0 ? int(lo[b]) + pos : -1;
+    const int pos = w[b].y == k ? 0 : w[b].z == k ? 1 : w[b].w == k ? 2 : -1;
+    idx[b] = in[b] && pos >= 0 ? int(w[b].x) + pos : -1;
   }
 }
 
@@ -369,6 +352,88 @@ __device__ __forceinline__ void walk(uint32_t t, const HvArgs& a, uint32_t& rep,
     rep = rep_o;
     orbit = (rep_p == rep_o) ? d : 2 * d;
     ci = (L - j_o) % L;
+  }
+}
+
+// The orbit walks of B parent states at once, each as in `walk`, with the
+// rotation and period loops shared by the B states so that the loop control
+// (scalar instructions: L is a launch parameter) is paid once per step
+// rather than once per state. Returns the representatives, orbit sizes and
+// the phases as (ci, sign).
+template <int SYM, bool SHIFT, int B>
+__device__ __forceinline__ void walk_batch(const uint32_t (&t)[B], const HvArgs& a, uint32_t (&rep)[B], int (&orbit)[B],
+                                           int (&ci)[B], int (&sign)[B]) {
+  const int L = a.L;
+  if (SYM == 2 || (SHIFT && L > (1 << (32 - L)))) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) walk_scan<SYM, SHIFT>(t[b], a, rep[b], orbit[b], ci[b], sign[b]);
+    return;
+  }
+  const int lowb = 32 - L;  // L <= 31
+  const uint32_t low = (1u << lowb) - 1u;
+  uint32_t hi[B], lo[B];
+  auto min_words = [&](uint32_t (&mn)[B]) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) mn[b] = SHIFT ? (hi[b] & ~low) : hi[b];
+    auto word = [&](int b, int j) {
+      const uint32_t w = __builtin_amdgcn_alignbit(hi[b], lo[b], uint32_t(32 - j));
+      return SHIFT ? (w & ~low) | uint32_t(j) : w;
+    };
+    int j = 1;
+    for (; j + 1 < L; j += 2) {
+#pragma unroll
+      for (int b = 0; b < B; ++b) mn[b] = min(mn[b], min(word(b, j), word(b, j + 1)));
+    }
+    if (j < L) {
+#pragma unroll
+      for (int b = 0; b < B; ++b) mn[b] = min(mn[b], word(b, j));
+    }
+  };
+  auto doubled = [&](int b, uint32_t s) {
+    const uint64_t y = (uint64_t(s) | (uint64_t(s) << L)) << (64 - 2 * L);
+    hi[b] = uint32_t(y >> 32);
+    lo[b] = uint32_t(y);
+  };
+  uint32_t mn[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) doubled(b, t[b]);
+  min_words(mn);
+  // the period: the smallest divisor dv of L with rotl(t, dv) = t
+  // (descending, the last match wins)
+  int d[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) d[b] = L;
+#pragma unroll 1
+  for (int i = a.n_div - 1; i >= 0; --i) {
+    const uint32_t sh = uint32_t(32 - a.divisors[i]);
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if ((__builtin_amdgcn_alignbit(hi[b], lo[b], sh) >> lowb) == t[b]) d[b] = a.divisors[i];
+  }
+  if (SYM == 1) {
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      rep[b] = mn[b] >> lowb;
+      orbit[b] = d[b];
+      const int j = SHIFT ? int(mn[b] & low) : 0;
+      ci[b] = j ? L - j : 0;
+      sign[b] = 1;
+    }
+    return;
+  }
+  uint32_t mnp[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) doubled(b, reflect_l(t[b], L));
+  min_words(mnp);
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    const uint32_t rep_o = mn[b] >> lowb, rep_p = mnp[b] >> lowb;
+    const bool p = rep_p < rep_o;
+    rep[b] = p ? rep_p : rep_o;
+    orbit[b] = rep_p == rep_o ? d[b] : 2 * d[b];
+    const int j = SHIFT ? int((p ? mnp[b] : mn[b]) & low) : 0;
+    ci[b] = j ? L - j : 0;
+    sign[b] = p ? a.parity : 1;
   }
 }
 
@@ -605,27 +670,33 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
     // loop, so the sums are the same).
     constexpr int B = NV > 1 ? 4 : 8;
     while (act) {
-      uint32_t key[B];
+      uint32_t tgt[B], key[B];
       bool in[B];
-      int offs[B], cis[B], sgn[B], orb[B];
+      int offs[B], cis[B], sgn[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) {
         in[b] = false;
-        key[b] = 0;
+        tgt[b] = r;
         offs[b] = 0;
-        cis[b] = 0;
-        sgn[b] = 1;
-        orb[b] = 1;
         if (act) {
           const int gi = __ffs(act) - 1;
           act &= act - 1;
           const int g = a.n_diag + w0 + gi;
-          const uint32_t t = r ^ s_meta[4 * g];
+          tgt[b] = r ^ s_meta[4 * g];
           offs[b] = int(s_meta[4 * g + 3]) + pattern_upto(r, s_meta[4 * g + 1], nmax);
-          uint32_t rep = t;
-          if (SYM) walk<SYM, SHIFT>(t, a, rep, orb[b], cis[b], sgn[b]);
-          key[b] = rep;
           in[b] = true;
+        }
+      }
+      int orb[B];
+      if constexpr (SYM != 0) {
+        walk_batch<SYM, SHIFT, B>(tgt, a, key, orb, cis, sgn);
+      } else {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          key[b] = tgt[b];
+          orb[b] = 1;
+          cis[b] = 0;
+          sgn[b] = 1;
         }
       }
       int idx[B];
@@ -1720,6 +1791,120 @@ __global__ void __launch_bounds__(kBlock) hv_scan_batch_kernel(HvArgs a, ScanBat
     if (on[b]) sb.y[b][row] = acc[b];
 }
 
+// The batched scan action of hv_scan_batch_kernel computed on the fly: per
+// target the orbit walk and the state-directory lookup of the walk-mode
+// hv_kernel, done once for the B points (T targets of a lane at a time, the
+// walks sharing their rotation loops), then one gathered line of the
+// interleaved inputs x[idx·B + b] serves all points. No index stream is
+// read. A row's targets are taken in group order with the unreachable ones
+// left out, which is the order of its stream words, and every factor is the
+// stream kernel's (the amplitude entry, the reflection sign, sqrt(N_r) from
+// the same LDS table and 1/sqrt(N_t) from the host table the orbit codes
+// use), so both kernels give bit-identical results.
+// LDS: B tables of the off-diagonal amplitudes (na entries each), sqrt(i),
+// 1/sqrt(i) (i < 65), the off-diagonal group words and the directory cells.
+template <typename SX, int B, int SYM>
+__global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatch sb, const SX* __restrict__ x,
+                                                              int off_base, const double* __restrict__ size_rsq) {
+  extern __shared__ double smem_d[];
+  const int na = a.n_amp - off_base;
+  double* s_amp = smem_d;                                          // B * na
+  double* s_sq = s_amp + B * na;                                   // 65
+  double* s_rsq = s_sq + 65;                                       // 65
+  uint32_t* s_meta = reinterpret_cast<uint32_t*>(s_rsq + 65);      // 4 * n_off
+  uint32_t* s_cells = s_meta + 4 * a.n_off;                        // 2 * ncells (8-byte aligned)
+  bool on[B];
+  bool any = false;
+#pragma unroll
+  for (int b = 0; b < B; ++b) {
+    on[b] = sb.y[b] != nullptr && !(sb.gate[b] && *sb.gate[b]);
+    any = any || on[b];
+  }
+  if (!any) return;
+  for (int i = threadIdx.x; i < B * na; i += kBlock) {
+    const int b = i / na;
+    s_amp[i] = on[b] ? sb.amp[b][off_base + i - b * na] : 0.0;
+  }
+  for (int i = threadIdx.x; i < 65; i += kBlock) {
+    s_sq[i] = sqrt(double(i));
+    s_rsq[i] = size_rsq[i];
+  }
+  for (int i = threadIdx.x; i < 4 * a.n_off; i += kBlock) s_meta[i] = a.meta[4 * a.n_diag + i];
+  for (int i = threadIdx.x; i < 2 * a.ncells; i += kBlock) s_cells[i] = a.dcells[i];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= a.n) return;
+  const uint32_t r = a.rstates[row];
+  double sq_r = 1;
+  if (SYM) sq_r = s_sq[a.orbits[row]];
+  double acc[B];
+#pragma unroll
+  for (int b = 0; b < B; ++b) acc[b] = on[b] ? sb.diag[b][row] * double(x[size_t(row) * B + b]) : 0.0;
+  const int nmax = a.max_sites;
+  constexpr int T = 4;
+  for (int w0 = 0; w0 < a.n_off; w0 += 32) {
+    uint32_t act = 0;
+    const int gend = min(32, a.n_off - w0);
+    for (int gi = 0; gi < gend; ++gi) {
+      const int g = w0 + gi;
+      const uint32_t packed = __builtin_amdgcn_readfirstlane(s_meta[4 * g + 1]);
+      const uint32_t nz = __builtin_amdgcn_readfirstlane(s_meta[4 * g + 2]);
+      act |= ((nz >> pattern_upto(r, packed, nmax)) & 1u) << gi;
+    }
+    while (act) {
+      uint32_t tgt[T], key[T];
+      bool in[T];
+      int offs[T], cis[T], sgn[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        in[t] = false;
+        tgt[t] = r;
+        offs[t] = 0;
+        if (act) {
+          const int g = w0 + __ffs(act) - 1;
+          act &= act - 1;
+          tgt[t] = r ^ s_meta[4 * g];
+          offs[t] = int(s_meta[4 * g + 3]) + pattern_upto(r, s_meta[4 * g + 1], nmax) - off_base;
+          in[t] = true;
+        }
+      }
+      int orb[T];
+      if constexpr (SYM != 0) {
+        walk_batch<SYM, false, T>(tgt, a, key, orb, cis, sgn);
+      } else {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          key[t] = tgt[t];
+          orb[t] = 1;
+          sgn[t] = 1;
+        }
+      }
+      int idx[T];
+      dir_find<T>(key, in, s_cells, a, idx);
+      // the T gathered lines are loaded before any is used (an unreachable
+      // target reads the row's own line and adds nothing)
+      SX xv[T][B];
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const size_t at = idx[t] >= 0 ? size_t(idx[t]) : size_t(row);
+#pragma unroll
+        for (int b = 0; b < B; ++b) xv[t][b] = x[at * B + b];
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        if (idx[t] < 0) continue;
+        double scale = sgn[t] < 0 ? -1.0 : 1.0;
+        if (SYM) scale *= sq_r * s_rsq[orb[t]];
+#pragma unroll
+        for (int b = 0; b < B; ++b) acc[b] += s_amp[b * na + offs[t]] * scale * double(xv[t][b]);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < B; ++b)
+    if (on[b]) sb.y[b][row] = acc[b];
+}
+
 // The same action with lanes of both precisions in one launch: lanes
 // b < W32 read the FP32 interleaved input x32[row·W32 + b], lanes
 // W32 <= b < W32 + W64 the FP64 input x64[row·W64 + (b − W32)]. Windows of a
@@ -1995,6 +2180,11 @@ struct Operator::DeviceTables {
   bool use_img = false;
   DeviceArray<uint32_t> img, hop, binom;
   DeviceArray<double> code_rsq;
+  // 1/sqrt(i) for orbit sizes i < 65, computed on the host as the orbit
+  // codes' factors are (the on-the-fly scan action), and whether the scan
+  // action runs on the fly (QMBED_SCAN_HV=walk) rather than on the stream
+  DeviceArray<double> size_rsq;
+  int scan_walk = -1;
   int img_bits = 0;
   // compiled index stream (autotuned against both)
   bool use_stream = false;
@@ -3073,6 +3263,75 @@ void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, do
   QMBED_HIP(hipEventRecord(t.staged, s));
 }
 
+// Whether the batched scan action runs on the fly (orbit walk + state
+// directory per target) instead of reading the compiled stream: asked for
+// with QMBED_SCAN_HV=walk, for sectors of built-in symmetries, ascending
+// rows and 32-bit states without character phases.
+static bool scan_on_the_fly(const Operator& op) {
+  auto& t = *op.tables_;
+  if (t.scan_walk < 0) {
+    const char* m = std::getenv("QMBED_SCAN_HV");
+    const SpinBasis1D& b = op.basis();
+    t.scan_walk = m && std::string(m) == "walk" && !b.general() && !b.reversed() && !b.wide() && !phased(b) ? 1 : 0;
+  }
+  return t.scan_walk == 1;
+}
+
+static void launch_scan_walk(const Operator& op, const void* x, bool fp32, int width, const ScanBatch& sb,
+                             hipStream_t s) {
+  auto& t = *op.tables_;
+  const SpinBasis1D& b = op.basis();
+  HvArgs a = t.args(b);
+  a.img = nullptr;
+  a.stream = nullptr;
+  if (a.n == 0) return;
+  if (t.size_rsq.size() == 0) {
+    std::vector<double> h(65, 0.0);
+    for (int i = 1; i < 65; ++i) h[size_t(i)] = 1.0 / std::sqrt(double(i));
+    t.size_rsq.resize(65);
+    QMBED_HIP(hipMemcpy(t.size_rsq.data(), h.data(), 65 * 8, hipMemcpyHostToDevice));
+  }
+  int off_base = 0;
+  for (auto& grp : op.diagonal_groups()) off_base += int(grp.amp.size());
+  const size_t sm = sizeof(double) * (size_t(width) * size_t(a.n_amp - off_base) + 130) +
+                    sizeof(uint32_t) * (4 * size_t(a.n_off) + 2 * size_t(a.ncells));
+  if (sm > 64 * 1024) throw internal_state("on-the-fly scan action beyond the LDS of its tables");
+  const dim3 grid(unsigned((size_t(a.n) + kBlock - 1) / kBlock)), block(kBlock);
+  const int sym = kernel_sym(b);
+  char kname[64];
+  std::snprintf(kname, sizeof kname, "hv_scan_walk_kernel<%s, %d, %d>", fp32 ? "float" : "double", width,
+                sym < 0 || sym > 3 ? 3 : sym);
+  const int timed = launch_timer_begin(kname, s);
+  const double* rsq = t.size_rsq.data();
+#define QMBED_WALK_LAUNCH(SX, W)                                                                                \
+  switch (sym) {                                                                                               \
+    case 0: hipLaunchKernelGGL((hv_scan_walk_kernel<SX, W, 0>), grid, block, sm, s, a, sb, static_cast<const SX*>(x), off_base, rsq); break; \
+    case 1: hipLaunchKernelGGL((hv_scan_walk_kernel<SX, W, 1>), grid, block, sm, s, a, sb, static_cast<const SX*>(x), off_base, rsq); break; \
+    case 2: hipLaunchKernelGGL((hv_scan_walk_kernel<SX, W, 2>), grid, block, sm, s, a, sb, static_cast<const SX*>(x), off_base, rsq); break; \
+    default: hipLaunchKernelGGL((hv_scan_walk_kernel<SX, W, 3>), grid, block, sm, s, a, sb, static_cast<const SX*>(x), off_base, rsq); break; \
+  }
+  if (fp32) {
+    switch (width) {
+      case 1: QMBED_WALK_LAUNCH(float, 1) break;
+      case 2: QMBED_WALK_LAUNCH(float, 2) break;
+      case 4: QMBED_WALK_LAUNCH(float, 4) break;
+      case 8: QMBED_WALK_LAUNCH(float, 8) break;
+      default: QMBED_WALK_LAUNCH(float, 16) break;
+    }
+  } else {
+    switch (width) {
+      case 1: QMBED_WALK_LAUNCH(double, 1) break;
+      case 2: QMBED_WALK_LAUNCH(double, 2) break;
+      case 4: QMBED_WALK_LAUNCH(double, 4) break;
+      case 8: QMBED_WALK_LAUNCH(double, 8) break;
+      default: QMBED_WALK_LAUNCH(double, 16) break;
+    }
+  }
+#undef QMBED_WALK_LAUNCH
+  QMBED_HIP(hipGetLastError());
+  launch_timer_end(timed, s);
+}
+
 void Operator::apply_scan_batch(const void* x, bool fp32, int width, const double* const* amps,
                                 const double* const* diags, double* const* y, const int* const* gates,
                                 hipStream_t s) const {
@@ -3080,6 +3339,17 @@ void Operator::apply_scan_batch(const void* x, bool fp32, int width, const doubl
   if (!(width == 1 || width == 2 || width == 4 || width == 8 || width == 16) || width > kMaxScanBatch)
     throw internal_state("batched scan width must be 1, 2, 4, 8 or 16");
   if (width > scan_batch_max_width()) throw internal_state("batched scan width beyond the LDS of its tables");
+  if (scan_on_the_fly(*this)) {
+    ScanBatch sb{};
+    for (int b = 0; b < width; ++b) {
+      sb.amp[b] = amps[b];
+      sb.diag[b] = diags[b];
+      sb.y[b] = y[b];
+      sb.gate[b] = gates ? gates[b] : nullptr;
+    }
+    launch_scan_walk(*this, x, fp32, width, sb, s);
+    return;
+  }
   if (!t.compact || t.stream.size() == 0) throw internal_state("batched scan action without a compact stream");
   HvArgs a = t.args(*basis_);
   a.stream = t.stream.data();
@@ -3139,6 +3409,13 @@ void Operator::apply_scan_mixed(const float* x32, int w32, const double* x64, in
     throw internal_state("mixed scan widths must be 0, 1, 2, 4, 8 or 16 (FP32) and 0, 1, 2, 4 or 8 (FP64)");
   if (w64 == 0) return apply_scan_batch(x32, true, w32, amps, diags, y, gates, s);
   if (w32 == 0) return apply_scan_batch(x64, false, w64, amps, diags, y, gates, s);
+  if (scan_on_the_fly(*this)) {
+    // one launch per precision (each lane's terms and order are those of
+    // the mixed launch)
+    apply_scan_batch(x32, true, w32, amps, diags, y, gates, s);
+    apply_scan_batch(x64, false, w64, amps + w32, diags + w32, y + w32, gates ? gates + w32 : nullptr, s);
+    return;
+  }
   const int B = w32 + w64;
   if (B > kMaxScanBatch) throw internal_state("mixed scan launch of more lanes than a batch holds");
   if (sizeof(double) * (size_t(B) * size_t(t.n_amp) + 81) > 64 * 1024)

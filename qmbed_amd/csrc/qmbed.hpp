@@ -231,13 +231,12 @@ class SpinBasis1D {
   // min(L, kDirCellBits) bits of a value name its cell; cell c splits its
   // value range into 2^(cb − s_c) sub-buckets of 2^s_c values, s_c the
   // largest shift with at most kDirBucket states in every sub-bucket.
-  // cells() holds per cell (first directory entry, s_c); directory()[j]
-  // the index of the first state of sub-bucket j. A lookup reads the
-  // kDirWindow states from that index rounded down to a 16-byte boundary,
-  // which cover the sub-bucket (the state table ends in an all-ones line).
+  // cells() holds per cell (first slot, s_c); directory() one aligned
+  // 16-byte slot per sub-bucket: the index of its first state and its
+  // states (ascending, padded with all ones), so one 16-byte read answers a
+  // lookup.
   static constexpr int kDirCellBits = 11;
-  static constexpr int kDirBucket = 8;
-  static constexpr int kDirWindow = 12;
+  static constexpr int kDirBucket = 3;
   const uint32_t* directory_cells() const { return dir_cells_.data(); }
   const uint32_t* directory() const { return dir_.data(); }
   int directory_shift() const { return dir_shift_; }  // cb = L − cell bits
