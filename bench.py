@@ -359,6 +359,9 @@ def main():
                     help="the headline start (the other is measured beside it)")
     ap.add_argument("--targets", choices=("correction", "projections", "continuation"), default="correction",
                     help="what the network learns")
+    ap.add_argument("--scan-hv", choices=("stream", "walk"), default="stream",
+                    help="form of the batched scan H·v: the compiled index stream, or on the fly (orbit walk and "
+                         "state directory per target); both give the same results bit for bit")
     ap.add_argument("--details", default=None,
                     help="write the per-point iterations, times and energy differences of every mode to this JSON file")
     ap.add_argument("--hv-profile", action="store_true",
@@ -400,6 +403,8 @@ def main():
         return
 
     # ---- setup: family, training scan, network ----
+    if args.scan_hv == "walk":
+        os.environ["QMBED_SCAN_HV"] = "walk"
     static, comps = ising_family(L_SCAN)
     t0 = time.perf_counter()
     model = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
@@ -480,11 +485,12 @@ def main():
     hv64 = hv_roofline(model, np.float64, 8)
 
     # roofline: the 16-lane FP32 batched scan H·v, the headline's dominant kernel
-    key16 = [k for k in launches if k.startswith("hv_scan_batch_kernel<float, 16,")]
+    scan_kernel = "hv_scan_walk_kernel" if args.scan_hv == "walk" else "hv_scan_batch_kernel"
+    key16 = [k for k in launches if k.startswith(scan_kernel + "<float, 16,")]
     rk = key16[0] if key16 else max(launches, key=lambda k: launches[k]["ms_total"]) if launches else None
     if rk is not None:
         width = int(rk.split(",")[1])
-        fp32k = rk.startswith("hv_scan_batch_kernel<float")
+        fp32k = "<float" in rk
         algo16 = scan_bytes_per_launch(model.dim, nnz, width, fp32k)
         ach = algo16 / (launches[rk]["ms_mean"] * 1e-3) / 1e9
     else:
@@ -492,7 +498,8 @@ def main():
     # the instrumented kernels of the timed region with the largest totals:
     # [launches, mean ms, total ms, GB/s of algorithmic bytes where stated]
     top = sorted(launches, key=lambda k: -launches[k]["ms_total"])[:6]
-    kernel_share = {k.replace("hv_scan_batch_kernel", "scan").replace("hv_scan_mixed_kernel", "scan_mixed"):
+    kernel_share = {k.replace("hv_scan_batch_kernel", "scan").replace("hv_scan_mixed_kernel", "scan_mixed")
+                    .replace("hv_scan_walk_kernel", "scan_walk"):
                     [launches[k]["launches"], sig(launches[k]["ms_mean"]), sig(launches[k]["ms_total"]),
                      sig(launches[k]["GBs"]) if launches[k]["bytes_total"] else None] for k in top}
     # the same kernel on its own (20 launches at 16 fixed points of the scan grid)
@@ -501,6 +508,24 @@ def main():
         ms = model.time_scan_batch(bench_pts[:w], fp32=True, iters=20)
         b = scan_bytes_per_launch(model.dim, nnz, w, True)
         alone[f"w{w}"] = [sig(ms), sig(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)]
+    # the other form of the same action (on the fly: orbit walk and state
+    # directory per target instead of the compiled stream; same results bit
+    # for bit), on a second instance of the family whose first batched
+    # launch fixes the form
+    other = "stream" if args.scan_hv == "walk" else "walk"
+    prev = os.environ.get("QMBED_SCAN_HV")
+    os.environ["QMBED_SCAN_HV"] = other
+    twin = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
+                    components=comps)
+    for w in (16, 8):
+        ms = twin.time_scan_batch(bench_pts[:w], fp32=True, iters=20)
+        b = scan_bytes_per_launch(model.dim, nnz, w, True)
+        alone[f"{other}_w{w}"] = [sig(ms), sig(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)]
+    twin.close()
+    if prev is None:
+        del os.environ["QMBED_SCAN_HV"]
+    else:
+        os.environ["QMBED_SCAN_HV"] = prev
 
     def hv_short(h):
         return [sig(h["ms"]), sig(h["frac"], 3)]
@@ -579,7 +604,9 @@ def main():
                                f"nnz={nnz}), 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT[args.start]}; FP32-matvec "
                                "schedule; ground state to residual <= 1e-8*max(|E|,1); one step = 16 held-out "
                                "points; the timed points solved in one call, 16 Lanczos windows in lockstep batches "
-                               "of 16 points; baseline plain FP64, one eigsh per point",
+                               "of 16 points; baseline plain FP64, one eigsh per point; batched H·v "
+                               + ("on the fly (orbit walk + state directory per target)" if args.scan_hv == "walk"
+                                  else "from the compiled index stream"),
                    "global_batch": BATCH},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS if ach else None,

@@ -147,22 +147,34 @@ __device__ __forceinline__ int pattern_of(uint32_t r, uint32_t packed) {
 // keys are loaded before any is used, unconditionally: every key below 2^L
 // names a slot, and in[b] enters only the result.
 template <int B>
-__device__ __forceinline__ void dir_find(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* cells,
-                                         const HvArgs& a, int (&idx)[B]) {
+__device__ __forceinline__ void dir_issue(const uint32_t (&key)[B], const uint32_t* cells, const HvArgs& a,
+                                          uint4 (&w)[B]) {
   static_assert(SpinBasis1D::kDirBucket == 3, "a slot holds 3 states");
   const uint32_t lmask = (1u << a.dcb) - 1u;
-  uint4 w[B];
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const uint2 d = reinterpret_cast<const uint2*>(cells)[key[b] >> a.dcb];
     w[b] = __ldg(reinterpret_cast<const uint4*>(a.dir) + (d.x + ((key[b] & lmask) >> d.y)));
   }
+}
+
+template <int B>
+__device__ __forceinline__ void dir_resolve(const uint32_t (&key)[B], const bool (&in)[B], const uint4 (&w)[B],
+                                            int (&idx)[B]) {
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const uint32_t k = key[b];
     const int pos = w[b].y == k ? 0 : w[b].z == k ? 1 : w[b].w == k ? 2 : -1;
     idx[b] = in[b] && pos >= 0 ? int(w[b].x) + pos : -1;
   }
+}
+
+template <int B>
+__device__ __forceinline__ void dir_find(const uint32_t (&key)[B], const bool (&in)[B], const uint32_t* cells,
+                                         const HvArgs& a, int (&idx)[B]) {
+  uint4 w[B];
+  dir_issue<B>(key, cells, a, w);
+  dir_resolve<B>(key, in, w, idx);
 }
 
 // dir_find for one key, with the cell table in global memory
@@ -665,10 +677,12 @@ __global__ void __launch_bounds__(kBlock) hv_kernel(HvArgs a, const S* __restric
     }
     act &= a.lane_groups << sub;
     // Batched as in the image mode: the walks of up to B targets (vector
-    // ALU), then their searches level by level, then the B input entries,
-    // then the B products in target order (the order of the one-at-a-time
-    // loop, so the sums are the same).
-    constexpr int B = NV > 1 ? 4 : 8;
+    // ALU, rotation loops shared), then their directory slots, then the B
+    // input entries, then the B products in target order (the order of the
+    // one-at-a-time loop, so the sums are the same). Four targets keep the
+    // kernel under 64 registers (eight waves per SIMD), which measured
+    // faster than eight targets at five waves.
+    constexpr int B = 4;
     while (act) {
       uint32_t tgt[B], key[B];
       bool in[B];
@@ -1841,7 +1855,25 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
 #pragma unroll
   for (int b = 0; b < B; ++b) acc[b] = on[b] ? sb.diag[b][row] * double(x[size_t(row) * B + b]) : 0.0;
   const int nmax = a.max_sites;
-  constexpr int T = 4;
+  // Software-pipelined over batches of T targets: while the gathers of one
+  // batch are in flight, the next batch is walked and its slots are
+  // requested; then the pending batch is accumulated (in batch order, so
+  // the sums are those of the one-batch-at-a-time loop), the new slots are
+  // resolved and the new gathers issued.
+  constexpr int T = 2;
+  SX xv[T][B];
+  int pidx[T], poffs[T], psgn[T], porb[T];
+  bool pending = false;
+  auto accumulate = [&]() {
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      if (pidx[t] < 0) continue;
+      double scale = psgn[t] < 0 ? -1.0 : 1.0;
+      if (SYM) scale *= sq_r * s_rsq[porb[t]];
+#pragma unroll
+      for (int b = 0; b < B; ++b) acc[b] += s_amp[b * na + poffs[t]] * scale * double(xv[t][b]);
+    }
+  };
   for (int w0 = 0; w0 < a.n_off; w0 += 32) {
     uint32_t act = 0;
     const int gend = min(32, a.n_off - w0);
@@ -1854,7 +1886,7 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
     while (act) {
       uint32_t tgt[T], key[T];
       bool in[T];
-      int offs[T], cis[T], sgn[T];
+      int offs[T], cis[T], sgn[T], orb[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
         in[t] = false;
@@ -1868,7 +1900,6 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
           in[t] = true;
         }
       }
-      int orb[T];
       if constexpr (SYM != 0) {
         walk_batch<SYM, false, T>(tgt, a, key, orb, cis, sgn);
       } else {
@@ -1879,27 +1910,25 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
           sgn[t] = 1;
         }
       }
-      int idx[T];
-      dir_find<T>(key, in, s_cells, a, idx);
-      // the T gathered lines are loaded before any is used (an unreachable
-      // target reads the row's own line and adds nothing)
-      SX xv[T][B];
+      uint4 slot[T];
+      dir_issue<T>(key, s_cells, a, slot);
+      if (pending) accumulate();
+      dir_resolve<T>(key, in, slot, pidx);
+      // the gathered lines (an unreachable target reads the row's own line
+      // and adds nothing)
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        const size_t at = idx[t] >= 0 ? size_t(idx[t]) : size_t(row);
+        const size_t at = pidx[t] >= 0 ? size_t(pidx[t]) : size_t(row);
 #pragma unroll
         for (int b = 0; b < B; ++b) xv[t][b] = x[at * B + b];
+        poffs[t] = offs[t];
+        psgn[t] = sgn[t];
+        porb[t] = orb[t];
       }
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        if (idx[t] < 0) continue;
-        double scale = sgn[t] < 0 ? -1.0 : 1.0;
-        if (SYM) scale *= sq_r * s_rsq[orb[t]];
-#pragma unroll
-        for (int b = 0; b < B; ++b) acc[b] += s_amp[b * na + offs[t]] * scale * double(xv[t][b]);
-      }
+      pending = true;
     }
   }
+  if (pending) accumulate();
 #pragma unroll
   for (int b = 0; b < B; ++b)
     if (on[b]) sb.y[b][row] = acc[b];
