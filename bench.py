@@ -173,9 +173,9 @@ def hv_profile(qa):
     """Kernels for counter runs: the FP32 and FP64 H·v of the headline sector
     at hx = 1, hz = 0.25 (20 timed launches each after the autotuner's own),
     the 16-lane and 8-lane FP32 batched scan H·v at held-out grid points (2
-    untimed and 20 timed launches each), and 5 device-to-device copies of
-    256 MiB, whose counter readings calibrate the bytes the H·v counters
-    report."""
+    untimed and 20 timed launches each, in the form --scan-hv names), and 5
+    device-to-device copies of 256 MiB, whose counter readings calibrate the
+    bytes the H·v counters report."""
     static, comps = ising_family(L_SCAN)
     model = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
                      components=comps)
@@ -383,6 +383,8 @@ def main():
 
     import qmbed_amd as qa
 
+    if args.scan_hv == "walk":
+        os.environ["QMBED_SCAN_HV"] = "walk"
     if args.hv_profile:
         hv_profile(qa)
         return
@@ -403,8 +405,6 @@ def main():
         return
 
     # ---- setup: family, training scan, network ----
-    if args.scan_hv == "walk":
-        os.environ["QMBED_SCAN_HV"] = "walk"
     static, comps = ising_family(L_SCAN)
     t0 = time.perf_counter()
     model = qa.Model({"kind": "spin", "sites": L_SCAN, "pauli": True, "momentum": 0, "parity": 1}, static,
