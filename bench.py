@@ -72,12 +72,11 @@ BATCH = 16
 # components of the training ground states.
 STARTS = {"network": True, "snapshot": "rayleigh_ritz"}
 HEADLINE_TEXT = {
-    "network": "learned start v0 = S (y(c) + MLP(c)): S a 32-dimensional snapshot space (the leading principal "
-               "components of 256 plain-Lanczos training ground states), y(c) the lowest eigenvector of the family "
-               "projected on S, MLP the 2-64-64-32 FP32 network trained on the GPU to map (hx, hz) to the step from "
-               "y(c) to the normalized projection of the ground state on S",
-    "snapshot": "snapshot start v0 = S y(c): S a 32-dimensional snapshot space (the leading principal components of "
-                "256 plain-Lanczos training ground states), y(c) the lowest eigenvector of the family projected on S",
+    "network": "learned start v0 = S(y(c)+MLP(c)) (S: 32 principal components of 256 plain-Lanczos training ground "
+               "states; y(c): Rayleigh-Ritz on S; MLP: 2-64-64-32 FP32, trained on the GPU on the step from y(c) to "
+               "the ground state's projection on S)",
+    "snapshot": "snapshot start v0 = S y(c) (S: 32 principal components of 256 plain-Lanczos training ground states; "
+                "y(c): Rayleigh-Ritz on S)",
 }
 EPOCHS = 4000
 SPACE = "principal"
@@ -536,7 +535,9 @@ def main():
             "fallbacks": fallbacks,
             "setup_s": [sig(build_s, 3), sig(train_s, 3), sig(rep["solve_seconds"], 3), sig(rep["train_seconds"], 3)],
             "loss": [sig(rep["loss_first"], 3), sig(rep["loss_last"], 3)],
-            "modes": summary,
+            # per mode [solves/s, mean iterations, max iterations, mean FP32 iterations, largest relative
+            # energy difference to plain FP64]
+            "modes": {k: [v["sps"], v["it"], v["it_max"], v.get("it32"), v.get("dE")] for k, v in summary.items()},
             "vs_plain_fp64": [sig(value / plain_rate), sig(summary["plain_fp64"]["it"] / summary["headline"]["it"])],
             "full_scan_incl_setup": [sig(amort_warm), sig(amort_plain), sig(amort_warm / amort_plain)],
         },
@@ -569,7 +570,9 @@ def main():
         j["fp32_rel_err"] = abs(e["fp32"] - e["fp64"]) / abs(e["fp64"])
         extras["j1j2_L28"] = j
         big.close()
-        extras["hv_modes_fp64"] = hv_modes(qa)
+        # per sector and mode [ms, fraction of peak] (the lanes per row are in --hv-modes)
+        extras["hv_modes_fp64"] = {c: {m: v[:2] if isinstance(v, list) else v for m, v in row.items()}
+                                   for c, row in hv_modes(qa).items()}
         sb = sector_batch(qa)["c4_sectors72"]
         extras["c4_sectors72"] = {f"k{k}": [sig(sb[f"eigenpairs_{k}"]["batch_s"], 3),
                                             sig(sb[f"eigenpairs_{k}"]["one_by_one_s"], 3),
@@ -600,12 +603,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 matvec, f64 Lanczos/reorth/polish",
         "data": "synthetic (Hamiltonian couplings of the scan grid; seeded training and benchmark points)",
-        "config": {"workload": f"Ising chain -ZZ - hx X - hz Z (Pauli), L=26 periodic, k=0, p=+1 (N={model.dim}, "
-                               f"nnz={nnz}), 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT[args.start]}; FP32-matvec "
-                               "schedule; ground state to residual <= 1e-8*max(|E|,1); one step = 16 held-out "
-                               "points; the timed points solved in one call, 16 Lanczos windows in lockstep batches "
-                               "of 16 points; baseline plain FP64, one eigsh per point; batched H·v "
-                               + ("on the fly (orbit walk + state directory per target)" if args.scan_hv == "walk"
+        "config": {"workload": f"Ising chain -ZZ-hx X-hz Z (Pauli), L=26 periodic, k=0, p=+1 (N={model.dim}, "
+                               f"nnz={nnz}); 64x64 grid hx in [0.2,2], hz in [0,0.5]; {HEADLINE_TEXT[args.start]}; "
+                               "FP32-matvec schedule; residual <= 1e-8*max(|E|,1); step = 16 held-out points, 16 "
+                               "Lanczos windows in lockstep; baseline plain FP64, one eigsh per point; batched Hv "
+                               + ("on the fly (orbit walk + state directory)" if args.scan_hv == "walk"
                                   else "from the compiled index stream"),
                    "global_batch": BATCH},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
