@@ -1805,6 +1805,9 @@ __global__ void __launch_bounds__(kBlock) hv_scan_batch_kernel(HvArgs a, ScanBat
     if (on[b]) sb.y[b][row] = acc[b];
 }
 
+// Lanes per row of the on-the-fly scan action for B points.
+constexpr int scan_walk_lanes(int b) { return b >= 2 ? 2 : 1; }
+
 // The batched scan action of hv_scan_batch_kernel computed on the fly: per
 // target the orbit walk and the state-directory lookup of the walk-mode
 // hv_kernel, done once for the B points (T targets of a lane at a time, the
@@ -1846,32 +1849,64 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
   for (int i = threadIdx.x; i < 4 * a.n_off; i += kBlock) s_meta[i] = a.meta[4 * a.n_diag + i];
   for (int i = threadIdx.x; i < 2 * a.ncells; i += kBlock) s_cells[i] = a.dcells[i];
   __syncthreads();
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= a.n) return;
+  // P lanes per row (scan_walk_lanes): the lanes of a row take its targets
+  // in turn for the walk and the lookup and share the results with lane
+  // shuffles; each then gathers and accumulates its share of the points (BH
+  // of them) for every target of the row, in the row's target order. Each
+  // point's sum is thus the one-lane sum, term for term.
+  constexpr int P = scan_walk_lanes(B);
+  constexpr int BH = B / P;
+  const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t row = gid / P;
+  const int half = int(gid % P);
+  if (row >= a.n) return;  // the lanes of a row leave together
+  const int p0 = half * BH;
+  const int lane0 = int(threadIdx.x & 63u) & ~(P - 1);  // the row's first lane in the wavefront
+  bool onh[BH];
+  const double* dg[BH];
+  double* yp[BH];
+#pragma unroll
+  for (int q = 0; q < BH; ++q) {
+    onh[q] = on[q];
+    dg[q] = sb.diag[q];
+    yp[q] = sb.y[q];
+#pragma unroll
+    for (int h = 1; h < P; ++h)
+      if (half == h) {
+        onh[q] = on[h * BH + q];
+        dg[q] = sb.diag[h * BH + q];
+        yp[q] = sb.y[h * BH + q];
+      }
+  }
   const uint32_t r = a.rstates[row];
   double sq_r = 1;
   if (SYM) sq_r = s_sq[a.orbits[row]];
-  double acc[B];
+  double acc[BH];
 #pragma unroll
-  for (int b = 0; b < B; ++b) acc[b] = on[b] ? sb.diag[b][row] * double(x[size_t(row) * B + b]) : 0.0;
+  for (int q = 0; q < BH; ++q) acc[q] = onh[q] ? dg[q][row] * double(x[size_t(row) * B + p0 + q]) : 0.0;
   const int nmax = a.max_sites;
-  // Software-pipelined over batches of T targets: while the gathers of one
-  // batch are in flight, the next batch is walked and its slots are
-  // requested; then the pending batch is accumulated (in batch order, so
-  // the sums are those of the one-batch-at-a-time loop), the new slots are
-  // resolved and the new gathers issued.
-  constexpr int T = 2;
-  SX xv[T][B];
-  int pidx[T], poffs[T], psgn[T], porb[T];
+  // Software-pipelined over batches (T targets walked per lane, P·T per
+  // row; lane pairs with one target each measured fastest, against two
+  // targets or four lanes, and a single lane with two): while the gathers
+  // of one batch are in flight, the next batch is walked and its slots are
+  // requested; then the pending batch is accumulated (in target order), the
+  // new slots are resolved, swapped and their gathers issued. Per target the
+  // swap carries the index and one word: amplitude entry | orbit size << 20
+  // | reflection sign << 31.
+  constexpr int T = P > 1 ? 1 : 2;
+  constexpr int TT = P * T;
+  SX xv[TT][BH];
+  int pidx[TT], pmeta[TT];
   bool pending = false;
   auto accumulate = [&]() {
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-      if (pidx[t] < 0) continue;
-      double scale = psgn[t] < 0 ? -1.0 : 1.0;
-      if (SYM) scale *= sq_r * s_rsq[porb[t]];
+    for (int u = 0; u < TT; ++u) {
+      if (pidx[u] < 0) continue;
+      const int offs = pmeta[u] & 0xFFFFF;
+      double scale = pmeta[u] < 0 ? -1.0 : 1.0;
+      if (SYM) scale *= sq_r * s_rsq[(pmeta[u] >> 20) & 127];
 #pragma unroll
-      for (int b = 0; b < B; ++b) acc[b] += s_amp[b * na + poffs[t]] * scale * double(xv[t][b]);
+      for (int q = 0; q < BH; ++q) acc[q] += s_amp[(p0 + q) * na + offs] * scale * double(xv[u][q]);
     }
   };
   for (int w0 = 0; w0 < a.n_off; w0 += 32) {
@@ -1889,15 +1924,23 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
       int offs[T], cis[T], sgn[T], orb[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        in[t] = false;
+        // the pair's next P targets in order; this lane walks number `half`
+        int sel = -1;
+#pragma unroll
+        for (int h = 0; h < P; ++h) {
+          if (act) {
+            const int gi = __ffs(act) - 1;
+            act &= act - 1;
+            if (h == half) sel = gi;
+          }
+        }
+        in[t] = sel >= 0;
         tgt[t] = r;
         offs[t] = 0;
-        if (act) {
-          const int g = w0 + __ffs(act) - 1;
-          act &= act - 1;
+        if (sel >= 0) {
+          const int g = w0 + sel;
           tgt[t] = r ^ s_meta[4 * g];
           offs[t] = int(s_meta[4 * g + 3]) + pattern_upto(r, s_meta[4 * g + 1], nmax) - off_base;
-          in[t] = true;
         }
       }
       if constexpr (SYM != 0) {
@@ -1913,25 +1956,32 @@ __global__ void __launch_bounds__(kBlock) hv_scan_walk_kernel(HvArgs a, ScanBatc
       uint4 slot[T];
       dir_issue<T>(key, s_cells, a, slot);
       if (pending) accumulate();
-      dir_resolve<T>(key, in, slot, pidx);
-      // the gathered lines (an unreachable target reads the row's own line
-      // and adds nothing)
+      int idx[T];
+      dir_resolve<T>(key, in, slot, idx);
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        const size_t at = pidx[t] >= 0 ? size_t(pidx[t]) : size_t(row);
+        const int m = offs[t] | (orb[t] << 20) | (sgn[t] < 0 ? int(0x80000000u) : 0);
 #pragma unroll
-        for (int b = 0; b < B; ++b) xv[t][b] = x[at * B + b];
-        poffs[t] = offs[t];
-        psgn[t] = sgn[t];
-        porb[t] = orb[t];
+        for (int h = 0; h < P; ++h) {
+          pidx[P * t + h] = P > 1 ? __shfl(idx[t], lane0 + h, 64) : idx[t];
+          pmeta[P * t + h] = P > 1 ? __shfl(m, lane0 + h, 64) : m;
+        }
+      }
+      // the gathered halves of the lines (an unreachable target reads the
+      // row's own line and adds nothing)
+#pragma unroll
+      for (int u = 0; u < TT; ++u) {
+        const size_t at = pidx[u] >= 0 ? size_t(pidx[u]) : size_t(row);
+#pragma unroll
+        for (int q = 0; q < BH; ++q) xv[u][q] = x[at * B + p0 + q];
       }
       pending = true;
     }
   }
   if (pending) accumulate();
 #pragma unroll
-  for (int b = 0; b < B; ++b)
-    if (on[b]) sb.y[b][row] = acc[b];
+  for (int q = 0; q < BH; ++q)
+    if (onh[q]) yp[q][row] = acc[q];
 }
 
 // The same action with lanes of both precisions in one launch: lanes
@@ -3292,16 +3342,35 @@ void Operator::scan_batch_tables(const std::vector<std::vector<cplx>>& coefs, do
   QMBED_HIP(hipEventRecord(t.staged, s));
 }
 
+// LDS bytes of an on-the-fly scan launch of `width` lanes: the lanes'
+// off-diagonal amplitude tables, sqrt(i) and 1/sqrt(i), the off-diagonal
+// group words and the directory cells.
+static size_t scan_walk_smem(const Operator& op, int width, int off_base) {
+  const HvArgs a = op.tables_->args(op.basis());
+  return sizeof(double) * (size_t(width) * size_t(a.n_amp - off_base) + 130) +
+         sizeof(uint32_t) * (4 * size_t(a.n_off) + 2 * size_t(a.ncells));
+}
+
+static int diagonal_amp_count(const Operator& op) {
+  int off_base = 0;
+  for (auto& grp : op.diagonal_groups()) off_base += int(grp.amp.size());
+  return off_base;
+}
+
 // Whether the batched scan action runs on the fly (orbit walk + state
 // directory per target) instead of reading the compiled stream: asked for
 // with QMBED_SCAN_HV=walk, for sectors of built-in symmetries, ascending
-// rows and 32-bit states without character phases.
+// rows and 32-bit states without character phases whose 16-lane launch fits
+// the LDS (otherwise the stream form runs; both give the same results).
 static bool scan_on_the_fly(const Operator& op) {
   auto& t = *op.tables_;
   if (t.scan_walk < 0) {
     const char* m = std::getenv("QMBED_SCAN_HV");
     const SpinBasis1D& b = op.basis();
-    t.scan_walk = m && std::string(m) == "walk" && !b.general() && !b.reversed() && !b.wide() && !phased(b) ? 1 : 0;
+    t.scan_walk = m && std::string(m) == "walk" && !b.general() && !b.reversed() && !b.wide() && !phased(b) &&
+                          scan_walk_smem(op, 16, diagonal_amp_count(op)) <= 64 * 1024
+                      ? 1
+                      : 0;
   }
   return t.scan_walk == 1;
 }
@@ -3320,12 +3389,12 @@ static void launch_scan_walk(const Operator& op, const void* x, bool fp32, int w
     t.size_rsq.resize(65);
     QMBED_HIP(hipMemcpy(t.size_rsq.data(), h.data(), 65 * 8, hipMemcpyHostToDevice));
   }
-  int off_base = 0;
-  for (auto& grp : op.diagonal_groups()) off_base += int(grp.amp.size());
-  const size_t sm = sizeof(double) * (size_t(width) * size_t(a.n_amp - off_base) + 130) +
-                    sizeof(uint32_t) * (4 * size_t(a.n_off) + 2 * size_t(a.ncells));
+  const int off_base = diagonal_amp_count(op);
+  const size_t sm = scan_walk_smem(op, width, off_base);
   if (sm > 64 * 1024) throw internal_state("on-the-fly scan action beyond the LDS of its tables");
-  const dim3 grid(unsigned((size_t(a.n) + kBlock - 1) / kBlock)), block(kBlock);
+  // the kernel's lanes per row
+  const size_t lanes = size_t(a.n) * size_t(scan_walk_lanes(width));
+  const dim3 grid(unsigned((lanes + kBlock - 1) / kBlock)), block(kBlock);
   const int sym = kernel_sym(b);
   char kname[64];
   std::snprintf(kname, sizeof kname, "hv_scan_walk_kernel<%s, %d, %d>", fp32 ? "float" : "double", width,
