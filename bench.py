@@ -233,6 +233,69 @@ def hv_modes(qa, iters=20):
     return out
 
 
+def f_rows(qa, iters=20):
+    """Timings of the rows next to the path, one GPU:
+    * a wide sector: the Heisenberg ring L=32 up=16 k=0 p=+1 on 64-bit states
+      (split stream words), its build, FP64 and FP32-schedule H·v, and the
+      FP32-schedule ground state;
+    * the same ring's general sector (translation k=0, even reflection, even
+      spin inversion; maximum representatives, descending rows): build, H·v
+      and ground state;
+    * a stored CSR matrix (XXZ L=24 up=12 k=0, assembled on the GPU): its
+      FP64 product beside the matrix-free one;
+    * the shift target on the same sector: the 2 eigenpairs nearest a shift
+      between the 4th and 5th lowest eigenvalues (MINRES inner solves).
+    H·v entries are [ms, fraction of HBM peak by N·(4+s) + nnz·(4+s)]."""
+    out = {}
+
+    def hv(m, dtype, size):
+        h = hv_roofline(m, dtype, size, iters)
+        return [sig(h["ms"]), sig(h["frac"], 3)]
+
+    def ground(m):
+        t0 = time.perf_counter()
+        r = m.eigsh(1, tolerance=1e-10, matvec_precision="fp32")
+        return [sig(time.perf_counter() - t0, 3), r["iterations"], r["eigenvalues"][0]]
+
+    L = 32
+    ring = xxz_terms(L, 1.0)
+    t0 = time.perf_counter()
+    wide = qa.Model({"kind": "spin", "sites": L, "up": L // 2, "momentum": 0, "parity": 1}, ring)
+    build = time.perf_counter() - t0
+    out["wide_L32_k0p1"] = {"N": wide.dim, "nnz": wide.nnz(), "build_s": sig(build, 3),
+                            "hv_fp64": hv(wide, np.float64, 8), "hv_fp32_in": hv(wide, "mixed", 4),
+                            "gs_s_it_E0": ground(wide)}
+    wide.close()
+    maps = [{"destinations": [(i + 1) % L for i in range(L)], "sector": 0},
+            {"destinations": [L - 1 - i for i in range(L)], "sector": 0},
+            {"destinations": list(range(L)), "local_permutations": [[1, 0]] * L, "sector": 0}]
+    t0 = time.perf_counter()
+    gen = qa.Model({"kind": "spin", "sites": L, "up": L // 2, "symmetries": maps, "reverse": True}, ring)
+    build = time.perf_counter() - t0
+    out["general_L32_tpz"] = {"N": gen.dim, "nnz": gen.nnz(), "build_s": sig(build, 3),
+                              "hv_fp64": hv(gen, np.float64, 8), "gs_s_it_E0": ground(gen)}
+    gen.close()
+    c2 = {"kind": "spin", "sites": 24, "up": 12, "momentum": 0}
+    free = qa.Model(c2, xxz_terms(24, 1.5))
+    t0 = time.perf_counter()
+    csr = qa.Model(c2, xxz_terms(24, 1.5), format="csr")
+    build = time.perf_counter() - t0
+    out["csr_xxz_L24"] = {"build_s": sig(build, 3), "hv_fp64": hv(csr, np.float64, 8),
+                          "matrix_free_hv_fp64": hv(free, np.float64, 8)}
+    csr.close()
+    w = free.eigsh(6, tolerance=1e-10)["eigenvalues"]
+    sigma = 0.5 * (w[3] + w[4]) + 0.1 * (w[4] - w[3])
+    free.close()
+    req = {"basis": c2, "terms": qa.terms_from_spec(xxz_terms(24, 1.5)), "format": "matrix_free",
+           "solver": {"eigenpairs": 2, "target": {"kind": "shift", "value": sigma}}}
+    qa.run(req)
+    t0 = time.perf_counter()
+    r = qa.run(req)
+    out["shift_xxz_L24"] = {"sigma": sig(sigma, 6), "s": sig(time.perf_counter() - t0, 3),
+                            "iterations": r["iterations"], "eigenvalues": r["eigenvalues"]}
+    return out
+
+
 def sector_batch(qa):
     """XXZ Δ=1, L=24, up ∈ {11,12,13} × k ∈ 0..23: the 72 sectors' ground
     states (and 6 lowest pairs) in one lockstep batch, beside the same solves
@@ -367,6 +430,9 @@ def main():
                     help="only the headline sector's FP32 and FP64 H·v (20 launches each), 20 launches of the "
                          "16-lane and 8-lane batched scan H·v and 5 device copies of 256 MiB, for hardware-counter "
                          "runs")
+    ap.add_argument("--f-rows", action="store_true",
+                    help="time the rows next to the path (wide and general sectors, stored CSR, shift target) and "
+                         "print them as one JSON object")
     ap.add_argument("--hv-modes", action="store_true",
                     help="only the FP64 H·v of the XXZ L=24, J1-J2 L=28 and Ising L=26 sectors in the walk, image "
                          "and stream modes (20 timed launches each)")
@@ -389,6 +455,9 @@ def main():
         return
     if args.hv_modes:
         print(json.dumps(hv_modes(qa)))
+        return
+    if args.f_rows:
+        print(json.dumps(f_rows(qa)))
         return
     if args.batch_profile:
         L = 24

@@ -104,9 +104,11 @@ enum { QMBED_F64 = 0, QMBED_F32 = 1, QMBED_C128 = 2, QMBED_C64 = 3 };
 enum { QMBED_F32_TO_F64 = 4, QMBED_C64_TO_C128 = 5 };
 
 /* model_json: {"basis":{...},"terms":[...],"components":[...]?,
- * "checks":{...}?} as in create_model (components: [{"name", "terms",
- * "default": [re, im]?}], bindings/capi/src/lib.rs:783-787). Builds the
- * sector on the GPU and compiles the operator.
+ * "checks":{...}?,"format":"csr"|"csc"|"dense"|"dia"|"matrix_free"?} as in
+ * create_model (components: [{"name", "terms", "default": [re, im]?}],
+ * bindings/capi/src/lib.rs:783-787). Builds the sector on the GPU and
+ * compiles the operator; with a stored format the products run on the
+ * stored matrix assembled on the GPU.
  * On failure returns NULL and stores an error string (free with
  * qmbed_string_free) in *error when error is non-NULL. */
 qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error);

@@ -215,13 +215,17 @@ def _terms_json(terms):
 class Model:
     """Device-resident sector + matrix-free operator (typed fast path)."""
 
-    def __init__(self, basis: dict, terms, checks=None, components=None):
+    def __init__(self, basis: dict, terms, checks=None, components=None, format=None):
         """components: [(name, terms, default or None), ...] — a family
-        H = H_static + Σ_p c_p H_p (the reference's create_model components)."""
+        H = H_static + Σ_p c_p H_p (the reference's create_model components).
+        format: a stored format ("csr", "csc", "dense", "dia") whose matrix
+        the products then run on, or None (matrix-free)."""
         L = library()
         req = {"basis": basis, "terms": _terms_json(terms)}
         if checks is not None:
             req["checks"] = checks
+        if format is not None:
+            req["format"] = format
         if components:
             req["components"] = []
             for name, cterms, default in components:

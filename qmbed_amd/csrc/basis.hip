@@ -204,9 +204,26 @@ struct GroupParams {
   const uint8_t* dest;   // [n][64]
   const uint64_t* flip;  // [n]
   const uint8_t* phase;  // [n]: character index of χ(g⁻¹) (0 iff χ(g) = 1)
+  // [n] when every element moves the sites as a rotation or a reflection of
+  // the ring: k (the state rotated left by k) or (k | 0x40: its bit
+  // reversal rotated left by k); null otherwise
+  const uint8_t* rot;
   int n;
   int max_rep;
 };
+
+// s rotated left by k inside L bits (0 <= k < L)
+template <typename ST>
+__device__ __forceinline__ ST rotl_sites(ST s, int k, int L, uint64_t mask) {
+  return k ? ST(((s << k) | (s >> (L - k))) & ST(mask)) : s;
+}
+
+// g(s) for an element of a rotation/reflection group: sr is s's bit reversal
+// inside L bits
+template <typename ST>
+__device__ __forceinline__ ST group_apply_ring(ST s, ST sr, uint8_t code, uint64_t flip, int L, uint64_t mask) {
+  return rotl_sites<ST>((code & 0x40) ? sr : s, code & 63, L, mask) ^ ST(flip);
+}
 
 template <typename ST>
 __device__ __forceinline__ bool in_parent(ST t, const EnumParams& p) {
@@ -220,8 +237,10 @@ __device__ int general_rep_orbit(ST s, const EnumParams& p, const GroupParams& g
   rep = s;
   int stab = 0;
   bool ok = true;
+  const ST sr = gp.rot ? reflect_st(s, p.L) : ST(0);
   for (int g = 0; g < gp.n; ++g) {
-    const ST t = group_apply(s, gp.dest + 64 * g, gp.flip[g], p.L);
+    const ST t = gp.rot ? group_apply_ring<ST>(s, sr, gp.rot[g], gp.flip[g], p.L, p.mask)
+                        : group_apply(s, gp.dest + 64 * g, gp.flip[g], p.L);
     if (t == s) {
       ++stab;
       if (gp.phase[g] != 0) ok = false;
@@ -666,6 +685,28 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
       const long c = (M - G.elems[i].chi) % M;
       phase[i] = uint8_t(std::lower_bound(vals.begin(), vals.end(), c) - vals.begin());
     }
+    // elements that move the sites as a rotation j -> j + k or a reflection
+    // j -> k - j (mod L) of the ring (translations, parities and their
+    // products, with any digit flips) apply as a rotation of the state or of
+    // its bit reversal, a few instructions instead of one step per site
+    std::vector<uint8_t> rot(n, 0);
+    bool ring = true;
+    for (size_t i = 0; i < n && ring; ++i) {
+      const auto& d = G.elems[i].dest;
+      const int k = int(d[0]);
+      bool is_rot = true, is_ref = true;
+      for (int j = 0; j < L; ++j) {
+        if (int(d[size_t(j)]) != (j + k) % L) is_rot = false;
+        if (int(d[size_t(j)]) != ((k - j) % L + L) % L) is_ref = false;
+      }
+      if (is_rot) rot[i] = uint8_t(k);
+      else if (is_ref) rot[i] = uint8_t(((k + 1) % L) | 0x40);
+      else ring = false;
+    }
+    if (ring) {
+      b->grot_.resize(n);
+      QMBED_HIP(hipMemcpyAsync(b->grot_.data(), rot.data(), n, hipMemcpyHostToDevice, stream));
+    }
     b->gdest_.resize(dest.size());
     b->gflip_.resize(n);
     b->gphase_.resize(n);
@@ -678,6 +719,7 @@ std::shared_ptr<SpinBasis1D> SpinBasis1D::build(const SpinBasisSpec& spec_in, hi
     gp.dest = b->gdest_.data();
     gp.flip = b->gflip_.data();
     gp.phase = b->gphase_.data();
+    gp.rot = b->group_rot();
     gp.n = int(n);
     gp.max_rep = spec.reverse ? 1 : 0;
   }

@@ -1105,7 +1105,10 @@ qmbed_hip_model* qmbed_hip_model_create(const char* model_json, char** error) {
     auto tm = new TypedModel();
     try {
       tm->m = make_model(need(v, "basis", "model"), need(v, "terms", "model"), v.get("checks"), v.get("components"));
-      tm->m->materialized();
+      Operator& op = tm->m->materialized();
+      // an optional stored format ("csr", "csc", "dense", "dia"): the typed
+      // products then run on the stored matrix, as the JSON requests do
+      if (const Value* f = v.get("format"); f && !f->is_null()) op.use_format(parse_format(f), tm->m->stream);
     } catch (...) {
       delete tm;
       throw;

@@ -83,9 +83,12 @@ struct HvArgs {
                          // general sector's character values
   int nchar;
   // general symmetry group (ngroup elements; 0: built-in symmetries)
-  const uint8_t* gdest;   // [ngroup][32] site destinations
+  const uint8_t* gdest;   // [ngroup][64] site destinations
   const uint64_t* gflip;  // destination-side flip masks
   const uint8_t* gphase;  // character index of χ(g⁻¹)
+  // per element k or (k | 0x40) for a rotation/reflection group of the ring
+  // (the state, or its bit reversal, rotated left by k), else null
+  const uint8_t* grot;
   int ngroup;
   int maxrep;             // representatives are orbit maxima
   // batched columns
@@ -457,10 +460,27 @@ template <typename ST>
 __device__ __forceinline__ void general_walk(ST t, const HvArgs& a, ST& rep, int& orbit, int& ci) {
   rep = t;
   int best = 0, stab = 0;
+  // a rotation/reflection group of the ring: each element is the state, or
+  // its bit reversal, rotated left by k (then the digit flips)
+  const int W = 8 * int(sizeof(ST));
+  ST tr = 0;
+  if (a.grot) {
+    if constexpr (sizeof(ST) == 4) tr = __brev(t) >> (W - a.L);
+    else tr = __brevll(t) >> (W - a.L);
+  }
+  const ST mask = ST(a.mask64);
   for (int g = 0; g < a.ngroup; ++g) {
-    const uint8_t* d = a.gdest + 64 * g;
-    ST u = 0;
-    for (int i = 0; i < a.L; ++i) u |= ((t >> i) & ST(1)) << d[i];
+    ST u;
+    if (a.grot) {
+      const uint8_t code = a.grot[g];
+      const ST v = (code & 0x40) ? tr : t;
+      const int k = code & 63;
+      u = k ? ST(((v << k) | (v >> (a.L - k))) & mask) : v;
+    } else {
+      const uint8_t* d = a.gdest + 64 * g;
+      u = 0;
+      for (int i = 0; i < a.L; ++i) u |= ((t >> i) & ST(1)) << d[i];
+    }
     u ^= ST(a.gflip[g]);
     stab += u == t ? 1 : 0;
     if (a.maxrep ? u > rep : u < rep) {
@@ -2311,6 +2331,7 @@ struct Operator::DeviceTables {
     a.gdest = b.group_dest();
     a.gflip = b.group_flip();
     a.gphase = b.group_phase();
+    a.grot = b.group_rot();
     a.maxrep = g.max_rep ? 1 : 0;
     a.nchar = g.n_group ? g.n_chars : (g.kmom >= 0 ? g.L : 1);
     a.n = uint32_t(b.len());

@@ -218,6 +218,10 @@ class SpinBasis1D {
   const uint8_t* group_dest() const { return gdest_.data(); }
   const uint64_t* group_flip() const { return gflip_.data(); }
   const uint8_t* group_phase() const { return gphase_.data(); }
+  // per element k or (k | 0x40) when every element moves the sites as a
+  // rotation j -> j + k or a reflection of the ring (the element is then the
+  // state, or its bit reversal, rotated left by k), else null
+  const uint8_t* group_rot() const { return grot_.size() ? grot_.data() : nullptr; }
   const std::vector<cplx>& characters() const { return chars_; }
   std::vector<uint32_t> states_host() const;
   std::vector<uint8_t> orbits_host() const;
@@ -253,6 +257,7 @@ class SpinBasis1D {
   DeviceArray<uint8_t> gdest_;
   DeviceArray<uint64_t> gflip_;
   DeviceArray<uint8_t> gphase_;
+  DeviceArray<uint8_t> grot_;
   std::vector<cplx> chars_;
   bool real_chars_ = true;
   DeviceArray<uint32_t> dir_cells_, dir_;

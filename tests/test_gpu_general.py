@@ -238,3 +238,49 @@ def test_general_sector_errors(qa):
     b = basis_json(6, 3)
     b["matrix_symmetry"] = {"dimension": 2, "selected_row": 0, "generators": []}
     assert "matrix-valued symmetry sectors materialize as projected operator models" in err(b)
+
+
+def ladder(L, jr=1.0, jl=0.8):
+    """Two-leg ladder on L = 2·rungs sites: rungs (2i, 2i+1), legs (2i, 2i+2)
+    and (2i+1, 2i+3) periodic; S·S couplings jr on rungs, jl on legs."""
+    bonds = [(jr, [2 * i, 2 * i + 1]) for i in range(L // 2)]
+    bonds += [(jl, [2 * i, (2 * i + 2) % L]) for i in range(L // 2)]
+    bonds += [(jl, [2 * i + 1, (2 * i + 3) % L]) for i in range(L // 2)]
+    return [("zz", [(c, s) for c, s in bonds]), ("+-", [(0.5 * c, s) for c, s in bonds]),
+            ("-+", [(0.5 * c, s) for c, s in bonds])]
+
+
+def leg_swap(L):
+    return [i + 1 if i % 2 == 0 else i - 1 for i in range(L)]
+
+
+# groups with an element that is neither a rotation nor a reflection of the
+# ring (the leg swap of a ladder): their elements move the sites one by one
+LADDER = [
+    ("ladder12_t2k1_s0", 12, 6, [general(translation(12, 2), 1), general(leg_swap(12), 0)]),
+    ("ladder12_t2k0_s1_z0", 12, 6,
+     [general(translation(12, 2), 0), general(leg_swap(12), 1), general(inversion(12), 0, flip=True)]),
+    ("ladder10_s1", 10, 5, [general(leg_swap(10), 1)]),
+]
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+@pytest.mark.parametrize("name,L,up,gens", LADDER, ids=[c[0] for c in LADDER])
+def test_general_non_ring_group(qa, oracle, name, L, up, gens, reverse):
+    print(f"[general ladder] {name} reverse={reverse}", flush=True)
+    spec = ladder(L)
+    om = _oracle_model(oracle, L, spec, up, None, gens, reverse)
+    states, orbits = om.states()
+    m = qa.Model(basis_json(L, up, gens, reverse), spec)
+    assert m.dim == om.dim
+    s, o = m.states()
+    np.testing.assert_array_equal(s.astype(np.uint64), states)
+    np.testing.assert_array_equal(o.astype(np.int32), orbits)
+    x = cases.rand_vec(m.dim, seed=L + 11)
+    yr = om.apply(x)
+    assert np.linalg.norm(m.apply(x) - yr) <= 1e-13 * max(np.linalg.norm(yr), 1.0)
+    w = np.linalg.eigvalsh(om.dense())
+    k = min(2, om.dim - 1)
+    res = qa.run({"basis": basis_json(L, up, gens, reverse), "terms": qa.terms_from_spec(spec),
+                  "solver": {"eigenpairs": k, "target": {"kind": "smallest_algebraic"}}})
+    np.testing.assert_allclose(res["eigenvalues"], w[:k], rtol=1e-10, atol=1e-12)

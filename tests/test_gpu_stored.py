@@ -201,3 +201,22 @@ def test_materialize_family_parameters(qa, oracle):
                           "parameters": {"hx": [hx, 0.0], "hz": [hz, 0.0]}})
         assert_same_entries(entries_of(res), om.triplets())
     qa.command({"operation": "release_model", "handle": h})
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc", "dense"])
+def test_typed_model_stored_format(qa, oracle, fmt):
+    # the typed path's format key: the products of the stored matrix equal
+    # the matrix-free ones (FP64 to 1e-13, FP32 to 2e-6) and the oracle's
+    L, up, k = 14, 7, 2
+    spec = cases.dm_chain(L)
+    print(f"[typed stored] {fmt}", flush=True)
+    om = oracle.Model(L, spec, up=up, momentum=k)
+    free = qa.Model(basis_json(L, up, k), spec)
+    stored = qa.Model(basis_json(L, up, k), spec, format=fmt)
+    x = cases.rand_vec(om.dim, seed=5)
+    y_ref = om.apply(x)
+    y = stored.apply(x.astype(np.complex128))
+    assert np.linalg.norm(y - y_ref) <= 1e-13 * np.linalg.norm(y_ref)
+    assert np.linalg.norm(y - free.apply(x.astype(np.complex128))) <= 1e-13 * np.linalg.norm(y_ref)
+    y32 = stored.apply(x.astype(np.complex64))
+    assert np.linalg.norm(y32 - y_ref) <= 2e-6 * np.linalg.norm(y_ref)
