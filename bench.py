@@ -233,8 +233,8 @@ def hv_modes(qa, iters=20):
     return out
 
 
-def f_rows(qa, iters=20):
-    """Timings of the rows next to the path, one GPU:
+def sector_forms(qa, iters=20):
+    """Timings of the forms beside the headline path, one GPU:
     * a wide sector: the Heisenberg ring L=32 up=16 k=0 p=+1 on 64-bit states
       (split stream words), its build, FP64 and FP32-schedule H·v, and the
       FP32-schedule ground state;
@@ -430,9 +430,9 @@ def main():
                     help="only the headline sector's FP32 and FP64 H·v (20 launches each), 20 launches of the "
                          "16-lane and 8-lane batched scan H·v and 5 device copies of 256 MiB, for hardware-counter "
                          "runs")
-    ap.add_argument("--f-rows", action="store_true",
-                    help="time the rows next to the path (wide and general sectors, stored CSR, shift target) and "
-                         "print them as one JSON object")
+    ap.add_argument("--sector-forms", action="store_true",
+                    help="time the wide and general sectors, a stored CSR matrix and the shift target, and print "
+                         "them as one JSON object")
     ap.add_argument("--hv-modes", action="store_true",
                     help="only the FP64 H·v of the XXZ L=24, J1-J2 L=28 and Ising L=26 sectors in the walk, image "
                          "and stream modes (20 timed launches each)")
@@ -456,8 +456,8 @@ def main():
     if args.hv_modes:
         print(json.dumps(hv_modes(qa)))
         return
-    if args.f_rows:
-        print(json.dumps(f_rows(qa)))
+    if args.sector_forms:
+        print(json.dumps(sector_forms(qa)))
         return
     if args.batch_profile:
         L = 24
