@@ -274,6 +274,7 @@ SolverReq parse_solver(const Value& s) {
   }
   if (auto* v = s.get("early_stop"); v && v->kind == Value::Bool) r.ctl.early_stop = v->b;
   if (auto* v = s.get("refill"); v && !v->is_null()) r.ctl.lockstep_refill = as_bool(v, false) ? 1 : 0;
+  if (auto* v = s.get("order_by_gap"); v && !v->is_null()) r.ctl.lockstep_order = as_bool(v, true);
   if (auto* v = s.get("refill_group"); v && !v->is_null()) {
     const long g = as_long(*v, "refill_group");
     if (g < 0 || g > 16) bad("refill_group must lie in [0, 16]");

@@ -556,6 +556,12 @@ struct EigshControls {
   // (or none is running); 0 or lockstep and above: batches that start and
   // end together
   int lockstep_refill = 0;
+  // warm-started lockstep scans: points are taken in ascending order of the
+  // gap between the two lowest eigenvalues of the family projected on the
+  // snapshot space (the hardest first), so that the points of a batch need
+  // similar step counts; false keeps the given order. The results do not
+  // depend on the order.
+  bool lockstep_order = true;
   // return the eigenvectors (false: only eigenvalues and residuals, with no
   // device allocation per solve)
   bool keep_vectors = true;

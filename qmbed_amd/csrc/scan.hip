@@ -27,8 +27,9 @@
 
 #include <algorithm>
 #include <chrono>
-#include <future>
 #include <cmath>
+#include <future>
+#include <limits>
 
 #include "internal.hpp"
 #include "lanczos_kernels.hpp"
@@ -119,7 +120,7 @@ struct Lcg {
 // eigenvalue by bisection, its tridiagonal eigenvector by inverse iteration,
 // and the reflectors applied back (O(n³) with small constants: tens of
 // microseconds at n = 32).
-std::vector<double> lowest_eigenvector(std::vector<double> a, int n) {
+std::vector<double> lowest_eigenvector(std::vector<double> a, int n, double* gap = nullptr) {
   auto A = [&](int i, int j) -> double& { return a[size_t(j) * n + i]; };
   std::vector<std::vector<double>> refl;  // v_k for rows k+1.. (unit, or empty)
   std::vector<double> p(static_cast<size_t>(n)), q(static_cast<size_t>(n));
@@ -163,7 +164,9 @@ std::vector<double> lowest_eigenvector(std::vector<double> a, int n) {
   std::vector<double> d(static_cast<size_t>(n)), e(static_cast<size_t>(std::max(n - 1, 0)));
   for (int i = 0; i < n; ++i) d[size_t(i)] = A(i, i);
   for (int i = 0; i + 1 < n; ++i) e[size_t(i)] = A(i + 1, i);
-  const double lam = tridiagonal_eigenvalue_range(d, e, 0, 1)[0];
+  const auto low = tridiagonal_eigenvalue_range(d, e, 0, std::min(n, 2));
+  const double lam = low[0];
+  if (gap) *gap = low.size() > 1 ? low[1] - low[0] : std::numeric_limits<double>::infinity();
   std::vector<double> z = tridiagonal_eigvec(d, e, lam);
   for (int k = int(refl.size()) - 1; k >= 0; --k) {
     const auto& v = refl[size_t(k)];
@@ -206,6 +209,18 @@ std::vector<std::vector<double>> WarmStartScan::continuations_of(const double* p
     }));
   for (auto& f : fs) f.get();
   return ys;
+}
+
+std::vector<double> WarmStartScan::gaps_of(const double* points, int n) const {
+  std::vector<double> g(static_cast<size_t>(std::max(n, 0)));
+  const int nt = std::min(8, std::max(n, 1));
+  std::vector<std::future<void>> fs;
+  for (int t = 0; t < nt; ++t)
+    fs.push_back(std::async(std::launch::async, [&, t] {
+      for (int i = t; i < n; i += nt) (void)continuation(points + size_t(i) * P_, &g[size_t(i)]);
+    }));
+  for (auto& f : fs) f.get();
+  return g;
 }
 
 std::vector<cplx> WarmStartScan::coefficients_of(const double* point) const {
@@ -664,11 +679,11 @@ WarmStartScan::TrainReport WarmStartScan::train(const double* points, int n, con
   return rep;
 }
 
-std::vector<double> WarmStartScan::continuation(const double* point) const {
+std::vector<double> WarmStartScan::continuation(const double* point, double* gap) const {
   std::vector<double> pm = proj_[0];
   for (int p = 0; p < P_; ++p)
     for (size_t e = 0; e < pm.size(); ++e) pm[e] += point[p] * proj_[size_t(p) + 1][e];
-  auto y = lowest_eigenvector(pm, ns_);
+  auto y = lowest_eigenvector(pm, ns_, gap);
   double g = 0;
   for (int k = 0; k < ns_; ++k) g += y[size_t(k)] * gauge_[size_t(k)];
   if (g < 0)
@@ -778,6 +793,12 @@ std::vector<WarmStartScan::PointResult> WarmStartScan::solve_lockstep(const doub
     if (start != Start::Plain)
       for (double v : ys[size_t(i)]) cn += v * v;
     if (start == Start::Plain || (std::isfinite(cn) && cn > 1.0e-6)) queue.push_back(i);
+  }
+  // warm starts: the hardest points first (ascending projected gap), so a
+  // batch's windows need similar step counts; ties keep the given order
+  if (start != Start::Plain && ctl.lockstep_order && queue.size() > 1) {
+    const auto gaps = gaps_of(points, n);
+    std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return gaps[size_t(a)] < gaps[size_t(b)]; });
   }
   // the usable points' coefficients in queue order, in one upload
   const size_t nq = queue.size();

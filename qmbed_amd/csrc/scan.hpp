@@ -96,8 +96,9 @@ class WarmStartScan {
 
  private:
   std::vector<cplx> coefficients_of(const double* point) const;
-  // gauge-fixed lowest eigenvector of Sᵀ H(c) S
-  std::vector<double> continuation(const double* point) const;
+  // gauge-fixed lowest eigenvector of Sᵀ H(c) S (and, with gap, the
+  // distance between its two lowest eigenvalues)
+  std::vector<double> continuation(const double* point, double* gap = nullptr) const;
   // snapshot space of the first n training states (sets S_, ns_, proj_,
   // gauge_); Lm and sel receive the coefficients and the selected states
   void build_space(const double* psi, int n, std::vector<double>& Lm, std::vector<int>& sel);
@@ -132,6 +133,8 @@ class WarmStartScan {
   // network's outputs, plus the continuation under correction targets
   std::vector<std::vector<double>> network_coefficients(const double* points, int n);
   std::vector<std::vector<double>> continuations_of(const double* points, int n) const;
+  // the gaps of Sᵀ H(c) S at n points (host threads)
+  std::vector<double> gaps_of(const double* points, int n) const;
   PinnedArray<double> amp_pin_;  // staging of the points' amplitude tables
 };
 

@@ -268,6 +268,11 @@ def test_lockstep_scan_matches_one_by_one(qa, oracle, kw):
         refill = scan.solve(held, warm="rayleigh_ritz", lockstep=ls, **rk, **kw)
         assert refill["iterations"] == one["iterations"] and refill["fp32_iterations"] == one["fp32_iterations"]
         np.testing.assert_allclose(refill["energies"], one["energies"], rtol=1e-10, atol=0)
+    # the queue in the given order instead of by ascending projected gap:
+    # the same per-point solves, as a batch's order changes no window's work
+    given = scan.solve(held, warm="rayleigh_ritz", lockstep=4, refill=True, order_by_gap=False, **kw)
+    assert given["iterations"] == one["iterations"] and given["fp32_iterations"] == one["fp32_iterations"]
+    np.testing.assert_allclose(given["energies"], one["energies"], rtol=1e-10, atol=0)
     with pytest.raises(qa.QmbedError, match="lockstep must lie"):
         scan.solve(held[:2], warm="rayleigh_ritz", lockstep=17)
     # the network's start and the plain start in lockstep too
