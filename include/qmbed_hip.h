@@ -326,6 +326,25 @@ int qmbed_hip_memcpy(void* dst, const void* src, size_t bytes, int kind);
 int qmbed_hip_synchronize(void);
 int qmbed_hip_device_count(void);
 
+/* Basis manifests: the versioned zip archive of ordered basis states
+ * (src/archive.rs:333-362, save_basis_zip over BasisArchive::new and
+ * insert_metadata, :246-331). Host-only; no device work.
+ * request: {"path": str, "width_bits": w, "states": [decimal strings or
+ * integers], "metadata": {key: str}}. When `states` is non-null its n 64-bit
+ * words are the states (and the request carries no "states"). The states are
+ * kept in the given row order; they must be unique and fit w bits. Answers
+ * {"status":"ok","result":{"path","width_bits","count"}} or the reference's
+ * error text ("archive error: …", "unsupported backend: integer needs …").
+ * Free the answer with qmbed_string_free. */
+char* qmbed_hip_basis_archive_save(const char* request, const uint64_t* states, size_t n);
+/* Read one (src/archive.rs:364-416, load_basis_zip): answers
+ * {"width_bits","count","fits_u64","written","metadata":{…}} plus
+ * "states": [decimal strings] when `decimal` is nonzero. The states are
+ * written to `states` (row order) when it is non-null, every state fits 64
+ * bits and count <= capacity ("written": true); call with capacity 0 first to
+ * learn the count. Stored and deflated entries are read, zip64 fields too. */
+char* qmbed_hip_basis_archive_load(const char* path, uint64_t* states, size_t capacity, int decimal);
+
 #ifdef __cplusplus
 }
 #endif

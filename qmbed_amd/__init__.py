@@ -50,6 +50,10 @@ def library():
         f.restype = c_void_p
         f.argtypes = [c_char_p]
     L.qmbed_string_free.argtypes = [c_void_p]
+    L.qmbed_hip_basis_archive_save.restype = c_void_p
+    L.qmbed_hip_basis_archive_save.argtypes = [c_char_p, c_void_p, ctypes.c_size_t]
+    L.qmbed_hip_basis_archive_load.restype = c_void_p
+    L.qmbed_hip_basis_archive_load.argtypes = [c_char_p, c_void_p, ctypes.c_size_t, c_int]
     L.qmbed_string_free.restype = None
     L.qmbed_hip_model_create.restype = c_void_p
     L.qmbed_hip_model_create.argtypes = [c_char_p, ctypes.POINTER(c_void_p)]
@@ -239,6 +243,7 @@ class Model:
             msg = _take(err.value) if err.value else "model creation failed"
             raise QmbedError(msg)
         self._h = h
+        self.basis = basis
         self.dim = int(L.qmbed_hip_model_dimension(h))
         self.is_real = bool(L.qmbed_hip_model_is_real(h))
 
@@ -269,6 +274,13 @@ class Model:
         if rc != 0:
             raise QmbedError("state copy failed")
         return s, o
+
+    def save_basis_archive(self, path, metadata=None):
+        """The sector's representatives, in row order, as a basis manifest of
+        width `sites` bits (src/archive.rs:282-295 BasisArchive::from_basis,
+        :333 save_basis_zip)."""
+        states, _ = self.states()
+        return save_basis_archive(path, int(self.basis["sites"]), states, metadata)
 
     def hv_config(self):
         lpr, img = ctypes.c_int(), ctypes.c_int()
@@ -486,6 +498,37 @@ def _call_model(h, req):
     if response.get("status") != "ok":
         raise QmbedError(response.get("error", "unknown error"))
     return response["result"]
+
+
+def save_basis_archive(path, width_bits, states, metadata=None):
+    """Write a basis manifest (src/archive.rs:333-362): states as an unsigned
+    integer array (64-bit words) or a list of Python ints / decimal strings of
+    any width, in row order."""
+    L = library()
+    req = {"path": os.fspath(path), "width_bits": int(width_bits), "metadata": dict(metadata or {})}
+    if isinstance(states, np.ndarray) and states.dtype.kind == "u":
+        words = np.ascontiguousarray(states, dtype=np.uint64)
+        return _envelope(L.qmbed_hip_basis_archive_save(json.dumps(req).encode(), words.ctypes.data, words.size))
+    req["states"] = [str(int(x)) if not isinstance(x, str) else x for x in states]
+    return _envelope(L.qmbed_hip_basis_archive_save(json.dumps(req).encode(), None, 0))
+
+
+def load_basis_archive(path):
+    """(width_bits, states, metadata) of a basis manifest (src/archive.rs:
+    364-416): states as a uint64 array when every one fits 64 bits, else a
+    list of Python ints."""
+    L = library()
+    p = os.fspath(path).encode()
+    head = _envelope(L.qmbed_hip_basis_archive_load(p, None, 0, 0))
+    if head["fits_u64"]:
+        states = np.zeros(head["count"], dtype=np.uint64)
+        res = _envelope(L.qmbed_hip_basis_archive_load(p, states.ctypes.data, states.size, 0))
+        if not res["written"]:
+            raise QmbedError("basis archive changed while it was read")
+    else:
+        res = _envelope(L.qmbed_hip_basis_archive_load(p, None, 0, 1))
+        states = [int(x) for x in res["states"]]
+    return res["width_bits"], states, res["metadata"]
 
 
 def _envelope(ptr):

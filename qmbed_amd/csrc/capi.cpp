@@ -20,6 +20,7 @@
 #include <sstream>
 
 #include "../../include/qmbed_hip.h"
+#include "archive.hpp"
 #include "internal.hpp"
 #include "lanczos_kernels.hpp"
 #include "json.hpp"
@@ -1733,5 +1734,93 @@ int qmbed_hip_gemm_atb_noalloc(const double* A, size_t n, int p, const double* B
 }
 
 int qmbed_hip_device_count(void) { return device_count(); }
+
+char* qmbed_hip_basis_archive_save(const char* request, const uint64_t* states, size_t n) {
+  try {
+    if (!request) bad("invalid binding request: request pointer is null");
+    Value v;
+    try {
+      v = json::parse(request);
+    } catch (json::ParseError& e) {
+      throw invalid_options(std::string("invalid binding request: ") + e.what());
+    }
+    if (v.kind != Value::Obj) bad("invalid binding request: expected an object");
+    const Value& path = need(v, "path", "invalid binding request");
+    if (path.kind != Value::String) bad("invalid binding request: `path` must be a string");
+    const long width = as_long(need(v, "width_bits", "invalid binding request"), "width_bits");
+    if (width < 0) bad("width_bits: expected a nonnegative integer");
+    BasisArchive a;
+    if (states) {
+      if (v.get("states")) bad("invalid binding request: states come from the request or the buffer, not both");
+      a = basis_archive(size_t(width), states, n);
+    } else {
+      std::vector<std::string> text;
+      for (const Value& x : need(v, "states", "invalid binding request").array()) {
+        if (x.kind == Value::String) text.push_back(x.raw);
+        else if (x.kind == Value::Number) text.push_back(x.raw);  // exact integer text as written
+        else bad("invalid binding request: a state must be a decimal string or an integer");
+      }
+      a = basis_archive(size_t(width), text);
+    }
+    if (const Value* md = v.get("metadata"); md && !md->is_null()) {
+      if (md->kind != Value::Obj) bad("invalid binding request: `metadata` must be an object of strings");
+      for (const auto& [k, x] : *md->obj) {
+        if (x.kind != Value::String) bad("invalid binding request: metadata values must be strings");
+        a.insert_metadata(k, x.raw);
+      }
+    }
+    save_basis_zip(path.raw, a);
+    std::string o = "{\"status\":\"ok\",\"result\":{\"path\":";
+    json::write_string(o, path.raw);
+    o += ",\"width_bits\":" + std::to_string(a.width_bits) + ",\"count\":" + std::to_string(a.states.size()) + "}}";
+    return dup_c(o);
+  } catch (Error& e) {
+    return dup_c(error_json(e.message));
+  } catch (std::exception& e) {
+    return dup_c(error_json(std::string("internal state error: ") + e.what()));
+  }
+}
+
+char* qmbed_hip_basis_archive_load(const char* path, uint64_t* states, size_t capacity, int decimal) {
+  try {
+    if (!path) bad("invalid binding request: path pointer is null");
+    const BasisArchive a = load_basis_zip(path);
+    bool narrow = true;
+    for (const auto& s : a.states)
+      if (!fits_u64(s)) {
+        narrow = false;
+        break;
+      }
+    const bool written = states && narrow && a.states.size() <= capacity;
+    if (written)
+      for (size_t i = 0; i < a.states.size(); ++i) states[i] = std::stoull(a.states[i]);
+    std::string o = "{\"status\":\"ok\",\"result\":{\"width_bits\":" + std::to_string(a.width_bits) +
+                    ",\"count\":" + std::to_string(a.states.size()) + ",\"fits_u64\":" + (narrow ? "true" : "false") +
+                    ",\"written\":" + (written ? "true" : "false") + ",\"metadata\":{";
+    bool first = true;
+    for (const auto& [k, x] : a.metadata) {
+      if (!first) o += ",";
+      first = false;
+      json::write_string(o, k);
+      o += ":";
+      json::write_string(o, x);
+    }
+    o += "}";
+    if (decimal) {
+      o += ",\"states\":[";
+      for (size_t i = 0; i < a.states.size(); ++i) {
+        if (i) o += ",";
+        o += "\"" + a.states[i] + "\"";
+      }
+      o += "]";
+    }
+    o += "}}";
+    return dup_c(o);
+  } catch (Error& e) {
+    return dup_c(error_json(e.message));
+  } catch (std::exception& e) {
+    return dup_c(error_json(std::string("internal state error: ") + e.what()));
+  }
+}
 
 }  // extern "C"

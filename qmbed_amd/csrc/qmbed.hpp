@@ -39,6 +39,7 @@ enum class ErrorKind {
   UnsupportedBackend,
   InternalState,
   NonConvergence,
+  Archive,
 };
 
 struct Error : std::exception {
