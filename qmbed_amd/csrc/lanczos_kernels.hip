@@ -9,8 +9,10 @@
 // in-kernel fence between blocks is needed (an agent-scope release writes back
 // the whole L2 of an XCD, which costs more than the redundant adds from L2).
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "device_common.hpp"
@@ -313,6 +315,66 @@ __device__ __forceinline__ void sub_body(const E* __restrict__ V, size_t n, int 
   const bool last_jm1 = last_j && t.vjm1 && mc >= 2 && t.vjm1 == V + size_t(mc - 2) * n;
   const int cend = mc - (last_j ? (last_jm1 ? 2 : 1) : 0);
   double s = 0;
+  bool pairs = false;
+  if constexpr (std::is_same<E, double>::value)
+    pairs = (n % 2 == 0) && ((reinterpret_cast<uintptr_t>(V) | reinterpret_cast<uintptr_t>(w)) % 16 == 0);
+  if (pairs) {
+    if constexpr (std::is_same<E, double>::value) {
+      // two adjacent rows per thread, one 16-byte load per column: each row's
+      // arithmetic is the single-row loop's below, term for term
+      const size_t np = n / 2;
+      const double2* __restrict__ V2 = reinterpret_cast<const double2*>(V);
+      double2* __restrict__ w2 = reinterpret_cast<double2*>(w);
+      for (size_t q = bx * size_t(kT) + threadIdx.x; q < np; q += size_t(nbx) * kT) {
+        double a0 = 0.0, a1 = 0.0;
+        int c = 0;
+        for (; c + 4 <= cend; c += 4) {
+          const double2 u0 = V2[size_t(c) * np + q], u1 = V2[size_t(c + 1) * np + q];
+          const double2 u2 = V2[size_t(c + 2) * np + q], u3 = V2[size_t(c + 3) * np + q];
+          a0 = add(a0, mul(u0.x, sh_h[c]));
+          a1 = add(a1, mul(u0.y, sh_h[c]));
+          a0 = add(a0, mul(u1.x, sh_h[c + 1]));
+          a1 = add(a1, mul(u1.y, sh_h[c + 1]));
+          a0 = add(a0, mul(u2.x, sh_h[c + 2]));
+          a1 = add(a1, mul(u2.y, sh_h[c + 2]));
+          a0 = add(a0, mul(u3.x, sh_h[c + 3]));
+          a1 = add(a1, mul(u3.y, sh_h[c + 3]));
+        }
+        for (; c < cend; ++c) {
+          const double2 u = V2[size_t(c) * np + q];
+          a0 = add(a0, mul(u.x, sh_h[c]));
+          a1 = add(a1, mul(u.y, sh_h[c]));
+        }
+        const double2 xw = w2[q];
+        double x0 = xw.x, x1 = xw.y;
+        const size_t i0 = 2 * q, i1 = 2 * q + 1;
+        if (last_j) {
+          const double2 vj = V2[size_t(mc - 1) * np + q];
+          double2 vm = make_double2(0.0, 0.0);
+          if (last_jm1) {
+            vm = V2[size_t(mc - 2) * np + q];
+            a0 = add(a0, mul(vm.x, sh_h[mc - 2]));
+            a1 = add(a1, mul(vm.y, sh_h[mc - 2]));
+          }
+          a0 = add(a0, mul(vj.x, sh_h[mc - 1]));
+          a1 = add(a1, mul(vj.y, sh_h[mc - 1]));
+          x0 = sub(x0, scale(vj.x, alpha));
+          x1 = sub(x1, scale(vj.y, alpha));
+          if (t.vjm1) {
+            x0 = sub(x0, scale(last_jm1 ? vm.x : t.vjm1[i0], beta));
+            x1 = sub(x1, scale(last_jm1 ? vm.y : t.vjm1[i1], beta));
+          }
+        } else if (t.vj) {
+          x0 = three_apply(t, x0, i0, alpha, beta);
+          x1 = three_apply(t, x1, i1, alpha, beta);
+        }
+        const double v0 = sub(x0, a0), v1 = sub(x1, a1);
+        w2[q] = make_double2(v0, v1);
+        s += sq(v0);
+        s += sq(v1);
+      }
+    }
+  } else
   for (size_t i = bx * size_t(kT) + threadIdx.x; i < n; i += size_t(nbx) * kT) {
     E acc = zero<E>();
     int c = 0;
